@@ -1,0 +1,215 @@
+/*
+ * pgfem_asm.h -- C ABI of the MI355X element-assembly library.
+ *
+ * This header is the drop-in boundary for the tangent (K) and internal-force
+ * (R = P_int) assembly of PIGFEM's AssemblerSmallStrainStructural.  Every entry
+ * point takes plain pointers and sizes; nothing in a signature is a torch or
+ * HIP type.  All functions return 0 on success and a negative PG_E* code on
+ * failure; pg_asm_last_error() then holds a message (thread-local).
+ *
+ * Reference interface replaced (file:line in brandyb2/PIGFEM):
+ *   Assembler::assemble(pmatrix& K, pvector& P_int, double delta_t,
+ *       vector<vector<vector<double>>>& update_ISVs, NodalData* solution,
+ *       bool assembleFunc, bool assembleJac)      include/Assembler.h:46-48,
+ *                                                 src/Assembler.cpp:217-298
+ *   Assembler::storeBmats / Mesh::store_shape_functions (set-up that the
+ *       GPU path replaces by on-the-fly recomputation)
+ *                                                 src/Assembler.cpp:754-773,
+ *                                                 src/Mesh.cpp:1475-1643
+ *   Assembler::set_matrix / set_vec (the scatter) src/Assembler.cpp:811-853
+ */
+#ifndef PGFEM_ASM_H
+#define PGFEM_ASM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------
+ * Status codes
+ * ------------------------------------------------------------------------- */
+#define PG_OK            0
+#define PG_E_ARG        -1   /* bad argument / inconsistent description      */
+#define PG_E_HIP        -2   /* HIP runtime error                            */
+#define PG_E_UNSUPPORTED -3  /* element class or flag combination unsupported */
+#define PG_E_MESH       -4   /* invalid mesh (negative Jacobian, ...)         */
+#define PG_E_NOMEM      -5
+
+/* ----------------------------------------------------------------------------
+ * Flags of pg_asm_assemble.  PG_FUNC / PG_JAC mirror assembleFunc / assembleJac
+ * (src/Assembler.cpp:217-220); both clear is a no-op, as in the reference.
+ * ------------------------------------------------------------------------- */
+#define PG_FUNC     1u   /* assemble the internal-force vector P_int        */
+#define PG_JAC      2u   /* assemble the tangent K (4 blocks)               */
+#define PG_FP32     4u   /* mixed precision: FP32 B^T.D.B / cohesive tangent,
+                            FP64 sliver Jacobians, damage update, CSR sums, P */
+#define PG_BATCHED  8u   /* class-binned schedule (cut-element waves +
+                            node-pair recompute); implied by PG_FP32         */
+
+/* ----------------------------------------------------------------------------
+ * Materials.  Reference: material_lei.cpp:52-97 (LEI D matrix),
+ * Material_BilinearCohesive.cpp:38-125, Material_BilinearCohesiveNU.cpp:48-138,
+ * Material_OPCohesive.cpp:38-98, Material_OPCohesiveNU.cpp:43-111.
+ * ------------------------------------------------------------------------- */
+#define PG_MAT_LEI        0  /* linear elastic isotropic (D precomputed)     */
+#define PG_MAT_BL         1  /* BilinearCohesive (delta_max read, never set) */
+#define PG_MAT_BL_NU      2  /* BilinearCohesiveNU                           */
+#define PG_MAT_OP         3  /* OPCohesive (delta_max history)               */
+#define PG_MAT_OP_NU      4  /* OPCohesiveNU                                 */
+
+typedef struct pg_material {
+    int32_t type;       /* PG_MAT_*                                           */
+    int32_t nstr;       /* LEI: rows of D (3 in 2D plane strain, 6 in 3D)     */
+    double  D[36];      /* LEI: row-major nstr x nstr constitutive matrix     */
+    double  sigma_c;    /* cohesive: critical traction                        */
+    double  delta_c;    /* OP: critical opening                               */
+    double  delta_c1;   /* BL: opening at peak traction                       */
+    double  delta_c2;   /* BL: opening at full separation                     */
+    double  beta2;      /* cohesive: beta^2 (tangential weight)               */
+    double  alpha;      /* BL: contact penalty sharpness                      */
+} pg_material;
+
+/* ----------------------------------------------------------------------------
+ * Cut (IGFEM-intersected) element record.  Element-local node numbering follows
+ * the reference: 0..nen-1 parent nodes, then enrichment nodes in cut-edge order,
+ * then (cohesive meshes) their mirror nodes (src/tri3.cpp:287-596,
+ * src/tet4.cpp:228-520, src/elem.cpp:428-559).  A child is an integration
+ * sub-element (tri3 / quad4 / tet4) listed by local node index; a cohesive
+ * child is a CohEdge2 (4 local nodes: 2 lower then 2 upper) or CohTri3 (6).
+ * ------------------------------------------------------------------------- */
+#define PG_CHILD_TRI3   0
+#define PG_CHILD_QUAD4  1
+#define PG_CHILD_TET4   2
+#define PG_MAX_ENR     12
+#define PG_MAX_CHILD    8
+#define PG_MAX_COH      3
+
+typedef struct pg_cut_rec {
+    int32_t enr[PG_MAX_ENR];        /* global node index of each local enrichment node */
+    int32_t isv_off;                /* first cohesive qp of this element in the ISV array */
+    uint8_t n_enr;                  /* local enrichment nodes (incl. mirrors)           */
+    uint8_t n_child;
+    uint8_t n_coh;
+    uint8_t n_vqp;                  /* volumetric qps (sum over children)               */
+    uint8_t child_type[PG_MAX_CHILD];
+    uint8_t child_mat[PG_MAX_CHILD];
+    uint8_t child_nodes[PG_MAX_CHILD][4];
+    uint8_t coh_nodes[PG_MAX_COH][6];
+    uint8_t coh_mat[PG_MAX_COH];
+    uint8_t coh_nn;                 /* nodes per cohesive child: 4 (CohEdge2) or 6 (CohTri3) */
+    uint8_t pad[2];
+} pg_cut_rec;                       /* 128 bytes */
+
+/* ----------------------------------------------------------------------------
+ * Static problem description (set-up output of the host harness; the
+ * reference's Mesh + DofObject + Solver::preallocate_matrix state).
+ *
+ *   nodes: n_nodes regular nodes, then n_enr enrichment nodes; global node index
+ *          n_nodes + k is enrichment node k (reference id ENRICH_START + k).
+ *   node_dof[(node)*dim + j]: global dof; free dofs are < n_free
+ *          (src/DofObject.cpp:241-278, 498-538).
+ *   K blocks: CSR of ff (n_free x n_free), fp (n_free x n_pres),
+ *          pf (n_pres x n_free), pp (n_pres x n_pres) with block-local column
+ *          indices (src/Assembler.cpp:811-831).  The library stores the values
+ *          of the four blocks back to back: [ff | fp | pf | pp].
+ *   P: [Pf (n_free) | Pp (n_pres)].
+ *   node-pair pattern: npat_ptr / npat_col list, per node a, the nodes b that
+ *          share an element with a (sorted); npat_map holds, per pair and per
+ *          (i,j) in dim x dim, the index into the concatenated K values of entry
+ *          (node_dof[a*dim+i], node_dof[b*dim+j])  -- the scatter indices.
+ *   n2e_ptr / n2e: elements incident to each node, ascending (summation order
+ *          of the reference element loop, src/Assembler.cpp:252).
+ * ------------------------------------------------------------------------- */
+typedef struct pg_problem_desc {
+    int32_t dim;            /* 2 or 3 */
+    int32_t nen;            /* parent nodes per element: 3 (tri3) or 4 (tet4) */
+    int32_t n_nodes;
+    int32_t n_enr;
+    int32_t n_elem;
+    int32_t n_cut;
+    int32_t n_mat;
+    int32_t n_cqp;          /* cohesive quadrature points (ISV entries) */
+    int32_t n_free;
+    int32_t n_pres;
+    int32_t cohesive;       /* mesh has cohesive interfaces */
+    int32_t pad0;
+    int64_t nnz[4];         /* ff, fp, pf, pp */
+    int64_t n_pairs;        /* length of npat_col */
+    const double*   coords;     /* (n_nodes+n_enr)*dim */
+    const int32_t*  conn;       /* n_elem*nen */
+    const int32_t*  elem_mat;   /* n_elem */
+    const int32_t*  cut_of_elem;/* n_elem: -1 or index into cut[] */
+    const int32_t*  cut_elem;   /* n_cut: element id of each cut record */
+    const pg_cut_rec* cut;      /* n_cut */
+    const pg_material* mats;    /* n_mat */
+    const int32_t*  node_dof;   /* (n_nodes+n_enr)*dim */
+    const int32_t*  rp[4];      /* block row pointers */
+    const int32_t*  ci[4];      /* block column indices */
+    const int32_t*  npat_ptr;   /* n_nodes+n_enr+1 */
+    const int32_t*  npat_col;   /* n_pairs */
+    const int32_t*  npat_row;   /* n_pairs: row node of each pair */
+    const int32_t*  npat_map;   /* n_pairs*dim*dim (total nnz < 2^31) */
+    const int32_t*  n2e_ptr;    /* n_nodes+n_enr+1 */
+    const int32_t*  n2e;        /* incidence list */
+} pg_problem_desc;
+
+/* ----------------------------------------------------------------------------
+ * Assembler handle.  Owns all device buffers of one problem, one HIP stream.
+ * One caller thread per handle.
+ * ------------------------------------------------------------------------- */
+typedef struct pg_asm pg_asm;
+
+/* Upload the static description to HBM and build the per-class schedules.
+ * Replaces Problem::init's storeBmats/store_shape_functions + preallocation. */
+int pg_asm_create(const pg_problem_desc* desc, pg_asm** out);
+int pg_asm_destroy(pg_asm* h);
+const char* pg_asm_last_error(void);
+
+/* Host-buffer drop-in for Assembler::assemble (src/Assembler.cpp:217-298).
+ *   U_all         global-dof-ordered solution, length n_free+n_pres
+ *   isv_committed committed cohesive history (delta_max), length n_cqp (may be
+ *                 NULL when n_cqp == 0)
+ *   isv_trial     out: trial history after this assembly (reset from committed
+ *                 first, as InternalVars::copy_data does, src/Assembler.cpp:244)
+ *   K             out: concatenated values [ff|fp|pf|pp] (NULL unless PG_JAC)
+ *   P             out: [Pf|Pp] (NULL unless PG_FUNC)
+ * Copies host->device, assembles, copies device->host, synchronises. */
+int pg_asm_assemble(pg_asm* h, const double* U_all, const double* isv_committed,
+                    double* isv_trial, double delta_t, unsigned flags,
+                    double* K, double* P);
+
+/* Device-resident variant: every pointer is device memory (HBM) of the sizes
+ * above; the launch is asynchronous on `stream` (a hipStream_t, or NULL for the
+ * handle's own stream).  No host synchronisation, no allocation. */
+int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
+                           const double* disv_committed, double* disv_trial,
+                           double delta_t, unsigned flags,
+                           double* dK, double* dP, void* stream);
+
+/* Sizes a caller needs for buffers of the calls above. */
+int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total,
+                 int64_t* n_cqp);
+
+/* Per-launch timing of the last pg_asm_assemble*_timed call, in ms, measured
+ * with HIP events on the stream the kernels run on.  kernel_ms[0] is the
+ * whole assembly, [1] the cut-element (or element) kernel, [2] the scatter /
+ * node-pair kernel. */
+int pg_asm_last_timing(const pg_asm* h, double kernel_ms[3]);
+int pg_asm_set_timing(pg_asm* h, int enable);
+
+/* Schedule parameters chosen by the autotuner (threads per block of the
+ * node-pair kernel, waves per block of the cut kernel).  0 keeps the default. */
+int pg_asm_set_tiles(pg_asm* h, int pair_block, int cut_block);
+
+/* Algorithmic bytes of one K+R assembly of this problem (the roofline
+ * numerator): connectivity, coordinates, U, cut records, ISVs, CSR values +
+ * column indices + row pointers, R. */
+int64_t pg_asm_algorithmic_bytes(const pg_asm* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGFEM_ASM_H */

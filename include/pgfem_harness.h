@@ -1,0 +1,76 @@
+/*
+ * pgfem_harness.h -- host set-up of synthetic IGFEM RVEs (C ABI).
+ *
+ * The harness restates, on flat arrays, the part of PIGFEM's Problem::init that
+ * produces what the assembly reads (src/Problem.cpp:100-318):
+ *   structured tri3 / tet4 mesh generation       src/Mesh.cpp:236-630
+ *   node detection against circles / spheres      src/Mesh.cpp:2883-2930,
+ *                                                 src/InclusionCircle.cpp:150-163,
+ *                                                 src/InclusionSphere.cpp:163-177
+ *   element cut classification + child topology   src/tri3.cpp:287-596,
+ *                                                 src/tet4.cpp:228-520,
+ *                                                 src/elem.cpp:428-559
+ *   enrichment nodes (set order, mirrors after)   src/Mesh.cpp:3104-3251,3661-3720
+ *   edge intersection (Steffensen, |da|<1e-7)     src/Inclusion.cpp:32-120
+ *   Dirichlet sets, free-first dof numbering      src/BoundaryObject.cpp:32-99,
+ *                                                 src/DofObject.cpp:241-278,498-538
+ *   4-block AIJ pattern (union of element blocks) src/Solver.cpp:414-490,
+ *                                                 src/Assembler.cpp:811-831
+ * It fills a pg_problem_desc (pgfem_asm.h) that the assembler consumes.
+ */
+#ifndef PGFEM_HARNESS_H
+#define PGFEM_HARNESS_H
+
+#include <stdint.h>
+#include "pgfem_asm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGH_COH_NONE 0
+#define PGH_COH_BL   1   /* Material_BilinearCohesive  */
+#define PGH_COH_OP   2   /* Material_OPCohesive        */
+#define PGH_COH_BL_NU 3
+#define PGH_COH_OP_NU 4
+
+typedef struct pgh_rve_params {
+    int32_t dim;            /* 2: unit square of tri3, 3: unit cube of tet4   */
+    int32_t n;              /* elements per side (n x n, or n x n x n cubes)   */
+    int32_t n_incl;         /* circles / spheres                               */
+    int32_t cohesive;       /* PGH_COH_*                                       */
+    double  r_min, r_max;   /* radius ~ U[r_min, r_max]                        */
+    double  gap;            /* minimum surface-to-surface gap between inclusions */
+    uint64_t seed;          /* std::mt19937_64 seed                             */
+    double  E_matrix, nu_matrix, E_fiber, nu_fiber;
+    double  sigma_c, delta_c, delta_c1, delta_c2, beta, alpha;
+    double  strain;         /* applied u_x on RIGHT = strain * width            */
+    int32_t max_tries;      /* rejection-sampling budget (0 = default)          */
+    int32_t pad;
+} pgh_rve_params;
+
+typedef struct pgh_problem pgh_problem;
+
+/* Build one RVE.  Returns PG_OK or a negative code (pg_asm_last_error-style
+ * message in pgh_last_error). */
+int  pgh_build_rve(const pgh_rve_params* p, pgh_problem** out);
+int  pgh_get_desc(const pgh_problem* h, pg_problem_desc* out);
+void pgh_free(pgh_problem* h);
+const char* pgh_last_error(void);
+
+/* Inclusions actually placed: centers (dim each) and radii. */
+int  pgh_inclusions(const pgh_problem* h, const double** centers,
+                    const double** radii, int32_t* n);
+/* Nodal detection value per regular node (-1 matrix, else inclusion index). */
+const int32_t* pgh_node_detect(const pgh_problem* h);
+/* Prescribed value at full load of each prescribed dof (length n_pres); the
+ * load-step value is time_fraction * value (src/Assembler.cpp:546-575). */
+const double* pgh_bc_values(const pgh_problem* h);
+/* Per element: number of volumetric + cohesive quadrature points. */
+int  pgh_elem_qps(const pgh_problem* h, int32_t e, int32_t* n_vqp, int32_t* n_cqp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGFEM_HARNESS_H */

@@ -1,0 +1,113 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+`Oracle(problem)` builds its own 4-block AIJ pattern by inserting every element
+dof block (src/Assembler.cpp:811-831 semantics) and caches shape data
+(src/Mesh.cpp:1475-1643, src/Assembler.cpp:754-773); `assemble` restates
+Assembler::assemble (src/Assembler.cpp:217-520) in FP64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    return SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        L = C.CDLL(SO)
+        L.oracle_new.restype = C.c_void_p
+        L.oracle_new.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_error.restype = C.c_char_p
+        L.oracle_error.argtypes = [C.c_void_p]
+        L.oracle_nnz.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.oracle_pattern.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.POINTER(C.c_int32)),
+                                     C.POINTER(C.POINTER(C.c_int32))]
+        L.oracle_assemble.restype = C.c_int
+        L.oracle_assemble.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    def __init__(self, problem, e0: int = 0, e1: int = -1):
+        self.problem = problem
+        L = lib()
+        self._h = L.oracle_new(C.addressof(problem.desc), int(e0), int(e1))
+        if not self._h:
+            raise MemoryError("oracle_new failed")
+        err = L.oracle_error(self._h)
+        if err:
+            raise RuntimeError("oracle set-up: " + err.decode())
+        nnz = (C.c_int64 * 4)()
+        L.oracle_nnz(self._h, nnz)
+        self.nnz = [int(x) for x in nnz]
+        rows = [problem.n_free, problem.n_free, problem.n_pres, problem.n_pres]
+        self.rp, self.ci = [], []
+        for b in range(4):
+            rp, ci = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+            L.oracle_pattern(self._h, b, C.byref(rp), C.byref(ci))
+            self.rp.append(np.ctypeslib.as_array(rp, shape=(rows[b] + 1,)).copy())
+            self.ci.append(np.ctypeslib.as_array(ci, shape=(max(self.nnz[b], 1),))[: self.nnz[b]].copy())
+
+    def assemble(self, U, isv_committed=None, func=True, jac=True):
+        p = self.problem
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        K = np.zeros(sum(self.nnz), dtype=np.float64)
+        P = np.zeros(p.n_dof, dtype=np.float64)
+        ic = np.ascontiguousarray(isv_committed if isv_committed is not None else np.zeros(p.n_cqp), dtype=np.float64)
+        it = np.zeros(max(p.n_cqp, 1), dtype=np.float64)
+        rc = lib().oracle_assemble(self._h, U.ctypes.data, ic.ctypes.data if p.n_cqp else None,
+                                   it.ctypes.data, int(func), int(jac), K.ctypes.data, P.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("oracle_assemble: " + lib().oracle_error(self._h).decode())
+        return K, P, it[: p.n_cqp]
+
+    def close(self):
+        if self._h:
+            lib().oracle_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cohesive_law(mat, dim, delta, isv=0.0):
+    """The oracle's cohesive constitutive update: returns (delta_used, isv_out, t, D)."""
+    L = lib()
+    L.oracle_cohesive_law.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    d = np.ascontiguousarray(np.array(delta, dtype=np.float64))
+    h = np.array([isv], dtype=np.float64)
+    t = np.zeros(dim)
+    D = np.zeros(dim * dim)
+    L.oracle_cohesive_law(C.addressof(mat), dim, d.ctypes.data, h.ctypes.data, t.ctypes.data, D.ctypes.data)
+    return d, float(h[0]), t, D.reshape(dim, dim)
+
+
+def elem_cache(orc, e):
+    L = lib()
+    L.oracle_elem_cache.restype = C.c_int64
+    L.oracle_elem_cache.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.POINTER(C.c_double))]
+    ptr = C.POINTER(C.c_double)()
+    n = L.oracle_elem_cache(orc._h, int(e), C.byref(ptr))
+    if n < 0:
+        raise IndexError(e)
+    return np.ctypeslib.as_array(ptr, shape=(int(n),)).copy()

@@ -1,0 +1,12 @@
+"""pigfem_amd -- MI355X-native K/R element assembly for PIGFEM-style IGFEM RVEs.
+
+The package holds only the assembly hot path: HIP kernels and their C ABI
+(csrc/, include/), the host set-up harness that produces the problem
+description, and thin Python mirrors of the reference's Assembler API.
+"""
+from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PgError, lib  # noqa: F401
+from .problem import RVEConfig, RVEProblem, config, small_config  # noqa: F401
+from .assembler import AssemblerGPU, PMatrix, PVector  # noqa: F401
+
+__all__ = ["RVEConfig", "RVEProblem", "config", "small_config", "AssemblerGPU", "PMatrix", "PVector",
+           "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

@@ -1,0 +1,131 @@
+"""ctypes declarations of the C ABI in include/pgfem_asm.h and include/pgfem_harness.h.
+
+The native library (pigfem_amd/lib/libpgfem_asm.so) is built in-tree by
+``pigfem_amd.build.build()``.  Loading fails loudly when it is missing: there is
+no Python or CPU fallback for the assembly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpgfem_asm.so")
+
+PG_FUNC = 1
+PG_JAC = 2
+PG_FP32 = 4
+PG_BATCHED = 8
+
+PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
+PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
+PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4 = range(3)
+PG_MAX_ENR, PG_MAX_CHILD, PG_MAX_COH = 12, 8, 3
+
+
+class PgMaterial(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("nstr", C.c_int32), ("D", C.c_double * 36),
+        ("sigma_c", C.c_double), ("delta_c", C.c_double), ("delta_c1", C.c_double),
+        ("delta_c2", C.c_double), ("beta2", C.c_double), ("alpha", C.c_double),
+    ]
+
+
+class PgCutRec(C.Structure):
+    _fields_ = [
+        ("enr", C.c_int32 * PG_MAX_ENR), ("isv_off", C.c_int32),
+        ("n_enr", C.c_uint8), ("n_child", C.c_uint8), ("n_coh", C.c_uint8), ("n_vqp", C.c_uint8),
+        ("child_type", C.c_uint8 * PG_MAX_CHILD), ("child_mat", C.c_uint8 * PG_MAX_CHILD),
+        ("child_nodes", (C.c_uint8 * 4) * PG_MAX_CHILD),
+        ("coh_nodes", (C.c_uint8 * 6) * PG_MAX_COH), ("coh_mat", C.c_uint8 * PG_MAX_COH),
+        ("coh_nn", C.c_uint8), ("pad", C.c_uint8 * 2),
+    ]
+
+
+assert C.sizeof(PgCutRec) == 128
+
+
+class PgProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("nen", C.c_int32), ("n_nodes", C.c_int32), ("n_enr", C.c_int32),
+        ("n_elem", C.c_int32), ("n_cut", C.c_int32), ("n_mat", C.c_int32), ("n_cqp", C.c_int32),
+        ("n_free", C.c_int32), ("n_pres", C.c_int32), ("cohesive", C.c_int32), ("pad0", C.c_int32),
+        ("nnz", C.c_int64 * 4), ("n_pairs", C.c_int64),
+        ("coords", C.POINTER(C.c_double)), ("conn", C.POINTER(C.c_int32)),
+        ("elem_mat", C.POINTER(C.c_int32)), ("cut_of_elem", C.POINTER(C.c_int32)),
+        ("cut_elem", C.POINTER(C.c_int32)), ("cut", C.POINTER(PgCutRec)),
+        ("mats", C.POINTER(PgMaterial)), ("node_dof", C.POINTER(C.c_int32)),
+        ("rp", C.POINTER(C.c_int32) * 4), ("ci", C.POINTER(C.c_int32) * 4),
+        ("npat_ptr", C.POINTER(C.c_int32)), ("npat_col", C.POINTER(C.c_int32)),
+        ("npat_row", C.POINTER(C.c_int32)), ("npat_map", C.POINTER(C.c_int32)),
+        ("n2e_ptr", C.POINTER(C.c_int32)), ("n2e", C.POINTER(C.c_int32)),
+    ]
+
+
+class PghRveParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("n", C.c_int32), ("n_incl", C.c_int32), ("cohesive", C.c_int32),
+        ("r_min", C.c_double), ("r_max", C.c_double), ("gap", C.c_double), ("seed", C.c_uint64),
+        ("E_matrix", C.c_double), ("nu_matrix", C.c_double), ("E_fiber", C.c_double), ("nu_fiber", C.c_double),
+        ("sigma_c", C.c_double), ("delta_c", C.c_double), ("delta_c1", C.c_double), ("delta_c2", C.c_double),
+        ("beta", C.c_double), ("alpha", C.c_double), ("strain", C.c_double),
+        ("max_tries", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+# exported symbols and their signatures
+SIGNATURES = {
+    # pgfem_asm.h
+    "pg_asm_create": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
+    "pg_asm_destroy": (C.c_int, [C.c_void_p]),
+    "pg_asm_last_error": (C.c_char_p, []),
+    "pg_asm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_uint,
+                                  C.c_void_p, C.c_void_p]),
+    "pg_asm_assemble_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_uint,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pg_asm_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pg_asm_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pg_asm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pg_asm_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "pg_asm_algorithmic_bytes": (C.c_int64, [C.c_void_p]),
+    # pgfem_harness.h
+    "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
+    "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),
+    "pgh_free": (None, [C.c_void_p]),
+    "pgh_last_error": (C.c_char_p, []),
+    "pgh_inclusions": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
+                                 C.POINTER(C.c_int32)]),
+    "pgh_node_detect": (C.POINTER(C.c_int32), [C.c_void_p]),
+    "pgh_bc_values": (C.POINTER(C.c_double), [C.c_void_p]),
+    "pgh_elem_qps": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree native library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"native library {LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class PgError(RuntimeError):
+    pass
+
+
+def check(rc: int, where: str) -> None:
+    if rc != 0:
+        L = lib()
+        msg = (L.pgh_last_error() if where.startswith("pgh") else L.pg_asm_last_error()) or b""
+        raise PgError(f"{where} failed ({rc}): {msg.decode(errors='replace')}")

@@ -1,0 +1,151 @@
+"""Host-side mirror of PIGFEM's Assembler plug-in API over the MI355X C ABI.
+
+`AssemblerGPU.assemble(K, P_int, delta_t, update_ISVs, solution, assembleFunc,
+assembleJac)` keeps the reference signature of Assembler::assemble
+(include/Assembler.h:46-48, src/Assembler.cpp:217-298):
+  * K is a `PMatrix` (the four AIJ blocks ff, fp, pf, pp, include/common.h:32-73),
+  * P_int a `PVector` (f, p; include/common.h:74-101),
+  * update_ISVs the trial internal variables, reset from the committed ones at
+    every call (src/Assembler.cpp:244) and updated by the cohesive laws,
+  * solution the global-dof-ordered nodal solution (NodalData),
+  * both flags False is a no-op; errors raise PgError (the reference returns a
+    PetscErrorCode / calls err_message).
+All arithmetic runs in the native library's HIP kernels; this module only moves
+pointers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, check, lib
+
+
+class PMatrix:
+    """Values of the four K blocks, stored back to back as [ff | fp | pf | pp]."""
+
+    def __init__(self, problem):
+        self.problem = problem
+        self.values = np.zeros(problem.nnz_total, dtype=np.float64)
+        off = np.cumsum([0] + problem.nnz)
+        self.offsets = off
+
+    def block(self, b: int) -> np.ndarray:
+        return self.values[self.offsets[b]:self.offsets[b + 1]]
+
+    def scipy(self, b: int):
+        import scipy.sparse as sp
+        p = self.problem
+        shape = [(p.n_free, p.n_free), (p.n_free, p.n_pres), (p.n_pres, p.n_free), (p.n_pres, p.n_pres)][b]
+        return sp.csr_matrix((self.block(b), p.ci[b], p.rp[b]), shape=shape)
+
+    @property
+    def ff(self):
+        return self.scipy(0)
+
+
+class PVector:
+    """Internal-force vector split into free and prescribed parts ([Pf | Pp])."""
+
+    def __init__(self, problem):
+        self.problem = problem
+        self.values = np.zeros(problem.n_dof, dtype=np.float64)
+
+    @property
+    def f(self) -> np.ndarray:
+        return self.values[: self.problem.n_free]
+
+    @property
+    def p(self) -> np.ndarray:
+        return self.values[self.problem.n_free:]
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class AssemblerGPU:
+    """Drop-in assembler for AssemblerSmallStrainStructural on one MI355X.
+
+    mode: "fp64" -- the plain FP64 kernels (reference arithmetic, staged
+                    element matrices, deterministic node-pair reduction);
+          "mixed" -- FP32 B^T.D.B / cohesive tangents, FP64 sliver Jacobians,
+                    damage update, CSR sums and residual; class-binned.
+    """
+
+    def __init__(self, problem, mode: str = "fp64"):
+        if mode not in ("fp64", "mixed"):
+            raise ValueError(mode)
+        self.problem = problem
+        self.mode = mode
+        L = lib()
+        h = C.c_void_p()
+        check(L.pg_asm_create(C.byref(problem.desc), C.byref(h)), "pg_asm_create")
+        self._h = h
+        self.committed_isvs = np.zeros(problem.n_cqp, dtype=np.float64)
+
+    # -- reference-shaped API ------------------------------------------------
+    def flags(self, func: bool, jac: bool) -> int:
+        f = (PG_FUNC if func else 0) | (PG_JAC if jac else 0)
+        if self.mode == "mixed":
+            f |= PG_FP32 | PG_BATCHED
+        return f
+
+    def assemble(self, K: PMatrix, P_int: PVector, delta_t: float, update_ISVs: np.ndarray,
+                 solution: np.ndarray, assembleFunc: bool = True, assembleJac: bool = True) -> int:
+        if not assembleFunc and not assembleJac:
+            return 0
+        p = self.problem
+        U = np.ascontiguousarray(solution, dtype=np.float64)
+        if U.shape != (p.n_dof,):
+            raise ValueError(f"solution must have {p.n_dof} entries")
+        if update_ISVs.shape != (p.n_cqp,) or update_ISVs.dtype != np.float64:
+            raise ValueError("update_ISVs must be a float64 array of n_cqp entries")
+        rc = lib().pg_asm_assemble(
+            self._h, _ptr(U), _ptr(self.committed_isvs) if p.n_cqp else None,
+            _ptr(update_ISVs) if p.n_cqp else None, float(delta_t), self.flags(assembleFunc, assembleJac),
+            _ptr(K.values) if assembleJac else None, _ptr(P_int.values) if assembleFunc else None)
+        check(rc, "pg_asm_assemble")
+        return 0
+
+    def commit(self, update_ISVs: np.ndarray) -> None:
+        """InternalVars::copy_data_in after a converged load step (src/SolverNonlinear.cpp:301)."""
+        self.committed_isvs[:] = update_ISVs
+
+    # -- device-resident API (torch tensors on cuda) ---------------------------
+    def assemble_device(self, U, isv_c, isv_t, K, P, func=True, jac=True, stream=None, delta_t=0.0) -> None:
+        """All arguments are contiguous float64 torch cuda tensors (isv_* may be None
+        for a perfect-bond problem); launches on `stream` (torch stream or None)."""
+        def dp(t):
+            return None if t is None else C.c_void_p(t.data_ptr())
+        st = None if stream is None else C.c_void_p(stream.cuda_stream)
+        rc = lib().pg_asm_assemble_device(self._h, dp(U), dp(isv_c), dp(isv_t), float(delta_t),
+                                          self.flags(func, jac), dp(K), dp(P), st)
+        check(rc, "pg_asm_assemble_device")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().pg_asm_set_timing(self._h, 1 if on else 0), "pg_asm_set_timing")
+
+    def last_timing(self):
+        ms = (C.c_double * 3)()
+        check(lib().pg_asm_last_timing(self._h, ms), "pg_asm_last_timing")
+        return list(ms)
+
+    def set_tiles(self, pair_block: int = 0, elem_block: int = 0) -> None:
+        check(lib().pg_asm_set_tiles(self._h, int(pair_block), int(elem_block)), "pg_asm_set_tiles")
+
+    def algorithmic_bytes(self) -> int:
+        return int(lib().pg_asm_algorithmic_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            lib().pg_asm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,60 @@
+"""In-tree build of the native library (HIP kernels for gfx950 + host harness).
+
+Produces pigfem_amd/lib/libpgfem_asm.so.  hipcc cross-compiles gfx950 code
+without a GPU; the build always recompiles.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "lib")
+INC = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    # keep the compiler from emitting packed FP32 arithmetic
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + INC, "-I" + CSRC,
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I" + INC, "-I" + CSRC]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    for src, kind in (("harness.cpp", "cxx"), ("pgfem_asm.hip", "hip")):
+        s = os.path.join(CSRC, src)
+        o = os.path.join(OUT, src.rsplit(".", 1)[0] + ".o")
+        if kind == "cxx":
+            cmd = ["g++", *CXX_FLAGS, "-c", s, "-o", o]
+        else:
+            cmd = [HIPCC, *HIP_FLAGS, "-c", s, "-o", o]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(o)
+    so = os.path.join(OUT, "libpgfem_asm.so")
+    tmp = so + ".tmp"
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs])
+    os.replace(tmp, so)
+    for o in objs:
+        os.remove(o)
+    return so
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

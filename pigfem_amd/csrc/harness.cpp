@@ -1,0 +1,971 @@
+// Host set-up of synthetic IGFEM RVEs: the part of PIGFEM's Problem::init that
+// produces the static data the element assembly reads, restated on flat arrays.
+// Citations are to brandyb2/PIGFEM.
+#include "pgfem_harness.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HarnessError : std::runtime_error {
+    int code;
+    HarnessError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw HarnessError(code, msg); }
+
+// Uniform double in [a, b) from the top 53 bits of one mt19937_64 draw.
+struct Rng {
+    std::mt19937_64 g;
+    explicit Rng(uint64_t s) : g(s) {}
+    double uni(double a, double b) {
+        double u = double(g() >> 11) * 0x1.0p-53;
+        return a + (b - a) * u;
+    }
+};
+
+struct Incl {
+    double c[3];
+    double r;
+};
+
+// Per cut element, the detection output before enrichment nodes are attached.
+struct CutInfo {
+    int32_t elem;
+    int ncut;                       // cut edges (enrichment slots before mirrors)
+    uint8_t cut_edge[PG_MAX_ENR];   // local edge index per slot
+    int32_t slot_incl[PG_MAX_ENR];  // inclusion per slot
+    int nchild;
+    uint8_t child_nn[PG_MAX_CHILD];
+    uint8_t child_nodes[PG_MAX_CHILD][6];
+    int32_t child_mat[PG_MAX_CHILD];
+    int ncoh;
+    uint8_t coh_nn;
+    uint8_t coh_nodes[PG_MAX_COH][8];
+};
+
+}  // namespace
+
+struct pgh_problem {
+    pgh_rve_params prm{};
+    int dim = 0, nen = 0;
+    int32_t n_nodes = 0, n_enr = 0, n_elem = 0;
+    std::vector<double> coords;          // (n_nodes + n_enr) * dim
+    std::vector<int32_t> conn, elem_mat, cut_of_elem, cut_elem;
+    std::vector<pg_cut_rec> cut;
+    std::vector<pg_material> mats;
+    std::vector<int32_t> node_detect;
+    std::vector<Incl> incl;
+    std::vector<double> incl_c, incl_r;
+    int32_t n_cqp = 0;
+    std::vector<int32_t> node_dof;
+    int32_t n_free = 0, n_pres = 0;
+    std::vector<double> bc_val;
+    std::vector<int32_t> rp[4], ci[4];
+    std::vector<int32_t> npat_ptr, npat_col, npat_row, npat_map;
+    std::vector<int32_t> n2e_ptr, n2e;
+};
+
+namespace {
+
+// --------------------------------------------------------------------------
+// Structured mesh, src/Mesh.cpp:236-630.  Node id = x + y(Nx+1) + z(Nx+1)(Ny+1);
+// coordinate = min + iter * ((max - min) / N).
+// --------------------------------------------------------------------------
+void generate_mesh(pgh_problem& P) {
+    const int n = P.prm.n;
+    const double h = (1.0 - 0.0) / n;
+    if (P.dim == 2) {
+        P.nen = 3;
+        P.n_nodes = (n + 1) * (n + 1);
+        P.coords.resize(size_t(P.n_nodes) * 2);
+        for (int y = 0; y <= n; ++y)
+            for (int x = 0; x <= n; ++x) {
+                size_t id = size_t(x) + size_t(y) * (n + 1);
+                P.coords[id * 2 + 0] = 0.0 + unsigned(x) * h;
+                P.coords[id * 2 + 1] = 0.0 + unsigned(y) * h;
+            }
+        P.n_elem = 2 * n * n;
+        P.conn.resize(size_t(P.n_elem) * 3);
+        size_t e = 0;
+        auto id = [n](int x, int y) { return int32_t(x + y * (n + 1)); };
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                int32_t* c = &P.conn[e * 3];
+                if ((x + y) % 2 == 0) {  // src/Mesh.cpp:418-429
+                    c[0] = id(x, y); c[1] = id(x + 1, y); c[2] = id(x + 1, y + 1);
+                    c[3] = id(x, y); c[4] = id(x + 1, y + 1); c[5] = id(x, y + 1);
+                } else {                 // src/Mesh.cpp:432-441
+                    c[0] = id(x, y); c[1] = id(x + 1, y); c[2] = id(x, y + 1);
+                    c[3] = id(x + 1, y); c[4] = id(x + 1, y + 1); c[5] = id(x, y + 1);
+                }
+                e += 2;
+            }
+    } else {
+        P.nen = 4;
+        const size_t nx1 = n + 1;
+        P.n_nodes = int32_t(nx1 * nx1 * nx1);
+        P.coords.resize(size_t(P.n_nodes) * 3);
+        for (int z = 0; z <= n; ++z)
+            for (int y = 0; y <= n; ++y)
+                for (int x = 0; x <= n; ++x) {
+                    size_t id = size_t(x) + size_t(y) * nx1 + size_t(z) * nx1 * nx1;
+                    P.coords[id * 3 + 0] = 0.0 + unsigned(x) * h;
+                    P.coords[id * 3 + 1] = 0.0 + unsigned(y) * h;
+                    P.coords[id * 3 + 2] = 0.0 + unsigned(z) * h;
+                }
+        P.n_elem = 6 * n * n * n;
+        P.conn.resize(size_t(P.n_elem) * 4);
+        auto id = [nx1](int x, int y, int z) {
+            return int32_t(size_t(x) + size_t(y) * nx1 + size_t(z) * nx1 * nx1);
+        };
+        size_t e = 0;
+        for (int z = 0; z < n; ++z)
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) {
+                    // cube corners 0..7 as in src/Mesh.cpp:540-583
+                    int32_t v[8] = {id(x, y, z),         id(x + 1, y, z),
+                                    id(x + 1, y + 1, z), id(x, y + 1, z),
+                                    id(x, y, z + 1),     id(x + 1, y, z + 1),
+                                    id(x + 1, y + 1, z + 1), id(x, y + 1, z + 1)};
+                    static const int T[6][4] = {{0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6},
+                                                {0, 5, 6, 4}, {1, 5, 6, 0}, {1, 6, 2, 0}};
+                    for (int t = 0; t < 6; ++t, ++e)
+                        for (int k = 0; k < 4; ++k) P.conn[e * 4 + k] = v[T[t][k]];
+                }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Inclusion placement (synthetic input generator): radius U[r_min, r_max],
+// centre U[r, 1-r]^dim, rejected if closer than `gap` to an earlier inclusion,
+// if a mesh node falls within 1e-9 r of its surface (the reference aborts on
+// nodes within 1e-12 r, src/Mesh.cpp:2906-2912), and, for spheres, if the
+// centre's z exceeds its y: Sphere_Inclusion::get_domain_lims bounds z by
+// center[1] + r (src/InclusionSphere.cpp:163-167), so such spheres would be
+// detected only partially and the reference's edge intersection would abort.
+// --------------------------------------------------------------------------
+struct Grid {
+    double cell;
+    int nc;
+    std::vector<std::vector<int32_t>> bins;
+    int dim;
+    Grid(double cell_, int dim_) : cell(cell_), dim(dim_) {
+        nc = std::max(1, int(std::ceil(1.0 / cell)));
+        size_t nb = size_t(nc) * nc * (dim == 3 ? nc : 1);
+        bins.resize(nb);
+    }
+    int idx(double v) const { return std::min(nc - 1, std::max(0, int(std::floor(v / cell)))); }
+    size_t bin(int i, int j, int k) const { return size_t(i) + size_t(j) * nc + size_t(k) * nc * nc; }
+};
+
+bool node_near_surface(const pgh_problem& P, const Incl& I) {
+    const int n = P.prm.n;
+    const double h = 1.0 / n;
+    int lo[3], hi[3];
+    for (int d = 0; d < P.dim; ++d) {
+        lo[d] = std::max(0, int(std::floor((I.c[d] - I.r) / h)) - 1);
+        hi[d] = std::min(n, int(std::ceil((I.c[d] + I.r) / h)) + 1);
+    }
+    if (P.dim == 2) { lo[2] = 0; hi[2] = 0; }
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                double X = 0.0 + unsigned(x) * h, Y = 0.0 + unsigned(y) * h, Z = 0.0 + unsigned(z) * h;
+                double val;
+                if (P.dim == 2) {
+                    double dx = X - I.c[0], dy = Y - I.c[1];
+                    val = (std::sqrt(dx * dx + dy * dy) - I.r) / I.r;
+                } else {
+                    double gx = (X - I.c[0]) / I.r, gy = (Y - I.c[1]) / I.r, gz = (Z - I.c[2]) / I.r;
+                    val = gx * gx + gy * gy + gz * gz - 1.0;
+                }
+                if (std::fabs(val) <= 1e-9) return true;
+            }
+    return false;
+}
+
+void place_inclusions(pgh_problem& P) {
+    const pgh_rve_params& p = P.prm;
+    Rng rng(p.seed);
+    const double reach = 2.0 * p.r_max + p.gap;
+    Grid G(std::max(reach, 1e-3), P.dim);
+    long budget = p.max_tries > 0 ? p.max_tries : 50000000L;
+    long tries = 0;
+    while (int(P.incl.size()) < p.n_incl) {
+        if (++tries > budget)
+            fail(PG_E_ARG, "inclusion placement did not converge (" +
+                               std::to_string(P.incl.size()) + " of " + std::to_string(p.n_incl) + " placed)");
+        Incl I{};
+        I.r = rng.uni(p.r_min, p.r_max);
+        for (int d = 0; d < P.dim; ++d) I.c[d] = rng.uni(I.r, 1.0 - I.r);
+        if (P.dim == 3 && I.c[2] > I.c[1]) continue;
+        int ci[3] = {G.idx(I.c[0]), G.idx(I.c[1]), P.dim == 3 ? G.idx(I.c[2]) : 0};
+        bool ok = true;
+        int kz0 = P.dim == 3 ? std::max(0, ci[2] - 1) : 0, kz1 = P.dim == 3 ? std::min(G.nc - 1, ci[2] + 1) : 0;
+        for (int k = kz0; k <= kz1 && ok; ++k)
+            for (int j = std::max(0, ci[1] - 1); j <= std::min(G.nc - 1, ci[1] + 1) && ok; ++j)
+                for (int i = std::max(0, ci[0] - 1); i <= std::min(G.nc - 1, ci[0] + 1) && ok; ++i)
+                    for (int32_t o : G.bins[G.bin(i, j, k)]) {
+                        const Incl& J = P.incl[o];
+                        double d2 = 0;
+                        for (int d = 0; d < P.dim; ++d) d2 += (I.c[d] - J.c[d]) * (I.c[d] - J.c[d]);
+                        double need = I.r + J.r + p.gap;
+                        if (d2 < need * need) { ok = false; break; }
+                    }
+        if (!ok) continue;
+        if (node_near_surface(P, I)) continue;
+        G.bins[G.bin(ci[0], ci[1], ci[2])].push_back(int32_t(P.incl.size()));
+        P.incl.push_back(I);
+    }
+    for (auto& I : P.incl) {
+        for (int d = 0; d < P.dim; ++d) P.incl_c.push_back(I.c[d]);
+        P.incl_r.push_back(I.r);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Node detection, Mesh::detect_node (src/Mesh.cpp:2883-2930): inclusions are
+// checked from the last to the first; the first one containing the node wins.
+// --------------------------------------------------------------------------
+int detect_one(const pgh_problem& P, const Incl& I, const double* x) {
+    if (P.dim == 2) {
+        // Circle_Inclusion::get_domain_lims / detect_node, src/InclusionCircle.cpp:141-163
+        if (!(I.c[0] - I.r <= x[0] && I.c[1] - I.r <= x[1] && x[0] <= I.c[0] + I.r && x[1] <= I.c[1] + I.r))
+            return 0;
+        double dx = x[0] - I.c[0], dy = x[1] - I.c[1];
+        double val = (std::sqrt(dx * dx + dy * dy) - I.r) / I.r;
+        if (val > 1e-12) return 0;
+        if (val < -1e-12) return 2;
+        return 1;
+    }
+    // Sphere_Inclusion, src/InclusionSphere.cpp:163-183 (z upper bound uses center[1])
+    if (!(I.c[0] - I.r <= x[0] && I.c[1] - I.r <= x[1] && I.c[2] - I.r <= x[2] &&
+          x[0] <= I.c[0] + I.r && x[1] <= I.c[1] + I.r && x[2] <= I.c[1] + I.r))
+        return 0;
+    double gx = (x[0] - I.c[0]) / I.r, gy = (x[1] - I.c[1]) / I.r, gz = (x[2] - I.c[2]) / I.r;
+    double val = gx * gx + gy * gy + gz * gz - 1.0;
+    if (val > 1e-12) return 0;
+    if (val < -1e-12) return 2;
+    return 1;
+}
+
+void detect_nodes(pgh_problem& P) {
+    const double reach = 2.0 * P.prm.r_max + P.prm.gap;
+    Grid G(std::max(reach, 1e-3), P.dim);
+    for (size_t j = 0; j < P.incl.size(); ++j) {
+        const Incl& I = P.incl[j];
+        G.bins[G.bin(G.idx(I.c[0]), G.idx(I.c[1]), P.dim == 3 ? G.idx(I.c[2]) : 0)].push_back(int32_t(j));
+    }
+    P.node_detect.assign(P.n_nodes, -1);
+    std::vector<int32_t> cand;
+    for (int32_t nd = 0; nd < P.n_nodes; ++nd) {
+        const double* x = &P.coords[size_t(nd) * P.dim];
+        int ci[3] = {G.idx(x[0]), G.idx(x[1]), P.dim == 3 ? G.idx(x[2]) : 0};
+        cand.clear();
+        int kz0 = P.dim == 3 ? std::max(0, ci[2] - 1) : 0, kz1 = P.dim == 3 ? std::min(G.nc - 1, ci[2] + 1) : 0;
+        for (int k = kz0; k <= kz1; ++k)
+            for (int j = std::max(0, ci[1] - 1); j <= std::min(G.nc - 1, ci[1] + 1); ++j)
+                for (int i = std::max(0, ci[0] - 1); i <= std::min(G.nc - 1, ci[0] + 1); ++i)
+                    for (int32_t o : G.bins[G.bin(i, j, k)]) cand.push_back(o);
+        std::sort(cand.begin(), cand.end(), std::greater<int32_t>());
+        for (int32_t j : cand) {
+            int res = detect_one(P, P.incl[j], x);
+            if (res == 0) continue;
+            if (res == 1) fail(PG_E_MESH, "node " + std::to_string(nd) + " lies on an inclusion surface");
+            P.node_detect[nd] = j;
+            break;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Element detection tables (single inclusion per element).
+//   Tri3::detection, polygon-decomposition branch, 2 unique values with the
+//   matrix present: src/tri3.cpp:323-356.
+//   Tet4::detection, prism-decomposition branch: src/tet4.cpp:493-558, then
+//   Tet4::decomposePrisms / partitionPrism: src/tet4.cpp:673-870.
+// Output uses the cohesive (C-1) numbering; transformDetectionCohToReg folds
+// mirrors back for perfect bond (src/elem.cpp:547-559).
+// mat codes: -1 = matrix (element material), k >= 0 inclusion k.
+// --------------------------------------------------------------------------
+struct Detect {
+    int ncut = 0;
+    uint8_t cut_edge[PG_MAX_ENR];
+    std::vector<std::vector<int>> ints;
+    std::vector<int> int_mat;  // detection value whose material the child takes
+    std::vector<std::vector<int>> cohs;
+    int incl = -1;
+};
+
+void set_tri3(Detect& D, int ce0, int ce1, int sep, int other, int sep_node,
+              const int a[3], const int b[3], bool sep_lower) {
+    (void)sep_node;
+    D.ncut = 2;
+    D.cut_edge[0] = uint8_t(ce0);
+    D.cut_edge[1] = uint8_t(ce1);
+    D.int_mat = {sep, other};
+    if (sep_lower) {
+        D.ints = {{a[0], 4, 3}, {a[1], a[2], 5, 6}};
+        D.cohs = {{3, 4, 5, 6}};
+    } else {
+        D.ints = {{b[0], 6, 5}, {b[1], b[2], 3, 4}};
+        D.cohs = {{4, 3, 6, 5}};
+    }
+}
+
+bool detect_tri3(const int nd[3], Detect& D) {
+    int n0 = nd[0], n1 = nd[1], n2 = nd[2];
+    if (n0 == n1 && n1 == n2) return false;
+    int vmin = std::min(n0, std::min(n1, n2)), vmax = std::max(n0, std::max(n1, n2));
+    int uniq = 1 + (n0 != n1 ? 1 : 0) + ((n2 != n0 && n2 != n1) ? 1 : 0);
+    if (uniq != 2 || vmin >= 0)
+        fail(PG_E_UNSUPPORTED, "tri3 element cut by more than one inclusion (polygon children) is not supported");
+    D.incl = vmax;
+    if (n1 == n2) {  // node 0 separate
+        const int a[3] = {0, 1, 2};
+        set_tri3(D, 2, 0, n0, n1, 0, a, a, n0 < n1);
+    } else if (n0 == n2) {  // node 1 separate
+        const int a[3] = {1, 2, 0};
+        set_tri3(D, 0, 1, n1, n0, 1, a, a, n1 < n0);
+    } else {  // node 2 separate
+        const int a[3] = {2, 0, 1};
+        set_tri3(D, 1, 2, n2, n0, 2, a, a, n2 < n0);
+    }
+    return true;
+}
+
+// Tet4::partitionPrism, src/tet4.cpp:771-870 (including its comparison of
+// g[3] with g[5] in the second branch).
+std::vector<std::vector<int>> partition_prism(const std::vector<int>& l, const std::vector<int64_t>& g,
+                                              bool bottom_known) {
+    std::vector<std::vector<int>> r;
+    if (bottom_known) {
+        if (g[0] < g[1] && g[0] < g[2]) {
+            if (g[1] < g[2]) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[4]}, {l[0], l[4], l[5], l[3]}};
+            else r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[5]}, {l[0], l[4], l[5], l[3]}};
+        } else if (g[1] < g[0] && g[1] < g[2]) {
+            if (g[0] < g[2]) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[3]}, {l[1], l[5], l[3], l[4]}};
+            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[5]}, {l[1], l[5], l[3], l[4]}};
+        } else if (g[2] < g[0] && g[2] < g[1]) {
+            if (g[0] < g[1]) r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[3]}, {l[2], l[3], l[4], l[5]}};
+            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[4]}, {l[2], l[3], l[4], l[5]}};
+        } else {
+            fail(PG_E_MESH, "impossible case for prism decomposition");
+        }
+    } else {
+        int min_enrich = 0;
+        if (g[1] < g[2] && g[1] < g[4] && g[1] < g[5]) min_enrich = 1;
+        else if (g[2] < g[1] && g[2] < g[4] && g[3] < g[5]) min_enrich = 2;
+        else if (g[4] < g[1] && g[4] < g[2] && g[4] < g[5]) min_enrich = 4;
+        else if (g[5] < g[1] && g[5] < g[2] && g[5] < g[4]) min_enrich = 5;
+        bool a15 = (min_enrich == 1 || min_enrich == 5);
+        if (g[0] < g[3]) {
+            if (a15) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[4]}, {l[0], l[4], l[5], l[3]}};
+            else r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[5]}, {l[0], l[4], l[5], l[3]}};
+        } else if (g[3] < g[0]) {
+            if (a15) r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[5]}, {l[1], l[5], l[3], l[4]}};
+            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[4]}, {l[2], l[3], l[4], l[5]}};
+        } else {
+            fail(PG_E_MESH, "impossible case for prism decomposition");
+        }
+    }
+    return r;
+}
+
+bool detect_tet4(const int nd[4], const int32_t gid[4], bool cohesive, Detect& D) {
+    int n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+    bool n0e1 = n0 == n1, n0e2 = n0 == n2, n0e3 = n0 == n3, n1e2 = n1 == n2, n1e3 = n1 == n3, n2e3 = n2 == n3;
+    std::vector<int> vec = {n0, n1, n2, n3};
+    std::sort(vec.begin(), vec.end());
+    vec.erase(std::unique(vec.begin(), vec.end()), vec.end());
+    if (vec.size() == 1) return false;
+    if (vec.size() > 2) fail(PG_E_UNSUPPORTED, "tet4 element cut by more than one interface");
+    if (vec[0] >= 0) fail(PG_E_UNSUPPORTED, "tet4 element cut by two inclusions is not supported");
+    int count0 = int(std::count(nd, nd + 4, vec[0]));
+    std::vector<std::vector<int>> ints, cohs;
+    std::vector<int> mats;
+    const std::vector<int> c_lo = {4, 6, 5, 7, 9, 8}, c_hi = {4, 5, 6, 7, 8, 9};
+    if (count0 == 1 || count0 == 3) {
+        if (n1e2 && n1e3) {
+            D.ncut = 3; D.cut_edge[0] = 0; D.cut_edge[1] = 3; D.cut_edge[2] = 2; mats = {n0, n1};
+            if (n0 < n1) { ints = {{4, 5, 6, 0}, {1, 3, 2, 7, 8, 9}}; cohs = {c_lo}; }
+            else { ints = {{7, 8, 9, 0}, {1, 3, 2, 4, 5, 6}}; cohs = {c_hi}; }
+        } else if (n0e2 && n0e3) {
+            D.ncut = 3; D.cut_edge[0] = 0; D.cut_edge[1] = 1; D.cut_edge[2] = 4; mats = {n1, n0};
+            if (n1 < n0) { ints = {{4, 5, 6, 1}, {0, 2, 3, 7, 8, 9}}; cohs = {c_lo}; }
+            else { ints = {{7, 8, 9, 1}, {0, 2, 3, 4, 5, 6}}; cohs = {c_hi}; }
+        } else if (n0e1 && n0e3) {
+            D.ncut = 3; D.cut_edge[0] = 1; D.cut_edge[1] = 2; D.cut_edge[2] = 5; mats = {n2, n1};
+            if (n2 < n0) { ints = {{4, 5, 6, 2}, {1, 0, 3, 7, 8, 9}}; cohs = {c_lo}; }
+            else { ints = {{7, 8, 9, 2}, {1, 0, 3, 4, 5, 6}}; cohs = {c_hi}; }
+        } else if (n0e1 && n1e2) {
+            D.ncut = 3; D.cut_edge[0] = 3; D.cut_edge[1] = 4; D.cut_edge[2] = 5; mats = {n3, n1};
+            if (n3 < n1) { ints = {{4, 5, 6, 3}, {0, 1, 2, 7, 8, 9}}; cohs = {c_lo}; }
+            else { ints = {{7, 8, 9, 3}, {0, 1, 2, 4, 5, 6}}; cohs = {c_hi}; }
+        } else {
+            fail(PG_E_MESH, "tet4 element has 1 node separated in an unknown manner");
+        }
+    } else {
+        const std::vector<int> q_lo = {4, 5, 6, 7, 8, 9, 10, 11}, q_hi = {7, 6, 5, 4, 11, 10, 9, 8};
+        if (n0e1 && n2e3 && !n0e2) {
+            D.ncut = 4; D.cut_edge[0] = 1; D.cut_edge[1] = 2; D.cut_edge[2] = 3; D.cut_edge[3] = 4; mats = {n0, n2};
+            if (n0 < n2) { ints = {{0, 5, 6, 1, 4, 7}, {2, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
+            else { ints = {{0, 9, 10, 1, 8, 11}, {2, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
+        } else if (n1e2 && n0e3 && !n0e1) {
+            D.ncut = 4; D.cut_edge[0] = 2; D.cut_edge[1] = 0; D.cut_edge[2] = 4; D.cut_edge[3] = 5; mats = {n1, n0};
+            if (n1 < n3) { ints = {{1, 5, 6, 2, 4, 7}, {0, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
+            else { ints = {{1, 9, 10, 2, 8, 11}, {0, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
+        } else if (n0e2 && n1e3 && !n0e1) {
+            D.ncut = 4; D.cut_edge[0] = 0; D.cut_edge[1] = 1; D.cut_edge[2] = 5; D.cut_edge[3] = 3; mats = {n2, n1};
+            if (n0 < n1) { ints = {{2, 5, 6, 0, 4, 7}, {1, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
+            else { ints = {{2, 9, 10, 0, 8, 11}, {1, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
+        } else {
+            fail(PG_E_MESH, "tet4 element has 2 nodes separated in an unknown manner");
+        }
+        if (cohesive) fail(PG_E_UNSUPPORTED, "3D cohesive quadrilateral interfaces (CohQuad4) are not supported");
+    }
+    D.incl = vec[1];
+    // decomposePrisms, src/tet4.cpp:673-735
+    for (size_t o = 0; o < ints.size(); ++o) {
+        if (ints[o].size() == 6) {
+            std::vector<int64_t> ids(6);
+            int known = 0;
+            for (int k = 0; k < 6; ++k) {
+                if (ints[o][k] < 4) { ids[k] = gid[ints[o][k]]; ++known; }
+                else ids[k] = ints[o][k];
+            }
+            auto dec = partition_prism(ints[o], ids, known != 2);
+            for (auto& t : dec) { D.ints.push_back(t); D.int_mat.push_back(mats[o]); }
+        } else {
+            D.ints.push_back(ints[o]);
+            D.int_mat.push_back(mats[o]);
+        }
+    }
+    D.cohs = cohs;
+    return true;
+}
+
+// --------------------------------------------------------------------------
+// Edge intersection, Inclusion::find_intersection (src/Inclusion.cpp:32-120):
+// GSL Steffensen iteration from alpha = 0.5 (GSL 2.1 roots/steffenson.c, the
+// published algorithm: Newton steps, Aitken delta^2 acceleration from the 3rd
+// iterate on), stopping when |alpha - alpha_prev| < 1e-7, 25 iterations max.
+// The reference then falls back to GSL Brent on [0,1]; here the fallback is
+// bisection to the same 1e-7 bracket (a path the generated RVEs do not take).
+// --------------------------------------------------------------------------
+struct EdgeFn {
+    const double* x0; const double* x1; const Incl* I; int dim;
+    void fdf(double a, double& f, double& df) const {
+        double g[3], dd[3];
+        for (int d = 0; d < dim; ++d) {
+            g[d] = x0[d] + a * (x1[d] - x0[d]) - I->c[d];
+            dd[d] = x1[d] - x0[d];
+        }
+        if (dim == 2) {
+            // Circle_Inclusion::int_f / int_df, src/InclusionCircle.cpp:248-287
+            f = (g[0] / I->r) * (g[0] / I->r) + (g[1] / I->r) * (g[1] / I->r) - 1;
+            df = (1 / (I->r * I->r)) * 2.0 * (g[0] * dd[0] + g[1] * dd[1]);
+        } else {
+            f = (g[0] / I->r) * (g[0] / I->r) + (g[1] / I->r) * (g[1] / I->r) + (g[2] / I->r) * (g[2] / I->r) - 1.0;
+            df = (1 / (I->r * I->r)) * 2.0 * (g[0] * dd[0] + g[1] * dd[1] + g[2] * dd[2]);
+        }
+    }
+    double f(double a) const { double F, D; fdf(a, F, D); return F; }
+};
+
+double intersect_edge(const EdgeFn& fn) {
+    if (fn.f(0.0) * fn.f(1.0) >= 0) fail(PG_E_MESH, "no valid intersection on a cut edge");
+    double x = 0.5, x_1 = 0.0, fx, dfx;
+    fn.fdf(x, fx, dfx);
+    int count = 1, iter = 0;
+    double root = 0.5, alpha = 0.5, alpha0;
+    bool done = false;
+    do {
+        ++iter;
+        if (dfx == 0.0) break;
+        double x_new = x - fx / dfx, f_new, df_new;
+        fn.fdf(x_new, f_new, df_new);
+        double x1_old = x_1;
+        x_1 = x;
+        double xo = x;
+        x = x_new;
+        fx = f_new;
+        dfx = df_new;
+        if (count < 3) { root = x_new; ++count; }
+        else {
+            double u = xo - x1_old, v = x_new - 2 * xo + x1_old;
+            root = (v == 0) ? x_new : x1_old - u * u / v;
+        }
+        alpha0 = alpha;
+        alpha = root;
+        done = std::fabs(alpha - alpha0) < 1e-7;
+    } while (!done && iter < 25);
+    if (iter == 25 || alpha < 0.0 || alpha > 1.0 || !done) {
+        double lo = 0.0, hi = 1.0, flo = fn.f(lo);
+        for (int it = 0; it < 100 && hi - lo >= 1e-7; ++it) {
+            double mid = 0.5 * (lo + hi), fm = fn.f(mid);
+            if ((fm < 0) == (flo < 0)) { lo = mid; flo = fm; } else hi = mid;
+        }
+        alpha = 0.5 * (lo + hi);
+    }
+    return alpha;
+}
+
+const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};                          // src/tri3.cpp:22-27
+const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};  // src/tet4.cpp:22-30
+
+void build_materials(pgh_problem& P) {
+    const pgh_rve_params& p = P.prm;
+    auto lei = [&](double E, double nu) {
+        pg_material m{};
+        m.type = PG_MAT_LEI;
+        // LinearElasticIsotropicMaterial::computeLinearDmat, src/material_lei.cpp:52-97
+        if (P.dim == 2) {
+            m.nstr = 3;
+            double coef = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+            m.D[0] = coef * (1.0 - nu); m.D[1] = coef * nu;
+            m.D[3] = coef * nu;         m.D[4] = coef * (1.0 - nu);
+            m.D[8] = coef * (1.0 - 2.0 * nu) / 2.0;
+        } else {
+            m.nstr = 6;
+            double coef = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) m.D[i * 6 + j] = coef * (i == j ? (1.0 - nu) : nu);
+            for (int i = 3; i < 6; ++i) m.D[i * 6 + i] = coef * (1.0 - 2.0 * nu) / 2;
+        }
+        return m;
+    };
+    P.mats.push_back(lei(p.E_matrix, p.nu_matrix));  // 0: matrix
+    P.mats.push_back(lei(p.E_fiber, p.nu_fiber));    // 1: fiber
+    if (p.cohesive != PGH_COH_NONE) {
+        pg_material m{};
+        m.sigma_c = p.sigma_c;
+        m.beta2 = p.beta * p.beta;
+        switch (p.cohesive) {
+            case PGH_COH_BL: m.type = PG_MAT_BL; break;
+            case PGH_COH_BL_NU: m.type = PG_MAT_BL_NU; break;
+            case PGH_COH_OP: m.type = PG_MAT_OP; break;
+            case PGH_COH_OP_NU: m.type = PG_MAT_OP_NU; break;
+            default: fail(PG_E_ARG, "unknown cohesive law");
+        }
+        if (m.type == PG_MAT_BL || m.type == PG_MAT_BL_NU) {
+            m.delta_c1 = p.delta_c1;
+            m.delta_c2 = p.delta_c2;
+            m.alpha = p.alpha;
+            if (!(m.delta_c2 >= m.delta_c1)) fail(PG_E_ARG, "bilinear law needs delta_c2 >= delta_c1");
+        } else {
+            m.delta_c = p.delta_c;
+        }
+        P.mats.push_back(m);  // 2: cohesive
+    }
+}
+
+void analyze_and_enrich(pgh_problem& P) {
+    const bool coh = P.prm.cohesive != PGH_COH_NONE;
+    const int nen = P.nen;
+    P.elem_mat.assign(P.n_elem, 0);
+    P.cut_of_elem.assign(P.n_elem, -1);
+    std::vector<CutInfo> infos;
+    std::vector<int> nd(nen);
+    // Mesh::analyze_elements, src/Mesh.cpp:2962-3071
+    for (int32_t e = 0; e < P.n_elem; ++e) {
+        const int32_t* c = &P.conn[size_t(e) * nen];
+        for (int k = 0; k < nen; ++k) nd[k] = P.node_detect[c[k]];
+        Detect D;
+        bool cut = (nen == 3) ? detect_tri3(nd.data(), D) : detect_tet4(nd.data(), c, coh, D);
+        if (!cut) {
+            if (nd[0] != -1) P.elem_mat[e] = 1;
+            continue;
+        }
+        CutInfo ci{};
+        ci.elem = e;
+        ci.ncut = D.ncut;
+        for (int s = 0; s < D.ncut; ++s) { ci.cut_edge[s] = D.cut_edge[s]; ci.slot_incl[s] = D.incl; }
+        ci.nchild = int(D.ints.size());
+        if (ci.nchild > PG_MAX_CHILD) fail(PG_E_UNSUPPORTED, "too many integration children");
+        for (int k = 0; k < ci.nchild; ++k) {
+            ci.child_nn[k] = uint8_t(D.ints[k].size());
+            for (size_t j = 0; j < D.ints[k].size(); ++j) {
+                int v = D.ints[k][j];
+                if (!coh && v >= nen + D.ncut) v -= D.ncut;  // transformDetectionCohToReg
+                ci.child_nodes[k][j] = uint8_t(v);
+            }
+            ci.child_mat[k] = D.int_mat[k] < 0 ? 0 : 1;
+        }
+        if (coh) {
+            ci.ncoh = int(D.cohs.size());
+            ci.coh_nn = uint8_t(D.cohs[0].size());
+            for (int k = 0; k < ci.ncoh; ++k)
+                for (size_t j = 0; j < D.cohs[k].size(); ++j) ci.coh_nodes[k][j] = uint8_t(D.cohs[k][j]);
+        }
+        infos.push_back(ci);
+    }
+
+    // Mesh::add_enrichments serial branch, src/Mesh.cpp:3104-3251: unique
+    // (sorted edge, inclusion) triples in std::set order.
+    struct Key { int32_t lo, hi, inc; };
+    std::vector<Key> keys;
+    auto edge_of = [&](const CutInfo& ci, int s, int32_t& lo, int32_t& hi) {
+        const int32_t* c = &P.conn[size_t(ci.elem) * nen];
+        int le = ci.cut_edge[s];
+        int a = nen == 3 ? TRI_EDGE[le][0] : TET_EDGE[le][0];
+        int b = nen == 3 ? TRI_EDGE[le][1] : TET_EDGE[le][1];
+        lo = std::min(c[a], c[b]);
+        hi = std::max(c[a], c[b]);
+    };
+    for (auto& ci : infos)
+        for (int s = 0; s < ci.ncut; ++s) {
+            Key k; edge_of(ci, s, k.lo, k.hi); k.inc = ci.slot_incl[s];
+            keys.push_back(k);
+        }
+    auto less = [](const Key& a, const Key& b) {
+        if (a.lo != b.lo) return a.lo < b.lo;
+        if (a.hi != b.hi) return a.hi < b.hi;
+        return a.inc < b.inc;
+    };
+    std::sort(keys.begin(), keys.end(), less);
+    keys.erase(std::unique(keys.begin(), keys.end(),
+                           [](const Key& a, const Key& b) { return a.lo == b.lo && a.hi == b.hi && a.inc == b.inc; }),
+               keys.end());
+    const int32_t n_init = int32_t(keys.size());
+    P.n_enr = coh ? 2 * n_init : n_init;
+    P.coords.resize(size_t(P.n_nodes + P.n_enr) * P.dim);
+    for (int32_t k = 0; k < n_init; ++k) {
+        const double* x0 = &P.coords[size_t(keys[k].lo) * P.dim];
+        const double* x1 = &P.coords[size_t(keys[k].hi) * P.dim];
+        EdgeFn fn{x0, x1, &P.incl[keys[k].inc], P.dim};
+        double a = intersect_edge(fn);
+        double* xe = &P.coords[size_t(P.n_nodes + k) * P.dim];
+        for (int d = 0; d < P.dim; ++d) xe[d] = x0[d] + a * (x1[d] - x0[d]);
+        if (coh) {
+            double* xm = &P.coords[size_t(P.n_nodes + n_init + k) * P.dim];
+            for (int d = 0; d < P.dim; ++d) xm[d] = xe[d];
+        }
+    }
+
+    // Records: attach enrichment nodes per cut-edge slot (add_enrich_node_to_elem,
+    // src/Mesh.cpp:3661-3720), mirrors at slot + ncut; qp and ISV bookkeeping.
+    P.cut.resize(infos.size());
+    P.cut_elem.resize(infos.size());
+    int32_t isv = 0;
+    for (size_t i = 0; i < infos.size(); ++i) {
+        const CutInfo& ci = infos[i];
+        pg_cut_rec& R = P.cut[i];
+        std::memset(&R, 0, sizeof(R));
+        for (int s = 0; s < ci.ncut; ++s) {
+            Key k; edge_of(ci, s, k.lo, k.hi); k.inc = ci.slot_incl[s];
+            auto it = std::lower_bound(keys.begin(), keys.end(), k, less);
+            int32_t idx = int32_t(it - keys.begin());
+            R.enr[s] = P.n_nodes + idx;
+            if (coh) R.enr[s + ci.ncut] = P.n_nodes + n_init + idx;
+        }
+        R.n_enr = uint8_t(coh ? 2 * ci.ncut : ci.ncut);
+        R.n_child = uint8_t(ci.nchild);
+        int nvqp = 0;
+        for (int k = 0; k < ci.nchild; ++k) {
+            int nn = ci.child_nn[k];
+            R.child_type[k] = uint8_t(nn == 3 ? PG_CHILD_TRI3 : (P.dim == 2 ? PG_CHILD_QUAD4 : PG_CHILD_TET4));
+            R.child_mat[k] = uint8_t(ci.child_mat[k]);
+            for (int j = 0; j < nn; ++j) R.child_nodes[k][j] = ci.child_nodes[k][j];
+            nvqp += (R.child_type[k] == PG_CHILD_TET4) ? 5 : 4;
+        }
+        R.n_vqp = uint8_t(nvqp);
+        R.n_coh = uint8_t(coh ? ci.ncoh : 0);
+        R.coh_nn = ci.coh_nn;
+        R.isv_off = isv;
+        for (int k = 0; k < R.n_coh; ++k) {
+            for (int j = 0; j < ci.coh_nn; ++j) R.coh_nodes[k][j] = ci.coh_nodes[k][j];
+            R.coh_mat[k] = 2;
+            isv += (ci.coh_nn == 4) ? 2 : 4;  // CohEdge2 2-pt Gauss, CohTri3 4-pt rule
+        }
+        P.cut_elem[i] = ci.elem;
+        P.cut_of_elem[ci.elem] = int32_t(i);
+        // base material of a cut element: the matrix (analyze_elements leaves it)
+        P.elem_mat[ci.elem] = 0;
+    }
+    P.n_cqp = isv;
+}
+
+// --------------------------------------------------------------------------
+// Boundary conditions (pattern of Tests/Main.cpp:562-570) and free-first dof
+// numbering (DofObject::set_local_owned_normal_dofs / _enrich_dofs,
+// src/DofObject.cpp:336-388, 498-538).
+// --------------------------------------------------------------------------
+void number_dofs(pgh_problem& P) {
+    const int d = P.dim;
+    const int n = P.prm.n;
+    const int32_t nall = P.n_nodes + P.n_enr;
+    std::vector<int8_t> has_bc(size_t(nall) * d, 0);
+    std::vector<double> val(size_t(nall) * d, 0.0);
+    const double app = (1.0 - 0.0) * P.prm.strain;
+    for (int32_t nd = 0; nd < P.n_nodes; ++nd) {
+        int x = int(nd % (n + 1));
+        if (x == 0) { has_bc[size_t(nd) * d] = 1; val[size_t(nd) * d] = 0.0; }     // LEFT, dof 0
+        if (x == n) { has_bc[size_t(nd) * d] = 1; val[size_t(nd) * d] = app; }     // RIGHT, dof 0
+    }
+    for (int j = 0; j < d; ++j) { has_bc[j] = 1; val[j] = 0.0; }                  // node 0, all dofs
+    int32_t nfree = 0, npres = 0;
+    for (size_t k = 0; k < has_bc.size(); ++k) (has_bc[k] ? npres : nfree)++;
+    P.n_free = nfree;
+    P.n_pres = npres;
+    P.node_dof.resize(size_t(nall) * d);
+    P.bc_val.assign(npres, 0.0);
+    int32_t f = 0, p = 0;
+    for (int32_t nd = 0; nd < nall; ++nd)
+        for (int j = 0; j < d; ++j) {
+            size_t k = size_t(nd) * d + j;
+            if (has_bc[k]) { P.node_dof[k] = nfree + p; P.bc_val[p] = val[k]; ++p; }
+            else P.node_dof[k] = f++;
+        }
+}
+
+// Element-local node list: parents, then enrichment nodes of a cut record.
+inline int local_nodes(const pgh_problem& P, int32_t e, int32_t* out) {
+    const int32_t* c = &P.conn[size_t(e) * P.nen];
+    int n = 0;
+    for (int k = 0; k < P.nen; ++k) out[n++] = c[k];
+    int32_t ci = P.cut_of_elem[e];
+    if (ci >= 0) {
+        const pg_cut_rec& R = P.cut[ci];
+        for (int k = 0; k < R.n_enr; ++k) out[n++] = R.enr[k];
+    }
+    return n;
+}
+
+// --------------------------------------------------------------------------
+// Node incidence, node-pair pattern, 4-block CSR (union of full element dof
+// blocks: set_matrix inserts every (i,j) of every element, explicit zeros
+// included, src/Assembler.cpp:811-831) and the scatter indices.
+// --------------------------------------------------------------------------
+void build_pattern(pgh_problem& P) {
+    const int d = P.dim;
+    const int32_t nall = P.n_nodes + P.n_enr;
+    int32_t ln[PG_MAX_ENR + 4];
+    P.n2e_ptr.assign(size_t(nall) + 1, 0);
+    for (int32_t e = 0; e < P.n_elem; ++e) {
+        int m = local_nodes(P, e, ln);
+        for (int k = 0; k < m; ++k) P.n2e_ptr[ln[k] + 1]++;
+    }
+    for (int32_t a = 0; a < nall; ++a) P.n2e_ptr[a + 1] += P.n2e_ptr[a];
+    P.n2e.resize(P.n2e_ptr[nall]);
+    {
+        std::vector<int32_t> fill(P.n2e_ptr.begin(), P.n2e_ptr.end() - 1);
+        for (int32_t e = 0; e < P.n_elem; ++e) {  // ascending element order
+            int m = local_nodes(P, e, ln);
+            for (int k = 0; k < m; ++k) P.n2e[fill[ln[k]]++] = e;
+        }
+    }
+    // node-pair pattern
+    P.npat_ptr.assign(size_t(nall) + 1, 0);
+    std::vector<int32_t> nb;
+    std::vector<std::vector<int32_t>> tmp;  // not stored; two passes instead
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int32_t a = 0; a < nall; ++a) {
+            nb.clear();
+            for (int32_t q = P.n2e_ptr[a]; q < P.n2e_ptr[a + 1]; ++q) {
+                int m = local_nodes(P, P.n2e[q], ln);
+                nb.insert(nb.end(), ln, ln + m);
+            }
+            std::sort(nb.begin(), nb.end());
+            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            if (pass == 0) P.npat_ptr[a + 1] = int32_t(nb.size());
+            else {
+                std::copy(nb.begin(), nb.end(), P.npat_col.begin() + P.npat_ptr[a]);
+                std::fill(P.npat_row.begin() + P.npat_ptr[a], P.npat_row.begin() + P.npat_ptr[a + 1], a);
+            }
+        }
+        if (pass == 0) {
+            for (int32_t a = 0; a < nall; ++a) P.npat_ptr[a + 1] += P.npat_ptr[a];
+            P.npat_col.resize(P.npat_ptr[nall]);
+            P.npat_row.resize(P.npat_ptr[nall]);
+        }
+    }
+    // dof -> (node, comp)
+    const int32_t ndof = P.n_free + P.n_pres;
+    std::vector<int32_t> dof_node(ndof), dof_comp(ndof);
+    for (int32_t a = 0; a < nall; ++a)
+        for (int j = 0; j < d; ++j) {
+            int32_t g = P.node_dof[size_t(a) * d + j];
+            dof_node[g] = a;
+            dof_comp[g] = j;
+        }
+    // row lengths per block
+    const int32_t nf = P.n_free;
+    for (int b = 0; b < 4; ++b) P.rp[b].clear();
+    P.rp[0].assign(size_t(nf) + 1, 0); P.rp[1].assign(size_t(nf) + 1, 0);
+    P.rp[2].assign(size_t(P.n_pres) + 1, 0); P.rp[3].assign(size_t(P.n_pres) + 1, 0);
+    for (int32_t r = 0; r < ndof; ++r) {
+        int32_t a = dof_node[r];
+        int64_t cf = 0, cp = 0;
+        for (int32_t q = P.npat_ptr[a]; q < P.npat_ptr[a + 1]; ++q) {
+            int32_t b = P.npat_col[q];
+            for (int j = 0; j < d; ++j) (P.node_dof[size_t(b) * d + j] < nf ? cf : cp)++;
+        }
+        if (r < nf) { P.rp[0][r + 1] = int32_t(cf); P.rp[1][r + 1] = int32_t(cp); }
+        else { P.rp[2][r - nf + 1] = int32_t(cf); P.rp[3][r - nf + 1] = int32_t(cp); }
+    }
+    int64_t base[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < 4; ++b) {
+        int64_t acc = 0;
+        for (size_t r = 1; r < P.rp[b].size(); ++r) { acc += P.rp[b][r]; }
+        if (acc >= (int64_t(1) << 31)) fail(PG_E_NOMEM, "block nnz exceeds 2^31");
+        for (size_t r = 1; r < P.rp[b].size(); ++r) P.rp[b][r] += P.rp[b][r - 1];
+        P.ci[b].resize(P.rp[b].back());
+        base[b + 1] = base[b] + P.rp[b].back();
+    }
+    if (base[4] >= (int64_t(1) << 31)) fail(PG_E_NOMEM, "total nnz exceeds 2^31");
+    std::vector<int32_t> cols;
+    for (int32_t r = 0; r < ndof; ++r) {
+        int32_t a = dof_node[r];
+        cols.clear();
+        for (int32_t q = P.npat_ptr[a]; q < P.npat_ptr[a + 1]; ++q) {
+            int32_t b = P.npat_col[q];
+            for (int j = 0; j < d; ++j) cols.push_back(P.node_dof[size_t(b) * d + j]);
+        }
+        std::sort(cols.begin(), cols.end());
+        int bl = r < nf ? 0 : 2, br = r < nf ? 1 : 3;
+        int32_t row = r < nf ? r : r - nf;
+        int32_t pl = P.rp[bl][row], pr = P.rp[br][row];
+        for (int32_t c : cols) {
+            if (c < nf) P.ci[bl][pl++] = c;
+            else P.ci[br][pr++] = c - nf;
+        }
+    }
+    // scatter indices per node pair and (i, j)
+    const int64_t npairs = int64_t(P.npat_col.size());
+    P.npat_map.resize(size_t(npairs) * d * d);
+    for (int64_t q = 0; q < npairs; ++q) {
+        int32_t a = P.npat_row[q], b = P.npat_col[q];
+        for (int i = 0; i < d; ++i) {
+            int32_t r = P.node_dof[size_t(a) * d + i];
+            for (int j = 0; j < d; ++j) {
+                int32_t c = P.node_dof[size_t(b) * d + j];
+                int blk = (r < nf ? 0 : 2) + (c < nf ? 0 : 1);
+                int32_t row = r < nf ? r : r - nf;
+                int32_t col = c < nf ? c : c - nf;
+                const int32_t* beg = P.ci[blk].data() + P.rp[blk][row];
+                const int32_t* end = P.ci[blk].data() + P.rp[blk][row + 1];
+                const int32_t* it = std::lower_bound(beg, end, col);
+                if (it == end || *it != col) fail(PG_E_MESH, "scatter index not found in pattern");
+                P.npat_map[size_t(q) * d * d + i * d + j] = int32_t(base[blk] + (it - P.ci[blk].data()));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pgh_last_error(void) { return g_err.c_str(); }
+
+int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
+    if (!p || !out) { g_err = "null argument"; return PG_E_ARG; }
+    *out = nullptr;
+    pgh_problem* P = new (std::nothrow) pgh_problem();
+    if (!P) { g_err = "out of memory"; return PG_E_NOMEM; }
+    try {
+        P->prm = *p;
+        P->dim = p->dim;
+        if (p->dim != 2 && p->dim != 3) fail(PG_E_ARG, "dim must be 2 or 3");
+        if (p->n < 1) fail(PG_E_ARG, "n must be >= 1");
+        if (p->n_incl < 0 || p->r_min <= 0 || p->r_max < p->r_min || 2 * p->r_max >= 1.0)
+            fail(PG_E_ARG, "bad inclusion parameters");
+        if (p->dim == 3 && p->cohesive != PGH_COH_NONE)
+            fail(PG_E_UNSUPPORTED, "3D cohesive RVEs are not supported");
+        generate_mesh(*P);
+        place_inclusions(*P);
+        detect_nodes(*P);
+        build_materials(*P);
+        analyze_and_enrich(*P);
+        number_dofs(*P);
+        build_pattern(*P);
+    } catch (const HarnessError& e) {
+        g_err = e.what();
+        delete P;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of memory";
+        delete P;
+        return PG_E_NOMEM;
+    }
+    *out = P;
+    return PG_OK;
+}
+
+int pgh_get_desc(const pgh_problem* h, pg_problem_desc* o) {
+    if (!h || !o) { g_err = "null argument"; return PG_E_ARG; }
+    std::memset(o, 0, sizeof(*o));
+    o->dim = h->dim;
+    o->nen = h->nen;
+    o->n_nodes = h->n_nodes;
+    o->n_enr = h->n_enr;
+    o->n_elem = h->n_elem;
+    o->n_cut = int32_t(h->cut.size());
+    o->n_mat = int32_t(h->mats.size());
+    o->n_cqp = h->n_cqp;
+    o->n_free = h->n_free;
+    o->n_pres = h->n_pres;
+    o->cohesive = h->prm.cohesive != PGH_COH_NONE;
+    for (int b = 0; b < 4; ++b) {
+        o->nnz[b] = int64_t(h->ci[b].size());
+        o->rp[b] = h->rp[b].data();
+        o->ci[b] = h->ci[b].data();
+    }
+    o->n_pairs = int64_t(h->npat_col.size());
+    o->coords = h->coords.data();
+    o->conn = h->conn.data();
+    o->elem_mat = h->elem_mat.data();
+    o->cut_of_elem = h->cut_of_elem.data();
+    o->cut_elem = h->cut_elem.data();
+    o->cut = h->cut.data();
+    o->mats = h->mats.data();
+    o->node_dof = h->node_dof.data();
+    o->npat_ptr = h->npat_ptr.data();
+    o->npat_col = h->npat_col.data();
+    o->npat_row = h->npat_row.data();
+    o->npat_map = h->npat_map.data();
+    o->n2e_ptr = h->n2e_ptr.data();
+    o->n2e = h->n2e.data();
+    return PG_OK;
+}
+
+void pgh_free(pgh_problem* h) { delete h; }
+
+int pgh_inclusions(const pgh_problem* h, const double** centers, const double** radii, int32_t* n) {
+    if (!h) return PG_E_ARG;
+    if (centers) *centers = h->incl_c.data();
+    if (radii) *radii = h->incl_r.data();
+    if (n) *n = int32_t(h->incl.size());
+    return PG_OK;
+}
+
+const int32_t* pgh_node_detect(const pgh_problem* h) { return h ? h->node_detect.data() : nullptr; }
+
+const double* pgh_bc_values(const pgh_problem* h) { return h ? h->bc_val.data() : nullptr; }
+
+int pgh_elem_qps(const pgh_problem* h, int32_t e, int32_t* n_vqp, int32_t* n_cqp) {
+    if (!h || e < 0 || e >= h->n_elem) return PG_E_ARG;
+    int32_t ci = h->cut_of_elem[e];
+    if (ci < 0) {
+        if (n_vqp) *n_vqp = h->nen == 3 ? 4 : 5;
+        if (n_cqp) *n_cqp = 0;
+    } else {
+        const pg_cut_rec& R = h->cut[ci];
+        if (n_vqp) *n_vqp = R.n_vqp;
+        if (n_cqp) *n_cqp = R.n_coh * (R.coh_nn == 4 ? 2 : 4);
+    }
+    return PG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Synthetic IGFEM RVE problems (host set-up) and the benchmark configurations.
+
+`RVEProblem` wraps the native harness (include/pgfem_harness.h), which restates
+PIGFEM's Problem::init set-up on flat arrays: structured mesh
+(src/Mesh.cpp:236-630), inclusion detection (src/Mesh.cpp:2883-2930), cut
+classification and child topology (src/tri3.cpp:287-596, src/tet4.cpp:228-520),
+enrichment nodes (src/Mesh.cpp:3104-3251), dofs (src/DofObject.cpp:241-538) and
+the 4-block AIJ pattern.  Array views are zero-copy numpy views of the native
+buffers and live as long as the problem object.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import PgProblemDesc, PghRveParams, check, lib
+
+
+@dataclass
+class RVEConfig:
+    """Generator parameters of one RVE (units: MPa, unit-length domain)."""
+
+    dim: int = 2
+    n: int = 100
+    n_incl: int = 16
+    r_min: float = 0.03
+    r_max: float = 0.05
+    gap_h: float = 2.0           # minimum surface gap in element sizes h = 1/n
+    cohesive: int = _ffi.PGH_COH_NONE
+    seed: int = 1
+    E_matrix: float = 3.5e3
+    nu_matrix: float = 0.35
+    E_fiber: float = 7.0e4
+    nu_fiber: float = 0.2
+    sigma_c: float = 50.0
+    delta_c: float = 1e-3
+    K0: float = 1e6
+    beta: float = 1.0
+    alpha: float = 50.0
+    strain: float = 0.02
+    name: str = "rve"
+
+    def params(self) -> PghRveParams:
+        p = PghRveParams()
+        p.dim, p.n, p.n_incl, p.cohesive = self.dim, self.n, self.n_incl, self.cohesive
+        p.r_min, p.r_max = self.r_min, self.r_max
+        h = 1.0 / self.n
+        p.gap = self.gap_h * h * (math.sqrt(3.0) if self.dim == 3 else 1.0)
+        p.seed = self.seed
+        p.E_matrix, p.nu_matrix, p.E_fiber, p.nu_fiber = self.E_matrix, self.nu_matrix, self.E_fiber, self.nu_fiber
+        p.sigma_c = self.sigma_c
+        p.delta_c = self.delta_c
+        # bilinear: peak opening sigma_c / K0, full separation delta_c
+        p.delta_c1 = self.sigma_c / self.K0
+        p.delta_c2 = self.delta_c
+        p.beta, p.alpha, p.strain = self.beta, self.alpha, self.strain
+        p.max_tries = 0
+        return p
+
+
+# The five workloads (sizes and laws of the reference RVE studies).
+def config(name: str) -> RVEConfig:
+    name = name.upper()
+    if name == "C1":   # 100x100 tri3, 16 fibers, perfect bond
+        return RVEConfig(dim=2, n=100, n_incl=16, r_min=0.03, r_max=0.05, seed=1, name="C1")
+    if name == "C2":   # 500x500 tri3, 200 fibers, bilinear cohesive
+        return RVEConfig(dim=2, n=500, n_incl=200, r_min=0.01, r_max=0.02, seed=2,
+                         cohesive=_ffi.PGH_COH_BL, name="C2")
+    if name == "C3":   # 2000x2000 tri3, 3000 fibers, Ortiz-Pandolfi cohesive
+        return RVEConfig(dim=2, n=2000, n_incl=3000, r_min=0.004, r_max=0.008, seed=3,
+                         cohesive=_ffi.PGH_COH_OP, name="C3")
+    if name == "C4":   # 120^3 cubes x 6 tet4, 500 spheres, perfect bond
+        return RVEConfig(dim=3, n=120, n_incl=500, r_min=0.02, r_max=0.05, seed=4, name="C4")
+    raise KeyError(name)
+
+
+def small_config(kind: str = "c0", n: int = 24, n_incl: int = 3, seed: int = 11) -> RVEConfig:
+    """Small RVEs for parity tests (oracle finishes in milliseconds)."""
+    if kind == "c0":
+        return RVEConfig(dim=2, n=n, n_incl=n_incl, r_min=0.08, r_max=0.14, seed=seed, name=f"small-c0-{n}")
+    if kind == "bl":
+        return RVEConfig(dim=2, n=n, n_incl=n_incl, r_min=0.08, r_max=0.14, seed=seed,
+                         cohesive=_ffi.PGH_COH_BL, name=f"small-bl-{n}")
+    if kind == "op":
+        return RVEConfig(dim=2, n=n, n_incl=n_incl, r_min=0.08, r_max=0.14, seed=seed,
+                         cohesive=_ffi.PGH_COH_OP, name=f"small-op-{n}")
+    if kind == "tet":
+        return RVEConfig(dim=3, n=n, n_incl=n_incl, r_min=0.12, r_max=0.2, seed=seed, name=f"small-tet-{n}")
+    raise KeyError(kind)
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(int(n),)).view(dtype)
+
+
+class RVEProblem:
+    """A built RVE: the static problem description plus numpy views of it."""
+
+    def __init__(self, cfg: RVEConfig):
+        self.cfg = cfg
+        L = lib()
+        h = C.c_void_p()
+        self._params = cfg.params()
+        check(L.pgh_build_rve(C.byref(self._params), C.byref(h)), "pgh_build_rve")
+        self._h = h
+        self.desc = PgProblemDesc()
+        check(L.pgh_get_desc(h, C.byref(self.desc)), "pgh_get_desc")
+        d = self.desc
+        self.dim, self.nen = d.dim, d.nen
+        self.n_nodes, self.n_enr, self.n_elem, self.n_cut = d.n_nodes, d.n_enr, d.n_elem, d.n_cut
+        self.n_free, self.n_pres, self.n_cqp = d.n_free, d.n_pres, d.n_cqp
+        self.n_dof = self.n_free + self.n_pres
+        self.nnz = [int(d.nnz[b]) for b in range(4)]
+        self.nnz_total = sum(self.nnz)
+        nall = self.n_nodes + self.n_enr
+        self.coords = _arr(d.coords, nall * self.dim, np.float64).reshape(nall, self.dim)
+        self.conn = _arr(d.conn, self.n_elem * self.nen, np.int32).reshape(self.n_elem, self.nen)
+        self.elem_mat = _arr(d.elem_mat, self.n_elem, np.int32)
+        self.cut_of_elem = _arr(d.cut_of_elem, self.n_elem, np.int32)
+        self.cut_elem = _arr(d.cut_elem, self.n_cut, np.int32)
+        self.node_dof = _arr(d.node_dof, nall * self.dim, np.int32).reshape(nall, self.dim)
+        rows = [self.n_free, self.n_free, self.n_pres, self.n_pres]
+        self.rp = [_arr(d.rp[b], rows[b] + 1, np.int32) for b in range(4)]
+        self.ci = [_arr(d.ci[b], self.nnz[b], np.int32) for b in range(4)]
+        self.n_pairs = int(d.n_pairs)
+        self.npat_ptr = _arr(d.npat_ptr, nall + 1, np.int32)
+        self.npat_col = _arr(d.npat_col, self.n_pairs, np.int32)
+        self.npat_row = _arr(d.npat_row, self.n_pairs, np.int32)
+        self.npat_map = _arr(d.npat_map, self.n_pairs * self.dim * self.dim, np.int32)
+        self.n2e_ptr = _arr(d.n2e_ptr, nall + 1, np.int32)
+        self.n2e = _arr(d.n2e, int(self.n2e_ptr[-1]), np.int32)
+        self.node_detect = _arr(L.pgh_node_detect(h), self.n_nodes, np.int32)
+        self.bc_values = _arr(L.pgh_bc_values(h), self.n_pres, np.float64)
+        cptr, rptr, ni = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.c_int32()
+        check(L.pgh_inclusions(h, C.byref(cptr), C.byref(rptr), C.byref(ni)), "pgh_inclusions")
+        self.incl_centers = _arr(cptr, ni.value * self.dim, np.float64).reshape(ni.value, self.dim)
+        self.incl_radii = _arr(rptr, ni.value, np.float64)
+        self.cut = [d.cut[i] for i in range(self.n_cut)] if self.n_cut < 200000 else None
+
+    def cut_record(self, i: int):
+        return self.desc.cut[i]
+
+    def random_u(self, seed: int = 0, scale: float = 1e-3) -> np.ndarray:
+        """U ~ U[-scale, scale] on every dof (free and prescribed, regular and enrichment)."""
+        rng = np.random.default_rng(seed)
+        return rng.uniform(-scale, scale, self.n_dof)
+
+    def stats(self) -> dict:
+        return {
+            "elements": self.n_elem, "nodes": self.n_nodes, "enrichment_nodes": self.n_enr,
+            "cut_elements": self.n_cut, "cohesive_qps": self.n_cqp, "dofs": self.n_dof,
+            "free_dofs": self.n_free, "nnz": self.nnz_total, "node_pairs": self.n_pairs,
+            "inclusions": int(len(self.incl_radii)),
+        }
+
+    def close(self):
+        if self._h:
+            lib().pgh_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

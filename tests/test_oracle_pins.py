@@ -1,0 +1,269 @@
+"""Pins of the CPU oracle against closed forms, invariants and finite differences.
+
+The reference ships no fixture that pins K or R (its Tests/ are driver programs
+without stored values), so each oracle function is pinned here by something the
+mathematics fixes, chosen so that a dropped term, a wrong sign / index or a
+transposed operand fails:
+  * CST / linear-tet stiffness from the textbook formulas (independent code path),
+    including the reference's tet weight quirk (sum of weights 1.0 -> 6x),
+  * homogeneous-material cut elements reproduce the uncut stiffness on the
+    regular dofs (child quadrature, parent gradients, child Jacobians),
+  * symmetry, rigid-body modes and P = K U for linear elasticity,
+  * closed-form traction curves of the cohesive laws, their central-difference
+    tangents, the history-variable rules and the zero-opening tangent,
+  * central-difference check of the assembled tangent of cohesive RVEs.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from pigfem_amd import RVEProblem, RVEConfig, small_config, _ffi
+from oracle.oracle import Oracle, cohesive_law, elem_cache
+from tests.helpers import full_matrix, rel, u_scale
+
+
+def _plain(dim, n, **kw):
+    return RVEProblem(RVEConfig(dim=dim, n=n, n_incl=0, r_min=0.1, r_max=0.1, **kw))
+
+
+def _D_plane_strain(E, nu):
+    c = E / ((1 + nu) * (1 - 2 * nu))
+    return c * np.array([[1 - nu, nu, 0], [nu, 1 - nu, 0], [0, 0, (1 - 2 * nu) / 2]])
+
+
+def _D_3d(E, nu):
+    c = E / ((1 + nu) * (1 - 2 * nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = c * nu
+    for i in range(3):
+        D[i, i] = c * (1 - nu)
+        D[3 + i, 3 + i] = c * (1 - 2 * nu) / 2
+    return D
+
+
+def test_cst_stiffness_matches_textbook():
+    """Plain tri3: K = A B^T D B with the textbook b_i = y_j - y_k, c_i = x_k - x_j."""
+    p = _plain(2, 3)
+    o = Oracle(p)
+    K, _, _ = o.assemble(np.zeros(p.n_dof))
+    Kf = full_matrix(p, K, o.rp, o.ci).toarray()
+    D = _D_plane_strain(p.cfg.E_matrix, p.cfg.nu_matrix)
+    Kt = np.zeros_like(Kf)
+    for e in range(p.n_elem):
+        x = p.coords[p.conn[e]]
+        A2 = (x[1, 0] - x[0, 0]) * (x[2, 1] - x[0, 1]) - (x[2, 0] - x[0, 0]) * (x[1, 1] - x[0, 1])
+        B = np.zeros((3, 6))
+        for i in range(3):
+            j, k = (i + 1) % 3, (i + 2) % 3
+            b, c = x[j, 1] - x[k, 1], x[k, 0] - x[j, 0]
+            B[0, 2 * i], B[1, 2 * i + 1], B[2, 2 * i], B[2, 2 * i + 1] = b, c, c, b
+        B /= A2
+        Ke = 0.5 * A2 * B.T @ D @ B
+        dofs = p.node_dof[p.conn[e]].ravel()
+        Kt[np.ix_(dofs, dofs)] += Ke
+    assert rel(Kf, Kt) < 1e-13
+
+
+def test_tet_stiffness_is_six_times_textbook():
+    """Plain tet4: the reference's order-3 weights sum to 1.0 while the reference
+    volume is 1/6 (src/tet4.cpp:113-121), so K = 6 V B^T D B."""
+    p = _plain(3, 1)
+    o = Oracle(p)
+    K, _, _ = o.assemble(np.zeros(p.n_dof))
+    Kf = full_matrix(p, K, o.rp, o.ci).toarray()
+    D = _D_3d(p.cfg.E_matrix, p.cfg.nu_matrix)
+    Kt = np.zeros_like(Kf)
+    for e in range(p.n_elem):
+        x = p.coords[p.conn[e]]
+        M = np.hstack([np.ones((4, 1)), x])
+        V = abs(np.linalg.det(M)) / 6.0
+        G = np.linalg.inv(M)[1:, :].T  # gradient of N_i in row i
+        B = np.zeros((6, 12))
+        for i in range(4):
+            gx, gy, gz = G[i]
+            c = 3 * i
+            B[0, c], B[1, c + 1], B[2, c + 2] = gx, gy, gz
+            B[3, c + 1], B[3, c + 2] = gz, gy
+            B[4, c], B[4, c + 2] = gz, gx
+            B[5, c], B[5, c + 1] = gy, gx
+        dofs = p.node_dof[p.conn[e]].ravel()
+        Kt[np.ix_(dofs, dofs)] += 6.0 * V * B.T @ D @ B
+    assert rel(Kf, Kt) < 1e-12
+    # and not the textbook value itself
+    assert rel(Kf, Kt / 6.0) > 1.0
+
+
+def test_plain_quadrature_weights():
+    p = _plain(2, 2)
+    o = Oracle(p)
+    c = elem_cache(o, 0)
+    stride = 2 + 3 * 6
+    w = c[0::stride][:4]
+    assert w[0] == -0.28125 and np.all(w[1:] == w[1])
+    assert abs(w.sum() - 0.5) < 1e-15
+    p3 = _plain(3, 1)
+    o3 = Oracle(p3)
+    c3 = elem_cache(o3, 0)
+    w3 = c3[0::2 + 6 * 12][:5]
+    assert abs(w3.sum() - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("kind,coh", [("c0", _ffi.PGH_COH_NONE), ("bl", _ffi.PGH_COH_BL), ("tet", _ffi.PGH_COH_NONE)])
+def test_homogeneous_cut_reproduces_uncut_regular_block(kind, coh):
+    """With fiber == matrix, the regular-dof block of every cut element integrates the
+    parent gradients over its children: it must equal the uncut stiffness."""
+    cfg = small_config(kind, n=12 if kind != "tet" else 6, n_incl=2 if kind != "tet" else 1)
+    cfg = dataclasses.replace(cfg, E_fiber=cfg.E_matrix, nu_fiber=cfg.nu_matrix)
+    p = RVEProblem(cfg)
+    assert p.n_cut > 0
+    q = RVEProblem(dataclasses.replace(cfg, n_incl=0))
+    o, oq = Oracle(p), Oracle(q)
+    U = np.zeros(p.n_dof)
+    K, _, _ = o.assemble(U, np.zeros(p.n_cqp))
+    Kq, _, _ = oq.assemble(np.zeros(q.n_dof))
+    A = full_matrix(p, K, o.rp, o.ci)
+    Aq = full_matrix(q, Kq, oq.rp, oq.ci)
+    reg = p.node_dof[: p.n_nodes].ravel()
+    regq = q.node_dof[: q.n_nodes].ravel()
+    sub = A[reg][:, reg].toarray()
+    subq = Aq[regq][:, regq].toarray()
+    assert rel(sub, subq) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["c0", "tet"])
+def test_linear_elastic_invariants(kind):
+    p = RVEProblem(small_config(kind, n=16 if kind == "c0" else 6, n_incl=3 if kind == "c0" else 1))
+    o = Oracle(p)
+    U = p.random_u(3)
+    K, P, _ = o.assemble(U)
+    A = full_matrix(p, K, o.rp, o.ci)
+    # symmetry
+    assert abs(A - A.T).max() <= 1e-12 * abs(A).max()
+    # P = K U (linear elasticity, perfect bond)
+    assert rel(P, A @ U) < 1e-12
+    # rigid translation and infinitesimal rotation of the regular nodes, enrichment dofs 0
+    for mode in range(2 if p.dim == 2 else 3):
+        V = np.zeros(p.n_dof)
+        V[p.node_dof[: p.n_nodes, mode]] = 1.0
+        _, Pr, _ = o.assemble(V)
+        assert np.abs(Pr).max() < 1e-9 * np.abs(P).max() / np.abs(U).max()
+    V = np.zeros(p.n_dof)
+    x = p.coords[: p.n_nodes]
+    V[p.node_dof[: p.n_nodes, 0]] = -x[:, 1]
+    V[p.node_dof[: p.n_nodes, 1]] = x[:, 0]
+    _, Pr, _ = o.assemble(V)
+    assert np.abs(Pr).max() < 1e-9 * np.abs(P).max() / np.abs(U).max()
+    # and a pure enrichment displacement is not a zero-energy mode
+    V = np.zeros(p.n_dof)
+    V[p.node_dof[p.n_nodes:].ravel()] = 1e-3
+    _, Pe, _ = o.assemble(V)
+    assert np.abs(Pe).max() > 0
+
+
+def _mat(kind, **kw):
+    m = _ffi.PgMaterial()
+    m.type = kind
+    m.sigma_c, m.delta_c, m.delta_c1, m.delta_c2 = 50.0, 1e-3, 5e-5, 1e-3
+    m.beta2, m.alpha = 1.0, 50.0
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def test_op_law_closed_form_and_history():
+    m = _mat(_ffi.PG_MAT_OP)
+    dc, sc = 1e-3, 50.0
+    for dn in [1e-4, 5e-4, 1e-3, 2.5e-3]:
+        _, h, t, _ = cohesive_law(m, 2, [0.0, dn], isv=0.0)
+        assert t[0] == 0.0
+        assert abs(t[1] - sc * (dn / dc) * np.exp(1 - dn / dc)) < 1e-12 * sc
+        assert h == pytest.approx(dn, rel=1e-15)        # delta_max := delta_eff on loading
+    _, _, t, _ = cohesive_law(m, 2, [0.0, dc])
+    assert t[1] == pytest.approx(sc, rel=1e-14)          # peak traction at delta_c
+    # unloading: linear to the origin from delta_max, history unchanged
+    _, h, t, D = cohesive_law(m, 2, [0.0, 4e-4], isv=8e-4)
+    tmax = sc * 0.8 * np.exp(1 - 0.8)
+    assert h == 8e-4 and t[1] == pytest.approx(tmax * 0.5, rel=1e-14)
+    assert D[1, 1] == pytest.approx(tmax / 8e-4, rel=1e-13)
+
+
+def test_bilinear_law_closed_form_contact_and_no_history_write():
+    m = _mat(_ffi.PG_MAT_BL)
+    _, h, t, _ = cohesive_law(m, 2, [0.0, 5e-5], isv=0.0)
+    assert t[1] == pytest.approx(50.0, rel=1e-14) and h == 0.0     # peak at delta_c1; Q5: never written
+    _, _, t, _ = cohesive_law(m, 2, [0.0, 2.0e-3])
+    assert t[1] == 0.0                                              # fully separated
+    _, _, t, _ = cohesive_law(m, 2, [0.0, 5.25e-4])
+    assert t[1] == pytest.approx(50.0 * (1 - (5.25e-4 - 5e-5) / (1e-3 - 5e-5)), rel=1e-13)
+    dn = -2e-5
+    _, _, t, D = cohesive_law(m, 2, [1e-6, dn])
+    assert t[1] == pytest.approx(50.0 / 50.0 * np.sinh(50.0 * dn / 5e-5), rel=1e-13)
+    assert D[1, 0] == 0.0 and D[1, 1] == pytest.approx(50.0 / 5e-5 * np.cosh(50.0 * dn / 5e-5), rel=1e-13)
+
+
+@pytest.mark.parametrize("kind", [_ffi.PG_MAT_OP, _ffi.PG_MAT_OP_NU, _ffi.PG_MAT_BL, _ffi.PG_MAT_BL_NU])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cohesive_tangent_is_derivative_of_traction(kind, dim):
+    m = _mat(kind, beta2=0.7)
+    rng = np.random.default_rng(kind * 10 + dim)
+    scale = 1e-3 if kind in (_ffi.PG_MAT_OP, _ffi.PG_MAT_OP_NU) else 3e-5
+    for _ in range(20):
+        d = rng.uniform(-scale, scale, dim)
+        d[-1] = abs(d[-1]) if kind in (_ffi.PG_MAT_OP, _ffi.PG_MAT_OP_NU) else d[-1]
+        _, _, t0, D = cohesive_law(m, dim, d)
+        h = 1e-7 * scale
+        Dfd = np.zeros((dim, dim))
+        for k in range(dim):
+            e = np.zeros(dim)
+            e[k] = h
+            Dfd[:, k] = (cohesive_law(m, dim, d + e)[2] - cohesive_law(m, dim, d - e)[2]) / (2 * h)
+        assert np.abs(D - Dfd).max() <= 1e-5 * np.abs(D).max() + 1e-9
+
+
+def test_zero_opening_tangent():
+    """Q6: an opening below 1e-20 is replaced by (1e-20, ...): the tangent is the
+    initial slope sigma_c e / delta_c (OP, beta = 1) on the diagonal."""
+    m = _mat(_ffi.PG_MAT_OP)
+    d, _, _, D = cohesive_law(m, 2, [0.0, 0.0])
+    assert np.all(d == 1e-20)
+    assert np.allclose(D, 50.0 * np.e / 1e-3 * np.eye(2), rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["op", "bl"])
+def test_assembled_tangent_central_difference(kind):
+    """K of a cohesive RVE is the derivative of P (signs of the cohesive fold, the
+    rotation, the opening and the law's tangent all enter)."""
+    cfg = small_config(kind, n=10, n_incl=1, seed=5)
+    p = RVEProblem(cfg)
+    o = Oracle(p)
+    U = p.random_u(1, scale=u_scale(cfg) * 0.3)
+    ic = np.zeros(p.n_cqp)
+    K, P, _ = o.assemble(U, ic)
+    A = full_matrix(p, K, o.rp, o.ci)
+    rng = np.random.default_rng(0)
+    enr = p.node_dof[p.n_nodes:].ravel()
+    cols = list(rng.choice(enr, 6, replace=False)) + list(rng.choice(p.n_dof, 4, replace=False))
+    h = 1e-6 * u_scale(cfg)
+    for j in cols:
+        e = np.zeros(p.n_dof)
+        e[j] = h
+        fd = (o.assemble(U + e, ic)[1] - o.assemble(U - e, ic)[1]) / (2 * h)
+        col = A[:, j].toarray().ravel()
+        assert np.abs(col - fd).max() <= 1e-5 * np.abs(col).max()
+
+
+def test_isv_trial_rules():
+    cfg = small_config("op", n=10, n_incl=1, seed=5)
+    p = RVEProblem(cfg)
+    o = Oracle(p)
+    U = p.random_u(2)
+    _, _, it = o.assemble(U, np.zeros(p.n_cqp))
+    assert np.all(it > 0)                      # OP: delta_max := delta_eff
+    big = np.full(p.n_cqp, 1.0)
+    _, _, it2 = o.assemble(U, big)
+    assert np.all(it2 == 1.0)                  # unloading keeps the committed value
+    pb = RVEProblem(small_config("bl", n=10, n_incl=1, seed=5))
+    ob = Oracle(pb)
+    _, _, itb = ob.assemble(pb.random_u(2, 1e-5), np.full(pb.n_cqp, 0.25))
+    assert np.all(itb == 0.25)                 # BL never writes delta_max (Q5)
