@@ -43,8 +43,7 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
-    from pigfem_amd import AssemblerGPU, RVEProblem, config
+    from pigfem_amd import AssemblerGPU, DeviceBuffer, RVEProblem, config, synchronize
 
     cfg = config(args.config)
     t0 = time.time()
@@ -53,54 +52,53 @@ def main():
     st = prob.stats()
     log(f"[bench] {cfg.name} built in {setup_s:.1f}s: {st}")
 
-    dev = torch.device("cuda:0")
     asm = AssemblerGPU(prob, "mixed")
     U = prob.random_u(0, 1e-3)
-    dU = torch.from_numpy(U).to(dev)
-    dIc = torch.zeros(max(prob.n_cqp, 1), dtype=torch.float64, device=dev)
-    dIt = torch.zeros_like(dIc)
-    dK = torch.zeros(prob.nnz_total, dtype=torch.float64, device=dev)
-    dP = torch.zeros(prob.n_dof, dtype=torch.float64, device=dev)
+    dU = DeviceBuffer.from_numpy(U)
+    dIc = DeviceBuffer.zeros(max(prob.n_cqp, 1))
+    dIt = DeviceBuffer.zeros(max(prob.n_cqp, 1))
+    dK = DeviceBuffer.zeros(prob.nnz_total)
+    dP = DeviceBuffer.zeros(prob.n_dof)
     ic = dIc if prob.n_cqp else None
     it = dIt if prob.n_cqp else None
-    stream = torch.cuda.current_stream()
 
     def run(mode, steps):
         asm.mode = mode
         for _ in range(steps):
-            asm.assemble_device(dU, ic, it, dK, dP, stream=stream)
+            asm.assemble_device(dU, ic, it, dK, dP)
 
     results = {}
     for mode in ("fp64", "mixed"):
         run(mode, args.warmup)
-        torch.cuda.synchronize()
+        synchronize()
         t = time.perf_counter()
         run(mode, args.steps)
-        torch.cuda.synchronize()
+        synchronize()
         ms = (time.perf_counter() - t) * 1e3 / args.steps
         results[mode] = {"ms": ms, "eps": prob.n_elem / (ms * 1e-3)}
         log(f"[bench] {mode}: {ms:.3f} ms/step, {results[mode]['eps']:.4g} elements/s")
 
     # accuracy of the mixed schedule against the FP64 schedule on the same inputs
-    asm.mode = "fp64"
-    asm.assemble_device(dU, ic, it, dK, dP, stream=stream)
-    torch.cuda.synchronize()
-    K64, P64 = dK.clone(), dP.clone()
-    I64 = dIt.clone()
-    asm.mode = "mixed"
-    asm.assemble_device(dU, ic, it, dK, dP, stream=stream)
-    torch.cuda.synchronize()
-    relK = float(torch.linalg.norm(dK - K64) / torch.linalg.norm(K64))
-    relP = float(torch.linalg.norm(dP - P64) / torch.linalg.norm(P64))
-    isv_diff = float((dIt - I64).abs().max()) if prob.n_cqp else 0.0
+    def snapshot(mode):
+        asm.mode = mode
+        asm.assemble_device(dU, ic, it, dK, dP)
+        synchronize()
+        return (dK.download(prob.nnz_total), dP.download(prob.n_dof),
+                dIt.download(prob.n_cqp) if prob.n_cqp else np.zeros(0))
+    K64, P64, I64 = snapshot("fp64")
+    K32, P32, I32 = snapshot("mixed")
+    relK = float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64))
+    relP = float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64))
+    isv_diff = float(np.abs(I32 - I64).max()) if prob.n_cqp else 0.0
     log(f"[bench] mixed vs fp64: rel K {relK:.3e}, rel P {relP:.3e}, max |isv diff| {isv_diff:.3e}")
-    del K64, P64, I64
+    del K64, P64, I64, K32, P32, I32
 
     # per-kernel durations with HIP events on the launch stream (mixed schedule)
+    asm.mode = "mixed"
     asm.set_timing(True)
     kt = []
     for _ in range(max(args.steps, 5)):
-        asm.assemble_device(dU, ic, it, dK, dP, stream=stream)
+        asm.assemble_device(dU, ic, it, dK, dP)
         kt.append(asm.last_timing())
     asm.set_timing(False)
     kt = np.array(kt)
@@ -114,11 +112,11 @@ def main():
         f"(algorithmic {pair_bytes/1e9:.3f} GB); whole assembly {whole:.0f} GB/s")
 
     if args.calib_copy:
-        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
-        b = torch.empty_like(a)
+        a = DeviceBuffer(1 << 30)
+        b = DeviceBuffer(1 << 30)
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
+            b.copy_from(a)
+        synchronize()
         log("[bench] calibration copy of 1 GiB done (3x)")
 
     cpu = None
@@ -131,7 +129,7 @@ def main():
         o.assemble(U, np.zeros(prob.n_cqp))
         rate = probe / (time.perf_counter() - t)
         o.close()
-        m = int(min(prob.n_elem, max(probe, rate * args.cpu_seconds / 1.0)))
+        m = int(min(prob.n_elem, 4_000_000, max(probe, rate * args.cpu_seconds)))
         o = Oracle(prob, 0, m)
         best = 1e30
         for _ in range(2):

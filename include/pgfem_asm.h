@@ -208,6 +208,20 @@ int pg_asm_set_tiles(pg_asm* h, int pair_block, int cut_block);
  * column indices + row pointers, R. */
 int64_t pg_asm_algorithmic_bytes(const pg_asm* h);
 
+/* ----------------------------------------------------------------------------
+ * Device buffers for callers that have no HIP allocator of their own (the
+ * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own
+ * device pointers).  No counterpart in the reference (CPU-only).  All act on
+ * HIP device 0 and are synchronous except pg_dev_copy, which is ordered on the
+ * null stream.
+ * ------------------------------------------------------------------------- */
+int pg_dev_alloc(void** ptr, int64_t bytes);
+int pg_dev_free(void* ptr);
+int pg_dev_upload(void* dst_device, const void* src_host, int64_t bytes);
+int pg_dev_download(void* dst_host, const void* src_device, int64_t bytes);
+int pg_dev_copy(void* dst_device, const void* src_device, int64_t bytes);
+int pg_dev_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

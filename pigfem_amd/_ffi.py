@@ -88,6 +88,12 @@ SIGNATURES = {
     "pg_asm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pg_asm_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "pg_asm_algorithmic_bytes": (C.c_int64, [C.c_void_p]),
+    "pg_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
+    "pg_dev_free": (C.c_int, [C.c_void_p]),
+    "pg_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "pg_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "pg_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "pg_dev_synchronize": (C.c_int, []),
     # pgfem_harness.h
     "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
     "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),

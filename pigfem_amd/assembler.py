@@ -114,13 +114,21 @@ class AssemblerGPU:
         """InternalVars::copy_data_in after a converged load step (src/SolverNonlinear.cpp:301)."""
         self.committed_isvs[:] = update_ISVs
 
-    # -- device-resident API (torch tensors on cuda) ---------------------------
+    # -- device-resident API ---------------------------------------------------
     def assemble_device(self, U, isv_c, isv_t, K, P, func=True, jac=True, stream=None, delta_t=0.0) -> None:
-        """All arguments are contiguous float64 torch cuda tensors (isv_* may be None
-        for a perfect-bond problem); launches on `stream` (torch stream or None)."""
+        """Arguments are device buffers: `DeviceBuffer`s, raw device addresses (int),
+        or objects with `data_ptr()` (isv_* may be None for a perfect-bond problem).
+        Launches asynchronously on `stream` (a hipStream_t address) or the handle's
+        own stream."""
         def dp(t):
-            return None if t is None else C.c_void_p(t.data_ptr())
-        st = None if stream is None else C.c_void_p(stream.cuda_stream)
+            if t is None:
+                return None
+            if hasattr(t, "ptr"):
+                return t.ptr
+            if hasattr(t, "data_ptr"):
+                return C.c_void_p(t.data_ptr())
+            return C.c_void_p(int(t))
+        st = None if stream is None else C.c_void_p(int(stream))
         rc = lib().pg_asm_assemble_device(self._h, dp(U), dp(isv_c), dp(isv_t), float(delta_t),
                                           self.flags(func, jac), dp(K), dp(P), st)
         check(rc, "pg_asm_assemble_device")

@@ -908,4 +908,39 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
     return PG_OK;
 }
 
+int pg_dev_alloc(void** ptr, int64_t bytes) {
+    if (!ptr || bytes < 0) { g_err = "bad argument"; return PG_E_ARG; }
+    *ptr = nullptr;
+    HIP_OK(hipMalloc(ptr, size_t(bytes > 0 ? bytes : 1)));
+    return PG_OK;
+}
+
+int pg_dev_free(void* ptr) {
+    if (ptr) HIP_OK(hipFree(ptr));
+    return PG_OK;
+}
+
+int pg_dev_upload(void* dst, const void* src, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) { g_err = "bad argument"; return PG_E_ARG; }
+    if (bytes) HIP_OK(hipMemcpy(dst, src, size_t(bytes), hipMemcpyHostToDevice));
+    return PG_OK;
+}
+
+int pg_dev_download(void* dst, const void* src, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) { g_err = "bad argument"; return PG_E_ARG; }
+    if (bytes) HIP_OK(hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_dev_copy(void* dst, const void* src, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) { g_err = "bad argument"; return PG_E_ARG; }
+    if (bytes) HIP_OK(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToDevice, nullptr));
+    return PG_OK;
+}
+
+int pg_dev_synchronize(void) {
+    HIP_OK(hipDeviceSynchronize());
+    return PG_OK;
+}
+
 }  // extern "C"
