@@ -68,7 +68,7 @@ def main():
             asm.assemble_device(dU, ic, it, dK, dP)
 
     results = {}
-    for mode in ("fp64", "mixed"):
+    for mode in ("fp64", "binned64", "mixed"):
         run(mode, args.warmup)
         synchronize()
         t = time.perf_counter()
@@ -167,6 +167,8 @@ def main():
         "fp64_elements_per_s": results["fp64"]["eps"],
         "fp64_ms_per_step": results["fp64"]["ms"],
         "speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],
+        "binned64_elements_per_s": results["binned64"]["eps"],
+        "binned64_ms_per_step": results["binned64"]["ms"],
         "rel_err_K": relK,
         "rel_err_P": relP,
         "whole_assembly_GBps": whole,

@@ -69,14 +69,17 @@ def _ptr(a):
 class AssemblerGPU:
     """Drop-in assembler for AssemblerSmallStrainStructural on one MI355X.
 
-    mode: "fp64" -- the plain FP64 kernels (reference arithmetic, staged
-                    element matrices, deterministic node-pair reduction);
-          "mixed" -- FP32 B^T.D.B / cohesive tangents, FP64 sliver Jacobians,
-                    damage update, CSR sums and residual; class-binned.
+    mode: "fp64"     -- the plain FP64 drop-in: one thread per element (reference
+                        arithmetic, no class binning), staged element matrices,
+                        deterministic node-pair reduction;
+          "binned64" -- class-binned schedule in FP64 (cut elements one wavefront
+                        each, uncut elements recomputed in the node-pair kernel);
+          "mixed"    -- class-binned with FP32 B^T.D.B / cohesive tangents and FP64
+                        sliver Jacobians, damage update, CSR sums and residual.
     """
 
     def __init__(self, problem, mode: str = "fp64"):
-        if mode not in ("fp64", "mixed"):
+        if mode not in ("fp64", "binned64", "mixed"):
             raise ValueError(mode)
         self.problem = problem
         self.mode = mode
@@ -91,6 +94,8 @@ class AssemblerGPU:
         f = (PG_FUNC if func else 0) | (PG_JAC if jac else 0)
         if self.mode == "mixed":
             f |= PG_FP32 | PG_BATCHED
+        elif self.mode == "binned64":
+            f |= PG_BATCHED
         return f
 
     def assemble(self, K: PMatrix, P_int: PVector, delta_t: float, update_ISVs: np.ndarray,

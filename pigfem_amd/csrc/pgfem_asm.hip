@@ -55,6 +55,8 @@ struct DevProb {
     const int32_t* n2e;
     const int64_t* cut_koff;   // per cut element: offset of its ndof^2 block
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
+    const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
+    int n_mat;
 };
 
 // --------------------------------------------------------------------------
@@ -67,108 +69,322 @@ __global__ void k_permute_u(int64_t n, const int32_t* __restrict__ node_dof, con
     if (k < n) Un[k] = U[node_dof[k]];
 }
 
-// --------------------------------------------------------------------------
-// Plain FP64 element kernel: one thread per uncut parent element.  Follows
-// Assembler::assemble_elem's non-intersected branch (src/Assembler.cpp:327-372)
-// with KernelVolumetric's B-matrix variant (src/AssemblerSmallStrainStructural.cpp:23-49):
-// per qp K_tmp = (B^T D) B, P_tmp = B^T sigma, K_el += (J w) K_tmp.  For a linear
-// simplex B, D and sigma are the same at every qp, so each entry is accumulated
-// over the qps in reference order from one K_tmp value.
-// Staged layout per element: Ke (nd*nd, row-major) then Pe (nd).
-// --------------------------------------------------------------------------
+constexpr int MAXN2 = 3 + 8;             // 2D local nodes (tri3 + 2x4 enrichment slots)
+constexpr int MAXN3 = 4 + PG_MAX_ENR;    // 3D local nodes
+constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
+constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
+constexpr int CUT_MAXC = PG_MAX_COH * 4; // cohesive qps per cut element
+
+template <int D> struct CutLim;
+template <> struct CutLim<2> { static constexpr int N = MAXN2, Q = MAXQ2; };
+template <> struct CutLim<3> { static constexpr int N = MAXN3, Q = MAXQ3; };
+
+// Geometry of one volumetric qp of a child: child gradients, child J and weight
+// (Elem::q_point + Elem::ShapeFunctions on the child, src/Mesh.cpp:1535-1562).
 template <int D>
-__global__ void k_plain_fp64(DevProb p, const double* __restrict__ Un, double* __restrict__ stage) {
-    constexpr int NEN = D + 1, ND = NEN * D, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n_elem) return;
-    if (p.cut_of_elem[e] >= 0) return;
-    double x[NEN][3], g[NEN][3], dNdr[NEN][D];
-    int nodes[NEN];
-    for (int n = 0; n < NEN; ++n) {
-        nodes[n] = p.conn[int64_t(e) * NEN + n];
-        for (int i = 0; i < D; ++i) x[n][i] = p.coords[int64_t(nodes[n]) * D + i];
-        for (int a = 0; a < D; ++a) dNdr[n][a] = simplex_dNdr(n, a);
+__device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3], double (*cg)[3], double& w) {
+    double cdN[4][D];
+    if (ct == PG_CHILD_QUAD4) {
+        const double r = (q % 2 == 0) ? -PG_GAUSS2 : PG_GAUSS2;
+        const double s = (q / 2 == 0) ? -PG_GAUSS2 : PG_GAUSS2;
+        double t4[4][2];
+        quad4_dNdr(r, s, t4);
+        for (int n = 0; n < 4; ++n)
+            for (int a = 0; a < D; ++a) cdN[n][a] = t4[n][a];
+        w = 1.0;
+    } else {
+        for (int n = 0; n < cnn; ++n)
+            for (int a = 0; a < D; ++a) cdN[n][a] = simplex_dNdr(n, a);
+        w = (D == 2) ? TRI_QW[q] : TET_QW[q];
     }
-    const double J = transform<D>(NEN, dNdr, x, g);
-    const pg_material& m = p.mats[p.elem_mat[e]];
-    double wq[NQ];
-    for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
-    // strain = B U (ProblemUtilities::B_U), stress (computeStress)
-    double eps[NS], sig[NS];
-    for (int k = 0; k < NS; ++k) eps[k] = 0.0;
-    for (int n = 0; n < NEN; ++n)
-        for (int c = 0; c < D; ++c) {
-            double b[NS];
-            bcol<D, double>(g[n], c, b);
-            double u = Un[int64_t(nodes[n]) * D + c];
-            for (int k = 0; k < NS; ++k) eps[k] += b[k] * u;
+    return transform<D>(cnn, cdN, cx, cg);
+}
+
+// Cohesive qp geometry: N, rotation R and dA on the lower surface
+// (CohesiveElem::ShapeFunctions / compute_rotation_general, src/CohesiveElem.cpp:145-238;
+// CohEdge2 2-pt Gauss src/CohEdge2.cpp:70-78,186-209; CohTri3 tri3 4-pt rule).
+template <int D>
+__device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], double* N, double R[D][D], double& w) {
+    double dNr[3][2];
+    const int cnn = coh_nn / 2;
+    if (coh_nn == 4) {
+        const double xi = (q == 0) ? -PG_GAUSS2 : PG_GAUSS2;
+        N[0] = 0.5 * (1.0 - xi); N[1] = 0.5 * (1 + xi);
+        dNr[0][0] = -0.5; dNr[1][0] = 0.5;
+        w = 1.0;
+    } else {
+        N[0] = 1.0 - TRI_QX[q] - TRI_QY[q]; N[1] = TRI_QX[q]; N[2] = TRI_QY[q];
+        dNr[0][0] = -1.0; dNr[0][1] = -1.0; dNr[1][0] = 1.0; dNr[1][1] = 0.0; dNr[2][0] = 0.0; dNr[2][1] = 1.0;
+        w = TRI_QW[q];
+    }
+    double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int n = 0; n < cnn; ++n)
+        for (int a = 0; a < D - 1; ++a)
+            for (int i = 0; i < D; ++i) Jc[a][i] += dNr[n][a] * lx[n][i];
+    double nrm[3];
+    if (D == 2) { nrm[0] = -Jc[0][1]; nrm[1] = Jc[0][0]; }
+    else {
+        nrm[0] = Jc[0][1] * Jc[1][2] - Jc[0][2] * Jc[1][1];
+        nrm[1] = Jc[0][2] * Jc[1][0] - Jc[0][0] * Jc[1][2];
+        nrm[2] = Jc[0][0] * Jc[1][1] - Jc[0][1] * Jc[1][0];
+    }
+    double mag[2];
+    for (int a = 0; a < D - 1; ++a) {
+        double s = 0;
+        for (int i = 0; i < D; ++i) s += Jc[a][i] * Jc[a][i];
+        mag[a] = sqrt(s);
+    }
+    double dA = 0;
+    for (int i = 0; i < D; ++i) dA += nrm[i] * nrm[i];
+    dA = sqrt(dA);
+    for (int i = 0; i < D; ++i) R[D - 1][i] = nrm[i] / dA;
+    for (int i = 0; i < D; ++i)
+        for (int a = 0; a < D - 1; ++a) R[a][i] = Jc[a][i] / mag[a];
+    return dA;
+}
+
+// One cohesive qp: opening, law (trial history in/out), R^T D R and R^T t
+// (KernelCohesive, src/AssemblerSmallStrainStructural.cpp:86-103 with
+// cohesiveDeltaFast / KFast / InternalForceFast, src/Utilities.cpp:528-594).
+template <int D>
+__device__ inline void cohesive_point(const pg_material& m, int cnn, const double* N, const double R[D][D],
+                                      const double* Ulo, const double* Uup, double& hist, double RDR[D][D],
+                                      double tR[D]) {
+    double dl[D], del[D];
+    for (int i = 0; i < D; ++i) {
+        double s = 0;
+        for (int n = 0; n < cnn; ++n) s += N[n] * (Uup[n * D + i] - Ulo[n * D + i]);
+        dl[i] = s;
+    }
+    for (int i = 0; i < D; ++i) {
+        double s = 0;
+        for (int j = 0; j < D; ++j) s += R[i][j] * dl[j];
+        del[i] = s;
+    }
+    double t[D], Dm[D][D];
+    cohesive_law<D>(m, del, hist, t, Dm);
+    double RD[D][D];
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+            for (int l = 0; l < D; ++l) s += R[l][i] * Dm[l][j];
+            RD[i][j] = s;
         }
-    lei_stress<D>(m.D, eps, sig);
-    double* Ke = stage + int64_t(e) * (ND * ND + ND);
-    double* Pe = Ke + ND * ND;
-    for (int i = 0; i < ND; ++i) {
-        double bi[NS], btd[NS];
-        bcol<D, double>(g[i / D], i % D, bi);
-        for (int l = 0; l < NS; ++l) {
-            double s = 0.0;
-            for (int k = 0; k < NS; ++k) s += bi[k] * m.D[k * NS + l];
-            btd[l] = s;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+            for (int l = 0; l < D; ++l) s += RD[i][l] * R[l][j];
+            RDR[i][j] = s;
         }
-        for (int j = 0; j < ND; ++j) {
-            double bj[NS];
-            bcol<D, double>(g[j / D], j % D, bj);
-            double kt = 0.0;
-            for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
-            double acc = 0.0;
-            for (int q = 0; q < NQ; ++q) acc += wq[q] * kt;
-            Ke[i * ND + j] = acc;
-        }
-        double pt = 0.0;
-        for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
-        double acc = 0.0;
-        for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
-        Pe[i] = acc;
+    for (int i = 0; i < D; ++i) {
+        double s = 0;
+        for (int l = 0; l < D; ++l) s += R[l][i] * t[l];
+        tR[i] = s;
     }
 }
 
 // --------------------------------------------------------------------------
-// Cut-element kernel: one wavefront (64 lanes, one block) per intersected
-// element.  Restates Mesh::store_shape_functions for intersected elements
-// (src/Mesh.cpp:1521-1641) and Assembler::assemble_elem's intersected branch
-// (src/Assembler.cpp:375-517):
-//   * per child qp: child shape gradients and J from the child's own Jacobian
-//     (FP64: children at fiber interfaces can be slivers), parent gradients
-//     (constant for linear parents), enrichment gradients = child gradient of
-//     the child node that is the enrichment node (scale factor 1.0,
-//     src/Mesh.cpp:2713-2725), weight = child J * w;
-//   * K_el += wJ B^T D B, P_el += wJ B^T sigma, in TK precision for K
-//     (double for the plain schedule, float for the mixed one), FP64 for P;
-//   * cohesive children: delta = R sum N (U+ - U-), constitutive update of the
-//     trial delta_max (FP64), K_coh = N N^T (x) R^T D R with +/- signs,
-//     P_coh = -/+ N R^T t (src/Utilities.cpp:528-594), folded into K_el / P_el
-//     through the cohesive element's local node list (src/Assembler.cpp:487-515).
+// Plain FP64 element kernel (the drop-in schedule, no class binning): one
+// thread per parent element, cut or not, following Assembler::assemble_elem
+// (src/Assembler.cpp:300-520) with KernelVolumetric's B-matrix variant: per qp
+// K_tmp = (B^T D) B, P_tmp = B^T sigma, K_el += (J w) K_tmp.  Uncut linear
+// simplices keep K_el in registers (B, D, sigma are the same at every qp, so each
+// entry accumulates the qp weights in reference order from one K_tmp value);
+// intersected elements accumulate their larger K_el directly in the staging
+// slot.  Staged layout per element: Ke (nd*nd, row-major) then Pe (nd), at
+// elem_off[e].
 // --------------------------------------------------------------------------
-constexpr int CUT_MAXN = 4 + PG_MAX_ENR;   // local nodes
-constexpr int CUT_MAXQ = 32;               // volumetric qps
-constexpr int CUT_MAXC = PG_MAX_COH * 4;   // cohesive qps
+template <int D>
+__global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const double* __restrict__ isv_c,
+                            double* __restrict__ isv_t, double* __restrict__ stage, int* __restrict__ err) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
+    constexpr int MAXN = CutLim<D>::N;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.n_elem) return;
+    double* Ke = stage + p.elem_off[e];
+    const int ci = p.cut_of_elem[e];
+    double px[NEN][3], pdN[NEN][D], pg[NEN][3];
+    int nodes[MAXN];
+    for (int n = 0; n < NEN; ++n) {
+        nodes[n] = p.conn[int64_t(e) * NEN + n];
+        for (int i = 0; i < D; ++i) px[n][i] = p.coords[int64_t(nodes[n]) * D + i];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    const double J = transform<D>(NEN, pdN, px, pg);
+    if (ci < 0) {
+        constexpr int ND = NEN * D;
+        const pg_material& m = p.mats[p.elem_mat[e]];
+        double wq[NQ];
+        for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
+        double eps[NS], sig[NS];
+        for (int k = 0; k < NS; ++k) eps[k] = 0.0;
+        for (int n = 0; n < NEN; ++n)
+            for (int c = 0; c < D; ++c) {
+                double b[NS];
+                bcol<D, double>(pg[n], c, b);
+                const double u = Un[int64_t(nodes[n]) * D + c];
+                for (int k = 0; k < NS; ++k) eps[k] += b[k] * u;
+            }
+        lei_stress<D>(m.D, eps, sig);
+        double* Pe = Ke + ND * ND;
+        for (int i = 0; i < ND; ++i) {
+            double bi[NS], btd[NS];
+            bcol<D, double>(pg[i / D], i % D, bi);
+            for (int l = 0; l < NS; ++l) {
+                double s = 0.0;
+                for (int k = 0; k < NS; ++k) s += bi[k] * m.D[k * NS + l];
+                btd[l] = s;
+            }
+            for (int j = 0; j < ND; ++j) {
+                double bj[NS];
+                bcol<D, double>(pg[j / D], j % D, bj);
+                double kt = 0.0;
+                for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
+                double acc = 0.0;
+                for (int q = 0; q < NQ; ++q) acc += wq[q] * kt;
+                Ke[i * ND + j] = acc;
+            }
+            double pt = 0.0;
+            for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
+            double acc = 0.0;
+            for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
+            Pe[i] = acc;
+        }
+        return;
+    }
+    // ---- intersected element ----
+    const pg_cut_rec& r = p.cut[ci];
+    const int nloc = NEN + r.n_enr, nd = nloc * D;
+    double* Pe = Ke + int64_t(nd) * nd;
+    double lx[MAXN][3], ue[MAXN * 3];
+    for (int l = 0; l < nloc; ++l) {
+        if (l >= NEN) nodes[l] = r.enr[l - NEN];
+        for (int i = 0; i < D; ++i) {
+            lx[l][i] = p.coords[int64_t(nodes[l]) * D + i];
+            ue[l * D + i] = Un[int64_t(nodes[l]) * D + i];
+        }
+    }
+    for (int i = 0; i < nd * nd; ++i) Ke[i] = 0.0;
+    for (int i = 0; i < nd; ++i) Pe[i] = 0.0;
+    for (int k = 0; k < r.n_child; ++k) {
+        const int ct = r.child_type[k], cnn = ct == PG_CHILD_TRI3 ? 3 : 4;
+        const pg_material& m = p.mats[r.child_mat[k]];
+        double cx[4][3], cg[4][3];
+        for (int n = 0; n < cnn; ++n)
+            for (int i = 0; i < D; ++i) cx[n][i] = lx[r.child_nodes[k][n]][i];
+        const int nq = ct == PG_CHILD_TET4 ? 5 : 4;
+        for (int q = 0; q < nq; ++q) {
+            double w;
+            const double Jc = child_qp<D>(ct, q, cnn, cx, cg, w);
+            if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
+            const double wgt = Jc * w;
+            // gradient of local node l at this qp: parent gradient, enrichment
+            // gradient from the child node that is the enrichment node, else 0
+            auto grad = [&](int l, double* g) {
+                if (l < NEN) { for (int a = 0; a < D; ++a) g[a] = pg[l][a]; return; }
+                for (int a = 0; a < D; ++a) g[a] = 0.0;
+                for (int n = 0; n < cnn; ++n)
+                    if (r.child_nodes[k][n] == l)
+                        for (int a = 0; a < D; ++a) g[a] = cg[n][a];
+            };
+            double eps[NS], sig[NS];
+            for (int s = 0; s < NS; ++s) eps[s] = 0.0;
+            for (int l = 0; l < nloc; ++l) {
+                double g[3];
+                grad(l, g);
+                for (int c = 0; c < D; ++c) {
+                    double b[NS];
+                    bcol<D, double>(g, c, b);
+                    for (int s = 0; s < NS; ++s) eps[s] += b[s] * ue[l * D + c];
+                }
+            }
+            lei_stress<D>(m.D, eps, sig);
+            for (int i = 0; i < nd; ++i) {
+                double gi[3], bi[NS], btd[NS];
+                grad(i / D, gi);
+                bcol<D, double>(gi, i % D, bi);
+                for (int l2 = 0; l2 < NS; ++l2) {
+                    double s = 0.0;
+                    for (int k2 = 0; k2 < NS; ++k2) s += bi[k2] * m.D[k2 * NS + l2];
+                    btd[l2] = s;
+                }
+                for (int j = 0; j < nd; ++j) {
+                    double gj[3], bj[NS];
+                    grad(j / D, gj);
+                    bcol<D, double>(gj, j % D, bj);
+                    double kt = 0.0;
+                    for (int l2 = 0; l2 < NS; ++l2) kt += btd[l2] * bj[l2];
+                    Ke[int64_t(i) * nd + j] += wgt * kt;
+                }
+                double pt = 0.0;
+                for (int s = 0; s < NS; ++s) pt += bi[s] * sig[s];
+                Pe[i] += wgt * pt;
+            }
+        }
+    }
+    const int cnn = r.coh_nn / 2, cnq = r.coh_nn == 4 ? 2 : 4;
+    for (int k = 0; k < r.n_coh; ++k) {
+        double lo[3][3], ulo[9], uup[9];
+        for (int n = 0; n < cnn; ++n)
+            for (int i = 0; i < D; ++i) {
+                lo[n][i] = lx[r.coh_nodes[k][n]][i];
+                ulo[n * D + i] = ue[r.coh_nodes[k][n] * D + i];
+                uup[n * D + i] = ue[r.coh_nodes[k][n + cnn] * D + i];
+            }
+        for (int q = 0; q < cnq; ++q) {
+            double N[3], R[D][D], w, RDR[D][D], tR[D];
+            const double dA = cohesive_qp<D>(r.coh_nn, q, lo, N, R, w);
+            const int iq = r.isv_off + k * cnq + q;
+            double hist = isv_c ? isv_c[iq] : 0.0;
+            cohesive_point<D>(p.mats[r.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
+            if (isv_t) isv_t[iq] = hist;
+            const double wgt = dA * w;
+            for (int m1 = 0; m1 < r.coh_nn; ++m1) {
+                const int n1 = m1 % cnn, l1 = r.coh_nodes[k][m1];
+                const double s1 = m1 < cnn ? -1.0 : 1.0;
+                for (int d1 = 0; d1 < D; ++d1) {
+                    Pe[l1 * D + d1] += wgt * (s1 * (N[n1] * tR[d1]));
+                    for (int m2 = 0; m2 < r.coh_nn; ++m2) {
+                        const int n2 = m2 % cnn, l2 = r.coh_nodes[k][m2];
+                        const double sg = ((m1 < cnn) == (m2 < cnn)) ? 1.0 : -1.0;
+                        for (int d2 = 0; d2 < D; ++d2)
+                            Ke[int64_t(l1 * D + d1) * nd + l2 * D + d2] += wgt * (sg * (N[n1] * N[n2] * RDR[d1][d2]));
+                    }
+                }
+            }
+        }
+    }
+}
 
+// --------------------------------------------------------------------------
+// Cut-element kernel of the class-binned schedule: one wavefront per
+// intersected element (src/Mesh.cpp:1521-1641 + src/Assembler.cpp:375-517).
+//   phase 1: one lane per child qp computes the child Jacobian in FP64 (sliver
+//            children), the gradients of all local nodes, wJ, wJ*D and the FP64
+//            stress; one lane per cohesive qp runs the FP64 law and history update;
+//   phase 2: qp by qp, lanes form (wJ D) B_j for every column j (TK), then every
+//            lane accumulates its K entries b_i . (wJ D B_j) in TK over the qps;
+//   phase 3: the cohesive fold and P (FP64) are added entry-wise.
+// --------------------------------------------------------------------------
 template <int D, typename TK>
 __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict__ Un,
                                             const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                             TK* __restrict__ Kst, double* __restrict__ Pst, int* __restrict__ err) {
     constexpr int NS = (D == 2 ? 3 : 6), NEN = D + 1;
-    __shared__ double xs[CUT_MAXN][3];
-    __shared__ double us[CUT_MAXN * 3];
-    __shared__ double qg[CUT_MAXQ][CUT_MAXN][D];
-    __shared__ double qw[CUT_MAXQ];
-    __shared__ int qm[CUT_MAXQ];
-    __shared__ double qs[CUT_MAXQ][NS];
+    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q, MAXND = MAXN * D;
+    constexpr int EPL = (MAXND * MAXND + 63) / 64;  // K entries per lane
+    __shared__ double xs[MAXN][3];
+    __shared__ double us[MAXN * D];
+    __shared__ double qg[MAXQ][MAXN][D];
+    __shared__ double qw[MAXQ];
+    __shared__ double qwD[MAXQ][NS * NS];
+    __shared__ double qs[MAXQ][NS];
+    __shared__ TK dbj[MAXND][NS];
     __shared__ double cN[CUT_MAXC][3];
     __shared__ double cw[CUT_MAXC];
     __shared__ double cRDR[CUT_MAXC][D][D];
     __shared__ double ctR[CUT_MAXC][D];
     __shared__ pg_cut_rec rec;
-    __shared__ int lnode[CUT_MAXN];
 
     const int ci = blockIdx.x;
     const int lane = threadIdx.x;
@@ -180,44 +396,28 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
     const int nloc = NEN + rec.n_enr;
     const int nd = nloc * D;
     if (lane < nloc) {
-        int nodeid = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
-        lnode[lane] = nodeid;
+        const int nodeid = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
         for (int i = 0; i < D; ++i) xs[lane][i] = p.coords[int64_t(nodeid) * D + i];
         for (int i = 0; i < D; ++i) us[lane * D + i] = Un[int64_t(nodeid) * D + i];
     }
     __syncthreads();
 
-    // ---- volumetric qps: one lane per (child, qp) ----
+    // ---- phase 1a: volumetric qps ----
     const int nvq = rec.n_vqp;
     if (lane < nvq) {
         int k = 0, q = lane;
         for (; k < rec.n_child; ++k) {
-            int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
+            const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
             if (q < nq) break;
             q -= nq;
         }
         const int ct = rec.child_type[k];
         const int cnn = ct == PG_CHILD_TRI3 ? 3 : 4;
-        double cx[4][3], cdN[4][D], cg[4][3];
+        double cx[4][3], cg[4][3], w;
         for (int n = 0; n < cnn; ++n)
             for (int i = 0; i < D; ++i) cx[n][i] = xs[rec.child_nodes[k][n]][i];
-        double w;
-        if (ct == PG_CHILD_QUAD4) {
-            double r = (q % 2 == 0) ? -PG_GAUSS2 : PG_GAUSS2;
-            double s = (q / 2 == 0) ? -PG_GAUSS2 : PG_GAUSS2;
-            double t4[4][2];
-            quad4_dNdr(r, s, t4);
-            for (int n = 0; n < 4; ++n)
-                for (int a = 0; a < D; ++a) cdN[n][a] = t4[n][a];
-            w = 1.0;
-        } else {
-            for (int n = 0; n < cnn; ++n)
-                for (int a = 0; a < D; ++a) cdN[n][a] = simplex_dNdr(n, a);
-            w = (D == 2) ? TRI_QW[q] : TET_QW[q];
-        }
-        const double Jc = transform<D>(cnn, cdN, cx, cg);
+        const double Jc = child_qp<D>(ct, q, cnn, cx, cg, w);
         if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
-        // parent gradients (linear parent: the same at every inverse-mapped point)
         double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
         for (int n = 0; n < NEN; ++n) {
             for (int i = 0; i < D; ++i) px[n][i] = xs[n][i];
@@ -231,9 +431,10 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                     if (rec.child_nodes[k][n] == l) m = n;
             for (int a = 0; a < D; ++a) qg[lane][l][a] = l < NEN ? pgr[l][a] : (m >= 0 ? cg[m][a] : 0.0);
         }
-        qw[lane] = Jc * w;
-        qm[lane] = rec.child_mat[k];
-        // strain (B_U over all local dofs), stress
+        const double wJ = Jc * w;
+        qw[lane] = wJ;
+        const pg_material& mat = p.mats[rec.child_mat[k]];
+        for (int s = 0; s < NS * NS; ++s) qwD[lane][s] = wJ * mat.D[s];
         double eps[NS], sig[NS];
         for (int s = 0; s < NS; ++s) eps[s] = 0.0;
         for (int l = 0; l < nloc; ++l)
@@ -242,106 +443,104 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                 bcol<D, double>(qg[lane][l], c, b);
                 for (int s = 0; s < NS; ++s) eps[s] += b[s] * us[l * D + c];
             }
-        lei_stress<D>(p.mats[qm[lane]].D, eps, sig);
+        lei_stress<D>(mat.D, eps, sig);
         for (int s = 0; s < NS; ++s) qs[lane][s] = sig[s];
     }
-
-    // ---- cohesive qps: one lane per (cohesive child, qp) ----
-    const int cnn = rec.coh_nn / 2;  // nodes per surface
+    // ---- phase 1b: cohesive qps ----
+    const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
     const int ncq = rec.n_coh * cnq;
     if (lane < ncq) {
         const int k = lane / cnq, q = lane % cnq;
-        double N[3], dNr[3][2];
-        double xi0 = 0, xi1 = 0, w = 1.0;
-        if (rec.coh_nn == 4) {  // CohEdge2, src/CohEdge2.cpp:70-78,186-209
-            xi0 = (q == 0) ? -PG_GAUSS2 : PG_GAUSS2;
-            N[0] = 0.5 * (1.0 - xi0); N[1] = 0.5 * (1 + xi0);
-            dNr[0][0] = -0.5; dNr[1][0] = 0.5;
-        } else {  // CohTri3: tri3 order-3 rule, src/CohTri3.cpp:43-204
-            xi0 = TRI_QX[q]; xi1 = TRI_QY[q]; w = TRI_QW[q];
-            N[0] = 1.0 - xi0 - xi1; N[1] = xi0; N[2] = xi1;
-            dNr[0][0] = -1.0; dNr[0][1] = -1.0; dNr[1][0] = 1.0; dNr[1][1] = 0.0; dNr[2][0] = 0.0; dNr[2][1] = 1.0;
-        }
-        // CohesiveElem::compute_rotation_general on the lower surface (src/CohesiveElem.cpp:157-238)
-        double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
         for (int n = 0; n < cnn; ++n)
-            for (int a = 0; a < D - 1; ++a)
-                for (int i = 0; i < D; ++i) Jc[a][i] += dNr[n][a] * xs[rec.coh_nodes[k][n]][i];
-        double nrm[3];
-        if (D == 2) { nrm[0] = -Jc[0][1]; nrm[1] = Jc[0][0]; }
-        else {
-            nrm[0] = Jc[0][1] * Jc[1][2] - Jc[0][2] * Jc[1][1];
-            nrm[1] = Jc[0][2] * Jc[1][0] - Jc[0][0] * Jc[1][2];
-            nrm[2] = Jc[0][0] * Jc[1][1] - Jc[0][1] * Jc[1][0];
-        }
-        double mag[2];
-        for (int a = 0; a < D - 1; ++a) {
-            double s = 0;
-            for (int i = 0; i < D; ++i) s += Jc[a][i] * Jc[a][i];
-            mag[a] = sqrt(s);
-        }
-        double dA = 0;
-        for (int i = 0; i < D; ++i) dA += nrm[i] * nrm[i];
-        dA = sqrt(dA);
-        double R[D][D];
-        for (int i = 0; i < D; ++i) R[D - 1][i] = nrm[i] / dA;
-        for (int i = 0; i < D; ++i)
-            for (int a = 0; a < D - 1; ++a) R[a][i] = Jc[a][i] / mag[a];
-        // opening, cohesiveDeltaFast (src/Utilities.cpp:528-540)
-        double dl[D], del[D];
-        for (int i = 0; i < D; ++i) {
-            double s = 0;
-            for (int n = 0; n < cnn; ++n)
-                s += N[n] * (us[rec.coh_nodes[k][n + cnn] * D + i] - us[rec.coh_nodes[k][n] * D + i]);
-            dl[i] = s;
-        }
-        for (int i = 0; i < D; ++i) {
-            double s = 0;
-            for (int j = 0; j < D; ++j) s += R[i][j] * dl[j];
-            del[i] = s;
-        }
+            for (int i = 0; i < D; ++i) {
+                lo[n][i] = xs[rec.coh_nodes[k][n]][i];
+                ulo[n * D + i] = us[rec.coh_nodes[k][n] * D + i];
+                uup[n * D + i] = us[rec.coh_nodes[k][n + cnn] * D + i];
+            }
+        const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
         const int iq = rec.isv_off + lane;
         double hist = isv_c ? isv_c[iq] : 0.0;
-        double t[D], Dm[D][D];
-        cohesive_law<D>(p.mats[rec.coh_mat[k]], del, hist, t, Dm);
+        cohesive_point<D>(p.mats[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
         if (isv_t) isv_t[iq] = hist;
-        // R^T D R and R^T t (cohesiveKFast / cohesiveInternalForceFast)
-        double RD[D][D];
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) {
-                double s = 0;
-                for (int l = 0; l < D; ++l) s += R[l][i] * Dm[l][j];
-                RD[i][j] = s;
-            }
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) {
-                double s = 0;
-                for (int l = 0; l < D; ++l) s += RD[i][l] * R[l][j];
-                cRDR[lane][i][j] = s;
-            }
         for (int i = 0; i < D; ++i) {
-            double s = 0;
-            for (int l = 0; l < D; ++l) s += R[l][i] * t[l];
-            ctR[lane][i] = s;
+            for (int j = 0; j < D; ++j) cRDR[lane][i][j] = RDR[i][j];
+            ctR[lane][i] = tR[i];
         }
         for (int n = 0; n < cnn; ++n) cN[lane][n] = N[n];
         cw[lane] = dA * w;
     }
     __syncthreads();
 
+    // ---- phase 2: volumetric K ----
+    TK acc[EPL];
+    for (int t = 0; t < EPL; ++t) acc[t] = TK(0);
+    for (int q = 0; q < nvq; ++q) {
+        for (int j = lane; j < nd; j += 64) {
+            TK g[D], b[NS];
+            for (int a = 0; a < D; ++a) g[a] = TK(qg[q][j / D][a]);
+            bcol<D, TK>(g, j % D, b);
+            for (int s = 0; s < NS; ++s) {
+                TK v = TK(0);
+                for (int k = 0; k < NS; ++k) v += TK(qwD[q][s * NS + k]) * b[k];
+                dbj[j][s] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < EPL; ++t) {
+            const int idx = lane + 64 * t;
+            if (idx < nd * nd) {
+                const int i = idx / nd, j = idx % nd;
+                TK g[D], b[NS];
+                for (int a = 0; a < D; ++a) g[a] = TK(qg[q][i / D][a]);
+                bcol<D, TK>(g, i % D, b);
+                TK v = TK(0);
+                for (int s = 0; s < NS; ++s) v += b[s] * dbj[j][s];
+                acc[t] += v;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- phase 3: cohesive fold, stores ----
     TK* Ke = Kst + p.cut_koff[ci];
+#pragma unroll
+    for (int t = 0; t < EPL; ++t) {
+        const int idx = lane + 64 * t;
+        if (idx < nd * nd) {
+            const int i = idx / nd, j = idx % nd;
+            const int li = i / D, c_i = i % D, lj = j / D, c_j = j % D;
+            TK v = acc[t];
+            for (int k = 0; k < rec.n_coh; ++k) {
+                int m1 = -1, m2 = -1;
+                for (int m = 0; m < rec.coh_nn; ++m) {
+                    if (rec.coh_nodes[k][m] == li) m1 = m;
+                    if (rec.coh_nodes[k][m] == lj) m2 = m;
+                }
+                if (m1 < 0 || m2 < 0) continue;
+                const int n1 = m1 % cnn, n2 = m2 % cnn;
+                const TK sgn = ((m1 < cnn) == (m2 < cnn)) ? TK(1) : TK(-1);
+                TK s = TK(0);
+                for (int q = 0; q < cnq; ++q) {
+                    const int cq = k * cnq + q;
+                    s += TK(cw[cq]) * (sgn * (TK(cN[cq][n1]) * TK(cN[cq][n2]) * TK(cRDR[cq][c_i][c_j])));
+                }
+                v += s;
+            }
+            Ke[idx] = v;
+        }
+    }
     double* Pe = Pst + p.cut_poff[ci];
-    // ---- P_el ----
     for (int i = lane; i < nd; i += 64) {
         const int l = i / D, c = i % D;
-        double acc = 0.0;
+        double a = 0.0;
         for (int q = 0; q < nvq; ++q) {
             double b[NS];
             bcol<D, double>(qg[q][l], c, b);
             double pt = 0.0;
             for (int s = 0; s < NS; ++s) pt += b[s] * qs[q][s];
-            acc += qw[q] * pt;
+            a += qw[q] * pt;
         }
         double pc = 0.0;
         for (int k = 0; k < rec.n_coh; ++k)
@@ -356,45 +555,7 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                     }
                     pc += s;
                 }
-        Pe[i] = acc + pc;
-    }
-    // ---- K_el ----
-    for (int idx = lane; idx < nd * nd; idx += 64) {
-        const int i = idx / nd, j = idx % nd;
-        const int li = i / D, c_i = i % D, lj = j / D, c_j = j % D;
-        TK acc = TK(0);
-        for (int q = 0; q < nvq; ++q) {
-            TK gi[D], gj[D], bi[NS], bj[NS];
-            for (int a = 0; a < D; ++a) { gi[a] = TK(qg[q][li][a]); gj[a] = TK(qg[q][lj][a]); }
-            bcol<D, TK>(gi, c_i, bi);
-            bcol<D, TK>(gj, c_j, bj);
-            const double* Dm = p.mats[qm[q]].D;
-            TK kt = TK(0);
-            for (int l = 0; l < NS; ++l) {
-                TK s = TK(0);
-                for (int k = 0; k < NS; ++k) s += bi[k] * TK(Dm[k * NS + l]);
-                kt += s * bj[l];
-            }
-            acc += TK(qw[q]) * kt;
-        }
-        // cohesive fold
-        for (int k = 0; k < rec.n_coh; ++k) {
-            int m1 = -1, m2 = -1;
-            for (int m = 0; m < rec.coh_nn; ++m) {
-                if (rec.coh_nodes[k][m] == li) m1 = m;
-                if (rec.coh_nodes[k][m] == lj) m2 = m;
-            }
-            if (m1 < 0 || m2 < 0) continue;
-            const int n1 = m1 % cnn, n2 = m2 % cnn;
-            const TK sgn = ((m1 < cnn) == (m2 < cnn)) ? TK(1) : TK(-1);
-            TK s = TK(0);
-            for (int q = 0; q < cnq; ++q) {
-                const int cq = k * cnq + q;
-                s += TK(cw[cq]) * (sgn * (TK(cN[cq][n1]) * TK(cN[cq][n2]) * TK(cRDR[cq][c_i][c_j])));
-            }
-            acc += s;
-        }
-        Ke[idx] = acc;
+        Pe[i] = a + pc;
     }
 }
 
@@ -411,19 +572,18 @@ __device__ inline int find_local(const DevProb& p, int e, int ci, int node, int 
 }
 
 // --------------------------------------------------------------------------
-// Node-pair kernel, plain schedule: thread per node pair (a, b); the d x d block
-// K[a-dofs, b-dofs] is the sum over the elements incident to a (ascending, the
-// reference's element loop order) that also contain b, of the staged element
-// matrices.  Written once through the scatter indices (set_matrix,
-// src/Assembler.cpp:811-831, without atomics).  Diagonal pairs also sum P
-// (set_vec, src/Assembler.cpp:833-853).
+// Node-pair kernel of the plain schedule: thread per node pair (a, b); the d x d
+// block K[a-dofs, b-dofs] sums, over the elements incident to a (ascending: the
+// reference's element loop order, src/Assembler.cpp:252) that also contain b,
+// the staged element matrices, and is written once through the scatter indices
+// (set_matrix, src/Assembler.cpp:811-831; no atomics).  Diagonal pairs also sum
+// P (set_vec, src/Assembler.cpp:833-853).
 // --------------------------------------------------------------------------
 template <int D>
-__global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const double* __restrict__ Kc,
-                              const double* __restrict__ Pc, double* __restrict__ K, double* __restrict__ P,
-                              unsigned flags) {
-    constexpr int NEN = D + 1, ND = NEN * D;
-    int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
+                              double* __restrict__ P, unsigned flags) {
+    constexpr int NEN = D + 1;
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= p.n_pairs) return;
     const int a = p.npat_row[q], b = p.npat_col[q];
     double acc[D][D];
@@ -439,22 +599,12 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
         const int la = find_local(p, e, ci, a, NEN);
         const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
         if (la < 0 || lb < 0) continue;
-        const double* Ke;
-        const double* Pe;
-        int nd;
-        if (ci < 0) {
-            Ke = stage + int64_t(e) * (ND * ND + ND);
-            Pe = Ke + ND * ND;
-            nd = ND;
-        } else {
-            Ke = Kc + p.cut_koff[ci];
-            Pe = Pc + p.cut_poff[ci];
-            nd = (NEN + p.cut[ci].n_enr) * D;
-        }
+        const int nd = (ci < 0 ? NEN : NEN + p.cut[ci].n_enr) * D;
+        const double* Ke = stage + p.elem_off[e];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) acc[i][j] += Ke[int64_t(la * D + i) * nd + lb * D + j];
         if (diag)
-            for (int i = 0; i < D; ++i) pa[i] += Pe[la * D + i];
+            for (int i = 0; i < D; ++i) pa[i] += Ke[int64_t(nd) * nd + la * D + i];
     }
     if (flags & PG_JAC) {
         const int32_t* mp = p.npat_map + q * D * D;
@@ -465,18 +615,63 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
         for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
 }
 
+// Gradients of a linear simplex in TC from FP64 coordinate differences
+// (closed-form adjugate of the affine Jacobian; the same map as
+// Elem::transform_gradients for a linear parent).  Returns J.
+template <int D, typename TC>
+__device__ inline TC simplex_grads(const double (*x)[3], TC (*g)[D]) {
+    TC d[D][D];  // d[k][i] = x_{k+1}[i] - x_0[i]  (row k: derivative along r_k)
+    for (int k = 0; k < D; ++k)
+        for (int i = 0; i < D; ++i) d[k][i] = TC(x[k + 1][i] - x[0][i]);
+    if constexpr (D == 2) {
+        const TC J = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+        const TC rJ = TC(1) / J;
+        g[1][0] = d[1][1] * rJ; g[1][1] = -d[1][0] * rJ;
+        g[2][0] = -d[0][1] * rJ; g[2][1] = d[0][0] * rJ;
+        g[0][0] = -(g[1][0] + g[2][0]); g[0][1] = -(g[1][1] + g[2][1]);
+        return J;
+    } else {
+        TC c[3][3];  // cofactors of d
+        c[0][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
+        c[0][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
+        c[0][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
+        c[1][0] = d[0][2] * d[2][1] - d[0][1] * d[2][2];
+        c[1][1] = d[0][0] * d[2][2] - d[0][2] * d[2][0];
+        c[1][2] = d[0][1] * d[2][0] - d[0][0] * d[2][1];
+        c[2][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+        c[2][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
+        c[2][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+        const TC J = d[0][0] * c[0][0] + d[0][1] * c[0][1] + d[0][2] * c[0][2];
+        const TC rJ = TC(1) / J;
+        // grad N_{k+1} = Jac^-1 e_k: component i = c[k][i] / J
+        for (int k = 0; k < 3; ++k)
+            for (int i = 0; i < 3; ++i) g[k + 1][i] = c[k][i] * rJ;
+        for (int i = 0; i < 3; ++i) g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
+        return J;
+    }
+}
+
 // --------------------------------------------------------------------------
-// Node-pair kernel, mixed schedule: uncut elements are recomputed here (FP32
-// B^T D B block of the pair, weight sum over the reference qps in FP64, sum in
-// FP64); cut elements contribute their staged FP32 tangent.  The internal force
-// of diagonal pairs is recomputed in FP64.
+// Node-pair kernel of the class-binned schedule: uncut elements are recomputed
+// here -- B_a^T (wD) B_b in TC from FP64 coordinate differences, summed in FP64
+// -- and never staged; cut elements contribute their staged TK tangent.  The
+// internal force of diagonal pairs is recomputed in FP64 with the reference's
+// gradient transform.  LEI matrices are staged in LDS once per block.
 // --------------------------------------------------------------------------
-template <int D>
-__global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const float* __restrict__ Kc,
+constexpr int MAX_LDS_MATS = 8;
+
+template <int D, typename TC>
+__global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const TC* __restrict__ Kc,
                              const double* __restrict__ Pc, double* __restrict__ K, double* __restrict__ P,
                              unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
+        const int m = k / (NS * NS);
+        sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
+    }
+    __syncthreads();
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= p.n_pairs) return;
     const int a = p.npat_row[q], b = p.npat_col[q];
     double acc[D][D];
@@ -486,6 +681,8 @@ __global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const flo
     for (int i = 0; i < D; ++i) pa[i] = 0.0;
     const bool diag = (a == b) && (flags & PG_FUNC);
     const bool jac = (flags & PG_JAC) != 0;
+    TC wsum = TC(0);
+    for (int k = 0; k < NQ; ++k) wsum += TC(D == 2 ? TRI_QW[k] : TET_QW[k]);
     const int e0 = p.n2e_ptr[a], e1 = p.n2e_ptr[a + 1];
     for (int t = e0; t < e1; ++t) {
         const int e = p.n2e[t];
@@ -499,39 +696,40 @@ __global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const flo
                 if (nodes[n] == b) lb = n;
             }
             if (la < 0 || lb < 0) continue;
-            // FP64 coordinates -> Jacobian, FP64 gradients (cheap), FP32 block
-            double x[NEN][3], dNdr[NEN][D], g[NEN][3];
-            for (int n = 0; n < NEN; ++n) {
+            double x[NEN][3];
+            for (int n = 0; n < NEN; ++n)
                 for (int i = 0; i < D; ++i) x[n][i] = p.coords[int64_t(nodes[n]) * D + i];
-                for (int c = 0; c < D; ++c) dNdr[n][c] = simplex_dNdr(n, c);
-            }
-            const double J = transform<D>(NEN, dNdr, x, g);
-            double wsum = 0.0;
-            for (int k = 0; k < NQ; ++k) wsum += J * (D == 2 ? TRI_QW[k] : TET_QW[k]);
-            const double* Dm = p.mats[p.elem_mat[e]].D;
+            const int mi = p.elem_mat[e];
             if (jac) {
-                float ga[D], gb[D];
-                for (int c = 0; c < D; ++c) { ga[c] = float(g[la][c]); gb[c] = float(g[lb][c]); }
-                const float wf = float(wsum);
+                TC g[NEN][D];
+                const TC J = simplex_grads<D, TC>(x, g);
+                const TC w = J * wsum;
+                const TC* Dm = mi < MAX_LDS_MATS ? sD[mi] : nullptr;
                 for (int i = 0; i < D; ++i) {
-                    float bi[NS];
-                    bcol<D, float>(ga, i, bi);
-                    float btd[NS];
+                    TC bi[NS], btd[NS];
+                    bcol<D, TC>(g[la], i, bi);
                     for (int l = 0; l < NS; ++l) {
-                        float s = 0.f;
-                        for (int k = 0; k < NS; ++k) s += bi[k] * float(Dm[k * NS + l]);
+                        TC s = TC(0);
+                        for (int k = 0; k < NS; ++k) s += bi[k] * (Dm ? Dm[k * NS + l] : TC(p.mats[mi].D[k * NS + l]));
                         btd[l] = s;
                     }
                     for (int j = 0; j < D; ++j) {
-                        float bj[NS];
-                        bcol<D, float>(gb, j, bj);
-                        float kt = 0.f;
+                        TC bj[NS];
+                        bcol<D, TC>(g[lb], j, bj);
+                        TC kt = TC(0);
                         for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
-                        acc[i][j] += double(wf * kt);
+                        acc[i][j] += double(w * kt);
                     }
                 }
             }
             if (diag) {
+                double pdN[NEN][D], g[NEN][3];
+                for (int n = 0; n < NEN; ++n)
+                    for (int c = 0; c < D; ++c) pdN[n][c] = simplex_dNdr(n, c);
+                const double J = transform<D>(NEN, pdN, x, g);
+                double wq = 0.0;
+                for (int k = 0; k < NQ; ++k) wq += J * (D == 2 ? TRI_QW[k] : TET_QW[k]);
+                const double* Dm = p.mats[mi].D;
                 double eps[NS], sig[NS];
                 for (int k = 0; k < NS; ++k) eps[k] = 0.0;
                 for (int n = 0; n < NEN; ++n)
@@ -547,7 +745,7 @@ __global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const flo
                     bcol<D, double>(g[la], i, bi);
                     double pt = 0.0;
                     for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
-                    pa[i] += wsum * pt;
+                    pa[i] += wq * pt;
                 }
             }
         } else {
@@ -556,7 +754,7 @@ __global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const flo
             if (la < 0 || lb < 0) continue;
             const int nd = (NEN + p.cut[ci].n_enr) * D;
             if (jac) {
-                const float* Ke = Kc + p.cut_koff[ci];
+                const TC* Ke = Kc + p.cut_koff[ci];
                 for (int i = 0; i < D; ++i)
                     for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
             }
@@ -593,7 +791,7 @@ struct pg_asm {
     int64_t cut_k_total = 0, cut_p_total = 0;
     std::vector<void*> allocs;
     double* Un = nullptr;        // node-major U
-    double* stage = nullptr;     // plain schedule: per uncut element Ke + Pe (FP64)
+    double* stage = nullptr;     // plain schedule: per element Ke + Pe (FP64)
     double* Kc64 = nullptr;      // cut tangents (FP64)
     float* Kc32 = nullptr;       // cut tangents (FP32)
     double* Pc = nullptr;        // cut residuals (FP64)
@@ -627,7 +825,9 @@ int check_desc(const pg_problem_desc* s) {
     if (s->n_cut > 0 && (!s->cut || !s->cut_elem)) { g_err = "missing cut records"; return PG_E_ARG; }
     for (int c = 0; c < s->n_cut; ++c) {
         const pg_cut_rec& r = s->cut[c];
-        if (r.n_enr > PG_MAX_ENR || r.n_child > PG_MAX_CHILD || r.n_coh > PG_MAX_COH || r.n_vqp > CUT_MAXQ) {
+        const int lim_n = s->dim == 2 ? MAXN2 : MAXN3, lim_q = s->dim == 2 ? MAXQ2 : MAXQ3;
+        if (r.n_enr > PG_MAX_ENR || s->nen + r.n_enr > lim_n || r.n_child > PG_MAX_CHILD || r.n_coh > PG_MAX_COH ||
+            r.n_vqp > lim_q) {
             g_err = "cut record exceeds the supported class sizes";
             return PG_E_UNSUPPORTED;
         }
@@ -699,36 +899,44 @@ int alloc(pg_asm* h, T** p, size_t n) {
     return PG_OK;
 }
 
+// Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
+// elements + staged node-pair reduction); PG_BATCHED -> class-binned FP64;
+// PG_FP32 (with or without PG_BATCHED) -> class-binned mixed precision.
 int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned flags, double* dK, double* dP,
            hipStream_t st) {
     const DevProb& d = h->d;
     const int64_t nall = int64_t(d.n_nodes) + d.n_enr;
     const int64_t nu = nall * d.dim;
-    const bool mixed = (flags & (PG_FP32 | PG_BATCHED)) != 0;
+    const bool fp32 = (flags & PG_FP32) != 0;
+    const bool binned = fp32 || (flags & PG_BATCHED) != 0;
     if (h->timing) HIP_OK(hipEventRecord(h->ev[0], st));
     k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, d.node_dof, dU, h->Un);
-    if (d.n_cut > 0) {
-        if (d.dim == 2) {
-            if (mixed) k_cut<2, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-            else k_cut<2, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
-        } else {
-            if (mixed) k_cut<3, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-            else k_cut<3, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
-        }
-    }
-    if (!mixed) {
-        const unsigned gb = unsigned((d.n_elem + h->elem_block - 1) / h->elem_block);
-        if (d.dim == 2) k_plain_fp64<2><<<gb, h->elem_block, 0, st>>>(d, h->Un, h->stage);
-        else k_plain_fp64<3><<<gb, h->elem_block, 0, st>>>(d, h->Un, h->stage);
-    }
-    if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
     const unsigned gp = unsigned((d.n_pairs + h->pair_block - 1) / h->pair_block);
-    if (mixed) {
-        if (d.dim == 2) k_pair_fused<2><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
-        else k_pair_fused<3><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
+    if (!binned) {
+        const unsigned gb = unsigned((d.n_elem + h->elem_block - 1) / h->elem_block);
+        if (d.dim == 2) k_elem_fp64<2><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
+        else k_elem_fp64<3><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
+        if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
+        if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
+        else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
     } else {
-        if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->Kc64, h->Pc, dK, dP, flags);
-        else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->Kc64, h->Pc, dK, dP, flags);
+        if (d.n_cut > 0) {
+            if (d.dim == 2) {
+                if (fp32) k_cut<2, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<2, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+            } else {
+                if (fp32) k_cut<3, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<3, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+            }
+        }
+        if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
+        if (d.dim == 2) {
+            if (fp32) k_pair_fused<2, float><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
+            else k_pair_fused<2, double><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+        } else {
+            if (fp32) k_pair_fused<3, float><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
+            else k_pair_fused<3, double><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+        }
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -764,6 +972,14 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     }
     h->cut_k_total = koff[s->n_cut];
     h->cut_p_total = poff[s->n_cut];
+    std::vector<int64_t> eoff(size_t(s->n_elem) + 1);
+    eoff[0] = 0;
+    for (int e = 0; e < s->n_elem; ++e) {
+        const int c = s->cut_of_elem[e];
+        const int64_t nd = int64_t(s->nen + (c >= 0 ? s->cut[c].n_enr : 0)) * s->dim;
+        eoff[e + 1] = eoff[e] + nd * nd + nd;
+    }
+    d.n_mat = s->n_mat;
     auto fail = [&](int code) { pg_asm_destroy(h); return code; };
 #define UP(field, src, n, T)                                   \
     do {                                                       \
@@ -788,10 +1004,12 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(n2e, s->n2e, s->n2e_ptr[nall], int32_t);
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
+    UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
 #undef UP
     const int ndp = s->nen * s->dim;
     if ((rc = alloc(h, &h->Un, size_t(nall * s->dim)))) return fail(rc);
-    if ((rc = alloc(h, &h->stage, size_t(s->n_elem) * size_t(ndp * ndp + ndp)))) return fail(rc);
+    (void)ndp;
+    if ((rc = alloc(h, &h->stage, size_t(eoff[s->n_elem])))) return fail(rc);
     if ((rc = alloc(h, &h->Kc64, size_t(h->cut_k_total)))) return fail(rc);
     if ((rc = alloc(h, &h->Kc32, size_t(h->cut_k_total)))) return fail(rc);
     if ((rc = alloc(h, &h->Pc, size_t(h->cut_p_total)))) return fail(rc);

@@ -40,19 +40,19 @@ def case(request):
     return p, U, ic, Kr, Pr, Ir
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed"])
 def test_parity_vs_oracle(case, mode):
     p, U, ic, Kr, Pr, Ir = case
     asm = AssemblerGPU(p, mode)
     K, P, It = _run(asm, p, U, ic)
-    tol = 1e-12 if mode == "fp64" else 1e-6
+    tol = 1e-6 if mode == "mixed" else 1e-12
     assert rel(K, Kr) < tol, rel(K, Kr)
     assert rel(P, Pr) < 1e-12, rel(P, Pr)
     if p.n_cqp:
         assert np.abs(It - Ir).max() <= 1e-12 * max(np.abs(Ir).max(), 1e-30)
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed"])
 def test_bitwise_repeatable(case, mode):
     p, U, ic, *_ = case
     asm = AssemblerGPU(p, mode)
@@ -83,7 +83,7 @@ def test_c1_full_config_fp64_and_mixed():
     o = Oracle(p)
     U = p.random_u(0)
     Kr, Pr, _ = o.assemble(U)
-    for mode, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+    for mode, tol in (("fp64", 1e-12), ("binned64", 1e-12), ("mixed", 1e-6)):
         K, P, _ = _run(AssemblerGPU(p, mode), p, U, np.zeros(0))
         print(mode, "relK", rel(K, Kr), "relP", rel(P, Pr), flush=True)
         assert rel(K, Kr) < tol and rel(P, Pr) < 1e-12
