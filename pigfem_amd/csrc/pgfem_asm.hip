@@ -48,6 +48,7 @@ struct DevProb {
     const pg_cut_rec* cut;
     const pg_material* mats;
     const int32_t* node_dof;
+    const int32_t* npat_ptr;
     const int32_t* npat_row;
     const int32_t* npat_col;
     const int32_t* npat_map;
@@ -74,6 +75,7 @@ constexpr int MAXN3 = 4 + PG_MAX_ENR;    // 3D local nodes
 constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
 constexpr int CUT_MAXC = PG_MAX_COH * 4; // cohesive qps per cut element
+constexpr int MAX_LDS_MATS = 8;          // materials staged in LDS
 
 template <int D> struct CutLim;
 template <> struct CutLim<2> { static constexpr int N = MAXN2, Q = MAXQ2; };
@@ -357,190 +359,211 @@ __global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const doub
 }
 
 // --------------------------------------------------------------------------
-// Cut-element kernel of the class-binned schedule: one wavefront per
-// intersected element (src/Mesh.cpp:1521-1641 + src/Assembler.cpp:375-517).
+// Cut-element kernel of the class-binned schedule (src/Mesh.cpp:1521-1641 +
+// src/Assembler.cpp:375-517).  EPW intersected elements share one wavefront, a
+// group of 64/EPW lanes each:
 //   phase 1: one lane per child qp computes the child Jacobian in FP64 (sliver
-//            children), the gradients of all local nodes, wJ, wJ*D and the FP64
-//            stress; one lane per cohesive qp runs the FP64 law and history update;
-//   phase 2: qp by qp, lanes form (wJ D) B_j for every column j (TK), then every
-//            lane accumulates its K entries b_i . (wJ D B_j) in TK over the qps;
-//   phase 3: the cohesive fold and P (FP64) are added entry-wise.
+//            children), the gradients of all local nodes, wJ and the FP64 stress;
+//            one lane per cohesive qp runs the FP64 law and the history update;
+//   phase 2: every lane accumulates its K entries b_i . (wJ D) b_j over the qps in
+//            TK, adds the cohesive fold and stores; P (FP64) likewise.
 // --------------------------------------------------------------------------
-template <int D, typename TK>
+template <int D, typename TK, int EPW>
 __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict__ Un,
                                             const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                             TK* __restrict__ Kst, double* __restrict__ Pst, int* __restrict__ err) {
-    constexpr int NS = (D == 2 ? 3 : 6), NEN = D + 1;
-    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q, MAXND = MAXN * D;
-    constexpr int EPL = (MAXND * MAXND + 63) / 64;  // K entries per lane
-    __shared__ double xs[MAXN][3];
-    __shared__ double us[MAXN * D];
-    __shared__ double qg[MAXQ][MAXN][D];
-    __shared__ double qw[MAXQ];
-    __shared__ double qwD[MAXQ][NS * NS];
-    __shared__ double qs[MAXQ][NS];
-    __shared__ TK dbj[MAXND][NS];
-    __shared__ double cN[CUT_MAXC][3];
-    __shared__ double cw[CUT_MAXC];
-    __shared__ double cRDR[CUT_MAXC][D][D];
-    __shared__ double ctR[CUT_MAXC][D];
-    __shared__ pg_cut_rec rec;
+    constexpr int NS = (D == 2 ? 3 : 6), NEN = D + 1, GS = 64 / EPW;
+    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q;
+    __shared__ double xs[EPW][MAXN][3];
+    __shared__ double us[EPW][MAXN * D];
+    __shared__ double pgs[EPW][NEN][D];     // parent gradients (constant)
+    __shared__ double qgc[EPW][MAXQ][4][D]; // child gradients per qp
+    __shared__ double qw[EPW][MAXQ];
+    __shared__ int qm[EPW][MAXQ];
+    __shared__ int qk[EPW][MAXQ];           // child of each qp
+    __shared__ double qs[EPW][MAXQ][NS];
+    __shared__ double cN[EPW][CUT_MAXC][3];
+    __shared__ double cw[EPW][CUT_MAXC];
+    __shared__ double cRDR[EPW][CUT_MAXC][D][D];
+    __shared__ double ctR[EPW][CUT_MAXC][D];
+    __shared__ pg_cut_rec recs[EPW];
+    __shared__ TK sD[MAX_LDS_MATS][NS * NS];
 
-    const int ci = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (ci >= p.n_cut) return;
-    const int e = p.cut_elem[ci];
-    if (lane < int(sizeof(pg_cut_rec) / 4))
-        reinterpret_cast<int32_t*>(&rec)[lane] = reinterpret_cast<const int32_t*>(&p.cut[ci])[lane];
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    const int ci = blockIdx.x * EPW + g;
+    const bool live = ci < p.n_cut;
+    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += 64) {
+        const int m = k / (NS * NS);
+        sD[m][k % (NS * NS)] = m < p.n_mat ? TK(p.mats[m].D[k % (NS * NS)]) : TK(0);
+    }
+    constexpr int RW = int(sizeof(pg_cut_rec) / 4);
+    for (int k = gl; k < RW; k += GS)
+        if (live) reinterpret_cast<int32_t*>(&recs[g])[k] = reinterpret_cast<const int32_t*>(&p.cut[ci])[k];
     __syncthreads();
-    const int nloc = NEN + rec.n_enr;
+    const pg_cut_rec& rec = recs[g];
+    const int e = live ? p.cut_elem[ci] : 0;
+    const int nloc = live ? NEN + rec.n_enr : 0;
     const int nd = nloc * D;
-    if (lane < nloc) {
-        const int nodeid = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
-        for (int i = 0; i < D; ++i) xs[lane][i] = p.coords[int64_t(nodeid) * D + i];
-        for (int i = 0; i < D; ++i) us[lane * D + i] = Un[int64_t(nodeid) * D + i];
+    for (int l = gl; l < nloc; l += GS) {
+        const int nodeid = l < NEN ? p.conn[int64_t(e) * NEN + l] : rec.enr[l - NEN];
+        for (int i = 0; i < D; ++i) {
+            xs[g][l][i] = p.coords[int64_t(nodeid) * D + i];
+            us[g][l * D + i] = Un[int64_t(nodeid) * D + i];
+        }
     }
     __syncthreads();
 
-    // ---- phase 1a: volumetric qps ----
-    const int nvq = rec.n_vqp;
-    if (lane < nvq) {
-        int k = 0, q = lane;
-        for (; k < rec.n_child; ++k) {
-            const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
-            if (q < nq) break;
-            q -= nq;
-        }
-        const int ct = rec.child_type[k];
-        const int cnn = ct == PG_CHILD_TRI3 ? 3 : 4;
-        double cx[4][3], cg[4][3], w;
-        for (int n = 0; n < cnn; ++n)
-            for (int i = 0; i < D; ++i) cx[n][i] = xs[rec.child_nodes[k][n]][i];
-        const double Jc = child_qp<D>(ct, q, cnn, cx, cg, w);
-        if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
-        double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
-        for (int n = 0; n < NEN; ++n) {
-            for (int i = 0; i < D; ++i) px[n][i] = xs[n][i];
-            for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
-        }
-        transform<D>(NEN, pdN, px, pgr);
-        for (int l = 0; l < nloc; ++l) {
-            int m = -1;
-            if (l >= NEN)
-                for (int n = 0; n < cnn; ++n)
-                    if (rec.child_nodes[k][n] == l) m = n;
-            for (int a = 0; a < D; ++a) qg[lane][l][a] = l < NEN ? pgr[l][a] : (m >= 0 ? cg[m][a] : 0.0);
-        }
-        const double wJ = Jc * w;
-        qw[lane] = wJ;
-        const pg_material& mat = p.mats[rec.child_mat[k]];
-        for (int s = 0; s < NS * NS; ++s) qwD[lane][s] = wJ * mat.D[s];
-        double eps[NS], sig[NS];
-        for (int s = 0; s < NS; ++s) eps[s] = 0.0;
-        for (int l = 0; l < nloc; ++l)
-            for (int c = 0; c < D; ++c) {
-                double b[NS];
-                bcol<D, double>(qg[lane][l], c, b);
-                for (int s = 0; s < NS; ++s) eps[s] += b[s] * us[l * D + c];
-            }
-        lei_stress<D>(mat.D, eps, sig);
-        for (int s = 0; s < NS; ++s) qs[lane][s] = sig[s];
-    }
-    // ---- phase 1b: cohesive qps ----
+    const int nvq = live ? rec.n_vqp : 0;
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
-    const int ncq = rec.n_coh * cnq;
-    if (lane < ncq) {
-        const int k = lane / cnq, q = lane % cnq;
-        double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
-        for (int n = 0; n < cnn; ++n)
-            for (int i = 0; i < D; ++i) {
-                lo[n][i] = xs[rec.coh_nodes[k][n]][i];
-                ulo[n * D + i] = us[rec.coh_nodes[k][n] * D + i];
-                uup[n * D + i] = us[rec.coh_nodes[k][n + cnn] * D + i];
+    const int ncq = live ? rec.n_coh * cnq : 0;
+    for (int t = gl; t < nvq + ncq; t += GS) {
+        if (t < nvq) {
+            // ---- volumetric qp ----
+            int k = 0, q = t;
+            for (; k < rec.n_child; ++k) {
+                const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
+                if (q < nq) break;
+                q -= nq;
             }
-        const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
-        const int iq = rec.isv_off + lane;
-        double hist = isv_c ? isv_c[iq] : 0.0;
-        cohesive_point<D>(p.mats[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
-        if (isv_t) isv_t[iq] = hist;
-        for (int i = 0; i < D; ++i) {
-            for (int j = 0; j < D; ++j) cRDR[lane][i][j] = RDR[i][j];
-            ctR[lane][i] = tR[i];
+            const int ct = rec.child_type[k];
+            const int ccn = ct == PG_CHILD_TRI3 ? 3 : 4;
+            double cx[4][3], cg[4][3], w;
+            int cl[4];
+            for (int n = 0; n < 4; ++n) {
+                cl[n] = n < ccn ? rec.child_nodes[k][n] : -1;
+                for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? xs[g][cl[n]][i] : 0.0;
+            }
+            const double Jc = child_qp<D>(ct, q, ccn, cx, cg, w);
+            if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
+            double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
+            for (int n = 0; n < NEN; ++n) {
+                for (int i = 0; i < D; ++i) px[n][i] = xs[g][n][i];
+                for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+            }
+            transform<D>(NEN, pdN, px, pgr);
+            if (t == 0)
+                for (int n = 0; n < NEN; ++n)
+                    for (int a = 0; a < D; ++a) pgs[g][n][a] = pgr[n][a];
+            for (int n = 0; n < 4; ++n)
+                for (int a = 0; a < D; ++a) qgc[g][t][n][a] = cg[n][a];
+            double eps[NS];
+            for (int s = 0; s < NS; ++s) eps[s] = 0.0;
+            for (int l = 0; l < nloc; ++l) {
+                double gr[D];
+                if (l < NEN) {
+                    for (int a = 0; a < D; ++a) gr[a] = pgr[l][a];
+                } else {
+                    for (int a = 0; a < D; ++a) gr[a] = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 4; ++n)
+                        if (cl[n] == l)
+                            for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
+                }
+                for (int c = 0; c < D; ++c) {
+                    double b[NS];
+                    bcol<D, double>(gr, c, b);
+                    for (int s = 0; s < NS; ++s) eps[s] += b[s] * us[g][l * D + c];
+                }
+            }
+            qw[g][t] = Jc * w;
+            qm[g][t] = rec.child_mat[k];
+            qk[g][t] = k;
+            double sig[NS];
+            lei_stress<D>(p.mats[rec.child_mat[k]].D, eps, sig);
+            for (int s = 0; s < NS; ++s) qs[g][t][s] = sig[s];
+        } else {
+            // ---- cohesive qp ----
+            const int cq = t - nvq, k = cq / cnq, q = cq % cnq;
+            double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
+            for (int n = 0; n < 3; ++n)
+                for (int i = 0; i < D; ++i) {
+                    const bool in = n < cnn;
+                    lo[n][i] = in ? xs[g][rec.coh_nodes[k][n]][i] : 0.0;
+                    ulo[n * D + i] = in ? us[g][rec.coh_nodes[k][n] * D + i] : 0.0;
+                    uup[n * D + i] = in ? us[g][rec.coh_nodes[k][n + cnn] * D + i] : 0.0;
+                }
+            const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
+            const int iq = rec.isv_off + cq;
+            double hist = isv_c ? isv_c[iq] : 0.0;
+            cohesive_point<D>(p.mats[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
+            if (isv_t) isv_t[iq] = hist;
+            for (int i = 0; i < D; ++i) {
+                for (int j = 0; j < D; ++j) cRDR[g][cq][i][j] = RDR[i][j];
+                ctR[g][cq][i] = tR[i];
+            }
+            for (int n = 0; n < 3; ++n) cN[g][cq][n] = N[n];
+            cw[g][cq] = dA * w;
         }
-        for (int n = 0; n < cnn; ++n) cN[lane][n] = N[n];
-        cw[lane] = dA * w;
     }
     __syncthreads();
+    if (!live) return;
 
-    // ---- phase 2: volumetric K ----
-    TK acc[EPL];
-    for (int t = 0; t < EPL; ++t) acc[t] = TK(0);
-    for (int q = 0; q < nvq; ++q) {
-        for (int j = lane; j < nd; j += 64) {
-            TK g[D], b[NS];
-            for (int a = 0; a < D; ++a) g[a] = TK(qg[q][j / D][a]);
-            bcol<D, TK>(g, j % D, b);
-            for (int s = 0; s < NS; ++s) {
-                TK v = TK(0);
-                for (int k = 0; k < NS; ++k) v += TK(qwD[q][s * NS + k]) * b[k];
-                dbj[j][s] = v;
-            }
+    // gradient of local node l at qp q: parent gradient, the child gradient of the
+    // child node that is enrichment node l, or 0 (src/Mesh.cpp:1596-1614)
+    auto grad = [&](int q, int l, double* gr) {
+        if (l < NEN) {
+            for (int a = 0; a < D; ++a) gr[a] = pgs[g][l][a];
+            return;
         }
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < EPL; ++t) {
-            const int idx = lane + 64 * t;
-            if (idx < nd * nd) {
-                const int i = idx / nd, j = idx % nd;
-                TK g[D], b[NS];
-                for (int a = 0; a < D; ++a) g[a] = TK(qg[q][i / D][a]);
-                bcol<D, TK>(g, i % D, b);
-                TK v = TK(0);
-                for (int s = 0; s < NS; ++s) v += b[s] * dbj[j][s];
-                acc[t] += v;
-            }
-        }
-        __syncthreads();
-    }
-    // ---- phase 3: cohesive fold, stores ----
+        const int k = qk[g][q];
+        const int ccn = rec.child_type[k] == PG_CHILD_TRI3 ? 3 : 4;
+        for (int a = 0; a < D; ++a) gr[a] = 0.0;
+        for (int n = 0; n < ccn; ++n)
+            if (rec.child_nodes[k][n] == l)
+                for (int a = 0; a < D; ++a) gr[a] = qgc[g][q][n][a];
+    };
     TK* Ke = Kst + p.cut_koff[ci];
-#pragma unroll
-    for (int t = 0; t < EPL; ++t) {
-        const int idx = lane + 64 * t;
-        if (idx < nd * nd) {
-            const int i = idx / nd, j = idx % nd;
-            const int li = i / D, c_i = i % D, lj = j / D, c_j = j % D;
-            TK v = acc[t];
-            for (int k = 0; k < rec.n_coh; ++k) {
-                int m1 = -1, m2 = -1;
-                for (int m = 0; m < rec.coh_nn; ++m) {
-                    if (rec.coh_nodes[k][m] == li) m1 = m;
-                    if (rec.coh_nodes[k][m] == lj) m2 = m;
-                }
-                if (m1 < 0 || m2 < 0) continue;
-                const int n1 = m1 % cnn, n2 = m2 % cnn;
-                const TK sgn = ((m1 < cnn) == (m2 < cnn)) ? TK(1) : TK(-1);
-                TK s = TK(0);
-                for (int q = 0; q < cnq; ++q) {
-                    const int cq = k * cnq + q;
-                    s += TK(cw[cq]) * (sgn * (TK(cN[cq][n1]) * TK(cN[cq][n2]) * TK(cRDR[cq][c_i][c_j])));
-                }
-                v += s;
+    for (int idx = gl; idx < nd * nd; idx += GS) {
+        const int i = idx / nd, j = idx % nd;
+        const int li = i / D, c_i = i % D, lj = j / D, c_j = j % D;
+        TK v = TK(0);
+        for (int q = 0; q < nvq; ++q) {
+            double gid[D], gjd[D];
+            grad(q, li, gid);
+            grad(q, lj, gjd);
+            TK gi[D], gj[D], bi[NS], bj[NS];
+            for (int a = 0; a < D; ++a) { gi[a] = TK(gid[a]); gj[a] = TK(gjd[a]); }
+            bcol<D, TK>(gi, c_i, bi);
+            bcol<D, TK>(gj, c_j, bj);
+            const TK* Dm = sD[qm[g][q]];
+            TK kt = TK(0);
+            for (int s = 0; s < NS; ++s) {
+                TK db = TK(0);
+                for (int k = 0; k < NS; ++k) db += Dm[s * NS + k] * bj[k];
+                kt += bi[s] * db;
             }
-            Ke[idx] = v;
+            v += TK(qw[g][q]) * kt;
         }
+        for (int k = 0; k < rec.n_coh; ++k) {
+            int m1 = -1, m2 = -1;
+            for (int m = 0; m < rec.coh_nn; ++m) {
+                if (rec.coh_nodes[k][m] == li) m1 = m;
+                if (rec.coh_nodes[k][m] == lj) m2 = m;
+            }
+            if (m1 < 0 || m2 < 0) continue;
+            const int n1 = m1 % cnn, n2 = m2 % cnn;
+            const TK sgn = ((m1 < cnn) == (m2 < cnn)) ? TK(1) : TK(-1);
+            TK s = TK(0);
+            for (int q = 0; q < cnq; ++q) {
+                const int cq = k * cnq + q;
+                s += TK(cw[g][cq]) * (sgn * (TK(cN[g][cq][n1]) * TK(cN[g][cq][n2]) * TK(cRDR[g][cq][c_i][c_j])));
+            }
+            v += s;
+        }
+        Ke[idx] = v;
     }
     double* Pe = Pst + p.cut_poff[ci];
-    for (int i = lane; i < nd; i += 64) {
+    for (int i = gl; i < nd; i += GS) {
         const int l = i / D, c = i % D;
         double a = 0.0;
         for (int q = 0; q < nvq; ++q) {
-            double b[NS];
-            bcol<D, double>(qg[q][l], c, b);
+            double b[NS], gr[D];
+            grad(q, l, gr);
+            bcol<D, double>(gr, c, b);
             double pt = 0.0;
-            for (int s = 0; s < NS; ++s) pt += b[s] * qs[q][s];
-            a += qw[q] * pt;
+            for (int s = 0; s < NS; ++s) pt += b[s] * qs[g][q][s];
+            a += qw[g][q] * pt;
         }
         double pc = 0.0;
         for (int k = 0; k < rec.n_coh; ++k)
@@ -551,7 +574,7 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                     double s = 0.0;
                     for (int q = 0; q < cnq; ++q) {
                         const int cq = k * cnq + q;
-                        s += cw[cq] * (sgn * (cN[cq][n] * ctR[cq][c]));
+                        s += cw[g][cq] * (sgn * (cN[g][cq][n] * ctR[g][cq][c]));
                     }
                     pc += s;
                 }
@@ -652,125 +675,242 @@ __device__ inline TC simplex_grads(const double (*x)[3], TC (*g)[D]) {
 }
 
 // --------------------------------------------------------------------------
-// Node-pair kernel of the class-binned schedule: uncut elements are recomputed
-// here -- B_a^T (wD) B_b in TC from FP64 coordinate differences, summed in FP64
-// -- and never staged; cut elements contribute their staged TK tangent.  The
-// internal force of diagonal pairs is recomputed in FP64 with the reference's
-// gradient transform.  LEI matrices are staged in LDS once per block.
+// Node-group kernel of the class-binned schedule.  A block owns NG consecutive
+// nodes.  Phase A stages, once per (node, incident element) incidence, the
+// element's connectivity, its TC gradients and weight (uncut elements,
+// recomputed from FP64 coordinate differences) and the FP64 internal-force
+// contribution of the owner node (uncut: B^T sigma with the reference's FP64
+// gradient transform; cut: the staged FP64 residual).  Phase B gives each node
+// pair (a, b) of the group one lane, which sums over a's incidences -- in
+// ascending element order, as the reference's element loop -- the TC blocks
+// w B_a^T D B_b of uncut elements containing b and the staged TK blocks of cut
+// elements, in FP64, and writes the d x d block once through the scatter
+// indices.  A group whose incidences overflow the LDS tables takes the same
+// arithmetic from global memory.
 // --------------------------------------------------------------------------
-constexpr int MAX_LDS_MATS = 8;
+template <int D> struct GroupCfg;
+template <> struct GroupCfg<2> { static constexpr int NG = 32, MAXI = 384; };
+template <> struct GroupCfg<3> { static constexpr int NG = 8, MAXI = 256; };
 
 template <int D, typename TC>
-__global__ void k_pair_fused(DevProb p, const double* __restrict__ Un, const TC* __restrict__ Kc,
-                             const double* __restrict__ Pc, double* __restrict__ K, double* __restrict__ P,
-                             unsigned flags) {
+__device__ inline void block_btdb(const TC* gA, const TC* gB, const TC* Dm, TC w, double acc[D][D]) {
+    constexpr int NS = (D == 2 ? 3 : 6);
+    for (int i = 0; i < D; ++i) {
+        TC bi[NS], btd[NS];
+        bcol<D, TC>(gA, i, bi);
+        for (int l = 0; l < NS; ++l) {
+            TC s = TC(0);
+            for (int k = 0; k < NS; ++k) s += bi[k] * Dm[k * NS + l];
+            btd[l] = s;
+        }
+        for (int j = 0; j < D; ++j) {
+            TC bj[NS];
+            bcol<D, TC>(gB, j, bj);
+            TC kt = TC(0);
+            for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
+            acc[i][j] += double(w * kt);
+        }
+    }
+}
+
+// FP64 internal-force contribution of node la of an uncut element (P_tmp = B^T
+// sigma with the reference gradient transform, weights accumulated per qp).
+template <int D>
+__device__ inline void plain_p(const DevProb& p, const double (*x)[3], const int* nodes, int la, const double* Dm,
+                               const double* Un, double* out) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
+    double pdN[NEN][D], g[NEN][3];
+    for (int n = 0; n < NEN; ++n)
+        for (int c = 0; c < D; ++c) pdN[n][c] = simplex_dNdr(n, c);
+    const double J = transform<D>(NEN, pdN, x, g);
+    double eps[NS], sig[NS];
+    for (int k = 0; k < NS; ++k) eps[k] = 0.0;
+    for (int n = 0; n < NEN; ++n)
+        for (int c = 0; c < D; ++c) {
+            double bb[NS];
+            bcol<D, double>(g[n], c, bb);
+            const double u = Un[int64_t(nodes[n]) * D + c];
+            for (int k = 0; k < NS; ++k) eps[k] += bb[k] * u;
+        }
+    lei_stress<D>(Dm, eps, sig);
+    for (int i = 0; i < D; ++i) {
+        double bi[NS];
+        bcol<D, double>(g[la], i, bi);
+        double pt = 0.0;
+        for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
+        double acc = 0.0;
+        for (int q = 0; q < NQ; ++q) acc += (J * (D == 2 ? TRI_QW[q] : TET_QW[q])) * pt;
+        out[i] = acc;
+    }
+}
+
+template <int D, typename TC>
+__global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __restrict__ Un,
+                                                    const TC* __restrict__ Kc, const double* __restrict__ Pc,
+                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
+    constexpr int NG = GroupCfg<D>::NG, MAXI = GroupCfg<D>::MAXI;
+    __shared__ int s_iptr[NG + 1];
+    __shared__ int s_qptr[NG + 1];
+    __shared__ int s_ci[MAXI];
+    __shared__ int s_nodes[MAXI][NEN];
+    __shared__ int s_la[MAXI];
+    __shared__ int s_mat[MAXI];
+    __shared__ TC s_g[MAXI][NEN][D];
+    __shared__ TC s_w[MAXI];
+    __shared__ double s_P[MAXI][D];
     __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+
+    const int nall = p.n_nodes + p.n_enr;
+    const int a0 = blockIdx.x * NG;
+    const int ng = min(NG, nall - a0);
+    const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
         const int m = k / (NS * NS);
         sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
     }
+    if (threadIdx.x <= ng) {
+        s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
+        s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
+    }
     __syncthreads();
-    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= p.n_pairs) return;
-    const int a = p.npat_row[q], b = p.npat_col[q];
-    double acc[D][D];
-    for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
-    double pa[D];
-    for (int i = 0; i < D; ++i) pa[i] = 0.0;
-    const bool diag = (a == b) && (flags & PG_FUNC);
-    const bool jac = (flags & PG_JAC) != 0;
+    const int ibase = s_iptr[0];
+    const int ni = s_iptr[ng] - ibase;
+    const bool staged = ni <= MAXI;
     TC wsum = TC(0);
     for (int k = 0; k < NQ; ++k) wsum += TC(D == 2 ? TRI_QW[k] : TET_QW[k]);
-    const int e0 = p.n2e_ptr[a], e1 = p.n2e_ptr[a + 1];
-    for (int t = e0; t < e1; ++t) {
-        const int e = p.n2e[t];
-        const int ci = p.cut_of_elem[e];
-        if (ci < 0) {
+
+    // ---- phase A: incidences ----
+    if (staged) {
+        for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+            int lo = 0, hi = ng;  // owner: s_iptr[lo] - ibase <= i < s_iptr[lo+1] - ibase
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
+            }
+            const int a = a0 + lo;
+            const int e = p.n2e[ibase + i];
+            const int ci = p.cut_of_elem[e];
             int nodes[NEN];
-            int la = -1, lb = -1;
+            int la = -1;
             for (int n = 0; n < NEN; ++n) {
                 nodes[n] = p.conn[int64_t(e) * NEN + n];
+                s_nodes[i][n] = nodes[n];
                 if (nodes[n] == a) la = n;
-                if (nodes[n] == b) lb = n;
             }
-            if (la < 0 || lb < 0) continue;
-            double x[NEN][3];
-            for (int n = 0; n < NEN; ++n)
-                for (int i = 0; i < D; ++i) x[n][i] = p.coords[int64_t(nodes[n]) * D + i];
-            const int mi = p.elem_mat[e];
-            if (jac) {
+            s_ci[i] = ci;
+            if (ci < 0) {
+                double x[NEN][3];
+                for (int n = 0; n < NEN; ++n)
+                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
                 TC g[NEN][D];
                 const TC J = simplex_grads<D, TC>(x, g);
-                const TC w = J * wsum;
-                const TC* Dm = mi < MAX_LDS_MATS ? sD[mi] : nullptr;
-                for (int i = 0; i < D; ++i) {
-                    TC bi[NS], btd[NS];
-                    bcol<D, TC>(g[la], i, bi);
-                    for (int l = 0; l < NS; ++l) {
-                        TC s = TC(0);
-                        for (int k = 0; k < NS; ++k) s += bi[k] * (Dm ? Dm[k * NS + l] : TC(p.mats[mi].D[k * NS + l]));
-                        btd[l] = s;
-                    }
-                    for (int j = 0; j < D; ++j) {
-                        TC bj[NS];
-                        bcol<D, TC>(g[lb], j, bj);
-                        TC kt = TC(0);
-                        for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
-                        acc[i][j] += double(w * kt);
-                    }
-                }
-            }
-            if (diag) {
-                double pdN[NEN][D], g[NEN][3];
                 for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) pdN[n][c] = simplex_dNdr(n, c);
-                const double J = transform<D>(NEN, pdN, x, g);
-                double wq = 0.0;
-                for (int k = 0; k < NQ; ++k) wq += J * (D == 2 ? TRI_QW[k] : TET_QW[k]);
-                const double* Dm = p.mats[mi].D;
-                double eps[NS], sig[NS];
-                for (int k = 0; k < NS; ++k) eps[k] = 0.0;
-                for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) {
-                        double bb[NS];
-                        bcol<D, double>(g[n], c, bb);
-                        const double u = Un[int64_t(nodes[n]) * D + c];
-                        for (int k = 0; k < NS; ++k) eps[k] += bb[k] * u;
-                    }
-                lei_stress<D>(Dm, eps, sig);
-                for (int i = 0; i < D; ++i) {
-                    double bi[NS];
-                    bcol<D, double>(g[la], i, bi);
-                    double pt = 0.0;
-                    for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
-                    pa[i] += wq * pt;
+                    for (int c = 0; c < D; ++c) s_g[i][n][c] = g[n][c];
+                s_w[i] = J * wsum;
+                const int mi = p.elem_mat[e];
+                s_mat[i] = mi;
+                if (func && la >= 0) plain_p<D>(p, x, nodes, la, p.mats[mi].D, Un, s_P[i]);
+            } else {
+                if (la < 0) {
+                    const pg_cut_rec& r = p.cut[ci];
+                    for (int k = 0; k < r.n_enr; ++k)
+                        if (r.enr[k] == a) la = NEN + k;
                 }
+                if (func && la >= 0)
+                    for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
             }
-        } else {
-            const int la = find_local(p, e, ci, a, NEN);
-            const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
-            if (la < 0 || lb < 0) continue;
-            const int nd = (NEN + p.cut[ci].n_enr) * D;
-            if (jac) {
-                const TC* Ke = Kc + p.cut_koff[ci];
-                for (int i = 0; i < D; ++i)
-                    for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
-            }
-            if (diag) {
-                const double* Pe = Pc + p.cut_poff[ci];
-                for (int i = 0; i < D; ++i) pa[i] += Pe[la * D + i];
-            }
+            s_la[i] = la;
         }
     }
-    if (jac) {
-        const int32_t* mp = p.npat_map + q * D * D;
+    __syncthreads();
+
+    // ---- phase B: node pairs ----
+    const int q0 = s_qptr[0], q1 = s_qptr[ng];
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        int lo = 0, hi = ng;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+        }
+        const int a = a0 + lo;
+        const int b = p.npat_col[q];
+        const bool diag = (a == b) && func;
+        double acc[D][D];
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+            for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
+        double pa[D];
+        for (int i = 0; i < D; ++i) pa[i] = 0.0;
+        const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
+        for (int t = t0; t < t1; ++t) {
+            if (staged) {
+                const int ci = s_ci[t];
+                const int la = s_la[t];
+                if (la < 0) continue;
+                if (ci < 0) {
+                    int lb = -1;
+                    for (int n = 0; n < NEN; ++n)
+                        if (s_nodes[t][n] == b) lb = n;
+                    if (lb < 0) continue;
+                    if (jac) block_btdb<D, TC>(s_g[t][la], s_g[t][lb], sD[s_mat[t]], s_w[t], acc);
+                } else {
+                    const int e = p.n2e[ibase + t];
+                    const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
+                    if (lb < 0) continue;
+                    if (jac) {
+                        const int nd = (NEN + p.cut[ci].n_enr) * D;
+                        const TC* Ke = Kc + p.cut_koff[ci];
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
+                    }
+                }
+                if (diag)
+                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
+            } else {
+                // same arithmetic from global memory (incidence tables exceed LDS)
+                const int e = p.n2e[ibase + t];
+                const int ci = p.cut_of_elem[e];
+                const int la = find_local(p, e, ci, a, NEN);
+                const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
+                if (la < 0 || lb < 0) continue;
+                if (ci < 0) {
+                    int nodes[NEN];
+                    double x[NEN][3];
+                    for (int n = 0; n < NEN; ++n) {
+                        nodes[n] = p.conn[int64_t(e) * NEN + n];
+                        for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
+                    }
+                    const int mi = p.elem_mat[e];
+                    if (jac) {
+                        TC g[NEN][D];
+                        const TC J = simplex_grads<D, TC>(x, g);
+                        block_btdb<D, TC>(g[la], g[lb], sD[mi], J * wsum, acc);
+                    }
+                    if (diag) {
+                        double pp[D];
+                        plain_p<D>(p, x, nodes, la, p.mats[mi].D, Un, pp);
+                        for (int i = 0; i < D; ++i) pa[i] += pp[i];
+                    }
+                } else {
+                    const int nd = (NEN + p.cut[ci].n_enr) * D;
+                    if (jac) {
+                        const TC* Ke = Kc + p.cut_koff[ci];
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
+                    }
+                    if (diag)
+                        for (int i = 0; i < D; ++i) pa[i] += Pc[p.cut_poff[ci] + la * D + i];
+                }
+            }
+        }
+        if (jac) {
+            const int32_t* mp = p.npat_map + int64_t(q) * D * D;
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+        }
+        if (diag)
+            for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
     }
-    if (diag)
-        for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
 }
 
 template <typename T>
@@ -869,6 +1009,7 @@ int check_desc(const pg_problem_desc* s) {
     }
     for (int64_t k = 0; k < int64_t(s->n_elem) * s->nen; ++k)
         if (s->conn[k] < 0 || s->conn[k] >= s->n_nodes) { g_err = "connectivity out of range"; return PG_E_ARG; }
+    if (s->n_mat > MAX_LDS_MATS) { g_err = "at most 8 materials are supported"; return PG_E_UNSUPPORTED; }
     for (int m = 0; m < s->n_mat; ++m)
         if (s->mats[m].type == PG_MAT_LEI && s->mats[m].nstr != (s->dim == 2 ? 3 : 6)) { g_err = "bad D size"; return PG_E_ARG; }
     const int64_t nnzt = s->nnz[0] + s->nnz[1] + s->nnz[2] + s->nnz[3];
@@ -922,20 +1063,23 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     } else {
         if (d.n_cut > 0) {
             if (d.dim == 2) {
-                if (fp32) k_cut<2, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<2, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32) k_cut<2, float, 4><<<(d.n_cut + 3) / 4, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<2, double, 4><<<(d.n_cut + 3) / 4, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             } else {
-                if (fp32) k_cut<3, float><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<3, double><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32) k_cut<3, float, 1><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<3, double, 1><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         }
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
+        const int nall = d.n_nodes + d.n_enr;
         if (d.dim == 2) {
-            if (fp32) k_pair_fused<2, float><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
-            else k_pair_fused<2, double><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+            const unsigned gg = unsigned((nall + GroupCfg<2>::NG - 1) / GroupCfg<2>::NG);
+            if (fp32) k_pair_group<2, float><<<gg, 256, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
+            else k_pair_group<2, double><<<gg, 256, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
         } else {
-            if (fp32) k_pair_fused<3, float><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
-            else k_pair_fused<3, double><<<gp, h->pair_block, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+            const unsigned gg = unsigned((nall + GroupCfg<3>::NG - 1) / GroupCfg<3>::NG);
+            if (fp32) k_pair_group<3, float><<<gg, 256, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
+            else k_pair_group<3, double><<<gg, 256, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
         }
     }
     HIP_OK(hipGetLastError());
@@ -997,6 +1141,7 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut, s->cut, s->n_cut, pg_cut_rec);
     UP(mats, s->mats, s->n_mat, pg_material);
     UP(node_dof, s->node_dof, nall * s->dim, int32_t);
+    UP(npat_ptr, s->npat_ptr, nall + 1, int32_t);
     UP(npat_row, s->npat_row, s->n_pairs, int32_t);
     UP(npat_col, s->npat_col, s->n_pairs, int32_t);
     UP(npat_map, s->npat_map, s->n_pairs * s->dim * s->dim, int32_t);
@@ -1028,16 +1173,13 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
 
 int pg_asm_destroy(pg_asm* h) {
     if (!h) return PG_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (void* p : h->allocs) hipFree(p);
-    if (h->dU) hipFree(h->dU);
-    if (h->dK) hipFree(h->dK);
-    if (h->dP) hipFree(h->dP);
-    if (h->dIc) hipFree(h->dIc);
-    if (h->dIt) hipFree(h->dIt);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : {(void*)h->dU, (void*)h->dK, (void*)h->dP, (void*)h->dIc, (void*)h->dIt})
+        if (p) (void)hipFree(p);
     for (auto& e : h->ev)
-        if (e) hipEventDestroy(e);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PG_OK;
 }
