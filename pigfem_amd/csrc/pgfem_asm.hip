@@ -513,27 +513,41 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
             if (rec.child_nodes[k][n] == l)
                 for (int a = 0; a < D; ++a) gr[a] = qgc[g][q][n][a];
     };
+    // local-node-pair blocks: K(li, lj) = sum_q B_li^T (wJ D) B_lj, then the
+    // cohesive fold; one lane per block
     TK* Ke = Kst + p.cut_koff[ci];
-    for (int idx = gl; idx < nd * nd; idx += GS) {
-        const int i = idx / nd, j = idx % nd;
-        const int li = i / D, c_i = i % D, lj = j / D, c_j = j % D;
-        TK v = TK(0);
+    for (int pb = gl; pb < nloc * nloc; pb += GS) {
+        const int li = pb / nloc, lj = pb % nloc;
+        TK acc[D][D];
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) acc[i][j] = TK(0);
         for (int q = 0; q < nvq; ++q) {
             double gid[D], gjd[D];
             grad(q, li, gid);
             grad(q, lj, gjd);
-            TK gi[D], gj[D], bi[NS], bj[NS];
+            bool zi = true, zj = true;
+            for (int a = 0; a < D; ++a) { zi = zi && gid[a] == 0.0; zj = zj && gjd[a] == 0.0; }
+            if (zi || zj) continue;
+            TK gi[D], gj[D];
             for (int a = 0; a < D; ++a) { gi[a] = TK(gid[a]); gj[a] = TK(gjd[a]); }
-            bcol<D, TK>(gi, c_i, bi);
-            bcol<D, TK>(gj, c_j, bj);
             const TK* Dm = sD[qm[g][q]];
-            TK kt = TK(0);
-            for (int s = 0; s < NS; ++s) {
-                TK db = TK(0);
-                for (int k = 0; k < NS; ++k) db += Dm[s * NS + k] * bj[k];
-                kt += bi[s] * db;
+            const TK w = TK(qw[g][q]);
+            for (int i = 0; i < D; ++i) {
+                TK bi[NS], btd[NS];
+                bcol<D, TK>(gi, i, bi);
+                for (int s = 0; s < NS; ++s) {
+                    TK v = TK(0);
+                    for (int k = 0; k < NS; ++k) v += bi[k] * Dm[k * NS + s];
+                    btd[s] = v;
+                }
+                for (int j = 0; j < D; ++j) {
+                    TK bj[NS];
+                    bcol<D, TK>(gj, j, bj);
+                    TK kt = TK(0);
+                    for (int s = 0; s < NS; ++s) kt += btd[s] * bj[s];
+                    acc[i][j] += w * kt;
+                }
             }
-            v += TK(qw[g][q]) * kt;
         }
         for (int k = 0; k < rec.n_coh; ++k) {
             int m1 = -1, m2 = -1;
@@ -544,14 +558,15 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
             if (m1 < 0 || m2 < 0) continue;
             const int n1 = m1 % cnn, n2 = m2 % cnn;
             const TK sgn = ((m1 < cnn) == (m2 < cnn)) ? TK(1) : TK(-1);
-            TK s = TK(0);
             for (int q = 0; q < cnq; ++q) {
                 const int cq = k * cnq + q;
-                s += TK(cw[g][cq]) * (sgn * (TK(cN[g][cq][n1]) * TK(cN[g][cq][n2]) * TK(cRDR[g][cq][c_i][c_j])));
+                const TK f = TK(cw[g][cq]) * sgn * TK(cN[g][cq][n1]) * TK(cN[g][cq][n2]);
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] += f * TK(cRDR[g][cq][i][j]);
             }
-            v += s;
         }
-        Ke[idx] = v;
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) Ke[int64_t(li * D + i) * nd + lj * D + j] = acc[i][j];
     }
     double* Pe = Pst + p.cut_poff[ci];
     for (int i = gl; i < nd; i += GS) {
@@ -638,23 +653,40 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
         for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
 }
 
-// Gradients of a linear simplex in TC from FP64 coordinate differences
-// (closed-form adjugate of the affine Jacobian; the same map as
-// Elem::transform_gradients for a linear parent).  Returns J.
-template <int D, typename TC>
-__device__ inline TC simplex_grads(const double (*x)[3], TC (*g)[D]) {
-    TC d[D][D];  // d[k][i] = x_{k+1}[i] - x_0[i]  (row k: derivative along r_k)
+// --------------------------------------------------------------------------
+// Node-group kernel of the class-binned schedule.  A block owns NG consecutive
+// nodes.  Phase A stages, once per (node a, incident element e) incidence, the
+// element's nodes, the TC gradients of its nodes and the left factor
+// L = w B_a^T D (uncut elements, recomputed from FP64 coordinate differences;
+// w = J * sum of the reference qp weights), and the FP64 internal-force
+// contribution of a (uncut: w B_a^T sigma with FP64 gradients; cut: the staged
+// FP64 residual).  Phase B gives each node pair (a, b) of the group one lane,
+// which sums over a's incidences -- in ascending element order, as the
+// reference's element loop -- L . B_b of uncut elements containing b and the
+// staged TK blocks of cut elements, in FP64, and writes the d x d block once
+// through the scatter indices.  A group whose incidences overflow the LDS
+// tables takes the same arithmetic from global memory.
+// --------------------------------------------------------------------------
+template <int D> struct GroupCfg;
+template <> struct GroupCfg<2> { static constexpr int NG = 32, MAXI = 320, MAXP = 512; };
+template <> struct GroupCfg<3> { static constexpr int NG = 8, MAXI = 256, MAXP = 384; };
+
+// Closed-form gradients of a linear simplex (adjugate of the affine Jacobian
+// over its determinant) from FP64 coordinate differences; returns J.
+template <int D, typename T>
+__device__ inline T simplex_grads(const double (*x)[3], T (*g)[D]) {
+    T d[D][D];  // d[k][i] = x_{k+1}[i] - x_0[i]  (row k: derivative along r_k)
     for (int k = 0; k < D; ++k)
-        for (int i = 0; i < D; ++i) d[k][i] = TC(x[k + 1][i] - x[0][i]);
+        for (int i = 0; i < D; ++i) d[k][i] = T(x[k + 1][i] - x[0][i]);
     if constexpr (D == 2) {
-        const TC J = d[0][0] * d[1][1] - d[0][1] * d[1][0];
-        const TC rJ = TC(1) / J;
+        const T J = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+        const T rJ = T(1) / J;
         g[1][0] = d[1][1] * rJ; g[1][1] = -d[1][0] * rJ;
         g[2][0] = -d[0][1] * rJ; g[2][1] = d[0][0] * rJ;
         g[0][0] = -(g[1][0] + g[2][0]); g[0][1] = -(g[1][1] + g[2][1]);
         return J;
     } else {
-        TC c[3][3];  // cofactors of d
+        T c[3][3];
         c[0][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
         c[0][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
         c[0][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
@@ -664,9 +696,8 @@ __device__ inline TC simplex_grads(const double (*x)[3], TC (*g)[D]) {
         c[2][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
         c[2][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
         c[2][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
-        const TC J = d[0][0] * c[0][0] + d[0][1] * c[0][1] + d[0][2] * c[0][2];
-        const TC rJ = TC(1) / J;
-        // grad N_{k+1} = Jac^-1 e_k: component i = c[k][i] / J
+        const T J = d[0][0] * c[0][0] + d[0][1] * c[0][1] + d[0][2] * c[0][2];
+        const T rJ = T(1) / J;
         for (int k = 0; k < 3; ++k)
             for (int i = 0; i < 3; ++i) g[k + 1][i] = c[k][i] * rJ;
         for (int i = 0; i < 3; ++i) g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
@@ -674,55 +705,40 @@ __device__ inline TC simplex_grads(const double (*x)[3], TC (*g)[D]) {
     }
 }
 
-// --------------------------------------------------------------------------
-// Node-group kernel of the class-binned schedule.  A block owns NG consecutive
-// nodes.  Phase A stages, once per (node, incident element) incidence, the
-// element's connectivity, its TC gradients and weight (uncut elements,
-// recomputed from FP64 coordinate differences) and the FP64 internal-force
-// contribution of the owner node (uncut: B^T sigma with the reference's FP64
-// gradient transform; cut: the staged FP64 residual).  Phase B gives each node
-// pair (a, b) of the group one lane, which sums over a's incidences -- in
-// ascending element order, as the reference's element loop -- the TC blocks
-// w B_a^T D B_b of uncut elements containing b and the staged TK blocks of cut
-// elements, in FP64, and writes the d x d block once through the scatter
-// indices.  A group whose incidences overflow the LDS tables takes the same
-// arithmetic from global memory.
-// --------------------------------------------------------------------------
-template <int D> struct GroupCfg;
-template <> struct GroupCfg<2> { static constexpr int NG = 32, MAXI = 384; };
-template <> struct GroupCfg<3> { static constexpr int NG = 8, MAXI = 256; };
-
+// acc += L . B(gb), L = w B_a^T D (D x NS row-major)
 template <int D, typename TC>
-__device__ inline void block_btdb(const TC* gA, const TC* gB, const TC* Dm, TC w, double acc[D][D]) {
+__device__ inline void apply_left(const TC* L, const TC* gb, double acc[D][D]) {
     constexpr int NS = (D == 2 ? 3 : 6);
-    for (int i = 0; i < D; ++i) {
-        TC bi[NS], btd[NS];
-        bcol<D, TC>(gA, i, bi);
-        for (int l = 0; l < NS; ++l) {
+    for (int j = 0; j < D; ++j) {
+        TC bj[NS];
+        bcol<D, TC>(gb, j, bj);
+        for (int i = 0; i < D; ++i) {
             TC s = TC(0);
-            for (int k = 0; k < NS; ++k) s += bi[k] * Dm[k * NS + l];
-            btd[l] = s;
-        }
-        for (int j = 0; j < D; ++j) {
-            TC bj[NS];
-            bcol<D, TC>(gB, j, bj);
-            TC kt = TC(0);
-            for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
-            acc[i][j] += double(w * kt);
+            for (int k = 0; k < NS; ++k) s += L[i * NS + k] * bj[k];
+            acc[i][j] += double(s);
         }
     }
 }
 
-// FP64 internal-force contribution of node la of an uncut element (P_tmp = B^T
-// sigma with the reference gradient transform, weights accumulated per qp).
+template <int D, typename TC>
+__device__ inline void left_factor(const TC* ga, const TC* Dm, TC w, TC* L) {
+    constexpr int NS = (D == 2 ? 3 : 6);
+    for (int i = 0; i < D; ++i) {
+        TC bi[NS];
+        bcol<D, TC>(ga, i, bi);
+        for (int l = 0; l < NS; ++l) {
+            TC s = TC(0);
+            for (int k = 0; k < NS; ++k) s += bi[k] * Dm[k * NS + l];
+            L[i * NS + l] = w * s;
+        }
+    }
+}
+
+// FP64 internal-force contribution w B_la^T sigma of an uncut element.
 template <int D>
-__device__ inline void plain_p(const DevProb& p, const double (*x)[3], const int* nodes, int la, const double* Dm,
+__device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, double w, const double* Dm,
                                const double* Un, double* out) {
-    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    double pdN[NEN][D], g[NEN][3];
-    for (int n = 0; n < NEN; ++n)
-        for (int c = 0; c < D; ++c) pdN[n][c] = simplex_dNdr(n, c);
-    const double J = transform<D>(NEN, pdN, x, g);
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
     double eps[NS], sig[NS];
     for (int k = 0; k < NS; ++k) eps[k] = 0.0;
     for (int n = 0; n < NEN; ++n)
@@ -738,9 +754,7 @@ __device__ inline void plain_p(const DevProb& p, const double (*x)[3], const int
         bcol<D, double>(g[la], i, bi);
         double pt = 0.0;
         for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
-        double acc = 0.0;
-        for (int q = 0; q < NQ; ++q) acc += (J * (D == 2 ? TRI_QW[q] : TET_QW[q])) * pt;
-        out[i] = acc;
+        out[i] = w * pt;
     }
 }
 
@@ -749,15 +763,15 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    constexpr int NG = GroupCfg<D>::NG, MAXI = GroupCfg<D>::MAXI;
+    constexpr int NG = GroupCfg<D>::NG, MAXI = GroupCfg<D>::MAXI, MAXP = GroupCfg<D>::MAXP;
     __shared__ int s_iptr[NG + 1];
     __shared__ int s_qptr[NG + 1];
-    __shared__ int s_ci[MAXI];
-    __shared__ int s_nodes[MAXI][NEN];
+    __shared__ unsigned char s_own[MAXP];   // owner node (offset in group) of each pair
+    __shared__ int s_ci[MAXI];              // cut index, or -1
     __shared__ int s_la[MAXI];
-    __shared__ int s_mat[MAXI];
+    __shared__ int s_nodes[MAXI][NEN];
     __shared__ TC s_g[MAXI][NEN][D];
-    __shared__ TC s_w[MAXI];
+    __shared__ TC s_L[MAXI][D * NS];
     __shared__ double s_P[MAXI][D];
     __shared__ TC sD[MAX_LDS_MATS][NS * NS];
 
@@ -776,14 +790,17 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     __syncthreads();
     const int ibase = s_iptr[0];
     const int ni = s_iptr[ng] - ibase;
-    const bool staged = ni <= MAXI;
-    TC wsum = TC(0);
-    for (int k = 0; k < NQ; ++k) wsum += TC(D == 2 ? TRI_QW[k] : TET_QW[k]);
+    const int q0 = s_qptr[0], q1 = s_qptr[ng];
+    const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
+    double wsum = 0.0;
+    for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
-    // ---- phase A: incidences ----
+    // ---- phase A: pair owners and incidences ----
     if (staged) {
+        if (threadIdx.x < ng)
+            for (int q = s_qptr[threadIdx.x]; q < s_qptr[threadIdx.x + 1]; ++q) s_own[q - q0] = threadIdx.x;
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
-            int lo = 0, hi = ng;  // owner: s_iptr[lo] - ibase <= i < s_iptr[lo+1] - ibase
+            int lo = 0, hi = ng;  // owner node of incidence i
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
@@ -803,14 +820,17 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
-                TC g[NEN][D];
-                const TC J = simplex_grads<D, TC>(x, g);
+                double g[NEN][D];
+                const double J = simplex_grads<D, double>(x, g);
+                const double w = J * wsum;
+                TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) s_g[i][n][c] = g[n][c];
-                s_w[i] = J * wsum;
+                    for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
                 const int mi = p.elem_mat[e];
-                s_mat[i] = mi;
-                if (func && la >= 0) plain_p<D>(p, x, nodes, la, p.mats[mi].D, Un, s_P[i]);
+                if (la >= 0) {
+                    if (jac) left_factor<D, TC>(gt[la], sD[mi], TC(w), s_L[i]);
+                    if (func) plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, s_P[i]);
+                }
             } else {
                 if (la < 0) {
                     const pg_cut_rec& r = p.cut[ci];
@@ -826,12 +846,17 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     __syncthreads();
 
     // ---- phase B: node pairs ----
-    const int q0 = s_qptr[0], q1 = s_qptr[ng];
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        int lo = 0, hi = ng;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+        int lo;
+        if (staged) {
+            lo = s_own[q - q0];
+        } else {
+            lo = 0;
+            int hi = ng;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+            }
         }
         const int a = a0 + lo;
         const int b = p.npat_col[q];
@@ -852,7 +877,7 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                     for (int n = 0; n < NEN; ++n)
                         if (s_nodes[t][n] == b) lb = n;
                     if (lb < 0) continue;
-                    if (jac) block_btdb<D, TC>(s_g[t][la], s_g[t][lb], sD[s_mat[t]], s_w[t], acc);
+                    if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
                 } else {
                     const int e = p.n2e[ibase + t];
                     const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
@@ -881,14 +906,18 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                         for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
                     }
                     const int mi = p.elem_mat[e];
+                    double g[NEN][D];
+                    const double J = simplex_grads<D, double>(x, g);
+                    const double w = J * wsum;
                     if (jac) {
-                        TC g[NEN][D];
-                        const TC J = simplex_grads<D, TC>(x, g);
-                        block_btdb<D, TC>(g[la], g[lb], sD[mi], J * wsum, acc);
+                        TC ga[D], gb[D], L[D * NS];
+                        for (int c = 0; c < D; ++c) { ga[c] = TC(g[la][c]); gb[c] = TC(g[lb][c]); }
+                        left_factor<D, TC>(ga, sD[mi], TC(w), L);
+                        apply_left<D, TC>(L, gb, acc);
                     }
                     if (diag) {
                         double pp[D];
-                        plain_p<D>(p, x, nodes, la, p.mats[mi].D, Un, pp);
+                        plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, pp);
                         for (int i = 0; i < D; ++i) pa[i] += pp[i];
                     }
                 } else {
