@@ -62,8 +62,17 @@ def main():
     ic = dIc if prob.n_cqp else None
     it = dIt if prob.n_cqp else None
 
+    from pigfem_amd.autotune import tune
+    tuned = {}
+    for mode in ("binned64", "mixed"):
+        asm.mode = mode
+        tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
+        log(f"[bench] tuned {mode}: group_nodes, cut_variant = {tuned[mode]}")
+
     def run(mode, steps):
         asm.mode = mode
+        if mode in tuned:
+            asm.set_tiles(*tuned[mode])
         for _ in range(steps):
             asm.assemble_device(dU, ic, it, dK, dP)
 
@@ -81,6 +90,8 @@ def main():
     # accuracy of the mixed schedule against the FP64 schedule on the same inputs
     def snapshot(mode):
         asm.mode = mode
+        if mode in tuned:
+            asm.set_tiles(*tuned[mode])
         asm.assemble_device(dU, ic, it, dK, dP)
         synchronize()
         return (dK.download(prob.nnz_total), dP.download(prob.n_dof),
@@ -95,6 +106,7 @@ def main():
 
     # per-kernel durations with HIP events on the launch stream (mixed schedule)
     asm.mode = "mixed"
+    asm.set_tiles(*tuned["mixed"])
     asm.set_timing(True)
     kt = []
     for _ in range(max(args.steps, 5)):
@@ -175,6 +187,7 @@ def main():
         "whole_assembly_frac": whole / HBM_PEAK_GBS,
         "algorithmic_bytes": B_alg,
         "setup_s": setup_s,
+        "tiles": {"mixed": tuned["mixed"], "binned64": tuned["binned64"]},
     }
     print(json.dumps(out), flush=True)
 

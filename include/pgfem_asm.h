@@ -199,9 +199,13 @@ int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total,
 int pg_asm_last_timing(const pg_asm* h, double kernel_ms[3]);
 int pg_asm_set_timing(pg_asm* h, int enable);
 
-/* Schedule parameters chosen by the autotuner (threads per block of the
- * node-pair kernel, waves per block of the cut kernel).  0 keeps the default. */
-int pg_asm_set_tiles(pg_asm* h, int pair_block, int cut_block);
+/* Schedule parameters of the class-binned schedules, chosen by the autotuner:
+ *   group_nodes  nodes per block of the node-group kernel: 16 or 32 in 2D,
+ *                4 or 8 in 3D (0: default 32 / 8);
+ *   cut_variant  small-class cut kernel: 0 = 4 elements per wavefront,
+ *                1 = 4 per wavefront with a 3-waves/SIMD register cap,
+ *                2 = 2 per wavefront, 3 = 8 per wavefront. */
+int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant);
 
 /* Algorithmic bytes of one K+R assembly of this problem (the roofline
  * numerator): connectivity, coordinates, U, cut records, ISVs, CSR values +

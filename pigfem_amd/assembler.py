@@ -146,8 +146,9 @@ class AssemblerGPU:
         check(lib().pg_asm_last_timing(self._h, ms), "pg_asm_last_timing")
         return list(ms)
 
-    def set_tiles(self, pair_block: int = 0, elem_block: int = 0) -> None:
-        check(lib().pg_asm_set_tiles(self._h, int(pair_block), int(elem_block)), "pg_asm_set_tiles")
+    def set_tiles(self, group_nodes: int = 0, cut_variant: int = 0) -> None:
+        """Schedule of the class-binned modes (see pg_asm_set_tiles)."""
+        check(lib().pg_asm_set_tiles(self._h, int(group_nodes), int(cut_variant)), "pg_asm_set_tiles")
 
     def algorithmic_bytes(self) -> int:
         return int(lib().pg_asm_algorithmic_bytes(self._h))

@@ -57,6 +57,7 @@ struct DevProb {
     const int64_t* cut_koff;   // per cut element: offset of its ndof^2 block
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
     const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
+    const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
     int n_mat;
 };
 
@@ -368,16 +369,16 @@ __global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const doub
 //   phase 2: every lane accumulates its K entries b_i . (wJ D) b_j over the qps in
 //            TK, adds the cohesive fold and stores; P (FP64) likewise.
 // --------------------------------------------------------------------------
-template <int D, typename TK, int EPW>
-__global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict__ Un,
+template <int D, typename TK, int EPW, int MAXN, int MAXQ, int MAXC, int OCC>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(OCC))) k_cut(DevProb p, const int32_t* __restrict__ list, int n_list,
+                                            const double* __restrict__ Un,
                                             const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                             TK* __restrict__ Kst, double* __restrict__ Pst, int* __restrict__ err) {
     constexpr int NS = (D == 2 ? 3 : 6), NEN = D + 1, GS = 64 / EPW;
-    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q;
+    constexpr int CUT_MAXC = MAXC;
     __shared__ double xs[EPW][MAXN][3];
     __shared__ double us[EPW][MAXN * D];
-    __shared__ double pgs[EPW][NEN][D];     // parent gradients (constant)
-    __shared__ double qgc[EPW][MAXQ][4][D]; // child gradients per qp
+    __shared__ double qg[EPW][MAXQ][MAXN][D];  // gradient of every local node at every qp
     __shared__ double qw[EPW][MAXQ];
     __shared__ int qm[EPW][MAXQ];
     __shared__ int qk[EPW][MAXQ];           // child of each qp
@@ -390,8 +391,9 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
     __shared__ TK sD[MAX_LDS_MATS][NS * NS];
 
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    const int ci = blockIdx.x * EPW + g;
-    const bool live = ci < p.n_cut;
+    const int li_ = blockIdx.x * EPW + g;
+    const bool live = li_ < n_list;
+    const int ci = live ? list[li_] : 0;
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += 64) {
         const int m = k / (NS * NS);
         sD[m][k % (NS * NS)] = m < p.n_mat ? TK(p.mats[m].D[k % (NS * NS)]) : TK(0);
@@ -417,8 +419,8 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
     const int ncq = live ? rec.n_coh * cnq : 0;
-    for (int t = gl; t < nvq + ncq; t += GS) {
-        if (t < nvq) {
+    for (int t = gl; t < nvq; t += GS) {
+        {
             // ---- volumetric qp ----
             int k = 0, q = t;
             for (; k < rec.n_child; ++k) {
@@ -442,11 +444,6 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                 for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
             }
             transform<D>(NEN, pdN, px, pgr);
-            if (t == 0)
-                for (int n = 0; n < NEN; ++n)
-                    for (int a = 0; a < D; ++a) pgs[g][n][a] = pgr[n][a];
-            for (int n = 0; n < 4; ++n)
-                for (int a = 0; a < D; ++a) qgc[g][t][n][a] = cg[n][a];
             double eps[NS];
             for (int s = 0; s < NS; ++s) eps[s] = 0.0;
             for (int l = 0; l < nloc; ++l) {
@@ -460,6 +457,7 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
                         if (cl[n] == l)
                             for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
                 }
+                for (int a = 0; a < D; ++a) qg[g][t][l][a] = gr[a];
                 for (int c = 0; c < D; ++c) {
                     double b[NS];
                     bcol<D, double>(gr, c, b);
@@ -472,9 +470,12 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
             double sig[NS];
             lei_stress<D>(p.mats[rec.child_mat[k]].D, eps, sig);
             for (int s = 0; s < NS; ++s) qs[g][t][s] = sig[s];
-        } else {
+        }
+    }
+    for (int cq = gl; cq < ncq; cq += GS) {
+        {
             // ---- cohesive qp ----
-            const int cq = t - nvq, k = cq / cnq, q = cq % cnq;
+            const int k = cq / cnq, q = cq % cnq;
             double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
             for (int n = 0; n < 3; ++n)
                 for (int i = 0; i < D; ++i) {
@@ -499,19 +500,10 @@ __global__ __launch_bounds__(64) void k_cut(DevProb p, const double* __restrict_
     __syncthreads();
     if (!live) return;
 
-    // gradient of local node l at qp q: parent gradient, the child gradient of the
-    // child node that is enrichment node l, or 0 (src/Mesh.cpp:1596-1614)
+    // gradient of local node l at qp q (parent gradient, the child gradient of the
+    // child node that is enrichment node l, or 0; src/Mesh.cpp:1596-1614)
     auto grad = [&](int q, int l, double* gr) {
-        if (l < NEN) {
-            for (int a = 0; a < D; ++a) gr[a] = pgs[g][l][a];
-            return;
-        }
-        const int k = qk[g][q];
-        const int ccn = rec.child_type[k] == PG_CHILD_TRI3 ? 3 : 4;
-        for (int a = 0; a < D; ++a) gr[a] = 0.0;
-        for (int n = 0; n < ccn; ++n)
-            if (rec.child_nodes[k][n] == l)
-                for (int a = 0; a < D; ++a) gr[a] = qgc[g][q][n][a];
+        for (int a = 0; a < D; ++a) gr[a] = qg[g][q][l][a];
     };
     // local-node-pair blocks: K(li, lj) = sum_q B_li^T (wJ D) B_lj, then the
     // cohesive fold; one lane per block
@@ -668,8 +660,12 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
 // tables takes the same arithmetic from global memory.
 // --------------------------------------------------------------------------
 template <int D> struct GroupCfg;
-template <> struct GroupCfg<2> { static constexpr int NG = 32, MAXI = 320, MAXP = 512; };
-template <> struct GroupCfg<3> { static constexpr int NG = 8, MAXI = 256, MAXP = 384; };
+// per-group LDS capacity (incidences, pairs) for NG nodes: about 10 incidences
+// and 16 pairs per node in 2D, 32 and 48 in 3D
+template <int D, int NG> struct GroupLim {
+    static constexpr int MAXI = (D == 2 ? 10 : 32) * NG;
+    static constexpr int MAXP = (D == 2 ? 16 : 48) * NG;
+};
 
 // Closed-form gradients of a linear simplex (adjugate of the affine Jacobian
 // over its determinant) from FP64 coordinate differences; returns J.
@@ -758,15 +754,16 @@ __device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, d
     }
 }
 
-template <int D, typename TC>
+template <int D, typename TC, int NG>
 __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    constexpr int NG = GroupCfg<D>::NG, MAXI = GroupCfg<D>::MAXI, MAXP = GroupCfg<D>::MAXP;
+    constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     __shared__ int s_iptr[NG + 1];
     __shared__ int s_qptr[NG + 1];
     __shared__ unsigned char s_own[MAXP];   // owner node (offset in group) of each pair
+    static_assert(NG <= 256, "owner offsets are bytes");
     __shared__ int s_ci[MAXI];              // cut index, or -1
     __shared__ int s_la[MAXI];
     __shared__ int s_nodes[MAXI][NEN];
@@ -792,6 +789,14 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     const int ni = s_iptr[ng] - ibase;
     const int q0 = s_qptr[0], q1 = s_qptr[ng];
     const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
+    // issue the first pass's pair loads now; they complete under phase A
+    const int qf = q0 + threadIdx.x;
+    int b_pre = -1;
+    int32_t mp_pre[D * D];
+    if (qf < q1) {
+        b_pre = p.npat_col[qf];
+        for (int k = 0; k < D * D; ++k) mp_pre[k] = p.npat_map[int64_t(qf) * D * D + k];
+    }
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
@@ -806,15 +811,16 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
             }
             const int a = a0 + lo;
-            const int e = p.n2e[ibase + i];
-            const int ci = p.cut_of_elem[e];
+            const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
             int nodes[NEN];
             int la = -1;
             for (int n = 0; n < NEN; ++n) {
-                nodes[n] = p.conn[int64_t(e) * NEN + n];
+                nodes[n] = rec[n];
                 s_nodes[i][n] = nodes[n];
                 if (nodes[n] == a) la = n;
             }
+            const int tag = rec[NEN];
+            const int ci = tag >= 0 ? tag : -1;
             s_ci[i] = ci;
             if (ci < 0) {
                 double x[NEN][3];
@@ -826,7 +832,7 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
-                const int mi = p.elem_mat[e];
+                const int mi = -1 - tag;
                 if (la >= 0) {
                     if (jac) left_factor<D, TC>(gt[la], sD[mi], TC(w), s_L[i]);
                     if (func) plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, s_P[i]);
@@ -859,7 +865,8 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
             }
         }
         const int a = a0 + lo;
-        const int b = p.npat_col[q];
+        const bool first = (q == qf);
+        const int b = first ? b_pre : p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -935,7 +942,7 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         if (jac) {
             const int32_t* mp = p.npat_map + int64_t(q) * D * D;
             for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+                for (int j = 0; j < D; ++j) K[first ? mp_pre[i * D + j] : mp[i * D + j]] = acc[i][j];
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
@@ -965,6 +972,8 @@ struct pg_asm {
     float* Kc32 = nullptr;       // cut tangents (FP32)
     double* Pc = nullptr;        // cut residuals (FP64)
     int* err = nullptr;
+    int32_t* class_list = nullptr;  // cut indices: small class, then general class
+    int n_class[2] = {0, 0};
     // host-API buffers
     double *dU = nullptr, *dK = nullptr, *dP = nullptr, *dIc = nullptr, *dIt = nullptr;
     hipStream_t stream = nullptr;
@@ -973,6 +982,8 @@ struct pg_asm {
     double last_ms[3] = {0, 0, 0};
     int pair_block = 256;
     int elem_block = 256;
+    int group_nodes = 0;   // node-group size of the binned schedule (0: default)
+    int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
     int64_t alg_bytes = 0;
 };
 
@@ -1090,26 +1101,56 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
         else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
     } else {
-        if (d.n_cut > 0) {
-            if (d.dim == 2) {
-                if (fp32) k_cut<2, float, 4><<<(d.n_cut + 3) / 4, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<2, double, 4><<<(d.n_cut + 3) / 4, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
-            } else {
-                if (fp32) k_cut<3, float, 1><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<3, double, 1><<<d.n_cut, 64, 0, st>>>(d, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+        // cut elements, binned by class: small single-inclusion records share a
+        // wavefront (EPW per wave); the general class keeps the full limits
+        const int n0 = h->n_class[0], n1 = h->n_class[1];
+        const int32_t* l0 = h->class_list;
+        const int32_t* l1 = h->class_list + n0;
+        const int cv = h->cut_variant;
+#define CUT_SMALL2(TK, KST, EPW, OCC) \
+    k_cut<2, TK, EPW, 7, 8, 2, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
+        if (d.dim == 2) {
+            if (n0) {
+                if (fp32) {
+                    if (cv == 1) CUT_SMALL2(float, h->Kc32, 4, 3);
+                    else if (cv == 2) CUT_SMALL2(float, h->Kc32, 2, 1);
+                    else if (cv == 3) CUT_SMALL2(float, h->Kc32, 8, 1);
+                    else CUT_SMALL2(float, h->Kc32, 4, 1);
+                } else {
+                    if (cv == 1) CUT_SMALL2(double, h->Kc64, 4, 3);
+                    else if (cv == 2) CUT_SMALL2(double, h->Kc64, 2, 1);
+                    else if (cv == 3) CUT_SMALL2(double, h->Kc64, 8, 1);
+                    else CUT_SMALL2(double, h->Kc64, 4, 1);
+                }
+            }
+            if (n1) {
+                if (fp32) k_cut<2, float, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<2, double, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+            }
+        } else {
+            if (n0) {
+                if (fp32) k_cut<3, float, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<3, double, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+            }
+            if (n1) {
+                if (fp32) k_cut<3, float, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<3, double, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         }
+#undef CUT_SMALL2
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         const int nall = d.n_nodes + d.n_enr;
+        const int ng = h->group_nodes;
+#define GROUP(DD, TC, KC, NGV) \
+    k_pair_group<DD, TC, NGV><<<unsigned((nall + NGV - 1) / NGV), 256, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
         if (d.dim == 2) {
-            const unsigned gg = unsigned((nall + GroupCfg<2>::NG - 1) / GroupCfg<2>::NG);
-            if (fp32) k_pair_group<2, float><<<gg, 256, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
-            else k_pair_group<2, double><<<gg, 256, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+            if (fp32) { if (ng == 16) GROUP(2, float, h->Kc32, 16); else GROUP(2, float, h->Kc32, 32); }
+            else { if (ng == 16) GROUP(2, double, h->Kc64, 16); else GROUP(2, double, h->Kc64, 32); }
         } else {
-            const unsigned gg = unsigned((nall + GroupCfg<3>::NG - 1) / GroupCfg<3>::NG);
-            if (fp32) k_pair_group<3, float><<<gg, 256, 0, st>>>(d, h->Un, h->Kc32, h->Pc, dK, dP, flags);
-            else k_pair_group<3, double><<<gg, 256, 0, st>>>(d, h->Un, h->Kc64, h->Pc, dK, dP, flags);
+            if (fp32) { if (ng == 4) GROUP(3, float, h->Kc32, 4); else GROUP(3, float, h->Kc32, 8); }
+            else { if (ng == 4) GROUP(3, double, h->Kc64, 4); else GROUP(3, double, h->Kc64, 8); }
         }
+#undef GROUP
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1153,6 +1194,17 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         eoff[e + 1] = eoff[e] + nd * nd + nd;
     }
     d.n_mat = s->n_mat;
+    // incidence records in n2e order: element nodes, then the cut index (>= 0)
+    // or -1 - material of an uncut element
+    const int64_t n_inc = s->n2e_ptr[nall];
+    std::vector<int32_t> incv(size_t(n_inc) * (s->nen + 1));
+    for (int64_t t = 0; t < n_inc; ++t) {
+        const int e = s->n2e[t];
+        int32_t* r = &incv[size_t(t) * (s->nen + 1)];
+        for (int n = 0; n < s->nen; ++n) r[n] = s->conn[int64_t(e) * s->nen + n];
+        const int c = s->cut_of_elem[e];
+        r[s->nen] = c >= 0 ? c : -1 - s->elem_mat[e];
+    }
     auto fail = [&](int code) { pg_asm_destroy(h); return code; };
 #define UP(field, src, n, T)                                   \
     do {                                                       \
@@ -1179,7 +1231,29 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
     UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
+    UP(inc, incv.data(), incv.size(), int32_t);
 #undef UP
+    {
+        std::vector<int32_t> small, general;
+        for (int c = 0; c < s->n_cut; ++c) {
+            const pg_cut_rec& r = s->cut[c];
+            const int nloc = s->nen + r.n_enr;
+            const int ncq = r.n_coh * (r.coh_nn == 4 ? 2 : 4);
+            const bool is_small = s->dim == 2 ? (nloc <= 7 && r.n_vqp <= 8 && ncq <= 2)
+                                              : (nloc <= 8 && r.n_vqp <= 30 && ncq <= 4);
+            (is_small ? small : general).push_back(c);
+        }
+        h->n_class[0] = int(small.size());
+        h->n_class[1] = int(general.size());
+        small.insert(small.end(), general.begin(), general.end());
+        int32_t* tmp_;
+        if ((rc = alloc(h, &tmp_, small.size()))) return fail(rc);
+        if (!small.empty() && hipMemcpy(tmp_, small.data(), small.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            g_err = "hipMemcpy of cut classes failed";
+            return fail(PG_E_HIP);
+        }
+        h->class_list = tmp_;
+    }
     const int ndp = s->nen * s->dim;
     if ((rc = alloc(h, &h->Un, size_t(nall * s->dim)))) return fail(rc);
     (void)ndp;
@@ -1229,12 +1303,14 @@ int pg_asm_set_timing(pg_asm* h, int enable) {
     return PG_OK;
 }
 
-int pg_asm_set_tiles(pg_asm* h, int pair_block, int cut_block) {
+int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
-    auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 1024 && b % 64 == 0); };
-    if (!ok(pair_block) || !ok(cut_block)) { g_err = "block sizes must be multiples of 64 in [64, 1024]"; return PG_E_ARG; }
-    if (pair_block) h->pair_block = pair_block;
-    if (cut_block) h->elem_block = cut_block;
+    const bool ok_g = group_nodes == 0 || (h->d.dim == 2 ? (group_nodes == 16 || group_nodes == 32)
+                                                        : (group_nodes == 4 || group_nodes == 8));
+    if (!ok_g) { g_err = "node-group size must be 16 or 32 (2D), 4 or 8 (3D)"; return PG_E_ARG; }
+    if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
+    h->group_nodes = group_nodes;
+    h->cut_variant = cut_variant;
     return PG_OK;
 }
 
