@@ -764,9 +764,10 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     __shared__ int s_qptr[NG + 1];
     __shared__ unsigned char s_own[MAXP];   // owner node (offset in group) of each pair
     static_assert(NG <= 256, "owner offsets are bytes");
-    __shared__ int s_ci[MAXI];              // cut index, or -1
-    __shared__ int s_la[MAXI];
-    __shared__ int s_nodes[MAXI][NEN];
+    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes staged per incidence
+    __shared__ int s_info[MAXI];            // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: not staged
+    __shared__ int s_nodes[MAXI][MAXLOC];   // element-local node list (parents, enrichment)
+    __shared__ int64_t s_koff[MAXI];        // cut: offset of the staged tangent
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_L[MAXI][D * NS];
     __shared__ double s_P[MAXI][D];
@@ -820,9 +821,8 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 if (nodes[n] == a) la = n;
             }
             const int tag = rec[NEN];
-            const int ci = tag >= 0 ? tag : -1;
-            s_ci[i] = ci;
-            if (ci < 0) {
+            int nloc = NEN;
+            if (tag < 0) {
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
@@ -838,15 +838,25 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                     if (func) plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, s_P[i]);
                 }
             } else {
-                if (la < 0) {
-                    const pg_cut_rec& r = p.cut[ci];
-                    for (int k = 0; k < r.n_enr; ++k)
-                        if (r.enr[k] == a) la = NEN + k;
+                const int ci = tag;
+                const pg_cut_rec& r = p.cut[ci];
+                const int ne = r.n_enr;
+                nloc = NEN + ne;
+                for (int k = 0; k < ne && NEN + k < MAXLOC; ++k) {
+                    const int en = r.enr[k];
+                    s_nodes[i][NEN + k] = en;
+                    if (en == a) la = NEN + k;
                 }
+                if (la < 0)
+                    for (int k = 0; k < ne; ++k)
+                        if (r.enr[k] == a) la = NEN + k;
+                s_koff[i] = p.cut_koff[ci];
                 if (func && la >= 0)
                     for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
             }
-            s_la[i] = la;
+            // incidences whose local list does not fit are resolved from global memory
+            const bool fits = nloc <= MAXLOC;
+            s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0) | (fits ? 0 : (1 << 17));
         }
     }
     __syncthreads();
@@ -876,22 +886,31 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
         for (int t = t0; t < t1; ++t) {
             if (staged) {
-                const int ci = s_ci[t];
-                const int la = s_la[t];
-                if (la < 0) continue;
-                if (ci < 0) {
+                const int info = s_info[t];
+                const int la = info & 255;
+                if (la == 255) continue;
+                const int nloc = (info >> 8) & 255;
+                if (!(info & (1 << 16))) {
                     int lb = -1;
                     for (int n = 0; n < NEN; ++n)
                         if (s_nodes[t][n] == b) lb = n;
                     if (lb < 0) continue;
                     if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
                 } else {
-                    const int e = p.n2e[ibase + t];
-                    const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
+                    int lb = -1;
+                    if (a == b) {
+                        lb = la;
+                    } else if (!(info & (1 << 17))) {
+                        for (int n = 0; n < MAXLOC; ++n)
+                            if (n < nloc && s_nodes[t][n] == b) lb = n;
+                    } else {
+                        const int e = p.n2e[ibase + t];
+                        lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
+                    }
                     if (lb < 0) continue;
                     if (jac) {
-                        const int nd = (NEN + p.cut[ci].n_enr) * D;
-                        const TC* Ke = Kc + p.cut_koff[ci];
+                        const int nd = nloc * D;
+                        const TC* Ke = Kc + s_koff[t];
                         for (int i = 0; i < D; ++i)
                             for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
                     }
