@@ -200,8 +200,8 @@ int pg_asm_last_timing(const pg_asm* h, double kernel_ms[3]);
 int pg_asm_set_timing(pg_asm* h, int enable);
 
 /* Schedule parameters of the class-binned schedules, chosen by the autotuner:
- *   group_nodes  nodes per block of the node-group kernel: 16 or 32 in 2D,
- *                4 or 8 in 3D (0: default 32 / 8);
+ *   group_nodes  nodes per block of the node-group kernel: 8, 16 or 32 in 2D,
+ *                2, 4 or 8 in 3D (0: default 32 / 8);
  *   cut_variant  small-class cut kernel: 0 = 4 elements per wavefront,
  *                1 = 4 per wavefront with a 3-waves/SIMD register cap,
  *                2 = 2 per wavefront, 3 = 8 per wavefront. */

@@ -17,7 +17,7 @@ import numpy as np
 from .device import DeviceBuffer, synchronize
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
-GROUPS = {2: (16, 32), 3: (4, 8)}
+GROUPS = {2: (8, 16, 32), 3: (2, 4, 8)}
 CUT_VARIANTS = (0, 1, 2, 3)
 
 

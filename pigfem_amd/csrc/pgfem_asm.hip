@@ -768,6 +768,10 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     __shared__ int s_info[MAXI];            // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: not staged
     __shared__ int s_nodes[MAXI][MAXLOC];   // element-local node list (parents, enrichment)
     __shared__ int64_t s_koff[MAXI];        // cut: offset of the staged tangent
+    constexpr int POOL = (D == 2 ? 3 : 8) * NG, RW = MAXLOC * D;
+    __shared__ TC s_krow[POOL][D * RW];     // cut: the owner's rows of the staged tangent
+    __shared__ short s_kslot[MAXI];         // pool slot of a cut incidence, -1: read from global
+    __shared__ int s_npool;
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_L[MAXI][D * NS];
     __shared__ double s_P[MAXI][D];
@@ -785,6 +789,7 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
         s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
     }
+    if (threadIdx.x == 0) s_npool = 0;
     __syncthreads();
     const int ibase = s_iptr[0];
     const int ni = s_iptr[ng] - ibase;
@@ -853,6 +858,19 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 s_koff[i] = p.cut_koff[ci];
                 if (func && la >= 0)
                     for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
+                int slot = -1;
+                if (jac && la >= 0 && nloc <= MAXLOC) {
+                    slot = atomicAdd(&s_npool, 1);  // slot order is irrelevant to the sums
+                    if (slot < POOL) {
+                        const int nd = nloc * D;
+                        const TC* src = Kc + p.cut_koff[ci] + int64_t(la * D) * nd;
+                        for (int r = 0; r < D; ++r)
+                            for (int j = 0; j < nd; ++j) s_krow[slot][r * RW + j] = src[r * nd + j];
+                    } else {
+                        slot = -1;
+                    }
+                }
+                s_kslot[i] = short(slot);
             }
             // incidences whose local list does not fit are resolved from global memory
             const bool fits = nloc <= MAXLOC;
@@ -909,10 +927,16 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                     }
                     if (lb < 0) continue;
                     if (jac) {
-                        const int nd = nloc * D;
-                        const TC* Ke = Kc + s_koff[t];
-                        for (int i = 0; i < D; ++i)
-                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
+                        const int slot = s_kslot[t];
+                        if (slot >= 0) {
+                            for (int i = 0; i < D; ++i)
+                                for (int j = 0; j < D; ++j) acc[i][j] += double(s_krow[slot][i * RW + lb * D + j]);
+                        } else {
+                            const int nd = nloc * D;
+                            const TC* Ke = Kc + s_koff[t];
+                            for (int i = 0; i < D; ++i)
+                                for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
+                        }
                     }
                 }
                 if (diag)
@@ -1163,11 +1187,25 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
 #define GROUP(DD, TC, KC, NGV) \
     k_pair_group<DD, TC, NGV><<<unsigned((nall + NGV - 1) / NGV), 256, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
         if (d.dim == 2) {
-            if (fp32) { if (ng == 16) GROUP(2, float, h->Kc32, 16); else GROUP(2, float, h->Kc32, 32); }
-            else { if (ng == 16) GROUP(2, double, h->Kc64, 16); else GROUP(2, double, h->Kc64, 32); }
+            if (fp32) {
+                if (ng == 8) GROUP(2, float, h->Kc32, 8);
+                else if (ng == 16) GROUP(2, float, h->Kc32, 16);
+                else GROUP(2, float, h->Kc32, 32);
+            } else {
+                if (ng == 8) GROUP(2, double, h->Kc64, 8);
+                else if (ng == 16) GROUP(2, double, h->Kc64, 16);
+                else GROUP(2, double, h->Kc64, 32);
+            }
         } else {
-            if (fp32) { if (ng == 4) GROUP(3, float, h->Kc32, 4); else GROUP(3, float, h->Kc32, 8); }
-            else { if (ng == 4) GROUP(3, double, h->Kc64, 4); else GROUP(3, double, h->Kc64, 8); }
+            if (fp32) {
+                if (ng == 2) GROUP(3, float, h->Kc32, 2);
+                else if (ng == 4) GROUP(3, float, h->Kc32, 4);
+                else GROUP(3, float, h->Kc32, 8);
+            } else {
+                if (ng == 2) GROUP(3, double, h->Kc64, 2);
+                else if (ng == 4) GROUP(3, double, h->Kc64, 4);
+                else GROUP(3, double, h->Kc64, 8);
+            }
         }
 #undef GROUP
     }
@@ -1324,9 +1362,9 @@ int pg_asm_set_timing(pg_asm* h, int enable) {
 
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
-    const bool ok_g = group_nodes == 0 || (h->d.dim == 2 ? (group_nodes == 16 || group_nodes == 32)
-                                                        : (group_nodes == 4 || group_nodes == 8));
-    if (!ok_g) { g_err = "node-group size must be 16 or 32 (2D), 4 or 8 (3D)"; return PG_E_ARG; }
+    const bool ok_g = group_nodes == 0 || (h->d.dim == 2 ? (group_nodes == 8 || group_nodes == 16 || group_nodes == 32)
+                                                        : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
+    if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;

@@ -116,7 +116,7 @@ def test_every_schedule_variant(kind, n, ni):
     U = p.random_u(9)
     ic = np.zeros(p.n_cqp)
     Kr, Pr, _ = o.assemble(U, ic)
-    groups = (16, 32) if p.dim == 2 else (4, 8)
+    groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
     for mode in ("binned64", "mixed"):
         ref = None
         for g in groups:
