@@ -768,10 +768,6 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
     __shared__ int s_info[MAXI];            // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: not staged
     __shared__ int s_nodes[MAXI][MAXLOC];   // element-local node list (parents, enrichment)
     __shared__ int64_t s_koff[MAXI];        // cut: offset of the staged tangent
-    constexpr int POOL = (D == 2 ? 3 : 8) * NG, RW = MAXLOC * D;
-    __shared__ TC s_krow[POOL][D * RW];     // cut: the owner's rows of the staged tangent
-    __shared__ short s_kslot[MAXI];         // pool slot of a cut incidence, -1: read from global
-    __shared__ int s_npool;
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_L[MAXI][D * NS];
     __shared__ double s_P[MAXI][D];
@@ -789,7 +785,6 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
         s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
     }
-    if (threadIdx.x == 0) s_npool = 0;
     __syncthreads();
     const int ibase = s_iptr[0];
     const int ni = s_iptr[ng] - ibase;
@@ -858,19 +853,6 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 s_koff[i] = p.cut_koff[ci];
                 if (func && la >= 0)
                     for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
-                int slot = -1;
-                if (jac && la >= 0 && nloc <= MAXLOC) {
-                    slot = atomicAdd(&s_npool, 1);  // slot order is irrelevant to the sums
-                    if (slot < POOL) {
-                        const int nd = nloc * D;
-                        const TC* src = Kc + p.cut_koff[ci] + int64_t(la * D) * nd;
-                        for (int r = 0; r < D; ++r)
-                            for (int j = 0; j < nd; ++j) s_krow[slot][r * RW + j] = src[r * nd + j];
-                    } else {
-                        slot = -1;
-                    }
-                }
-                s_kslot[i] = short(slot);
             }
             // incidences whose local list does not fit are resolved from global memory
             const bool fits = nloc <= MAXLOC;
@@ -902,46 +884,56 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         double pa[D];
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
         const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
-        for (int t = t0; t < t1; ++t) {
-            if (staged) {
-                const int info = s_info[t];
-                const int la = info & 255;
-                if (la == 255) continue;
-                const int nloc = (info >> 8) & 255;
-                if (!(info & (1 << 16))) {
-                    int lb = -1;
-                    for (int n = 0; n < NEN; ++n)
-                        if (s_nodes[t][n] == b) lb = n;
-                    if (lb < 0) continue;
-                    if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
-                } else {
-                    int lb = -1;
-                    if (a == b) {
-                        lb = la;
-                    } else if (!(info & (1 << 17))) {
-                        for (int n = 0; n < MAXLOC; ++n)
-                            if (n < nloc && s_nodes[t][n] == b) lb = n;
+        if (staged) {
+            // incidences in unrolled batches: the LDS reads of a batch issue back to
+            // back; contributions are added in ascending incidence (element) order
+            constexpr int BATCH = 4;
+            for (int tb = t0; tb < t1; tb += BATCH) {
+                int info[BATCH], nodes[BATCH][NEN];
+#pragma unroll
+                for (int k = 0; k < BATCH; ++k) {
+                    const int t = min(tb + k, t1 - 1);
+                    info[k] = (tb + k < t1) ? s_info[t] : 255;
+                    for (int n = 0; n < NEN; ++n) nodes[k][n] = s_nodes[t][n];
+                }
+#pragma unroll
+                for (int k = 0; k < BATCH; ++k) {
+                    const int t = tb + k;
+                    const int la = info[k] & 255;
+                    if (la == 255) continue;
+                    if (!(info[k] & (1 << 16))) {
+                        int lb = -1;
+#pragma unroll
+                        for (int n = 0; n < NEN; ++n)
+                            if (nodes[k][n] == b) lb = n;
+                        if (lb < 0) continue;
+                        if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
                     } else {
-                        const int e = p.n2e[ibase + t];
-                        lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
-                    }
-                    if (lb < 0) continue;
-                    if (jac) {
-                        const int slot = s_kslot[t];
-                        if (slot >= 0) {
-                            for (int i = 0; i < D; ++i)
-                                for (int j = 0; j < D; ++j) acc[i][j] += double(s_krow[slot][i * RW + lb * D + j]);
+                        const int nloc = (info[k] >> 8) & 255;
+                        int lb = -1;
+                        if (a == b) {
+                            lb = la;
+                        } else if (!(info[k] & (1 << 17))) {
+                            for (int n = 0; n < MAXLOC; ++n)
+                                if (n < nloc && s_nodes[t][n] == b) lb = n;
                         } else {
+                            const int e = p.n2e[ibase + t];
+                            lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
+                        }
+                        if (lb < 0) continue;
+                        if (jac) {
                             const int nd = nloc * D;
                             const TC* Ke = Kc + s_koff[t];
                             for (int i = 0; i < D; ++i)
                                 for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
                         }
                     }
+                    if (diag)
+                        for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
                 }
-                if (diag)
-                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-            } else {
+            }
+        } else {
+            for (int t = t0; t < t1; ++t) {
                 // same arithmetic from global memory (incidence tables exceed LDS)
                 const int e = p.n2e[ibase + t];
                 const int ci = p.cut_of_elem[e];
