@@ -885,8 +885,45 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
         const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
         if (staged) {
-            // incidences in unrolled batches: the LDS reads of a batch issue back to
-            // back; contributions are added in ascending incidence (element) order
+            // (1) cut incidences: find b in the staged local lists and issue the
+            //     loads of the staged tangent entries right away (up to MAXCUT);
+            // (2) uncut incidences from LDS while those loads are in flight;
+            // (3) add the cut contributions.  Deterministic order: uncut elements in
+            //     ascending order, then cut elements in ascending order.
+            constexpr int MAXCUT = 4;
+            TC kc[MAXCUT][D][D];
+            int ncut = 0;
+            bool overflow = false;
+            for (int t = t0; t < t1; ++t) {
+                const int info = s_info[t];
+                const int la = info & 255;
+                if (la == 255 || !(info & (1 << 16))) continue;
+                const int nloc = (info >> 8) & 255;
+                int lb = -1;
+                if (a == b) lb = la;
+                else if (!(info & (1 << 17))) {
+                    for (int n = 0; n < MAXLOC; ++n)
+                        if (n < nloc && s_nodes[t][n] == b) lb = n;
+                } else {
+                    const int e = p.n2e[ibase + t];
+                    lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
+                }
+                if (lb < 0) continue;
+                if (jac) {
+                    const int nd = nloc * D;
+                    const TC* Ke = Kc + s_koff[t] + int64_t(la * D) * nd + lb * D;
+                    if (ncut < MAXCUT) {
+#pragma unroll
+                        for (int s = 0; s < MAXCUT; ++s)
+                            if (s == ncut)
+                                for (int i = 0; i < D; ++i)
+                                    for (int j = 0; j < D; ++j) kc[s][i][j] = Ke[int64_t(i) * nd + j];
+                        ++ncut;
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
             constexpr int BATCH = 4;
             for (int tb = t0; tb < t1; tb += BATCH) {
                 int info[BATCH], nodes[BATCH][NEN];
@@ -901,35 +938,37 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                     const int t = tb + k;
                     const int la = info[k] & 255;
                     if (la == 255) continue;
-                    if (!(info[k] & (1 << 16))) {
-                        int lb = -1;
-#pragma unroll
-                        for (int n = 0; n < NEN; ++n)
-                            if (nodes[k][n] == b) lb = n;
-                        if (lb < 0) continue;
-                        if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
-                    } else {
-                        const int nloc = (info[k] >> 8) & 255;
-                        int lb = -1;
-                        if (a == b) {
-                            lb = la;
-                        } else if (!(info[k] & (1 << 17))) {
-                            for (int n = 0; n < MAXLOC; ++n)
-                                if (n < nloc && s_nodes[t][n] == b) lb = n;
-                        } else {
-                            const int e = p.n2e[ibase + t];
-                            lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
-                        }
-                        if (lb < 0) continue;
-                        if (jac) {
-                            const int nd = nloc * D;
-                            const TC* Ke = Kc + s_koff[t];
-                            for (int i = 0; i < D; ++i)
-                                for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
-                        }
-                    }
                     if (diag)
                         for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
+                    if (info[k] & (1 << 16)) continue;
+                    int lb = -1;
+#pragma unroll
+                    for (int n = 0; n < NEN; ++n)
+                        if (nodes[k][n] == b) lb = n;
+                    if (lb < 0) continue;
+                    if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < MAXCUT; ++s)
+                if (s < ncut)
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(kc[s][i][j]);
+            if (overflow) {  // more than MAXCUT cut elements share this pair
+                int seen = 0;
+                for (int t = t0; t < t1; ++t) {
+                    const int info = s_info[t];
+                    const int la = info & 255;
+                    if (la == 255 || !(info & (1 << 16))) continue;
+                    const int nloc = (info >> 8) & 255;
+                    const int e = p.n2e[ibase + t];
+                    const int lb = (a == b) ? la : find_local(p, e, p.cut_of_elem[e], b, NEN);
+                    if (lb < 0) continue;
+                    if (++seen <= MAXCUT) continue;
+                    const int nd = nloc * D;
+                    const TC* Ke = Kc + s_koff[t] + int64_t(la * D) * nd + lb * D;
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(i) * nd + j]);
                 }
             }
         } else {
