@@ -760,18 +760,22 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
+    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes staged per incidence
+    constexpr int CAP = (D == 2 ? 8 : 32);    // contributions per pair slot
+    static_assert(MAXI <= 4096 && MAXLOC <= 16, "contribution codes are (incidence << 4 | local node)");
     __shared__ int s_iptr[NG + 1];
     __shared__ int s_qptr[NG + 1];
-    __shared__ unsigned char s_own[MAXP];   // owner node (offset in group) of each pair
-    static_assert(NG <= 256, "owner offsets are bytes");
-    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes staged per incidence
-    __shared__ int s_info[MAXI];            // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: not staged
-    __shared__ int s_nodes[MAXI][MAXLOC];   // element-local node list (parents, enrichment)
-    __shared__ int64_t s_koff[MAXI];        // cut: offset of the staged tangent
+    __shared__ int s_pcol[MAXP];             // column node of each pair of the group
+    __shared__ int s_cnt[MAXP];              // contributions collected per pair
+    __shared__ unsigned short s_list[MAXP][CAP];
+    __shared__ int s_info[MAXI];             // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: absent
+    __shared__ int s_nodes[MAXI][MAXLOC];
+    __shared__ int64_t s_koff[MAXI];
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_L[MAXI][D * NS];
     __shared__ double s_P[MAXI][D];
     __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+    __shared__ int s_bad;
 
     const int nall = p.n_nodes + p.n_enr;
     const int a0 = blockIdx.x * NG;
@@ -785,26 +789,22 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
         s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
     }
+    if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
     const int ibase = s_iptr[0];
     const int ni = s_iptr[ng] - ibase;
     const int q0 = s_qptr[0], q1 = s_qptr[ng];
     const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
-    // issue the first pass's pair loads now; they complete under phase A
-    const int qf = q0 + threadIdx.x;
-    int b_pre = -1;
-    int32_t mp_pre[D * D];
-    if (qf < q1) {
-        b_pre = p.npat_col[qf];
-        for (int k = 0; k < D * D; ++k) mp_pre[k] = p.npat_map[int64_t(qf) * D * D + k];
-    }
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
-    // ---- phase A: pair owners and incidences ----
+    // ---- phase A: pair columns and incidences ----
     if (staged) {
-        if (threadIdx.x < ng)
-            for (int q = s_qptr[threadIdx.x]; q < s_qptr[threadIdx.x + 1]; ++q) s_own[q - q0] = threadIdx.x;
+        for (int s = threadIdx.x; s < q1 - q0; s += blockDim.x) {
+            s_pcol[s] = p.npat_col[q0 + s];
+            s_cnt[s] = 0;
+        }
+        __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             int lo = 0, hi = ng;  // owner node of incidence i
             while (hi - lo > 1) {
@@ -813,19 +813,19 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
             }
             const int a = a0 + lo;
             const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
-            int nodes[NEN];
+            int ln[MAXLOC];
             int la = -1;
             for (int n = 0; n < NEN; ++n) {
-                nodes[n] = rec[n];
-                s_nodes[i][n] = nodes[n];
-                if (nodes[n] == a) la = n;
+                ln[n] = rec[n];
+                s_nodes[i][n] = ln[n];
+                if (ln[n] == a) la = n;
             }
             const int tag = rec[NEN];
             int nloc = NEN;
             if (tag < 0) {
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
+                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
                 double g[NEN][D];
                 const double J = simplex_grads<D, double>(x, g);
                 const double w = J * wsum;
@@ -835,145 +835,98 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
                 const int mi = -1 - tag;
                 if (la >= 0) {
                     if (jac) left_factor<D, TC>(gt[la], sD[mi], TC(w), s_L[i]);
-                    if (func) plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, s_P[i]);
+                    if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
                 }
             } else {
                 const int ci = tag;
                 const pg_cut_rec& r = p.cut[ci];
                 const int ne = r.n_enr;
                 nloc = NEN + ne;
-                for (int k = 0; k < ne && NEN + k < MAXLOC; ++k) {
-                    const int en = r.enr[k];
-                    s_nodes[i][NEN + k] = en;
-                    if (en == a) la = NEN + k;
+                if (nloc > MAXLOC) {
+                    s_bad = 1;  // local list does not fit: this group takes the global path
+                } else {
+                    for (int k = 0; k < ne; ++k) {
+                        ln[NEN + k] = r.enr[k];
+                        s_nodes[i][NEN + k] = ln[NEN + k];
+                        if (ln[NEN + k] == a) la = NEN + k;
+                    }
+                    s_koff[i] = p.cut_koff[ci];
+                    if (func && la >= 0)
+                        for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
                 }
-                if (la < 0)
-                    for (int k = 0; k < ne; ++k)
-                        if (r.enr[k] == a) la = NEN + k;
-                s_koff[i] = p.cut_koff[ci];
-                if (func && la >= 0)
-                    for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
             }
-            // incidences whose local list does not fit are resolved from global memory
-            const bool fits = nloc <= MAXLOC;
-            s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0) | (fits ? 0 : (1 << 17));
+            s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
+            if (la < 0 || nloc > MAXLOC) continue;
+            // contribution of this incidence to each pair (a, ln[m]) of a's row
+            const int ps0 = s_qptr[lo] - q0, ps1 = s_qptr[lo + 1] - q0;
+            for (int m = 0; m < nloc; ++m) {
+                int l2 = ps0, h2 = ps1;  // first slot with column >= ln[m]
+                while (l2 < h2) {
+                    const int mid = (l2 + h2) >> 1;
+                    if (s_pcol[mid] < ln[m]) l2 = mid + 1; else h2 = mid;
+                }
+                if (l2 >= ps1 || s_pcol[l2] != ln[m]) { s_bad = 1; continue; }
+                const int k = atomicAdd(&s_cnt[l2], 1);
+                if (k < CAP) s_list[l2][k] = (unsigned short)((i << 4) | m);
+                else s_bad = 1;
+            }
         }
     }
     __syncthreads();
+    const bool use_lists = staged && !s_bad;
 
     // ---- phase B: node pairs ----
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        int lo;
-        if (staged) {
-            lo = s_own[q - q0];
-        } else {
-            lo = 0;
-            int hi = ng;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_qptr[mid] <= q) lo = mid; else hi = mid;
-            }
+        int lo = 0, hi = ng;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
         }
         const int a = a0 + lo;
-        const bool first = (q == qf);
-        const int b = first ? b_pre : p.npat_col[q];
+        const int b = staged ? s_pcol[q - q0] : p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
         double pa[D];
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
-        const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
-        if (staged) {
-            // (1) cut incidences: find b in the staged local lists and issue the
-            //     loads of the staged tangent entries right away (up to MAXCUT);
-            // (2) uncut incidences from LDS while those loads are in flight;
-            // (3) add the cut contributions.  Deterministic order: uncut elements in
-            //     ascending order, then cut elements in ascending order.
-            constexpr int MAXCUT = 4;
-            TC kc[MAXCUT][D][D];
-            int ncut = 0;
-            bool overflow = false;
-            for (int t = t0; t < t1; ++t) {
+        if (use_lists) {
+            // the slot's contributions, sorted by incidence = ascending element order
+            const int s = q - q0;
+            const int cnt = s_cnt[s];
+            unsigned code[CAP];
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) code[k] = k < cnt ? s_list[s][k] : 0xffffu;
+#pragma unroll
+            for (int k = 1; k < CAP; ++k)
+#pragma unroll
+                for (int j = k; j > 0; --j) {
+                    const unsigned lo2 = min(code[j - 1], code[j]), hi2 = max(code[j - 1], code[j]);
+                    code[j - 1] = lo2;
+                    code[j] = hi2;
+                }
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) {
+                if (k >= cnt) break;
+                const int t = int(code[k] >> 4), lb = int(code[k] & 15);
                 const int info = s_info[t];
                 const int la = info & 255;
-                if (la == 255 || !(info & (1 << 16))) continue;
-                const int nloc = (info >> 8) & 255;
-                int lb = -1;
-                if (a == b) lb = la;
-                else if (!(info & (1 << 17))) {
-                    for (int n = 0; n < MAXLOC; ++n)
-                        if (n < nloc && s_nodes[t][n] == b) lb = n;
+                if (diag)
+                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
+                if (!jac) continue;
+                if (!(info & (1 << 16))) {
+                    apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
                 } else {
-                    const int e = p.n2e[ibase + t];
-                    lb = find_local(p, e, p.cut_of_elem[e], b, NEN);
-                }
-                if (lb < 0) continue;
-                if (jac) {
-                    const int nd = nloc * D;
-                    const TC* Ke = Kc + s_koff[t] + int64_t(la * D) * nd + lb * D;
-                    if (ncut < MAXCUT) {
-#pragma unroll
-                        for (int s = 0; s < MAXCUT; ++s)
-                            if (s == ncut)
-                                for (int i = 0; i < D; ++i)
-                                    for (int j = 0; j < D; ++j) kc[s][i][j] = Ke[int64_t(i) * nd + j];
-                        ++ncut;
-                    } else {
-                        overflow = true;
-                    }
-                }
-            }
-            constexpr int BATCH = 4;
-            for (int tb = t0; tb < t1; tb += BATCH) {
-                int info[BATCH], nodes[BATCH][NEN];
-#pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
-                    const int t = min(tb + k, t1 - 1);
-                    info[k] = (tb + k < t1) ? s_info[t] : 255;
-                    for (int n = 0; n < NEN; ++n) nodes[k][n] = s_nodes[t][n];
-                }
-#pragma unroll
-                for (int k = 0; k < BATCH; ++k) {
-                    const int t = tb + k;
-                    const int la = info[k] & 255;
-                    if (la == 255) continue;
-                    if (diag)
-                        for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-                    if (info[k] & (1 << 16)) continue;
-                    int lb = -1;
-#pragma unroll
-                    for (int n = 0; n < NEN; ++n)
-                        if (nodes[k][n] == b) lb = n;
-                    if (lb < 0) continue;
-                    if (jac) apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < MAXCUT; ++s)
-                if (s < ncut)
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(kc[s][i][j]);
-            if (overflow) {  // more than MAXCUT cut elements share this pair
-                int seen = 0;
-                for (int t = t0; t < t1; ++t) {
-                    const int info = s_info[t];
-                    const int la = info & 255;
-                    if (la == 255 || !(info & (1 << 16))) continue;
-                    const int nloc = (info >> 8) & 255;
-                    const int e = p.n2e[ibase + t];
-                    const int lb = (a == b) ? la : find_local(p, e, p.cut_of_elem[e], b, NEN);
-                    if (lb < 0) continue;
-                    if (++seen <= MAXCUT) continue;
-                    const int nd = nloc * D;
+                    const int nd = ((info >> 8) & 255) * D;
                     const TC* Ke = Kc + s_koff[t] + int64_t(la * D) * nd + lb * D;
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(i) * nd + j]);
                 }
             }
         } else {
+            // same arithmetic from global memory (group exceeds the LDS tables)
+            const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
             for (int t = t0; t < t1; ++t) {
-                // same arithmetic from global memory (incidence tables exceed LDS)
                 const int e = p.n2e[ibase + t];
                 const int ci = p.cut_of_elem[e];
                 const int la = find_local(p, e, ci, a, NEN);
@@ -1016,7 +969,7 @@ __global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __r
         if (jac) {
             const int32_t* mp = p.npat_map + int64_t(q) * D * D;
             for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[first ? mp_pre[i * D + j] : mp[i * D + j]] = acc[i][j];
+                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
