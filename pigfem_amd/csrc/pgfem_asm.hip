@@ -754,8 +754,13 @@ __device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, d
     }
 }
 
+template <int D, int NG> struct GroupThreads {
+    // about one lane per node pair: ~8 pairs per node in 2D, ~16 in 3D
+    static constexpr int value = (D == 2 ? 8 : 16) * NG < 64 ? 64 : ((D == 2 ? 8 : 16) * NG > 256 ? 256 : (D == 2 ? 8 : 16) * NG);
+};
+
 template <int D, typename TC, int NG>
-__global__ __launch_bounds__(256) void k_pair_group(DevProb p, const double* __restrict__ Un,
+__global__ __launch_bounds__(GroupThreads<D, NG>::value) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
@@ -1169,7 +1174,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
 #define GROUP(DD, TC, KC, NGV) \
-    k_pair_group<DD, TC, NGV><<<unsigned((nall + NGV - 1) / NGV), 256, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
+    k_pair_group<DD, TC, NGV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
         if (d.dim == 2) {
             if (fp32) {
                 if (ng == 8) GROUP(2, float, h->Kc32, 8);
