@@ -759,8 +759,8 @@ template <int D, int NG> struct GroupThreads {
     static constexpr int value = (D == 2 ? 8 : 16) * NG < 64 ? 64 : ((D == 2 ? 8 : 16) * NG > 256 ? 256 : (D == 2 ? 8 : 16) * NG);
 };
 
-template <int D, typename TC, int NG>
-__global__ __launch_bounds__(GroupThreads<D, NG>::value) void k_pair_group(DevProb p, const double* __restrict__ Un,
+template <int D, typename TC, int NG, int BS = GroupThreads<D, NG>::value>
+__global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
