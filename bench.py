@@ -172,7 +172,7 @@ def main():
                                f"U~U[-1e-3,1e-3]",
                    "elements": prob.n_elem, "cut_elements": prob.n_cut, "cohesive_qps": prob.n_cqp,
                    "dofs": prob.n_dof, "nnz": prob.nnz_total, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_pair_fused<2>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": f"k_pair_group<{prob.dim}, float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms},
         "cpu_baseline": cpu,
