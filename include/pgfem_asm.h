@@ -11,12 +11,12 @@
  *   Assembler::assemble(pmatrix& K, pvector& P_int, double delta_t,
  *       vector<vector<vector<double>>>& update_ISVs, NodalData* solution,
  *       bool assembleFunc, bool assembleJac)      include/Assembler.h:46-48,
- *                                                 src/Assembler.cpp:217-298
+ *                                                 src/Assembler.cpp:65-146
  *   Assembler::storeBmats / Mesh::store_shape_functions (set-up that the
  *       GPU path replaces by on-the-fly recomputation)
- *                                                 src/Assembler.cpp:754-773,
+ *                                                 src/Assembler.cpp:602-621,
  *                                                 src/Mesh.cpp:1475-1643
- *   Assembler::set_matrix / set_vec (the scatter) src/Assembler.cpp:811-853
+ *   Assembler::set_matrix / set_vec (the scatter) src/Assembler.cpp:659-701
  */
 #ifndef PGFEM_ASM_H
 #define PGFEM_ASM_H
@@ -39,7 +39,7 @@ extern "C" {
 
 /* ----------------------------------------------------------------------------
  * Flags of pg_asm_assemble.  PG_FUNC / PG_JAC mirror assembleFunc / assembleJac
- * (src/Assembler.cpp:217-220); both clear is a no-op, as in the reference.
+ * (src/Assembler.cpp:69-70); both clear is a no-op, as in the reference.
  * ------------------------------------------------------------------------- */
 #define PG_FUNC     1u   /* assemble the internal-force vector P_int        */
 #define PG_JAC      2u   /* assemble the tangent K (4 blocks)               */
@@ -112,7 +112,7 @@ typedef struct pg_cut_rec {
  *          (src/DofObject.cpp:241-278, 498-538).
  *   K blocks: CSR of ff (n_free x n_free), fp (n_free x n_pres),
  *          pf (n_pres x n_free), pp (n_pres x n_pres) with block-local column
- *          indices (src/Assembler.cpp:811-831).  The library stores the values
+ *          indices (src/Assembler.cpp:659-679).  The library stores the values
  *          of the four blocks back to back: [ff | fp | pf | pp].
  *   P: [Pf (n_free) | Pp (n_pres)].
  *   node-pair pattern: npat_ptr / npat_col list, per node a, the nodes b that
@@ -120,7 +120,7 @@ typedef struct pg_cut_rec {
  *          (i,j) in dim x dim, the index into the concatenated K values of entry
  *          (node_dof[a*dim+i], node_dof[b*dim+j])  -- the scatter indices.
  *   n2e_ptr / n2e: elements incident to each node, ascending (summation order
- *          of the reference element loop, src/Assembler.cpp:252).
+ *          of the reference element loop, src/Assembler.cpp:100).
  * ------------------------------------------------------------------------- */
 typedef struct pg_problem_desc {
     int32_t dim;            /* 2 or 3 */
@@ -167,22 +167,32 @@ int pg_asm_create(const pg_problem_desc* desc, pg_asm** out);
 int pg_asm_destroy(pg_asm* h);
 const char* pg_asm_last_error(void);
 
-/* Host-buffer drop-in for Assembler::assemble (src/Assembler.cpp:217-298).
+/* Host-buffer drop-in for Assembler::assemble (src/Assembler.cpp:65-146).
  *   U_all         global-dof-ordered solution, length n_free+n_pres
  *   isv_committed committed cohesive history (delta_max), length n_cqp (may be
  *                 NULL when n_cqp == 0)
  *   isv_trial     out: trial history after this assembly (reset from committed
- *                 first, as InternalVars::copy_data does, src/Assembler.cpp:244)
- *   K             out: concatenated values [ff|fp|pf|pp] (NULL unless PG_JAC)
- *   P             out: [Pf|Pp] (NULL unless PG_FUNC)
+ *                 first, as InternalVars::copy_data does, src/Assembler.cpp:92)
+ *   Kff..Kpp      out: CSR value arrays of the four blocks of pmatrix
+ *                 (include/common.h:32-73), in the order of rp/ci of the
+ *                 description -- what MatSeqAIJGetArray returns for K.ff,
+ *                 K.fp, K.pf, K.pp (NULL unless PG_JAC; a block with nnz 0
+ *                 may be NULL)
+ *   Pf, Pp        out: P_int.f and P_int.p of pvector (include/common.h:74-101),
+ *                 lengths n_free and n_pres (NULL unless PG_FUNC)
+ * Blocks are overwritten, not added to: Assembler::assemble zeroes K and P_int
+ * itself before its element loop (src/Assembler.cpp:80-89).
  * Copies host->device, assembles, copies device->host, synchronises. */
 int pg_asm_assemble(pg_asm* h, const double* U_all, const double* isv_committed,
                     double* isv_trial, double delta_t, unsigned flags,
-                    double* K, double* P);
+                    double* Kff, double* Kfp, double* Kpf, double* Kpp,
+                    double* Pf, double* Pp);
 
-/* Device-resident variant: every pointer is device memory (HBM) of the sizes
- * above; the launch is asynchronous on `stream` (a hipStream_t, or NULL for the
- * handle's own stream).  No host synchronisation, no allocation. */
+/* Device-resident variant: every pointer is device memory (HBM).  dK holds the
+ * four blocks back to back, [ff | fp | pf | pp] (nnz_total values, block b at
+ * the sum of the nnz of the blocks before it), dP holds [Pf | Pp] (n_dof
+ * values).  The launch is asynchronous on `stream` (a hipStream_t, or NULL for
+ * the handle's own stream).  No host synchronisation, no allocation. */
 int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
                            const double* disv_committed, double* disv_trial,
                            double delta_t, unsigned flags,

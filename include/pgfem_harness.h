@@ -15,7 +15,7 @@
  *   Dirichlet sets, free-first dof numbering      src/BoundaryObject.cpp:32-99,
  *                                                 src/DofObject.cpp:241-278,498-538
  *   4-block AIJ pattern (union of element blocks) src/Solver.cpp:414-490,
- *                                                 src/Assembler.cpp:811-831
+ *                                                 src/Assembler.cpp:659-679
  * It fills a pg_problem_desc (pgfem_asm.h) that the assembler consumes.
  */
 #ifndef PGFEM_HARNESS_H
@@ -64,7 +64,7 @@ int  pgh_inclusions(const pgh_problem* h, const double** centers,
 /* Nodal detection value per regular node (-1 matrix, else inclusion index). */
 const int32_t* pgh_node_detect(const pgh_problem* h);
 /* Prescribed value at full load of each prescribed dof (length n_pres); the
- * load-step value is time_fraction * value (src/Assembler.cpp:546-575). */
+ * load-step value is time_fraction * value (src/Assembler.cpp:394-423). */
 const double* pgh_bc_values(const pgh_problem* h);
 /* Per element: number of volumetric + cohesive quadrature points. */
 int  pgh_elem_qps(const pgh_problem* h, int32_t e, int32_t* n_vqp, int32_t* n_cqp);

@@ -8,7 +8,7 @@
  * What it restates (brandyb2/PIGFEM, file:line):
  *   Mesh::store_shape_functions                     src/Mesh.cpp:1475-1643
  *   Assembler::storeBmats / Assemble_Small_Strain_B_Mat
- *                                                   src/Assembler.cpp:754-773,
+ *                                                   src/Assembler.cpp:602-621,
  *                                                   src/Utilities.cpp:68-127
  *   Elem::ShapeFunctions / transform_gradients,
  *   Utilities::det / inverse                        src/elem.cpp:115-142,
@@ -18,7 +18,7 @@
  *                                                   src/tet4.cpp:87-160, src/edge2.cpp:101-126
  *   CohesiveElem::ShapeFunctions / rotation         src/CohesiveElem.cpp:145-238,
  *                                                   src/CohEdge2.cpp:55-209
- *   Assembler::assemble / assemble_elem             src/Assembler.cpp:217-520
+ *   Assembler::assemble / assemble_elem             src/Assembler.cpp:65-368
  *   KernelVolumetric (B variant) / KernelCohesive   src/AssemblerSmallStrainStructural.cpp:23-103
  *   B_U, Bt_D_B, Bt_F (CBLAS dgemv/dgemm order)     src/Utilities.cpp:491-513
  *   cohesiveDeltaFast / KFast / InternalForceFast   src/Utilities.cpp:528-594
@@ -28,7 +28,7 @@
  *                                                   src/Material_OPCohesive.cpp:38-98,
  *                                                   src/Material_OPCohesiveNU.cpp:43-111
  *   set_matrix / set_vec (MatSetValue ADD_VALUES,
- *     modelled as a binary-search add into CSR)     src/Assembler.cpp:811-853
+ *     modelled as a binary-search add into CSR)     src/Assembler.cpp:659-701
  *   AIJ pattern = union of full element dof blocks (explicit zeros inserted)
  *
  * Tet4 parents: the reference inverse-maps child qps with GSL multiroots

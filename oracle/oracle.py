@@ -1,9 +1,9 @@
 """ctypes wrapper of the CPU oracle (test infrastructure only).
 
 `Oracle(problem)` builds its own 4-block AIJ pattern by inserting every element
-dof block (src/Assembler.cpp:811-831 semantics) and caches shape data
-(src/Mesh.cpp:1475-1643, src/Assembler.cpp:754-773); `assemble` restates
-Assembler::assemble (src/Assembler.cpp:217-520) in FP64.
+dof block (src/Assembler.cpp:659-679 semantics) and caches shape data
+(src/Mesh.cpp:1475-1643, src/Assembler.cpp:602-621); `assemble` restates
+Assembler::assemble (src/Assembler.cpp:65-368) in FP64.
 """
 from __future__ import annotations
 

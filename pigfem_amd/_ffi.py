@@ -80,7 +80,7 @@ SIGNATURES = {
     "pg_asm_destroy": (C.c_int, [C.c_void_p]),
     "pg_asm_last_error": (C.c_char_p, []),
     "pg_asm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_uint,
-                                  C.c_void_p, C.c_void_p]),
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg_asm_assemble_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_uint,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg_asm_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -2,11 +2,11 @@
 
 `AssemblerGPU.assemble(K, P_int, delta_t, update_ISVs, solution, assembleFunc,
 assembleJac)` keeps the reference signature of Assembler::assemble
-(include/Assembler.h:46-48, src/Assembler.cpp:217-298):
+(include/Assembler.h:46-48, src/Assembler.cpp:65-146):
   * K is a `PMatrix` (the four AIJ blocks ff, fp, pf, pp, include/common.h:32-73),
   * P_int a `PVector` (f, p; include/common.h:74-101),
   * update_ISVs the trial internal variables, reset from the committed ones at
-    every call (src/Assembler.cpp:244) and updated by the cohesive laws,
+    every call (src/Assembler.cpp:92) and updated by the cohesive laws,
   * solution the global-dof-ordered nodal solution (NodalData),
   * both flags False is a no-op; errors raise PgError (the reference returns a
     PetscErrorCode / calls err_message).
@@ -108,10 +108,12 @@ class AssemblerGPU:
             raise ValueError(f"solution must have {p.n_dof} entries")
         if update_ISVs.shape != (p.n_cqp,) or update_ISVs.dtype != np.float64:
             raise ValueError("update_ISVs must be a float64 array of n_cqp entries")
+        Kb = [_ptr(K.block(b)) if assembleJac else None for b in range(4)]
+        Pb = [_ptr(P_int.f), _ptr(P_int.p)] if assembleFunc else [None, None]
         rc = lib().pg_asm_assemble(
             self._h, _ptr(U), _ptr(self.committed_isvs) if p.n_cqp else None,
             _ptr(update_ISVs) if p.n_cqp else None, float(delta_t), self.flags(assembleFunc, assembleJac),
-            _ptr(K.values) if assembleJac else None, _ptr(P_int.values) if assembleFunc else None)
+            *Kb, *Pb)
         check(rc, "pg_asm_assemble")
         return 0
 

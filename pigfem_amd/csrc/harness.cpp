@@ -745,7 +745,7 @@ inline int local_nodes(const pgh_problem& P, int32_t e, int32_t* out) {
 // --------------------------------------------------------------------------
 // Node incidence, node-pair pattern, 4-block CSR (union of full element dof
 // blocks: set_matrix inserts every (i,j) of every element, explicit zeros
-// included, src/Assembler.cpp:811-831) and the scatter indices.
+// included, src/Assembler.cpp:659-679) and the scatter indices.
 // --------------------------------------------------------------------------
 void build_pattern(pgh_problem& P) {
     const int d = P.dim;

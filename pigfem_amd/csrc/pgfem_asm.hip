@@ -3,7 +3,7 @@
 // Two schedules behind one C ABI (pgfem_asm.h):
 //   plain  (FP64, drop-in): one thread per parent element computes the element
 //          matrix exactly as the reference does (per-qp B^T.D.B accumulation,
-//          src/Assembler.cpp:300-520) and stages it in HBM; cut elements are
+//          src/Assembler.cpp:148-368) and stages it in HBM; cut elements are
 //          integrated by one wavefront each; a node-pair kernel then sums the
 //          staged blocks into the CSR values in element order (deterministic,
 //          no atomics) through the precomputed scatter indices.
@@ -189,7 +189,7 @@ __device__ inline void cohesive_point(const pg_material& m, int cnn, const doubl
 // --------------------------------------------------------------------------
 // Plain FP64 element kernel (the drop-in schedule, no class binning): one
 // thread per parent element, cut or not, following Assembler::assemble_elem
-// (src/Assembler.cpp:300-520) with KernelVolumetric's B-matrix variant: per qp
+// (src/Assembler.cpp:148-368) with KernelVolumetric's B-matrix variant: per qp
 // K_tmp = (B^T D) B, P_tmp = B^T sigma, K_el += (J w) K_tmp.  Uncut linear
 // simplices keep K_el in registers (B, D, sigma are the same at every qp, so each
 // entry accumulates the qp weights in reference order from one K_tmp value);
@@ -361,7 +361,7 @@ __global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const doub
 
 // --------------------------------------------------------------------------
 // Cut-element kernel of the class-binned schedule (src/Mesh.cpp:1521-1641 +
-// src/Assembler.cpp:375-517).  EPW intersected elements share one wavefront, a
+// src/Assembler.cpp:223-365).  EPW intersected elements share one wavefront, a
 // group of 64/EPW lanes each:
 //   phase 1: one lane per child qp computes the child Jacobian in FP64 (sliver
 //            children), the gradients of all local nodes, wJ and the FP64 stress;
@@ -604,10 +604,10 @@ __device__ inline int find_local(const DevProb& p, int e, int ci, int node, int 
 // --------------------------------------------------------------------------
 // Node-pair kernel of the plain schedule: thread per node pair (a, b); the d x d
 // block K[a-dofs, b-dofs] sums, over the elements incident to a (ascending: the
-// reference's element loop order, src/Assembler.cpp:252) that also contain b,
+// reference's element loop order, src/Assembler.cpp:100) that also contain b,
 // the staged element matrices, and is written once through the scatter indices
-// (set_matrix, src/Assembler.cpp:811-831; no atomics).  Diagonal pairs also sum
-// P (set_vec, src/Assembler.cpp:833-853).
+// (set_matrix, src/Assembler.cpp:659-679; no atomics).  Diagonal pairs also sum
+// P (set_vec, src/Assembler.cpp:681-701).
 // --------------------------------------------------------------------------
 template <int D>
 __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
@@ -1389,10 +1389,15 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
 }
 
 int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, double delta_t, unsigned flags,
-                    double* K, double* P) {
+                    double* Kff, double* Kfp, double* Kpf, double* Kpp, double* Pf, double* Pp) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
     if (!(flags & (PG_FUNC | PG_JAC))) return PG_OK;
-    if (!U || ((flags & PG_JAC) && !K) || ((flags & PG_FUNC) && !P)) { g_err = "missing host buffer"; return PG_E_ARG; }
+    double* Kb[4] = {Kff, Kfp, Kpf, Kpp};
+    bool missing = !U;
+    if (flags & PG_JAC)
+        for (int b = 0; b < 4; ++b) missing |= h->nnz[b] > 0 && !Kb[b];
+    if (flags & PG_FUNC) missing |= (h->d.n_free > 0 && !Pf) || (h->d.n_pres > 0 && !Pp);
+    if (missing) { g_err = "missing host buffer"; return PG_E_ARG; }
     const int64_t ncq = h->d.n_cqp;
     if (ncq > 0 && (!Ic || !It)) { g_err = "cohesive problem needs ISV buffers"; return PG_E_ARG; }
     if (!h->dU) {
@@ -1408,9 +1413,19 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
     if (ncq > 0) HIP_OK(hipMemcpyAsync(h->dIc, Ic, size_t(ncq) * sizeof(double), hipMemcpyHostToDevice, st));
     int rc = pg_asm_assemble_device(h, h->dU, h->dIc, h->dIt, delta_t, flags, h->dK, h->dP, st);
     if (rc) return rc;
-    if (flags & PG_JAC)
-        HIP_OK(hipMemcpyAsync(K, h->dK, size_t(h->nnz_total) * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (flags & PG_FUNC) HIP_OK(hipMemcpyAsync(P, h->dP, size_t(h->n_dof) * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (flags & PG_JAC) {
+        int64_t off = 0;
+        for (int b = 0; b < 4; ++b) {
+            if (h->nnz[b] > 0)
+                HIP_OK(hipMemcpyAsync(Kb[b], h->dK + off, size_t(h->nnz[b]) * sizeof(double), hipMemcpyDeviceToHost, st));
+            off += h->nnz[b];
+        }
+    }
+    if (flags & PG_FUNC) {
+        const int64_t nf = h->d.n_free, np = h->d.n_pres;
+        if (nf > 0) HIP_OK(hipMemcpyAsync(Pf, h->dP, size_t(nf) * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (np > 0) HIP_OK(hipMemcpyAsync(Pp, h->dP + nf, size_t(np) * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     if (ncq > 0) HIP_OK(hipMemcpyAsync(It, h->dIt, size_t(ncq) * sizeof(double), hipMemcpyDeviceToHost, st));
     int herr = 0;
     HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -5,7 +5,7 @@ history of the GPU assembly can be compared with the FP64 oracle's:
   SolverNonlinear::solve            src/SolverNonlinear.cpp:70-385
   SolverNonlinear::update_time_step src/SolverNonlinear.cpp:386-422
   SolverNonlinearKSP::nonlinearStep src/SolverNonlinearKSP.cpp:55-155
-  Assembler::assemble_new_load_step src/Assembler.cpp:546-663 (Dirichlet values
+  Assembler::assemble_new_load_step src/Assembler.cpp:394-511 (Dirichlet values
                                     scaled by the time fraction, no body loads)
 The linear solve of K_ff (MUMPS Cholesky through PETSc KSP in the reference,
 src/Solver.cpp:108-160) is a sparse LU (SuperLU via scipy) here.  The explicit
