@@ -154,6 +154,17 @@ def main():
                          f"best of 2 single-thread runs of the oracle restatement"}
         log(f"[bench] cpu baseline: {cpu['value']:.4g} elements/s over {m} elements")
 
+    # HBM traffic of the same kernel instantiation from the committed rocprofv3
+    # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
+    ng = tuned["mixed"][0]
+    bs = min(256, max(64, (8 if prob.dim == 2 else 16) * ng))
+    kname = f"k_pair_group<{prob.dim}, float, {ng}, {bs}>"
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith("traffic_bytes_per_launch:") and kname in line:
+                traffic = int(line.rsplit("=", 1)[1])
     out = {
         "metric": "K+R assembled elements/s, % HBM peak; FP32+batched vs FP64 at <=1e-6 rel err",
         "value": results["mixed"]["eps"],
@@ -172,8 +183,8 @@ def main():
                                f"U~U[-1e-3,1e-3]",
                    "elements": prob.n_elem, "cut_elements": prob.n_cut, "cohesive_qps": prob.n_cqp,
                    "dofs": prob.n_dof, "nnz": prob.nnz_total, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": f"k_pair_group<{prob.dim}, float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms},
         "cpu_baseline": cpu,
         "fp64_elements_per_s": results["fp64"]["eps"],
