@@ -10,7 +10,7 @@
  * Reference interface replaced (file:line in brandyb2/PIGFEM):
  *   Assembler::assemble(pmatrix& K, pvector& P_int, double delta_t,
  *       vector<vector<vector<double>>>& update_ISVs, NodalData* solution,
- *       bool assembleFunc, bool assembleJac)      include/Assembler.h:46-48,
+ *       bool assembleFunc, bool assembleJac)      include/Assembler.h:45-47,
  *                                                 src/Assembler.cpp:65-146
  *   Assembler::storeBmats / Mesh::store_shape_functions (set-up that the
  *       GPU path replaces by on-the-fly recomputation)
@@ -221,6 +221,11 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant);
  * numerator): connectivity, coordinates, U, cut records, ISVs, CSR values +
  * column indices + row pointers, R. */
 int64_t pg_asm_algorithmic_bytes(const pg_asm* h);
+
+/* Device memory (bytes) held by the handle: the static description, the class
+ * lists, the staging buffers, and the host-API buffers once pg_asm_assemble
+ * has been called. */
+int64_t pg_asm_device_bytes(const pg_asm* h);
 
 /* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the

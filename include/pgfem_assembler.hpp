@@ -10,7 +10,7 @@
  *     preallocate_internal_vars_object) src/InternalVars.cpp:43-170
  *   Assembler::assemble(K, P_int, delta_t, update_ISVs, solution,
  *                       assembleFunc, assembleJac)
- *                                     include/Assembler.h:46-48,
+ *                                     include/Assembler.h:45-47,
  *                                     src/Assembler.cpp:65-146
  *   Solver::preallocate_matrix        src/Solver.cpp:414-490
  * `assemble` returns 0 or a PG_E_* status, as the reference returns a

@@ -55,6 +55,14 @@ typedef struct pgh_problem pgh_problem;
  * message in pgh_last_error). */
 int  pgh_build_rve(const pgh_rve_params* p, pgh_problem** out);
 int  pgh_get_desc(const pgh_problem* h, pg_problem_desc* out);
+/* Pattern and scatter indices for a mesh that was set up elsewhere (an
+ * integration inside PIGFEM): reads the mesh part of `mesh` (counts, coords,
+ * conn, elem_mat, cut_of_elem, cut_elem, cut, mats, node_dof, n_free, n_pres,
+ * n_cqp, cohesive) and builds the 4-block AIJ pattern (union of full element
+ * dof blocks, as set_matrix inserts them, src/Assembler.cpp:659-679), the
+ * node-pair pattern, the scatter indices and the node incidence.  The full
+ * description is then read with pgh_get_desc; the input arrays are copied. */
+int  pgh_build_pattern(const pg_problem_desc* mesh, pgh_problem** out);
 void pgh_free(pgh_problem* h);
 const char* pgh_last_error(void);
 

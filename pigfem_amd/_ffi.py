@@ -88,6 +88,7 @@ SIGNATURES = {
     "pg_asm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pg_asm_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "pg_asm_algorithmic_bytes": (C.c_int64, [C.c_void_p]),
+    "pg_asm_device_bytes": (C.c_int64, [C.c_void_p]),
     "pg_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "pg_dev_free": (C.c_int, [C.c_void_p]),
     "pg_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
@@ -97,6 +98,7 @@ SIGNATURES = {
     # pgfem_harness.h
     "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
     "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),
+    "pgh_build_pattern": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
     "pgh_free": (None, [C.c_void_p]),
     "pgh_last_error": (C.c_char_p, []),
     "pgh_inclusions": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),

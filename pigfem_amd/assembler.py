@@ -2,7 +2,7 @@
 
 `AssemblerGPU.assemble(K, P_int, delta_t, update_ISVs, solution, assembleFunc,
 assembleJac)` keeps the reference signature of Assembler::assemble
-(include/Assembler.h:46-48, src/Assembler.cpp:65-146):
+(include/Assembler.h:45-47, src/Assembler.cpp:65-146):
   * K is a `PMatrix` (the four AIJ blocks ff, fp, pf, pp, include/common.h:32-73),
   * P_int a `PVector` (f, p; include/common.h:74-101),
   * update_ISVs the trial internal variables, reset from the committed ones at
@@ -154,6 +154,9 @@ class AssemblerGPU:
 
     def algorithmic_bytes(self) -> int:
         return int(lib().pg_asm_algorithmic_bytes(self._h))
+
+    def device_bytes(self) -> int:
+        return int(lib().pg_asm_device_bytes(self._h))
 
     def close(self):
         if self._h:

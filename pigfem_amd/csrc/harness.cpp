@@ -58,6 +58,7 @@ struct CutInfo {
 struct pgh_problem {
     pgh_rve_params prm{};
     int dim = 0, nen = 0;
+    int cohesive = 0;
     int32_t n_nodes = 0, n_enr = 0, n_elem = 0;
     std::vector<double> coords;          // (n_nodes + n_enr) * dim
     std::vector<int32_t> conn, elem_mat, cut_of_elem, cut_elem;
@@ -883,12 +884,73 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
             fail(PG_E_ARG, "bad inclusion parameters");
         if (p->dim == 3 && p->cohesive != PGH_COH_NONE)
             fail(PG_E_UNSUPPORTED, "3D cohesive RVEs are not supported");
+        P->cohesive = p->cohesive != PGH_COH_NONE;
         generate_mesh(*P);
         place_inclusions(*P);
         detect_nodes(*P);
         build_materials(*P);
         analyze_and_enrich(*P);
         number_dofs(*P);
+        build_pattern(*P);
+    } catch (const HarnessError& e) {
+        g_err = e.what();
+        delete P;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of memory";
+        delete P;
+        return PG_E_NOMEM;
+    }
+    *out = P;
+    return PG_OK;
+}
+
+int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) {
+    if (!m || !out) { g_err = "null argument"; return PG_E_ARG; }
+    *out = nullptr;
+    if ((m->dim != 2 && m->dim != 3) || m->nen != m->dim + 1 || m->n_nodes <= 0 || m->n_elem <= 0 || m->n_enr < 0 ||
+        m->n_cut < 0 || m->n_mat <= 0 || m->n_free < 0 || m->n_pres < 0 ||
+        m->n_free + m->n_pres != (m->n_nodes + m->n_enr) * m->dim) {
+        g_err = "inconsistent mesh description";
+        return PG_E_ARG;
+    }
+    if (!m->coords || !m->conn || !m->elem_mat || !m->cut_of_elem || !m->mats || !m->node_dof ||
+        (m->n_cut > 0 && (!m->cut || !m->cut_elem))) {
+        g_err = "missing mesh array";
+        return PG_E_ARG;
+    }
+    pgh_problem* P = new (std::nothrow) pgh_problem();
+    if (!P) { g_err = "out of memory"; return PG_E_NOMEM; }
+    try {
+        const int64_t nall = int64_t(m->n_nodes) + m->n_enr;
+        P->dim = m->dim;
+        P->nen = m->nen;
+        P->cohesive = m->cohesive != 0;
+        P->n_nodes = m->n_nodes;
+        P->n_enr = m->n_enr;
+        P->n_elem = m->n_elem;
+        P->n_cqp = m->n_cqp;
+        P->n_free = m->n_free;
+        P->n_pres = m->n_pres;
+        P->coords.assign(m->coords, m->coords + nall * m->dim);
+        P->conn.assign(m->conn, m->conn + int64_t(m->n_elem) * m->nen);
+        P->elem_mat.assign(m->elem_mat, m->elem_mat + m->n_elem);
+        P->cut_of_elem.assign(m->cut_of_elem, m->cut_of_elem + m->n_elem);
+        P->cut_elem.assign(m->cut_elem, m->cut_elem + m->n_cut);
+        P->cut.assign(m->cut, m->cut + m->n_cut);
+        P->mats.assign(m->mats, m->mats + m->n_mat);
+        P->node_dof.assign(m->node_dof, m->node_dof + nall * m->dim);
+        for (int64_t k = 0; k < int64_t(m->n_elem) * m->nen; ++k)
+            if (P->conn[k] < 0 || P->conn[k] >= m->n_nodes) fail(PG_E_ARG, "connectivity out of range");
+        for (int32_t c = 0; c < m->n_cut; ++c) {
+            const pg_cut_rec& R = P->cut[c];
+            if (P->cut_elem[c] < 0 || P->cut_elem[c] >= m->n_elem || R.n_enr > PG_MAX_ENR)
+                fail(PG_E_ARG, "bad cut record");
+            for (int k = 0; k < R.n_enr; ++k)
+                if (R.enr[k] < m->n_nodes || R.enr[k] >= nall) fail(PG_E_ARG, "enrichment node out of range");
+        }
+        for (int64_t k = 0; k < nall * m->dim; ++k)
+            if (P->node_dof[k] < 0 || P->node_dof[k] >= m->n_free + m->n_pres) fail(PG_E_ARG, "dof out of range");
         build_pattern(*P);
     } catch (const HarnessError& e) {
         g_err = e.what();
@@ -916,7 +978,7 @@ int pgh_get_desc(const pgh_problem* h, pg_problem_desc* o) {
     o->n_cqp = h->n_cqp;
     o->n_free = h->n_free;
     o->n_pres = h->n_pres;
-    o->cohesive = h->prm.cohesive != PGH_COH_NONE;
+    o->cohesive = h->cohesive;
     for (int b = 0; b < 4; ++b) {
         o->nnz[b] = int64_t(h->ci[b].size());
         o->rp[b] = h->rp[b].data();

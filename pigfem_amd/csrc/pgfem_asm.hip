@@ -981,15 +981,6 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     }
 }
 
-template <typename T>
-int dup(const T* src, size_t n, T** dst) {
-    *dst = nullptr;
-    if (n == 0) return PG_OK;
-    HIP_OK(hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)));
-    HIP_OK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-    return PG_OK;
-}
-
 }  // namespace
 
 struct pg_asm {
@@ -1017,6 +1008,7 @@ struct pg_asm {
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
     int64_t alg_bytes = 0;
+    int64_t dev_bytes = 0;  // device memory held by the handle
 };
 
 namespace {
@@ -1109,6 +1101,7 @@ int alloc(pg_asm* h, T** p, size_t n) {
     if (n == 0) n = 1;
     HIP_OK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
     h->allocs.push_back(*p);
+    h->dev_bytes += int64_t(n * sizeof(T));
     return PG_OK;
 }
 
@@ -1343,6 +1336,8 @@ int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total, int64_t* n
 
 int64_t pg_asm_algorithmic_bytes(const pg_asm* h) { return h ? h->alg_bytes : -1; }
 
+int64_t pg_asm_device_bytes(const pg_asm* h) { return h ? h->dev_bytes : -1; }
+
 int pg_asm_set_timing(pg_asm* h, int enable) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
     h->timing = enable != 0;
@@ -1406,6 +1401,7 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
         HIP_OK(hipMalloc(&h->dP, size_t(h->n_dof) * sizeof(double)));
         HIP_OK(hipMalloc(&h->dIc, size_t(std::max<int64_t>(ncq, 1)) * sizeof(double)));
         HIP_OK(hipMalloc(&h->dIt, size_t(std::max<int64_t>(ncq, 1)) * sizeof(double)));
+        h->dev_bytes += 8 * (2 * h->n_dof + std::max<int64_t>(h->nnz_total, 1) + 2 * std::max<int64_t>(ncq, 1));
     }
     hipStream_t st = h->stream;
     HIP_OK(hipMemsetAsync(h->err, 0, sizeof(int), st));

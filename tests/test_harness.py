@@ -167,3 +167,44 @@ def test_incidence_is_ascending_and_complete():
 def test_rejects_unsupported():
     with pytest.raises(Exception):
         RVEProblem(RVEConfig(dim=3, n=4, n_incl=1, r_min=0.1, r_max=0.2, cohesive=_ffi.PGH_COH_OP))
+
+
+@pytest.mark.parametrize("kind", ["op", "tet"])
+def test_build_pattern_from_external_mesh(kind):
+    """pgh_build_pattern (the integration path: mesh set up by PIGFEM) rebuilds
+    exactly the pattern, scatter indices and incidence of the generator."""
+    import ctypes as C
+    from pigfem_amd import _ffi
+    p = RVEProblem(small_config(kind, n=24 if kind == "op" else 8, n_incl=3 if kind == "op" else 1))
+    m = _ffi.PgProblemDesc()
+    C.memmove(C.addressof(m), C.addressof(p.desc), C.sizeof(m))
+    for f in ("npat_ptr", "npat_col", "npat_row", "npat_map", "n2e_ptr", "n2e"):
+        setattr(m, f, None)
+    for b in range(4):
+        m.rp[b] = None
+        m.ci[b] = None
+    h = C.c_void_p()
+    L = _ffi.lib()
+    _ffi.check(L.pgh_build_pattern(C.byref(m), C.byref(h)), "pgh_build_pattern")
+    try:
+        q = _ffi.PgProblemDesc()
+        _ffi.check(L.pgh_get_desc(h, C.byref(q)), "pgh_get_desc")
+        assert [q.nnz[b] for b in range(4)] == p.nnz and q.n_pairs == p.n_pairs
+        rows = [p.n_free, p.n_free, p.n_pres, p.n_pres]
+        for b in range(4):
+            assert np.array_equal(np.ctypeslib.as_array(q.rp[b], (rows[b] + 1,)), p.rp[b])
+            if p.nnz[b]:
+                assert np.array_equal(np.ctypeslib.as_array(q.ci[b], (p.nnz[b],)), p.ci[b])
+        nall = p.n_nodes + p.n_enr
+        assert np.array_equal(np.ctypeslib.as_array(q.npat_map, (p.npat_map.size,)), p.npat_map)
+        assert np.array_equal(np.ctypeslib.as_array(q.npat_col, (p.n_pairs,)), p.npat_col)
+        assert np.array_equal(np.ctypeslib.as_array(q.n2e_ptr, (nall + 1,)), p.n2e_ptr)
+        assert np.array_equal(np.ctypeslib.as_array(q.n2e, (p.n2e.size,)), p.n2e)
+        # a broken mesh is rejected, not read out of range
+        bad = _ffi.PgProblemDesc()
+        C.memmove(C.addressof(bad), C.addressof(m), C.sizeof(m))
+        bad.n_free += 1
+        h2 = C.c_void_p()
+        assert L.pgh_build_pattern(C.byref(bad), C.byref(h2)) != 0
+    finally:
+        L.pgh_free(h)
