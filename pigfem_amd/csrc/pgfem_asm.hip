@@ -58,8 +58,19 @@ struct DevProb {
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
     const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
+    const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
     int n_mat;
 };
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumbering so that each XCD walks one contiguous range of the work keeps
+// neighbouring node rows -- which share coordinates, U and staged tangents --
+// in the same L2.  A bijection of [0, nb) for any grid size.
+__device__ inline int xcd_order(int b, int nb) {
+    constexpr int X = 8;
+    const int per = nb / X, rem = nb % X, x = b % X;
+    return x * per + min(x, rem) + b / X;
+}
 
 // --------------------------------------------------------------------------
 // U in global-dof order -> node-major (node*dim + comp).  The reference gathers
@@ -391,7 +402,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ TK sD[MAX_LDS_MATS][NS * NS];
 
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    const int li_ = blockIdx.x * EPW + g;
+    const int li_ = xcd_order(blockIdx.x, gridDim.x) * EPW + g;
     const bool live = li_ < n_list;
     const int ci = live ? list[li_] : 0;
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += 64) {
@@ -506,7 +517,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         for (int a = 0; a < D; ++a) gr[a] = qg[g][q][l][a];
     };
     // local-node-pair blocks: K(li, lj) = sum_q B_li^T (wJ D) B_lj, then the
-    // cohesive fold; one lane per block
+    // cohesive fold; one lane per block.  Staged block-major: the d x d block of
+    // (li, lj) is contiguous at (li * nloc + lj) * d * d.
     TK* Ke = Kst + p.cut_koff[ci];
     for (int pb = gl; pb < nloc * nloc; pb += GS) {
         const int li = pb / nloc, lj = pb % nloc;
@@ -558,7 +570,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             }
         }
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) Ke[int64_t(li * D + i) * nd + lj * D + j] = acc[i][j];
+            for (int j = 0; j < D; ++j) Ke[int64_t(pb) * D * D + i * D + j] = acc[i][j];
     }
     double* Pe = Pst + p.cut_poff[ci];
     for (int i = gl; i < nd; i += GS) {
@@ -783,7 +795,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const int nall = p.n_nodes + p.n_enr;
-    const int a0 = blockIdx.x * NG;
+    const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
     const int ng = min(NG, nall - a0);
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
@@ -843,21 +855,22 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
                 }
             } else {
-                const int ci = tag;
-                const pg_cut_rec& r = p.cut[ci];
-                const int ne = r.n_enr;
+                const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
+                const int4 c0 = cl[0], c1 = cl[1];  // n_enr, enr0..2 | enr3, P offset, K offset lo / hi
+                const int ne = c0.x;
                 nloc = NEN + ne;
                 if (nloc > MAXLOC) {
                     s_bad = 1;  // local list does not fit: this group takes the global path
                 } else {
+                    const int en[4] = {c0.y, c0.z, c0.w, c1.x};
                     for (int k = 0; k < ne; ++k) {
-                        ln[NEN + k] = r.enr[k];
+                        ln[NEN + k] = en[k];
                         s_nodes[i][NEN + k] = ln[NEN + k];
                         if (ln[NEN + k] == a) la = NEN + k;
                     }
-                    s_koff[i] = p.cut_koff[ci];
+                    s_koff[i] = int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32);
                     if (func && la >= 0)
-                        for (int c = 0; c < D; ++c) s_P[i][c] = Pc[p.cut_poff[ci] + la * D + c];
+                        for (int c = 0; c < D; ++c) s_P[i][c] = Pc[int64_t(c1.y) + la * D + c];
                 }
             }
             s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
@@ -922,10 +935,10 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                 if (!(info & (1 << 16))) {
                     apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
                 } else {
-                    const int nd = ((info >> 8) & 255) * D;
-                    const TC* Ke = Kc + s_koff[t] + int64_t(la * D) * nd + lb * D;
+                    const int nl = (info >> 8) & 255;
+                    const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
                     for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(i) * nd + j]);
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
                 }
             }
         } else {
@@ -960,11 +973,11 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                         for (int i = 0; i < D; ++i) pa[i] += pp[i];
                     }
                 } else {
-                    const int nd = (NEN + p.cut[ci].n_enr) * D;
+                    const int nl = NEN + p.cut[ci].n_enr;
                     if (jac) {
-                        const TC* Ke = Kc + p.cut_koff[ci];
+                        const TC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
                         for (int i = 0; i < D; ++i)
-                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[int64_t(la * D + i) * nd + lb * D + j]);
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
                     }
                     if (diag)
                         for (int i = 0; i < D; ++i) pa[i] += Pc[p.cut_poff[ci] + la * D + i];
@@ -1225,6 +1238,11 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     }
     h->cut_k_total = koff[s->n_cut];
     h->cut_p_total = poff[s->n_cut];
+    if (h->cut_p_total >= (int64_t(1) << 31)) {
+        delete h;
+        g_err = "cut-element dofs exceed 2^31";
+        return PG_E_UNSUPPORTED;
+    }
     std::vector<int64_t> eoff(size_t(s->n_elem) + 1);
     eoff[0] = 0;
     for (int e = 0; e < s->n_elem; ++e) {
@@ -1271,6 +1289,19 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
     UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
     UP(inc, incv.data(), incv.size(), int32_t);
+    {
+        std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
+        for (int c = 0; c < s->n_cut; ++c) {
+            const pg_cut_rec& r = s->cut[c];
+            int32_t* o = &cl[size_t(c) * 8];
+            o[0] = r.n_enr;
+            for (int k = 0; k < 4 && k < r.n_enr; ++k) o[1 + k] = r.enr[k];
+            o[5] = int32_t(poff[c]);
+            o[6] = int32_t(uint32_t(uint64_t(koff[c]) & 0xffffffffu));
+            o[7] = int32_t(uint64_t(koff[c]) >> 32);
+        }
+        UP(cut_loc, cl.data(), cl.size(), int32_t);
+    }
 #undef UP
     {
         std::vector<int32_t> small, general;
