@@ -46,8 +46,12 @@ typedef struct pgh_rve_params {
     double  sigma_c, delta_c, delta_c1, delta_c2, beta, alpha;
     double  strain;         /* applied u_x on RIGHT = strain * width            */
     int32_t max_tries;      /* rejection-sampling budget (0 = default)          */
-    int32_t pad;
+    int32_t flags;          /* PGH_PLACE_*                                      */
 } pgh_rve_params;
+
+/* Stop placing inclusions when the rejection budget is spent and keep those
+ * already placed (at least one), instead of failing. */
+#define PGH_PLACE_BEST_EFFORT 1
 
 typedef struct pgh_problem pgh_problem;
 
@@ -65,6 +69,19 @@ int  pgh_get_desc(const pgh_problem* h, pg_problem_desc* out);
 int  pgh_build_pattern(const pg_problem_desc* mesh, pgh_problem** out);
 void pgh_free(pgh_problem* h);
 const char* pgh_last_error(void);
+
+/* A batch of independent 2D RVEs (one parameter set each, same materials)
+ * merged into one block-diagonal problem: instance i owns a contiguous range
+ * of elements, regular nodes, enrichment nodes, free and prescribed dofs (so
+ * contiguous rows and columns of all four K blocks), cohesive qps and cut
+ * elements.  Instances are generated in parallel host threads. */
+int  pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out);
+/* Per-instance first indices, PGH_BATCH_COLS columns per row, n_inst + 1 rows
+ * (the last row holds the totals): element, regular node, enrichment node,
+ * free dof, prescribed dof, cohesive qp, cut element, inclusion.  n_inst is 0
+ * for a problem that is not a batch. */
+#define PGH_BATCH_COLS 8
+int  pgh_batch_info(const pgh_problem* h, int32_t* n_inst, const int64_t** offsets);
 
 /* Inclusions actually placed: centers (dim each) and radii. */
 int  pgh_inclusions(const pgh_problem* h, const double** centers,

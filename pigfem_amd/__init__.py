@@ -5,9 +5,9 @@ The package holds only the assembly hot path: HIP kernels and their C ABI
 description, and thin Python mirrors of the reference's Assembler API.
 """
 from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PgError, lib  # noqa: F401
-from .problem import RVEConfig, RVEProblem, config, small_config  # noqa: F401
+from .problem import RVEConfig, RVEProblem, batch_config, config, small_config  # noqa: F401
 from .assembler import AssemblerGPU, PMatrix, PVector  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
 
-__all__ = ["RVEConfig", "RVEProblem", "config", "small_config", "AssemblerGPU", "PMatrix", "PVector",
+__all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU", "PMatrix", "PVector",
            "DeviceBuffer", "synchronize", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

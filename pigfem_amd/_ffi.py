@@ -19,6 +19,8 @@ PG_BATCHED = 8
 
 PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
+PGH_PLACE_BEST_EFFORT = 1
+PGH_BATCH_COLS = 8
 PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4 = range(3)
 PG_MAX_ENR, PG_MAX_CHILD, PG_MAX_COH = 12, 8, 3
 
@@ -69,7 +71,7 @@ class PghRveParams(C.Structure):
         ("E_matrix", C.c_double), ("nu_matrix", C.c_double), ("E_fiber", C.c_double), ("nu_fiber", C.c_double),
         ("sigma_c", C.c_double), ("delta_c", C.c_double), ("delta_c1", C.c_double), ("delta_c2", C.c_double),
         ("beta", C.c_double), ("alpha", C.c_double), ("strain", C.c_double),
-        ("max_tries", C.c_int32), ("pad", C.c_int32),
+        ("max_tries", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -99,6 +101,8 @@ SIGNATURES = {
     "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
     "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),
     "pgh_build_pattern": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
+    "pgh_build_batch": (C.c_int, [C.POINTER(PghRveParams), C.c_int32, C.POINTER(C.c_void_p)]),
+    "pgh_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.POINTER(C.c_int64))]),
     "pgh_free": (None, [C.c_void_p]),
     "pgh_last_error": (C.c_char_p, []),
     "pgh_inclusions": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),

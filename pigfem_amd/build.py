@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INC, "-I" + CSRC,
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I" + INC, "-I" + CSRC]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-I" + INC, "-I" + CSRC]
 
 
 def _run(cmd):
@@ -49,7 +49,7 @@ def build(verbose: bool = False) -> str:
         objs.append(o)
     so = os.path.join(OUT, "libpgfem_asm.so")
     tmp = so + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, so)
     for o in objs:
         os.remove(o)

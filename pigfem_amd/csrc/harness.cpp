@@ -4,12 +4,14 @@
 #include "pgfem_harness.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -74,6 +76,11 @@ struct pgh_problem {
     std::vector<int32_t> rp[4], ci[4];
     std::vector<int32_t> npat_ptr, npat_col, npat_row, npat_map;
     std::vector<int32_t> n2e_ptr, n2e;
+    // batch of independent RVEs: per instance the first element, regular node,
+    // enrichment node, free dof, prescribed dof, cohesive qp and cut element,
+    // plus one closing row (PGH_BATCH_COLS columns)
+    int32_t n_inst = 0;
+    std::vector<int64_t> inst_off;
 };
 
 namespace {
@@ -203,9 +210,11 @@ void place_inclusions(pgh_problem& P) {
     long budget = p.max_tries > 0 ? p.max_tries : 50000000L;
     long tries = 0;
     while (int(P.incl.size()) < p.n_incl) {
-        if (++tries > budget)
+        if (++tries > budget) {
+            if ((p.flags & PGH_PLACE_BEST_EFFORT) && !P.incl.empty()) break;  // keep what fits
             fail(PG_E_ARG, "inclusion placement did not converge (" +
                                std::to_string(P.incl.size()) + " of " + std::to_string(p.n_incl) + " placed)");
+        }
         Incl I{};
         I.r = rng.uni(p.r_min, p.r_max);
         for (int d = 0; d < P.dim; ++d) I.c[d] = rng.uni(I.r, 1.0 - I.r);
@@ -965,6 +974,168 @@ int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) {
     return PG_OK;
 }
 
+int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) {
+    if (!params || !out || n <= 0) { g_err = "bad argument"; return PG_E_ARG; }
+    *out = nullptr;
+    std::vector<pgh_problem*> inst(size_t(n), nullptr);
+    std::vector<int> code(size_t(n), 0);
+    std::vector<std::string> msg((size_t)n);
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+        for (int i = next++; i < n; i = next++) {
+            pgh_problem* P = new (std::nothrow) pgh_problem();
+            if (!P) { code[i] = PG_E_NOMEM; msg[i] = "out of memory"; continue; }
+            try {
+                const pgh_rve_params& p = params[i];
+                P->prm = p;
+                P->dim = p.dim;
+                if (p.dim != 2) fail(PG_E_UNSUPPORTED, "batches are 2D");
+                if (p.n < 1 || p.n_incl < 0 || p.r_min <= 0 || p.r_max < p.r_min || 2 * p.r_max >= 1.0)
+                    fail(PG_E_ARG, "bad instance parameters");
+                P->cohesive = p.cohesive != PGH_COH_NONE;
+                generate_mesh(*P);
+                place_inclusions(*P);
+                detect_nodes(*P);
+                build_materials(*P);
+                analyze_and_enrich(*P);
+                number_dofs(*P);
+            } catch (const HarnessError& e) {
+                code[i] = e.code;
+                msg[i] = e.what();
+            } catch (const std::bad_alloc&) {
+                code[i] = PG_E_NOMEM;
+                msg[i] = "out of memory";
+            }
+            inst[i] = P;
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    auto cleanup = [&]() { for (auto* P : inst) delete P; };
+    for (int i = 0; i < n; ++i)
+        if (code[i]) {
+            g_err = "instance " + std::to_string(i) + ": " + msg[i];
+            const int c = code[i];
+            cleanup();
+            return c;
+        }
+    for (int i = 1; i < n; ++i)
+        if (inst[i]->mats.size() != inst[0]->mats.size() || inst[i]->cohesive != inst[0]->cohesive ||
+            std::memcmp(inst[i]->mats.data(), inst[0]->mats.data(), inst[0]->mats.size() * sizeof(pg_material)) != 0) {
+            g_err = "instances of a batch must share their materials";
+            cleanup();
+            return PG_E_UNSUPPORTED;
+        }
+    pgh_problem* B = new (std::nothrow) pgh_problem();
+    if (!B) { cleanup(); g_err = "out of memory"; return PG_E_NOMEM; }
+    try {
+        // merged numbering: regular nodes instance by instance, then every
+        // instance's enrichment nodes; free dofs instance by instance, then the
+        // prescribed dofs -- so each instance owns contiguous rows and columns of
+        // all four blocks and the merged pattern is block diagonal
+        constexpr int C = PGH_BATCH_COLS;
+        B->n_inst = n;
+        B->inst_off.assign(size_t(n + 1) * C, 0);
+        for (int i = 0; i < n; ++i) {
+            const pgh_problem& P = *inst[i];
+            int64_t* o = &B->inst_off[size_t(i) * C];
+            int64_t* q = o + C;
+            q[0] = o[0] + P.n_elem;
+            q[1] = o[1] + P.n_nodes;
+            q[2] = o[2] + P.n_enr;
+            q[3] = o[3] + P.n_free;
+            q[4] = o[4] + P.n_pres;
+            q[5] = o[5] + P.n_cqp;
+            q[6] = o[6] + int64_t(P.cut.size());
+            q[7] = o[7] + int64_t(P.incl_r.size());
+        }
+        const int64_t* T = &B->inst_off[size_t(n) * C];
+        if (T[0] >= (int64_t(1) << 31) || T[1] + T[2] >= (int64_t(1) << 31) || (T[3] + T[4]) >= (int64_t(1) << 31))
+            fail(PG_E_UNSUPPORTED, "batch exceeds 32-bit indices");
+        B->dim = 2;
+        B->nen = 3;
+        B->cohesive = inst[0]->cohesive;
+        B->prm = inst[0]->prm;
+        B->mats = inst[0]->mats;
+        B->n_elem = int32_t(T[0]);
+        B->n_nodes = int32_t(T[1]);
+        B->n_enr = int32_t(T[2]);
+        B->n_free = int32_t(T[3]);
+        B->n_pres = int32_t(T[4]);
+        B->n_cqp = int32_t(T[5]);
+        const int64_t nall = T[1] + T[2];
+        B->coords.resize(size_t(nall) * 2);
+        B->conn.resize(size_t(T[0]) * 3);
+        B->elem_mat.resize(size_t(T[0]));
+        B->cut_of_elem.resize(size_t(T[0]));
+        B->cut_elem.resize(size_t(T[6]));
+        B->cut.resize(size_t(T[6]));
+        B->node_dof.resize(size_t(nall) * 2);
+        B->bc_val.resize(size_t(T[4]));
+        B->node_detect.resize(size_t(T[1]));
+        for (int i = 0; i < n; ++i) {
+            const pgh_problem& P = *inst[i];
+            const int64_t* o = &B->inst_off[size_t(i) * C];
+            const int32_t e0 = int32_t(o[0]), r0 = int32_t(o[1]), c0 = int32_t(o[6]);
+            const int32_t x0 = int32_t(T[1] + o[2]);  // first merged id of this instance's enrichment nodes
+            auto node = [&](int32_t a) { return a < P.n_nodes ? r0 + a : x0 + (a - P.n_nodes); };
+            for (int32_t a = 0; a < P.n_nodes + P.n_enr; ++a) {
+                const int32_t m = node(a);
+                for (int c = 0; c < 2; ++c) {
+                    B->coords[size_t(m) * 2 + c] = P.coords[size_t(a) * 2 + c];
+                    const int32_t dl = P.node_dof[size_t(a) * 2 + c];
+                    B->node_dof[size_t(m) * 2 + c] =
+                        dl < P.n_free ? int32_t(o[3] + dl) : int32_t(T[3] + o[4] + (dl - P.n_free));
+                }
+            }
+            for (int32_t a = 0; a < P.n_nodes; ++a)
+                B->node_detect[size_t(r0) + a] = P.node_detect[a] < 0 ? -1 : P.node_detect[a];
+            for (int32_t k = 0; k < P.n_pres; ++k) B->bc_val[size_t(o[4]) + k] = P.bc_val[k];
+            for (int32_t e = 0; e < P.n_elem; ++e) {
+                for (int k = 0; k < 3; ++k) B->conn[size_t(e0 + e) * 3 + k] = r0 + P.conn[size_t(e) * 3 + k];
+                B->elem_mat[size_t(e0) + e] = P.elem_mat[e];
+                B->cut_of_elem[size_t(e0) + e] = P.cut_of_elem[e] < 0 ? -1 : c0 + P.cut_of_elem[e];
+            }
+            for (size_t c = 0; c < P.cut.size(); ++c) {
+                pg_cut_rec R = P.cut[c];
+                for (int k = 0; k < R.n_enr; ++k) R.enr[k] = node(R.enr[k]);
+                R.isv_off += int32_t(o[5]);
+                B->cut[size_t(c0) + c] = R;
+                B->cut_elem[size_t(c0) + c] = e0 + P.cut_elem[c];
+            }
+            for (size_t k = 0; k < P.incl_r.size(); ++k) {
+                B->incl_c.push_back(P.incl_c[k * 2]);
+                B->incl_c.push_back(P.incl_c[k * 2 + 1]);
+                B->incl_r.push_back(P.incl_r[k]);
+            }
+        }
+        cleanup();
+        inst.clear();
+        build_pattern(*B);
+    } catch (const HarnessError& e) {
+        g_err = e.what();
+        cleanup();
+        delete B;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of memory";
+        cleanup();
+        delete B;
+        return PG_E_NOMEM;
+    }
+    *out = B;
+    return PG_OK;
+}
+
+int pgh_batch_info(const pgh_problem* h, int32_t* n_inst, const int64_t** offsets) {
+    if (!h || !n_inst || !offsets) { g_err = "null argument"; return PG_E_ARG; }
+    *n_inst = h->n_inst;
+    *offsets = h->n_inst ? h->inst_off.data() : nullptr;
+    return PG_OK;
+}
+
 int pgh_get_desc(const pgh_problem* h, pg_problem_desc* o) {
     if (!h || !o) { g_err = "null argument"; return PG_E_ARG; }
     std::memset(o, 0, sizeof(*o));
@@ -1008,7 +1179,7 @@ int pgh_inclusions(const pgh_problem* h, const double** centers, const double** 
     if (!h) return PG_E_ARG;
     if (centers) *centers = h->incl_c.data();
     if (radii) *radii = h->incl_r.data();
-    if (n) *n = int32_t(h->incl.size());
+    if (n) *n = int32_t(h->incl_r.size());
     return PG_OK;
 }
 

@@ -99,15 +99,44 @@ def _arr(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(int(n),)).view(dtype)
 
 
-class RVEProblem:
-    """A built RVE: the static problem description plus numpy views of it."""
+def batch_config(n_inst: int = 256, seed: int = 5, n_min: int = 64, n_max: int = 512) -> list:
+    """The batch of independent cohesive RVEs: mesh n x n with n log-uniform in
+    [n_min, n_max], 5-150 fibers each (count capped at an expected area fraction
+    of 0.45), radius U[4h, 10h], bilinear cohesive law of the 500x500 study,
+    one seed per instance."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        h = 1.0 / n
+        r_min, r_max = 4 * h, 10 * h
+        mean_area = math.pi * (r_min * r_min + r_min * r_max + r_max * r_max) / 3.0
+        nf = int(rng.integers(5, 151))
+        nf = max(1, min(nf, int(0.45 / mean_area)))
+        out.append(RVEConfig(dim=2, n=n, n_incl=nf, r_min=r_min, r_max=r_max, seed=seed * 100003 + i,
+                             cohesive=_ffi.PGH_COH_BL, name=f"C5-{i}"))
+    return out
 
-    def __init__(self, cfg: RVEConfig):
+
+class RVEProblem:
+    """A built RVE (or a batch of RVEs merged into one block-diagonal problem):
+    the static problem description plus numpy views of it."""
+
+    def __init__(self, cfg: RVEConfig, _cfgs=None):
         self.cfg = cfg
         L = lib()
         h = C.c_void_p()
-        self._params = cfg.params()
-        check(L.pgh_build_rve(C.byref(self._params), C.byref(h)), "pgh_build_rve")
+        if _cfgs is None:
+            self._params = cfg.params()
+            check(L.pgh_build_rve(C.byref(self._params), C.byref(h)), "pgh_build_rve")
+        else:
+            arr = (PghRveParams * len(_cfgs))()
+            for i, c in enumerate(_cfgs):
+                arr[i] = c.params()
+                arr[i].flags = _ffi.PGH_PLACE_BEST_EFFORT
+                arr[i].max_tries = 2_000_000
+            self._params = arr
+            check(L.pgh_build_batch(arr, len(_cfgs), C.byref(h)), "pgh_build_batch")
         self._h = h
         self.desc = PgProblemDesc()
         check(L.pgh_get_desc(h, C.byref(self.desc)), "pgh_get_desc")
@@ -142,6 +171,29 @@ class RVEProblem:
         self.incl_centers = _arr(cptr, ni.value * self.dim, np.float64).reshape(ni.value, self.dim)
         self.incl_radii = _arr(rptr, ni.value, np.float64)
         self.cut = [d.cut[i] for i in range(self.n_cut)] if self.n_cut < 200000 else None
+        ni, off = C.c_int32(), C.POINTER(C.c_int64)()
+        check(L.pgh_batch_info(h, C.byref(ni), C.byref(off)), "pgh_batch_info")
+        self.n_inst = ni.value
+        self.inst_off = (_arr(off, (ni.value + 1) * _ffi.PGH_BATCH_COLS, np.int64)
+                         .reshape(ni.value + 1, _ffi.PGH_BATCH_COLS) if ni.value else None)
+
+    @classmethod
+    def batch(cls, cfgs):
+        """Merge independent RVEs into one problem (pgh_build_batch)."""
+        return cls(cfgs[0], _cfgs=list(cfgs))
+
+    def instance_ranges(self, i: int):
+        """Value ranges of instance i in the concatenated K [ff|fp|pf|pp] and P [f|p]."""
+        o, q = self.inst_off[i], self.inst_off[i + 1]
+        f0, f1, p0, p1 = int(o[3]), int(q[3]), int(o[4]), int(q[4])
+        base = np.cumsum([0] + self.nnz)
+        rows = [(f0, f1), (f0, f1), (p0, p1), (p0, p1)]
+        k = [(int(base[b] + self.rp[b][r0]), int(base[b] + self.rp[b][r1])) for b, (r0, r1) in enumerate(rows)]
+        p = [(f0, f1), (self.n_free + p0, self.n_free + p1)]
+        return k, p
+
+    def instance_elements(self, i: int) -> int:
+        return int(self.inst_off[i + 1][0] - self.inst_off[i][0])
 
     def cut_record(self, i: int):
         return self.desc.cut[i]
