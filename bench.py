@@ -89,7 +89,7 @@ def main():
     for mode in ("binned64", "mixed"):
         asm.mode = mode
         tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
-        log(f"[bench] tuned {mode}: group_nodes, cut_variant = {tuned[mode]}")
+        log(f"[bench] tuned {mode}: group_nodes, cut_variant, group_variant = {tuned[mode]}")
 
     def run(mode, steps):
         asm.mode = mode
@@ -226,9 +226,9 @@ def main():
 
     # HBM traffic of the same kernel instantiation from the committed rocprofv3
     # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
-    ng = tuned["mixed"][0]
+    ng, occ = tuned["mixed"][0], (8 if tuned["mixed"][2] == 1 else 1)
     bs = min(256, max(64, (8 if prob.dim == 2 else 16) * ng))
-    kname = f"k_pair_group<{prob.dim}, float, {ng}, {bs}>"
+    kname = f"k_pair_group<{prob.dim}, float, {ng}, {occ}, {bs}>"
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):

@@ -214,8 +214,11 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                2, 4 or 8 in 3D (0: default 32 / 8);
  *   cut_variant  small-class cut kernel: 0 = 4 elements per wavefront,
  *                1 = 4 per wavefront with a 3-waves/SIMD register cap,
- *                2 = 2 per wavefront, 3 = 8 per wavefront. */
-int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant);
+ *                2 = 2 per wavefront, 3 = 8 per wavefront;
+ *   group_variant node-group kernel: 0 = registers uncapped, 1 = capped for
+ *                8 waves per SIMD.
+ * Every choice gives bit-identical results. */
+int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 
 /* Algorithmic bytes of one K+R assembly of this problem (the roofline
  * numerator): connectivity, coordinates, U, cut records, ISVs, CSR values +

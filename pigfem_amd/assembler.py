@@ -148,9 +148,10 @@ class AssemblerGPU:
         check(lib().pg_asm_last_timing(self._h, ms), "pg_asm_last_timing")
         return list(ms)
 
-    def set_tiles(self, group_nodes: int = 0, cut_variant: int = 0) -> None:
+    def set_tiles(self, group_nodes: int = 0, cut_variant: int = 0, group_variant: int = 0) -> None:
         """Schedule of the class-binned modes (see pg_asm_set_tiles)."""
-        check(lib().pg_asm_set_tiles(self._h, int(group_nodes), int(cut_variant)), "pg_asm_set_tiles")
+        check(lib().pg_asm_set_tiles(self._h, int(group_nodes), int(cut_variant), int(group_variant)),
+              "pg_asm_set_tiles")
 
     def algorithmic_bytes(self) -> int:
         return int(lib().pg_asm_algorithmic_bytes(self._h))

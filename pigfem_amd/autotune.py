@@ -1,9 +1,10 @@
 """Autotuner of the class-binned schedules.
 
-Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block of the
-node-group kernel and the variant of the small-class cut kernel (elements per
-wavefront, register cap) -- timing each with HIP events on the assembly stream,
-and keeps the fastest.  Results are cached in a JSON file keyed by the problem
+Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block and
+register cap of the node-group kernel, then the variant of the small-class cut
+kernel (elements per wavefront, register cap) -- timing each kernel with HIP
+events on the assembly stream, and keeps the fastest.  Every choice gives
+bit-identical results.  Results are cached in a JSON file keyed by the problem
 class (dimension, cohesive law, size bucket) and the mode.
 """
 from __future__ import annotations
@@ -18,6 +19,7 @@ from .device import DeviceBuffer, synchronize
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 GROUPS = {2: (8, 16, 32), 3: (2, 4, 8)}
+GROUP_VARIANTS = (0, 1)
 CUT_VARIANTS = (0, 1, 2, 3)
 
 
@@ -35,15 +37,15 @@ def _load():
 
 
 def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
-    """Pick (group_nodes, cut_variant) for `asm` (an AssemblerGPU in a binned mode);
-    returns (choice, table of mean kernel ms per candidate)."""
+    """Pick (group_nodes, cut_variant, group_variant) for `asm` (an AssemblerGPU in
+    a binned mode); returns (choice, table of mean kernel ms per candidate)."""
     p = asm.problem
     key = _key(p, asm.mode)
     cache = _load()
-    if use_cache and key in cache:
-        g, c = cache[key]["choice"]
-        asm.set_tiles(g, c)
-        return (g, c), cache[key].get("table", {})
+    if use_cache and key in cache and len(cache[key]["choice"]) == 3:
+        choice = tuple(cache[key]["choice"])
+        asm.set_tiles(*choice)
+        return choice, cache[key].get("table", {})
     if U is None:
         U = p.random_u(0)
     dU = DeviceBuffer.from_numpy(U)
@@ -54,29 +56,40 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
     ic = dIc if p.n_cqp else None
     it = dIt if p.n_cqp else None
     table = {}
+
+    def timed(g, c, v, which):
+        asm.set_tiles(g, c, v)
+        asm.assemble_device(dU, ic, it, dK, dP)  # warm
+        ms = []
+        for _ in range(reps):
+            asm.assemble_device(dU, ic, it, dK, dP)
+            ms.append(asm.last_timing()[which])
+        return float(np.mean(ms))
+
     asm.set_timing(True)
     try:
+        # node-group kernel: nodes per block x register cap (timing index 2)
         for g in GROUPS[p.dim]:
-            for c in CUT_VARIANTS:
-                asm.set_tiles(g, c)
-                asm.assemble_device(dU, ic, it, dK, dP)  # warm
-                ms = []
-                for _ in range(reps):
-                    asm.assemble_device(dU, ic, it, dK, dP)
-                    ms.append(asm.last_timing()[0])
-                table[f"{g},{c}"] = float(np.mean(ms))
+            for v in GROUP_VARIANTS:
+                table[f"group {g},{v}"] = timed(g, 0, v, 2)
                 if log:
-                    log(f"[tune] group_nodes={g} cut_variant={c}: {table[f'{g},{c}']:.3f} ms")
+                    log(f"[tune] group_nodes={g} group_variant={v}: {table[f'group {g},{v}']:.3f} ms")
+        gbest = min((k for k in table if k.startswith("group")), key=table.get)
+        g, v = (int(x) for x in gbest.split()[1].split(","))
+        # cut-element kernel variants (timing index 1: permutation + cut kernels)
+        for c in CUT_VARIANTS:
+            table[f"cut {c}"] = timed(g, c, v, 1)
+            if log:
+                log(f"[tune] cut_variant={c}: {table[f'cut {c}']:.3f} ms")
+        c = int(min((k for k in table if k.startswith("cut")), key=table.get).split()[1])
     finally:
         asm.set_timing(False)
         synchronize()
-    best = min(table, key=table.get)
-    g, c = (int(x) for x in best.split(","))
-    asm.set_tiles(g, c)
-    cache[key] = {"choice": [g, c], "table": table}
+    asm.set_tiles(g, c, v)
+    cache[key] = {"choice": [g, c, v], "table": table}
     try:
         with open(CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     except OSError:
         pass
-    return (g, c), table
+    return (g, c, v), table

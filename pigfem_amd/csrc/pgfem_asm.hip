@@ -672,11 +672,13 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
 // tables takes the same arithmetic from global memory.
 // --------------------------------------------------------------------------
 template <int D> struct GroupCfg;
-// per-group LDS capacity (incidences, pairs) for NG nodes: about 10 incidences
-// and 16 pairs per node in 2D, 32 and 48 in 3D
+// per-group LDS capacity (incidences, pairs) for NG nodes: 7 incidences and 16
+// pairs per node in 2D (structured tri3: at most 8 and 6 on average per regular
+// node, 2 per enrichment node), 24 and 36 in 3D (6-tet cubes: at most 24 and
+// 27 + enrichment).  A group beyond these takes the global-memory path.
 template <int D, int NG> struct GroupLim {
-    static constexpr int MAXI = (D == 2 ? 10 : 32) * NG;
-    static constexpr int MAXP = (D == 2 ? 16 : 48) * NG;
+    static constexpr int MAXI = (D == 2 ? 7 : 24) * NG;
+    static constexpr int MAXP = (D == 2 ? 16 : 36) * NG;
 };
 
 // Closed-form gradients of a linear simplex (adjugate of the affine Jacobian
@@ -771,14 +773,19 @@ template <int D, int NG> struct GroupThreads {
     static constexpr int value = (D == 2 ? 8 : 16) * NG < 64 ? 64 : ((D == 2 ? 8 : 16) * NG > 256 ? 256 : (D == 2 ? 8 : 16) * NG);
 };
 
-template <int D, typename TC, int NG, int BS = GroupThreads<D, NG>::value>
-__global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
+// OCC: waves per SIMD the register allocation must allow (1: no cap; 8 caps a
+// lane at 64 VGPRs so that the LDS tables, not registers, bound occupancy).
+template <int D, typename TC, int NG, int OCC, int BS = GroupThreads<D, NG>::value>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
-    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes staged per incidence
-    constexpr int CAP = (D == 2 ? 8 : 32);    // contributions per pair slot
+    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
+    // contributions per off-diagonal pair: the elements sharing the edge (a, b),
+    // 2 in a tri3 mesh and up to 6 around a tet4 edge (cut elements add their
+    // enrichment pairs); the diagonal pair reads the node's incidences directly
+    constexpr int CAP = (D == 2 ? 4 : 8);
     static_assert(MAXI <= 4096 && MAXLOC <= 16, "contribution codes are (incidence << 4 | local node)");
     __shared__ int s_iptr[NG + 1];
     __shared__ int s_qptr[NG + 1];
@@ -786,7 +793,6 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_cnt[MAXP];              // contributions collected per pair
     __shared__ unsigned short s_list[MAXP][CAP];
     __shared__ int s_info[MAXI];             // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: absent
-    __shared__ int s_nodes[MAXI][MAXLOC];
     __shared__ int64_t s_koff[MAXI];
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_L[MAXI][D * NS];
@@ -834,7 +840,6 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             int la = -1;
             for (int n = 0; n < NEN; ++n) {
                 ln[n] = rec[n];
-                s_nodes[i][n] = ln[n];
                 if (ln[n] == a) la = n;
             }
             const int tag = rec[NEN];
@@ -865,7 +870,6 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     const int en[4] = {c0.y, c0.z, c0.w, c1.x};
                     for (int k = 0; k < ne; ++k) {
                         ln[NEN + k] = en[k];
-                        s_nodes[i][NEN + k] = ln[NEN + k];
                         if (ln[NEN + k] == a) la = NEN + k;
                     }
                     s_koff[i] = int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32);
@@ -875,9 +879,10 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             }
             s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
             if (la < 0 || nloc > MAXLOC) continue;
-            // contribution of this incidence to each pair (a, ln[m]) of a's row
+            // contribution of this incidence to each off-diagonal pair (a, ln[m]) of a's row
             const int ps0 = s_qptr[lo] - q0, ps1 = s_qptr[lo + 1] - q0;
             for (int m = 0; m < nloc; ++m) {
+                if (m == la) continue;
                 int l2 = ps0, h2 = ps1;  // first slot with column >= ln[m]
                 while (l2 < h2) {
                     const int mid = (l2 + h2) >> 1;
@@ -908,7 +913,26 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
         double pa[D];
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
-        if (use_lists) {
+        if (use_lists && a == b) {
+            // diagonal pair: every incidence of a, in ascending element order
+            const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
+            for (int t = t0; t < t1; ++t) {
+                const int info = s_info[t];
+                const int la = info & 255;
+                if (la == 255) continue;
+                if (func)
+                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
+                if (!jac) continue;
+                if (!(info & (1 << 16))) {
+                    apply_left<D, TC>(s_L[t], s_g[t][la], acc);
+                } else {
+                    const int nl = (info >> 8) & 255;
+                    const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
+                }
+            }
+        } else if (use_lists) {
             // the slot's contributions, sorted by incidence = ascending element order
             const int s = q - q0;
             const int cnt = s_cnt[s];
@@ -923,22 +947,21 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     code[j - 1] = lo2;
                     code[j] = hi2;
                 }
+            if (jac) {
 #pragma unroll
-            for (int k = 0; k < CAP; ++k) {
-                if (k >= cnt) break;
-                const int t = int(code[k] >> 4), lb = int(code[k] & 15);
-                const int info = s_info[t];
-                const int la = info & 255;
-                if (diag)
-                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-                if (!jac) continue;
-                if (!(info & (1 << 16))) {
-                    apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
-                } else {
-                    const int nl = (info >> 8) & 255;
-                    const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
+                for (int k = 0; k < CAP; ++k) {
+                    if (k >= cnt) break;
+                    const int t = int(code[k] >> 4), lb = int(code[k] & 15);
+                    const int info = s_info[t];
+                    const int la = info & 255;
+                    if (!(info & (1 << 16))) {
+                        apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
+                    } else {
+                        const int nl = (info >> 8) & 255;
+                        const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
+                    }
                 }
             }
         } else {
@@ -1020,6 +1043,7 @@ struct pg_asm {
     int elem_block = 256;
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
+    int group_variant = 0; // node-group kernel register cap (0: none, 1: 8 waves / SIMD)
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
 };
@@ -1179,8 +1203,10 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
+#define GROUP1(DD, TC, KC, NGV, OCCV) \
+    k_pair_group<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
 #define GROUP(DD, TC, KC, NGV) \
-    k_pair_group<DD, TC, NGV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
+    do { if (h->group_variant == 1) GROUP1(DD, TC, KC, NGV, 8); else GROUP1(DD, TC, KC, NGV, 1); } while (0)
         if (d.dim == 2) {
             if (fp32) {
                 if (ng == 8) GROUP(2, float, h->Kc32, 8);
@@ -1203,6 +1229,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             }
         }
 #undef GROUP
+#undef GROUP1
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1375,14 +1402,16 @@ int pg_asm_set_timing(pg_asm* h, int enable) {
     return PG_OK;
 }
 
-int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant) {
+int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
     const bool ok_g = group_nodes == 0 || (h->d.dim == 2 ? (group_nodes == 8 || group_nodes == 16 || group_nodes == 32)
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 1) { g_err = "group variant must be 0 or 1"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
+    h->group_variant = group_variant;
     return PG_OK;
 }
 

@@ -109,8 +109,9 @@ def test_device_api_matches_host_api():
 
 @pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1)])
 def test_every_schedule_variant(kind, n, ni):
-    """All node-group sizes and cut-kernel variants the autotuner may pick give
-    the same result, bit for bit (same arithmetic, same summation order)."""
+    """All node-group sizes, register caps and cut-kernel variants the autotuner
+    may pick give the same result, bit for bit (same arithmetic, same summation
+    order)."""
     p = RVEProblem(small_config(kind, n=n, n_incl=ni))
     o = Oracle(p)
     U = p.random_u(9)
@@ -119,14 +120,13 @@ def test_every_schedule_variant(kind, n, ni):
     groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
     for mode in ("binned64", "mixed"):
         ref = None
-        for g in groups:
-            for c in range(4):
-                asm = AssemblerGPU(p, mode)
-                asm.set_tiles(g, c)
-                K, P, _ = _run(asm, p, U, ic)
-                assert rel(K, Kr) < (1e-12 if mode == "binned64" else 1e-6)
-                assert rel(P, Pr) < 1e-12
-                if ref is None:
-                    ref = (K, P)
-                else:
-                    assert np.array_equal(ref[0], K) and np.array_equal(ref[1], P)
+        for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in range(2)]:
+            asm = AssemblerGPU(p, mode)
+            asm.set_tiles(g, c, v)
+            K, P, _ = _run(asm, p, U, ic)
+            assert rel(K, Kr) < (1e-12 if mode == "binned64" else 1e-6)
+            assert rel(P, Pr) < 1e-12
+            if ref is None:
+                ref = (K, P)
+            else:
+                assert np.array_equal(ref[0], K) and np.array_equal(ref[1], P)
