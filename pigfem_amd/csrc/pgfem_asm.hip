@@ -59,6 +59,7 @@ struct DevProb {
     const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
     const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
+    const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
     int n_mat;
 };
 
@@ -789,9 +790,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     static_assert(MAXI <= 4096 && MAXLOC <= 16, "contribution codes are (incidence << 4 | local node)");
     __shared__ int s_iptr[NG + 1];
     __shared__ int s_qptr[NG + 1];
-    __shared__ int s_pcol[MAXP];             // column node of each pair of the group
-    __shared__ int s_cnt[MAXP];              // contributions collected per pair
-    __shared__ unsigned short s_list[MAXP][CAP];
+    __shared__ unsigned short s_list[MAXP][CAP];  // contributions per pair, by rank (0xffff: none)
     __shared__ int s_info[MAXI];             // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: absent
     __shared__ int64_t s_koff[MAXI];
     __shared__ TC s_g[MAXI][NEN][D];
@@ -821,12 +820,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
-    // ---- phase A: pair columns and incidences ----
+    // ---- phase A: incidences ----
     if (staged) {
-        for (int s = threadIdx.x; s < q1 - q0; s += blockDim.x) {
-            s_pcol[s] = p.npat_col[q0 + s];
-            s_cnt[s] = 0;
-        }
+        for (int k = threadIdx.x; k < (q1 - q0) * CAP; k += blockDim.x) s_list[k / CAP][k % CAP] = 0xffffu;
         __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             int lo = 0, hi = ng;  // owner node of incidence i
@@ -879,19 +875,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
             }
             s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
             if (la < 0 || nloc > MAXLOC) continue;
-            // contribution of this incidence to each off-diagonal pair (a, ln[m]) of a's row
-            const int ps0 = s_qptr[lo] - q0, ps1 = s_qptr[lo + 1] - q0;
+            // contribution of this incidence to each off-diagonal pair (a, ln[m]) of
+            // a's row, at its precomputed slot and rank (ascending element order)
+            const uint64_t sw = p.inc_slot[ibase + i];
+            if (sw >> 63) { s_bad = 1; continue; }
+            const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
+            int k = 0;
             for (int m = 0; m < nloc; ++m) {
                 if (m == la) continue;
-                int l2 = ps0, h2 = ps1;  // first slot with column >= ln[m]
-                while (l2 < h2) {
-                    const int mid = (l2 + h2) >> 1;
-                    if (s_pcol[mid] < ln[m]) l2 = mid + 1; else h2 = mid;
-                }
-                if (l2 >= ps1 || s_pcol[l2] != ln[m]) { s_bad = 1; continue; }
-                const int k = atomicAdd(&s_cnt[l2], 1);
-                if (k < CAP) s_list[l2][k] = (unsigned short)((i << 4) | m);
-                else s_bad = 1;
+                const int f = int((sw >> (9 * k)) & 511u);
+                ++k;
+                const int slot = f & 63, rank = f >> 6;
+                if (slot >= np || rank >= CAP) { s_bad = 1; continue; }
+                s_list[ps0 + slot][rank] = (unsigned short)((i << 4) | m);
             }
         }
     }
@@ -906,7 +902,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
             if (s_qptr[mid] <= q) lo = mid; else hi = mid;
         }
         const int a = a0 + lo;
-        const int b = staged ? s_pcol[q - q0] : p.npat_col[q];
+        const int b = p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -933,25 +929,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
                 }
             }
         } else if (use_lists) {
-            // the slot's contributions, sorted by incidence = ascending element order
+            // the slot's contributions, stored by rank = ascending element order
             const int s = q - q0;
-            const int cnt = s_cnt[s];
-            unsigned code[CAP];
-#pragma unroll
-            for (int k = 0; k < CAP; ++k) code[k] = k < cnt ? s_list[s][k] : 0xffffu;
-#pragma unroll
-            for (int k = 1; k < CAP; ++k)
-#pragma unroll
-                for (int j = k; j > 0; --j) {
-                    const unsigned lo2 = min(code[j - 1], code[j]), hi2 = max(code[j - 1], code[j]);
-                    code[j - 1] = lo2;
-                    code[j] = hi2;
-                }
             if (jac) {
 #pragma unroll
                 for (int k = 0; k < CAP; ++k) {
-                    if (k >= cnt) break;
-                    const int t = int(code[k] >> 4), lb = int(code[k] & 15);
+                    const unsigned code = s_list[s][k];
+                    if (code == 0xffffu) continue;
+                    const int t = int(code >> 4), lb = int(code & 15);
                     const int info = s_info[t];
                     const int la = info & 255;
                     if (!(info & (1 << 16))) {
@@ -1289,6 +1274,44 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         const int c = s->cut_of_elem[e];
         r[s->nen] = c >= 0 ? c : -1 - s->elem_mat[e];
     }
+    // Static contribution slots of the node-group kernel.  For incidence t of
+    // node a (element e) and each other local node b of e, in local order: the
+    // slot of pair (a, b) within a's row of the node-pair pattern, and the rank
+    // of e among the incidences of a that contain b (= position in ascending
+    // element order).  9 bits each, packed from bit 0; bit 63 marks an
+    // incidence that does not fit (it then takes the global-memory path).
+    std::vector<uint64_t> slotv(size_t(n_inc), 0);
+    {
+        std::vector<int> cntv;
+        for (int64_t a = 0; a < nall; ++a) {
+            const int p0 = s->npat_ptr[a], p1 = s->npat_ptr[a + 1];
+            cntv.assign(size_t(p1 - p0), 0);
+            for (int64_t t = s->n2e_ptr[a]; t < s->n2e_ptr[a + 1]; ++t) {
+                const int e = s->n2e[t];
+                const int c = s->cut_of_elem[e];
+                int32_t ln[4 + PG_MAX_ENR];
+                int nloc = 0;
+                for (int n = 0; n < s->nen; ++n) ln[nloc++] = s->conn[int64_t(e) * s->nen + n];
+                if (c >= 0)
+                    for (int k = 0; k < s->cut[c].n_enr; ++k) ln[nloc++] = s->cut[c].enr[k];
+                uint64_t w = 0;
+                bool ok = nloc <= 8;
+                int k = 0;
+                for (int m = 0; m < nloc && ok; ++m) {
+                    if (ln[m] == a) continue;
+                    const int32_t* b0 = s->npat_col + p0;
+                    const int32_t* it = std::lower_bound(b0, s->npat_col + p1, ln[m]);
+                    const int slot = int(it - b0);
+                    if (it == s->npat_col + p1 || *it != ln[m]) { ok = false; break; }
+                    const int rank = cntv[size_t(slot)]++;
+                    if (slot >= 64 || rank >= 8) { ok = false; break; }
+                    w |= uint64_t(slot | (rank << 6)) << (9 * k);
+                    ++k;
+                }
+                slotv[size_t(t)] = ok ? w : (uint64_t(1) << 63);
+            }
+        }
+    }
     auto fail = [&](int code) { pg_asm_destroy(h); return code; };
 #define UP(field, src, n, T)                                   \
     do {                                                       \
@@ -1316,6 +1339,7 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
     UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
     UP(inc, incv.data(), incv.size(), int32_t);
+    UP(inc_slot, slotv.data(), slotv.size(), uint64_t);
     {
         std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
         for (int c = 0; c < s->n_cut; ++c) {
