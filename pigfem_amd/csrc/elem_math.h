@@ -104,6 +104,21 @@ __device__ inline void bcol(const T* g, int c, T* b) {
     }
 }
 
+// The same column as (row, value) pairs of its D nonzeros, rows ascending: the
+// products a dense column would add as exact zeros are skipped, which leaves
+// every sum unchanged.
+template <int D, typename T>
+__device__ inline void bnz(const T* g, int c, int* r, T* v) {
+    if constexpr (D == 2) {
+        if (c == 0) { r[0] = 0; v[0] = g[0]; r[1] = 2; v[1] = g[1]; }
+        else        { r[0] = 1; v[0] = g[1]; r[1] = 2; v[1] = g[0]; }
+    } else {
+        if (c == 0)      { r[0] = 0; v[0] = g[0]; r[1] = 4; v[1] = g[2]; r[2] = 5; v[2] = g[1]; }
+        else if (c == 1) { r[0] = 1; v[0] = g[1]; r[1] = 3; v[1] = g[2]; r[2] = 5; v[2] = g[0]; }
+        else             { r[0] = 2; v[0] = g[2]; r[1] = 3; v[1] = g[1]; r[2] = 4; v[2] = g[0]; }
+    }
+}
+
 // LinearElasticIsotropicMaterial::computeStress, src/material_lei.cpp:232-261.
 template <int D>
 __device__ inline void lei_stress(const double* Dm, const double* e, double* s) {

@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -517,15 +518,21 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     auto grad = [&](int q, int l, double* gr) {
         for (int a = 0; a < D; ++a) gr[a] = qg[g][q][l][a];
     };
-    // local-node-pair blocks: K(li, lj) = sum_q B_li^T (wJ D) B_lj, then the
-    // cohesive fold; one lane per block.  Staged block-major: the d x d block of
+    // local-node-pair blocks: K(li, lj) = V(li, lj) + C(li, lj) with the
+    // volumetric part V(li, lj) = sum_q B_li^T (wJ D) B_lj and the cohesive
+    // fold C.  D is symmetric, so V(lj, li) = V(li, lj)^T, and C(lj, li) =
+    // C(li, lj) (the fold scales R^T D R by N_li N_lj): one lane per unordered
+    // pair li <= lj writes both blocks.  Staged block-major: the d x d block of
     // (li, lj) is contiguous at (li * nloc + lj) * d * d.
     TK* Ke = Kst + p.cut_koff[ci];
-    for (int pb = gl; pb < nloc * nloc; pb += GS) {
-        const int li = pb / nloc, lj = pb % nloc;
-        TK acc[D][D];
+    const int nup = nloc * (nloc + 1) / 2;
+    for (int pb = gl; pb < nup; pb += GS) {
+        int li = 0, rem = pb;
+        while (rem >= nloc - li) { rem -= nloc - li; ++li; }
+        const int lj = li + rem;
+        TK vol[D][D];
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) acc[i][j] = TK(0);
+            for (int j = 0; j < D; ++j) vol[i][j] = TK(0);
         for (int q = 0; q < nvq; ++q) {
             double gid[D], gjd[D];
             grad(q, li, gid);
@@ -538,22 +545,27 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             const TK* Dm = sD[qm[g][q]];
             const TK w = TK(qw[g][q]);
             for (int i = 0; i < D; ++i) {
-                TK bi[NS], btd[NS];
-                bcol<D, TK>(gi, i, bi);
+                int ri[D];
+                TK vi[D], btd[NS];
+                bnz<D, TK>(gi, i, ri, vi);
                 for (int s = 0; s < NS; ++s) {
                     TK v = TK(0);
-                    for (int k = 0; k < NS; ++k) v += bi[k] * Dm[k * NS + s];
+                    for (int n = 0; n < D; ++n) v += vi[n] * Dm[ri[n] * NS + s];
                     btd[s] = v;
                 }
                 for (int j = 0; j < D; ++j) {
-                    TK bj[NS];
-                    bcol<D, TK>(gj, j, bj);
+                    int rj[D];
+                    TK vj[D];
+                    bnz<D, TK>(gj, j, rj, vj);
                     TK kt = TK(0);
-                    for (int s = 0; s < NS; ++s) kt += btd[s] * bj[s];
-                    acc[i][j] += w * kt;
+                    for (int n = 0; n < D; ++n) kt += btd[rj[n]] * vj[n];
+                    vol[i][j] += w * kt;
                 }
             }
         }
+        TK coh[D][D];
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) coh[i][j] = TK(0);
         for (int k = 0; k < rec.n_coh; ++k) {
             int m1 = -1, m2 = -1;
             for (int m = 0; m < rec.coh_nn; ++m) {
@@ -567,22 +579,28 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 const int cq = k * cnq + q;
                 const TK f = TK(cw[g][cq]) * sgn * TK(cN[g][cq][n1]) * TK(cN[g][cq][n2]);
                 for (int i = 0; i < D; ++i)
-                    for (int j = 0; j < D; ++j) acc[i][j] += f * TK(cRDR[g][cq][i][j]);
+                    for (int j = 0; j < D; ++j) coh[i][j] += f * TK(cRDR[g][cq][i][j]);
             }
         }
+        TK* B1 = Ke + int64_t(li * nloc + lj) * D * D;
+        TK* B2 = Ke + int64_t(lj * nloc + li) * D * D;
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) Ke[int64_t(pb) * D * D + i * D + j] = acc[i][j];
+            for (int j = 0; j < D; ++j) B1[i * D + j] = vol[i][j] + coh[i][j];
+        if (li != lj)
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) B2[i * D + j] = vol[j][i] + coh[i][j];
     }
     double* Pe = Pst + p.cut_poff[ci];
     for (int i = gl; i < nd; i += GS) {
         const int l = i / D, c = i % D;
         double a = 0.0;
         for (int q = 0; q < nvq; ++q) {
-            double b[NS], gr[D];
+            double gr[D], v[D];
+            int r[D];
             grad(q, l, gr);
-            bcol<D, double>(gr, c, b);
+            bnz<D, double>(gr, c, r, v);
             double pt = 0.0;
-            for (int s = 0; s < NS; ++s) pt += b[s] * qs[g][q][s];
+            for (int n = 0; n < D; ++n) pt += v[n] * qs[g][q][r[n]];
             a += qw[g][q] * pt;
         }
         double pc = 0.0;
@@ -721,11 +739,12 @@ template <int D, typename TC>
 __device__ inline void apply_left(const TC* L, const TC* gb, double acc[D][D]) {
     constexpr int NS = (D == 2 ? 3 : 6);
     for (int j = 0; j < D; ++j) {
-        TC bj[NS];
-        bcol<D, TC>(gb, j, bj);
+        int r[D];
+        TC v[D];
+        bnz<D, TC>(gb, j, r, v);
         for (int i = 0; i < D; ++i) {
             TC s = TC(0);
-            for (int k = 0; k < NS; ++k) s += L[i * NS + k] * bj[k];
+            for (int n = 0; n < D; ++n) s += L[i * NS + r[n]] * v[n];
             acc[i][j] += double(s);
         }
     }
@@ -735,11 +754,12 @@ template <int D, typename TC>
 __device__ inline void left_factor(const TC* ga, const TC* Dm, TC w, TC* L) {
     constexpr int NS = (D == 2 ? 3 : 6);
     for (int i = 0; i < D; ++i) {
-        TC bi[NS];
-        bcol<D, TC>(ga, i, bi);
+        int r[D];
+        TC v[D];
+        bnz<D, TC>(ga, i, r, v);
         for (int l = 0; l < NS; ++l) {
             TC s = TC(0);
-            for (int k = 0; k < NS; ++k) s += bi[k] * Dm[k * NS + l];
+            for (int n = 0; n < D; ++n) s += v[n] * Dm[r[n] * NS + l];
             L[i * NS + l] = w * s;
         }
     }
@@ -1049,6 +1069,20 @@ int check_desc(const pg_problem_desc* s) {
         return PG_E_ARG;
     }
     if (s->n_cut > 0 && (!s->cut || !s->cut_elem)) { g_err = "missing cut records"; return PG_E_ARG; }
+    if (s->n_mat <= 0 || s->n_mat > MAX_LDS_MATS) { g_err = "1 to 8 materials are supported"; return PG_E_UNSUPPORTED; }
+    for (int m = 0; m < s->n_mat; ++m) {
+        // the kernels use the major symmetry of the elasticity tensor (V(b, a) = V(a, b)^T)
+        if (s->mats[m].type != PG_MAT_LEI) continue;
+        const int ns = s->dim == 2 ? 3 : 6;
+        double mx = 0.0;
+        for (int k = 0; k < ns * ns; ++k) mx = std::max(mx, std::fabs(s->mats[m].D[k]));
+        for (int k = 0; k < ns; ++k)
+            for (int l = 0; l < k; ++l)
+                if (std::fabs(s->mats[m].D[k * ns + l] - s->mats[m].D[l * ns + k]) > 1e-12 * mx) {
+                    g_err = "material D matrix is not symmetric";
+                    return PG_E_UNSUPPORTED;
+                }
+    }
     for (int c = 0; c < s->n_cut; ++c) {
         const pg_cut_rec& r = s->cut[c];
         const int lim_n = s->dim == 2 ? MAXN2 : MAXN3, lim_q = s->dim == 2 ? MAXQ2 : MAXQ3;
@@ -1095,7 +1129,6 @@ int check_desc(const pg_problem_desc* s) {
     }
     for (int64_t k = 0; k < int64_t(s->n_elem) * s->nen; ++k)
         if (s->conn[k] < 0 || s->conn[k] >= s->n_nodes) { g_err = "connectivity out of range"; return PG_E_ARG; }
-    if (s->n_mat > MAX_LDS_MATS) { g_err = "at most 8 materials are supported"; return PG_E_UNSUPPORTED; }
     for (int m = 0; m < s->n_mat; ++m)
         if (s->mats[m].type == PG_MAT_LEI && s->mats[m].nstr != (s->dim == 2 ? 3 : 6)) { g_err = "bad D size"; return PG_E_ARG; }
     const int64_t nnzt = s->nnz[0] + s->nnz[1] + s->nnz[2] + s->nnz[3];
