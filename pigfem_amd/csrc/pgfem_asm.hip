@@ -750,6 +750,22 @@ __device__ inline void apply_left(const TC* L, const TC* gb, double acc[D][D]) {
     }
 }
 
+// out = L . B(gb) in TC: the d x d block that apply_left adds.
+template <int D, typename TC>
+__device__ inline void left_times(const TC* L, const TC* gb, TC* out) {
+    constexpr int NS = (D == 2 ? 3 : 6);
+    for (int j = 0; j < D; ++j) {
+        int r[D];
+        TC v[D];
+        bnz<D, TC>(gb, j, r, v);
+        for (int i = 0; i < D; ++i) {
+            TC s = TC(0);
+            for (int n = 0; n < D; ++n) s += L[i * NS + r[n]] * v[n];
+            out[i * D + j] = s;
+        }
+    }
+}
+
 template <int D, typename TC>
 __device__ inline void left_factor(const TC* ga, const TC* Dm, TC w, TC* L) {
     constexpr int NS = (D == 2 ? 3 : 6);
@@ -796,8 +812,236 @@ template <int D, int NG> struct GroupThreads {
 
 // OCC: waves per SIMD the register allocation must allow (1: no cap; 8 caps a
 // lane at 64 VGPRs so that the LDS tables, not registers, bound occupancy).
+// One node pair (a, b) from global memory: the incidences [t0, t1) of a in
+// element order, with the arithmetic of the LDS-staged paths (used by groups
+// whose incidences or pairs exceed the LDS tables).
+template <int D, typename TC>
+__device__ inline void pair_from_global(const DevProb& p, const double* __restrict__ Un, const TC* __restrict__ Kc,
+                                        const double* __restrict__ Pc, const TC (*sD)[D == 2 ? 9 : 36], double wsum,
+                                        int t0, int t1, int a, int b, bool jac, bool diag, double acc[D][D],
+                                        double pa[D]) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
+    for (int t = t0; t < t1; ++t) {
+        const int e = p.n2e[t];
+        const int ci = p.cut_of_elem[e];
+        const int la = find_local(p, e, ci, a, NEN);
+        const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
+        if (la < 0 || lb < 0) continue;
+        if (ci < 0) {
+            int nodes[NEN];
+            double x[NEN][3];
+            for (int n = 0; n < NEN; ++n) {
+                nodes[n] = p.conn[int64_t(e) * NEN + n];
+                for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
+            }
+            const int mi = p.elem_mat[e];
+            double g[NEN][D];
+            const double J = simplex_grads<D, double>(x, g);
+            const double w = J * wsum;
+            if (jac) {
+                TC ga[D], gb[D], L[D * NS];
+                for (int c = 0; c < D; ++c) { ga[c] = TC(g[la][c]); gb[c] = TC(g[lb][c]); }
+                left_factor<D, TC>(ga, sD[mi], TC(w), L);
+                apply_left<D, TC>(L, gb, acc);
+            }
+            if (diag) {
+                double pp[D];
+                plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, pp);
+                for (int i = 0; i < D; ++i) pa[i] += pp[i];
+            }
+        } else {
+            const int nl = NEN + p.cut[ci].n_enr;
+            if (jac) {
+                const TC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
+            }
+            if (diag)
+                for (int i = 0; i < D; ++i) pa[i] += Pc[p.cut_poff[ci] + la * D + i];
+        }
+    }
+}
+
+// Node-group kernel (tri3 meshes).
 template <int D, typename TC, int NG, int OCC, int BS = GroupThreads<D, NG>::value>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void k_pair_group(DevProb p, const double* __restrict__ Un,
+                                                    const TC* __restrict__ Kc, const double* __restrict__ Pc,
+                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
+    constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
+    constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
+    // contributions per off-diagonal pair: the elements sharing the edge (a, b),
+    // 2 in a tri3 mesh (cut elements included: an enrichment node lies on one
+    // cut edge) and up to 6 around a tet4 edge; the diagonal pair of a node takes
+    // one contribution per incidence
+    constexpr int CAPO = (D == 2 ? 2 : 8);
+    constexpr int CAPD = (D == 2 ? 8 : 24);
+    __shared__ int s_iptr[NG + 1];
+    __shared__ int s_qptr[NG + 1];
+    __shared__ TC s_blk[MAXP][CAPO][D * D];   // off-diagonal d x d blocks by pair slot and rank (0 when absent)
+    __shared__ TC s_dblk[NG][CAPD][D * D];    // diagonal blocks by node and incidence
+    __shared__ double s_dP[NG][CAPD][D];      // internal force by node and incidence
+    __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+    __shared__ int s_bad;
+
+    const int nall = p.n_nodes + p.n_enr;
+    const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
+    const int ng = min(NG, nall - a0);
+    const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
+        const int m = k / (NS * NS);
+        sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
+    }
+    if (threadIdx.x <= ng) {
+        s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
+        s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
+    }
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    const int ibase = s_iptr[0];
+    const int ni = s_iptr[ng] - ibase;
+    const int q0 = s_qptr[0], q1 = s_qptr[ng];
+    const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
+    double wsum = 0.0;
+    for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
+
+    // ---- phase A: every incidence computes its element's d x d blocks of the
+    // owner's row and stores them at their (slot, rank) -- ascending element
+    // order is the rank order, so phase B sums them in the reference order ----
+    if (staged) {
+        for (int k = threadIdx.x; k < (q1 - q0) * CAPO * D * D; k += blockDim.x)
+            s_blk[k / (CAPO * D * D)][(k / (D * D)) % CAPO][k % (D * D)] = TC(0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+            int lo = 0, hi = ng;  // owner node of incidence i
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
+            }
+            const int a = a0 + lo;
+            const int rd = i - (s_iptr[lo] - ibase);  // rank among a's incidences
+            const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
+            const uint64_t sw = p.inc_slot[ibase + i];
+            int ln[MAXLOC];
+            int la = -1;
+            for (int n = 0; n < NEN; ++n) {
+                ln[n] = rec[n];
+                if (ln[n] == a) la = n;
+            }
+            const int tag = rec[NEN];
+            if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
+            const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
+            if (tag < 0) {
+                double x[NEN][3];
+                for (int n = 0; n < NEN; ++n)
+                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                double g[NEN][D];
+                const double J = simplex_grads<D, double>(x, g);
+                const double w = J * wsum;
+                const int mi = -1 - tag;
+                if (la < 0) { s_bad = 1; continue; }
+                if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_dP[lo][rd]);
+                if (!jac) continue;
+                TC gt[NEN][D], L[D * NS];
+                for (int n = 0; n < NEN; ++n)
+                    for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
+                left_factor<D, TC>(gt[la], sD[mi], TC(w), L);
+                left_times<D, TC>(L, gt[la], s_dblk[lo][rd]);
+                int k = 0;
+                for (int m = 0; m < NEN; ++m) {
+                    if (m == la) continue;
+                    const int f = int((sw >> (9 * k)) & 511u);
+                    ++k;
+                    const int slot = f & 63, rank = f >> 6;
+                    if (slot >= np || rank >= CAPO) { s_bad = 1; continue; }
+                    left_times<D, TC>(L, gt[m], s_blk[ps0 + slot][rank]);
+                }
+            } else {
+                const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
+                const int4 c0 = cl[0], c1 = cl[1];  // n_enr, enr0..2 | enr3, P offset, K offset lo / hi
+                const int ne = c0.x;
+                const int nloc = NEN + ne;
+                if (nloc > MAXLOC) { s_bad = 1; continue; }  // local list does not fit: global path
+                const int en[4] = {c0.y, c0.z, c0.w, c1.x};
+                for (int k = 0; k < ne; ++k) {
+                    ln[NEN + k] = en[k];
+                    if (ln[NEN + k] == a) la = NEN + k;
+                }
+                if (la < 0) { s_bad = 1; continue; }
+                if (func)
+                    for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(c1.y) + la * D + c];
+                if (!jac) continue;
+                const TC* Ke = Kc + (int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32)) + int64_t(la * nloc) * D * D;
+                for (int k = 0; k < D * D; ++k) s_dblk[lo][rd][k] = Ke[la * D * D + k];
+                int k = 0;
+                for (int m = 0; m < nloc; ++m) {
+                    if (m == la) continue;
+                    const int f = int((sw >> (9 * k)) & 511u);
+                    ++k;
+                    const int slot = f & 63, rank = f >> 6;
+                    if (slot >= np || rank >= CAPO) { s_bad = 1; continue; }
+                    for (int kk = 0; kk < D * D; ++kk) s_blk[ps0 + slot][rank][kk] = Ke[m * D * D + kk];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const bool use_lists = staged && !s_bad;
+
+    // ---- phase B: node pairs ----
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        int lo = 0, hi = ng;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+        }
+        const int a = a0 + lo;
+        const int b = p.npat_col[q];
+        const bool diag = (a == b) && func;
+        double acc[D][D];
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
+        double pa[D];
+        for (int i = 0; i < D; ++i) pa[i] = 0.0;
+        if (use_lists && a == b) {
+            // diagonal pair: every incidence of a, in ascending element order
+            const int nd = s_iptr[lo + 1] - s_iptr[lo];
+            for (int r = 0; r < nd; ++r) {
+                if (func)
+                    for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][r][i];
+                if (jac)
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(s_dblk[lo][r][i * D + j]);
+            }
+        } else if (use_lists) {
+            // off-diagonal pair: its contributions by rank (absent ranks hold 0)
+            if (jac) {
+                const int s = q - q0;
+#pragma unroll
+                for (int r = 0; r < CAPO; ++r)
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[s][r][i * D + j]);
+            }
+        } else {
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sD, wsum, s_iptr[lo], s_iptr[lo + 1],
+                                    a, b, jac, diag, acc, pa);
+        }
+        if (jac) {
+            const int32_t* mp = p.npat_map + int64_t(q) * D * D;
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+        }
+        if (diag)
+            for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
+    }
+}
+
+// The same schedule for tet4 meshes, where the per-pair block tables would not
+// fit in LDS: phase A stores each incidence's gradients and left factor
+// L = w B_a^T D and registers (incidence, local node) codes at their static
+// (slot, rank); phase B multiplies out L B_b per contribution.
+template <int D, typename TC, int NG, int OCC, int BS = GroupThreads<D, NG>::value>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
@@ -970,47 +1214,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
                 }
             }
         } else {
-            // same arithmetic from global memory (group exceeds the LDS tables)
-            const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
-            for (int t = t0; t < t1; ++t) {
-                const int e = p.n2e[ibase + t];
-                const int ci = p.cut_of_elem[e];
-                const int la = find_local(p, e, ci, a, NEN);
-                const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
-                if (la < 0 || lb < 0) continue;
-                if (ci < 0) {
-                    int nodes[NEN];
-                    double x[NEN][3];
-                    for (int n = 0; n < NEN; ++n) {
-                        nodes[n] = p.conn[int64_t(e) * NEN + n];
-                        for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
-                    }
-                    const int mi = p.elem_mat[e];
-                    double g[NEN][D];
-                    const double J = simplex_grads<D, double>(x, g);
-                    const double w = J * wsum;
-                    if (jac) {
-                        TC ga[D], gb[D], L[D * NS];
-                        for (int c = 0; c < D; ++c) { ga[c] = TC(g[la][c]); gb[c] = TC(g[lb][c]); }
-                        left_factor<D, TC>(ga, sD[mi], TC(w), L);
-                        apply_left<D, TC>(L, gb, acc);
-                    }
-                    if (diag) {
-                        double pp[D];
-                        plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, pp);
-                        for (int i = 0; i < D; ++i) pa[i] += pp[i];
-                    }
-                } else {
-                    const int nl = NEN + p.cut[ci].n_enr;
-                    if (jac) {
-                        const TC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
-                        for (int i = 0; i < D; ++i)
-                            for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
-                    }
-                    if (diag)
-                        for (int i = 0; i < D; ++i) pa[i] += Pc[p.cut_poff[ci] + la * D + i];
-                }
-            }
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sD, wsum, s_iptr[lo], s_iptr[lo + 1],
+                                    a, b, jac, diag, acc, pa);
         }
         if (jac) {
             const int32_t* mp = p.npat_map + int64_t(q) * D * D;
@@ -1221,8 +1426,15 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
-#define GROUP1(DD, TC, KC, NGV, OCCV) \
-    k_pair_group<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>(d, h->Un, KC, h->Pc, dK, dP, flags)
+#define GROUP1(DD, TC, KC, NGV, OCCV)                                                                              \
+    do {                                                                                                        \
+        if constexpr (DD == 2)                                                                                  \
+            k_pair_group<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>( \
+                d, h->Un, KC, h->Pc, dK, dP, flags);                                                           \
+        else                                                                                                    \
+            k_pair_lists<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>( \
+                d, h->Un, KC, h->Pc, dK, dP, flags);                                                           \
+    } while (0)
 #define GROUP(DD, TC, KC, NGV) \
     do { if (h->group_variant == 1) GROUP1(DD, TC, KC, NGV, 8); else GROUP1(DD, TC, KC, NGV, 1); } while (0)
         if (d.dim == 2) {
