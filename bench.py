@@ -226,9 +226,10 @@ def main():
 
     # HBM traffic of the same kernel instantiation from the committed rocprofv3
     # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
-    ng, occ = tuned["mixed"][0], (8 if tuned["mixed"][2] == 1 else 1)
-    bs = min(256, max(64, (8 if prob.dim == 2 else 16) * ng))
-    kname = f"k_pair_group<{prob.dim}, float, {ng}, {occ}, {bs}>"
+    ng, tv, lists = tuned["mixed"][0], tuned["mixed"][2] & 1, prob.dim == 3 or bool(tuned["mixed"][2] & 2)
+    per = (8 if prob.dim == 2 else 16) * (2 if tv else 1) * ng
+    bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
+    kname = f"k_pair_{'lists' if lists else 'group'}<{prob.dim}, float, {ng}, {tv}, {bs}>"
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):

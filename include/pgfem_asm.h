@@ -215,8 +215,9 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *   cut_variant  small-class cut kernel: 0 = 4 elements per wavefront,
  *                1 = 4 per wavefront with a 3-waves/SIMD register cap,
  *                2 = 2 per wavefront, 3 = 8 per wavefront;
- *   group_variant node-group kernel: 0 = registers uncapped, 1 = capped for
- *                8 waves per SIMD.
+ *   group_variant node-group kernel: bit 0 clear = about one thread per node
+ *                pair (8 per node in 2D, 16 in 3D), set = twice as many; bit 1
+ *                set = the list form of the kernel in 2D (always used in 3D).
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

@@ -1,8 +1,8 @@
 """Autotuner of the class-binned schedules.
 
-Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block and
-register cap of the node-group kernel, then the variant of the small-class cut
-kernel (elements per wavefront, register cap) -- timing each kernel with HIP
+Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block, threads
+per node and form of the node-group kernel, then the variant of the small-class
+cut kernel (elements per wavefront, register cap) -- timing each kernel with HIP
 events on the assembly stream, and keeps the fastest.  Every choice gives
 bit-identical results.  Results are cached in a JSON file keyed by the problem
 class (dimension, cohesive law, size bucket) and the mode.
@@ -19,7 +19,7 @@ from .device import DeviceBuffer, synchronize
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 GROUPS = {2: (8, 16, 32), 3: (2, 4, 8)}
-GROUP_VARIANTS = (0, 1)
+GROUP_VARIANTS = {2: (0, 1, 2, 3), 3: (0, 1)}
 CUT_VARIANTS = (0, 1, 2, 3)
 
 
@@ -68,9 +68,9 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
 
     asm.set_timing(True)
     try:
-        # node-group kernel: nodes per block x register cap (timing index 2)
+        # node-group kernel: nodes per block x variant (timing index 2)
         for g in GROUPS[p.dim]:
-            for v in GROUP_VARIANTS:
+            for v in GROUP_VARIANTS[p.dim]:
                 table[f"group {g},{v}"] = timed(g, 0, v, 2)
                 if log:
                     log(f"[tune] group_nodes={g} group_variant={v}: {table[f'group {g},{v}']:.3f} ms")

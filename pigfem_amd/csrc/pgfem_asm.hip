@@ -805,13 +805,14 @@ __device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, d
     }
 }
 
-template <int D, int NG> struct GroupThreads {
-    // about one lane per node pair: ~8 pairs per node in 2D, ~16 in 3D
-    static constexpr int value = (D == 2 ? 8 : 16) * NG < 64 ? 64 : ((D == 2 ? 8 : 16) * NG > 256 ? 256 : (D == 2 ? 8 : 16) * NG);
+// Threads per node group: TV = 0 gives about one lane per node pair (8 per
+// node in 2D, 16 in 3D), TV = 1 twice that, so that a group's pairs fit in one
+// pass even where nodes have more pairs; whole wavefronts, at most 512.
+template <int D, int NG, int TV> struct GroupThreads {
+    static constexpr int per = (D == 2 ? 8 : 16) * (TV ? 2 : 1) * NG;
+    static constexpr int value = per < 64 ? 64 : (per > 512 ? 512 : (per + 63) / 64 * 64);
 };
 
-// OCC: waves per SIMD the register allocation must allow (1: no cap; 8 caps a
-// lane at 64 VGPRs so that the LDS tables, not registers, bound occupancy).
 // One node pair (a, b) from global memory: the incidences [t0, t1) of a in
 // element order, with the arithmetic of the LDS-staged paths (used by groups
 // whose incidences or pairs exceed the LDS tables).
@@ -863,8 +864,8 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
 }
 
 // Node-group kernel (tri3 meshes).
-template <int D, typename TC, int NG, int OCC, int BS = GroupThreads<D, NG>::value>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void k_pair_group(DevProb p, const double* __restrict__ Un,
+template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
+__global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
@@ -1040,8 +1041,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void 
 // fit in LDS: phase A stores each incidence's gradients and left factor
 // L = w B_a^T D and registers (incidence, local node) codes at their static
 // (slot, rank); phase B multiplies out L B_b per contribution.
-template <int D, typename TC, int NG, int OCC, int BS = GroupThreads<D, NG>::value>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC))) void k_pair_lists(DevProb p, const double* __restrict__ Un,
+template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
+__global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
@@ -1253,7 +1254,7 @@ struct pg_asm {
     int elem_block = 256;
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
-    int group_variant = 0; // node-group kernel register cap (0: none, 1: 8 waves / SIMD)
+    int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form of the kernel (2D)
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
 };
@@ -1365,6 +1366,21 @@ int alloc(pg_asm* h, T** p, size_t n) {
     return PG_OK;
 }
 
+// Node-group kernel launch: the block-table form for tri3 meshes unless the
+// list form is asked for; tet4 meshes always use the list form.
+template <int D, typename TC, int NG, int TV>
+void launch_group(bool lists, unsigned grid, hipStream_t st, const DevProb& d, const double* Un, const TC* Kc,
+                  const double* Pc, double* K, double* P, unsigned flags) {
+    constexpr int BS = GroupThreads<D, NG, TV>::value;
+    if constexpr (D == 2) {
+        if (!lists) {
+            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags);
+            return;
+        }
+    }
+    k_pair_lists<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags);
+}
+
 // Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
 // elements + staged node-pair reduction); PG_BATCHED -> class-binned FP64;
 // PG_FP32 (with or without PG_BATCHED) -> class-binned mixed precision.
@@ -1426,17 +1442,11 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
-#define GROUP1(DD, TC, KC, NGV, OCCV)                                                                              \
-    do {                                                                                                        \
-        if constexpr (DD == 2)                                                                                  \
-            k_pair_group<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>( \
-                d, h->Un, KC, h->Pc, dK, dP, flags);                                                           \
-        else                                                                                                    \
-            k_pair_lists<DD, TC, NGV, OCCV><<<unsigned((nall + NGV - 1) / NGV), GroupThreads<DD, NGV>::value, 0, st>>>( \
-                d, h->Un, KC, h->Pc, dK, dP, flags);                                                           \
-    } while (0)
+#define GROUP1(DD, TC, KC, NGV, TVV)                                                                          \
+    launch_group<DD, TC, NGV, TVV>((h->group_variant & 2) != 0, unsigned((nall + NGV - 1) / NGV), st, d, h->Un, KC, \
+                                   h->Pc, dK, dP, flags)
 #define GROUP(DD, TC, KC, NGV) \
-    do { if (h->group_variant == 1) GROUP1(DD, TC, KC, NGV, 8); else GROUP1(DD, TC, KC, NGV, 1); } while (0)
+    do { if (h->group_variant & 1) GROUP1(DD, TC, KC, NGV, 1); else GROUP1(DD, TC, KC, NGV, 0); } while (0)
         if (d.dim == 2) {
             if (fp32) {
                 if (ng == 8) GROUP(2, float, h->Kc32, 8);
@@ -1677,7 +1687,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 1) { g_err = "group variant must be 0 or 1"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 3) { g_err = "group variant must be 0..3"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;
