@@ -889,6 +889,24 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
     const int ng = min(NG, nall - a0);
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    // Issue the first incidence record, its slot word and the first node pair
+    // of this thread before the set-up barrier, so that their latency overlaps
+    // the pointer loads (the group's ranges come from wave-uniform loads).
+    static_assert(D == 2, "the block-table form is instantiated for tri3 meshes");
+    const int ib0 = p.n2e_ptr[a0], ni0 = p.n2e_ptr[a0 + ng] - ib0;
+    const int qa = p.npat_ptr[a0], qb = p.npat_ptr[a0 + ng];
+    int4 rec0 = make_int4(0, 0, 0, 0);
+    uint64_t sw0 = 0;
+    if (int(threadIdx.x) < ni0) {
+        rec0 = reinterpret_cast<const int4*>(p.inc)[ib0 + threadIdx.x];
+        sw0 = p.inc_slot[ib0 + threadIdx.x];
+    }
+    int col0 = -1;
+    int4 map0 = make_int4(0, 0, 0, 0);
+    if (qa + int(threadIdx.x) < qb) {
+        col0 = p.npat_col[qa + threadIdx.x];
+        map0 = reinterpret_cast<const int4*>(p.npat_map)[qa + threadIdx.x];
+    }
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
         const int m = k / (NS * NS);
         sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
@@ -921,15 +939,14 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             }
             const int a = a0 + lo;
             const int rd = i - (s_iptr[lo] - ibase);  // rank among a's incidences
-            const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
-            const uint64_t sw = p.inc_slot[ibase + i];
-            int ln[MAXLOC];
+            const bool first = i == int(threadIdx.x);
+            const int4 rec = first ? rec0 : reinterpret_cast<const int4*>(p.inc)[ibase + i];
+            const uint64_t sw = first ? sw0 : p.inc_slot[ibase + i];
+            int ln[MAXLOC] = {rec.x, rec.y, rec.z};
             int la = -1;
-            for (int n = 0; n < NEN; ++n) {
-                ln[n] = rec[n];
+            for (int n = 0; n < NEN; ++n)
                 if (ln[n] == a) la = n;
-            }
-            const int tag = rec[NEN];
+            const int tag = rec.w;
             if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
             if (tag < 0) {
@@ -997,7 +1014,8 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             if (s_qptr[mid] <= q) lo = mid; else hi = mid;
         }
         const int a = a0 + lo;
-        const int b = p.npat_col[q];
+        const bool firstq = q == q0 + int(threadIdx.x);
+        const int b = firstq ? col0 : p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -1028,9 +1046,11 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
-            const int32_t* mp = p.npat_map + int64_t(q) * D * D;
-            for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+            const int4 mp = firstq ? map0 : reinterpret_cast<const int4*>(p.npat_map)[q];
+            K[mp.x] = acc[0][0];
+            K[mp.y] = acc[0][1];
+            K[mp.z] = acc[1][0];
+            K[mp.w] = acc[1][1];
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
@@ -1068,6 +1088,24 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
     const int ng = min(NG, nall - a0);
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    // Issue the first incidence record, its slot word and the first node pair
+    // of this thread before the set-up barrier (see k_pair_group).
+    const int ib0 = p.n2e_ptr[a0], ni0 = p.n2e_ptr[a0 + ng] - ib0;
+    const int qa = p.npat_ptr[a0], qb = p.npat_ptr[a0 + ng];
+    int rec0[NEN + 1];
+    uint64_t sw0 = 0;
+    for (int n = 0; n <= NEN; ++n) rec0[n] = 0;
+    if (int(threadIdx.x) < ni0) {
+        for (int n = 0; n <= NEN; ++n) rec0[n] = p.inc[int64_t(ib0 + threadIdx.x) * (NEN + 1) + n];
+        sw0 = p.inc_slot[ib0 + threadIdx.x];
+    }
+    int col0 = -1;
+    int map0[D * D];
+    for (int k = 0; k < D * D; ++k) map0[k] = 0;
+    if (qa + int(threadIdx.x) < qb) {
+        col0 = p.npat_col[qa + threadIdx.x];
+        for (int k = 0; k < D * D; ++k) map0[k] = p.npat_map[int64_t(qa + threadIdx.x) * D * D + k];
+    }
     for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
         const int m = k / (NS * NS);
         sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
@@ -1096,14 +1134,15 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                 if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
             }
             const int a = a0 + lo;
+            const bool first = i == int(threadIdx.x);
             const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
             int ln[MAXLOC];
             int la = -1;
             for (int n = 0; n < NEN; ++n) {
-                ln[n] = rec[n];
+                ln[n] = first ? rec0[n] : rec[n];
                 if (ln[n] == a) la = n;
             }
-            const int tag = rec[NEN];
+            const int tag = first ? rec0[NEN] : rec[NEN];
             int nloc = NEN;
             if (tag < 0) {
                 double x[NEN][3];
@@ -1142,7 +1181,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             if (la < 0 || nloc > MAXLOC) continue;
             // contribution of this incidence to each off-diagonal pair (a, ln[m]) of
             // a's row, at its precomputed slot and rank (ascending element order)
-            const uint64_t sw = p.inc_slot[ibase + i];
+            const uint64_t sw = first ? sw0 : p.inc_slot[ibase + i];
             if (sw >> 63) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
             int k = 0;
@@ -1167,7 +1206,8 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             if (s_qptr[mid] <= q) lo = mid; else hi = mid;
         }
         const int a = a0 + lo;
-        const int b = p.npat_col[q];
+        const bool firstq = q == q0 + int(threadIdx.x);
+        const int b = firstq ? col0 : p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -1221,7 +1261,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         if (jac) {
             const int32_t* mp = p.npat_map + int64_t(q) * D * D;
             for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+                for (int j = 0; j < D; ++j) K[firstq ? map0[i * D + j] : mp[i * D + j]] = acc[i][j];
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
