@@ -2,18 +2,20 @@
 
 Restates PIGFEM's load stepping and Newton-Raphson loop so that the Newton
 history of the GPU assembly can be compared with the FP64 oracle's:
-  SolverNonlinear::solve            src/SolverNonlinear.cpp:70-385
+  SolverNonlinear::solve            src/SolverNonlinear.cpp:70-318
   SolverNonlinear::update_time_step src/SolverNonlinear.cpp:386-422
+  SolverNonlinear::explicit_solve   src/SolverNonlinear.cpp:430-510 (the last
+                                    resort after 20 failed repeats or dt < min)
   SolverNonlinearKSP::nonlinearStep src/SolverNonlinearKSP.cpp:55-155
   Assembler::assemble_new_load_step src/Assembler.cpp:394-511 (Dirichlet values
                                     scaled by the time fraction, no body loads)
-The linear solve of K_ff (MUMPS Cholesky through PETSc KSP in the reference,
-src/Solver.cpp:108-160) is a sparse LU (SuperLU via scipy) here.  The explicit
-fallback the reference enters after 20 failed repeats is not restated: the driver
-raises instead.
+The linear solve of K_ff (MUMPS through PETSc KSP in the reference,
+src/Solver.cpp:108-160) is a sparse direct LU: MKL PARDISO when the MKL runtime
+is present (nested-dissection ordering, multi-threaded), else SuperLU (scipy).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass, field
 
@@ -31,6 +33,7 @@ class StepRecord:
     converged: bool
     iterations: int
     residuals: list = field(default_factory=list)
+    explicit: bool = False
 
 
 class NewtonError(RuntimeError):
@@ -43,13 +46,94 @@ def _kff(problem, values):
     return sp.csr_matrix((values[:nnz], problem.ci[0], problem.rp[0]), shape=(n, n))
 
 
+_MKL_PATHS = ("/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1", "libmkl_rt.so")
+
+
+def _load_mkl():
+    for path in _MKL_PATHS:
+        try:
+            return C.CDLL(path)
+        except OSError:
+            continue
+    return None
+
+
+class PardisoLU:
+    """K_ff x = r with MKL PARDISO (real unsymmetric, zero-based CSR): the
+    symbolic analysis is done once for the fixed AIJ pattern, the numerical
+    factorization at every Newton iterate."""
+
+    def __init__(self, mkl, n, rp, ci):
+        self.mkl = mkl
+        self.n = n
+        self.ia = np.ascontiguousarray(rp, dtype=np.int32)
+        self.ja = np.ascontiguousarray(ci, dtype=np.int32)
+        self.pt = (C.c_void_p * 64)()
+        self.iparm = (C.c_int32 * 64)()
+        self.mtype = C.c_int32(11)
+        mkl.pardisoinit(self.pt, C.byref(self.mtype), self.iparm)
+        self.iparm[34] = 1            # zero-based indices
+        self.iparm[10] = 0            # no scaling and no weighted matching: both would
+        self.iparm[12] = 0            # make the one-time analysis depend on the values
+        self.a = np.zeros(len(self.ja))
+        self._call(11, self.a)
+
+    def _call(self, phase, a, b=None, x=None):
+        one, ph, nn = C.c_int32(1), C.c_int32(phase), C.c_int32(self.n)
+        msg, err = C.c_int32(0), C.c_int32(0)
+        bp = None if b is None else b.ctypes.data_as(C.c_void_p)
+        xp = None if x is None else x.ctypes.data_as(C.c_void_p)
+        self.mkl.pardiso(self.pt, C.byref(one), C.byref(one), C.byref(self.mtype), C.byref(ph), C.byref(nn),
+                         a.ctypes.data_as(C.c_void_p), self.ia.ctypes.data_as(C.c_void_p),
+                         self.ja.ctypes.data_as(C.c_void_p), None, C.byref(one), self.iparm, C.byref(msg),
+                         bp, xp, C.byref(err))
+        return err.value
+
+    def solve(self, values, r):
+        self.a = np.ascontiguousarray(values, dtype=np.float64)
+        if self._call(22, self.a) != 0:
+            raise RuntimeError("PARDISO factorization failed")
+        b = np.ascontiguousarray(r, dtype=np.float64)
+        x = np.zeros(self.n)
+        if self._call(33, self.a, b, x) != 0:
+            raise RuntimeError("PARDISO solve failed")
+        return x
+
+    def close(self):
+        if self.pt is not None:
+            self._call(-1, self.a)
+            self.pt = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LinearSolver:
+    def __init__(self, problem, prefer="pardiso"):
+        self.p = problem
+        mkl = _load_mkl() if prefer == "pardiso" else None
+        self.kind = "pardiso" if mkl is not None else "superlu"
+        self._pardiso = PardisoLU(mkl, problem.n_free, problem.rp[0], problem.ci[0]) if mkl is not None else None
+
+    def solve(self, K, r):
+        if self._pardiso is not None:
+            return self._pardiso.solve(K[: self.p.nnz[0]], r)
+        return spl.splu(_kff(self.p, K).tocsc()).solve(r)
+
+
 class NewtonDriver:
     """assemble(U, isv_committed) -> (K values [ff|fp|pf|pp], P [Pf|Pp], isv_trial)."""
 
     def __init__(self, problem, assemble, T_final=1.0, max_time_step=0.1, min_time_step=1e-5,
-                 rel_tol=1e-8, abs_tol=1e-8, max_iter=25, max_repeats=20, log=None):
+                 rel_tol=1e-8, abs_tol=1e-8, max_iter=25, max_repeats=20, explicit_steps=100,
+                 solver="pardiso", log=None):
         self.p = problem
         self.assemble = assemble
+        self.explicit_steps = explicit_steps
+        self.lin = LinearSolver(problem, solver)
         self.T_final, self.max_dt, self.min_dt = T_final, max_time_step, min_time_step
         self.rel_tol, self.abs_tol, self.max_iter, self.max_repeats = rel_tol, abs_tol, max_iter, max_repeats
         self.log = log
@@ -69,7 +153,7 @@ class NewtonDriver:
         nr, inc = 0, 0
         while True:
             try:
-                du = spl.splu(_kff(p, K).tocsc()).solve(res)
+                du = self.lin.solve(K, res)
             except RuntimeError:                     # singular factor: treat as divergence
                 return False, nr, hist, it, U
             U = U.copy()
@@ -90,17 +174,51 @@ class NewtonDriver:
             if inc >= 3:
                 return False, nr, hist, it, U
 
+    # SolverNonlinear::explicit_solve: n steps of dt = min_dt, one assembly and
+    # one linear solve each, history committed after every step
+    def _explicit(self, t, step):
+        p = self.p
+        dt = self.min_dt
+        n = self.explicit_steps
+        n_till_end = int((self.T_final - t) / self.min_dt + 0.5)
+        cont = True
+        if n_till_end < n:
+            n, cont = n_till_end, False
+        bc = np.asarray(p.bc_values)
+        for k in range(n):
+            U = self.U.copy()
+            U[p.n_free:] = ((t + dt) / self.T_final) * bc
+            K, P, it = self.assemble(U, self.isv)
+            res = -P[: p.n_free]
+            r = float(np.linalg.norm(res))
+            if not math.isfinite(r):
+                raise NewtonError("invalid residual norm during an explicit step")
+            U[: p.n_free] += self.lin.solve(K, res)
+            self.U = U
+            self.isv = it.copy()
+            t += dt
+            self.history.append(StepRecord(step, t, dt, 0, True, 1, [r], explicit=True))
+        if self.log:
+            self.log(f"[newton] explicit: {n} steps of dt={dt:.1e} to t={t:.6f}")
+        return t, cont
+
     def solve(self):
         p = self.p
         t, dt, step = 0.0, self.max_dt, 0
         bc = np.asarray(p.bc_values)
         while t < self.T_final:
-            repeats, converged = 0, False
+            repeats, converged, last_failed = 0, False, False
             while not converged:
                 if repeats == self.max_repeats or dt < self.min_dt:
-                    # the reference switches to its explicit solver here
-                    self.stopped = f"load step at t={t} needs the explicit fallback (dt={dt:.3e}, repeats={repeats})"
-                    raise NewtonError(self.stopped)
+                    if last_failed:
+                        self.stopped = f"load step at t={t} did not converge after the explicit steps"
+                        raise NewtonError(self.stopped)
+                    last_failed = True
+                    t, cont = self._explicit(t, step + 1)
+                    if not cont:
+                        break
+                    dt *= 4                                    # the step size before the failure
+                    dt = min(dt, self.T_final - t)
                 repeats += 1
                 U = self.U.copy()
                 U[p.n_free:] = ((t + dt) / self.T_final) * bc   # assemble_new_load_step
@@ -131,7 +249,8 @@ class NewtonDriver:
         return self.history
 
     def iteration_counts(self):
-        """Newton iterations of every attempted load step (repeats included)."""
+        """Newton iterations of every attempted load step (repeats included;
+        an explicit step counts as one iteration)."""
         return [(r.step, r.repeats, r.converged, r.iterations) for r in self.history]
 
 

@@ -33,17 +33,27 @@ def test_cohesive_newton_commits_history():
     assert np.all(d.isv >= 0) and d.isv.max() > 0      # delta_max committed
 
 
-def test_time_step_cut_and_explicit_fallback_signal():
-    """A step that cannot converge is repeated with dt * 0.25 until dt < min_dt,
-    where the reference enters its explicit solver (not restated: signalled)."""
+def test_time_step_cut_and_explicit_fallback():
+    """A step that cannot converge is repeated with dt * 0.25 until dt < min_dt;
+    then the explicit steps run (dt = min_dt, one solve each, history committed),
+    dt returns to 4x its last value, and a second failure ends the solve
+    (src/SolverNonlinear.cpp:132-176, 386-422, 430-510)."""
     cfg = dataclasses.replace(small_config("op", n=16, n_incl=2, seed=4), strain=0.02)
     p = RVEProblem(cfg)
     o = Oracle(p)
-    d = NewtonDriver(p, lambda U, isv: o.assemble(U, isv), max_iter=1, min_time_step=1e-3)
+    d = NewtonDriver(p, lambda U, isv: o.assemble(U, isv), max_iter=1, min_time_step=1e-3, explicit_steps=3)
     with pytest.raises(NewtonError):
         d.solve()
-    dts = [r.dt for r in d.history if not r.converged]
-    assert dts and all(b == pytest.approx(0.25 * a) for a, b in zip(dts, dts[1:]))
+    h = d.history
+    k = next(i for i, r in enumerate(h) if r.explicit)
+    dts = [r.dt for r in h[:k]]
+    assert not any(r.converged for r in h[:k])
+    assert all(b == pytest.approx(0.25 * a) for a, b in zip(dts, dts[1:])) and 0.25 * dts[-1] < 1e-3
+    ex = [r for r in h if r.explicit]
+    assert len(ex) == 3 and all(r.dt == 1e-3 and r.iterations == 1 for r in ex)
+    assert ex[-1].t == pytest.approx(3e-3)
+    after = h[k + 3]
+    assert not after.explicit and after.dt == pytest.approx(4 * 0.25 * dts[-1])
 
 
 def _perturbed_counts(p, eps, seed=0):
