@@ -119,6 +119,19 @@ __device__ inline void bnz(const T* g, int c, int* r, T* v) {
     }
 }
 
+// The d x d block w B_a^T D B_b for the isotropic D of an LEI material
+// (D = lambda 1 1^T + mu diag(2, .., 2, 1, .., 1) in Voigt order, the D that
+// LinearElasticIsotropicMaterial builds, src/material_lei.cpp:52-97):
+//   out[i][j] = w lambda ga_i gb_j + w mu ga_j gb_i + w mu (ga . gb) delta_ij,
+// with wl = w lambda and wm = w mu.
+template <int D, typename T>
+__device__ inline void iso_block(const T* ga, const T* gb, T wl, T wm, T (*out)[D]) {
+    T dot = T(0);
+    for (int k = 0; k < D; ++k) dot += ga[k] * gb[k];
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) out[i][j] = wl * ga[i] * gb[j] + wm * ga[j] * gb[i] + (i == j ? wm * dot : T(0));
+}
+
 // LinearElasticIsotropicMaterial::computeStress, src/material_lei.cpp:232-261.
 template <int D>
 __device__ inline void lei_stress(const double* Dm, const double* e, double* s) {

@@ -401,15 +401,16 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double cRDR[EPW][CUT_MAXC][D][D];
     __shared__ double ctR[EPW][CUT_MAXC][D];
     __shared__ pg_cut_rec recs[EPW];
-    __shared__ TK sD[MAX_LDS_MATS][NS * NS];
+    __shared__ TK sLM[MAX_LDS_MATS][2];  // lambda, mu of each LEI material
 
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const int li_ = xcd_order(blockIdx.x, gridDim.x) * EPW + g;
     const bool live = li_ < n_list;
     const int ci = live ? list[li_] : 0;
-    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += 64) {
-        const int m = k / (NS * NS);
-        sD[m][k % (NS * NS)] = m < p.n_mat ? TK(p.mats[m].D[k % (NS * NS)]) : TK(0);
+    if (threadIdx.x < MAX_LDS_MATS) {
+        const int m = threadIdx.x;
+        sLM[m][0] = m < p.n_mat ? TK(p.mats[m].D[1]) : TK(0);
+        sLM[m][1] = m < p.n_mat ? TK(p.mats[m].D[NS * NS - 1]) : TK(0);
     }
     constexpr int RW = int(sizeof(pg_cut_rec) / 4);
     for (int k = gl; k < RW; k += GS)
@@ -519,7 +520,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         for (int a = 0; a < D; ++a) gr[a] = qg[g][q][l][a];
     };
     // local-node-pair blocks: K(li, lj) = V(li, lj) + C(li, lj) with the
-    // volumetric part V(li, lj) = sum_q B_li^T (wJ D) B_lj and the cohesive
+    // volumetric part V(li, lj) = sum_q B_li^T (wJ D) B_lj (isotropic closed
+    // form) and the cohesive
     // fold C.  D is symmetric, so V(lj, li) = V(li, lj)^T, and C(lj, li) =
     // C(li, lj) (the fold scales R^T D R by N_li N_lj): one lane per unordered
     // pair li <= lj writes both blocks.  Staged block-major: the d x d block of
@@ -542,26 +544,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             if (zi || zj) continue;
             TK gi[D], gj[D];
             for (int a = 0; a < D; ++a) { gi[a] = TK(gid[a]); gj[a] = TK(gjd[a]); }
-            const TK* Dm = sD[qm[g][q]];
             const TK w = TK(qw[g][q]);
-            for (int i = 0; i < D; ++i) {
-                int ri[D];
-                TK vi[D], btd[NS];
-                bnz<D, TK>(gi, i, ri, vi);
-                for (int s = 0; s < NS; ++s) {
-                    TK v = TK(0);
-                    for (int n = 0; n < D; ++n) v += vi[n] * Dm[ri[n] * NS + s];
-                    btd[s] = v;
-                }
-                for (int j = 0; j < D; ++j) {
-                    int rj[D];
-                    TK vj[D];
-                    bnz<D, TK>(gj, j, rj, vj);
-                    TK kt = TK(0);
-                    for (int n = 0; n < D; ++n) kt += btd[rj[n]] * vj[n];
-                    vol[i][j] += w * kt;
-                }
-            }
+            TK blk[D][D];
+            iso_block<D, TK>(gi, gj, w * sLM[qm[g][q]][0], w * sLM[qm[g][q]][1], blk);
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) vol[i][j] += blk[i][j];
         }
         TK coh[D][D];
         for (int i = 0; i < D; ++i)
@@ -734,53 +721,6 @@ __device__ inline T simplex_grads(const double (*x)[3], T (*g)[D]) {
     }
 }
 
-// acc += L . B(gb), L = w B_a^T D (D x NS row-major)
-template <int D, typename TC>
-__device__ inline void apply_left(const TC* L, const TC* gb, double acc[D][D]) {
-    constexpr int NS = (D == 2 ? 3 : 6);
-    for (int j = 0; j < D; ++j) {
-        int r[D];
-        TC v[D];
-        bnz<D, TC>(gb, j, r, v);
-        for (int i = 0; i < D; ++i) {
-            TC s = TC(0);
-            for (int n = 0; n < D; ++n) s += L[i * NS + r[n]] * v[n];
-            acc[i][j] += double(s);
-        }
-    }
-}
-
-// out = L . B(gb) in TC: the d x d block that apply_left adds.
-template <int D, typename TC>
-__device__ inline void left_times(const TC* L, const TC* gb, TC* out) {
-    constexpr int NS = (D == 2 ? 3 : 6);
-    for (int j = 0; j < D; ++j) {
-        int r[D];
-        TC v[D];
-        bnz<D, TC>(gb, j, r, v);
-        for (int i = 0; i < D; ++i) {
-            TC s = TC(0);
-            for (int n = 0; n < D; ++n) s += L[i * NS + r[n]] * v[n];
-            out[i * D + j] = s;
-        }
-    }
-}
-
-template <int D, typename TC>
-__device__ inline void left_factor(const TC* ga, const TC* Dm, TC w, TC* L) {
-    constexpr int NS = (D == 2 ? 3 : 6);
-    for (int i = 0; i < D; ++i) {
-        int r[D];
-        TC v[D];
-        bnz<D, TC>(ga, i, r, v);
-        for (int l = 0; l < NS; ++l) {
-            TC s = TC(0);
-            for (int n = 0; n < D; ++n) s += v[n] * Dm[r[n] * NS + l];
-            L[i * NS + l] = w * s;
-        }
-    }
-}
-
 // FP64 internal-force contribution w B_la^T sigma of an uncut element.
 template <int D>
 __device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, double w, const double* Dm,
@@ -818,10 +758,10 @@ template <int D, int NG, int TV> struct GroupThreads {
 // whose incidences or pairs exceed the LDS tables).
 template <int D, typename TC>
 __device__ inline void pair_from_global(const DevProb& p, const double* __restrict__ Un, const TC* __restrict__ Kc,
-                                        const double* __restrict__ Pc, const TC (*sD)[D == 2 ? 9 : 36], double wsum,
+                                        const double* __restrict__ Pc, const TC (*sLM)[2], double wsum,
                                         int t0, int t1, int a, int b, bool jac, bool diag, double acc[D][D],
                                         double pa[D]) {
-    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
+    constexpr int NEN = D + 1;
     for (int t = t0; t < t1; ++t) {
         const int e = p.n2e[t];
         const int ci = p.cut_of_elem[e];
@@ -840,10 +780,11 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
             const double J = simplex_grads<D, double>(x, g);
             const double w = J * wsum;
             if (jac) {
-                TC ga[D], gb[D], L[D * NS];
+                TC ga[D], gb[D], blk[D][D];
                 for (int c = 0; c < D; ++c) { ga[c] = TC(g[la][c]); gb[c] = TC(g[lb][c]); }
-                left_factor<D, TC>(ga, sD[mi], TC(w), L);
-                apply_left<D, TC>(L, gb, acc);
+                iso_block<D, TC>(ga, gb, TC(w) * sLM[mi][0], TC(w) * sLM[mi][1], blk);
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
             }
             if (diag) {
                 double pp[D];
@@ -882,7 +823,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ TC s_blk[MAXP][CAPO][D * D];   // off-diagonal d x d blocks by pair slot and rank (0 when absent)
     __shared__ TC s_dblk[NG][CAPD][D * D];    // diagonal blocks by node and incidence
     __shared__ double s_dP[NG][CAPD][D];      // internal force by node and incidence
-    __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+    __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
 
     const int nall = p.n_nodes + p.n_enr;
@@ -907,9 +848,10 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
         col0 = p.npat_col[qa + threadIdx.x];
         map0 = reinterpret_cast<const int4*>(p.npat_map)[qa + threadIdx.x];
     }
-    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
-        const int m = k / (NS * NS);
-        sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
+    if (threadIdx.x < MAX_LDS_MATS) {
+        const int m = threadIdx.x;
+        sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
+        sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
     if (threadIdx.x <= ng) {
         s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
@@ -960,11 +902,11 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                 if (la < 0) { s_bad = 1; continue; }
                 if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_dP[lo][rd]);
                 if (!jac) continue;
-                TC gt[NEN][D], L[D * NS];
+                TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
-                left_factor<D, TC>(gt[la], sD[mi], TC(w), L);
-                left_times<D, TC>(L, gt[la], s_dblk[lo][rd]);
+                const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+                iso_block<D, TC>(gt[la], gt[la], wl, wm, reinterpret_cast<TC (*)[D]>(s_dblk[lo][rd]));
                 int k = 0;
                 for (int m = 0; m < NEN; ++m) {
                     if (m == la) continue;
@@ -972,7 +914,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     ++k;
                     const int slot = f & 63, rank = f >> 6;
                     if (slot >= np || rank >= CAPO) { s_bad = 1; continue; }
-                    left_times<D, TC>(L, gt[m], s_blk[ps0 + slot][rank]);
+                    iso_block<D, TC>(gt[la], gt[m], wl, wm, reinterpret_cast<TC (*)[D]>(s_blk[ps0 + slot][rank]));
                 }
             } else {
                 const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
@@ -1042,7 +984,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                         for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[s][r][i * D + j]);
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sD, wsum, s_iptr[lo], s_iptr[lo + 1],
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_iptr[lo], s_iptr[lo + 1],
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
@@ -1079,9 +1021,9 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ int s_info[MAXI];             // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: absent
     __shared__ int64_t s_koff[MAXI];
     __shared__ TC s_g[MAXI][NEN][D];
-    __shared__ TC s_L[MAXI][D * NS];
+    __shared__ TC s_wlm[MAXI][2];            // w lambda, w mu of each uncut incidence
     __shared__ double s_P[MAXI][D];
-    __shared__ TC sD[MAX_LDS_MATS][NS * NS];
+    __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
 
     const int nall = p.n_nodes + p.n_enr;
@@ -1106,9 +1048,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         col0 = p.npat_col[qa + threadIdx.x];
         for (int k = 0; k < D * D; ++k) map0[k] = p.npat_map[int64_t(qa + threadIdx.x) * D * D + k];
     }
-    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
-        const int m = k / (NS * NS);
-        sD[m][k % (NS * NS)] = m < p.n_mat ? TC(p.mats[m].D[k % (NS * NS)]) : TC(0);
+    if (threadIdx.x < MAX_LDS_MATS) {
+        const int m = threadIdx.x;
+        sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
+        sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
     if (threadIdx.x <= ng) {
         s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
@@ -1156,7 +1099,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
                 const int mi = -1 - tag;
                 if (la >= 0) {
-                    if (jac) left_factor<D, TC>(gt[la], sD[mi], TC(w), s_L[i]);
+                    if (jac) {
+                        s_wlm[i][0] = TC(w) * sLM[mi][0];
+                        s_wlm[i][1] = TC(w) * sLM[mi][1];
+                    }
                     if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
                 }
             } else {
@@ -1225,7 +1171,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
                 if (!jac) continue;
                 if (!(info & (1 << 16))) {
-                    apply_left<D, TC>(s_L[t], s_g[t][la], acc);
+                    TC blk[D][D];
+                    iso_block<D, TC>(s_g[t][la], s_g[t][la], s_wlm[t][0], s_wlm[t][1], blk);
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                 } else {
                     const int nl = (info >> 8) & 255;
                     const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
@@ -1245,7 +1194,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     const int info = s_info[t];
                     const int la = info & 255;
                     if (!(info & (1 << 16))) {
-                        apply_left<D, TC>(s_L[t], s_g[t][lb], acc);
+                        TC blk[D][D];
+                        iso_block<D, TC>(s_g[t][la], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                     } else {
                         const int nl = (info >> 8) & 255;
                         const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
@@ -1255,7 +1207,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                 }
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sD, wsum, s_iptr[lo], s_iptr[lo + 1],
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_iptr[lo], s_iptr[lo + 1],
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
@@ -1317,17 +1269,23 @@ int check_desc(const pg_problem_desc* s) {
     if (s->n_cut > 0 && (!s->cut || !s->cut_elem)) { g_err = "missing cut records"; return PG_E_ARG; }
     if (s->n_mat <= 0 || s->n_mat > MAX_LDS_MATS) { g_err = "1 to 8 materials are supported"; return PG_E_UNSUPPORTED; }
     for (int m = 0; m < s->n_mat; ++m) {
-        // the kernels use the major symmetry of the elasticity tensor (V(b, a) = V(a, b)^T)
+        // LEI materials are isotropic (LinearElasticIsotropicMaterial): the
+        // binned kernels use the closed form of B_a^T D B_b in lambda = D[0][1]
+        // and mu = D[last][last], so D must have the isotropic pattern
         if (s->mats[m].type != PG_MAT_LEI) continue;
-        const int ns = s->dim == 2 ? 3 : 6;
+        const int ns = s->dim == 2 ? 3 : 6, nd = s->dim;
+        const double* Dm = s->mats[m].D;
+        const double lam = Dm[1], mu = Dm[ns * ns - 1];
         double mx = 0.0;
-        for (int k = 0; k < ns * ns; ++k) mx = std::max(mx, std::fabs(s->mats[m].D[k]));
+        for (int k = 0; k < ns * ns; ++k) mx = std::max(mx, std::fabs(Dm[k]));
         for (int k = 0; k < ns; ++k)
-            for (int l = 0; l < k; ++l)
-                if (std::fabs(s->mats[m].D[k * ns + l] - s->mats[m].D[l * ns + k]) > 1e-12 * mx) {
-                    g_err = "material D matrix is not symmetric";
+            for (int l = 0; l < ns; ++l) {
+                const double iso = (k < nd && l < nd) ? lam + (k == l ? 2.0 * mu : 0.0) : (k == l ? mu : 0.0);
+                if (std::fabs(Dm[k * ns + l] - iso) > 1e-12 * mx) {
+                    g_err = "LEI material D matrix is not isotropic";
                     return PG_E_UNSUPPORTED;
                 }
+            }
     }
     for (int c = 0; c < s->n_cut; ++c) {
         const pg_cut_rec& r = s->cut[c];
