@@ -124,12 +124,25 @@ __device__ inline void bnz(const T* g, int c, int* r, T* v) {
 // LinearElasticIsotropicMaterial builds, src/material_lei.cpp:52-97):
 //   out[i][j] = w lambda ga_i gb_j + w mu ga_j gb_i + w mu (ga . gb) delta_ij,
 // with wl = w lambda and wm = w mu.
+// Written with explicit fused multiply-adds so that every kernel that inlines
+// it rounds the same way (bit-identical results across the schedules).
+template <typename T>
+__device__ inline T fused(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
 template <int D, typename T>
 __device__ inline void iso_block(const T* ga, const T* gb, T wl, T wm, T (*out)[D]) {
-    T dot = T(0);
-    for (int k = 0; k < D; ++k) dot += ga[k] * gb[k];
-    for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j) out[i][j] = wl * ga[i] * gb[j] + wm * ga[j] * gb[i] + (i == j ? wm * dot : T(0));
+    T dot = ga[0] * gb[0];
+    for (int k = 1; k < D; ++k) dot = fused(ga[k], gb[k], dot);
+    const T wd = wm * dot;
+    for (int i = 0; i < D; ++i) {
+        const T t = wl * ga[i];
+        for (int j = 0; j < D; ++j) {
+            const T u = wm * ga[j];
+            out[i][j] = fused(t, gb[j], fused(u, gb[i], i == j ? wd : T(0)));
+        }
+    }
 }
 
 // LinearElasticIsotropicMaterial::computeStress, src/material_lei.cpp:232-261.
