@@ -202,11 +202,15 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
 int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total,
                  int64_t* n_cqp);
 
-/* Per-launch timing of the last pg_asm_assemble*_timed call, in ms, measured
- * with HIP events on the stream the kernels run on.  kernel_ms[0] is the
- * whole assembly, [1] the cut-element (or element) kernel, [2] the scatter /
- * node-pair kernel. */
-int pg_asm_last_timing(const pg_asm* h, double kernel_ms[3]);
+/* Timing of the last assembly while pg_asm_set_timing is on, in ms, measured
+ * with HIP events on the streams the kernels run on:
+ *   kernel_ms[0] the whole assembly,
+ *   kernel_ms[1] U permutation + element (plain) or cut-element (binned) kernels,
+ *   kernel_ms[2] the node-pair kernels ([3] + [4]),
+ *   kernel_ms[3] binned: the node groups without cut incidences, which run on
+ *                the handle's second stream during the cut kernels (0 if none),
+ *   kernel_ms[4] the remaining node groups (plain: the staged node-pair kernel). */
+int pg_asm_last_timing(const pg_asm* h, double kernel_ms[5]);
 int pg_asm_set_timing(pg_asm* h, int enable);
 
 /* Schedule parameters of the class-binned schedules, chosen by the autotuner:
@@ -217,7 +221,10 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                2 = 2 per wavefront, 3 = 8 per wavefront;
  *   group_variant node-group kernel: bit 0 clear = about one thread per node
  *                pair (8 per node in 2D, 16 in 3D), set = twice as many; bit 1
- *                set = the list form of the kernel in 2D (always used in 3D).
+ *                set = the list form of the kernel in 2D (always used in 3D);
+ *                bit 2 set = no overlap (all node groups after the cut kernels
+ *                on one stream; by default the groups of nodes without cut
+ *                incidences run on a second stream during the cut kernels).
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

@@ -144,7 +144,9 @@ class AssemblerGPU:
         check(lib().pg_asm_set_timing(self._h, 1 if on else 0), "pg_asm_set_timing")
 
     def last_timing(self):
-        ms = (C.c_double * 3)()
+        """[total, permutation + cut / element kernels, node-pair kernels,
+        overlapped node groups, remaining node groups] in ms."""
+        ms = (C.c_double * 5)()
         check(lib().pg_asm_last_timing(self._h, ms), "pg_asm_last_timing")
         return list(ms)
 

@@ -19,7 +19,7 @@ from .device import DeviceBuffer, synchronize
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 GROUPS = {2: (8, 16, 32), 3: (2, 4, 8)}
-GROUP_VARIANTS = {2: (0, 1, 2, 3), 3: (0, 1)}
+GROUP_VARIANTS = {2: (0, 1, 2, 3, 4, 6), 3: (0, 1, 4, 5)}
 CUT_VARIANTS = (0, 1, 2, 3)
 
 
@@ -68,17 +68,18 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
 
     asm.set_timing(True)
     try:
-        # node-group kernel: nodes per block x variant (timing index 2)
+        # node-group kernel: nodes per block x variant, timed on the whole
+        # assembly (groups of nodes without cut incidences overlap the cut kernel)
         for g in GROUPS[p.dim]:
             for v in GROUP_VARIANTS[p.dim]:
-                table[f"group {g},{v}"] = timed(g, 0, v, 2)
+                table[f"group {g},{v}"] = timed(g, 0, v, 0)
                 if log:
                     log(f"[tune] group_nodes={g} group_variant={v}: {table[f'group {g},{v}']:.3f} ms")
         gbest = min((k for k in table if k.startswith("group")), key=table.get)
         g, v = (int(x) for x in gbest.split()[1].split(","))
-        # cut-element kernel variants (timing index 1: permutation + cut kernels)
+        # cut-element kernel variants, timed on the whole assembly
         for c in CUT_VARIANTS:
-            table[f"cut {c}"] = timed(g, c, v, 1)
+            table[f"cut {c}"] = timed(g, c, v, 0)
             if log:
                 log(f"[tune] cut_variant={c}: {table[f'cut {c}']:.3f} ms")
         c = int(min((k for k in table if k.startswith("cut")), key=table.get).split()[1])

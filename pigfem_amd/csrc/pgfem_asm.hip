@@ -61,6 +61,7 @@ struct DevProb {
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
     const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
     const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
+    const int32_t* norder;     // node order of the node-group kernel: nodes without cut incidences, then the others
     int n_mat;
 };
 
@@ -808,7 +809,8 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
 template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
 __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
-                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags) {
+                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags,
+                                                    int list_base, int list_len) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -818,51 +820,46 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     // one contribution per incidence
     constexpr int CAPO = (D == 2 ? 2 : 8);
     constexpr int CAPD = (D == 2 ? 8 : 24);
-    __shared__ int s_iptr[NG + 1];
-    __shared__ int s_qptr[NG + 1];
+    __shared__ int s_node[NG];               // the group's nodes (from the node order)
+    __shared__ int s_ibeg[NG], s_qbeg[NG];   // their first incidence and first node pair
+    __shared__ int s_iptr[NG + 1];           // local prefix of incidences per node
+    __shared__ int s_qptr[NG + 1];           // local prefix of node pairs per node
     __shared__ TC s_blk[MAXP][CAPO][D * D];   // off-diagonal d x d blocks by pair slot and rank (0 when absent)
     __shared__ TC s_dblk[NG][CAPD][D * D];    // diagonal blocks by node and incidence
     __shared__ double s_dP[NG][CAPD][D];      // internal force by node and incidence
     __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
 
-    const int nall = p.n_nodes + p.n_enr;
-    const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
-    const int ng = min(NG, nall - a0);
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    // Issue the first incidence record, its slot word and the first node pair
-    // of this thread before the set-up barrier, so that their latency overlaps
-    // the pointer loads (the group's ranges come from wave-uniform loads).
-    static_assert(D == 2, "the block-table form is instantiated for tri3 meshes");
-    const int ib0 = p.n2e_ptr[a0], ni0 = p.n2e_ptr[a0 + ng] - ib0;
-    const int qa = p.npat_ptr[a0], qb = p.npat_ptr[a0 + ng];
-    int4 rec0 = make_int4(0, 0, 0, 0);
-    uint64_t sw0 = 0;
-    if (int(threadIdx.x) < ni0) {
-        rec0 = reinterpret_cast<const int4*>(p.inc)[ib0 + threadIdx.x];
-        sw0 = p.inc_slot[ib0 + threadIdx.x];
-    }
-    int col0 = -1;
-    int4 map0 = make_int4(0, 0, 0, 0);
-    if (qa + int(threadIdx.x) < qb) {
-        col0 = p.npat_col[qa + threadIdx.x];
-        map0 = reinterpret_cast<const int4*>(p.npat_map)[qa + threadIdx.x];
-    }
+    const int k0 = xcd_order(blockIdx.x, gridDim.x) * NG;
+    const int ng = min(NG, list_len - k0);
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
         sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
-    if (threadIdx.x <= ng) {
-        s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
-        s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
+    if (int(threadIdx.x) < ng) {
+        const int a = p.norder[list_base + k0 + threadIdx.x];
+        const int ib = p.n2e_ptr[a], qb = p.npat_ptr[a];
+        s_node[threadIdx.x] = a;
+        s_ibeg[threadIdx.x] = ib;
+        s_qbeg[threadIdx.x] = qb;
+        s_iptr[threadIdx.x + 1] = p.n2e_ptr[a + 1] - ib;
+        s_qptr[threadIdx.x + 1] = p.npat_ptr[a + 1] - qb;
     }
     if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
-    const int ibase = s_iptr[0];
-    const int ni = s_iptr[ng] - ibase;
-    const int q0 = s_qptr[0], q1 = s_qptr[ng];
-    const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
+    if (threadIdx.x == 0) {
+        s_iptr[0] = 0;
+        s_qptr[0] = 0;
+        for (int k = 0; k < ng; ++k) {
+            s_iptr[k + 1] += s_iptr[k];
+            s_qptr[k + 1] += s_qptr[k];
+        }
+    }
+    __syncthreads();
+    const int ni = s_iptr[ng], nq = s_qptr[ng];
+    const bool staged = ni <= MAXI && nq <= MAXP;
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
@@ -870,27 +867,27 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     // owner's row and stores them at their (slot, rank) -- ascending element
     // order is the rank order, so phase B sums them in the reference order ----
     if (staged) {
-        for (int k = threadIdx.x; k < (q1 - q0) * CAPO * D * D; k += blockDim.x)
+        for (int k = threadIdx.x; k < nq * CAPO * D * D; k += blockDim.x)
             s_blk[k / (CAPO * D * D)][(k / (D * D)) % CAPO][k % (D * D)] = TC(0);
         __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             int lo = 0, hi = ng;  // owner node of incidence i
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
+                if (s_iptr[mid] <= i) lo = mid; else hi = mid;
             }
-            const int a = a0 + lo;
-            const int rd = i - (s_iptr[lo] - ibase);  // rank among a's incidences
-            const bool first = i == int(threadIdx.x);
-            const int4 rec = first ? rec0 : reinterpret_cast<const int4*>(p.inc)[ibase + i];
-            const uint64_t sw = first ? sw0 : p.inc_slot[ibase + i];
+            const int a = s_node[lo];
+            const int tg = s_ibeg[lo] + (i - s_iptr[lo]);  // global incidence index
+            const int rd = i - s_iptr[lo];  // rank among a's incidences
+            const int4 rec = reinterpret_cast<const int4*>(p.inc)[tg];
+            const uint64_t sw = p.inc_slot[tg];
             int ln[MAXLOC] = {rec.x, rec.y, rec.z};
             int la = -1;
             for (int n = 0; n < NEN; ++n)
                 if (ln[n] == a) la = n;
             const int tag = rec.w;
             if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
-            const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
+            const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             if (tag < 0) {
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
@@ -949,15 +946,15 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     const bool use_lists = staged && !s_bad;
 
     // ---- phase B: node pairs ----
-    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    for (int kq = threadIdx.x; kq < nq; kq += blockDim.x) {
         int lo = 0, hi = ng;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
         }
-        const int a = a0 + lo;
-        const bool firstq = q == q0 + int(threadIdx.x);
-        const int b = firstq ? col0 : p.npat_col[q];
+        const int a = s_node[lo];
+        const int q = s_qbeg[lo] + (kq - s_qptr[lo]);  // global node-pair index
+        const int b = p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -977,18 +974,18 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
         } else if (use_lists) {
             // off-diagonal pair: its contributions by rank (absent ranks hold 0)
             if (jac) {
-                const int s = q - q0;
+                const int s = kq;
 #pragma unroll
                 for (int r = 0; r < CAPO; ++r)
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[s][r][i * D + j]);
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_iptr[lo], s_iptr[lo + 1],
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
-            const int4 mp = firstq ? map0 : reinterpret_cast<const int4*>(p.npat_map)[q];
+            const int4 mp = reinterpret_cast<const int4*>(p.npat_map)[q];
             K[mp.x] = acc[0][0];
             K[mp.y] = acc[0][1];
             K[mp.z] = acc[1][0];
@@ -1006,7 +1003,8 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
 template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
 __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
-                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags) {
+                                                    double* __restrict__ K, double* __restrict__ P, unsigned flags,
+                                                    int list_base, int list_len) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -1015,8 +1013,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     // enrichment pairs); the diagonal pair reads the node's incidences directly
     constexpr int CAP = (D == 2 ? 4 : 8);
     static_assert(MAXI <= 4096 && MAXLOC <= 16, "contribution codes are (incidence << 4 | local node)");
-    __shared__ int s_iptr[NG + 1];
-    __shared__ int s_qptr[NG + 1];
+    __shared__ int s_node[NG];               // the group's nodes (from the node order)
+    __shared__ int s_ibeg[NG], s_qbeg[NG];   // their first incidence and first node pair
+    __shared__ int s_iptr[NG + 1];           // local prefix of incidences per node
+    __shared__ int s_qptr[NG + 1];           // local prefix of node pairs per node
     __shared__ unsigned short s_list[MAXP][CAP];  // contributions per pair, by rank (0xffff: none)
     __shared__ int s_info[MAXI];             // la | nloc << 8 | (cut ? 1 << 16 : 0); la = 255: absent
     __shared__ int64_t s_koff[MAXI];
@@ -1026,66 +1026,59 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
 
-    const int nall = p.n_nodes + p.n_enr;
-    const int a0 = xcd_order(blockIdx.x, gridDim.x) * NG;
-    const int ng = min(NG, nall - a0);
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    // Issue the first incidence record, its slot word and the first node pair
-    // of this thread before the set-up barrier (see k_pair_group).
-    const int ib0 = p.n2e_ptr[a0], ni0 = p.n2e_ptr[a0 + ng] - ib0;
-    const int qa = p.npat_ptr[a0], qb = p.npat_ptr[a0 + ng];
-    int rec0[NEN + 1];
-    uint64_t sw0 = 0;
-    for (int n = 0; n <= NEN; ++n) rec0[n] = 0;
-    if (int(threadIdx.x) < ni0) {
-        for (int n = 0; n <= NEN; ++n) rec0[n] = p.inc[int64_t(ib0 + threadIdx.x) * (NEN + 1) + n];
-        sw0 = p.inc_slot[ib0 + threadIdx.x];
-    }
-    int col0 = -1;
-    int map0[D * D];
-    for (int k = 0; k < D * D; ++k) map0[k] = 0;
-    if (qa + int(threadIdx.x) < qb) {
-        col0 = p.npat_col[qa + threadIdx.x];
-        for (int k = 0; k < D * D; ++k) map0[k] = p.npat_map[int64_t(qa + threadIdx.x) * D * D + k];
-    }
+    const int k0 = xcd_order(blockIdx.x, gridDim.x) * NG;
+    const int ng = min(NG, list_len - k0);
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
         sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
-    if (threadIdx.x <= ng) {
-        s_iptr[threadIdx.x] = p.n2e_ptr[a0 + threadIdx.x];
-        s_qptr[threadIdx.x] = p.npat_ptr[a0 + threadIdx.x];
+    if (int(threadIdx.x) < ng) {
+        const int a = p.norder[list_base + k0 + threadIdx.x];
+        const int ib = p.n2e_ptr[a], qb = p.npat_ptr[a];
+        s_node[threadIdx.x] = a;
+        s_ibeg[threadIdx.x] = ib;
+        s_qbeg[threadIdx.x] = qb;
+        s_iptr[threadIdx.x + 1] = p.n2e_ptr[a + 1] - ib;
+        s_qptr[threadIdx.x + 1] = p.npat_ptr[a + 1] - qb;
     }
     if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
-    const int ibase = s_iptr[0];
-    const int ni = s_iptr[ng] - ibase;
-    const int q0 = s_qptr[0], q1 = s_qptr[ng];
-    const bool staged = ni <= MAXI && (q1 - q0) <= MAXP;
+    if (threadIdx.x == 0) {
+        s_iptr[0] = 0;
+        s_qptr[0] = 0;
+        for (int k = 0; k < ng; ++k) {
+            s_iptr[k + 1] += s_iptr[k];
+            s_qptr[k + 1] += s_qptr[k];
+        }
+    }
+    __syncthreads();
+    const int ni = s_iptr[ng], nq = s_qptr[ng];
+    const bool staged = ni <= MAXI && nq <= MAXP;
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
     // ---- phase A: incidences ----
     if (staged) {
-        for (int k = threadIdx.x; k < (q1 - q0) * CAP; k += blockDim.x) s_list[k / CAP][k % CAP] = 0xffffu;
+        for (int k = threadIdx.x; k < nq * CAP; k += blockDim.x) s_list[k / CAP][k % CAP] = 0xffffu;
         __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             int lo = 0, hi = ng;  // owner node of incidence i
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (s_iptr[mid] - ibase <= i) lo = mid; else hi = mid;
+                if (s_iptr[mid] <= i) lo = mid; else hi = mid;
             }
-            const int a = a0 + lo;
-            const bool first = i == int(threadIdx.x);
-            const int32_t* rec = p.inc + int64_t(ibase + i) * (NEN + 1);
+            const int a = s_node[lo];
+            const int tg = s_ibeg[lo] + (i - s_iptr[lo]);  // global incidence index
+            const int32_t* rec = p.inc + int64_t(tg) * (NEN + 1);
             int ln[MAXLOC];
             int la = -1;
             for (int n = 0; n < NEN; ++n) {
-                ln[n] = first ? rec0[n] : rec[n];
+                ln[n] = rec[n];
                 if (ln[n] == a) la = n;
             }
-            const int tag = first ? rec0[NEN] : rec[NEN];
+            const int tag = rec[NEN];
             int nloc = NEN;
             if (tag < 0) {
                 double x[NEN][3];
@@ -1127,9 +1120,9 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             if (la < 0 || nloc > MAXLOC) continue;
             // contribution of this incidence to each off-diagonal pair (a, ln[m]) of
             // a's row, at its precomputed slot and rank (ascending element order)
-            const uint64_t sw = first ? sw0 : p.inc_slot[ibase + i];
+            const uint64_t sw = p.inc_slot[tg];
             if (sw >> 63) { s_bad = 1; continue; }
-            const int ps0 = s_qptr[lo] - q0, np = s_qptr[lo + 1] - s_qptr[lo];
+            const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             int k = 0;
             for (int m = 0; m < nloc; ++m) {
                 if (m == la) continue;
@@ -1145,15 +1138,15 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     const bool use_lists = staged && !s_bad;
 
     // ---- phase B: node pairs ----
-    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    for (int kq = threadIdx.x; kq < nq; kq += blockDim.x) {
         int lo = 0, hi = ng;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= q) lo = mid; else hi = mid;
+            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
         }
-        const int a = a0 + lo;
-        const bool firstq = q == q0 + int(threadIdx.x);
-        const int b = firstq ? col0 : p.npat_col[q];
+        const int a = s_node[lo];
+        const int q = s_qbeg[lo] + (kq - s_qptr[lo]);  // global node-pair index
+        const int b = p.npat_col[q];
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -1162,7 +1155,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
         if (use_lists && a == b) {
             // diagonal pair: every incidence of a, in ascending element order
-            const int t0 = s_iptr[lo] - ibase, t1 = s_iptr[lo + 1] - ibase;
+            const int t0 = s_iptr[lo], t1 = s_iptr[lo + 1];
             for (int t = t0; t < t1; ++t) {
                 const int info = s_info[t];
                 const int la = info & 255;
@@ -1184,7 +1177,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             }
         } else if (use_lists) {
             // the slot's contributions, stored by rank = ascending element order
-            const int s = q - q0;
+            const int s = kq;
             if (jac) {
 #pragma unroll
                 for (int k = 0; k < CAP; ++k) {
@@ -1207,13 +1200,13 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                 }
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_iptr[lo], s_iptr[lo + 1],
+            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
             const int32_t* mp = p.npat_map + int64_t(q) * D * D;
             for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[firstq ? map0[i * D + j] : mp[i * D + j]] = acc[i][j];
+                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
@@ -1239,14 +1232,17 @@ struct pg_asm {
     // host-API buffers
     double *dU = nullptr, *dK = nullptr, *dP = nullptr, *dIc = nullptr, *dIt = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t side = nullptr;     // second stream: node groups without cut elements overlap the cut kernel
+    hipEvent_t sync[2] = {nullptr, nullptr};
+    int n_clean = 0;                // nodes with no cut incidence (first in norder)
     bool timing = false;
-    double last_ms[3] = {0, 0, 0};
+    double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
     int elem_block = 256;
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
-    int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form of the kernel (2D)
+    int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
 };
@@ -1367,16 +1363,18 @@ int alloc(pg_asm* h, T** p, size_t n) {
 // Node-group kernel launch: the block-table form for tri3 meshes unless the
 // list form is asked for; tet4 meshes always use the list form.
 template <int D, typename TC, int NG, int TV>
-void launch_group(bool lists, unsigned grid, hipStream_t st, const DevProb& d, const double* Un, const TC* Kc,
+void launch_group(bool lists, int base, int len, hipStream_t st, const DevProb& d, const double* Un, const TC* Kc,
                   const double* Pc, double* K, double* P, unsigned flags) {
     constexpr int BS = GroupThreads<D, NG, TV>::value;
+    if (len <= 0) return;
+    const unsigned grid = unsigned((len + NG - 1) / NG);
     if constexpr (D == 2) {
         if (!lists) {
-            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags);
+            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
             return;
         }
     }
-    k_pair_lists<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags);
+    k_pair_lists<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
 }
 
 // Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
@@ -1397,9 +1395,56 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (d.dim == 2) k_elem_fp64<2><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
         else k_elem_fp64<3><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
+        if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
         if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
         else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
     } else {
+        const int nall = d.n_nodes + d.n_enr;
+        const int ng = h->group_nodes;
+        const bool overlap = !(h->group_variant & 4) && h->n_clean > 0 && h->n_clean < nall;
+        // node groups: [base, base + len) of the node order on stream s
+#define GROUP1(DD, TC, KC, NGV, TVV, S, BASE, LEN) \
+    launch_group<DD, TC, NGV, TVV>((h->group_variant & 2) != 0, BASE, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
+#define GROUP(DD, TC, KC, NGV, S, BASE, LEN)                                     \
+    do {                                                                         \
+        if (h->group_variant & 1) GROUP1(DD, TC, KC, NGV, 1, S, BASE, LEN);      \
+        else GROUP1(DD, TC, KC, NGV, 0, S, BASE, LEN);                           \
+    } while (0)
+        auto groups = [&](hipStream_t s, int base, int len) {
+            if (d.dim == 2) {
+                if (fp32) {
+                    if (ng == 8) GROUP(2, float, h->Kc32, 8, s, base, len);
+                    else if (ng == 16) GROUP(2, float, h->Kc32, 16, s, base, len);
+                    else GROUP(2, float, h->Kc32, 32, s, base, len);
+                } else {
+                    if (ng == 8) GROUP(2, double, h->Kc64, 8, s, base, len);
+                    else if (ng == 16) GROUP(2, double, h->Kc64, 16, s, base, len);
+                    else GROUP(2, double, h->Kc64, 32, s, base, len);
+                }
+            } else {
+                if (fp32) {
+                    if (ng == 2) GROUP(3, float, h->Kc32, 2, s, base, len);
+                    else if (ng == 4) GROUP(3, float, h->Kc32, 4, s, base, len);
+                    else GROUP(3, float, h->Kc32, 8, s, base, len);
+                } else {
+                    if (ng == 2) GROUP(3, double, h->Kc64, 2, s, base, len);
+                    else if (ng == 4) GROUP(3, double, h->Kc64, 4, s, base, len);
+                    else GROUP(3, double, h->Kc64, 8, s, base, len);
+                }
+            }
+        };
+#undef GROUP
+#undef GROUP1
+        if (overlap) {
+            // nodes without cut incidences need no staged cut tangents: their
+            // groups run on the side stream while the cut kernels run on st
+            HIP_OK(hipEventRecord(h->sync[0], st));
+            HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
+            if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
+            groups(h->side, 0, h->n_clean);
+            if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
+            HIP_OK(hipEventRecord(h->sync[1], h->side));
+        }
         // cut elements, binned by class: small single-inclusion records share a
         // wavefront (EPW per wave); the general class keeps the full limits
         const int n0 = h->n_class[0], n1 = h->n_class[1];
@@ -1438,36 +1483,10 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         }
 #undef CUT_SMALL2
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
-        const int nall = d.n_nodes + d.n_enr;
-        const int ng = h->group_nodes;
-#define GROUP1(DD, TC, KC, NGV, TVV)                                                                          \
-    launch_group<DD, TC, NGV, TVV>((h->group_variant & 2) != 0, unsigned((nall + NGV - 1) / NGV), st, d, h->Un, KC, \
-                                   h->Pc, dK, dP, flags)
-#define GROUP(DD, TC, KC, NGV) \
-    do { if (h->group_variant & 1) GROUP1(DD, TC, KC, NGV, 1); else GROUP1(DD, TC, KC, NGV, 0); } while (0)
-        if (d.dim == 2) {
-            if (fp32) {
-                if (ng == 8) GROUP(2, float, h->Kc32, 8);
-                else if (ng == 16) GROUP(2, float, h->Kc32, 16);
-                else GROUP(2, float, h->Kc32, 32);
-            } else {
-                if (ng == 8) GROUP(2, double, h->Kc64, 8);
-                else if (ng == 16) GROUP(2, double, h->Kc64, 16);
-                else GROUP(2, double, h->Kc64, 32);
-            }
-        } else {
-            if (fp32) {
-                if (ng == 2) GROUP(3, float, h->Kc32, 2);
-                else if (ng == 4) GROUP(3, float, h->Kc32, 4);
-                else GROUP(3, float, h->Kc32, 8);
-            } else {
-                if (ng == 2) GROUP(3, double, h->Kc64, 2);
-                else if (ng == 4) GROUP(3, double, h->Kc64, 4);
-                else GROUP(3, double, h->Kc64, 8);
-            }
-        }
-#undef GROUP
-#undef GROUP1
+        if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
+        if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
+        if (overlap) groups(st, h->n_clean, nall - h->n_clean);
+        else groups(st, 0, nall);
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1594,6 +1613,20 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(inc, incv.data(), incv.size(), int32_t);
     UP(inc_slot, slotv.data(), slotv.size(), uint64_t);
     {
+        std::vector<int32_t> order;
+        order.reserve(size_t(nall));
+        std::vector<char> dirty(size_t(nall), 0);
+        for (int64_t a = 0; a < nall; ++a)
+            for (int64_t t = s->n2e_ptr[a]; t < s->n2e_ptr[a + 1] && !dirty[a]; ++t)
+                dirty[a] = s->cut_of_elem[s->n2e[t]] >= 0;
+        for (int64_t a = 0; a < nall; ++a)
+            if (!dirty[a]) order.push_back(int32_t(a));
+        h->n_clean = int(order.size());
+        for (int64_t a = 0; a < nall; ++a)
+            if (dirty[a]) order.push_back(int32_t(a));
+        UP(norder, order.data(), order.size(), int32_t);
+    }
+    {
         std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
         for (int c = 0; c < s->n_cut; ++c) {
             const pg_cut_rec& r = s->cut[c];
@@ -1638,8 +1671,14 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     if ((rc = alloc(h, &h->err, 1))) return fail(rc);
     if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) { g_err = "hipMemset failed"; return fail(PG_E_HIP); }
     if (hipStreamCreate(&h->stream) != hipSuccess) { g_err = "hipStreamCreate failed"; return fail(PG_E_HIP); }
+    if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "hipStreamCreateWithFlags failed";
+        return fail(PG_E_HIP);
+    }
     for (auto& e : h->ev)
         if (hipEventCreate(&e) != hipSuccess) { g_err = "hipEventCreate failed"; return fail(PG_E_HIP); }
+    for (auto& e : h->sync)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { g_err = "hipEventCreate failed"; return fail(PG_E_HIP); }
     // algorithmic bytes of one K+R assembly
     int64_t B = 4LL * s->nen * s->n_elem + 8LL * s->dim * nall + 8LL * h->n_dof + int64_t(sizeof(pg_cut_rec)) * s->n_cut +
                 16LL * s->n_cqp + 12LL * h->nnz_total + 4LL * (int64_t(s->n_free) + s->n_pres + 2) + 8LL * h->n_dof;
@@ -1654,8 +1693,12 @@ int pg_asm_destroy(pg_asm* h) {
     for (void* p : h->allocs) (void)hipFree(p);
     for (void* p : {(void*)h->dU, (void*)h->dK, (void*)h->dP, (void*)h->dIc, (void*)h->dIt})
         if (p) (void)hipFree(p);
+    if (h->side) (void)hipStreamSynchronize(h->side);
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : h->sync)
+        if (e) (void)hipEventDestroy(e);
+    if (h->side) (void)hipStreamDestroy(h->side);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PG_OK;
@@ -1685,16 +1728,16 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 3) { g_err = "group variant must be 0..3"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 7) { g_err = "group variant must be 0..7"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;
     return PG_OK;
 }
 
-int pg_asm_last_timing(const pg_asm* h, double ms[3]) {
+int pg_asm_last_timing(const pg_asm* h, double ms[5]) {
     if (!h || !ms) { g_err = "null argument"; return PG_E_ARG; }
-    for (int i = 0; i < 3; ++i) ms[i] = h->last_ms[i];
+    for (int i = 0; i < 5; ++i) ms[i] = h->last_ms[i];
     return PG_OK;
 }
 
@@ -1710,12 +1753,18 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
     if (rc) return rc;
     if (h->timing) {
         HIP_OK(hipEventSynchronize(h->ev[2]));
-        float a = 0, b = 0;
-        HIP_OK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-        HIP_OK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-        h->last_ms[0] = double(a) + double(b);
-        h->last_ms[1] = a;
-        h->last_ms[2] = b;
+        const bool ov = (flags & (PG_FP32 | PG_BATCHED)) && !(h->group_variant & 4) && h->n_clean > 0 &&
+                        h->n_clean < h->d.n_nodes + h->d.n_enr;
+        float tot = 0, cut = 0, dirty = 0, clean = 0;
+        HIP_OK(hipEventElapsedTime(&tot, h->ev[0], h->ev[2]));
+        HIP_OK(hipEventElapsedTime(&cut, h->ev[0], h->ev[1]));
+        HIP_OK(hipEventElapsedTime(&dirty, h->ev[5], h->ev[2]));
+        if (ov) HIP_OK(hipEventElapsedTime(&clean, h->ev[3], h->ev[4]));
+        h->last_ms[0] = tot;
+        h->last_ms[1] = cut;
+        h->last_ms[2] = double(clean) + double(dirty);
+        h->last_ms[3] = clean;
+        h->last_ms[4] = dirty;
     }
     return PG_OK;
 }

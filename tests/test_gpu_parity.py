@@ -120,7 +120,8 @@ def test_every_schedule_variant(kind, n, ni):
     groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
     for mode in ("binned64", "mixed"):
         ref = None
-        for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in range(4 if p.dim == 2 else 2)]:
+        variants = range(8) if p.dim == 2 else (0, 1, 4, 5)
+        for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
             asm = AssemblerGPU(p, mode)
             asm.set_tiles(g, c, v)
             K, P, _ = _run(asm, p, U, ic)
