@@ -86,7 +86,7 @@ def main():
 
     from pigfem_amd.autotune import tune
     tuned = {}
-    for mode in ("binned64", "mixed"):
+    for mode in ("binned64", "mixed", "mixed_cut64"):
         asm.mode = mode
         tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
         log(f"[bench] tuned {mode}: group_nodes, cut_variant, group_variant = {tuned[mode]}")
@@ -99,7 +99,7 @@ def main():
             asm.assemble_device(dU, ic, it, dK, dP)
 
     results = {}
-    for mode in ("fp64", "binned64", "mixed"):
+    for mode in ("fp64", "binned64", "mixed", "mixed_cut64"):
         run(mode, args.warmup)
         synchronize()
         t = time.perf_counter()
@@ -124,7 +124,12 @@ def main():
     relP = float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64))
     isv_diff = float(np.abs(I32 - I64).max()) if prob.n_cqp else 0.0
     log(f"[bench] mixed vs fp64: rel K {relK:.3e}, rel P {relP:.3e}, max |isv diff| {isv_diff:.3e}")
-    del K64, P64, I64, K32, P32, I32
+    del K32, P32, I32
+    Kc, Pc_, _ = snapshot("mixed_cut64")
+    relK_c64 = float(np.linalg.norm(Kc - K64) / np.linalg.norm(K64))
+    relP_c64 = float(np.linalg.norm(Pc_ - P64) / np.linalg.norm(P64))
+    log(f"[bench] mixed_cut64 vs fp64: rel K {relK_c64:.3e}, rel P {relP_c64:.3e}")
+    del K64, P64, I64, Kc, Pc_
 
     per_instance = None
     if batch:
@@ -263,6 +268,10 @@ def main():
         "fp64_ms_per_step": results["fp64"]["ms"],
         "speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],
         "binned64_elements_per_s": results["binned64"]["eps"],
+        "mixed_cut64_elements_per_s": results["mixed_cut64"]["eps"],
+        "mixed_cut64_ms_per_step": results["mixed_cut64"]["ms"],
+        "mixed_cut64_rel_err_K": relK_c64,
+        "mixed_cut64_rel_err_P": relP_c64,
         "binned64_ms_per_step": results["binned64"]["ms"],
         "rel_err_K": relK,
         "rel_err_P": relP,
@@ -272,7 +281,7 @@ def main():
         "setup_s": setup_s,
         "host_api_ms_per_call": host_ms,
         "device_bytes": dev_bytes,
-        "tiles": {"mixed": tuned["mixed"], "binned64": tuned["binned64"]},
+        "tiles": {"mixed": tuned["mixed"], "binned64": tuned["binned64"], "mixed_cut64": tuned["mixed_cut64"]},
     }
     if per_instance:
         out["per_instance"] = per_instance

@@ -47,6 +47,9 @@ extern "C" {
                             FP64 sliver Jacobians, damage update, CSR sums, P */
 #define PG_BATCHED  8u   /* class-binned schedule (cut-element waves +
                             node-pair recompute); implied by PG_FP32         */
+#define PG_CUT64   16u   /* with PG_FP32: cut elements (children, enrichment,
+                            cohesive tangents) stay FP64 end to end; FP32 is
+                            used only for uncut elements                     */
 
 /* ----------------------------------------------------------------------------
  * Materials.  Reference: material_lei.cpp:52-97 (LEI D matrix),

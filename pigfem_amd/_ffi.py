@@ -16,6 +16,7 @@ PG_FUNC = 1
 PG_JAC = 2
 PG_FP32 = 4
 PG_BATCHED = 8
+PG_CUT64 = 16
 
 PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)

@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, check, lib
+from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PG_CUT64, check, lib
 
 
 class PMatrix:
@@ -75,11 +75,13 @@ class AssemblerGPU:
           "binned64" -- class-binned schedule in FP64 (cut elements one wavefront
                         each, uncut elements recomputed in the node-pair kernel);
           "mixed"    -- class-binned with FP32 B^T.D.B / cohesive tangents and FP64
-                        sliver Jacobians, damage update, CSR sums and residual.
+                        sliver Jacobians, damage update, CSR sums and residual;
+          "mixed_cut64" -- class-binned with FP32 only for uncut elements: cut
+                        elements (children, enrichment, cohesive tangents) in FP64.
     """
 
     def __init__(self, problem, mode: str = "fp64"):
-        if mode not in ("fp64", "binned64", "mixed"):
+        if mode not in ("fp64", "binned64", "mixed", "mixed_cut64"):
             raise ValueError(mode)
         self.problem = problem
         self.mode = mode
@@ -94,6 +96,8 @@ class AssemblerGPU:
         f = (PG_FUNC if func else 0) | (PG_JAC if jac else 0)
         if self.mode == "mixed":
             f |= PG_FP32 | PG_BATCHED
+        elif self.mode == "mixed_cut64":
+            f |= PG_FP32 | PG_BATCHED | PG_CUT64
         elif self.mode == "binned64":
             f |= PG_BATCHED
         return f

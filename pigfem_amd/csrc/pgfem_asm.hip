@@ -17,6 +17,7 @@
 #include <string.h>
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -757,8 +758,8 @@ template <int D, int NG, int TV> struct GroupThreads {
 // One node pair (a, b) from global memory: the incidences [t0, t1) of a in
 // element order, with the arithmetic of the LDS-staged paths (used by groups
 // whose incidences or pairs exceed the LDS tables).
-template <int D, typename TC>
-__device__ inline void pair_from_global(const DevProb& p, const double* __restrict__ Un, const TC* __restrict__ Kc,
+template <int D, typename TC, typename TKC>
+__device__ inline void pair_from_global(const DevProb& p, const double* __restrict__ Un, const TKC* __restrict__ Kc,
                                         const double* __restrict__ Pc, const TC (*sLM)[2], double wsum,
                                         int t0, int t1, int a, int b, bool jac, bool diag, double acc[D][D],
                                         double pa[D]) {
@@ -795,7 +796,7 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
         } else {
             const int nl = NEN + p.cut[ci].n_enr;
             if (jac) {
-                const TC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
+                const TKC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
                 for (int i = 0; i < D; ++i)
                     for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
             }
@@ -981,7 +982,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                         for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[s][r][i * D + j]);
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
+            pair_from_global<D, TC, TC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
@@ -1000,9 +1001,9 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
 // fit in LDS: phase A stores each incidence's gradients and left factor
 // L = w B_a^T D and registers (incidence, local node) codes at their static
 // (slot, rank); phase B multiplies out L B_b per contribution.
-template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
+template <int D, typename TC, typename TKC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
 __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
-                                                    const TC* __restrict__ Kc, const double* __restrict__ Pc,
+                                                    const TKC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
                                                     int list_base, int list_len) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
@@ -1170,7 +1171,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                         for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                 } else {
                     const int nl = (info >> 8) & 255;
-                    const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
+                    const TKC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
                 }
@@ -1193,14 +1194,14 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                             for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                     } else {
                         const int nl = (info >> 8) & 255;
-                        const TC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
+                        const TKC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
                         for (int i = 0; i < D; ++i)
                             for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
                     }
                 }
             }
         } else {
-            pair_from_global<D, TC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
+            pair_from_global<D, TC, TKC>(p, Un, Kc, Pc, sLM, wsum, s_ibeg[lo], s_ibeg[lo] + (s_iptr[lo + 1] - s_iptr[lo]),
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
@@ -1362,19 +1363,19 @@ int alloc(pg_asm* h, T** p, size_t n) {
 
 // Node-group kernel launch: the block-table form for tri3 meshes unless the
 // list form is asked for; tet4 meshes always use the list form.
-template <int D, typename TC, int NG, int TV>
-void launch_group(bool lists, int base, int len, hipStream_t st, const DevProb& d, const double* Un, const TC* Kc,
+template <int D, typename TC, typename TKC, int NG, int TV>
+void launch_group(bool lists, int base, int len, hipStream_t st, const DevProb& d, const double* Un, const TKC* Kc,
                   const double* Pc, double* K, double* P, unsigned flags) {
     constexpr int BS = GroupThreads<D, NG, TV>::value;
     if (len <= 0) return;
     const unsigned grid = unsigned((len + NG - 1) / NG);
-    if constexpr (D == 2) {
+    if constexpr (D == 2 && std::is_same<TC, TKC>::value) {
         if (!lists) {
             k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
             return;
         }
     }
-    k_pair_lists<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
+    k_pair_lists<D, TC, TKC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
 }
 
 // Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
@@ -1401,38 +1402,41 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     } else {
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
+        const bool cut64 = fp32 && (flags & PG_CUT64) != 0;  // cut elements staged in FP64
         const bool overlap = !(h->group_variant & 4) && h->n_clean > 0 && h->n_clean < nall;
         // node groups: [base, base + len) of the node order on stream s
-#define GROUP1(DD, TC, KC, NGV, TVV, S, BASE, LEN) \
-    launch_group<DD, TC, NGV, TVV>((h->group_variant & 2) != 0, BASE, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
-#define GROUP(DD, TC, KC, NGV, S, BASE, LEN)                                     \
+#define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
+    launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
+#define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                                 \
     do {                                                                         \
-        if (h->group_variant & 1) GROUP1(DD, TC, KC, NGV, 1, S, BASE, LEN);      \
-        else GROUP1(DD, TC, KC, NGV, 0, S, BASE, LEN);                           \
+        if (h->group_variant & 1) GROUP1(DD, TC, TKC, KC, NGV, 1, S, BASE, LEN); \
+        else GROUP1(DD, TC, TKC, KC, NGV, 0, S, BASE, LEN);                      \
+    } while (0)
+#define GROUPS2(TC, TKC, KC, S, BASE, LEN)                              \
+    do {                                                                \
+        if (ng == 8) GROUP(2, TC, TKC, KC, 8, S, BASE, LEN);            \
+        else if (ng == 16) GROUP(2, TC, TKC, KC, 16, S, BASE, LEN);     \
+        else GROUP(2, TC, TKC, KC, 32, S, BASE, LEN);                   \
+    } while (0)
+#define GROUPS3(TC, TKC, KC, S, BASE, LEN)                              \
+    do {                                                                \
+        if (ng == 2) GROUP(3, TC, TKC, KC, 2, S, BASE, LEN);            \
+        else if (ng == 4) GROUP(3, TC, TKC, KC, 4, S, BASE, LEN);       \
+        else GROUP(3, TC, TKC, KC, 8, S, BASE, LEN);                    \
     } while (0)
         auto groups = [&](hipStream_t s, int base, int len) {
             if (d.dim == 2) {
-                if (fp32) {
-                    if (ng == 8) GROUP(2, float, h->Kc32, 8, s, base, len);
-                    else if (ng == 16) GROUP(2, float, h->Kc32, 16, s, base, len);
-                    else GROUP(2, float, h->Kc32, 32, s, base, len);
-                } else {
-                    if (ng == 8) GROUP(2, double, h->Kc64, 8, s, base, len);
-                    else if (ng == 16) GROUP(2, double, h->Kc64, 16, s, base, len);
-                    else GROUP(2, double, h->Kc64, 32, s, base, len);
-                }
+                if (!fp32) GROUPS2(double, double, h->Kc64, s, base, len);
+                else if (cut64) GROUPS2(float, double, h->Kc64, s, base, len);
+                else GROUPS2(float, float, h->Kc32, s, base, len);
             } else {
-                if (fp32) {
-                    if (ng == 2) GROUP(3, float, h->Kc32, 2, s, base, len);
-                    else if (ng == 4) GROUP(3, float, h->Kc32, 4, s, base, len);
-                    else GROUP(3, float, h->Kc32, 8, s, base, len);
-                } else {
-                    if (ng == 2) GROUP(3, double, h->Kc64, 2, s, base, len);
-                    else if (ng == 4) GROUP(3, double, h->Kc64, 4, s, base, len);
-                    else GROUP(3, double, h->Kc64, 8, s, base, len);
-                }
+                if (!fp32) GROUPS3(double, double, h->Kc64, s, base, len);
+                else if (cut64) GROUPS3(float, double, h->Kc64, s, base, len);
+                else GROUPS3(float, float, h->Kc32, s, base, len);
             }
         };
+#undef GROUPS2
+#undef GROUPS3
 #undef GROUP
 #undef GROUP1
         if (overlap) {
@@ -1455,7 +1459,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     k_cut<2, TK, EPW, 7, 8, 2, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
         if (d.dim == 2) {
             if (n0) {
-                if (fp32) {
+                if (fp32 && !cut64) {
                     if (cv == 1) CUT_SMALL2(float, h->Kc32, 4, 3);
                     else if (cv == 2) CUT_SMALL2(float, h->Kc32, 2, 1);
                     else if (cv == 3) CUT_SMALL2(float, h->Kc32, 8, 1);
@@ -1468,16 +1472,16 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
                 }
             }
             if (n1) {
-                if (fp32) k_cut<2, float, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                if (fp32 && !cut64) k_cut<2, float, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
                 else k_cut<2, double, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         } else {
             if (n0) {
-                if (fp32) k_cut<3, float, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                if (fp32 && !cut64) k_cut<3, float, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
                 else k_cut<3, double, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
             if (n1) {
-                if (fp32) k_cut<3, float, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                if (fp32 && !cut64) k_cut<3, float, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
                 else k_cut<3, double, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         }

@@ -40,19 +40,19 @@ def case(request):
     return p, U, ic, Kr, Pr, Ir
 
 
-@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed"])
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed", "mixed_cut64"])
 def test_parity_vs_oracle(case, mode):
     p, U, ic, Kr, Pr, Ir = case
     asm = AssemblerGPU(p, mode)
     K, P, It = _run(asm, p, U, ic)
-    tol = 1e-6 if mode == "mixed" else 1e-12
+    tol = 1e-6 if mode.startswith("mixed") else 1e-12
     assert rel(K, Kr) < tol, rel(K, Kr)
     assert rel(P, Pr) < 1e-12, rel(P, Pr)
     if p.n_cqp:
         assert np.abs(It - Ir).max() <= 1e-12 * max(np.abs(Ir).max(), 1e-30)
 
 
-@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed"])
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed", "mixed_cut64"])
 def test_bitwise_repeatable(case, mode):
     p, U, ic, *_ = case
     asm = AssemblerGPU(p, mode)
@@ -118,7 +118,7 @@ def test_every_schedule_variant(kind, n, ni):
     ic = np.zeros(p.n_cqp)
     Kr, Pr, _ = o.assemble(U, ic)
     groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
-    for mode in ("binned64", "mixed"):
+    for mode in ("binned64", "mixed", "mixed_cut64"):
         ref = None
         variants = range(8) if p.dim == 2 else (0, 1, 4, 5)
         for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
