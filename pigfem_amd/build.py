@@ -20,6 +20,10 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # keep the compiler from emitting packed FP32 arithmetic
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no implicit contraction of a * b + c: every kernel then rounds the same
+    # expression the same way (results do not depend on the schedule chosen);
+    # fused operations are written out where they are wanted
+    "-ffp-contract=off",
     "-I" + INC, "-I" + CSRC,
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-I" + INC, "-I" + CSRC]
