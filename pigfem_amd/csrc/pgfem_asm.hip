@@ -755,6 +755,34 @@ template <int D, int NG, int TV> struct GroupThreads {
     static constexpr int value = per < 64 ? 64 : (per > 512 ? 512 : (per + 63) / 64 * 64);
 };
 
+// Diagonal block (la, la) of an uncut element.  With FP32 tangents it is minus
+// the FP64 sum of the row's off-diagonal FP32 blocks: the gradients of a
+// simplex sum to zero, so the exact block equals that sum, and taking it this
+// way makes every element row sum to zero to FP64 rounding -- the assembled
+// tangent keeps annihilating rigid translations, which FP32 rounding of each
+// block on its own would spoil (loosely bonded fibres turn that spurious
+// stiffness into slow Newton convergence).  FP64 tangents use the closed form.
+template <int D, typename TC>
+__device__ inline void diag_block(const TC (*gt)[D], int la, TC wl, TC wm, double (*out)[D]) {
+    constexpr int NEN = D + 1;
+    if constexpr (sizeof(TC) == 4) {
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) out[i][j] = 0.0;
+        for (int m = 0; m < NEN; ++m) {
+            if (m == la) continue;
+            TC blk[D][D];
+            iso_block<D, TC>(gt[la], gt[m], wl, wm, blk);
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) out[i][j] -= double(blk[i][j]);
+        }
+    } else {
+        TC blk[D][D];
+        iso_block<D, TC>(gt[la], gt[la], wl, wm, blk);
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) out[i][j] = double(blk[i][j]);
+    }
+}
+
 // One node pair (a, b) from global memory: the incidences [t0, t1) of a in
 // element order, with the arithmetic of the LDS-staged paths (used by groups
 // whose incidences or pairs exceed the LDS tables).
@@ -782,11 +810,20 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
             const double J = simplex_grads<D, double>(x, g);
             const double w = J * wsum;
             if (jac) {
-                TC ga[D], gb[D], blk[D][D];
-                for (int c = 0; c < D; ++c) { ga[c] = TC(g[la][c]); gb[c] = TC(g[lb][c]); }
-                iso_block<D, TC>(ga, gb, TC(w) * sLM[mi][0], TC(w) * sLM[mi][1], blk);
-                for (int i = 0; i < D; ++i)
-                    for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                TC gt[NEN][D], blk[D][D];
+                for (int n = 0; n < NEN; ++n)
+                    for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
+                const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+                if (la == lb) {
+                    double dd[D][D];
+                    diag_block<D, TC>(gt, la, wl, wm, dd);
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += dd[i][j];
+                } else {
+                    iso_block<D, TC>(gt[la], gt[lb], wl, wm, blk);
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                }
             }
             if (diag) {
                 double pp[D];
@@ -826,7 +863,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_iptr[NG + 1];           // local prefix of incidences per node
     __shared__ int s_qptr[NG + 1];           // local prefix of node pairs per node
     __shared__ TC s_blk[MAXP][CAPO][D * D];   // off-diagonal d x d blocks by pair slot and rank (0 when absent)
-    __shared__ TC s_dblk[NG][CAPD][D * D];    // diagonal blocks by node and incidence
+    __shared__ double s_dblk[NG][CAPD][D * D];  // diagonal blocks by node and incidence
     __shared__ double s_dP[NG][CAPD][D];      // internal force by node and incidence
     __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
@@ -904,7 +941,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
                 const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
-                iso_block<D, TC>(gt[la], gt[la], wl, wm, reinterpret_cast<TC (*)[D]>(s_dblk[lo][rd]));
+                diag_block<D, TC>(gt, la, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[lo][rd]));
                 int k = 0;
                 for (int m = 0; m < NEN; ++m) {
                     if (m == la) continue;
@@ -930,7 +967,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(c1.y) + la * D + c];
                 if (!jac) continue;
                 const TC* Ke = Kc + (int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32)) + int64_t(la * nloc) * D * D;
-                for (int k = 0; k < D * D; ++k) s_dblk[lo][rd][k] = Ke[la * D * D + k];
+                for (int k = 0; k < D * D; ++k) s_dblk[lo][rd][k] = double(Ke[la * D * D + k]);
                 int k = 0;
                 for (int m = 0; m < nloc; ++m) {
                     if (m == la) continue;
@@ -970,7 +1007,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][r][i];
                 if (jac)
                     for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(s_dblk[lo][r][i * D + j]);
+                        for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[lo][r][i * D + j];
             }
         } else if (use_lists) {
             // off-diagonal pair: its contributions by rank (absent ranks hold 0)
@@ -1165,10 +1202,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
                 if (!jac) continue;
                 if (!(info & (1 << 16))) {
-                    TC blk[D][D];
-                    iso_block<D, TC>(s_g[t][la], s_g[t][la], s_wlm[t][0], s_wlm[t][1], blk);
+                    double dd[D][D];
+                    diag_block<D, TC>(s_g[t], la, s_wlm[t][0], s_wlm[t][1], dd);
                     for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                        for (int j = 0; j < D; ++j) acc[i][j] += dd[i][j];
                 } else {
                     const int nl = (info >> 8) & 255;
                     const TKC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
