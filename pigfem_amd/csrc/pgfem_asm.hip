@@ -62,7 +62,8 @@ struct DevProb {
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
     const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
     const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
-    const int32_t* norder;     // node order of the node-group kernel: nodes without cut incidences, then the others
+    const int32_t* ntab;       // node-group kernel, per position of its node order (nodes without cut
+                               // incidences first): node, first incidence, first pair, incidences | pairs << 16
     int n_mat;
 };
 
@@ -876,23 +877,31 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
         sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
-    if (int(threadIdx.x) < ng) {
-        const int a = p.norder[list_base + k0 + threadIdx.x];
-        const int ib = p.n2e_ptr[a], qb = p.npat_ptr[a];
-        s_node[threadIdx.x] = a;
-        s_ibeg[threadIdx.x] = ib;
-        s_qbeg[threadIdx.x] = qb;
-        s_iptr[threadIdx.x + 1] = p.n2e_ptr[a + 1] - ib;
-        s_qptr[threadIdx.x + 1] = p.npat_ptr[a + 1] - qb;
-    }
-    if (threadIdx.x == 0) s_bad = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        s_iptr[0] = 0;
-        s_qptr[0] = 0;
-        for (int k = 0; k < ng; ++k) {
-            s_iptr[k + 1] += s_iptr[k];
-            s_qptr[k + 1] += s_qptr[k];
+    if (threadIdx.x < 64) {
+        // the group's nodes and their incidence / pair counts, prefix-summed
+        // across the first wavefront (NG <= 32)
+        int ci = 0, cq = 0;
+        if (int(threadIdx.x) < ng) {
+            const int4 r = reinterpret_cast<const int4*>(p.ntab)[list_base + k0 + threadIdx.x];
+            s_node[threadIdx.x] = r.x;
+            s_ibeg[threadIdx.x] = r.y;
+            s_qbeg[threadIdx.x] = r.z;
+            ci = r.w & 0xffff;
+            cq = r.w >> 16;
+        }
+#pragma unroll
+        for (int off = 1; off < NG; off <<= 1) {
+            const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+            if (int(threadIdx.x) >= off) { ci += ti; cq += tq; }
+        }
+        if (int(threadIdx.x) < ng) {
+            s_iptr[threadIdx.x + 1] = ci;
+            s_qptr[threadIdx.x + 1] = cq;
+        }
+        if (threadIdx.x == 0) {
+            s_iptr[0] = 0;
+            s_qptr[0] = 0;
+            s_bad = 0;
         }
     }
     __syncthreads();
@@ -1072,23 +1081,31 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
         sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
     }
-    if (int(threadIdx.x) < ng) {
-        const int a = p.norder[list_base + k0 + threadIdx.x];
-        const int ib = p.n2e_ptr[a], qb = p.npat_ptr[a];
-        s_node[threadIdx.x] = a;
-        s_ibeg[threadIdx.x] = ib;
-        s_qbeg[threadIdx.x] = qb;
-        s_iptr[threadIdx.x + 1] = p.n2e_ptr[a + 1] - ib;
-        s_qptr[threadIdx.x + 1] = p.npat_ptr[a + 1] - qb;
-    }
-    if (threadIdx.x == 0) s_bad = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        s_iptr[0] = 0;
-        s_qptr[0] = 0;
-        for (int k = 0; k < ng; ++k) {
-            s_iptr[k + 1] += s_iptr[k];
-            s_qptr[k + 1] += s_qptr[k];
+    if (threadIdx.x < 64) {
+        // the group's nodes and their incidence / pair counts, prefix-summed
+        // across the first wavefront (NG <= 32)
+        int ci = 0, cq = 0;
+        if (int(threadIdx.x) < ng) {
+            const int4 r = reinterpret_cast<const int4*>(p.ntab)[list_base + k0 + threadIdx.x];
+            s_node[threadIdx.x] = r.x;
+            s_ibeg[threadIdx.x] = r.y;
+            s_qbeg[threadIdx.x] = r.z;
+            ci = r.w & 0xffff;
+            cq = r.w >> 16;
+        }
+#pragma unroll
+        for (int off = 1; off < NG; off <<= 1) {
+            const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+            if (int(threadIdx.x) >= off) { ci += ti; cq += tq; }
+        }
+        if (int(threadIdx.x) < ng) {
+            s_iptr[threadIdx.x + 1] = ci;
+            s_qptr[threadIdx.x + 1] = cq;
+        }
+        if (threadIdx.x == 0) {
+            s_iptr[0] = 0;
+            s_qptr[0] = 0;
+            s_bad = 0;
         }
     }
     __syncthreads();
@@ -1665,7 +1682,20 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         h->n_clean = int(order.size());
         for (int64_t a = 0; a < nall; ++a)
             if (dirty[a]) order.push_back(int32_t(a));
-        UP(norder, order.data(), order.size(), int32_t);
+        std::vector<int32_t> tab(size_t(nall) * 4);
+        for (int64_t k = 0; k < nall; ++k) {
+            const int32_t a = order[size_t(k)];
+            const int32_t ic = s->n2e_ptr[a + 1] - s->n2e_ptr[a], qc = s->npat_ptr[a + 1] - s->npat_ptr[a];
+            if (ic > 0xffff || qc > 0x7fff) {
+                g_err = "node with too many incidences or node pairs";
+                return fail(PG_E_UNSUPPORTED);
+            }
+            tab[size_t(k) * 4 + 0] = a;
+            tab[size_t(k) * 4 + 1] = s->n2e_ptr[a];
+            tab[size_t(k) * 4 + 2] = s->npat_ptr[a];
+            tab[size_t(k) * 4 + 3] = ic | (qc << 16);
+        }
+        UP(ntab, tab.data(), tab.size(), int32_t);
     }
     {
         std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
