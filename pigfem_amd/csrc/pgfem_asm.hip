@@ -62,8 +62,8 @@ struct DevProb {
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
     const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
     const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
-    const int32_t* ntab;       // node-group kernel, per position of its node order (nodes without cut
-                               // incidences first): node, first incidence, first pair, incidences | pairs << 16
+    const int32_t* ntab;       // per node: node, first incidence, first pair, incidences | pairs << 16
+    const int32_t* glist;      // node-group starts for each group size: groups without cut incidences first
     int n_mat;
 };
 
@@ -849,7 +849,7 @@ template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::
 __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
-                                                    int list_base, int list_len) {
+                                                    int gbase) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -870,8 +870,8 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int k0 = xcd_order(blockIdx.x, gridDim.x) * NG;
-    const int ng = min(NG, list_len - k0);
+    const int k0 = p.glist[gbase + xcd_order(blockIdx.x, gridDim.x)];  // first node of the group
+    const int ng = min(NG, p.n_nodes + p.n_enr - k0);
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -882,7 +882,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
         // across the first wavefront (NG <= 32)
         int ci = 0, cq = 0;
         if (int(threadIdx.x) < ng) {
-            const int4 r = reinterpret_cast<const int4*>(p.ntab)[list_base + k0 + threadIdx.x];
+            const int4 r = reinterpret_cast<const int4*>(p.ntab)[k0 + threadIdx.x];
             s_node[threadIdx.x] = r.x;
             s_ibeg[threadIdx.x] = r.y;
             s_qbeg[threadIdx.x] = r.z;
@@ -1051,7 +1051,7 @@ template <int D, typename TC, typename TKC, int NG, int TV, int BS = GroupThread
 __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TKC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
-                                                    int list_base, int list_len) {
+                                                    int gbase) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -1074,8 +1074,8 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int k0 = xcd_order(blockIdx.x, gridDim.x) * NG;
-    const int ng = min(NG, list_len - k0);
+    const int k0 = p.glist[gbase + xcd_order(blockIdx.x, gridDim.x)];  // first node of the group
+    const int ng = min(NG, p.n_nodes + p.n_enr - k0);
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         // across the first wavefront (NG <= 32)
         int ci = 0, cq = 0;
         if (int(threadIdx.x) < ng) {
-            const int4 r = reinterpret_cast<const int4*>(p.ntab)[list_base + k0 + threadIdx.x];
+            const int4 r = reinterpret_cast<const int4*>(p.ntab)[k0 + threadIdx.x];
             s_node[threadIdx.x] = r.x;
             s_ibeg[threadIdx.x] = r.y;
             s_qbeg[threadIdx.x] = r.z;
@@ -1290,7 +1290,10 @@ struct pg_asm {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t side = nullptr;     // second stream: node groups without cut elements overlap the cut kernel
     hipEvent_t sync[2] = {nullptr, nullptr};
-    int n_clean = 0;                // nodes with no cut incidence (first in norder)
+    int gopt_ng[3] = {0, 0, 0};     // group sizes with a group list
+    int goff[3] = {0, 0, 0};        // offset of each list in glist
+    int ngroups[3] = {0, 0, 0};     // groups of each list
+    int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
     bool timing = false;
     double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
@@ -1418,18 +1421,18 @@ int alloc(pg_asm* h, T** p, size_t n) {
 // Node-group kernel launch: the block-table form for tri3 meshes unless the
 // list form is asked for; tet4 meshes always use the list form.
 template <int D, typename TC, typename TKC, int NG, int TV>
-void launch_group(bool lists, int base, int len, hipStream_t st, const DevProb& d, const double* Un, const TKC* Kc,
-                  const double* Pc, double* K, double* P, unsigned flags) {
+void launch_group(bool lists, int gbase, int ngroups, hipStream_t st, const DevProb& d, const double* Un,
+                  const TKC* Kc, const double* Pc, double* K, double* P, unsigned flags) {
     constexpr int BS = GroupThreads<D, NG, TV>::value;
-    if (len <= 0) return;
-    const unsigned grid = unsigned((len + NG - 1) / NG);
+    if (ngroups <= 0) return;
+    const unsigned grid = unsigned(ngroups);
     if constexpr (D == 2 && std::is_same<TC, TKC>::value) {
         if (!lists) {
-            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
+            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase);
             return;
         }
     }
-    k_pair_lists<D, TC, TKC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, base, len);
+    k_pair_lists<D, TC, TKC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase);
 }
 
 // Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
@@ -1457,8 +1460,11 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const int nall = d.n_nodes + d.n_enr;
         const int ng = h->group_nodes;
         const bool cut64 = fp32 && (flags & PG_CUT64) != 0;  // cut elements staged in FP64
-        const bool overlap = !(h->group_variant & 4) && h->n_clean > 0 && h->n_clean < nall;
-        // node groups: [base, base + len) of the node order on stream s
+        int o = 0;
+        while (o < 2 && h->gopt_ng[o] != ng) ++o;
+        const int nclean = h->nclean[o], nrest = h->ngroups[o] - h->nclean[o];
+        const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
+        // node groups: BASE .. BASE + LEN of the group list on stream s
 #define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
     launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
 #define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                                 \
@@ -1499,7 +1505,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(h->side, 0, h->n_clean);
+            groups(h->side, h->goff[o], nclean);
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -1543,8 +1549,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
-        if (overlap) groups(st, h->n_clean, nall - h->n_clean);
-        else groups(st, 0, nall);
+        if (overlap) groups(st, h->goff[o] + nclean, nrest);
+        else groups(st, h->goff[o], h->ngroups[o]);
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1671,31 +1677,44 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(inc, incv.data(), incv.size(), int32_t);
     UP(inc_slot, slotv.data(), slotv.size(), uint64_t);
     {
-        std::vector<int32_t> order;
-        order.reserve(size_t(nall));
+        // node table, and for each group size the list of group starts: groups
+        // none of whose nodes touches a cut element first (they can run while
+        // the cut kernels run), then the others; each list keeps mesh order
         std::vector<char> dirty(size_t(nall), 0);
         for (int64_t a = 0; a < nall; ++a)
             for (int64_t t = s->n2e_ptr[a]; t < s->n2e_ptr[a + 1] && !dirty[a]; ++t)
                 dirty[a] = s->cut_of_elem[s->n2e[t]] >= 0;
-        for (int64_t a = 0; a < nall; ++a)
-            if (!dirty[a]) order.push_back(int32_t(a));
-        h->n_clean = int(order.size());
-        for (int64_t a = 0; a < nall; ++a)
-            if (dirty[a]) order.push_back(int32_t(a));
         std::vector<int32_t> tab(size_t(nall) * 4);
-        for (int64_t k = 0; k < nall; ++k) {
-            const int32_t a = order[size_t(k)];
+        for (int64_t a = 0; a < nall; ++a) {
             const int32_t ic = s->n2e_ptr[a + 1] - s->n2e_ptr[a], qc = s->npat_ptr[a + 1] - s->npat_ptr[a];
             if (ic > 0xffff || qc > 0x7fff) {
                 g_err = "node with too many incidences or node pairs";
                 return fail(PG_E_UNSUPPORTED);
             }
-            tab[size_t(k) * 4 + 0] = a;
-            tab[size_t(k) * 4 + 1] = s->n2e_ptr[a];
-            tab[size_t(k) * 4 + 2] = s->npat_ptr[a];
-            tab[size_t(k) * 4 + 3] = ic | (qc << 16);
+            tab[size_t(a) * 4 + 0] = int32_t(a);
+            tab[size_t(a) * 4 + 1] = s->n2e_ptr[a];
+            tab[size_t(a) * 4 + 2] = s->npat_ptr[a];
+            tab[size_t(a) * 4 + 3] = ic | (qc << 16);
         }
         UP(ntab, tab.data(), tab.size(), int32_t);
+        const int opts2[3] = {8, 16, 32}, opts3[3] = {2, 4, 8};
+        std::vector<int32_t> gl;
+        for (int o = 0; o < 3; ++o) {
+            const int NGo = s->dim == 2 ? opts2[o] : opts3[o];
+            std::vector<int32_t> clean, rest;
+            for (int64_t a0 = 0; a0 < nall; a0 += NGo) {
+                bool any = false;
+                for (int64_t a = a0; a < std::min<int64_t>(nall, a0 + NGo) && !any; ++a) any = dirty[a] != 0;
+                (any ? rest : clean).push_back(int32_t(a0));
+            }
+            h->gopt_ng[o] = NGo;
+            h->goff[o] = int(gl.size());
+            h->ngroups[o] = int(clean.size() + rest.size());
+            h->nclean[o] = int(clean.size());
+            gl.insert(gl.end(), clean.begin(), clean.end());
+            gl.insert(gl.end(), rest.begin(), rest.end());
+        }
+        UP(glist, gl.data(), gl.size(), int32_t);
     }
     {
         std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
@@ -1824,8 +1843,10 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
     if (rc) return rc;
     if (h->timing) {
         HIP_OK(hipEventSynchronize(h->ev[2]));
-        const bool ov = (flags & (PG_FP32 | PG_BATCHED)) && !(h->group_variant & 4) && h->n_clean > 0 &&
-                        h->n_clean < h->d.n_nodes + h->d.n_enr;
+        int o = 0;
+        while (o < 2 && h->gopt_ng[o] != h->group_nodes) ++o;
+        const bool ov = (flags & (PG_FP32 | PG_BATCHED)) && !(h->group_variant & 4) && h->nclean[o] > 0 &&
+                        h->nclean[o] < h->ngroups[o];
         float tot = 0, cut = 0, dirty = 0, clean = 0;
         HIP_OK(hipEventElapsedTime(&tot, h->ev[0], h->ev[2]));
         HIP_OK(hipEventElapsedTime(&cut, h->ev[0], h->ev[1]));
