@@ -756,6 +756,40 @@ template <int D, int NG, int TV> struct GroupThreads {
     static constexpr int value = per < 64 ? 64 : (per > 512 ? 512 : (per + 63) / 64 * 64);
 };
 
+// Node-pair metadata of a group's local pair kq: owner slot lo, row node a,
+// global pair index q, column node b, the row's dofs and (2D) the scatter
+// indices of the d x d block.  It depends only on the group's node table, so
+// the kernels load it before phase A and its latency overlaps the incidence
+// loads (3D keeps the nine scatter indices for after phase A: registers).
+template <int D> struct PairMeta {
+    int lo, a, q, b;
+    int dof[D];
+    int4 map;  // 2D scatter indices
+};
+
+template <int D, int NG>
+__device__ inline PairMeta<D> pair_meta(const DevProb& p, const int* s_qptr, const int* s_qbeg, const int* s_node,
+                                        int ng, int kq, bool jac) {
+    PairMeta<D> m;
+    int lo = 0, hi = ng;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
+    }
+    m.lo = lo;
+    m.a = s_node[lo];
+    m.q = s_qbeg[lo] + (kq - s_qptr[lo]);
+    m.b = p.npat_col[m.q];
+    for (int i = 0; i < D; ++i) m.dof[i] = p.node_dof[int64_t(m.a) * D + i];
+    if constexpr (D == 2) {
+        if (jac) m.map = reinterpret_cast<const int4*>(p.npat_map)[m.q];
+        else m.map = make_int4(0, 0, 0, 0);
+    } else {
+        m.map = make_int4(0, 0, 0, 0);
+    }
+    return m;
+}
+
 // Diagonal block (la, la) of an uncut element.  With FP32 tangents it is minus
 // the FP64 sum of the row's off-diagonal FP32 blocks: the gradients of a
 // simplex sum to zero, so the exact block equals that sum, and taking it this
@@ -910,6 +944,9 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
+    PairMeta<D> pm0;
+    if (int(threadIdx.x) < nq) pm0 = pair_meta<D, NG>(p, s_qptr, s_qbeg, s_node, ng, threadIdx.x, jac);
+
     // ---- phase A: every incidence computes its element's d x d blocks of the
     // owner's row and stores them at their (slot, rank) -- ascending element
     // order is the rank order, so phase B sums them in the reference order ----
@@ -994,14 +1031,8 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
 
     // ---- phase B: node pairs ----
     for (int kq = threadIdx.x; kq < nq; kq += blockDim.x) {
-        int lo = 0, hi = ng;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
-        }
-        const int a = s_node[lo];
-        const int q = s_qbeg[lo] + (kq - s_qptr[lo]);  // global node-pair index
-        const int b = p.npat_col[q];
+        const PairMeta<D> pm = kq == int(threadIdx.x) ? pm0 : pair_meta<D, NG>(p, s_qptr, s_qbeg, s_node, ng, kq, jac);
+        const int lo = pm.lo, a = pm.a, q = pm.q, b = pm.b;
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -1032,14 +1063,13 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
-            const int4 mp = reinterpret_cast<const int4*>(p.npat_map)[q];
-            K[mp.x] = acc[0][0];
-            K[mp.y] = acc[0][1];
-            K[mp.z] = acc[1][0];
-            K[mp.w] = acc[1][1];
+            K[pm.map.x] = acc[0][0];
+            K[pm.map.y] = acc[0][1];
+            K[pm.map.z] = acc[1][0];
+            K[pm.map.w] = acc[1][1];
         }
         if (diag)
-            for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
+            for (int i = 0; i < D; ++i) P[pm.dof[i]] = pa[i];
     }
 }
 
@@ -1070,6 +1100,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ TC s_g[MAXI][NEN][D];
     __shared__ TC s_wlm[MAXI][2];            // w lambda, w mu of each uncut incidence
     __shared__ double s_P[MAXI][D];
+    __shared__ double s_dblk[MAXI][D * D];    // diagonal block (la, la) of each incidence
     __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
     __shared__ int s_bad;
 
@@ -1114,6 +1145,9 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
+    PairMeta<D> pm0;
+    if (D == 2 && int(threadIdx.x) < nq) pm0 = pair_meta<D, NG>(p, s_qptr, s_qbeg, s_node, ng, threadIdx.x, jac);
+
     // ---- phase A: incidences ----
     if (staged) {
         for (int k = threadIdx.x; k < nq * CAP; k += blockDim.x) s_list[k / CAP][k % CAP] = 0xffffu;
@@ -1147,11 +1181,13 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
                 const int mi = -1 - tag;
                 if (la >= 0) {
-                    if (jac) {
-                        s_wlm[i][0] = TC(w) * sLM[mi][0];
-                        s_wlm[i][1] = TC(w) * sLM[mi][1];
-                    }
                     if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
+                    if (jac) {
+                        const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+                        s_wlm[i][0] = wl;
+                        s_wlm[i][1] = wm;
+                        diag_block<D, TC>(gt, la, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[i]));
+                    }
                 }
             } else {
                 const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
@@ -1169,6 +1205,10 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     s_koff[i] = int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32);
                     if (func && la >= 0)
                         for (int c = 0; c < D; ++c) s_P[i][c] = Pc[int64_t(c1.y) + la * D + c];
+                    if (jac && la >= 0) {
+                        const TKC* Ke = Kc + s_koff[i] + int64_t(la * nloc + la) * D * D;
+                        for (int k = 0; k < D * D; ++k) s_dblk[i][k] = double(Ke[k]);
+                    }
                 }
             }
             s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
@@ -1194,14 +1234,8 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
 
     // ---- phase B: node pairs ----
     for (int kq = threadIdx.x; kq < nq; kq += blockDim.x) {
-        int lo = 0, hi = ng;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
-        }
-        const int a = s_node[lo];
-        const int q = s_qbeg[lo] + (kq - s_qptr[lo]);  // global node-pair index
-        const int b = p.npat_col[q];
+        const PairMeta<D> pm = (D == 2 && kq == int(threadIdx.x)) ? pm0 : pair_meta<D, NG>(p, s_qptr, s_qbeg, s_node, ng, kq, jac);
+        const int lo = pm.lo, a = pm.a, q = pm.q, b = pm.b;
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
@@ -1217,18 +1251,9 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                 if (la == 255) continue;
                 if (func)
                     for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-                if (!jac) continue;
-                if (!(info & (1 << 16))) {
-                    double dd[D][D];
-                    diag_block<D, TC>(s_g[t], la, s_wlm[t][0], s_wlm[t][1], dd);
+                if (jac)
                     for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += dd[i][j];
-                } else {
-                    const int nl = (info >> 8) & 255;
-                    const TKC* Ke = Kc + s_koff[t] + int64_t(la * nl + la) * D * D;
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
-                }
+                        for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[t][i * D + j];
             }
         } else if (use_lists) {
             // the slot's contributions, stored by rank = ascending element order
@@ -1259,12 +1284,19 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
-            const int32_t* mp = p.npat_map + int64_t(q) * D * D;
-            for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+            if constexpr (D == 2) {
+                K[pm.map.x] = acc[0][0];
+                K[pm.map.y] = acc[0][1];
+                K[pm.map.z] = acc[1][0];
+                K[pm.map.w] = acc[1][1];
+            } else {
+                const int32_t* mp = p.npat_map + int64_t(q) * D * D;
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
+            }
         }
         if (diag)
-            for (int i = 0; i < D; ++i) P[p.node_dof[int64_t(a) * D + i]] = pa[i];
+            for (int i = 0; i < D; ++i) P[pm.dof[i]] = pa[i];
     }
 }
 
