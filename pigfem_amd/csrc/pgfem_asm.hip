@@ -60,10 +60,11 @@ struct DevProb {
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
     const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
     const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
-    const int32_t* cut_loc;    // per cut element, 8 words: n_enr, enr[0..3], P offset, K offset (lo, hi)
     const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
-    const int32_t* ntab;       // per node: node, first incidence, first pair, incidences | pairs << 16
-    const int32_t* glist;      // node-group starts for each group size: groups without cut incidences first
+    // per group size, NG node entries per group in group-list order (groups
+    // without cut incidences first): node, first incidence, first pair,
+    // incidences | pairs << 16; entries past the last node count nothing
+    const int32_t* gtab;
     int n_mat;
 };
 
@@ -904,8 +905,8 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int k0 = p.glist[gbase + xcd_order(blockIdx.x, gridDim.x)];  // first node of the group
-    const int ng = min(NG, p.n_nodes + p.n_enr - k0);
+    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(xcd_order(blockIdx.x, gridDim.x)) * NG;
+    constexpr int ng = NG;
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -916,7 +917,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
         // across the first wavefront (NG <= 32)
         int ci = 0, cq = 0;
         if (int(threadIdx.x) < ng) {
-            const int4 r = reinterpret_cast<const int4*>(p.ntab)[k0 + threadIdx.x];
+            const int4 r = gt[threadIdx.x];
             s_node[threadIdx.x] = r.x;
             s_ibeg[threadIdx.x] = r.y;
             s_qbeg[threadIdx.x] = r.z;
@@ -965,14 +966,14 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             const int rd = i - s_iptr[lo];  // rank among a's incidences
             const int4 rec = reinterpret_cast<const int4*>(p.inc)[tg];
             const uint64_t sw = p.inc_slot[tg];
-            int ln[MAXLOC] = {rec.x, rec.y, rec.z};
-            int la = -1;
-            for (int n = 0; n < NEN; ++n)
-                if (ln[n] == a) la = n;
             const int tag = rec.w;
             if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             if (tag < 0) {
+                const int ln[NEN] = {rec.x, rec.y, rec.z};
+                int la = -1;
+                for (int n = 0; n < NEN; ++n)
+                    if (ln[n] == a) la = n;
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
@@ -998,21 +999,13 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     iso_block<D, TC>(gt[la], gt[m], wl, wm, reinterpret_cast<TC (*)[D]>(s_blk[ps0 + slot][rank]));
                 }
             } else {
-                const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
-                const int4 c0 = cl[0], c1 = cl[1];  // n_enr, enr0..2 | enr3, P offset, K offset lo / hi
-                const int ne = c0.x;
-                const int nloc = NEN + ne;
-                if (nloc > MAXLOC) { s_bad = 1; continue; }  // local list does not fit: global path
-                const int en[4] = {c0.y, c0.z, c0.w, c1.x};
-                for (int k = 0; k < ne; ++k) {
-                    ln[NEN + k] = en[k];
-                    if (ln[NEN + k] == a) la = NEN + k;
-                }
-                if (la < 0) { s_bad = 1; continue; }
+                // cut element: the record holds the staged row of a directly
+                const int la = rec.z & 255, nloc = rec.z >> 8;
+                if (nloc > MAXLOC || la >= nloc) { s_bad = 1; continue; }  // does not fit: global path
                 if (func)
-                    for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(c1.y) + la * D + c];
+                    for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(rec.y) + c];
                 if (!jac) continue;
-                const TC* Ke = Kc + (int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32)) + int64_t(la * nloc) * D * D;
+                const TC* Ke = Kc + int64_t(uint32_t(rec.x)) * (D * D);  // blocks (la, 0 .. nloc - 1)
                 for (int k = 0; k < D * D; ++k) s_dblk[lo][rd][k] = double(Ke[la * D * D + k]);
                 int k = 0;
                 for (int m = 0; m < nloc; ++m) {
@@ -1105,8 +1098,8 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int k0 = p.glist[gbase + xcd_order(blockIdx.x, gridDim.x)];  // first node of the group
-    const int ng = min(NG, p.n_nodes + p.n_enr - k0);
+    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(xcd_order(blockIdx.x, gridDim.x)) * NG;
+    constexpr int ng = NG;
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -1117,7 +1110,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         // across the first wavefront (NG <= 32)
         int ci = 0, cq = 0;
         if (int(threadIdx.x) < ng) {
-            const int4 r = reinterpret_cast<const int4*>(p.ntab)[k0 + threadIdx.x];
+            const int4 r = gt[threadIdx.x];
             s_node[threadIdx.x] = r.x;
             s_ibeg[threadIdx.x] = r.y;
             s_qbeg[threadIdx.x] = r.z;
@@ -1161,15 +1154,15 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             const int a = s_node[lo];
             const int tg = s_ibeg[lo] + (i - s_iptr[lo]);  // global incidence index
             const int32_t* rec = p.inc + int64_t(tg) * (NEN + 1);
-            int ln[MAXLOC];
-            int la = -1;
-            for (int n = 0; n < NEN; ++n) {
-                ln[n] = rec[n];
-                if (ln[n] == a) la = n;
-            }
             const int tag = rec[NEN];
+            int la = -1;
             int nloc = NEN;
             if (tag < 0) {
+                int ln[NEN];
+                for (int n = 0; n < NEN; ++n) {
+                    ln[n] = rec[n];
+                    if (ln[n] == a) la = n;
+                }
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
@@ -1190,23 +1183,18 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     }
                 }
             } else {
-                const int4* cl = reinterpret_cast<const int4*>(p.cut_loc + int64_t(tag) * 8);
-                const int4 c0 = cl[0], c1 = cl[1];  // n_enr, enr0..2 | enr3, P offset, K offset lo / hi
-                const int ne = c0.x;
-                nloc = NEN + ne;
+                // cut element: the record holds the staged row of a directly
+                la = rec[2] & 255;
+                nloc = rec[2] >> 8;
+                if (la >= nloc) la = -1;
                 if (nloc > MAXLOC) {
                     s_bad = 1;  // local list does not fit: this group takes the global path
                 } else {
-                    const int en[4] = {c0.y, c0.z, c0.w, c1.x};
-                    for (int k = 0; k < ne; ++k) {
-                        ln[NEN + k] = en[k];
-                        if (ln[NEN + k] == a) la = NEN + k;
-                    }
-                    s_koff[i] = int64_t(uint32_t(c1.z)) | (int64_t(c1.w) << 32);
+                    s_koff[i] = int64_t(uint32_t(rec[0])) * (D * D);  // blocks (la, 0 .. nloc - 1)
                     if (func && la >= 0)
-                        for (int c = 0; c < D; ++c) s_P[i][c] = Pc[int64_t(c1.y) + la * D + c];
+                        for (int c = 0; c < D; ++c) s_P[i][c] = Pc[int64_t(rec[1]) + c];
                     if (jac && la >= 0) {
-                        const TKC* Ke = Kc + s_koff[i] + int64_t(la * nloc + la) * D * D;
+                        const TKC* Ke = Kc + s_koff[i] + int64_t(la) * D * D;
                         for (int k = 0; k < D * D; ++k) s_dblk[i][k] = double(Ke[k]);
                     }
                 }
@@ -1272,8 +1260,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                         for (int i = 0; i < D; ++i)
                             for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                     } else {
-                        const int nl = (info >> 8) & 255;
-                        const TKC* Ke = Kc + s_koff[t] + int64_t(la * nl + lb) * D * D;
+                        const TKC* Ke = Kc + s_koff[t] + int64_t(lb) * D * D;
                         for (int i = 0; i < D; ++i)
                             for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
                     }
@@ -1323,7 +1310,7 @@ struct pg_asm {
     hipStream_t side = nullptr;     // second stream: node groups without cut elements overlap the cut kernel
     hipEvent_t sync[2] = {nullptr, nullptr};
     int gopt_ng[3] = {0, 0, 0};     // group sizes with a group list
-    int goff[3] = {0, 0, 0};        // offset of each list in glist
+    int goff[3] = {0, 0, 0};        // first gtab entry of each group list
     int ngroups[3] = {0, 0, 0};     // groups of each list
     int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
     bool timing = false;
@@ -1537,7 +1524,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(h->side, h->goff[o], nclean);
+            groups(h->side, h->goff[o], nclean);  // gtab entries of the clean groups
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -1581,7 +1568,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
-        if (overlap) groups(st, h->goff[o] + nclean, nrest);
+        if (overlap) groups(st, h->goff[o] + nclean * h->gopt_ng[o], nrest);
         else groups(st, h->goff[o], h->ngroups[o]);
     }
     HIP_OK(hipGetLastError());
@@ -1631,17 +1618,40 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         eoff[e + 1] = eoff[e] + nd * nd + nd;
     }
     d.n_mat = s->n_mat;
-    // incidence records in n2e order: element nodes, then the cut index (>= 0)
-    // or -1 - material of an uncut element
-    const int64_t n_inc = s->n2e_ptr[nall];
-    std::vector<int32_t> incv(size_t(n_inc) * (s->nen + 1));
-    for (int64_t t = 0; t < n_inc; ++t) {
-        const int e = s->n2e[t];
-        int32_t* r = &incv[size_t(t) * (s->nen + 1)];
-        for (int n = 0; n < s->nen; ++n) r[n] = s->conn[int64_t(e) * s->nen + n];
-        const int c = s->cut_of_elem[e];
-        r[s->nen] = c >= 0 ? c : -1 - s->elem_mat[e];
+    // incidence records in n2e order, nen + 1 words, the last a tag.  Uncut
+    // element: its nodes, tag -1 - material.  Cut element: the staged row of
+    // the owner node a -- block index (in d x d blocks) of block (la, 0) of the
+    // element's tangent, the offset of a's residual entries, la | nloc << 8
+    // (la = 255: a is not a node of the element) -- tag = cut index
+    if (h->cut_k_total / (int64_t(s->dim) * s->dim) >= (int64_t(1) << 32)) {
+        delete h;
+        g_err = "cut-element tangent blocks exceed 2^32";
+        return PG_E_UNSUPPORTED;
     }
+    const int64_t n_inc = s->n2e_ptr[nall];
+    std::vector<int32_t> incv(size_t(n_inc) * (s->nen + 1), 0);
+    for (int64_t a = 0; a < nall; ++a)
+        for (int64_t t = s->n2e_ptr[a]; t < s->n2e_ptr[a + 1]; ++t) {
+            const int e = s->n2e[t];
+            int32_t* r = &incv[size_t(t) * (s->nen + 1)];
+            const int c = s->cut_of_elem[e];
+            if (c < 0) {
+                for (int n = 0; n < s->nen; ++n) r[n] = s->conn[int64_t(e) * s->nen + n];
+                r[s->nen] = -1 - s->elem_mat[e];
+                continue;
+            }
+            const int nloc = s->nen + s->cut[c].n_enr;
+            int la = -1;
+            for (int n = 0; n < s->nen && la < 0; ++n)
+                if (s->conn[int64_t(e) * s->nen + n] == a) la = n;
+            for (int k = 0; k < s->cut[c].n_enr && la < 0; ++k)
+                if (s->cut[c].enr[k] == a) la = s->nen + k;
+            const int lr = la < 0 ? 0 : la;
+            r[0] = int32_t(uint32_t(uint64_t(koff[c] / (int64_t(s->dim) * s->dim) + int64_t(lr) * nloc)));
+            r[1] = int32_t(poff[c] + int64_t(lr) * s->dim);
+            r[2] = (la < 0 ? 255 : la) | (nloc << 8);
+            r[s->nen] = c;
+        }
     // Static contribution slots of the node-group kernel.  For incidence t of
     // node a (element e) and each other local node b of e, in local order: the
     // slot of pair (a, b) within a's row of the node-pair pattern, and the rank
@@ -1728,38 +1738,32 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             tab[size_t(a) * 4 + 2] = s->npat_ptr[a];
             tab[size_t(a) * 4 + 3] = ic | (qc << 16);
         }
-        UP(ntab, tab.data(), tab.size(), int32_t);
         const int opts2[3] = {8, 16, 32}, opts3[3] = {2, 4, 8};
         std::vector<int32_t> gl;
         for (int o = 0; o < 3; ++o) {
             const int NGo = s->dim == 2 ? opts2[o] : opts3[o];
-            std::vector<int32_t> clean, rest;
+            std::vector<int64_t> clean, rest;
             for (int64_t a0 = 0; a0 < nall; a0 += NGo) {
                 bool any = false;
                 for (int64_t a = a0; a < std::min<int64_t>(nall, a0 + NGo) && !any; ++a) any = dirty[a] != 0;
-                (any ? rest : clean).push_back(int32_t(a0));
+                (any ? rest : clean).push_back(a0);
+            }
+            if (int64_t(gl.size()) / 4 + (int64_t(clean.size()) + int64_t(rest.size())) * NGo >= (int64_t(1) << 31)) {
+                g_err = "node-group tables exceed 2^31 entries";
+                return fail(PG_E_UNSUPPORTED);
             }
             h->gopt_ng[o] = NGo;
-            h->goff[o] = int(gl.size());
+            h->goff[o] = int(gl.size() / 4);
             h->ngroups[o] = int(clean.size() + rest.size());
             h->nclean[o] = int(clean.size());
-            gl.insert(gl.end(), clean.begin(), clean.end());
-            gl.insert(gl.end(), rest.begin(), rest.end());
+            for (const auto* lst : {&clean, &rest})
+                for (const int64_t a0 : *lst)
+                    for (int64_t a = a0; a < a0 + NGo; ++a) {
+                        if (a < nall) gl.insert(gl.end(), &tab[size_t(a) * 4], &tab[size_t(a) * 4] + 4);
+                        else gl.insert(gl.end(), {0, 0, 0, 0});
+                    }
         }
-        UP(glist, gl.data(), gl.size(), int32_t);
-    }
-    {
-        std::vector<int32_t> cl(size_t(s->n_cut) * 8, 0);
-        for (int c = 0; c < s->n_cut; ++c) {
-            const pg_cut_rec& r = s->cut[c];
-            int32_t* o = &cl[size_t(c) * 8];
-            o[0] = r.n_enr;
-            for (int k = 0; k < 4 && k < r.n_enr; ++k) o[1 + k] = r.enr[k];
-            o[5] = int32_t(poff[c]);
-            o[6] = int32_t(uint32_t(uint64_t(koff[c]) & 0xffffffffu));
-            o[7] = int32_t(uint64_t(koff[c]) >> 32);
-        }
-        UP(cut_loc, cl.data(), cl.size(), int32_t);
+        UP(gtab, gl.data(), gl.size(), int32_t);
     }
 #undef UP
     {
