@@ -1,14 +1,20 @@
-"""Newton history of the 500 x 500 bilinear-cohesive RVE (C2: 200 fibers,
-10 load steps to 2 % uniaxial strain), restating PIGFEM's load stepping
-(src/SolverNonlinear.cpp:70-510, src/SolverNonlinearKSP.cpp:55-155).
+"""Newton load-step histories of the cohesive RVEs, restating PIGFEM's load
+stepping (src/SolverNonlinear.cpp:70-510, src/SolverNonlinearKSP.cpp:55-155).
 
-  python tests/golden/newton_c2.py --oracle
-      CPU oracle assembly; writes tests/golden/newton_c2_oracle.json (the
+Cases:
+  c2    the 500 x 500 bilinear-cohesive RVE with 200 fibers, 10 load steps to
+        2 % uniaxial strain;
+  c5x4  the four smallest RVEs (64-66 elements a side) of the 256-RVE batch,
+        merged into one block-diagonal problem as the batched assembly sees
+        them, the same load history.
+
+  python tests/golden/newton_history.py --case c2 --oracle
+      CPU oracle assembly; writes tests/golden/newton_<case>_oracle.json (the
       committed fixture: iteration counts of every attempted load step, the
       explicit steps, the time reached and how the run ended).
-  python tests/golden/newton_c2.py --modes fp64,mixed [--out DIR]
+  python tests/golden/newton_history.py --case c2 --modes fp64,mixed [--out DIR]
       the same history with the GPU assembly (one MI355X); prints each step,
-      writes DIR/newton_c2_<mode>.json and compares with the fixture.
+      writes DIR/newton_<case>_<mode>.json and compares with the fixture.
 """
 import argparse
 import json
@@ -19,10 +25,22 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from pigfem_amd import RVEProblem, config  # noqa: E402
+from pigfem_amd import RVEProblem, batch_config, config  # noqa: E402
 from pigfem_amd.newton import NewtonDriver, NewtonError  # noqa: E402
 
-FIXTURE = os.path.join(HERE, "newton_c2_oracle.json")
+
+def fixture(case):
+    return os.path.join(HERE, f"newton_{case}_oracle.json")
+
+
+def problem(case):
+    if case == "c2":
+        return RVEProblem(config("C2"))
+    if case == "c5x4":
+        cfgs = batch_config(256)
+        pick = sorted(range(len(cfgs)), key=lambda i: (cfgs[i].n, i))[:4]
+        return RVEProblem.batch([cfgs[i] for i in pick])
+    raise ValueError(f"unknown case {case}")
 
 
 def run(prob, assemble, label):
@@ -50,28 +68,29 @@ def compare(ref, out):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="c2", choices=("c2", "c5x4"))
     ap.add_argument("--oracle", action="store_true")
     ap.add_argument("--modes", default="")
     ap.add_argument("--out", default="gpurun_out")
     args = ap.parse_args()
-    prob = RVEProblem(config("C2"))
-    print("C2:", prob.stats(), flush=True)
+    prob = problem(args.case)
+    print(f"{args.case}:", prob.stats(), flush=True)
     if args.oracle:
         from oracle.oracle import Oracle
         o = Oracle(prob)
         res = run(prob, lambda U, isv: o.assemble(U, isv), "oracle")
-        with open(FIXTURE, "w") as f:
+        with open(fixture(args.case), "w") as f:
             json.dump(res, f)
         print(json.dumps({k: res[k] for k in ("status", "t_reached", "seconds")}))
     if args.modes:
         from pigfem_amd import AssemblerGPU
         from pigfem_amd.newton import gpu_assemble_fn
-        ref = json.load(open(FIXTURE))
+        ref = json.load(open(fixture(args.case)))
         os.makedirs(args.out, exist_ok=True)
         for mode in args.modes.split(","):
             res = run(prob, gpu_assemble_fn(AssemblerGPU(prob, mode)), mode)
             res["vs_oracle"] = compare(ref, res)
-            with open(os.path.join(args.out, f"newton_c2_{mode}.json"), "w") as f:
+            with open(os.path.join(args.out, f"newton_{args.case}_{mode}.json"), "w") as f:
                 json.dump(res, f)
             print(json.dumps({"mode": mode, "status": res["status"], "t_reached": res["t_reached"],
                               "vs_oracle": res["vs_oracle"], "seconds": res["seconds"]}), flush=True)
