@@ -65,6 +65,16 @@ struct DevProb {
     // without cut incidences first): node, first incidence, first pair,
     // incidences | pairs << 16; entries past the last node count nothing
     const int32_t* gtab;
+    // tri3 block-table kernel: per group size, each group's incidence records
+    // and slot words in a block of 7 NG slots (group-list order, then the
+    // incidence order of the group's nodes), so that a lane loads its record
+    // without the node table.  Uncut: the element's nodes and a word
+    // 1 << 31 | material | lo << 3 | rank << 8 | la << 13; cut: block index of
+    // the owner's staged row, residual offset, la | nloc << 8 | lo << 16 |
+    // rank << 21, cut index (lo: the owner's position in the group; rank: the
+    // incidence's position among the owner's; la: the owner's local index)
+    const int32_t* incg;
+    const uint64_t* slotg;
     int n_mat;
 };
 
@@ -884,7 +894,7 @@ template <int D, typename TC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::
 __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __restrict__ Un,
                                                     const TC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
-                                                    int gbase) {
+                                                    int gbase, int64_t ibase) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -905,8 +915,18 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(xcd_order(blockIdx.x, gridDim.x)) * NG;
+    const int grp = xcd_order(blockIdx.x, gridDim.x);
+    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(grp) * NG;
     constexpr int ng = NG;
+    // the group's incidence records and slot words do not depend on the node
+    // table: their loads go out together with it
+    const int64_t ib = ibase + int64_t(grp) * MAXI;
+    int4 rec0 = make_int4(0, 0, 0, 0);
+    uint64_t sw0 = 0;
+    if (int(threadIdx.x) < MAXI) {
+        rec0 = reinterpret_cast<const int4*>(p.incg)[ib + threadIdx.x];
+        sw0 = p.slotg[ib + threadIdx.x];
+    }
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -956,32 +976,24 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             s_blk[k / (CAPO * D * D)][(k / (D * D)) % CAPO][k % (D * D)] = TC(0);
         __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
-            int lo = 0, hi = ng;  // owner node of incidence i
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_iptr[mid] <= i) lo = mid; else hi = mid;
-            }
-            const int a = s_node[lo];
-            const int tg = s_ibeg[lo] + (i - s_iptr[lo]);  // global incidence index
-            const int rd = i - s_iptr[lo];  // rank among a's incidences
-            const int4 rec = reinterpret_cast<const int4*>(p.inc)[tg];
-            const uint64_t sw = p.inc_slot[tg];
+            // staged: ni <= MAXI, so slot i lies in the group's block
+            const int4 rec = i == int(threadIdx.x) ? rec0 : reinterpret_cast<const int4*>(p.incg)[ib + i];
+            const uint64_t sw = i == int(threadIdx.x) ? sw0 : p.slotg[ib + i];
             const int tag = rec.w;
+            const int lo = tag < 0 ? (tag >> 3) & 31 : (rec.z >> 16) & 31;  // owner's position in the group
+            const int rd = tag < 0 ? (tag >> 8) & 31 : (rec.z >> 21) & 31;  // rank among the owner's incidences
             if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             if (tag < 0) {
                 const int ln[NEN] = {rec.x, rec.y, rec.z};
-                int la = -1;
-                for (int n = 0; n < NEN; ++n)
-                    if (ln[n] == a) la = n;
+                const int la = (tag >> 13) & 3;
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
                 double g[NEN][D];
                 const double J = simplex_grads<D, double>(x, g);
                 const double w = J * wsum;
-                const int mi = -1 - tag;
-                if (la < 0) { s_bad = 1; continue; }
+                const int mi = tag & 7;
                 if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_dP[lo][rd]);
                 if (!jac) continue;
                 TC gt[NEN][D];
@@ -1000,7 +1012,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                 }
             } else {
                 // cut element: the record holds the staged row of a directly
-                const int la = rec.z & 255, nloc = rec.z >> 8;
+                const int la = rec.z & 255, nloc = (rec.z >> 8) & 255;
                 if (nloc > MAXLOC || la >= nloc) { s_bad = 1; continue; }  // does not fit: global path
                 if (func)
                     for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(rec.y) + c];
@@ -1311,6 +1323,7 @@ struct pg_asm {
     hipEvent_t sync[2] = {nullptr, nullptr};
     int gopt_ng[3] = {0, 0, 0};     // group sizes with a group list
     int goff[3] = {0, 0, 0};        // first gtab entry of each group list
+    int64_t gioff[3] = {0, 0, 0};   // first incg / slotg slot of each group list (tri3)
     int ngroups[3] = {0, 0, 0};     // groups of each list
     int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
     bool timing = false;
@@ -1440,14 +1453,14 @@ int alloc(pg_asm* h, T** p, size_t n) {
 // Node-group kernel launch: the block-table form for tri3 meshes unless the
 // list form is asked for; tet4 meshes always use the list form.
 template <int D, typename TC, typename TKC, int NG, int TV>
-void launch_group(bool lists, int gbase, int ngroups, hipStream_t st, const DevProb& d, const double* Un,
-                  const TKC* Kc, const double* Pc, double* K, double* P, unsigned flags) {
+void launch_group(bool lists, int gbase, int64_t ibase, int ngroups, hipStream_t st, const DevProb& d,
+                  const double* Un, const TKC* Kc, const double* Pc, double* K, double* P, unsigned flags) {
     constexpr int BS = GroupThreads<D, NG, TV>::value;
     if (ngroups <= 0) return;
     const unsigned grid = unsigned(ngroups);
     if constexpr (D == 2 && std::is_same<TC, TKC>::value) {
         if (!lists) {
-            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase);
+            k_pair_group<D, TC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase, ibase);
             return;
         }
     }
@@ -1483,9 +1496,9 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         while (o < 2 && h->gopt_ng[o] != ng) ++o;
         const int nclean = h->nclean[o], nrest = h->ngroups[o] - h->nclean[o];
         const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
-        // node groups: BASE .. BASE + LEN of the group list on stream s
+        // LEN node groups on stream S: node-table entries from BASE, incidence slots from ibase
 #define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
-    launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
+    launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, ibase, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
 #define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                                 \
     do {                                                                         \
         if (h->group_variant & 1) GROUP1(DD, TC, TKC, KC, NGV, 1, S, BASE, LEN); \
@@ -1503,7 +1516,10 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         else if (ng == 4) GROUP(3, TC, TKC, KC, 4, S, BASE, LEN);       \
         else GROUP(3, TC, TKC, KC, 8, S, BASE, LEN);                    \
     } while (0)
-        auto groups = [&](hipStream_t s, int base, int len) {
+        // node groups first .. first + len - 1 of the list of this group size
+        auto groups = [&](hipStream_t s, int first, int len) {
+            const int base = h->goff[o] + first * h->gopt_ng[o];
+            const int64_t ibase = h->gioff[o] + int64_t(first) * 7 * h->gopt_ng[o];
             if (d.dim == 2) {
                 if (!fp32) GROUPS2(double, double, h->Kc64, s, base, len);
                 else if (cut64) GROUPS2(float, double, h->Kc64, s, base, len);
@@ -1524,7 +1540,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(h->side, h->goff[o], nclean);  // gtab entries of the clean groups
+            groups(h->side, 0, nclean);
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -1568,8 +1584,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
-        if (overlap) groups(st, h->goff[o] + nclean * h->gopt_ng[o], nrest);
-        else groups(st, h->goff[o], h->ngroups[o]);
+        if (overlap) groups(st, nclean, nrest);
+        else groups(st, 0, h->ngroups[o]);
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1764,6 +1780,48 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
                     }
         }
         UP(gtab, gl.data(), gl.size(), int32_t);
+        if (s->dim == 2) {
+            std::vector<int32_t> gi;
+            std::vector<uint64_t> gs;
+            for (int o = 0; o < 3; ++o) {
+                const int NGo = h->gopt_ng[o], MAXIo = 7 * NGo;
+                h->gioff[o] = int64_t(gs.size());
+                gi.resize(gi.size() + size_t(h->ngroups[o]) * MAXIo * 4, 0);
+                gs.resize(gs.size() + size_t(h->ngroups[o]) * MAXIo, 0);
+                for (int g = 0; g < h->ngroups[o]; ++g) {
+                    const size_t base = size_t(h->gioff[o]) + size_t(g) * MAXIo;
+                    int i = 0;
+                    for (int lo = 0; lo < NGo; ++lo) {
+                        const int32_t* e4 = &gl[(size_t(h->goff[o]) + size_t(g) * NGo + lo) * 4];
+                        const int ic = e4[3] & 0xffff;
+                        if (ic == 0) continue;  // padding entry past the last node
+                        const int64_t a = e4[0];
+                        for (int r = 0; r < ic; ++r, ++i) {
+                            if (i >= MAXIo) break;  // the group overflows: it takes the global path
+                            const int64_t t = e4[1] + r;
+                            const int32_t* src = &incv[size_t(t) * 4];
+                            int32_t* dst = &gi[(base + i) * 4];
+                            const int rk = std::min(r, 31);
+                            if (src[3] < 0) {
+                                int la = 0;
+                                for (int n = 0; n < 3; ++n)
+                                    if (src[n] == a) la = n;
+                                dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+                                dst[3] = int32_t(0x80000000u | uint32_t(-1 - src[3]) | uint32_t(lo) << 3 |
+                                                 uint32_t(rk) << 8 | uint32_t(la) << 13);
+                            } else {
+                                dst[0] = src[0]; dst[1] = src[1];
+                                dst[2] = (src[2] & 0xffff) | (lo << 16) | (rk << 21);
+                                dst[3] = src[3];
+                            }
+                            gs[base + i] = slotv[size_t(t)];
+                        }
+                    }
+                }
+            }
+            UP(incg, gi.data(), gi.size(), int32_t);
+            UP(slotg, gs.data(), gs.size(), uint64_t);
+        }
     }
 #undef UP
     {
