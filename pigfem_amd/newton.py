@@ -129,7 +129,7 @@ class NewtonDriver:
 
     def __init__(self, problem, assemble, T_final=1.0, max_time_step=0.1, min_time_step=1e-5,
                  rel_tol=1e-8, abs_tol=1e-8, max_iter=25, max_repeats=20, explicit_steps=100,
-                 solver="pardiso", log=None):
+                 solver="pardiso", log=None, max_steps=None):
         self.p = problem
         self.assemble = assemble
         self.explicit_steps = explicit_steps
@@ -137,6 +137,7 @@ class NewtonDriver:
         self.T_final, self.max_dt, self.min_dt = T_final, max_time_step, min_time_step
         self.rel_tol, self.abs_tol, self.max_iter, self.max_repeats = rel_tol, abs_tol, max_iter, max_repeats
         self.log = log
+        self.max_steps = max_steps  # stop after this many load steps (None: run to T_final)
         self.history: list[StepRecord] = []
         self.U = np.zeros(problem.n_dof)
         self.isv = np.zeros(problem.n_cqp)
@@ -206,7 +207,7 @@ class NewtonDriver:
         p = self.p
         t, dt, step = 0.0, self.max_dt, 0
         bc = np.asarray(p.bc_values)
-        while t < self.T_final:
+        while t < self.T_final and (self.max_steps is None or step < self.max_steps):
             repeats, converged, last_failed = 0, False, False
             while not converged:
                 if repeats == self.max_repeats or dt < self.min_dt:

@@ -4,6 +4,9 @@ stepping (src/SolverNonlinear.cpp:70-510, src/SolverNonlinearKSP.cpp:55-155).
 Cases:
   c2    the 500 x 500 bilinear-cohesive RVE with 200 fibers, 10 load steps to
         2 % uniaxial strain;
+  c3s5  the 2000 x 2000 Ortiz-Pandolfi RVE with 3000 fibers, the same load
+        history, first five load steps only (each Newton iteration factorises
+        a 9.9 M-dof system on the host);
   c5x4  the four smallest RVEs (64-66 elements a side) of the 256-RVE batch,
         merged into one block-diagonal problem as the batched assembly sees
         them, the same load history.
@@ -36,6 +39,8 @@ def fixture(case):
 def problem(case):
     if case == "c2":
         return RVEProblem(config("C2"))
+    if case == "c3s5":
+        return RVEProblem(config("C3"))
     if case == "c5x4":
         cfgs = batch_config(256)
         pick = sorted(range(len(cfgs)), key=lambda i: (cfgs[i].n, i))[:4]
@@ -43,10 +48,10 @@ def problem(case):
     raise ValueError(f"unknown case {case}")
 
 
-def run(prob, assemble, label):
+def run(prob, assemble, label, max_steps=None):
     t0 = time.time()
     log = lambda m: print(f"[{label}] {m} [{time.time() - t0:.0f}s]", flush=True)
-    d = NewtonDriver(prob, assemble, log=log)
+    d = NewtonDriver(prob, assemble, log=log, max_steps=max_steps)
     status = "converged"
     try:
         d.solve()
@@ -68,17 +73,18 @@ def compare(ref, out):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="c2", choices=("c2", "c5x4"))
+    ap.add_argument("--case", default="c2", choices=("c2", "c3s5", "c5x4"))
     ap.add_argument("--oracle", action="store_true")
     ap.add_argument("--modes", default="")
     ap.add_argument("--out", default="gpurun_out")
     args = ap.parse_args()
     prob = problem(args.case)
+    max_steps = 5 if args.case == "c3s5" else None
     print(f"{args.case}:", prob.stats(), flush=True)
     if args.oracle:
         from oracle.oracle import Oracle
         o = Oracle(prob)
-        res = run(prob, lambda U, isv: o.assemble(U, isv), "oracle")
+        res = run(prob, lambda U, isv: o.assemble(U, isv), "oracle", max_steps)
         with open(fixture(args.case), "w") as f:
             json.dump(res, f)
         print(json.dumps({k: res[k] for k in ("status", "t_reached", "seconds")}))
@@ -88,7 +94,7 @@ def main():
         ref = json.load(open(fixture(args.case)))
         os.makedirs(args.out, exist_ok=True)
         for mode in args.modes.split(","):
-            res = run(prob, gpu_assemble_fn(AssemblerGPU(prob, mode)), mode)
+            res = run(prob, gpu_assemble_fn(AssemblerGPU(prob, mode)), mode, max_steps)
             res["vs_oracle"] = compare(ref, res)
             with open(os.path.join(args.out, f"newton_{args.case}_{mode}.json"), "w") as f:
                 json.dump(res, f)
