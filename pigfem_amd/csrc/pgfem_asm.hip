@@ -59,16 +59,15 @@ struct DevProb {
     const int64_t* cut_koff;   // per cut element: offset of its ndof^2 block
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
     const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
-    const int32_t* inc;        // per incidence (n2e order): element nodes (nen) + cut index, or -1 - material
-    const uint64_t* inc_slot;  // per incidence: 9 bits per non-owner local node (pair slot | rank << 6); bit 63: none
     // per group size, NG node entries per group in group-list order (groups
     // without cut incidences first): node, first incidence, first pair,
     // incidences | pairs << 16; entries past the last node count nothing
     const int32_t* gtab;
-    // tri3 block-table kernel: per group size, each group's incidence records
-    // and slot words in a block of 7 NG slots (group-list order, then the
-    // incidence order of the group's nodes), so that a lane loads its record
-    // without the node table.  Uncut: the element's nodes and a word
+    // node-group kernels: per group size, each group's incidence records (d + 2
+    // words) and slot words in a block of 7 NG (tri3) or 24 NG (tet4) slots
+    // (group-list order, then the incidence order of the group's nodes), so
+    // that a lane loads its record without the node table.  Uncut: the
+    // element's nodes and a word
     // 1 << 31 | material | lo << 3 | rank << 8 | la << 13; cut: block index of
     // the owner's staged row, residual offset, la | nloc << 8 | lo << 16 |
     // rank << 21, cut index (lo: the owner's position in the group; rank: the
@@ -1086,7 +1085,7 @@ template <int D, typename TC, typename TKC, int NG, int TV, int BS = GroupThread
 __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TKC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
-                                                    int gbase) {
+                                                    int gbase, int64_t ibase) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
     constexpr int MAXI = GroupLim<D, NG>::MAXI, MAXP = GroupLim<D, NG>::MAXP;
     constexpr int MAXLOC = (D == 2 ? 7 : 8);  // local nodes per incidence
@@ -1110,8 +1109,19 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
-    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(xcd_order(blockIdx.x, gridDim.x)) * NG;
+    const int grp = xcd_order(blockIdx.x, gridDim.x);
+    const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(grp) * NG;
     constexpr int ng = NG;
+    // the first pass's incidence records and slot words do not depend on the
+    // node table: their loads go out together with it
+    const int64_t ib = ibase + int64_t(grp) * MAXI;
+    int rec0[NEN + 1];
+    uint64_t sw0 = 0;
+    for (int n = 0; n <= NEN; ++n) rec0[n] = 0;
+    if (int(threadIdx.x) < MAXI) {
+        for (int n = 0; n <= NEN; ++n) rec0[n] = p.incg[(ib + threadIdx.x) * (NEN + 1) + n];
+        sw0 = p.slotg[ib + threadIdx.x];
+    }
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
@@ -1158,23 +1168,19 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
         for (int k = threadIdx.x; k < nq * CAP; k += blockDim.x) s_list[k / CAP][k % CAP] = 0xffffu;
         __syncthreads();
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
-            int lo = 0, hi = ng;  // owner node of incidence i
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_iptr[mid] <= i) lo = mid; else hi = mid;
-            }
-            const int a = s_node[lo];
-            const int tg = s_ibeg[lo] + (i - s_iptr[lo]);  // global incidence index
-            const int32_t* rec = p.inc + int64_t(tg) * (NEN + 1);
+            // staged: ni <= MAXI, so slot i lies in the group's block
+            const bool first = i == int(threadIdx.x);
+            int rec[NEN + 1];
+            for (int n = 0; n <= NEN; ++n) rec[n] = first ? rec0[n] : p.incg[(ib + i) * (NEN + 1) + n];
+            const uint64_t sw = first ? sw0 : p.slotg[ib + i];
             const int tag = rec[NEN];
+            const int lo = tag < 0 ? (tag >> 3) & 31 : (rec[2] >> 16) & 31;  // owner's position in the group
             int la = -1;
             int nloc = NEN;
             if (tag < 0) {
                 int ln[NEN];
-                for (int n = 0; n < NEN; ++n) {
-                    ln[n] = rec[n];
-                    if (ln[n] == a) la = n;
-                }
+                for (int n = 0; n < NEN; ++n) ln[n] = rec[n];
+                la = (tag >> 13) & 3;
                 double x[NEN][3];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
@@ -1184,7 +1190,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                 TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
-                const int mi = -1 - tag;
+                const int mi = tag & 7;
                 if (la >= 0) {
                     if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
                     if (jac) {
@@ -1197,7 +1203,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             } else {
                 // cut element: the record holds the staged row of a directly
                 la = rec[2] & 255;
-                nloc = rec[2] >> 8;
+                nloc = (rec[2] >> 8) & 255;
                 if (la >= nloc) la = -1;
                 if (nloc > MAXLOC) {
                     s_bad = 1;  // local list does not fit: this group takes the global path
@@ -1215,7 +1221,6 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             if (la < 0 || nloc > MAXLOC) continue;
             // contribution of this incidence to each off-diagonal pair (a, ln[m]) of
             // a's row, at its precomputed slot and rank (ascending element order)
-            const uint64_t sw = p.inc_slot[tg];
             if (sw >> 63) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             int k = 0;
@@ -1323,7 +1328,7 @@ struct pg_asm {
     hipEvent_t sync[2] = {nullptr, nullptr};
     int gopt_ng[3] = {0, 0, 0};     // group sizes with a group list
     int goff[3] = {0, 0, 0};        // first gtab entry of each group list
-    int64_t gioff[3] = {0, 0, 0};   // first incg / slotg slot of each group list (tri3)
+    int64_t gioff[3] = {0, 0, 0};   // first incg / slotg slot of each group list
     int ngroups[3] = {0, 0, 0};     // groups of each list
     int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
     bool timing = false;
@@ -1464,7 +1469,7 @@ void launch_group(bool lists, int gbase, int64_t ibase, int ngroups, hipStream_t
             return;
         }
     }
-    k_pair_lists<D, TC, TKC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase);
+    k_pair_lists<D, TC, TKC, NG, TV><<<grid, BS, 0, st>>>(d, Un, Kc, Pc, K, P, flags, gbase, ibase);
 }
 
 // Schedules: no PG_FP32 / PG_BATCHED -> plain FP64 (element kernel over all
@@ -1519,7 +1524,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         // node groups first .. first + len - 1 of the list of this group size
         auto groups = [&](hipStream_t s, int first, int len) {
             const int base = h->goff[o] + first * h->gopt_ng[o];
-            const int64_t ibase = h->gioff[o] + int64_t(first) * 7 * h->gopt_ng[o];
+            const int64_t ibase = h->gioff[o] + int64_t(first) * (d.dim == 2 ? 7 : 24) * h->gopt_ng[o];
             if (d.dim == 2) {
                 if (!fp32) GROUPS2(double, double, h->Kc64, s, base, len);
                 else if (cut64) GROUPS2(float, double, h->Kc64, s, base, len);
@@ -1634,7 +1639,8 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         eoff[e + 1] = eoff[e] + nd * nd + nd;
     }
     d.n_mat = s->n_mat;
-    // incidence records in n2e order, nen + 1 words, the last a tag.  Uncut
+    // incidence records in n2e order (host only; the kernels read them regrouped
+    // per node group below), nen + 1 words, the last a tag.  Uncut
     // element: its nodes, tag -1 - material.  Cut element: the staged row of
     // the owner node a -- block index (in d x d blocks) of block (la, 0) of the
     // element's tangent, the offset of a's residual entries, la | nloc << 8
@@ -1732,8 +1738,6 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
     UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
-    UP(inc, incv.data(), incv.size(), int32_t);
-    UP(inc_slot, slotv.data(), slotv.size(), uint64_t);
     {
         // node table, and for each group size the list of group starts: groups
         // none of whose nodes touches a cut element first (they can run while
@@ -1780,13 +1784,14 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
                     }
         }
         UP(gtab, gl.data(), gl.size(), int32_t);
-        if (s->dim == 2) {
+        {
+            const int W = s->nen + 1;  // words per record
             std::vector<int32_t> gi;
             std::vector<uint64_t> gs;
             for (int o = 0; o < 3; ++o) {
-                const int NGo = h->gopt_ng[o], MAXIo = 7 * NGo;
+                const int NGo = h->gopt_ng[o], MAXIo = (s->dim == 2 ? 7 : 24) * NGo;
                 h->gioff[o] = int64_t(gs.size());
-                gi.resize(gi.size() + size_t(h->ngroups[o]) * MAXIo * 4, 0);
+                gi.resize(gi.size() + size_t(h->ngroups[o]) * MAXIo * W, 0);
                 gs.resize(gs.size() + size_t(h->ngroups[o]) * MAXIo, 0);
                 for (int g = 0; g < h->ngroups[o]; ++g) {
                     const size_t base = size_t(h->gioff[o]) + size_t(g) * MAXIo;
@@ -1799,20 +1804,21 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
                         for (int r = 0; r < ic; ++r, ++i) {
                             if (i >= MAXIo) break;  // the group overflows: it takes the global path
                             const int64_t t = e4[1] + r;
-                            const int32_t* src = &incv[size_t(t) * 4];
-                            int32_t* dst = &gi[(base + i) * 4];
+                            const int32_t* src = &incv[size_t(t) * W];
+                            int32_t* dst = &gi[(base + i) * W];
                             const int rk = std::min(r, 31);
-                            if (src[3] < 0) {
+                            if (src[s->nen] < 0) {
                                 int la = 0;
-                                for (int n = 0; n < 3; ++n)
+                                for (int n = 0; n < s->nen; ++n) {
+                                    dst[n] = src[n];
                                     if (src[n] == a) la = n;
-                                dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
-                                dst[3] = int32_t(0x80000000u | uint32_t(-1 - src[3]) | uint32_t(lo) << 3 |
-                                                 uint32_t(rk) << 8 | uint32_t(la) << 13);
+                                }
+                                dst[s->nen] = int32_t(0x80000000u | uint32_t(-1 - src[s->nen]) | uint32_t(lo) << 3 |
+                                                      uint32_t(rk) << 8 | uint32_t(la) << 13);
                             } else {
                                 dst[0] = src[0]; dst[1] = src[1];
                                 dst[2] = (src[2] & 0xffff) | (lo << 16) | (rk << 21);
-                                dst[3] = src[3];
+                                dst[s->nen] = src[s->nen];
                             }
                             gs[base + i] = slotv[size_t(t)];
                         }
