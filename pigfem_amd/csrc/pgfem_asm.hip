@@ -828,6 +828,30 @@ __device__ inline void diag_block(const TC (*gt)[D], int la, TC wl, TC wm, doubl
     }
 }
 
+// Diagonal block (la, la) of a staged cut-element tangent, from the element's
+// row la (Ke points at block (la, 0)).  With FP32 tangents and la a parent
+// node it is minus the FP64 sum of the row's other parent-node blocks: the
+// parent gradients sum to zero at every child qp and the cohesive fold couples
+// only enrichment nodes, so the exact block equals that sum, and taking it this
+// way keeps rigid translations (parent dofs equal, enrichment dofs zero) in the
+// null space of the assembled tangent, as for uncut elements.  Otherwise the
+// staged block itself.
+template <int D, typename TK>
+__device__ inline void cut_diag(const TK* Ke, int la, double* out) {
+    constexpr int NEN = D + 1;
+    if constexpr (sizeof(TK) == 4) {
+        if (la < NEN) {
+            for (int k = 0; k < D * D; ++k) out[k] = 0.0;
+            for (int m = 0; m < NEN; ++m) {
+                if (m == la) continue;
+                for (int k = 0; k < D * D; ++k) out[k] -= double(Ke[m * D * D + k]);
+            }
+            return;
+        }
+    }
+    for (int k = 0; k < D * D; ++k) out[k] = double(Ke[la * D * D + k]);
+}
+
 // One node pair (a, b) from global memory: the incidences [t0, t1) of a in
 // element order, with the arithmetic of the LDS-staged paths (used by groups
 // whose incidences or pairs exceed the LDS tables).
@@ -878,9 +902,15 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
         } else {
             const int nl = NEN + p.cut[ci].n_enr;
             if (jac) {
-                const TKC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl + lb) * D * D;
+                const TKC* Ke = Kc + p.cut_koff[ci] + int64_t(la * nl) * D * D;  // row la
+                double blk[D * D];
+                if (la == lb) {
+                    cut_diag<D, TKC>(Ke, la, blk);
+                } else {
+                    for (int k = 0; k < D * D; ++k) blk[k] = double(Ke[lb * D * D + k]);
+                }
                 for (int i = 0; i < D; ++i)
-                    for (int j = 0; j < D; ++j) acc[i][j] += double(Ke[i * D + j]);
+                    for (int j = 0; j < D; ++j) acc[i][j] += blk[i * D + j];
             }
             if (diag)
                 for (int i = 0; i < D; ++i) pa[i] += Pc[p.cut_poff[ci] + la * D + i];
@@ -1017,7 +1047,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(rec.y) + c];
                 if (!jac) continue;
                 const TC* Ke = Kc + int64_t(uint32_t(rec.x)) * (D * D);  // blocks (la, 0 .. nloc - 1)
-                for (int k = 0; k < D * D; ++k) s_dblk[lo][rd][k] = double(Ke[la * D * D + k]);
+                cut_diag<D, TC>(Ke, la, s_dblk[lo][rd]);
                 int k = 0;
                 for (int m = 0; m < nloc; ++m) {
                     if (m == la) continue;
@@ -1211,10 +1241,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     s_koff[i] = int64_t(uint32_t(rec[0])) * (D * D);  // blocks (la, 0 .. nloc - 1)
                     if (func && la >= 0)
                         for (int c = 0; c < D; ++c) s_P[i][c] = Pc[int64_t(rec[1]) + c];
-                    if (jac && la >= 0) {
-                        const TKC* Ke = Kc + s_koff[i] + int64_t(la) * D * D;
-                        for (int k = 0; k < D * D; ++k) s_dblk[i][k] = double(Ke[k]);
-                    }
+                    if (jac && la >= 0) cut_diag<D, TKC>(Kc + s_koff[i], la, s_dblk[i]);
                 }
             }
             s_info[i] = (la < 0 ? 255 : la) | (nloc << 8) | (tag >= 0 ? (1 << 16) : 0);
