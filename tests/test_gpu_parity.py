@@ -10,7 +10,7 @@ import pytest
 
 from pigfem_amd import AssemblerGPU, PMatrix, PVector, RVEProblem, config, small_config
 from oracle.oracle import Oracle
-from tests.helpers import rel, u_scale
+from tests.helpers import full_matrix, rel, u_scale
 
 pytestmark = pytest.mark.gpu
 
@@ -87,6 +87,38 @@ def test_c1_full_config_fp64_and_mixed():
         K, P, _ = _run(AssemblerGPU(p, mode), p, U, np.zeros(0))
         print(mode, "relK", rel(K, Kr), "relP", rel(P, Pr), flush=True)
         assert rel(K, Kr) < tol and rel(P, Pr) < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed", "mixed_cut64"])
+def test_tangent_keeps_rigid_translations(case, mode):
+    """A rigid translation (regular-node dofs equal, enrichment dofs zero) is a
+    zero-energy mode of the exact tangent: the parent gradients sum to zero and
+    the cohesive fold couples only enrichment nodes.  Every schedule keeps it to
+    FP64 rounding on the rows of regular nodes -- the FP32 ones because their
+    diagonal blocks are taken as minus the FP64 sum of the row's other (parent)
+    blocks; element-wise FP32 rounding alone leaves about 1e-8 of the row scale,
+    which costs Newton iterations on large meshes.  Enrichment rows hold it to
+    the tangent's own precision."""
+    p, U, ic, Kr, Pr, Ir = case
+    K, _, _ = _run(AssemblerGPU(p, mode), p, U, ic)
+    A = full_matrix(p, K)
+    Ar = full_matrix(p, Kr)
+    reg = p.node_dof[: p.n_nodes].ravel()
+    enr = p.node_dof[p.n_nodes:].ravel()
+    s_reg = abs(Ar[reg]).max()  # entry scale of each row class (cohesive
+    s_enr = abs(Ar[enr]).max() if len(enr) else 0.0  # springs sit on enrichment rows)
+    Af = full_matrix(p, Kr.astype(np.float32).astype(np.float64))
+    for comp in range(p.dim):
+        V = np.zeros(p.n_dof)
+        V[p.node_dof[: p.n_nodes, comp]] = 1.0
+        assert np.abs((Ar @ V)[reg]).max() <= 1e-12 * s_reg  # the premise, on the oracle
+        # and the check has teeth: the oracle's tangent rounded to FP32 entry by entry fails it
+        assert np.abs((Af @ V)[reg]).max() > 1e-12 * s_reg
+        r = A @ V
+        assert np.abs(r[reg]).max() <= 1e-12 * s_reg, mode
+        if len(enr):
+            tol = 1e-12 if mode in ("fp64", "binned64", "mixed_cut64") else 1e-6
+            assert np.abs(r[enr]).max() <= tol * s_enr, mode
 
 
 def test_device_api_matches_host_api():
