@@ -74,6 +74,14 @@ struct DevProb {
     // incidence's position among the owner's; la: the owner's local index)
     const int32_t* incg;
     const uint64_t* slotg;
+    // cut-kernel inputs in class order (small class, then general): the cut
+    // records, each element's local node list (parent nodes, then enrichment
+    // nodes; -1 past the last; CUT_NODES words) and the offsets of its staged
+    // tangent and residual, so that a lane finds everything from its position
+    const pg_cut_rec* ccut;
+    const int32_t* cnodes;
+    const int64_t* ckoff;
+    const int64_t* cpoff;
     int n_mat;
 };
 
@@ -98,6 +106,7 @@ __global__ void k_permute_u(int64_t n, const int32_t* __restrict__ node_dof, con
 }
 
 constexpr int MAXN2 = 3 + 8;             // 2D local nodes (tri3 + 2x4 enrichment slots)
+constexpr int CUT_NODES = 4 + PG_MAX_ENR;  // words per local node list of a cut element
 constexpr int MAXN3 = 4 + PG_MAX_ENR;    // 3D local nodes
 constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
@@ -396,7 +405,7 @@ __global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const doub
 //            TK, adds the cohesive fold and stores; P (FP64) likewise.
 // --------------------------------------------------------------------------
 template <int D, typename TK, int EPW, int MAXN, int MAXQ, int MAXC, int OCC>
-__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(OCC))) k_cut(DevProb p, const int32_t* __restrict__ list, int n_list,
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(OCC))) k_cut(DevProb p, int cbase, int n_list,
                                             const double* __restrict__ Un,
                                             const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                             TK* __restrict__ Kst, double* __restrict__ Pst, int* __restrict__ err) {
@@ -419,28 +428,29 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const int li_ = xcd_order(blockIdx.x, gridDim.x) * EPW + g;
     const bool live = li_ < n_list;
-    const int ci = live ? list[li_] : 0;
+    const int64_t pos = int64_t(cbase) + (live ? li_ : 0);  // class position
     if (threadIdx.x < MAX_LDS_MATS) {
         const int m = threadIdx.x;
         sLM[m][0] = m < p.n_mat ? TK(p.mats[m].D[1]) : TK(0);
         sLM[m][1] = m < p.n_mat ? TK(p.mats[m].D[NS * NS - 1]) : TK(0);
     }
+    // the record and the local node list are both indexed by the position:
+    // their loads go out together, and the node data follows the node list
     constexpr int RW = int(sizeof(pg_cut_rec) / 4);
     for (int k = gl; k < RW; k += GS)
-        if (live) reinterpret_cast<int32_t*>(&recs[g])[k] = reinterpret_cast<const int32_t*>(&p.cut[ci])[k];
-    __syncthreads();
-    const pg_cut_rec& rec = recs[g];
-    const int e = live ? p.cut_elem[ci] : 0;
-    const int nloc = live ? NEN + rec.n_enr : 0;
-    const int nd = nloc * D;
-    for (int l = gl; l < nloc; l += GS) {
-        const int nodeid = l < NEN ? p.conn[int64_t(e) * NEN + l] : rec.enr[l - NEN];
+        if (live) reinterpret_cast<int32_t*>(&recs[g])[k] = reinterpret_cast<const int32_t*>(&p.ccut[pos])[k];
+    for (int l = gl; l < MAXN; l += GS) {
+        const int nodeid = live ? p.cnodes[pos * CUT_NODES + l] : -1;
+        if (nodeid < 0) continue;
         for (int i = 0; i < D; ++i) {
             xs[g][l][i] = p.coords[int64_t(nodeid) * D + i];
             us[g][l * D + i] = Un[int64_t(nodeid) * D + i];
         }
     }
     __syncthreads();
+    const pg_cut_rec& rec = recs[g];
+    const int nloc = live ? NEN + rec.n_enr : 0;
+    const int nd = nloc * D;
 
     const int nvq = live ? rec.n_vqp : 0;
     const int cnn = rec.coh_nn / 2;
@@ -539,7 +549,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     // C(li, lj) (the fold scales R^T D R by N_li N_lj): one lane per unordered
     // pair li <= lj writes both blocks.  Staged block-major: the d x d block of
     // (li, lj) is contiguous at (li * nloc + lj) * d * d.
-    TK* Ke = Kst + p.cut_koff[ci];
+    TK* Ke = Kst + p.ckoff[pos];
     const int nup = nloc * (nloc + 1) / 2;
     for (int pb = gl; pb < nup; pb += GS) {
         int li = 0, rem = pb;
@@ -590,7 +600,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             for (int i = 0; i < D; ++i)
                 for (int j = 0; j < D; ++j) B2[i * D + j] = vol[j][i] + coh[i][j];
     }
-    double* Pe = Pst + p.cut_poff[ci];
+    double* Pe = Pst + p.cpoff[pos];
     for (int i = gl; i < nd; i += GS) {
         const int l = i / D, c = i % D;
         double a = 0.0;
@@ -1579,8 +1589,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         // cut elements, binned by class: small single-inclusion records share a
         // wavefront (EPW per wave); the general class keeps the full limits
         const int n0 = h->n_class[0], n1 = h->n_class[1];
-        const int32_t* l0 = h->class_list;
-        const int32_t* l1 = h->class_list + n0;
+        const int l0 = 0, l1 = n0;  // class positions of the two classes
         const int cv = h->cut_variant;
 #define CUT_SMALL2(TK, KST, EPW, OCC) \
     k_cut<2, TK, EPW, 7, 8, 2, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
@@ -1877,6 +1886,40 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             return fail(PG_E_HIP);
         }
         h->class_list = tmp_;
+        // the records, node lists and offsets in class order
+        std::vector<pg_cut_rec> cr(small.size());
+        std::vector<int32_t> cn(small.size() * CUT_NODES, -1);
+        std::vector<int64_t> ck(small.size()), cp(small.size());
+        for (size_t k = 0; k < small.size(); ++k) {
+            const int c = small[k];
+            cr[k] = s->cut[c];
+            const int e = s->cut_elem[c];
+            for (int n = 0; n < s->nen; ++n) cn[k * CUT_NODES + n] = s->conn[int64_t(e) * s->nen + n];
+            for (int n = 0; n < s->cut[c].n_enr && s->nen + n < CUT_NODES; ++n)
+                cn[k * CUT_NODES + s->nen + n] = s->cut[c].enr[n];
+            ck[k] = koff[c];
+            cp[k] = poff[c];
+        }
+        auto upload = [&](auto** dst, const auto& v) -> int {
+            using T = typename std::remove_reference<decltype(v)>::type::value_type;
+            T* q;
+            if (int r = alloc(h, &q, v.size())) return r;
+            if (!v.empty() && hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the class-ordered cut data failed";
+                return PG_E_HIP;
+            }
+            *dst = q;
+            return PG_OK;
+        };
+        pg_cut_rec* q0 = nullptr;
+        int32_t* q1 = nullptr;
+        int64_t *q2 = nullptr, *q3 = nullptr;
+        if ((rc = upload(&q0, cr)) || (rc = upload(&q1, cn)) || (rc = upload(&q2, ck)) || (rc = upload(&q3, cp)))
+            return fail(rc);
+        d.ccut = q0;
+        d.cnodes = q1;
+        d.ckoff = q2;
+        d.cpoff = q3;
     }
     const int ndp = s->nen * s->dim;
     if ((rc = alloc(h, &h->Un, size_t(nall * s->dim)))) return fail(rc);
