@@ -50,6 +50,7 @@ extern "C" {
 #define PG_CUT64   16u   /* with PG_FP32: cut elements (children, enrichment,
                             cohesive tangents) stay FP64 end to end; FP32 is
                             used only for uncut elements                     */
+/* Any other bit in `flags` is rejected with PG_E_ARG. */
 
 /* ----------------------------------------------------------------------------
  * Materials.  Reference: material_lei.cpp:52-97 (LEI D matrix),
@@ -227,7 +228,10 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                set = the list form of the kernel in 2D (always used in 3D);
  *                bit 2 set = no overlap (all node groups after the cut kernels
  *                on one stream; by default the groups of nodes without cut
- *                incidences run on a second stream during the cut kernels).
+ *                incidences run on a second stream during the cut kernels);
+ *                bit 3 set = every node group takes the global-memory path
+ *                (the fallback of groups that overflow the LDS tables; slow,
+ *                for testing).
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

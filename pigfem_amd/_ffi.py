@@ -18,6 +18,8 @@ PG_FP32 = 4
 PG_BATCHED = 8
 PG_CUT64 = 16
 
+PG_OK, PG_E_ARG, PG_E_HIP, PG_E_UNSUPPORTED, PG_E_MESH, PG_E_NOMEM = 0, -1, -2, -3, -4, -5
+
 PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
 PGH_PLACE_BEST_EFFORT = 1

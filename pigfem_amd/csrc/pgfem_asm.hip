@@ -107,6 +107,9 @@ __global__ void k_permute_u(int64_t n, const int32_t* __restrict__ node_dof, con
 
 constexpr int MAXN2 = 3 + 8;             // 2D local nodes (tri3 + 2x4 enrichment slots)
 constexpr int CUT_NODES = 4 + PG_MAX_ENR;  // words per local node list of a cut element
+// internal launch flag: every node group takes the global-memory path (the
+// fallback of groups that overflow the LDS tables; group variant bit 3)
+constexpr unsigned PG_GLOBAL_PATH = 1u << 31;
 constexpr int MAXN3 = 4 + PG_MAX_ENR;    // 3D local nodes
 constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
@@ -1000,7 +1003,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     }
     __syncthreads();
     const int ni = s_iptr[ng], nq = s_qptr[ng];
-    const bool staged = ni <= MAXI && nq <= MAXP;
+    const bool staged = ni <= MAXI && nq <= MAXP && !(flags & PG_GLOBAL_PATH);
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
@@ -1196,7 +1199,7 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     }
     __syncthreads();
     const int ni = s_iptr[ng], nq = s_qptr[ng];
-    const bool staged = ni <= MAXI && nq <= MAXP;
+    const bool staged = ni <= MAXI && nq <= MAXP && !(flags & PG_GLOBAL_PATH);
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += (D == 2 ? TRI_QW[k] : TET_QW[k]);
 
@@ -1540,7 +1543,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
         // LEN node groups on stream S: node-table entries from BASE, incidence slots from ibase
 #define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
-    launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, ibase, LEN, S, d, h->Un, KC, h->Pc, dK, dP, flags)
+    launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, ibase, LEN, S, d, h->Un, KC, h->Pc, dK, dP, \
+                                        flags | ((h->group_variant & 8) ? PG_GLOBAL_PATH : 0u))
 #define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                                 \
     do {                                                                         \
         if (h->group_variant & 1) GROUP1(DD, TC, TKC, KC, NGV, 1, S, BASE, LEN); \
@@ -1988,7 +1992,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 7) { g_err = "group variant must be 0..7"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 15) { g_err = "group variant must be 0..15"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;
@@ -2005,6 +2009,7 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
                            unsigned flags, double* dK, double* dP, void* stream) {
     (void)delta_t;
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (flags & ~(PG_FUNC | PG_JAC | PG_FP32 | PG_BATCHED | PG_CUT64)) { g_err = "unknown flag bits"; return PG_E_ARG; }
     if (!(flags & (PG_FUNC | PG_JAC))) return PG_OK;
     if (!dU || ((flags & PG_JAC) && !dK) || ((flags & PG_FUNC) && !dP)) { g_err = "missing device buffer"; return PG_E_ARG; }
     if (h->d.n_cqp > 0 && (!dIc || !dIt)) { g_err = "cohesive problem needs ISV buffers"; return PG_E_ARG; }
@@ -2034,6 +2039,7 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
 int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, double delta_t, unsigned flags,
                     double* Kff, double* Kfp, double* Kpf, double* Kpp, double* Pf, double* Pp) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (flags & ~(PG_FUNC | PG_JAC | PG_FP32 | PG_BATCHED | PG_CUT64)) { g_err = "unknown flag bits"; return PG_E_ARG; }
     if (!(flags & (PG_FUNC | PG_JAC))) return PG_OK;
     double* Kb[4] = {Kff, Kfp, Kpf, Kpp};
     bool missing = !U;

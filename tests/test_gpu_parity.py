@@ -8,7 +8,8 @@ relative (Frobenius over the four K blocks, 2-norm of P); mixed schedule within
 import numpy as np
 import pytest
 
-from pigfem_amd import AssemblerGPU, PMatrix, PVector, RVEProblem, config, small_config
+from pigfem_amd import AssemblerGPU, DeviceBuffer, PMatrix, PVector, RVEProblem, _ffi, config, small_config
+from pigfem_amd._ffi import lib
 from oracle.oracle import Oracle
 from tests.helpers import full_matrix, rel, u_scale
 
@@ -121,6 +122,16 @@ def test_tangent_keeps_rigid_translations(case, mode):
             assert np.abs(r[enr]).max() <= tol * s_enr, mode
 
 
+def test_unknown_flag_bits_are_rejected():
+    p = RVEProblem(small_config("c0", n=8, n_incl=1))
+    asm = AssemblerGPU(p, "mixed")
+    dU = DeviceBuffer.from_numpy(p.random_u(0))
+    dK, dP = DeviceBuffer.zeros(p.nnz_total), DeviceBuffer.zeros(p.n_dof)
+    rc = lib().pg_asm_assemble_device(asm._h, dU.ptr, None, None, 0.0,
+                                      _ffi.PG_FUNC | _ffi.PG_JAC | (1 << 31), dK.ptr, dP.ptr, None)
+    assert rc == _ffi.PG_E_ARG
+
+
 def test_device_api_matches_host_api():
     from pigfem_amd import DeviceBuffer, synchronize
     p = RVEProblem(small_config("op", n=24, n_incl=3))
@@ -141,9 +152,9 @@ def test_device_api_matches_host_api():
 
 @pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1)])
 def test_every_schedule_variant(kind, n, ni):
-    """All node-group sizes, register caps and cut-kernel variants the autotuner
-    may pick give the same result, bit for bit (same arithmetic, same summation
-    order)."""
+    """All node-group sizes, group-kernel forms (and the global-memory fallback
+    that groups overflowing the LDS tables take) and cut-kernel variants give
+    the same result, bit for bit (same arithmetic, same summation order)."""
     p = RVEProblem(small_config(kind, n=n, n_incl=ni))
     o = Oracle(p)
     U = p.random_u(9)
@@ -152,7 +163,8 @@ def test_every_schedule_variant(kind, n, ni):
     groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
     for mode in ("binned64", "mixed", "mixed_cut64"):
         ref = None
-        variants = range(8) if p.dim == 2 else (0, 1, 4, 5)
+        # bit 3: every group on the global-memory fallback path
+        variants = (*range(8), 8, 11) if p.dim == 2 else (0, 1, 4, 5, 8, 13)
         for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
             asm = AssemblerGPU(p, mode)
             asm.set_tiles(g, c, v)
