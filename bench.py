@@ -208,7 +208,7 @@ def main():
         synchronize()
         log("[bench] calibration copy of 1 GiB done (3x)")
 
-    cpu = None
+    cpu = cpu_threads = None
     if not args.skip_cpu:
         from oracle.oracle import Oracle
         # size the element range for ~cpu_seconds of work from a short probe
@@ -225,11 +225,23 @@ def main():
             t = time.perf_counter()
             o.assemble(U, np.zeros(prob.n_cqp))
             best = min(best, time.perf_counter() - t)
-        o.close()
         cpu = {"value": m / best, "unit": "elements/s", "cores": 1, "kind": "port",
                "sample": f"first {m} elements (element-loop order) of the {cfg.name} RVE, K+R, FP64, "
                          f"best of 2 single-thread runs of the oracle restatement"}
         log(f"[bench] cpu baseline: {cpu['value']:.4g} elements/s over {m} elements")
+        # the same sample on every host thread the run may use, element colour
+        # by colour (elements of one colour share no node)
+        nt = max(1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
+        o.assemble(U, np.zeros(prob.n_cqp), threads=nt)  # colouring + warm-up
+        best_t = 1e30
+        for _ in range(3):
+            t = time.perf_counter()
+            o.assemble(U, np.zeros(prob.n_cqp), threads=nt)
+            best_t = min(best_t, time.perf_counter() - t)
+        o.close()
+        cpu_threads = {"value": m / best_t, "unit": "elements/s", "cores": nt, "kind": "port",
+                       "sample": f"the same {m} elements, colour by colour on {nt} OpenMP threads, best of 3"}
+        log(f"[bench] cpu baseline, {nt} threads: {cpu_threads['value']:.4g} elements/s")
 
     # HBM traffic of the same kernel instantiation from the committed rocprofv3
     # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
@@ -264,6 +276,7 @@ def main():
                      "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms,
                      "kernel_ms_overlapped_groups": clean_ms, "kernel_ms_remaining_groups": dirty_ms},
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_threads,
         "fp64_elements_per_s": results["fp64"]["eps"],
         "fp64_ms_per_step": results["fp64"]["ms"],
         "speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],

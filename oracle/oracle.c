@@ -64,6 +64,12 @@ typedef struct oracle {
     /* cached shape data per element (store_shape_functions + storeBmats) */
     int64_t* off;     /* per element offset into cache */
     double* cache;
+    /* element colouring for the threaded baseline (built on first use):
+     * elements of one colour share no node, so they write disjoint K rows,
+     * P entries and ISVs */
+    int ncolour;
+    int32_t* colour_ptr;  /* [ncolour + 1] */
+    int32_t* colour_elem; /* elements grouped by colour, ascending within one */
     char err[256];
 } oracle;
 
@@ -392,6 +398,8 @@ void oracle_free(oracle* o) {
     for (int b = 0; b < 4; ++b) { free(o->rp[b]); free(o->ci[b]); }
     free(o->off);
     free(o->cache);
+    free(o->colour_ptr);
+    free(o->colour_elem);
     free(o);
 }
 
@@ -551,6 +559,8 @@ static int add_value(oracle* o, double* K, int32_t r, int32_t c, double v) {
     return 0;
 }
 
+static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int func, int jac, double* K, double* P);
+
 /* Assembler::assemble over elements [e0, e1).  K: concatenated [ff|fp|pf|pp]
  * values in this oracle's pattern; P: [Pf|Pp]; isv_t reset from isv_c first. */
 int oracle_assemble(oracle* o, const double* U, const double* isv_c, double* isv_t, int func, int jac, double* K,
@@ -558,13 +568,22 @@ int oracle_assemble(oracle* o, const double* U, const double* isv_c, double* isv
     const pg_problem_desc* d = o->d;
     if (!func && !jac) return 0;
     if (o->err[0]) return -1;
-    const int dim = d->dim, nstr = o->nstr, nen = d->nen;
     if (jac) memset(K, 0, sizeof(double) * (size_t)(o->nnz[0] + o->nnz[1] + o->nnz[2] + o->nnz[3]));
     if (func) memset(P, 0, sizeof(double) * (size_t)(d->n_free + d->n_pres));
     if (d->n_cqp > 0) memcpy(isv_t, isv_c, sizeof(double) * (size_t)d->n_cqp); /* InternalVars::copy_data */
+    for (int e = o->e0; e < o->e1; ++e)
+        if (assemble_elem(o, e, U, isv_t, func, jac, K, P)) return -1;
+    return 0;
+}
+
+/* assemble_elem: one element of the loop of Assembler::assemble
+ * (src/Assembler.cpp:100-140): assemble_elem + set_matrix / set_vec. */
+static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int func, int jac, double* K, double* P) {
+    const pg_problem_desc* d = o->d;
+    const int dim = d->dim, nstr = o->nstr, nen = d->nen;
     int32_t dofs[MAXND];
     double Ue[MAXND], Ke[MAXND * MAXND], Pe[MAXND], Kt[MAXND * MAXND], Pt[MAXND], BtD[MAXND * 6];
-    for (int e = o->e0; e < o->e1; ++e) {
+    {
         const double* cc = o->cache + o->off[e - o->e0];
         int nd = elem_dofs(d, e, dofs);
         for (int i = 0; i < nd; ++i) Ue[i] = U[dofs[i]]; /* formElementalVector */
@@ -714,6 +733,69 @@ int oracle_assemble(oracle* o, const double* U, const double* isv_c, double* isv
     }
     (void)nen;
     return 0;
+}
+
+/* Greedy element colouring through the node -> element incidence: an element
+ * takes the smallest colour none of its already coloured neighbours (elements
+ * sharing a node, enrichment nodes included) holds. */
+static int build_colours(oracle* o) {
+    const pg_problem_desc* d = o->d;
+    const int ne = o->e1 - o->e0;
+    int32_t* col = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ne ? ne : 1));
+    if (!col) return -1;
+    int nc = 0;
+    unsigned char used[256];
+    int32_t ln[4 + PG_MAX_ENR];
+    for (int e = o->e0; e < o->e1; ++e) {
+        memset(used, 0, sizeof used);
+        int nl = local_list(d, e, ln);
+        for (int n = 0; n < nl; ++n)
+            for (int32_t t = d->n2e_ptr[ln[n]]; t < d->n2e_ptr[ln[n] + 1]; ++t) {
+                int f = d->n2e[t];
+                if (f >= o->e0 && f < e) used[col[f - o->e0]] = 1;
+            }
+        int c = 0;
+        while (c < 256 && used[c]) ++c;
+        if (c == 256) { free(col); strcpy(o->err, "element colouring needs more than 256 colours"); return -1; }
+        col[e - o->e0] = c;
+        if (c + 1 > nc) nc = c + 1;
+    }
+    o->colour_ptr = (int32_t*)calloc((size_t)nc + 1, sizeof(int32_t));
+    o->colour_elem = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ne ? ne : 1));
+    if (!o->colour_ptr || !o->colour_elem) { free(col); return -1; }
+    for (int k = 0; k < ne; ++k) o->colour_ptr[col[k] + 1]++;
+    for (int c = 0; c < nc; ++c) o->colour_ptr[c + 1] += o->colour_ptr[c];
+    int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nc ? nc : 1));
+    if (!fill) { free(col); return -1; }
+    memcpy(fill, o->colour_ptr, sizeof(int32_t) * (size_t)nc);
+    for (int k = 0; k < ne; ++k) o->colour_elem[fill[col[k]]++] = o->e0 + k;
+    free(fill);
+    free(col);
+    o->ncolour = nc;
+    return 0;
+}
+
+/* The same assembly on `nthreads` OpenMP threads, colour by colour (the
+ * threaded CPU baseline; within a colour no two elements write the same K
+ * row, P entry or ISV).  The sums then run in colour order, not element
+ * order: the results agree with oracle_assemble to rounding, not bit for bit. */
+int oracle_assemble_threads(oracle* o, const double* U, const double* isv_c, double* isv_t, int func, int jac,
+                            double* K, double* P, int nthreads) {
+    const pg_problem_desc* d = o->d;
+    if (!func && !jac) return 0;
+    if (o->err[0]) return -1;
+    if (!o->colour_ptr && build_colours(o)) return -1;
+    if (jac) memset(K, 0, sizeof(double) * (size_t)(o->nnz[0] + o->nnz[1] + o->nnz[2] + o->nnz[3]));
+    if (func) memset(P, 0, sizeof(double) * (size_t)(d->n_free + d->n_pres));
+    if (d->n_cqp > 0) memcpy(isv_t, isv_c, sizeof(double) * (size_t)d->n_cqp);
+    int bad = 0;
+    for (int c = 0; c < o->ncolour; ++c) {
+        const int k0 = o->colour_ptr[c], k1 = o->colour_ptr[c + 1];
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(| : bad)
+        for (int k = k0; k < k1; ++k)
+            bad |= assemble_elem(o, o->colour_elem[k], U, isv_t, func, jac, K, P) != 0;
+    }
+    return bad ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -40,6 +40,9 @@ def lib():
         L.oracle_assemble.restype = C.c_int
         L.oracle_assemble.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                       C.c_void_p, C.c_void_p]
+        L.oracle_assemble_threads.restype = C.c_int
+        L.oracle_assemble_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                              C.c_void_p, C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -65,15 +68,19 @@ class Oracle:
             self.rp.append(np.ctypeslib.as_array(rp, shape=(rows[b] + 1,)).copy())
             self.ci.append(np.ctypeslib.as_array(ci, shape=(max(self.nnz[b], 1),))[: self.nnz[b]].copy())
 
-    def assemble(self, U, isv_committed=None, func=True, jac=True):
+    def assemble(self, U, isv_committed=None, func=True, jac=True, threads=0):
+        """threads = 0: the element loop in element order (the checker);
+        threads > 0: colour by colour on that many OpenMP threads (the threaded
+        CPU baseline; equal to the serial result up to rounding)."""
         p = self.problem
         U = np.ascontiguousarray(U, dtype=np.float64)
         K = np.zeros(sum(self.nnz), dtype=np.float64)
         P = np.zeros(p.n_dof, dtype=np.float64)
         ic = np.ascontiguousarray(isv_committed if isv_committed is not None else np.zeros(p.n_cqp), dtype=np.float64)
         it = np.zeros(max(p.n_cqp, 1), dtype=np.float64)
-        rc = lib().oracle_assemble(self._h, U.ctypes.data, ic.ctypes.data if p.n_cqp else None,
-                                   it.ctypes.data, int(func), int(jac), K.ctypes.data, P.ctypes.data)
+        args = (self._h, U.ctypes.data, ic.ctypes.data if p.n_cqp else None, it.ctypes.data, int(func), int(jac),
+                K.ctypes.data, P.ctypes.data)
+        rc = lib().oracle_assemble_threads(*args, int(threads)) if threads > 0 else lib().oracle_assemble(*args)
         if rc != 0:
             raise RuntimeError("oracle_assemble: " + lib().oracle_error(self._h).decode())
         return K, P, it[: p.n_cqp]

@@ -267,3 +267,19 @@ def test_isv_trial_rules():
     ob = Oracle(pb)
     _, _, itb = ob.assemble(pb.random_u(2, 1e-5), np.full(pb.n_cqp, 0.25))
     assert np.all(itb == 0.25)                 # BL never writes delta_max (Q5)
+
+
+@pytest.mark.parametrize("kind", ["c0", "op", "tet"])
+def test_threaded_oracle_matches_the_element_loop(kind):
+    """The coloured multi-threaded assembly (the threaded CPU baseline) gives
+    the serial element loop's K, P and trial ISVs up to the rounding of a
+    different summation order."""
+    cfg = small_config(kind, n=16 if kind != "tet" else 6, n_incl=3 if kind != "tet" else 1)
+    p = RVEProblem(cfg)
+    o = Oracle(p)
+    U = p.random_u(4, u_scale(cfg))
+    ic = np.random.default_rng(2).uniform(0, 5e-4, p.n_cqp) if p.n_cqp else None
+    K1, P1, I1 = o.assemble(U, ic)
+    K4, P4, I4 = o.assemble(U, ic, threads=4)
+    assert rel(K4, K1) < 1e-13 and rel(P4, P1) < 1e-13
+    assert np.array_equal(I4, I1)
