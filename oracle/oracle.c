@@ -306,7 +306,7 @@ static int store_shapes(oracle* o, int e, double* out) {
     int cnn = r->coh_nn / 2, cnq = r->coh_nn == 4 ? 2 : 4;
     for (int k = 0; k < r->n_coh; ++k)
         for (int q = 0; q < cnq; ++q) {
-            double cN[3], dNr[3][2], R[3][3], dA;
+            double cN[3] = {0, 0, 0}, dNr[3][2], R[3][3], dA;
             double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
             if (r->coh_nn == 4) {
                 double s = q == 0 ? -G2 : G2;
@@ -525,7 +525,7 @@ static void cohesive(const pg_material* m, int dim, double* delta, double* dmax_
             dtr = tm / dmax;
         }
     }
-    double br[3], dd[3];
+    double br[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
     for (int i = 0; i < dim; ++i) br[i] = b2 * delta[i];
     br[dim - 1] += (1.0 - b2) * delta[dim - 1];
     for (int i = 0; i < dim; ++i) t[i] = (tr / de) * br[i];
