@@ -231,7 +231,8 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                incidences run on a second stream during the cut kernels);
  *                bit 3 set = every node group takes the global-memory path
  *                (the fallback of groups that overflow the LDS tables; slow,
- *                for testing).
+ *                for testing); bit 4 set = the groups of enrichment nodes
+ *                take twice the threads per node pair whatever bit 0 says.
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

@@ -1371,13 +1371,15 @@ struct pg_asm {
     int64_t gioff[3] = {0, 0, 0};   // first incg / slotg slot of each group list
     int ngroups[3] = {0, 0, 0};     // groups of each list
     int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
+    int nenrg[3] = {0, 0, 0};       // groups of enrichment nodes (listed last)
     bool timing = false;
     double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
     int elem_block = 256;
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
-    int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap
+    int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap;
+                           // bit 3: global-memory path; bit 4: two threads per pair for enrichment-node groups
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
 };
@@ -1541,14 +1543,15 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         while (o < 2 && h->gopt_ng[o] != ng) ++o;
         const int nclean = h->nclean[o], nrest = h->ngroups[o] - h->nclean[o];
         const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
+        const bool tv0 = (h->group_variant & 1) != 0;
         // LEN node groups on stream S: node-table entries from BASE, incidence slots from ibase
 #define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
     launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, ibase, LEN, S, d, h->Un, KC, h->Pc, dK, dP, \
                                         flags | ((h->group_variant & 8) ? PG_GLOBAL_PATH : 0u))
-#define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                                 \
-    do {                                                                         \
-        if (h->group_variant & 1) GROUP1(DD, TC, TKC, KC, NGV, 1, S, BASE, LEN); \
-        else GROUP1(DD, TC, TKC, KC, NGV, 0, S, BASE, LEN);                      \
+#define GROUP(DD, TC, TKC, KC, NGV, S, BASE, LEN)                     \
+    do {                                                             \
+        if (tv) GROUP1(DD, TC, TKC, KC, NGV, 1, S, BASE, LEN);       \
+        else GROUP1(DD, TC, TKC, KC, NGV, 0, S, BASE, LEN);          \
     } while (0)
 #define GROUPS2(TC, TKC, KC, S, BASE, LEN)                              \
     do {                                                                \
@@ -1562,8 +1565,9 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         else if (ng == 4) GROUP(3, TC, TKC, KC, 4, S, BASE, LEN);       \
         else GROUP(3, TC, TKC, KC, 8, S, BASE, LEN);                    \
     } while (0)
-        // node groups first .. first + len - 1 of the list of this group size
-        auto groups = [&](hipStream_t s, int first, int len) {
+        // node groups first .. first + len - 1 of the list of this group size,
+        // tv: two threads per node pair
+        auto groups = [&](hipStream_t s, int first, int len, bool tv) {
             const int base = h->goff[o] + first * h->gopt_ng[o];
             const int64_t ibase = h->gioff[o] + int64_t(first) * (d.dim == 2 ? 7 : 24) * h->gopt_ng[o];
             if (d.dim == 2) {
@@ -1586,7 +1590,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(h->side, 0, nclean);
+            groups(h->side, 0, nclean, tv0);
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -1629,8 +1633,13 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
-        if (overlap) groups(st, nclean, nrest);
-        else groups(st, 0, h->ngroups[o]);
+        // the groups of enrichment nodes (the tail of the list) may take twice
+        // the threads per pair (group variant bit 4): they have few incidences
+        // and many pairs
+        const int first = overlap ? nclean : 0, nenr = h->nenrg[o];
+        const int nreg = h->ngroups[o] - first - nenr;
+        groups(st, first, nreg, tv0);
+        groups(st, first + nreg, nenr, tv0 || (h->group_variant & 16) != 0);
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1816,6 +1825,8 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             h->goff[o] = int(gl.size() / 4);
             h->ngroups[o] = int(clean.size() + rest.size());
             h->nclean[o] = int(clean.size());
+            h->nenrg[o] = 0;
+            for (const int64_t a0 : rest) h->nenrg[o] += a0 >= s->n_nodes ? 1 : 0;
             for (const auto* lst : {&clean, &rest})
                 for (const int64_t a0 : *lst)
                     for (int64_t a = a0; a < a0 + NGo; ++a) {
@@ -1992,7 +2003,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 15) { g_err = "group variant must be 0..15"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 31) { g_err = "group variant must be 0..31"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;

@@ -163,8 +163,8 @@ def test_every_schedule_variant(kind, n, ni):
     groups = (8, 16, 32) if p.dim == 2 else (2, 4, 8)
     for mode in ("binned64", "mixed", "mixed_cut64"):
         ref = None
-        # bit 3: every group on the global-memory fallback path
-        variants = (*range(8), 8, 11) if p.dim == 2 else (0, 1, 4, 5, 8, 13)
+        # bit 3: every group on the global-memory fallback path; bit 4: enrichment groups with TV = 1
+        variants = (*range(8), 8, 11, 16, 18) if p.dim == 2 else (0, 1, 4, 5, 8, 13, 16, 20)
         for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
             asm = AssemblerGPU(p, mode)
             asm.set_tiles(g, c, v)
