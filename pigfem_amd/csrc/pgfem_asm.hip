@@ -418,8 +418,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double us[EPW][MAXN * D];
     __shared__ double qg[EPW][MAXQ][MAXN][D];  // gradient of every local node at every qp
     __shared__ double qw[EPW][MAXQ];
-    __shared__ int qm[EPW][MAXQ];
-    __shared__ int qk[EPW][MAXQ];           // child of each qp
+    __shared__ TK qwl[EPW][MAXQ][2];        // w lambda, w mu of each qp (its child's material)
     __shared__ double qs[EPW][MAXQ][NS];
     __shared__ double cN[EPW][CUT_MAXC][3];
     __shared__ double cw[EPW][CUT_MAXC];
@@ -505,8 +504,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 }
             }
             qw[g][t] = Jc * w;
-            qm[g][t] = rec.child_mat[k];
-            qk[g][t] = k;
+            {
+                const TK wt = TK(Jc * w);
+                qwl[g][t][0] = wt * sLM[rec.child_mat[k]][0];
+                qwl[g][t][1] = wt * sLM[rec.child_mat[k]][1];
+            }
             double sig[NS];
             lei_stress<D>(p.mats[rec.child_mat[k]].D, eps, sig);
             for (int s = 0; s < NS; ++s) qs[g][t][s] = sig[s];
@@ -570,9 +572,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             if (zi || zj) continue;
             TK gi[D], gj[D];
             for (int a = 0; a < D; ++a) { gi[a] = TK(gid[a]); gj[a] = TK(gjd[a]); }
-            const TK w = TK(qw[g][q]);
+
             TK blk[D][D];
-            iso_block<D, TK>(gi, gj, w * sLM[qm[g][q]][0], w * sLM[qm[g][q]][1], blk);
+            iso_block<D, TK>(gi, gj, qwl[g][q][0], qwl[g][q][1], blk);
             for (int i = 0; i < D; ++i)
                 for (int j = 0; j < D; ++j) vol[i][j] += blk[i][j];
         }
