@@ -419,6 +419,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double qg[EPW][MAXQ][MAXN][D];  // gradient of every local node at every qp
     __shared__ double qw[EPW][MAXQ];
     __shared__ TK qwl[EPW][MAXQ][2];        // w lambda, w mu of each qp (its child's material)
+    __shared__ uint32_t qmask[EPW][MAXN];   // qps at which a local node's gradient is not zero
+    static_assert(MAXQ <= 32, "one mask bit per qp");
     __shared__ double qs[EPW][MAXQ][NS];
     __shared__ double cN[EPW][CUT_MAXC][3];
     __shared__ double cw[EPW][CUT_MAXC];
@@ -442,6 +444,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     for (int k = gl; k < RW; k += GS)
         if (live) reinterpret_cast<int32_t*>(&recs[g])[k] = reinterpret_cast<const int32_t*>(&p.ccut[pos])[k];
     for (int l = gl; l < MAXN; l += GS) {
+        qmask[g][l] = 0u;
         const int nodeid = live ? p.cnodes[pos * CUT_NODES + l] : -1;
         if (nodeid < 0) continue;
         for (int i = 0; i < D; ++i) {
@@ -497,6 +500,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                             for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
                 }
                 for (int a = 0; a < D; ++a) qg[g][t][l][a] = gr[a];
+                bool nz = false;
+                for (int a = 0; a < D; ++a) nz = nz || gr[a] != 0.0;
+                if (nz) atomicOr(&qmask[g][l], 1u << t);
                 for (int c = 0; c < D; ++c) {
                     double b[NS];
                     bcol<D, double>(gr, c, b);
@@ -563,13 +569,13 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         TK vol[D][D];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) vol[i][j] = TK(0);
-        for (int q = 0; q < nvq; ++q) {
+        // the qps at which both gradients are not zero, ascending (the others
+        // contribute nothing)
+        for (uint32_t qm = qmask[g][li] & qmask[g][lj]; qm; qm &= qm - 1u) {
+            const int q = __builtin_ctz(qm);
             double gid[D], gjd[D];
             grad(q, li, gid);
             grad(q, lj, gjd);
-            bool zi = true, zj = true;
-            for (int a = 0; a < D; ++a) { zi = zi && gid[a] == 0.0; zj = zj && gjd[a] == 0.0; }
-            if (zi || zj) continue;
             TK gi[D], gj[D];
             for (int a = 0; a < D; ++a) { gi[a] = TK(gid[a]); gj[a] = TK(gjd[a]); }
 
@@ -609,7 +615,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     for (int i = gl; i < nd; i += GS) {
         const int l = i / D, c = i % D;
         double a = 0.0;
-        for (int q = 0; q < nvq; ++q) {
+        for (uint32_t qm = qmask[g][l]; qm; qm &= qm - 1u) {
+            const int q = __builtin_ctz(qm);
             double gr[D], v[D];
             int r[D];
             grad(q, l, gr);
