@@ -52,6 +52,10 @@ typedef struct pgh_rve_params {
 /* Stop placing inclusions when the rejection budget is spent and keep those
  * already placed (at least one), instead of failing. */
 #define PGH_PLACE_BEST_EFFORT 1
+/* Centre each inclusion at a random interior mesh node, radius < h / 2: only
+ * the elements around that node are cut, each with one node inside (in 3D the
+ * interface is then made of CohTri3 triangles only). */
+#define PGH_PLACE_AT_NODES 2
 
 typedef struct pgh_problem pgh_problem;
 

@@ -23,6 +23,7 @@ PG_OK, PG_E_ARG, PG_E_HIP, PG_E_UNSUPPORTED, PG_E_MESH, PG_E_NOMEM = 0, -1, -2, 
 PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
 PGH_PLACE_BEST_EFFORT = 1
+PGH_PLACE_AT_NODES = 2
 PGH_BATCH_COLS = 8
 PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4 = range(3)
 PG_MAX_ENR, PG_MAX_CHILD, PG_MAX_COH = 12, 8, 3

@@ -217,7 +217,20 @@ void place_inclusions(pgh_problem& P) {
         }
         Incl I{};
         I.r = rng.uni(p.r_min, p.r_max);
-        for (int d = 0; d < P.dim; ++d) I.c[d] = rng.uni(I.r, 1.0 - I.r);
+        if (p.flags & PGH_PLACE_AT_NODES) {
+            // an interior mesh node; r < h / 2 keeps every other node outside, so
+            // the only elements cut are those of the node's star, one node inside
+            const double h = 1.0 / p.n;
+            if (p.r_max >= 0.5 * h || p.n < 2) fail(PG_E_ARG, "node-centred inclusions need n >= 2 and r_max < h / 2");
+            for (int d = 0; d < P.dim; ++d) {
+                int k = 1 + int(rng.uni(0.0, 1.0) * (p.n - 1));
+                if (k > p.n - 1) k = p.n - 1;
+                I.c[d] = k * h;
+            }
+        } else {
+            for (int d = 0; d < P.dim; ++d) I.c[d] = rng.uni(I.r, 1.0 - I.r);
+        }
+        // the reference's sphere box tests z against the y centre: keep z <= y
         if (P.dim == 3 && I.c[2] > I.c[1]) continue;
         int ci[3] = {G.idx(I.c[0]), G.idx(I.c[1]), P.dim == 3 ? G.idx(I.c[2]) : 0};
         bool ok = true;
@@ -442,7 +455,10 @@ bool detect_tet4(const int nd[4], const int32_t gid[4], bool cohesive, Detect& D
         } else {
             fail(PG_E_MESH, "tet4 element has 2 nodes separated in an unknown manner");
         }
-        if (cohesive) fail(PG_E_UNSUPPORTED, "3D cohesive quadrilateral interfaces (CohQuad4) are not supported");
+        if (cohesive)
+            fail(PG_E_UNSUPPORTED, "3D cohesive quadrilateral interfaces (CohQuad4, a tet4 with two nodes on each "
+                                   "side) are not supported; node-centred inclusions (PGH_PLACE_AT_NODES) cut "
+                                   "tets one node against three only");
     }
     D.incl = vec[1];
     // decomposePrisms, src/tet4.cpp:673-735
@@ -891,8 +907,6 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
         if (p->n < 1) fail(PG_E_ARG, "n must be >= 1");
         if (p->n_incl < 0 || p->r_min <= 0 || p->r_max < p->r_min || 2 * p->r_max >= 1.0)
             fail(PG_E_ARG, "bad inclusion parameters");
-        if (p->dim == 3 && p->cohesive != PGH_COH_NONE)
-            fail(PG_E_UNSUPPORTED, "3D cohesive RVEs are not supported");
         P->cohesive = p->cohesive != PGH_COH_NONE;
         generate_mesh(*P);
         place_inclusions(*P);

@@ -42,6 +42,7 @@ class RVEConfig:
     beta: float = 1.0
     alpha: float = 50.0
     strain: float = 0.02
+    place: int = 0               # PGH_PLACE_* flags of the inclusion placement
     name: str = "rve"
 
     def params(self) -> PghRveParams:
@@ -59,6 +60,7 @@ class RVEConfig:
         p.delta_c2 = self.delta_c
         p.beta, p.alpha, p.strain = self.beta, self.alpha, self.strain
         p.max_tries = 0
+        p.flags = self.place
         return p
 
 
@@ -90,6 +92,12 @@ def small_config(kind: str = "c0", n: int = 24, n_incl: int = 3, seed: int = 11)
                          cohesive=_ffi.PGH_COH_OP, name=f"small-op-{n}")
     if kind == "tet":
         return RVEConfig(dim=3, n=n, n_incl=n_incl, r_min=0.12, r_max=0.2, seed=seed, name=f"small-tet-{n}")
+    if kind in ("tet-op", "tet-bl"):
+        # spheres centred at mesh nodes, r < h / 2: CohTri3 interfaces only
+        h = 1.0 / n
+        return RVEConfig(dim=3, n=n, n_incl=n_incl, r_min=0.25 * h, r_max=0.4 * h, gap_h=0.5, seed=seed,
+                         cohesive=_ffi.PGH_COH_OP if kind == "tet-op" else _ffi.PGH_COH_BL,
+                         place=_ffi.PGH_PLACE_AT_NODES, name=f"small-{kind}-{n}")
     raise KeyError(kind)
 
 
@@ -133,7 +141,7 @@ class RVEProblem:
             arr = (PghRveParams * len(_cfgs))()
             for i, c in enumerate(_cfgs):
                 arr[i] = c.params()
-                arr[i].flags = _ffi.PGH_PLACE_BEST_EFFORT
+                arr[i].flags = c.place | _ffi.PGH_PLACE_BEST_EFFORT
                 arr[i].max_tries = 2_000_000
             self._params = arr
             check(L.pgh_build_batch(arr, len(_cfgs), C.byref(h)), "pgh_build_batch")

@@ -15,7 +15,7 @@ from tests.helpers import full_matrix, rel, u_scale
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("c0", 24, 3), ("bl", 24, 3), ("op", 24, 3), ("tet", 8, 1)]
+CASES = [("c0", 24, 3), ("bl", 24, 3), ("op", 24, 3), ("tet", 8, 1), ("tet-op", 6, 3)]
 
 
 def _run(asm, p, U, ic, func=True, jac=True):
@@ -150,7 +150,7 @@ def test_device_api_matches_host_api():
     assert np.array_equal(dit.download(p.n_cqp), Ih)
 
 
-@pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1)])
+@pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1), ("tet-op", 6, 3)])
 def test_every_schedule_variant(kind, n, ni):
     """All node-group sizes, group-kernel forms (and the global-memory fallback
     that groups overflowing the LDS tables take) and cut-kernel variants give

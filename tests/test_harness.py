@@ -165,8 +165,14 @@ def test_incidence_is_ascending_and_complete():
 
 
 def test_rejects_unsupported():
-    with pytest.raises(Exception):
-        RVEProblem(RVEConfig(dim=3, n=4, n_incl=1, r_min=0.1, r_max=0.2, cohesive=_ffi.PGH_COH_OP))
+    # a large sphere cuts tets two nodes against two: a quadrilateral cohesive
+    # interface (CohQuad4), which the harness does not build
+    with pytest.raises(Exception, match="CohQuad4"):
+        RVEProblem(RVEConfig(dim=3, n=8, n_incl=1, r_min=0.3, r_max=0.3, cohesive=_ffi.PGH_COH_OP))
+    # node-centred spheres cut one node against three only: CohTri3 interfaces
+    p = RVEProblem(small_config("tet-op", n=6, n_incl=2))
+    assert p.n_cut > 0 and p.n_cqp == 4 * sum(p.cut_record(i).n_coh for i in range(p.n_cut))
+    assert all(p.cut_record(i).coh_nn == 6 for i in range(p.n_cut))
 
 
 @pytest.mark.parametrize("kind", ["op", "tet"])

@@ -230,7 +230,7 @@ def test_zero_opening_tangent():
     assert np.allclose(D, 50.0 * np.e / 1e-3 * np.eye(2), rtol=1e-12)
 
 
-@pytest.mark.parametrize("kind", ["op", "bl"])
+@pytest.mark.parametrize("kind", ["op", "bl", "tet-op", "tet-bl"])
 def test_assembled_tangent_central_difference(kind):
     """K of a cohesive RVE is the derivative of P (signs of the cohesive fold, the
     rotation, the opening and the law's tangent all enter)."""
@@ -283,3 +283,42 @@ def test_threaded_oracle_matches_the_element_loop(kind):
     K4, P4, I4 = o.assemble(U, ic, threads=4)
     assert rel(K4, K1) < 1e-13 and rel(P4, P1) < 1e-13
     assert np.array_equal(I4, I1)
+
+
+def test_cohtri3_geometry():
+    """CohTri3 (src/CohTri3.cpp:13-200, src/CohesiveElem.cpp:145-238): the
+    4-point triangle rule's weights times the area Jacobian |dx/dr x dx/ds| sum
+    to the interface triangle's area; the shape functions of the lower face
+    form a partition of unity; the last row of the 'rotation' is the unit
+    normal, and the other two are dx/dr and dx/ds normalised (the reference's
+    construction: they are not orthogonal to each other in general)."""
+    p = RVEProblem(small_config("tet-op", n=6, n_incl=2))
+    o = Oracle(p)
+    nst, checked = 6, 0
+    for ci in range(p.n_cut):
+        e = int(p.cut_elem[ci])
+        rec = p.cut[ci]
+        local = list(p.conn[e]) + [rec.enr[k] for k in range(rec.n_enr)]
+        nd = 3 * len(local)
+        cache = elem_cache(o, e)
+        off = rec.n_vqp * (2 + nst * nd)
+        assert rec.coh_nn == 6
+        for k in range(rec.n_coh):
+            x = np.array([p.coords[local[rec.coh_nodes[k][n]]] for n in range(3)])
+            t0, t1 = x[1] - x[0], x[2] - x[0]
+            nrm = np.cross(t0, t1)
+            area = 0.5 * np.linalg.norm(nrm)
+            s = 0.0
+            for q in range(4):
+                W, J = cache[off], cache[off + 1]
+                N = cache[off + 2: off + 5]
+                R = cache[off + 5: off + 14].reshape(3, 3)
+                s += W * J
+                assert abs(N.sum() - 1.0) < 1e-14 and (N >= 0).all()
+                assert np.allclose(R[2], nrm / np.linalg.norm(nrm), atol=1e-12)
+                assert np.allclose(R[0], t0 / np.linalg.norm(t0), atol=1e-12)
+                assert np.allclose(R[1], t1 / np.linalg.norm(t1), atol=1e-12)
+                off += 14
+            assert abs(s - area) <= 1e-12 * area
+            checked += 1
+    assert checked > 0
