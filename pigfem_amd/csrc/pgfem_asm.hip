@@ -699,18 +699,18 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
 }
 
 // --------------------------------------------------------------------------
-// Node-group kernel of the class-binned schedule.  A block owns NG consecutive
-// nodes.  Phase A stages, once per (node a, incident element e) incidence, the
-// element's nodes, the TC gradients of its nodes and the left factor
-// L = w B_a^T D (uncut elements, recomputed from FP64 coordinate differences;
-// w = J * sum of the reference qp weights), and the FP64 internal-force
-// contribution of a (uncut: w B_a^T sigma with FP64 gradients; cut: the staged
-// FP64 residual).  Phase B gives each node pair (a, b) of the group one lane,
-// which sums over a's incidences -- in ascending element order, as the
-// reference's element loop -- L . B_b of uncut elements containing b and the
-// staged TK blocks of cut elements, in FP64, and writes the d x d block once
-// through the scatter indices.  A group whose incidences overflow the LDS
-// tables takes the same arithmetic from global memory.
+// Node-group kernels of the class-binned schedule.  A block owns NG nodes of a
+// group list (rows of the node-pair pattern).  Phase A takes one lane per
+// (node a, incident element e) incidence: for an uncut element it computes the
+// simplex gradients from FP64 coordinate differences, the FP64 internal force
+// w B_a^T sigma (w = J * sum of the reference qp weights) and the element's
+// d x d blocks (a, m) with the isotropic closed form of B^T D B in TC; for a
+// cut element it reads the row of a in the staged tangent and residual.  Each
+// contribution lands at its static (slot, rank) -- rank = ascending element
+// order among a's incidences that contain b -- so phase B, one lane per node
+// pair (a, b), sums it in the reference's element order, in FP64, and writes
+// the block once through the scatter indices.  A group whose incidences or
+// pairs overflow the LDS tables takes the same arithmetic from global memory.
 // --------------------------------------------------------------------------
 template <int D> struct GroupCfg;
 // per-group LDS capacity (incidences, pairs) for NG nodes: 7 incidences and 16
@@ -1130,9 +1130,10 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
 }
 
 // The same schedule for tet4 meshes, where the per-pair block tables would not
-// fit in LDS: phase A stores each incidence's gradients and left factor
-// L = w B_a^T D and registers (incidence, local node) codes at their static
-// (slot, rank); phase B multiplies out L B_b per contribution.
+// fit in LDS (and a tri3 variant): phase A stores each incidence's TC
+// gradients, w lambda and w mu, its diagonal block and internal force, and
+// registers (incidence, local node) codes at their static (slot, rank); phase
+// B evaluates the closed-form block of each contribution.
 template <int D, typename TC, typename TKC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
 __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TKC* __restrict__ Kc, const double* __restrict__ Pc,
