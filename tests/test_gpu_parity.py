@@ -63,18 +63,21 @@ def test_bitwise_repeatable(case, mode):
         assert np.array_equal(x, y)
 
 
-def test_func_only_and_jac_only(case):
-    p, U, ic, Kr, Pr, _ = case
-    asm = AssemblerGPU(p, "fp64")
+@pytest.mark.parametrize("mode", ["fp64", "binned64", "mixed", "mixed_cut64"])
+def test_func_only_and_jac_only(case, mode):
+    p, U, ic, Kr, Pr, Ir = case
+    asm = AssemblerGPU(p, mode)
+    tol = 1e-12 if mode in ("fp64", "binned64") else 1e-6
     K, P = PMatrix(p), PVector(p)
     K.values[:] = 7.0
     it = np.zeros(p.n_cqp)
     asm.committed_isvs[:] = ic
     asm.assemble(K, P, 0.0, it, U, True, False)
     assert np.all(K.values == 7.0) and rel(P.values, Pr) < 1e-12
+    assert np.abs(it - Ir).max(initial=0.0) <= 1e-12 * max(np.abs(Ir).max(initial=0.0), 1e-30)
     P.values[:] = 3.0
     asm.assemble(K, P, 0.0, it, U, False, True)
-    assert np.all(P.values == 3.0) and rel(K.values, Kr) < 1e-12
+    assert np.all(P.values == 3.0) and rel(K.values, Kr) < tol
     assert asm.assemble(K, P, 0.0, it, U, False, False) == 0
 
 
