@@ -162,6 +162,34 @@ __device__ inline void lei_stress(const double* Dm, const double* e, double* s) 
     }
 }
 
+// The isotropic LEI stress from three entries of D: c0 = D[0][0] (lambda +
+// 2 mu), c1 = D[0][1] (lambda) and c2 = D[last][last] (mu).  For the D that
+// LinearElasticIsotropicMaterial builds (src/material_lei.cpp:52-97), whose
+// equal entries are equal to the bit, this is lei_stress's arithmetic with its
+// exact-zero products skipped; the node-group kernels load these three values
+// instead of D.
+template <int D> struct LeiCoef {
+    static constexpr int N = 3;
+    __host__ __device__ static constexpr int index(int k) {
+        return k == 0 ? 0 : (k == 1 ? 1 : (D == 2 ? 8 : 35));
+    }
+};
+template <int D>
+__device__ inline void lei_stress_c(const double* c, const double* e, double* s) {
+    if constexpr (D == 2) {
+        s[0] = c[0] * e[0] + c[1] * e[1];
+        s[1] = c[1] * e[0] + c[0] * e[1];
+        s[2] = c[2] * e[2];
+    } else {
+        s[0] = c[0] * e[0] + c[1] * e[1] + c[1] * e[2];
+        s[1] = c[1] * e[0] + c[0] * e[1] + c[1] * e[2];
+        s[2] = c[1] * e[0] + c[1] * e[1] + c[0] * e[2];
+        s[3] = c[2] * e[3];
+        s[4] = c[2] * e[4];
+        s[5] = c[2] * e[5];
+    }
+}
+
 // Cohesive constitutive update.  getDeltaEff (src/Material_BilinearCohesiveNU.cpp:48-66,
 // src/Material_OPCohesiveNU.cpp:43-61) overwrites a near-zero opening with 1e-20;
 // traction and tangent follow BilinearCohesiveMaterial::Constitutive

@@ -67,9 +67,9 @@ struct DevProb {
     // words) and slot words in a block of 7 NG (tri3) or 24 NG (tet4) slots
     // (group-list order, then the incidence order of the group's nodes), so
     // that a lane loads its record without the node table.  Uncut: the
-    // element's nodes and a word
-    // 1 << 31 | material | lo << 3 | rank << 8 | la << 13; cut: block index of
-    // the owner's staged row, residual offset, la | nloc << 8 | lo << 16 |
+    // element's nodes, owner first (owner_first order), and a word
+    // 1 << 31 | material | lo << 3 | rank << 8; cut: block index of the
+    // owner's staged row, residual offset, la | nloc << 8 | lo << 16 |
     // rank << 21, cut index (lo: the owner's position in the group; rank: the
     // incidence's position among the owner's; la: the owner's local index)
     const int32_t* incg;
@@ -115,6 +115,15 @@ constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
 constexpr int CUT_MAXC = PG_MAX_COH * 4; // cohesive qps per cut element
 constexpr int MAX_LDS_MATS = 8;          // materials staged in LDS
+constexpr int64_t KC_PAD = 8 * 9;        // staged cut tangents: padding past the last row
+
+// Local node at position n of an uncut element listed from its local node la:
+// cyclic for tri3, la xor n for tet4 -- both even permutations of the node
+// order, so the orientation (the sign of the Jacobian) is kept.  The node-group
+// kernels compute an incidence's element in this order, owner first.
+__host__ __device__ inline int owner_first(int nen, int la, int n) {
+    return nen == 3 ? (la + n) % 3 : (la ^ n);
+}
 
 template <int D> struct CutLim;
 template <> struct CutLim<2> { static constexpr int N = MAXN2, Q = MAXQ2; };
@@ -756,27 +765,84 @@ __device__ inline T simplex_grads(const double (*x)[3], T (*g)[D]) {
     }
 }
 
-// FP64 internal-force contribution w B_la^T sigma of an uncut element.
+// Row la of a NEN x D register array (a select chain: la varies per lane).
+template <int NEN, int D, typename T>
+__device__ inline void sel_row(const T (*a)[D], int la, T* out) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        T v = a[NEN - 1][c];
+#pragma unroll
+        for (int n = NEN - 2; n >= 0; --n) v = la == n ? a[n][c] : v;
+        out[c] = v;
+    }
+}
+
+// FP64 internal-force contribution w B_la^T sigma of an uncut element, from
+// the element's nodal U and the LEI coefficients of its material (LeiCoef).
+// The sums run over the nonzero entries of the Voigt B only, in the order of
+// the dense products (B U by node and component, then B^T sigma by row): the
+// skipped terms are exact zeros.
 template <int D>
-__device__ inline void plain_p(const double (*g)[D], const int* nodes, int la, double w, const double* Dm,
-                               const double* Un, double* out) {
+__device__ inline void plain_p(const double (*g)[D], const double (*u)[D], int la, double w, const double* lc,
+                               double* out) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
-    double eps[NS], sig[NS];
-    for (int k = 0; k < NS; ++k) eps[k] = 0.0;
-    for (int n = 0; n < NEN; ++n)
-        for (int c = 0; c < D; ++c) {
-            double bb[NS];
-            bcol<D, double>(g[n], c, bb);
-            const double u = Un[int64_t(nodes[n]) * D + c];
-            for (int k = 0; k < NS; ++k) eps[k] += bb[k] * u;
+    double e[NS], sig[NS], gl[D];
+    if constexpr (D == 2) {
+        // B column (n, x): rows xx (g_x), xy (g_y); column (n, y): rows yy (g_y), xy (g_x)
+        e[0] = g[0][0] * u[0][0];
+        e[2] = g[0][1] * u[0][0];
+        e[1] = g[0][1] * u[0][1];
+        e[2] = e[2] + g[0][0] * u[0][1];
+        for (int n = 1; n < NEN; ++n) {
+            e[0] = e[0] + g[n][0] * u[n][0];
+            e[2] = e[2] + g[n][1] * u[n][0];
+            e[1] = e[1] + g[n][1] * u[n][1];
+            e[2] = e[2] + g[n][0] * u[n][1];
         }
-    lei_stress<D>(Dm, eps, sig);
-    for (int i = 0; i < D; ++i) {
-        double bi[NS];
-        bcol<D, double>(g[la], i, bi);
-        double pt = 0.0;
-        for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
-        out[i] = w * pt;
+    } else {
+        // columns (n, x): rows xx, xz, xy; (n, y): yy, yz, xy; (n, z): zz, yz, xz
+        e[0] = g[0][0] * u[0][0];
+        e[4] = g[0][2] * u[0][0];
+        e[5] = g[0][1] * u[0][0];
+        e[1] = g[0][1] * u[0][1];
+        e[3] = g[0][2] * u[0][1];
+        e[5] = e[5] + g[0][0] * u[0][1];
+        e[2] = g[0][2] * u[0][2];
+        e[3] = e[3] + g[0][1] * u[0][2];
+        e[4] = e[4] + g[0][0] * u[0][2];
+        for (int n = 1; n < NEN; ++n) {
+            e[0] = e[0] + g[n][0] * u[n][0];
+            e[4] = e[4] + g[n][2] * u[n][0];
+            e[5] = e[5] + g[n][1] * u[n][0];
+            e[1] = e[1] + g[n][1] * u[n][1];
+            e[3] = e[3] + g[n][2] * u[n][1];
+            e[5] = e[5] + g[n][0] * u[n][1];
+            e[2] = e[2] + g[n][2] * u[n][2];
+            e[3] = e[3] + g[n][1] * u[n][2];
+            e[4] = e[4] + g[n][0] * u[n][2];
+        }
+    }
+    lei_stress_c<D>(lc, e, sig);
+    sel_row<NEN, D, double>(g, la, gl);
+    if constexpr (D == 2) {
+        out[0] = w * (gl[0] * sig[0] + gl[1] * sig[2]);
+        out[1] = w * (gl[1] * sig[1] + gl[0] * sig[2]);
+    } else {
+        out[0] = w * (gl[0] * sig[0] + gl[2] * sig[4] + gl[1] * sig[5]);
+        out[1] = w * (gl[1] * sig[1] + gl[2] * sig[3] + gl[0] * sig[5]);
+        out[2] = w * (gl[2] * sig[2] + gl[1] * sig[3] + gl[0] * sig[4]);
+    }
+}
+
+// Stages lambda, mu of every material in LDS (zero past the last one).  The
+// loads use a clamped index and a select, so that they go out together with
+// the other set-up loads.
+template <typename TC, int BS>
+__device__ inline void stage_lm(const DevProb& p, int ns, TC (*lm)[2]) {
+    for (int k = threadIdx.x; k < MAX_LDS_MATS * 2; k += BS) {
+        const int m = k >> 1, j = k & 1;
+        const double v0 = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[j == 0 ? 1 : ns * ns - 1];
+        lm[m][j] = m < p.n_mat ? TC(v0) : TC(0);
     }
 }
 
@@ -791,8 +857,9 @@ template <int D, int NG, int TV> struct GroupThreads {
 // Node-pair metadata of a group's local pair kq: owner slot lo, row node a,
 // global pair index q, column node b, the row's dofs and (2D) the scatter
 // indices of the d x d block.  It depends only on the group's node table, so
-// the kernels load it before phase A and its latency overlaps the incidence
-// loads (3D keeps the nine scatter indices for after phase A: registers).
+// the 2D kernels load it before phase A and its latency overlaps the incidence
+// loads (the 3D list kernel loads it in phase B: at 128 VGPRs, holding it
+// through phase A spills).
 template <int D> struct PairMeta {
     int lo, a, q, b;
     int dof[D];
@@ -829,24 +896,38 @@ __device__ inline PairMeta<D> pair_meta(const DevProb& p, const int* s_qptr, con
 // tangent keeps annihilating rigid translations, which FP32 rounding of each
 // block on its own would spoil (loosely bonded fibres turn that spurious
 // stiffness into slow Newton convergence).  FP64 tangents use the closed form.
+// The off-diagonal blocks of row la come in ascending order of the other
+// local node m = k < la ? k : k + 1 (k = 0 .. NEN - 2), which is also the
+// order of the slot fields; diag_acc folds block k into the diagonal.
+template <int D, typename TC>
+__device__ inline void diag_acc(int k, const TC (*blk)[D], double (*out)[D]) {
+    if constexpr (sizeof(TC) == 4)
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) out[i][j] = k == 0 ? -double(blk[i][j]) : out[i][j] - double(blk[i][j]);
+}
+template <int D, typename TC>
+__device__ inline void diag_fin(const TC* ga, TC wl, TC wm, double (*out)[D]) {
+    if constexpr (sizeof(TC) != 4) {
+        TC blk[D][D];
+        iso_block<D, TC>(ga, ga, wl, wm, blk);
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) out[i][j] = double(blk[i][j]);
+    }
+}
 template <int D, typename TC>
 __device__ inline void diag_block(const TC (*gt)[D], int la, TC wl, TC wm, double (*out)[D]) {
     constexpr int NEN = D + 1;
+    TC ga[D];
+    sel_row<NEN, D, TC>(gt, la, ga);
     if constexpr (sizeof(TC) == 4) {
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) out[i][j] = 0.0;
-        for (int m = 0; m < NEN; ++m) {
-            if (m == la) continue;
-            TC blk[D][D];
-            iso_block<D, TC>(gt[la], gt[m], wl, wm, blk);
-            for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) out[i][j] -= double(blk[i][j]);
+        for (int k = 0; k < NEN - 1; ++k) {
+            TC gb[D], blk[D][D];
+            for (int c = 0; c < D; ++c) gb[c] = k < la ? gt[k][c] : gt[k + 1][c];
+            iso_block<D, TC>(ga, gb, wl, wm, blk);
+            diag_acc<D, TC>(k, blk, out);
         }
     } else {
-        TC blk[D][D];
-        iso_block<D, TC>(gt[la], gt[la], wl, wm, blk);
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) out[i][j] = double(blk[i][j]);
+        diag_fin<D, TC>(ga, wl, wm, out);
     }
 }
 
@@ -874,6 +955,25 @@ __device__ inline void cut_diag(const TK* Ke, int la, double* out) {
     for (int k = 0; k < D * D; ++k) out[k] = double(Ke[la * D * D + k]);
 }
 
+// cut_diag from the row's blocks in registers (kb: blocks 0 .. M - 1 of the
+// row, kd: block la), with the same arithmetic.
+template <int D, typename TK, int M>
+__device__ inline void cut_diag_rows(const TK (*kb)[D * D], const TK* kd, int la, double* out) {
+    constexpr int NEN = D + 1;
+    if constexpr (sizeof(TK) == 4) {
+        if (la < NEN) {
+            for (int k = 0; k < D * D; ++k) out[k] = 0.0;
+#pragma unroll
+            for (int m = 0; m < NEN; ++m) {
+                if (m == la) continue;
+                for (int k = 0; k < D * D; ++k) out[k] -= double(kb[m][k]);
+            }
+            return;
+        }
+    }
+    for (int k = 0; k < D * D; ++k) out[k] = double(kd[k]);
+}
+
 // One node pair (a, b) from global memory: the incidences [t0, t1) of a in
 // element order, with the arithmetic of the LDS-staged paths (used by groups
 // whose incidences or pairs exceed the LDS tables).
@@ -890,11 +990,16 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
         const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
         if (la < 0 || lb < 0) continue;
         if (ci < 0) {
+            // the element owner first, as the node-group kernels list it
+            const int lbr = NEN == 3 ? (lb - la + 3) % 3 : (lb ^ la);  // b's position in that order
             int nodes[NEN];
-            double x[NEN][3];
+            double x[NEN][3], u[NEN][D];
             for (int n = 0; n < NEN; ++n) {
-                nodes[n] = p.conn[int64_t(e) * NEN + n];
-                for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
+                nodes[n] = p.conn[int64_t(e) * NEN + owner_first(NEN, la, n)];
+                for (int c = 0; c < D; ++c) {
+                    x[n][c] = p.coords[int64_t(nodes[n]) * D + c];
+                    u[n][c] = Un[int64_t(nodes[n]) * D + c];
+                }
             }
             const int mi = p.elem_mat[e];
             double g[NEN][D];
@@ -907,18 +1012,19 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
                 const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
                 if (la == lb) {
                     double dd[D][D];
-                    diag_block<D, TC>(gt, la, wl, wm, dd);
+                    diag_block<D, TC>(gt, 0, wl, wm, dd);
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += dd[i][j];
                 } else {
-                    iso_block<D, TC>(gt[la], gt[lb], wl, wm, blk);
+                    iso_block<D, TC>(gt[0], gt[lbr], wl, wm, blk);
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                 }
             }
             if (diag) {
-                double pp[D];
-                plain_p<D>(g, nodes, la, w, p.mats[mi].D, Un, pp);
+                double pp[D], lc[LeiCoef<D>::N];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                plain_p<D>(g, u, 0, w, lc, pp);
                 for (int i = 0; i < D; ++i) pa[i] += pp[i];
             }
         } else {
@@ -966,49 +1072,39 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    const int tid = threadIdx.x;
     const int grp = xcd_order(blockIdx.x, gridDim.x);
     const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(grp) * NG;
     constexpr int ng = NG;
-    // the group's incidence records and slot words do not depend on the node
-    // table: their loads go out together with it
+    // every set-up load goes out before the first wait: the group's node table
+    // (NG entries, every lane loads one), its incidence records and slot words
+    // (their block does not depend on the node table; lanes past the block
+    // reload its last slot) and lambda, mu of the materials
+    const int4 r = gt[tid & (NG - 1)];
     const int64_t ib = ibase + int64_t(grp) * MAXI;
-    int4 rec0 = make_int4(0, 0, 0, 0);
-    uint64_t sw0 = 0;
-    if (int(threadIdx.x) < MAXI) {
-        rec0 = reinterpret_cast<const int4*>(p.incg)[ib + threadIdx.x];
-        sw0 = p.slotg[ib + threadIdx.x];
-    }
-    if (threadIdx.x < MAX_LDS_MATS) {
-        const int m = threadIdx.x;
-        sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
-        sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
-    }
-    if (threadIdx.x < 64) {
-        // the group's nodes and their incidence / pair counts, prefix-summed
-        // across the first wavefront (NG <= 32)
-        int ci = 0, cq = 0;
-        if (int(threadIdx.x) < ng) {
-            const int4 r = gt[threadIdx.x];
-            s_node[threadIdx.x] = r.x;
-            s_ibeg[threadIdx.x] = r.y;
-            s_qbeg[threadIdx.x] = r.z;
-            ci = r.w & 0xffff;
-            cq = r.w >> 16;
-        }
+    const int is = tid < MAXI ? tid : MAXI - 1;
+    const int4 rec0 = reinterpret_cast<const int4*>(p.incg)[ib + is];
+    const uint64_t sw0 = p.slotg[ib + is];
+    stage_lm<TC, BS>(p, NS, sLM);
+    // the group's nodes and their incidence / pair counts, prefix-summed across
+    // the wavefront (NG <= 32: the first wavefront holds the group)
+    int ci = tid < ng ? (r.w & 0xffff) : 0, cq = tid < ng ? (r.w >> 16) : 0;
 #pragma unroll
-        for (int off = 1; off < NG; off <<= 1) {
-            const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
-            if (int(threadIdx.x) >= off) { ci += ti; cq += tq; }
-        }
-        if (int(threadIdx.x) < ng) {
-            s_iptr[threadIdx.x + 1] = ci;
-            s_qptr[threadIdx.x + 1] = cq;
-        }
-        if (threadIdx.x == 0) {
-            s_iptr[0] = 0;
-            s_qptr[0] = 0;
-            s_bad = 0;
-        }
+    for (int off = 1; off < NG; off <<= 1) {
+        const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+        if ((tid & 63) >= off) { ci += ti; cq += tq; }
+    }
+    if (tid < ng) {
+        s_node[tid] = r.x;
+        s_ibeg[tid] = r.y;
+        s_qbeg[tid] = r.z;
+        s_iptr[tid + 1] = ci;
+        s_qptr[tid + 1] = cq;
+    }
+    if (tid == 0) {
+        s_iptr[0] = 0;
+        s_qptr[0] = 0;
+        s_bad = 0;
     }
     __syncthreads();
     const int ni = s_iptr[ng], nq = s_qptr[ng];
@@ -1036,48 +1132,73 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
             if (rd >= CAPD || (sw >> 63)) { s_bad = 1; continue; }
             const int ps0 = s_qptr[lo], np = s_qptr[lo + 1] - s_qptr[lo];
             if (tag < 0) {
+                // the element's nodes, owner first (the record lists them so)
                 const int ln[NEN] = {rec.x, rec.y, rec.z};
-                const int la = (tag >> 13) & 3;
-                double x[NEN][3];
+                double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
                 for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                    for (int c = 0; c < D; ++c) {
+                        x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                        u[n][c] = Un[int64_t(ln[n]) * D + c];
+                    }
+                const int mi = tag & 7;
+                double lc[LeiCoef<D>::N];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                // every load of the incidence is out before the arithmetic
+                __builtin_amdgcn_sched_barrier(0);
                 double g[NEN][D];
                 const double J = simplex_grads<D, double>(x, g);
                 const double w = J * wsum;
-                const int mi = tag & 7;
-                if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_dP[lo][rd]);
+                // (also without PG_FUNC, where phase B ignores it: the U loads
+                // then stay ahead of the gradient arithmetic)
+                plain_p<D>(g, u, 0, w, lc, s_dP[lo][rd]);
                 if (!jac) continue;
                 TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
                 const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
-                diag_block<D, TC>(gt, la, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[lo][rd]));
-                int k = 0;
-                for (int m = 0; m < NEN; ++m) {
-                    if (m == la) continue;
+                // each off-diagonal block (owner, n) once: to its slot, and
+                // (FP32) into the row-sum diagonal
+                double dd[D][D];
+#pragma unroll
+                for (int k = 0; k < NEN - 1; ++k) {
+                    TC blk[D][D];
+                    iso_block<D, TC>(gt[0], gt[k + 1], wl, wm, blk);
+                    diag_acc<D, TC>(k, blk, dd);
                     const int f = int((sw >> (9 * k)) & 511u);
-                    ++k;
                     const int slot = f & 63, rank = f >> 6;
                     if (slot >= np || rank >= CAPO) { s_bad = 1; continue; }
-                    iso_block<D, TC>(gt[la], gt[m], wl, wm, reinterpret_cast<TC (*)[D]>(s_blk[ps0 + slot][rank]));
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) s_blk[ps0 + slot][rank][i * D + j] = blk[i][j];
                 }
+                diag_fin<D, TC>(gt[0], wl, wm, dd);
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) s_dblk[lo][rd][i * D + j] = dd[i][j];
             } else {
-                // cut element: the record holds the staged row of a directly
+                // cut element: the record holds the staged row of a directly.
+                // The row's MAXLOC blocks (the staged rows are padded), the
+                // owner's diagonal block and its residual go out at once.
                 const int la = rec.z & 255, nloc = (rec.z >> 8) & 255;
-                if (nloc > MAXLOC || la >= nloc) { s_bad = 1; continue; }  // does not fit: global path
-                if (func)
-                    for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = Pc[int64_t(rec.y) + c];
-                if (!jac) continue;
                 const TC* Ke = Kc + int64_t(uint32_t(rec.x)) * (D * D);  // blocks (la, 0 .. nloc - 1)
-                cut_diag<D, TC>(Ke, la, s_dblk[lo][rd]);
-                int k = 0;
-                for (int m = 0; m < nloc; ++m) {
-                    if (m == la) continue;
+                TC kb[MAXLOC][D * D], kd[D * D];
+                double pc[D];
+                for (int m = 0; m < MAXLOC; ++m)
+                    for (int kk = 0; kk < D * D; ++kk) kb[m][kk] = Ke[m * D * D + kk];
+                const int lc = la < MAXLOC ? la : 0;
+                for (int kk = 0; kk < D * D; ++kk) kd[kk] = Ke[lc * D * D + kk];
+                for (int c = 0; c < D; ++c) pc[c] = Pc[int64_t(rec.y) + c];
+                __builtin_amdgcn_sched_barrier(0);
+                if (nloc > MAXLOC || la >= nloc) { s_bad = 1; continue; }  // does not fit: global path
+                for (int c = 0; c < D; ++c) s_dP[lo][rd][c] = pc[c];
+                if (!jac) continue;
+                cut_diag_rows<D, TC, MAXLOC>(kb, kd, la, s_dblk[lo][rd]);
+#pragma unroll
+                for (int m = 0; m < MAXLOC; ++m) {
+                    if (m >= nloc || m == la) continue;
+                    const int k = m < la ? m : m - 1;  // slot field of m
                     const int f = int((sw >> (9 * k)) & 511u);
-                    ++k;
                     const int slot = f & 63, rank = f >> 6;
                     if (slot >= np || rank >= CAPO) { s_bad = 1; continue; }
-                    for (int kk = 0; kk < D * D; ++kk) s_blk[ps0 + slot][rank][kk] = Ke[m * D * D + kk];
+                    for (int kk = 0; kk < D * D; ++kk) s_blk[ps0 + slot][rank][kk] = kb[m][kk];
                 }
             }
         }
@@ -1134,8 +1255,9 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
 // gradients, w lambda and w mu, its diagonal block and internal force, and
 // registers (incidence, local node) codes at their static (slot, rank); phase
 // B evaluates the closed-form block of each contribution.
+// (at most 128 VGPRs: four wavefronts per SIMD where LDS allows them)
 template <int D, typename TC, typename TKC, int NG, int TV, int BS = GroupThreads<D, NG, TV>::value>
-__global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __restrict__ Un,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_lists(DevProb p, const double* __restrict__ Un,
                                                     const TKC* __restrict__ Kc, const double* __restrict__ Pc,
                                                     double* __restrict__ K, double* __restrict__ P, unsigned flags,
                                                     int gbase, int64_t ibase) {
@@ -1162,50 +1284,40 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    const int tid = threadIdx.x;
     const int grp = xcd_order(blockIdx.x, gridDim.x);
     const int4* gt = reinterpret_cast<const int4*>(p.gtab) + gbase + int64_t(grp) * NG;
     constexpr int ng = NG;
-    // the first pass's incidence records and slot words do not depend on the
-    // node table: their loads go out together with it
+    // every set-up load goes out before the first wait: the group's node table
+    // (NG entries, every lane loads one), the first pass's incidence records and
+    // slot words (their block does not depend on the node table; lanes past the
+    // block reload its last slot) and lambda, mu of the materials
+    const int4 r = gt[tid & (NG - 1)];
     const int64_t ib = ibase + int64_t(grp) * MAXI;
+    const int is = tid < MAXI ? tid : MAXI - 1;
     int rec0[NEN + 1];
-    uint64_t sw0 = 0;
-    for (int n = 0; n <= NEN; ++n) rec0[n] = 0;
-    if (int(threadIdx.x) < MAXI) {
-        for (int n = 0; n <= NEN; ++n) rec0[n] = p.incg[(ib + threadIdx.x) * (NEN + 1) + n];
-        sw0 = p.slotg[ib + threadIdx.x];
-    }
-    if (threadIdx.x < MAX_LDS_MATS) {
-        const int m = threadIdx.x;
-        sLM[m][0] = m < p.n_mat ? TC(p.mats[m].D[1]) : TC(0);
-        sLM[m][1] = m < p.n_mat ? TC(p.mats[m].D[NS * NS - 1]) : TC(0);
-    }
-    if (threadIdx.x < 64) {
-        // the group's nodes and their incidence / pair counts, prefix-summed
-        // across the first wavefront (NG <= 32)
-        int ci = 0, cq = 0;
-        if (int(threadIdx.x) < ng) {
-            const int4 r = gt[threadIdx.x];
-            s_node[threadIdx.x] = r.x;
-            s_ibeg[threadIdx.x] = r.y;
-            s_qbeg[threadIdx.x] = r.z;
-            ci = r.w & 0xffff;
-            cq = r.w >> 16;
-        }
+    for (int n = 0; n <= NEN; ++n) rec0[n] = p.incg[(ib + is) * (NEN + 1) + n];
+    const uint64_t sw0 = p.slotg[ib + is];
+    stage_lm<TC, BS>(p, NS, sLM);
+    // the group's nodes and their incidence / pair counts, prefix-summed across
+    // the wavefront (NG <= 32: the first wavefront holds the group)
+    int ci = tid < ng ? (r.w & 0xffff) : 0, cq = tid < ng ? (r.w >> 16) : 0;
 #pragma unroll
-        for (int off = 1; off < NG; off <<= 1) {
-            const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
-            if (int(threadIdx.x) >= off) { ci += ti; cq += tq; }
-        }
-        if (int(threadIdx.x) < ng) {
-            s_iptr[threadIdx.x + 1] = ci;
-            s_qptr[threadIdx.x + 1] = cq;
-        }
-        if (threadIdx.x == 0) {
-            s_iptr[0] = 0;
-            s_qptr[0] = 0;
-            s_bad = 0;
-        }
+    for (int off = 1; off < NG; off <<= 1) {
+        const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+        if ((tid & 63) >= off) { ci += ti; cq += tq; }
+    }
+    if (tid < ng) {
+        s_node[tid] = r.x;
+        s_ibeg[tid] = r.y;
+        s_qbeg[tid] = r.z;
+        s_iptr[tid + 1] = ci;
+        s_qptr[tid + 1] = cq;
+    }
+    if (tid == 0) {
+        s_iptr[0] = 0;
+        s_qptr[0] = 0;
+        s_bad = 0;
     }
     __syncthreads();
     const int ni = s_iptr[ng], nq = s_qptr[ng];
@@ -1231,27 +1343,35 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
             int la = -1;
             int nloc = NEN;
             if (tag < 0) {
+                // the element's nodes, owner first (the record lists them so)
                 int ln[NEN];
                 for (int n = 0; n < NEN; ++n) ln[n] = rec[n];
-                la = (tag >> 13) & 3;
-                double x[NEN][3];
+                la = 0;
+                double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
                 for (int n = 0; n < NEN; ++n)
-                    for (int c = 0; c < D; ++c) x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                    for (int c = 0; c < D; ++c) {
+                        x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                        u[n][c] = Un[int64_t(ln[n]) * D + c];
+                    }
+                const int mi = tag & 7;
+                double lc[LeiCoef<D>::N];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                // every load of the incidence is out before the arithmetic
+                __builtin_amdgcn_sched_barrier(0);
                 double g[NEN][D];
                 const double J = simplex_grads<D, double>(x, g);
                 const double w = J * wsum;
                 TC gt[NEN][D];
                 for (int n = 0; n < NEN; ++n)
                     for (int c = 0; c < D; ++c) { gt[n][c] = TC(g[n][c]); s_g[i][n][c] = gt[n][c]; }
-                const int mi = tag & 7;
-                if (la >= 0) {
-                    if (func) plain_p<D>(g, ln, la, w, p.mats[mi].D, Un, s_P[i]);
-                    if (jac) {
-                        const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
-                        s_wlm[i][0] = wl;
-                        s_wlm[i][1] = wm;
-                        diag_block<D, TC>(gt, la, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[i]));
-                    }
+                // (also without PG_FUNC, where phase B ignores it: the U loads
+                // then stay ahead of the gradient arithmetic)
+                plain_p<D>(g, u, 0, w, lc, s_P[i]);
+                if (jac) {
+                    const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+                    s_wlm[i][0] = wl;
+                    s_wlm[i][1] = wm;
+                    diag_block<D, TC>(gt, 0, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[i]));
                 }
             } else {
                 // cut element: the record holds the staged row of a directly
@@ -1320,10 +1440,9 @@ __global__ __launch_bounds__(BS) void k_pair_lists(DevProb p, const double* __re
                     if (code == 0xffffu) continue;
                     const int t = int(code >> 4), lb = int(code & 15);
                     const int info = s_info[t];
-                    const int la = info & 255;
                     if (!(info & (1 << 16))) {
-                        TC blk[D][D];
-                        iso_block<D, TC>(s_g[t][la], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
+                        TC blk[D][D];  // uncut: the owner is local node 0
+                        iso_block<D, TC>(s_g[t][0], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
                         for (int i = 0; i < D; ++i)
                             for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                     } else {
@@ -1869,13 +1988,26 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
                             int32_t* dst = &gi[(base + i) * W];
                             const int rk = std::min(r, 31);
                             if (src[s->nen] < 0) {
+                                // uncut: the element's nodes owner first, in the
+                                // even permutation owner_first(la, .) of its node
+                                // order, and the slot fields in that order
                                 int la = 0;
-                                for (int n = 0; n < s->nen; ++n) {
-                                    dst[n] = src[n];
+                                for (int n = 0; n < s->nen; ++n)
                                     if (src[n] == a) la = n;
+                                const uint64_t w0 = slotv[size_t(t)];
+                                uint64_t w1 = w0 & (uint64_t(1) << 63);
+                                for (int n = 0; n < s->nen; ++n) {
+                                    const int m = owner_first(s->nen, la, n);
+                                    dst[n] = src[m];
+                                    if (n > 0 && !(w0 >> 63)) {
+                                        const int k0 = m < la ? m : m - 1;  // field of m in local order
+                                        w1 |= ((w0 >> (9 * k0)) & 511u) << (9 * (n - 1));
+                                    }
                                 }
                                 dst[s->nen] = int32_t(0x80000000u | uint32_t(-1 - src[s->nen]) | uint32_t(lo) << 3 |
-                                                      uint32_t(rk) << 8 | uint32_t(la) << 13);
+                                                      uint32_t(rk) << 8);
+                                gs[base + i] = w1;
+                                continue;
                             } else {
                                 dst[0] = src[0]; dst[1] = src[1];
                                 dst[2] = (src[2] & 0xffff) | (lo << 16) | (rk << 21);
@@ -1950,8 +2082,9 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     if ((rc = alloc(h, &h->Un, size_t(nall * s->dim)))) return fail(rc);
     (void)ndp;
     if ((rc = alloc(h, &h->stage, size_t(eoff[s->n_elem])))) return fail(rc);
-    if ((rc = alloc(h, &h->Kc64, size_t(h->cut_k_total)))) return fail(rc);
-    if ((rc = alloc(h, &h->Kc32, size_t(h->cut_k_total)))) return fail(rc);
+    // the node-group kernels read a staged row's first 8 blocks at once: pad
+    if ((rc = alloc(h, &h->Kc64, size_t(h->cut_k_total) + KC_PAD))) return fail(rc);
+    if ((rc = alloc(h, &h->Kc32, size_t(h->cut_k_total) + KC_PAD))) return fail(rc);
     if ((rc = alloc(h, &h->Pc, size_t(h->cut_p_total)))) return fail(rc);
     if ((rc = alloc(h, &h->err, 1))) return fail(rc);
     if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) { g_err = "hipMemset failed"; return fail(PG_E_HIP); }
