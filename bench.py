@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--tiles", default=None,
+                    help="group_nodes,cut_variant,group_variant for every schedule instead of autotuning "
+                         "(short profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only",
@@ -88,7 +91,10 @@ def main():
     tuned = {}
     for mode in ("binned64", "mixed", "mixed_cut64"):
         asm.mode = mode
-        tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
+        if args.tiles:
+            tuned[mode] = tuple(int(x) for x in args.tiles.split(","))
+        else:
+            tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
         log(f"[bench] tuned {mode}: group_nodes, cut_variant, group_variant = {tuned[mode]}")
 
     def run(mode, steps):

@@ -4,6 +4,7 @@
 // it restates (brandyb2/PIGFEM, file:line).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 #include "pgfem_asm.h"
 
@@ -27,6 +28,21 @@ __constant__ const double TET_Q[5][3] = {{0.25, 0.25, 0.25},
                                          {0.166666666666667, 0.166666666666667, 0.5},
                                          {0.166666666666667, 0.5, 0.166666666666667}};
 __constant__ const double TET_QW[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+
+// Entry q of a rule table for a q that varies per lane, as a chain of selects
+// between immediates (an indexed load would put a memory round trip in the
+// middle of the qp arithmetic).
+template <int N>
+__device__ inline double pick(const double (&t)[N], int q) {
+    double v = t[N - 1];
+#pragma unroll
+    for (int k = N - 2; k >= 0; --k) {
+        double c = t[k];
+        asm("" : "+v"(c));  // keeps the chain from being folded back into a table load
+        v = q == k ? c : v;
+    }
+    return v;
+}
 
 // Reference-coordinate gradients of the linear simplex shape functions
 // (Tri3::compute_shape_grad src/tri3.cpp:219-225, Tet4 src/tet4.cpp:147-154).
@@ -198,8 +214,18 @@ __device__ inline void lei_stress_c(const double* c, const double* e, double* s)
 // OPCohesiveMaterial::Constitutive (src/Material_OPCohesive.cpp:38-98, delta_max
 // updated on loading) and OPCohesiveNU (src/Material_OPCohesiveNU.cpp:64-111).
 // delta: in/out (ttn frame, normal last); isv: trial delta_max (in/out).
-template <int D>
-__device__ inline void cohesive_law(const pg_material& m, double* delta, double& isv, double* t, double Dm[D][D]) {
+// M: pg_material, or any type with its cohesive fields (type, sigma_c,
+// delta_c, delta_c1, delta_c2, beta2, alpha), such as CohMat.
+struct CohMat {
+    int32_t type;
+    double sigma_c, delta_c, delta_c1, delta_c2, beta2, alpha;
+};
+// the six doubles from sigma_c to alpha are contiguous in both (kernels copy them
+// as an array)
+static_assert(offsetof(pg_material, alpha) == offsetof(pg_material, sigma_c) + 5 * sizeof(double), "layout");
+static_assert(offsetof(CohMat, alpha) == offsetof(CohMat, sigma_c) + 5 * sizeof(double), "layout");
+template <int D, typename M>
+__device__ inline void cohesive_law(const M& m, double* delta, double& isv, double* t, double Dm[D][D]) {
     double norm = 0;
     for (int i = 0; i < D; ++i) norm += delta[i] * delta[i];
     if (norm < 1e-40)

@@ -145,7 +145,7 @@ __device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3],
     } else {
         for (int n = 0; n < cnn; ++n)
             for (int a = 0; a < D; ++a) cdN[n][a] = simplex_dNdr(n, a);
-        w = (D == 2) ? TRI_QW[q] : TET_QW[q];
+        w = (D == 2) ? pick(TRI_QW, q) : pick(TET_QW, q);
     }
     return transform<D>(cnn, cdN, cx, cg);
 }
@@ -163,9 +163,10 @@ __device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], d
         dNr[0][0] = -0.5; dNr[1][0] = 0.5;
         w = 1.0;
     } else {
-        N[0] = 1.0 - TRI_QX[q] - TRI_QY[q]; N[1] = TRI_QX[q]; N[2] = TRI_QY[q];
+        const double qx = pick(TRI_QX, q), qy = pick(TRI_QY, q);
+        N[0] = 1.0 - qx - qy; N[1] = qx; N[2] = qy;
         dNr[0][0] = -1.0; dNr[0][1] = -1.0; dNr[1][0] = 1.0; dNr[1][1] = 0.0; dNr[2][0] = 0.0; dNr[2][1] = 1.0;
-        w = TRI_QW[q];
+        w = pick(TRI_QW, q);
     }
     double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
     for (int n = 0; n < cnn; ++n)
@@ -196,8 +197,8 @@ __device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], d
 // One cohesive qp: opening, law (trial history in/out), R^T D R and R^T t
 // (KernelCohesive, src/AssemblerSmallStrainStructural.cpp:86-103 with
 // cohesiveDeltaFast / KFast / InternalForceFast, src/Utilities.cpp:528-594).
-template <int D>
-__device__ inline void cohesive_point(const pg_material& m, int cnn, const double* N, const double R[D][D],
+template <int D, typename M>
+__device__ inline void cohesive_point(const M& m, int cnn, const double* N, const double R[D][D],
                                       const double* Ulo, const double* Uup, double& hist, double RDR[D][D],
                                       double tR[D]) {
     double dl[D], del[D];
@@ -212,7 +213,7 @@ __device__ inline void cohesive_point(const pg_material& m, int cnn, const doubl
         del[i] = s;
     }
     double t[D], Dm[D][D];
-    cohesive_law<D>(m, del, hist, t, Dm);
+    cohesive_law<D, M>(m, del, hist, t, Dm);
     double RD[D][D];
     for (int i = 0; i < D; ++i)
         for (int j = 0; j < D; ++j) {
@@ -425,7 +426,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     constexpr int CUT_MAXC = MAXC;
     __shared__ double xs[EPW][MAXN][3];
     __shared__ double us[EPW][MAXN * D];
-    __shared__ double qg[EPW][MAXQ][MAXN][D];  // gradient of every local node at every qp
+    // gradients: a parent node's is the same at every qp (linear simplex
+    // parent), an enrichment node's is its child's at that qp (or zero)
+    __shared__ double pg[EPW][NEN][D];
+    __shared__ double qg[EPW][MAXQ][MAXN - NEN][D];
     __shared__ double qw[EPW][MAXQ];
     __shared__ TK qwl[EPW][MAXQ][2];        // w lambda, w mu of each qp (its child's material)
     __shared__ uint32_t qmask[EPW][MAXN];   // qps at which a local node's gradient is not zero
@@ -437,29 +441,60 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double ctR[EPW][CUT_MAXC][D];
     __shared__ pg_cut_rec recs[EPW];
     __shared__ TK sLM[MAX_LDS_MATS][2];  // lambda, mu of each LEI material
+    __shared__ double sLC[MAX_LDS_MATS][LeiCoef<D>::N];  // LEI stress coefficients
+    __shared__ CohMat sCM[MAX_LDS_MATS];                  // cohesive law parameters
 
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const int li_ = xcd_order(blockIdx.x, gridDim.x) * EPW + g;
     const bool live = li_ < n_list;
     const int64_t pos = int64_t(cbase) + (live ? li_ : 0);  // class position
-    if (threadIdx.x < MAX_LDS_MATS) {
-        const int m = threadIdx.x;
-        sLM[m][0] = m < p.n_mat ? TK(p.mats[m].D[1]) : TK(0);
-        sLM[m][1] = m < p.n_mat ? TK(p.mats[m].D[NS * NS - 1]) : TK(0);
+    // every set-up load goes out before the first wait: the materials' lambda,
+    // mu and stress coefficients, the record and the local node list (both
+    // indexed by the class position); the node data follows the node list.
+    // Loads use clamped indices so that none sits behind a branch.
+    const int mm = threadIdx.x >> 2, mj = threadIdx.x & 3;  // 64 lanes: 16 x 4 >= 8 materials x 3
+    const double mv0 = p.mats[mm < p.n_mat ? mm : p.n_mat - 1].D[mj < 3 ? LeiCoef<D>::index(mj) : 0];
+    // cohesive parameters: lane t < 48 loads field t % 6 (sigma_c .. alpha,
+    // contiguous) of material t / 6, and every lane the type of material t % 8
+    const int cm = threadIdx.x < 48 ? threadIdx.x / 6 : 0, cf = threadIdx.x < 48 ? threadIdx.x % 6 : 0;
+    const int ct8 = threadIdx.x & 7;
+    const double cv0 = (&p.mats[cm < p.n_mat ? cm : p.n_mat - 1].sigma_c)[cf];
+    const int32_t cty0 = p.mats[ct8 < p.n_mat ? ct8 : p.n_mat - 1].type;
+    constexpr int RW = int(sizeof(pg_cut_rec) / 4), RWL = (RW + GS - 1) / GS;
+    static_assert(MAXN <= GS, "one local node per lane");
+    int32_t rw[RWL];
+#pragma unroll
+    for (int k = 0; k < RWL; ++k) {
+        const int kk = gl + k * GS;
+        rw[k] = reinterpret_cast<const int32_t*>(&p.ccut[pos])[kk < RW ? kk : RW - 1];
     }
-    // the record and the local node list are both indexed by the position:
-    // their loads go out together, and the node data follows the node list
-    constexpr int RW = int(sizeof(pg_cut_rec) / 4);
-    for (int k = gl; k < RW; k += GS)
-        if (live) reinterpret_cast<int32_t*>(&recs[g])[k] = reinterpret_cast<const int32_t*>(&p.ccut[pos])[k];
-    for (int l = gl; l < MAXN; l += GS) {
-        qmask[g][l] = 0u;
-        const int nodeid = live ? p.cnodes[pos * CUT_NODES + l] : -1;
-        if (nodeid < 0) continue;
-        for (int i = 0; i < D; ++i) {
-            xs[g][l][i] = p.coords[int64_t(nodeid) * D + i];
-            us[g][l * D + i] = Un[int64_t(nodeid) * D + i];
-        }
+    const int nodeid = p.cnodes[pos * CUT_NODES + (gl < MAXN ? gl : MAXN - 1)];
+    const int nid = nodeid < 0 ? 0 : nodeid;
+    double nx[D], nu[D];
+    for (int i = 0; i < D; ++i) {
+        nx[i] = p.coords[int64_t(nid) * D + i];
+        nu[i] = Un[int64_t(nid) * D + i];
+    }
+    {
+        const double v = mm < p.n_mat ? mv0 : 0.0;
+        if (mm < MAX_LDS_MATS && mj < 3) sLC[mm][mj] = v;
+        if (mm < MAX_LDS_MATS && mj == 1) sLM[mm][0] = TK(v);  // D[0][1] = lambda
+        if (mm < MAX_LDS_MATS && mj == 2) sLM[mm][1] = TK(v);  // D[last][last] = mu
+        if (threadIdx.x < 48) (&sCM[cm].sigma_c)[cf] = cv0;
+        if (threadIdx.x < MAX_LDS_MATS) sCM[ct8].type = cty0;
+    }
+#pragma unroll
+    for (int k = 0; k < RWL; ++k) {
+        const int kk = gl + k * GS;
+        if (live && kk < RW) reinterpret_cast<int32_t*>(&recs[g])[kk] = rw[k];
+    }
+    if (gl < MAXN) {
+        qmask[g][gl] = 0u;
+        if (live && nodeid >= 0)
+            for (int i = 0; i < D; ++i) {
+                xs[g][gl][i] = nx[i];
+                us[g][gl * D + i] = nu[i];
+            }
     }
     __syncthreads();
     const pg_cut_rec& rec = recs[g];
@@ -470,6 +505,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
     const int ncq = live ? rec.n_coh * cnq : 0;
+    // the committed history of this lane's first cohesive qp goes out before
+    // the volumetric work (any valid index when the element has none)
+    double hist0 = 0.0;
+    if (isv_c && p.n_cqp > 0) hist0 = isv_c[ncq > 0 ? rec.isv_off + (gl < ncq ? gl : ncq - 1) : 0];
     for (int t = gl; t < nvq; t += GS) {
         {
             // ---- volumetric qp ----
@@ -508,7 +547,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                         if (cl[n] == l)
                             for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
                 }
-                for (int a = 0; a < D; ++a) qg[g][t][l][a] = gr[a];
+                for (int a = 0; a < D; ++a) {
+                    if (l < NEN) pg[g][l][a] = gr[a];  // (every qp lane writes the same value)
+                    else qg[g][t][l - NEN][a] = gr[a];
+                }
                 bool nz = false;
                 for (int a = 0; a < D; ++a) nz = nz || gr[a] != 0.0;
                 if (nz) atomicOr(&qmask[g][l], 1u << t);
@@ -525,7 +567,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 qwl[g][t][1] = wt * sLM[rec.child_mat[k]][1];
             }
             double sig[NS];
-            lei_stress<D>(p.mats[rec.child_mat[k]].D, eps, sig);
+            lei_stress_c<D>(sLC[rec.child_mat[k]], eps, sig);
             for (int s = 0; s < NS; ++s) qs[g][t][s] = sig[s];
         }
     }
@@ -543,8 +585,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 }
             const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
             const int iq = rec.isv_off + cq;
-            double hist = isv_c ? isv_c[iq] : 0.0;
-            cohesive_point<D>(p.mats[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
+            double hist = cq == gl ? hist0 : (isv_c ? isv_c[iq] : 0.0);
+            cohesive_point<D>(sCM[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
             if (isv_t) isv_t[iq] = hist;
             for (int i = 0; i < D; ++i) {
                 for (int j = 0; j < D; ++j) cRDR[g][cq][i][j] = RDR[i][j];
@@ -560,7 +602,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     // gradient of local node l at qp q (parent gradient, the child gradient of the
     // child node that is enrichment node l, or 0; src/Mesh.cpp:1596-1614)
     auto grad = [&](int q, int l, double* gr) {
-        for (int a = 0; a < D; ++a) gr[a] = qg[g][q][l][a];
+        for (int a = 0; a < D; ++a) gr[a] = l < NEN ? pg[g][l][a] : qg[g][q][l - NEN][a];
     };
     // local-node-pair blocks: K(li, lj) = V(li, lj) + C(li, lj) with the
     // volumetric part V(li, lj) = sum_q B_li^T (wJ D) B_lj (isotropic closed
