@@ -232,7 +232,10 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                bit 3 set = every node group takes the global-memory path
  *                (the fallback of groups that overflow the LDS tables; slow,
  *                for testing); bit 4 set = the groups of enrichment nodes
- *                take twice the threads per node pair whatever bit 0 says.
+ *                take twice the threads per node pair whatever bit 0 says;
+ *                bits 5-6 = 0: the groups that run after the cut kernels have
+ *                group_nodes nodes too, k = 1..3: they have the k-th size of
+ *                the list above (8, 16, 32 in 2D; 2, 4, 8 in 3D).
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

@@ -1,9 +1,10 @@
 """Autotuner of the class-binned schedules.
 
 Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block, threads
-per node and form of the node-group kernel, then the variant of the small-class
-cut kernel (elements per wavefront, register cap) -- timing each kernel with HIP
-events on the assembly stream, and keeps the fastest.  Every choice gives
+per node and form of the node-group kernel, the size of the groups that run
+after the cut kernels, then the variant of the small-class cut kernel (elements
+per wavefront, register cap) -- timing each kernel with HIP events on the
+assembly stream, and keeps the fastest.  Every choice gives
 bit-identical results.  Results are cached in a JSON file keyed by the problem
 class (dimension, cohesive law, size bucket) and the mode.
 """
@@ -77,6 +78,18 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
                     log(f"[tune] group_nodes={g} group_variant={v}: {table[f'group {g},{v}']:.3f} ms")
         gbest = min((k for k in table if k.startswith("group")), key=table.get)
         g, v = (int(x) for x in gbest.split()[1].split(","))
+        # the groups that wait for the cut kernels may take another size
+        # (group variant bits 5-6: 1 + index of the size)
+        if not v & 4:
+            for k, g2 in enumerate(GROUPS[p.dim]):
+                if g2 == g:
+                    continue
+                v2 = v | ((k + 1) << 5)
+                table[f"group {g},{v2}"] = timed(g, 0, v2, 0)
+                if log:
+                    log(f"[tune] group_nodes={g} group_variant={v2}: {table[f'group {g},{v2}']:.3f} ms")
+            gbest = min((k for k in table if k.startswith("group")), key=table.get)
+            g, v = (int(x) for x in gbest.split()[1].split(","))
         # cut-element kernel variants, timed on the whole assembly
         for c in CUT_VARIANTS:
             table[f"cut {c}"] = timed(g, c, v, 0)

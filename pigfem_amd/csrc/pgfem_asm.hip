@@ -1419,7 +1419,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 // cut element: the record holds the staged row of a directly
                 la = rec[2] & 255;
                 nloc = (rec[2] >> 8) & 255;
-                if (la >= nloc) la = -1;
+                if (la >= nloc) {
+                    // no row of a in this element: its diagonal entries add zero
+                    la = -1;
+                    for (int c = 0; c < D; ++c) s_P[i][c] = 0.0;
+                    for (int k = 0; k < D * D; ++k) s_dblk[i][k] = 0.0;
+                }
                 if (nloc > MAXLOC) {
                     s_bad = 1;  // local list does not fit: this group takes the global path
                 } else {
@@ -1461,11 +1466,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
         if (use_lists && a == b) {
             // diagonal pair: every incidence of a, in ascending element order
+            // (an incidence without a row holds zeros: no test, so that the
+            // LDS reads of later incidences go out early)
             const int t0 = s_iptr[lo], t1 = s_iptr[lo + 1];
+#pragma unroll 4
             for (int t = t0; t < t1; ++t) {
-                const int info = s_info[t];
-                const int la = info & 255;
-                if (la == 255) continue;
                 if (func)
                     for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
                 if (jac)
@@ -1550,7 +1555,8 @@ struct pg_asm {
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
     int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap;
-                           // bit 3: global-memory path; bit 4: two threads per pair for enrichment-node groups
+                           // bit 3: global-memory path; bit 4: two threads per pair for enrichment-node groups;
+                           // bits 5-6: 1 + option index of the group size of the groups after the cut kernels
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
 };
@@ -1708,10 +1714,13 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
     } else {
         const int nall = d.n_nodes + d.n_enr;
-        const int ng = h->group_nodes;
         const bool cut64 = fp32 && (flags & PG_CUT64) != 0;  // cut elements staged in FP64
         int o = 0;
-        while (o < 2 && h->gopt_ng[o] != ng) ++o;
+        while (o < 2 && h->gopt_ng[o] != h->group_nodes) ++o;
+        // the groups that wait for the cut kernels: the same size, or the one
+        // that group variant bits 5-6 name (1 + option index)
+        const int rsel = (h->group_variant >> 5) & 3;
+        const int orr = rsel ? rsel - 1 : o;
         const int nclean = h->nclean[o], nrest = h->ngroups[o] - h->nclean[o];
         const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
         const bool tv0 = (h->group_variant & 1) != 0;
@@ -1738,9 +1747,10 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     } while (0)
         // node groups first .. first + len - 1 of the list of this group size,
         // tv: two threads per node pair
-        auto groups = [&](hipStream_t s, int first, int len, bool tv) {
-            const int base = h->goff[o] + first * h->gopt_ng[o];
-            const int64_t ibase = h->gioff[o] + int64_t(first) * (d.dim == 2 ? 7 : 24) * h->gopt_ng[o];
+        auto groups = [&](int oo, hipStream_t s, int first, int len, bool tv) {
+            const int ng = h->gopt_ng[oo];
+            const int base = h->goff[oo] + first * ng;
+            const int64_t ibase = h->gioff[oo] + int64_t(first) * (d.dim == 2 ? 7 : 24) * ng;
             if (d.dim == 2) {
                 if (!fp32) GROUPS2(double, double, h->Kc64, s, base, len);
                 else if (cut64) GROUPS2(float, double, h->Kc64, s, base, len);
@@ -1761,7 +1771,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(h->side, 0, nclean, tv0);
+            groups(o, h->side, 0, nclean, tv0);
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -1807,10 +1817,10 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         // the groups of enrichment nodes (the tail of the list) may take twice
         // the threads per pair (group variant bit 4): they have few incidences
         // and many pairs
-        const int first = overlap ? nclean : 0, nenr = h->nenrg[o];
-        const int nreg = h->ngroups[o] - first - nenr;
-        groups(st, first, nreg, tv0);
-        groups(st, first + nreg, nenr, tv0 || (h->group_variant & 16) != 0);
+        const int first = overlap ? h->nclean[orr] : 0, nenr = h->nenrg[orr];
+        const int nreg = h->ngroups[orr] - first - nenr;
+        groups(orr, st, first, nreg, tv0);
+        groups(orr, st, first + nreg, nenr, tv0 || (h->group_variant & 16) != 0);
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -1979,31 +1989,46 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             tab[size_t(a) * 4 + 3] = ic | (qc << 16);
         }
         const int opts2[3] = {8, 16, 32}, opts3[3] = {2, 4, 8};
+        // blocks of B consecutive nodes (B = the smallest group size) are the
+        // unit of the partition: blocks of regular nodes none of which touches a
+        // cut element, the other blocks of regular nodes, the blocks of
+        // enrichment nodes, each list in mesh order.  A group of NG nodes takes
+        // NG / B consecutive blocks of one list (the last group of a list is
+        // padded), so every group size covers the same three node sets, and the
+        // groups that wait for the cut kernels may use another size than the
+        // groups that overlap them.
+        const int B = s->dim == 2 ? opts2[0] : opts3[0];
+        std::vector<int64_t> blk[3];
+        for (int64_t a0 = 0; a0 < nall; a0 += B) {
+            bool any = false;
+            for (int64_t a = a0; a < std::min<int64_t>(nall, a0 + B) && !any; ++a) any = dirty[a] != 0;
+            blk[!any ? 0 : (a0 >= s->n_nodes ? 2 : 1)].push_back(a0);
+        }
         std::vector<int32_t> gl;
         for (int o = 0; o < 3; ++o) {
-            const int NGo = s->dim == 2 ? opts2[o] : opts3[o];
-            std::vector<int64_t> clean, rest;
-            for (int64_t a0 = 0; a0 < nall; a0 += NGo) {
-                bool any = false;
-                for (int64_t a = a0; a < std::min<int64_t>(nall, a0 + NGo) && !any; ++a) any = dirty[a] != 0;
-                (any ? rest : clean).push_back(a0);
+            const int NGo = s->dim == 2 ? opts2[o] : opts3[o], per = NGo / B;
+            int64_t ngl[3], ngt = 0;
+            for (int l = 0; l < 3; ++l) {
+                ngl[l] = (int64_t(blk[l].size()) + per - 1) / per;
+                ngt += ngl[l];
             }
-            if (int64_t(gl.size()) / 4 + (int64_t(clean.size()) + int64_t(rest.size())) * NGo >= (int64_t(1) << 31)) {
+            if (int64_t(gl.size()) / 4 + ngt * NGo >= (int64_t(1) << 31)) {
                 g_err = "node-group tables exceed 2^31 entries";
                 return fail(PG_E_UNSUPPORTED);
             }
             h->gopt_ng[o] = NGo;
             h->goff[o] = int(gl.size() / 4);
-            h->ngroups[o] = int(clean.size() + rest.size());
-            h->nclean[o] = int(clean.size());
-            h->nenrg[o] = 0;
-            for (const int64_t a0 : rest) h->nenrg[o] += a0 >= s->n_nodes ? 1 : 0;
-            for (const auto* lst : {&clean, &rest})
-                for (const int64_t a0 : *lst)
-                    for (int64_t a = a0; a < a0 + NGo; ++a) {
+            h->ngroups[o] = int(ngt);
+            h->nclean[o] = int(ngl[0]);
+            h->nenrg[o] = int(ngl[2]);
+            for (int l = 0; l < 3; ++l)
+                for (int64_t k = 0; k < ngl[l] * per; ++k) {
+                    const int64_t a0 = k < int64_t(blk[l].size()) ? blk[l][size_t(k)] : nall;  // nall: padding
+                    for (int64_t a = a0; a < a0 + B; ++a) {
                         if (a < nall) gl.insert(gl.end(), &tab[size_t(a) * 4], &tab[size_t(a) * 4] + 4);
                         else gl.insert(gl.end(), {0, 0, 0, 0});
                     }
+                }
         }
         UP(gtab, gl.data(), gl.size(), int32_t);
         {
@@ -2188,7 +2213,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 31) { g_err = "group variant must be 0..31"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 127) { g_err = "group variant must be 0..127"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;
