@@ -252,9 +252,16 @@ def main():
     # HBM traffic of the same kernel instantiation from the committed rocprofv3
     # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
     ng, tv, lists = tuned["mixed"][0], tuned["mixed"][2] & 1, prob.dim == 3 or bool(tuned["mixed"][2] & 2)
-    per = (8 if prob.dim == 2 else 16) * (2 if tv else 1) * ng
-    bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
-    kname = f"k_pair_{'lists' if lists else 'group'}<{prob.dim}, float, {ng}, {tv}, {bs}>"
+
+    def kernel_name(g):
+        per = (8 if prob.dim == 2 else 16) * (2 if tv else 1) * g
+        bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
+        return f"k_pair_{'lists' if lists else 'group'}<{prob.dim}, float, {g}, {tv}, {bs}>"
+
+    # group variant bits 5-6: the groups after the cut kernels may have another size
+    rsel = (tuned["mixed"][2] >> 5) & 3
+    ng_rest = ((8, 16, 32) if prob.dim == 2 else (2, 4, 8))[rsel - 1] if rsel else ng
+    kname = kernel_name(ng) if ng_rest == ng else f"{kernel_name(ng)} + {kernel_name(ng_rest)}"
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):
