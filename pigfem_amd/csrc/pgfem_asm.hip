@@ -115,6 +115,7 @@ constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
 constexpr int CUT_MAXC = PG_MAX_COH * 4; // cohesive qps per cut element
 constexpr int MAX_LDS_MATS = 8;          // materials staged in LDS
+typedef double dpair __attribute__((ext_vector_type(2), aligned(8)));  // two adjacent CSR values
 constexpr int64_t KC_PAD = 8 * 9;        // staged cut tangents: padding past the last row
 
 // Local node at position n of an uncut element listed from its local node la:
@@ -1282,10 +1283,17 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
-            K[pm.map.x] = acc[0][0];
-            K[pm.map.y] = acc[0][1];
-            K[pm.map.z] = acc[1][0];
-            K[pm.map.w] = acc[1][1];
+            if (pm.map.y == pm.map.x + 1 && pm.map.w == pm.map.z + 1) {
+                // both columns of b in one block, side by side: one 16-byte
+                // store per row (8-byte aligned)
+                *reinterpret_cast<dpair*>(K + pm.map.x) = dpair{acc[0][0], acc[0][1]};
+                *reinterpret_cast<dpair*>(K + pm.map.z) = dpair{acc[1][0], acc[1][1]};
+            } else {
+                K[pm.map.x] = acc[0][0];
+                K[pm.map.y] = acc[0][1];
+                K[pm.map.z] = acc[1][0];
+                K[pm.map.w] = acc[1][1];
+            }
         }
         if (diag)
             for (int i = 0; i < D; ++i) P[pm.dof[i]] = pa[i];
@@ -1504,12 +1512,20 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                     a, b, jac, diag, acc, pa);
         }
         if (jac) {
+            // 2D: a row's two columns of b side by side in one block go out as
+            // one 16-byte store (8-byte aligned)
             if constexpr (D == 2) {
-                K[pm.map.x] = acc[0][0];
-                K[pm.map.y] = acc[0][1];
-                K[pm.map.z] = acc[1][0];
-                K[pm.map.w] = acc[1][1];
+                if (pm.map.y == pm.map.x + 1 && pm.map.w == pm.map.z + 1) {
+                    *reinterpret_cast<dpair*>(K + pm.map.x) = dpair{acc[0][0], acc[0][1]};
+                    *reinterpret_cast<dpair*>(K + pm.map.z) = dpair{acc[1][0], acc[1][1]};
+                } else {
+                    K[pm.map.x] = acc[0][0];
+                    K[pm.map.y] = acc[0][1];
+                    K[pm.map.z] = acc[1][0];
+                    K[pm.map.w] = acc[1][1];
+                }
             } else {
+                // (merging a 3D row's stores, 16 + 8 bytes, was 2 % slower on C4)
                 const int32_t* mp = p.npat_map + int64_t(q) * D * D;
                 for (int i = 0; i < D; ++i)
                     for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
