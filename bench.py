@@ -266,8 +266,10 @@ def main():
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):
         for line in open(tf):
-            if line.startswith("traffic_bytes_per_launch:") and kname in line:
-                traffic = int(line.rsplit("=", 1)[1])
+            if line.startswith("traffic_bytes_per_launch:"):
+                name, val = line.split(":", 1)[1].rsplit("=", 1)
+                if name.strip() == kname:
+                    traffic = int(val.split()[0])
     out = {
         "metric": "K+R assembled elements/s, % HBM peak; FP32+batched vs FP64 at <=1e-6 rel err",
         "value": results["mixed"]["eps"],
