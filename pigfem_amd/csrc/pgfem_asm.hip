@@ -763,6 +763,13 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
 // pair (a, b), sums it in the reference's element order, in FP64, and writes
 // the block once through the scatter indices.  A group whose incidences or
 // pairs overflow the LDS tables takes the same arithmetic from global memory.
+//
+// The kernels are bound by the latency of their dependent loads, so each phase
+// issues all of its loads before its first wait: the set-up (node table,
+// incidence records, slot words, materials), then the pair metadata with the
+// element data (coordinates, U and stress coefficients of an uncut element,
+// listed owner first so that no lane selects its row at run time; the whole
+// staged row of a cut element), then the stores.
 // --------------------------------------------------------------------------
 template <int D> struct GroupCfg;
 // per-group LDS capacity (incidences, pairs) for NG nodes: 7 incidences and 16
