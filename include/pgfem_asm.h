@@ -249,6 +249,19 @@ int64_t pg_asm_algorithmic_bytes(const pg_asm* h);
  * has been called. */
 int64_t pg_asm_device_bytes(const pg_asm* h);
 
+/* Cohesive openings of a 2D cohesive mesh from device data: the data of
+ * WriterCohesiveOpenings::writeFromStream (src/WriterCohesiveOpenings.cpp:73-147)
+ * before its sort.  For cohesive edge k of cut record c, at row isv_off / 2 + k
+ * of out (3 doubles): the angle of the edge midpoint (its lower nodes) about
+ * centers[c] (dim doubles: the centre of the inclusion that cuts the element),
+ * then the opening at the midpoint in the edge frame, tangential and normal,
+ * each divided by the law's delta_c.  U: global-dof order.  All pointers are
+ * device pointers; asynchronous on stream (0: the handle's stream).
+ * Ortiz-Pandolfi laws only, as in the reference (PG_E_UNSUPPORTED otherwise);
+ * n_out = rows of out = n_cqp / 2. */
+int pg_asm_cohesive_openings(pg_asm* h, const double* U, const double* centers, double* out, int64_t n_out,
+                             void* stream);
+
 /* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the
  * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own

@@ -118,3 +118,39 @@ def elem_cache(orc, e):
     if n < 0:
         raise IndexError(e)
     return np.ctypeslib.as_array(ptr, shape=(int(n),)).copy()
+
+
+def cohesive_openings(problem, U):
+    """Cohesive openings of a 2D cohesive mesh, restated from
+    WriterCohesiveOpenings::writeFromStream (src/WriterCohesiveOpenings.cpp:73-147):
+    for cohesive edge k of cut record c, row isv_off / 2 + k holds the angle of the
+    midpoint of its lower nodes about the centre of its inclusion, then the opening
+    R . sum_n N_n (U_upper - U_lower) at N = (1/2, 1/2) (cohesiveDeltaFast,
+    src/Utilities.cpp:528-545) in the edge frame of CohEdge2
+    (src/CohesiveElem.cpp:145-238: t = dx/dr over |t|, n = (-t_y, t_x) over |n|),
+    tangential then normal, each over delta_c of the Ortiz-Pandolfi law."""
+    det = problem.node_detect
+    Un = np.asarray(U, np.float64)[problem.node_dof]
+    rows = np.zeros((problem.n_cqp // 2, 3))
+    inc = np.zeros(problem.n_cqp // 2, np.int64)
+    for c in range(problem.n_cut):
+        r = problem.cut_record(c)
+        e = int(problem.cut_elem[c])
+        gl = [int(x) for x in problem.conn[e]] + [int(r.enr[k]) for k in range(r.n_enr)]
+        ic = int(det[problem.conn[e]].max())
+        for k in range(r.n_coh):
+            dc = problem.desc.mats[r.coh_mat[k]].delta_c
+            lo = [gl[r.coh_nodes[k][n]] for n in range(2)]
+            up = [gl[r.coh_nodes[k][n + 2]] for n in range(2)]
+            x0, x1 = problem.coords[lo[0]], problem.coords[lo[1]]
+            t = 0.5 * (x1 - x0)
+            nrm = np.array([-t[1], t[0]])
+            R = np.array([t / np.linalg.norm(t), nrm / np.linalg.norm(nrm)])
+            xm = 0.5 * x0 + 0.5 * x1
+            dl = 0.5 * (Un[up[0]] - Un[lo[0]]) + 0.5 * (Un[up[1]] - Un[lo[1]])
+            d = R @ dl
+            cen = problem.incl_centers[ic]
+            row = r.isv_off // 2 + k
+            rows[row] = (np.arctan2(xm[1] - cen[1], xm[0] - cen[0]), d[0] / dc, d[1] / dc)
+            inc[row] = ic
+    return inc, rows

@@ -144,6 +144,32 @@ class AssemblerGPU:
                                           self.flags(func, jac), dp(K), dp(P), st)
         check(rc, "pg_asm_assemble_device")
 
+    def cohesive_openings(self, U) -> tuple:
+        """Cohesive openings of a 2D cohesive mesh (WriterCohesiveOpenings,
+        src/WriterCohesiveOpenings.cpp:73-147) from the GPU: U is a host array in
+        global-dof order.  Returns (inclusion, rows): per cohesive edge, the index
+        of its inclusion and (angle about its centre, tangential opening / delta_c,
+        normal opening / delta_c), unsorted."""
+        from .device import DeviceBuffer, synchronize
+        p = self.problem
+        n_out = p.n_cqp // 2
+        incl = cut_inclusions(p)
+        centers = np.ascontiguousarray(p.incl_centers[np.maximum(incl, 0)], dtype=np.float64)
+        rows = np.zeros((n_out, 3))
+        if n_out == 0:
+            return np.zeros(0, np.int64), rows
+        dU, dC, dO = DeviceBuffer.from_numpy(np.asarray(U, np.float64)), DeviceBuffer.from_numpy(centers), \
+            DeviceBuffer.zeros(n_out * 3)
+        check(lib().pg_asm_cohesive_openings(self._h, dU.ptr, dC.ptr, dO.ptr, n_out, None), "pg_asm_cohesive_openings")
+        synchronize()
+        rows = dO.download(n_out * 3).reshape(n_out, 3)
+        inc = np.zeros(n_out, np.int64)
+        for c in range(p.n_cut):
+            r = p.cut_record(c)
+            for k in range(r.n_coh):
+                inc[r.isv_off // 2 + k] = incl[c]
+        return inc, rows
+
     def set_timing(self, on: bool) -> None:
         check(lib().pg_asm_set_timing(self._h, 1 if on else 0), "pg_asm_set_timing")
 
@@ -175,3 +201,21 @@ class AssemblerGPU:
             self.close()
         except Exception:
             pass
+
+
+def cut_inclusions(problem) -> np.ndarray:
+    """Inclusion that cuts each cut record: the detection value of its parent
+    nodes inside an inclusion (single-inclusion cuts; -1 if none is inside)."""
+    det = problem.node_detect
+    parents = problem.conn[problem.cut_elem]
+    return det[parents].max(axis=1).astype(np.int64)
+
+
+def write_cohesive_openings(filename: str, inclusion, rows) -> None:
+    """WriterCohesiveOpenings::write (src/WriterCohesiveOpenings.cpp:44-59,
+    182-197): the rows sorted by (inclusion, angle, openings), one line
+    "inclusion,angle,tangential,normal" each, 16 significant digits."""
+    data = sorted((int(i), float(r[0]), float(r[1]), float(r[2])) for i, r in zip(inclusion, rows))
+    with open(filename, "w") as f:
+        for i, th, dt, dn in data:
+            f.write(f"{i},{th:.16g},{dt:.16g},{dn:.16g}\n")

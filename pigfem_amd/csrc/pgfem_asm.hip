@@ -714,6 +714,56 @@ __device__ inline int find_local(const DevProb& p, int e, int ci, int node, int 
 // (set_matrix, src/Assembler.cpp:659-679; no atomics).  Diagonal pairs also sum
 // P (set_vec, src/Assembler.cpp:681-701).
 // --------------------------------------------------------------------------
+// Cohesive openings of a 2D cohesive mesh (WriterCohesiveOpenings::
+// writeFromStream, src/WriterCohesiveOpenings.cpp:73-147): one thread per cut
+// record; for each cohesive edge, N = (1/2, 1/2) at its midpoint, the angle of
+// the midpoint of its lower nodes about the inclusion centre, and the opening
+// R * sum_n N_n (U_upper - U_lower) (cohesiveDeltaFast, src/Utilities.cpp:528-545,
+// with the edge's rotation, constant along a straight CohEdge2) divided by
+// delta_c.  err bit 2: a law other than Ortiz-Pandolfi.
+__global__ void k_coh_openings(DevProb p, const double* __restrict__ Un, const double* __restrict__ centers,
+                               double* __restrict__ out, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.n_cut) return;
+    const pg_cut_rec& rec = p.cut[c];
+    if (rec.n_coh == 0 || rec.coh_nn != 4) return;
+    const int e = p.cut_elem[c];
+    auto gnode = [&](int l) { return l < 3 ? p.conn[int64_t(e) * 3 + l] : rec.enr[l - 3]; };
+    for (int k = 0; k < rec.n_coh; ++k) {
+        const pg_material& m = p.mats[rec.coh_mat[k]];
+        if (m.type != PG_MAT_OP && m.type != PG_MAT_OP_NU) { atomicOr(err, 4); return; }
+        const double N[3] = {0.5, 0.5, 0.0};
+        double lo[3][3], ulo[6], uup[6];
+        for (int n = 0; n < 2; ++n) {
+            const int a = gnode(rec.coh_nodes[k][n]), b = gnode(rec.coh_nodes[k][n + 2]);
+            for (int i = 0; i < 3; ++i) lo[n][i] = i < 2 ? p.coords[int64_t(a) * 2 + i] : 0.0;
+            for (int i = 0; i < 2; ++i) {
+                ulo[n * 2 + i] = Un[int64_t(a) * 2 + i];
+                uup[n * 2 + i] = Un[int64_t(b) * 2 + i];
+            }
+        }
+        double Nq[3], R[2][2], w;
+        cohesive_qp<2>(4, 0, lo, Nq, R, w);
+        double xm[2] = {0.0, 0.0}, dl[2], del[2];
+        for (int d = 0; d < 2; ++d)
+            for (int n = 0; n < 2; ++n) xm[d] += N[n] * lo[n][d];
+        for (int i = 0; i < 2; ++i) {
+            double s = 0;
+            for (int n = 0; n < 2; ++n) s += N[n] * (uup[n * 2 + i] - ulo[n * 2 + i]);
+            dl[i] = s;
+        }
+        for (int i = 0; i < 2; ++i) {
+            double s = 0;
+            for (int j = 0; j < 2; ++j) s += R[i][j] * dl[j];
+            del[i] = s;
+        }
+        double* o = out + (int64_t(rec.isv_off) / 2 + k) * 3;
+        o[0] = atan2(xm[1] - centers[int64_t(c) * 2 + 1], xm[0] - centers[int64_t(c) * 2]);
+        o[1] = del[0] / m.delta_c;
+        o[2] = del[1] / m.delta_c;
+    }
+}
+
 template <int D>
 __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
                               double* __restrict__ P, unsigned flags) {
@@ -2227,6 +2277,26 @@ int64_t pg_asm_device_bytes(const pg_asm* h) { return h ? h->dev_bytes : -1; }
 int pg_asm_set_timing(pg_asm* h, int enable) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
     h->timing = enable != 0;
+    return PG_OK;
+}
+
+int pg_asm_cohesive_openings(pg_asm* h, const double* dU, const double* centers, double* out, int64_t n_out,
+                             void* stream) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (h->d.dim != 2) { g_err = "cohesive openings are defined for 2D only"; return PG_E_UNSUPPORTED; }
+    if (n_out != int64_t(h->d.n_cqp) / 2) { g_err = "n_out must be n_cqp / 2"; return PG_E_ARG; }
+    if (n_out == 0) return PG_OK;
+    if (!dU || !centers || !out) { g_err = "missing device buffer"; return PG_E_ARG; }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const int64_t nu = int64_t(h->d.n_nodes + h->d.n_enr) * h->d.dim;
+    HIP_OK(hipMemsetAsync(h->err, 0, sizeof(int), st));
+    k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, h->d.node_dof, dU, h->Un);
+    k_coh_openings<<<dim3(unsigned((h->d.n_cut + 255) / 256)), dim3(256), 0, st>>>(h->d, h->Un, centers, out, h->err);
+    HIP_OK(hipGetLastError());
+    int herr = 0;
+    HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (herr & 4) { g_err = "cohesive openings need Ortiz-Pandolfi laws"; return PG_E_UNSUPPORTED; }
     return PG_OK;
 }
 
