@@ -262,6 +262,17 @@ int64_t pg_asm_device_bytes(const pg_asm* h);
 int pg_asm_cohesive_openings(pg_asm* h, const double* U, const double* centers, double* out, int64_t n_out,
                              void* stream);
 
+/* Cohesive failure of a 2D cohesive mesh from device data: the per-edge data of
+ * WriterCohesiveFailureLengths (store_mesh and writeFromStream,
+ * src/WriterCohesiveFailureLengths.cpp:64-86,183-224).  For cohesive edge k of
+ * cut record c, at row isv_off / 2 + k of out (2 doubles): the distance between
+ * its two lower nodes, then the mean over its Gauss points of the law's
+ * normalised failure (delta_eff / delta_c for PG_MAT_OP_NU, delta_eff / delta_c1
+ * for PG_MAT_BL_NU, 0 for PG_MAT_OP and PG_MAT_BL, as the reference's
+ * getNormalizedCohesiveFailure).  U: global-dof order.  Device pointers;
+ * asynchronous on stream (0: the handle's stream).  n_out = n_cqp / 2. */
+int pg_asm_cohesive_failure(pg_asm* h, const double* U, double* out, int64_t n_out, void* stream);
+
 /* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the
  * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own

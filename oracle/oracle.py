@@ -154,3 +154,41 @@ def cohesive_openings(problem, U):
             rows[row] = (np.arctan2(xm[1] - cen[1], xm[0] - cen[0]), d[0] / dc, d[1] / dc)
             inc[row] = ic
     return inc, rows
+
+
+def cohesive_failure(problem, U):
+    """Per-edge data of WriterCohesiveFailureLengths, restated from store_mesh and
+    writeFromStream (src/WriterCohesiveFailureLengths.cpp:64-86,183-224): for
+    cohesive edge k of cut record c, row isv_off / 2 + k holds the distance between
+    its two lower nodes and the mean over the two Gauss points (xi = -+1/sqrt(3),
+    src/CohEdge2.cpp:70-78) of getNormalizedCohesiveFailure of the opening in the
+    edge frame: delta_eff / delta_c for OPCohesiveNU
+    (src/Material_OPCohesiveNU.cpp:43-60,217-221), delta_eff / delta_c1 for
+    BilinearCohesiveNU (src/Material_BilinearCohesiveNU.cpp:48-65,250-254), 0 for
+    the other laws (include/material.h:123); delta_eff = sqrt(d_n^2 + beta2 d_t^2)
+    after an opening of squared norm below 1e-40 is replaced by 1e-20 in each
+    component."""
+    Un = np.asarray(U, np.float64)[problem.node_dof]
+    rows = np.zeros((problem.n_cqp // 2, 2))
+    g = 1.0 / np.sqrt(3.0)
+    for c in range(problem.n_cut):
+        r = problem.cut_record(c)
+        e = int(problem.cut_elem[c])
+        gl = [int(x) for x in problem.conn[e]] + [int(r.enr[k]) for k in range(r.n_enr)]
+        for k in range(r.n_coh):
+            m = problem.desc.mats[r.coh_mat[k]]
+            lo = [gl[r.coh_nodes[k][n]] for n in range(2)]
+            up = [gl[r.coh_nodes[k][n + 2]] for n in range(2)]
+            x0, x1 = problem.coords[lo[0]], problem.coords[lo[1]]
+            t = 0.5 * (x1 - x0)
+            R = np.array([t / np.linalg.norm(t), np.array([-t[1], t[0]]) / np.linalg.norm(t)])
+            f = 0.0
+            for xi in (-g, g):
+                N = (0.5 * (1 - xi), 0.5 * (1 + xi))
+                d = R @ (N[0] * (Un[up[0]] - Un[lo[0]]) + N[1] * (Un[up[1]] - Un[lo[1]]))
+                if d @ d < 1e-40:
+                    d = np.full(2, 1e-20)
+                de = np.sqrt(d[1] ** 2 + m.beta2 * d[0] ** 2)
+                f += de / m.delta_c if m.type == 4 else de / m.delta_c1 if m.type == 2 else 0.0
+            rows[r.isv_off // 2 + k] = (np.linalg.norm(x1 - x0), f / 2)
+    return rows

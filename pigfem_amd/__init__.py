@@ -6,9 +6,10 @@ description, and thin Python mirrors of the reference's Assembler API.
 """
 from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PgError, lib  # noqa: F401
 from .problem import RVEConfig, RVEProblem, batch_config, config, small_config  # noqa: F401
-from .assembler import AssemblerGPU, PMatrix, PVector, cut_inclusions, write_cohesive_openings  # noqa: F401
+from .assembler import AssemblerGPU, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
+    write_cohesive_failure_lengths  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
 
 __all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU", "PMatrix", "PVector",
-           "cut_inclusions", "write_cohesive_openings",
+           "cut_inclusions", "write_cohesive_openings", "write_cohesive_failure_lengths",
            "DeviceBuffer", "synchronize", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

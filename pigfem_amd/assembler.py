@@ -170,6 +170,21 @@ class AssemblerGPU:
                 inc[r.isv_off // 2 + k] = incl[c]
         return inc, rows
 
+    def cohesive_failure(self, U) -> np.ndarray:
+        """Per-edge data of WriterCohesiveFailureLengths
+        (src/WriterCohesiveFailureLengths.cpp:64-86,183-224) from the GPU for a 2D
+        cohesive mesh: U is a host array in global-dof order.  Returns rows of
+        (edge length, mean normalised failure over the edge's Gauss points), in
+        cut-element order."""
+        from .device import DeviceBuffer, synchronize
+        n_out = self.problem.n_cqp // 2
+        if n_out == 0:
+            return np.zeros((0, 2))
+        dU, dO = DeviceBuffer.from_numpy(np.asarray(U, np.float64)), DeviceBuffer.zeros(n_out * 2)
+        check(lib().pg_asm_cohesive_failure(self._h, dU.ptr, dO.ptr, n_out, None), "pg_asm_cohesive_failure")
+        synchronize()
+        return dO.download(n_out * 2).reshape(n_out, 2)
+
     def set_timing(self, on: bool) -> None:
         check(lib().pg_asm_set_timing(self._h, 1 if on else 0), "pg_asm_set_timing")
 
@@ -219,3 +234,34 @@ def write_cohesive_openings(filename: str, inclusion, rows) -> None:
     with open(filename, "w") as f:
         for i, th, dt, dn in data:
             f.write(f"{i},{th:.16g},{dt:.16g},{dn:.16g}\n")
+
+
+def write_cohesive_failure_lengths(filename: str, rows, step: int, individual: bool = False) -> None:
+    """WriterCohesiveFailureLengths::writeConsecutive
+    (src/WriterCohesiveFailureLengths.cpp:93-130,137-258) for one rank: step 0
+    creates the file and first writes the edge lengths (each followed by a comma
+    when individual, else their total); every step then appends either each
+    edge's failure followed by a comma (individual) or, for j = 1..5, the total
+    length of the edges whose failure exceeds j.  Values print as %.16g."""
+    rows = np.asarray(rows, np.float64).reshape(-1, 2)
+    g = lambda x: f"{float(x):.16g}"
+    out = []
+    if step == 0:
+        if individual:
+            out.append("".join(g(v) + "," for v in rows[:, 0]) + "\n")
+        else:
+            tot = 0.0
+            for v in rows[:, 0]:
+                tot += v
+            out.append(g(tot) + "\n")
+    if individual:
+        out.append("".join(g(v) + "," for v in rows[:, 1]) + "\n")
+    else:
+        failed = [0.0] * 5
+        for L, f in rows:
+            for j in range(5):
+                if f > j + 1:
+                    failed[j] += L
+        out.append(",".join(g(v) for v in failed) + "\n")
+    with open(filename, "w" if step == 0 else "a") as fh:
+        fh.write("".join(out))

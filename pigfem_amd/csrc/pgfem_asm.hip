@@ -764,6 +764,59 @@ __global__ void k_coh_openings(DevProb p, const double* __restrict__ Un, const d
     }
 }
 
+// Cohesive failure lengths of a 2D cohesive mesh (WriterCohesiveFailureLengths,
+// src/WriterCohesiveFailureLengths.cpp:64-86 and 183-224): one thread per cut
+// record; for each cohesive edge, the distance between its two lower nodes and
+// the mean over its two Gauss points of the law's normalised failure
+// (getNormalizedCohesiveFailure): delta_eff / delta_c for OPCohesiveNU
+// (src/Material_OPCohesiveNU.cpp:43-60,217-221), delta_eff / delta_c1 for
+// BilinearCohesiveNU (src/Material_BilinearCohesiveNU.cpp:48-65,250-254), and 0
+// for the laws that keep the base definition (include/material.h:123).  An
+// opening of squared norm below 1e-40 counts as 1e-20 in each component.
+__global__ void k_coh_failure(DevProb p, const double* __restrict__ Un, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.n_cut) return;
+    const pg_cut_rec& rec = p.cut[c];
+    if (rec.n_coh == 0 || rec.coh_nn != 4) return;
+    const int e = p.cut_elem[c];
+    auto gnode = [&](int l) { return l < 3 ? p.conn[int64_t(e) * 3 + l] : rec.enr[l - 3]; };
+    for (int k = 0; k < rec.n_coh; ++k) {
+        const pg_material& m = p.mats[rec.coh_mat[k]];
+        double lo[3][3], ulo[6], uup[6];
+        for (int n = 0; n < 2; ++n) {
+            const int a = gnode(rec.coh_nodes[k][n]), b = gnode(rec.coh_nodes[k][n + 2]);
+            for (int i = 0; i < 3; ++i) lo[n][i] = i < 2 ? p.coords[int64_t(a) * 2 + i] : 0.0;
+            for (int i = 0; i < 2; ++i) {
+                ulo[n * 2 + i] = Un[int64_t(a) * 2 + i];
+                uup[n * 2 + i] = Un[int64_t(b) * 2 + i];
+            }
+        }
+        double fsum = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            double Nq[3], R[2][2], w;
+            cohesive_qp<2>(4, q, lo, Nq, R, w);
+            double dl[2], del[2];
+            for (int i = 0; i < 2; ++i) {
+                double s = 0;
+                for (int n = 0; n < 2; ++n) s += Nq[n] * (uup[n * 2 + i] - ulo[n * 2 + i]);
+                dl[i] = s;
+            }
+            for (int i = 0; i < 2; ++i) {
+                double s = 0;
+                for (int j = 0; j < 2; ++j) s += R[i][j] * dl[j];
+                del[i] = s;
+            }
+            if (del[0] * del[0] + del[1] * del[1] < 1e-40) del[0] = del[1] = 1e-20;
+            const double de = sqrt(del[1] * del[1] + m.beta2 * (del[0] * del[0]));
+            fsum += m.type == PG_MAT_OP_NU ? de / m.delta_c : m.type == PG_MAT_BL_NU ? de / m.delta_c1 : 0.0;
+        }
+        const double dx = lo[1][0] - lo[0][0], dy = lo[1][1] - lo[0][1];
+        double* o = out + (int64_t(rec.isv_off) / 2 + k) * 2;
+        o[0] = sqrt(dx * dx + dy * dy);
+        o[1] = fsum / 2.0;
+    }
+}
+
 template <int D>
 __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
                               double* __restrict__ P, unsigned flags) {
@@ -2297,6 +2350,20 @@ int pg_asm_cohesive_openings(pg_asm* h, const double* dU, const double* centers,
     HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (herr & 4) { g_err = "cohesive openings need Ortiz-Pandolfi laws"; return PG_E_UNSUPPORTED; }
+    return PG_OK;
+}
+
+int pg_asm_cohesive_failure(pg_asm* h, const double* dU, double* out, int64_t n_out, void* stream) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (h->d.dim != 2) { g_err = "cohesive failure lengths are defined for 2D only"; return PG_E_UNSUPPORTED; }
+    if (n_out != int64_t(h->d.n_cqp) / 2) { g_err = "n_out must be n_cqp / 2"; return PG_E_ARG; }
+    if (n_out == 0) return PG_OK;
+    if (!dU || !out) { g_err = "missing device buffer"; return PG_E_ARG; }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const int64_t nu = int64_t(h->d.n_nodes + h->d.n_enr) * h->d.dim;
+    k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, h->d.node_dof, dU, h->Un);
+    k_coh_failure<<<dim3(unsigned((h->d.n_cut + 255) / 256)), dim3(256), 0, st>>>(h->d, h->Un, out);
+    HIP_OK(hipGetLastError());
     return PG_OK;
 }
 
