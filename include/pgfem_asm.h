@@ -273,6 +273,21 @@ int pg_asm_cohesive_openings(pg_asm* h, const double* U, const double* centers, 
  * asynchronous on stream (0: the handle's stream).  n_out = n_cqp / 2. */
 int pg_asm_cohesive_failure(pg_asm* h, const double* U, double* out, int64_t n_out, void* stream);
 
+/* Linear solve of a Newton iteration on the device: Jacobi-preconditioned
+ * conjugate gradients on an n x n CSR matrix (row_ptr n + 1, col_idx and vals
+ * nnz entries, 0-based), replacing the KSP solve of SolverNonlinearKSP
+ * (src/SolverNonlinearKSP.cpp:94, src/Solver.cpp:108-160) for symmetric
+ * positive definite tangents.  x holds the initial guess on entry and the
+ * solution on return.  Stops when |b - A x| <= rtol |b| (the recursive
+ * residual) or after max_iter iterations; *iters and *rel_res report where it
+ * stopped (a caller checks rel_res against rtol).  Device pointers; ordered on
+ * stream (0: the null stream) and synchronous: the host reads |r| once per
+ * iteration.  Allocates 5 n + 1032 doubles of workspace for the call.  Results
+ * are bitwise repeatable.  PG_E_ARG if the iteration breaks down (an
+ * indefinite matrix). */
+int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
+               double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream);
+
 /* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the
  * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own

@@ -817,6 +817,118 @@ __global__ void k_coh_failure(DevProb p, const double* __restrict__ Un, double* 
     }
 }
 
+// --------------------------------------------------------------------------
+// Jacobi-preconditioned conjugate gradients on a CSR matrix (the linear solve
+// of each Newton iteration, SolverNonlinearKSP / Solver::solve,
+// src/SolverNonlinearKSP.cpp:94, src/Solver.cpp:108-160).  One thread per row
+// for the product; dot products reduce over a fixed grid and then one block in
+// a fixed order, so results are bitwise repeatable.  The CG scalars stay on
+// the device in sc[]: sc[0], sc[1] hold r.z of alternate iterations, sc[2]
+// p.Ap, sc[3] r.r, sc[4] a second copy a fused dot does not need.
+constexpr int PCG_T = 256, PCG_B = 512;
+
+__global__ void k_csr_diag_inv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                               const double* __restrict__ v, double* __restrict__ dinv) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) d = v[k];
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;  // a zero diagonal keeps the identity, as PCJACOBI does
+}
+
+__global__ void k_csr_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s = fma(v[k], x[ci[k]], s);
+    y[i] = s;
+}
+
+// part[2 * block + {0, 1}] = partial sums of a.b and c.d
+__global__ void __launch_bounds__(PCG_T) k_dot2(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                                const double* __restrict__ c, const double* __restrict__ d,
+                                                double* __restrict__ part) {
+    __shared__ double l0[PCG_T], l1[PCG_T];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * PCG_T + threadIdx.x; i < n; i += int64_t(gridDim.x) * PCG_T) {
+        s0 = fma(a[i], b[i], s0);
+        s1 = fma(c[i], d[i], s1);
+    }
+    l0[threadIdx.x] = s0;
+    l1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = PCG_T / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            l0[threadIdx.x] += l0[threadIdx.x + w];
+            l1[threadIdx.x] += l1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = l0[0];
+        part[2 * blockIdx.x + 1] = l1[0];
+    }
+}
+
+__global__ void __launch_bounds__(PCG_T) k_dot_fin(const double* __restrict__ part, int nb, double* __restrict__ sc,
+                                                   int o0, int o1) {
+    __shared__ double l0[PCG_T], l1[PCG_T];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += PCG_T) {
+        s0 += part[2 * i];
+        s1 += part[2 * i + 1];
+    }
+    l0[threadIdx.x] = s0;
+    l1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = PCG_T / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            l0[threadIdx.x] += l0[threadIdx.x + w];
+            l1[threadIdx.x] += l1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sc[o0] = l0[0];
+        sc[o1] = l1[0];
+    }
+}
+
+// r = b - q, z = dinv r, p = z
+__global__ void k_pcg_init(int64_t n, const double* __restrict__ b, const double* __restrict__ q,
+                           const double* __restrict__ dinv, double* __restrict__ r, double* __restrict__ z,
+                           double* __restrict__ pv) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ri = b[i] - q[i];
+    r[i] = ri;
+    z[i] = dinv[i] * ri;
+    pv[i] = dinv[i] * ri;
+}
+
+// alpha = sc[irz] / sc[2]; x += alpha p; r -= alpha q; z = dinv r
+__global__ void k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                         const double* __restrict__ pv, const double* __restrict__ q, const double* __restrict__ dinv,
+                         const double* __restrict__ sc, int irz) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = sc[irz] / sc[2];
+    x[i] = fma(alpha, pv[i], x[i]);
+    const double ri = fma(-alpha, q[i], r[i]);
+    r[i] = ri;
+    z[i] = dinv[i] * ri;
+}
+
+// beta = sc[inew] / sc[iold]; p = z + beta p
+__global__ void k_pcg_p(int64_t n, double* __restrict__ pv, const double* __restrict__ z,
+                        const double* __restrict__ sc, int inew, int iold) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pv[i] = fma(sc[inew] / sc[iold], pv[i], z[i]);
+}
+
 template <int D>
 __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
                               double* __restrict__ P, unsigned flags) {
@@ -2462,6 +2574,64 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
     HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (herr) { g_err = "negative child Jacobian in a cut element"; return PG_E_MESH; }
+    return PG_OK;
+}
+
+int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
+               double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream) {
+    if (n < 0 || max_iter < 0 || !(rtol >= 0.0)) { g_err = "bad size, iteration limit or tolerance"; return PG_E_ARG; }
+    if (iters) *iters = 0;
+    if (rel_res) *rel_res = 0.0;
+    if (n == 0) return PG_OK;
+    if (!row_ptr || !col_idx || !vals || !b || !x) { g_err = "missing device buffer"; return PG_E_ARG; }
+    struct Mem {
+        void* p = nullptr;
+        ~Mem() { if (p) (void)hipFree(p); }
+    } mem;
+    const size_t nv = size_t(n), words = 5 * nv + 2 * PCG_B + 8;
+    if (hipMalloc(&mem.p, words * sizeof(double)) != hipSuccess) { g_err = "device allocation failed"; return PG_E_NOMEM; }
+    double* r = static_cast<double*>(mem.p);
+    double *z = r + nv, *pv = z + nv, *q = pv + nv, *dinv = q + nv, *part = dinv + nv, *sc = part + 2 * PCG_B;
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : nullptr;
+    const dim3 g(unsigned((n + 255) / 256)), t(256);
+    const int nb = int(std::min<int64_t>(PCG_B, (n + PCG_T - 1) / PCG_T));
+    auto dot2 = [&](const double* a, const double* bb, const double* c, const double* d, int o0, int o1) {
+        k_dot2<<<dim3(nb), dim3(PCG_T), 0, st>>>(n, a, bb, c, d, part);
+        k_dot_fin<<<dim3(1), dim3(PCG_T), 0, st>>>(part, nb, sc, o0, o1);
+    };
+    auto read = [&](int o, double& v) -> int {
+        HIP_OK(hipMemcpyAsync(&v, sc + o, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        return PG_OK;
+    };
+    k_csr_diag_inv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, dinv);
+    dot2(b, b, b, b, 3, 4);
+    double bb = 0.0, rr = 0.0;
+    if (int e = read(3, bb)) return e;
+    if (bb == 0.0) {
+        HIP_OK(hipMemsetAsync(x, 0, nv * sizeof(double), st));
+        HIP_OK(hipStreamSynchronize(st));
+        return PG_OK;
+    }
+    k_csr_spmv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, x, q);
+    k_pcg_init<<<g, t, 0, st>>>(n, b, q, dinv, r, z, pv);
+    dot2(r, z, r, r, 0, 3);
+    if (int e = read(3, rr)) return e;
+    int it = 0, cur = 0;
+    while (it < max_iter && std::sqrt(rr) > rtol * std::sqrt(bb)) {
+        k_csr_spmv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, pv, q);
+        dot2(pv, q, pv, q, 2, 4);
+        k_pcg_xr<<<g, t, 0, st>>>(n, x, r, z, pv, q, dinv, sc, cur);
+        dot2(r, z, r, r, 1 - cur, 3);
+        k_pcg_p<<<g, t, 0, st>>>(n, pv, z, sc, 1 - cur, cur);
+        HIP_OK(hipGetLastError());
+        cur = 1 - cur;
+        ++it;
+        if (int e = read(3, rr)) return e;
+        if (!std::isfinite(rr)) { g_err = "PCG broke down (matrix not positive definite?)"; return PG_E_ARG; }
+    }
+    if (iters) *iters = it;
+    if (rel_res) *rel_res = std::sqrt(rr / bb);
     return PG_OK;
 }
 

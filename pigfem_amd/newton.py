@@ -114,11 +114,19 @@ class PardisoLU:
 class LinearSolver:
     def __init__(self, problem, prefer="pardiso"):
         self.p = problem
+        self._gpu = None
+        if prefer == "gpu-pcg":
+            from .solve import DeviceCSR
+            self.kind, self._pardiso = "gpu-pcg", None
+            self._gpu = DeviceCSR(problem.rp[0], problem.ci[0])
+            return
         mkl = _load_mkl() if prefer == "pardiso" else None
         self.kind = "pardiso" if mkl is not None else "superlu"
         self._pardiso = PardisoLU(mkl, problem.n_free, problem.rp[0], problem.ci[0]) if mkl is not None else None
 
     def solve(self, K, r):
+        if self._gpu is not None:
+            return self._gpu.solve(K[: self.p.nnz[0]], r, rtol=1e-12)[0]
         if self._pardiso is not None:
             return self._pardiso.solve(K[: self.p.nnz[0]], r)
         return spl.splu(_kff(self.p, K).tocsc()).solve(r)
