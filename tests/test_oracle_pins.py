@@ -11,7 +11,9 @@ transposed operand fails:
   * symmetry, rigid-body modes and P = K U for linear elasticity,
   * closed-form traction curves of the cohesive laws, their central-difference
     tangents, the history-variable rules and the zero-opening tangent,
-  * central-difference check of the assembled tangent of cohesive RVEs.
+  * central-difference check of the assembled tangent of cohesive RVEs,
+  * an IGFEM patch test on single cut elements (enrichment shape gradients),
+  * the cohesive integration (weights x area Jacobian, frame) with a linear law.
 """
 import dataclasses
 
@@ -322,3 +324,209 @@ def test_cohtri3_geometry():
             assert abs(s - area) <= 1e-12 * area
             checked += 1
     assert checked > 0
+
+
+def _voigt(G):
+    """Engineering strain of a displacement gradient G[i][j] = du_i/dx_j in the
+    reference's Voigt order (2D: xx, yy, xy; 3D: xx, yy, zz, yz, xz, xy)."""
+    if G.shape[0] == 2:
+        return np.array([G[0, 0], G[1, 1], G[0, 1] + G[1, 0]])
+    return np.array([G[0, 0], G[1, 1], G[2, 2], G[1, 2] + G[2, 1], G[0, 2] + G[2, 0], G[0, 1] + G[1, 0]])
+
+
+def _bt(g, sig):
+    """B(g)^T sigma for one node with gradient g (the transpose of the Voigt B column pair)."""
+    if len(g) == 2:
+        return np.array([g[0] * sig[0] + g[1] * sig[2], g[1] * sig[1] + g[0] * sig[2]])
+    return np.array([g[0] * sig[0] + g[2] * sig[4] + g[1] * sig[5],
+                     g[1] * sig[1] + g[2] * sig[3] + g[0] * sig[5],
+                     g[2] * sig[2] + g[1] * sig[3] + g[0] * sig[4]])
+
+
+def _simplex_grads(X):
+    """Gradients of the barycentric coordinates of a simplex (rows) and its measure."""
+    d = X.shape[1]
+    M = np.hstack([np.ones((d + 1, 1)), X])
+    G = np.linalg.inv(M)[1:, :].T
+    return G, abs(np.linalg.det(M)) / (2.0 if d == 2 else 6.0)
+
+
+def _polygon_grad_integrals(X):
+    """For a counter-clockwise polygon with vertices X: its area and, for each
+    vertex j, the integral over the polygon of the gradient of any function
+    that is 1 at j, 0 at the other vertices and linear along each edge (a
+    triangle's barycentric coordinate, a quad's bilinear shape): by the
+    divergence theorem, half the outward normal times length of j's two edges."""
+    n = len(X)
+    area = 0.5 * sum(X[k, 0] * X[(k + 1) % n, 1] - X[(k + 1) % n, 0] * X[k, 1] for k in range(n))
+    I = np.zeros((n, 2))
+    for k in range(n):
+        a, b = X[k], X[(k + 1) % n]
+        nl = np.array([b[1] - a[1], -(b[0] - a[0])])  # outward normal x length (CCW)
+        I[k] += 0.5 * nl
+        I[(k + 1) % n] += 0.5 * nl
+    return area, I
+
+
+@pytest.mark.parametrize("kind", ["c0", "tet"])
+def test_enrichment_patch_test_on_cut_elements(kind):
+    """IGFEM patch test on each cut element alone (pins the enrichment shape
+    gradients of src/Mesh.cpp:1596-1614: scale factor 1, child node -> local
+    enrichment node).  A displacement field v that is continuous and linear on
+    every child (2D: kinked along the element's interface chord; 3D: arbitrary
+    values at the parent and enrichment nodes of the child tets) is reproduced
+    exactly by regular dofs v(x_i) and enrichment dofs v(x_j) - (parent
+    interpolation of v at x_j).  Then U^T K_e U = sum_children |c| eps_c^T D_c eps_c
+    and P_e[l] = sum_children B(int_c grad phi_l)^T sigma_c with the child-wise
+    constant strain, in closed form (3D: 6 |c|, the tet weights sum to 1)."""
+    cfg = small_config(kind, n=12 if kind == "c0" else 6, n_incl=2 if kind == "c0" else 1)
+    p = RVEProblem(cfg)
+    dim = p.dim
+    rng = np.random.default_rng(17)
+    scale = 1.0 if dim == 2 else 6.0
+    checked = 0
+    for ci in range(0, p.n_cut, max(1, p.n_cut // 12)):
+        e = int(p.cut_elem[ci])
+        rec = p.cut[ci]
+        local = [int(x) for x in p.conn[e]] + [int(rec.enr[k]) for k in range(rec.n_enr)]
+        X = p.coords[local]
+        nl = len(local)
+        Np, _ = _simplex_grads(X[:dim + 1])  # parent gradients (rows)
+        if dim == 2:
+            # kinked field: v = A x + c max(0, n . (x - x_e0)), chord through the
+            # two enrichment nodes
+            assert rec.n_enr == 2
+            A = rng.uniform(-1e-3, 1e-3, (2, 2))
+            cvec = rng.uniform(-1e-3, 1e-3, 2)
+            t = X[dim + 1] - X[dim + 2]
+            nrm = np.array([-t[1], t[0]]) / np.linalg.norm(t)
+
+            def field(x):
+                return A @ x + cvec * max(0.0, float(nrm @ (x - X[dim + 1])))
+
+            def grad_on(xc):
+                return A + (np.outer(cvec, nrm) if nrm @ (xc - X[dim + 1]) > 0 else 0.0)
+            V = np.array([field(x) for x in X])
+        else:
+            V = rng.uniform(-1e-3, 1e-3, (nl, dim))
+        # dofs: parent nodes take v, enrichment nodes v - parent interpolation
+        M = np.hstack([np.ones((dim + 1, 1)), X[:dim + 1]])
+        Uloc = V.copy()
+        for j in range(dim + 1, nl):
+            lam = np.linalg.solve(M.T, np.concatenate([[1.0], X[j]]))
+            Uloc[j] = V[j] - lam @ V[:dim + 1]
+        U = np.zeros(p.n_dof)
+        for l in range(nl):
+            U[p.node_dof[local[l]]] = Uloc[l]
+        energy = 0.0
+        Pexp = np.zeros((nl, dim))
+        for k in range(rec.n_child):
+            cn = [int(rec.child_nodes[k][n]) for n in range(3 if rec.child_type[k] == _ffi.PG_CHILD_TRI3 else 4)]
+            Dm = np.array(p.desc.mats[rec.child_mat[k]].D[:(3 if dim == 2 else 6) ** 2]).reshape((3, 3) if dim == 2 else (6, 6))
+            Xc = X[cn]
+            if dim == 2:
+                area, I = _polygon_grad_integrals(Xc)
+                assert area > 0
+                G = grad_on(Xc.mean(axis=0))
+            else:
+                Gb, area = _simplex_grads(Xc)
+                I = area * Gb
+                G = V[cn].T @ Gb  # du_i/dx_j of the linear interpolant on the child
+            eps = _voigt(G)
+            sig = Dm @ eps
+            energy += scale * area * eps @ Dm @ eps
+            for m, l in enumerate(cn):
+                if l > dim:
+                    Pexp[l] += scale * _bt(I[m], sig)  # enrichment node of this child
+            for i in range(dim + 1):
+                Pexp[i] += scale * area * _bt(Np[i], sig)  # parent node over the child
+        o = Oracle(p, e, e + 1)
+        K, P, _ = o.assemble(U)
+        Kf = full_matrix(p, K, o.rp, o.ci)
+        assert abs(U @ (Kf @ U) - energy) <= 1e-12 * energy
+        Pl = np.array([P[p.node_dof[local[l]]] for l in range(nl)])
+        assert np.abs(Pl - Pexp).max() <= 1e-12 * np.abs(Pexp).max()
+        checked += 1
+    assert checked >= 10
+
+
+@pytest.mark.parametrize("kind", ["bl", "tet-bl"])
+def test_cohesive_integration_with_a_linear_law(kind):
+    """Cohesive geometry and integration (CohEdge2 src/CohEdge2.cpp:70-78,186-209,
+    CohTri3 src/CohTri3.cpp:13-200, CohesiveElem::ShapeFunctions
+    src/CohesiveElem.cpp:145-238): every cohesive qp's weight times area
+    Jacobian sums to the interface segment's length (triangle's area); and, for
+    a jump v of one side of a cut element alone (every node of the children on
+    that side moves by v, the other side stays), the bulk is rigid on each side
+    and the bilinear law is in its linear branch, so the element's internal
+    force is the cohesive one only: the upper nodes of each interface carry
+    sum = K0 |Gamma| R^T diag(beta^2, .., 1) R v, the lower nodes minus that
+    (K0 = sigma_c / delta_c1; R built here from the interface geometry)."""
+    cfg = small_config(kind, n=12 if kind == "bl" else 6, n_incl=2 if kind == "bl" else 3)
+    cfg = dataclasses.replace(cfg, beta=1.5)
+    p = RVEProblem(cfg)
+    dim = p.dim
+    nst = 3 if dim == 2 else 6
+    checked = 0
+    for ci in range(0, p.n_cut, max(1, p.n_cut // 12)):
+        e = int(p.cut_elem[ci])
+        rec = p.cut[ci]
+        if rec.n_coh != 1:
+            continue
+        local = [int(x) for x in p.conn[e]] + [int(rec.enr[k]) for k in range(rec.n_enr)]
+        X = p.coords[local]
+        cnn = rec.coh_nn // 2
+        lo = [int(rec.coh_nodes[0][n]) for n in range(cnn)]
+        up = [int(rec.coh_nodes[0][cnn + n]) for n in range(cnn)]
+        mat = p.desc.mats[rec.coh_mat[0]]
+        # interface frame from the lower nodes, as the reference builds it
+        if dim == 2:
+            t = X[lo[1]] - X[lo[0]]
+            size = np.linalg.norm(t)
+            R = np.array([t, [-t[1], t[0]]]) / size
+        else:
+            t0, t1 = X[lo[1]] - X[lo[0]], X[lo[2]] - X[lo[0]]
+            nr = np.cross(t0, t1)
+            size = 0.5 * np.linalg.norm(nr)
+            R = np.array([t0 / np.linalg.norm(t0), t1 / np.linalg.norm(t1), nr / np.linalg.norm(nr)])
+        # cached qps: weight x area Jacobian sums to the segment length / area
+        o = Oracle(p, e, e + 1)
+        cache = elem_cache(o, e)
+        off = rec.n_vqp * (2 + nst * dim * len(local))
+        s = 0.0
+        for q in range(2 if dim == 2 else 4):
+            s += cache[off] * cache[off + 1]
+            if dim == 2:
+                assert cache[off] == 1.0 and abs(cache[off + 1] - 0.5 * size) <= 1e-14 * size
+            off += 2 + 3 + dim * dim
+        assert abs(s - size) <= 1e-13 * size
+        # a small jump, opening along the normal (linear branch, no contact)
+        v = 1e-5 * (R[-1] + 0.3 * R[0])
+        upper_side = set()
+        for k in range(rec.n_child):
+            cn = [int(rec.child_nodes[k][n]) for n in range(3 if rec.child_type[k] == _ffi.PG_CHILD_TRI3 else 4)]
+            if any(c in up for c in cn):
+                upper_side.update(cn)
+        assert not upper_side & set(lo)
+        Vn = np.array([v if l in upper_side else np.zeros(dim) for l in range(len(local))])
+        M = np.hstack([np.ones((dim + 1, 1)), X[:dim + 1]])
+        U = np.zeros(p.n_dof)
+        for l in range(len(local)):
+            val = Vn[l]
+            if l > dim:
+                lam = np.linalg.solve(M.T, np.concatenate([[1.0], X[l]]))
+                val = Vn[l] - lam @ Vn[:dim + 1]
+            U[p.node_dof[local[l]]] = val
+        _, P, _ = o.assemble(U, np.zeros(p.n_cqp))
+        K0 = mat.sigma_c / mat.delta_c1
+        W = np.diag([mat.beta2] * (dim - 1) + [1.0])
+        expect = K0 * size * R.T @ W @ R @ v
+        Pu = sum(P[p.node_dof[local[l]]] for l in up)
+        Plo = sum(P[p.node_dof[local[l]]] for l in lo)
+        assert np.abs(Pu - expect).max() <= 1e-9 * np.abs(expect).max()
+        assert np.abs(Plo + expect).max() <= 1e-9 * np.abs(expect).max()
+        # parent nodes carry nothing: the bulk moves rigidly on each side
+        Pp = np.array([P[p.node_dof[local[l]]] for l in range(dim + 1)])
+        assert np.abs(Pp).max() <= 1e-9 * np.abs(expect).max()
+        checked += 1
+    assert checked >= 5
