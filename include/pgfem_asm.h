@@ -196,11 +196,22 @@ int pg_asm_assemble(pg_asm* h, const double* U_all, const double* isv_committed,
  * four blocks back to back, [ff | fp | pf | pp] (nnz_total values, block b at
  * the sum of the nnz of the blocks before it), dP holds [Pf | Pp] (n_dof
  * values).  The launch is asynchronous on `stream` (a hipStream_t, or NULL for
- * the handle's own stream).  No host synchronisation, no allocation. */
+ * the handle's own stream).  No host synchronisation, no allocation.
+ * The negative-child-Jacobian check of the reference (store_shape_functions
+ * aborts, src/Mesh.cpp:1561) lands in a device flag that each assembly clears
+ * first; pg_asm_check_error reports it.
+ * Calls on one handle share its scratch buffers: a call on a stream other
+ * than the previous call's waits for the previous call to complete. */
 int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
                            const double* disv_committed, double* disv_trial,
                            double delta_t, unsigned flags,
                            double* dK, double* dP, void* stream);
+
+/* Status of the handle's last assembly (either API): waits for it to complete
+ * and returns PG_E_MESH if a cut element had a child of negative Jacobian (the
+ * reference aborts there, src/Mesh.cpp:1561), PG_OK otherwise.  The host API
+ * pg_asm_assemble makes this check itself. */
+int pg_asm_check_error(pg_asm* h);
 
 /* Sizes a caller needs for buffers of the calls above. */
 int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total,
@@ -258,7 +269,9 @@ int64_t pg_asm_device_bytes(const pg_asm* h);
  * each divided by the law's delta_c.  U: global-dof order.  All pointers are
  * device pointers; asynchronous on stream (0: the handle's stream).
  * Ortiz-Pandolfi laws only, as in the reference (PG_E_UNSUPPORTED otherwise);
- * n_out = rows of out = n_cqp / 2. */
+ * n_out = rows of out = n_cqp / 2.  It uses the handle's scratch, so it is
+ * ordered after the handle's previous call (as pg_asm_assemble_device is) and
+ * returns once the rows are written. */
 int pg_asm_cohesive_openings(pg_asm* h, const double* U, const double* centers, double* out, int64_t n_out,
                              void* stream);
 
@@ -270,7 +283,8 @@ int pg_asm_cohesive_openings(pg_asm* h, const double* U, const double* centers, 
  * normalised failure (delta_eff / delta_c for PG_MAT_OP_NU, delta_eff / delta_c1
  * for PG_MAT_BL_NU, 0 for PG_MAT_OP and PG_MAT_BL, as the reference's
  * getNormalizedCohesiveFailure).  U: global-dof order.  Device pointers;
- * asynchronous on stream (0: the handle's stream).  n_out = n_cqp / 2. */
+ * asynchronous on stream (0: the handle's stream), ordered after the handle's
+ * previous call as pg_asm_assemble_device is.  n_out = n_cqp / 2. */
 int pg_asm_cohesive_failure(pg_asm* h, const double* U, double* out, int64_t n_out, void* stream);
 
 /* Linear solve of a Newton iteration on the device: Jacobi-preconditioned
@@ -283,8 +297,9 @@ int pg_asm_cohesive_failure(pg_asm* h, const double* U, double* out, int64_t n_o
  * stopped (a caller checks rel_res against rtol).  Device pointers; ordered on
  * stream (0: the null stream) and synchronous: the host reads |r| once per
  * iteration.  Allocates 5 n + 1032 doubles of workspace for the call.  Results
- * are bitwise repeatable.  PG_E_ARG if the iteration breaks down (an
- * indefinite matrix). */
+ * are bitwise repeatable.  PG_E_ARG if b, the matrix or x is not finite, or as
+ * soon as the iteration breaks down: r.z <= 0 or p.Ap <= 0, which an
+ * indefinite matrix (or a non-positive diagonal) produces. */
 int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
                double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream);
 

@@ -70,6 +70,7 @@ typedef struct oracle {
     int ncolour;
     int32_t* colour_ptr;  /* [ncolour + 1] */
     int32_t* colour_elem; /* elements grouped by colour, ascending within one */
+    int own_pattern;      /* 0: rp / ci are the description's (harness) pattern */
     char err[256];
 } oracle;
 
@@ -395,7 +396,8 @@ static void oracle_free_rows(oracle* o) {
 void oracle_free(oracle* o) {
     if (!o) return;
     oracle_free_rows(o);
-    for (int b = 0; b < 4; ++b) { free(o->rp[b]); free(o->ci[b]); }
+    if (o->own_pattern)
+        for (int b = 0; b < 4; ++b) { free(o->rp[b]); free(o->ci[b]); }
     free(o->off);
     free(o->cache);
     free(o->colour_ptr);
@@ -403,16 +405,38 @@ void oracle_free(oracle* o) {
     free(o);
 }
 
+static oracle* oracle_create(const pg_problem_desc* d, int e0, int e1, int own_pattern);
+
 /* Build the pattern over elements [e0, e1) and cache their shape data. */
-oracle* oracle_new(const pg_problem_desc* d, int e0, int e1) {
+oracle* oracle_new(const pg_problem_desc* d, int e0, int e1) { return oracle_create(d, e0, e1, 1); }
+
+/* The same over the description's own 4-block pattern (the harness's): K
+ * values land in the layout of d->rp / d->ci, and an element entry missing
+ * from it is an error.  Lets a large mesh be checked in element ranges with
+ * bounded memory (oracle_assemble_range). */
+oracle* oracle_new_desc_pattern(const pg_problem_desc* d, int e0, int e1) { return oracle_create(d, e0, e1, 0); }
+
+static oracle* oracle_create(const pg_problem_desc* d, int e0, int e1, int own_pattern) {
     oracle* o = (oracle*)calloc(1, sizeof(oracle));
     if (!o) return NULL;
     o->d = d;
     o->e0 = e0 < 0 ? 0 : e0;
     o->e1 = (e1 < 0 || e1 > d->n_elem) ? d->n_elem : e1;
     o->nstr = d->dim == 2 ? 3 : 6;
+    o->own_pattern = own_pattern;
     int nf = d->n_free, np = d->n_pres;
     o->nrows[0] = nf; o->nrows[1] = nf; o->nrows[2] = np; o->nrows[3] = np;
+    if (!own_pattern) {
+        int64_t acc = 0;
+        for (int b = 0; b < 4; ++b) {
+            o->rp[b] = (int32_t*)d->rp[b];
+            o->ci[b] = (int32_t*)d->ci[b];
+            o->nnz[b] = d->nnz[b];
+            o->base[b] = acc;
+            acc += d->nnz[b];
+        }
+        goto cache;
+    }
     for (int b = 0; b < 4; ++b) {
         o->rows[b] = (row_t*)calloc((size_t)(o->nrows[b] ? o->nrows[b] : 1), sizeof(row_t));
         if (!o->rows[b]) { oracle_free(o); return NULL; }
@@ -446,6 +470,7 @@ oracle* oracle_new(const pg_problem_desc* d, int e0, int e1) {
         acc += n;
     }
     oracle_free_rows(o);
+cache:
     /* shape cache */
     o->off = (int64_t*)malloc(sizeof(int64_t) * ((size_t)(o->e1 - o->e0) + 1));
     if (!o->off) { oracle_free(o); return NULL; }
@@ -453,9 +478,17 @@ oracle* oracle_new(const pg_problem_desc* d, int e0, int e1) {
     for (int e = o->e0; e < o->e1; ++e) o->off[e - o->e0 + 1] = o->off[e - o->e0] + cache_size(d, e);
     o->cache = (double*)malloc(sizeof(double) * (size_t)(o->off[o->e1 - o->e0] + 1));
     if (!o->cache) { oracle_free(o); return NULL; }
-    for (int e = o->e0; e < o->e1; ++e)
-        if (store_shapes(o, e, o->cache + o->off[e - o->e0])) return o; /* error kept in o->err */
+    /* elements fill disjoint cache slots, so they run in parallel; the first
+     * element (in element order) that fails is then run again alone to keep
+     * its message */
+    int bad = o->e1;
+#pragma omp parallel for schedule(static, 4096) reduction(min : bad)
+    for (int e = o->e0; e < o->e1; ++e) {
+        oracle tmp = *o;
+        if (store_shapes(&tmp, e, o->cache + o->off[e - o->e0]) && e < bad) bad = e;
+    }
     o->err[0] = 0;
+    if (bad < o->e1) store_shapes(o, bad, o->cache + o->off[bad - o->e0]); /* error kept in o->err */
     return o;
 }
 
@@ -788,6 +821,38 @@ int oracle_assemble_threads(oracle* o, const double* U, const double* isv_c, dou
     if (jac) memset(K, 0, sizeof(double) * (size_t)(o->nnz[0] + o->nnz[1] + o->nnz[2] + o->nnz[3]));
     if (func) memset(P, 0, sizeof(double) * (size_t)(d->n_free + d->n_pres));
     if (d->n_cqp > 0) memcpy(isv_t, isv_c, sizeof(double) * (size_t)d->n_cqp);
+    int bad = 0;
+    for (int c = 0; c < o->ncolour; ++c) {
+        const int k0 = o->colour_ptr[c], k1 = o->colour_ptr[c + 1];
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(| : bad)
+        for (int k = k0; k < k1; ++k)
+            bad |= assemble_elem(o, o->colour_elem[k], U, isv_t, func, jac, K, P) != 0;
+    }
+    return bad ? -1 : 0;
+}
+
+/* Assembly of this oracle's element range into K / P that already hold the
+ * ranges before it: first != 0 clears K and P and resets the trial ISVs from
+ * the committed ones (as oracle_assemble does), first == 0 adds to them.  Over
+ * consecutive ranges in element order, nthreads == 0 gives oracle_assemble's
+ * result over their union bit for bit (every entry receives its adds in
+ * element order); nthreads > 0 runs each range colour by colour. */
+int oracle_assemble_range(oracle* o, const double* U, const double* isv_c, double* isv_t, int func, int jac,
+                          double* K, double* P, int first, int nthreads) {
+    const pg_problem_desc* d = o->d;
+    if (!func && !jac) return 0;
+    if (o->err[0]) return -1;
+    if (first) {
+        if (jac) memset(K, 0, sizeof(double) * (size_t)(o->nnz[0] + o->nnz[1] + o->nnz[2] + o->nnz[3]));
+        if (func) memset(P, 0, sizeof(double) * (size_t)(d->n_free + d->n_pres));
+        if (d->n_cqp > 0) memcpy(isv_t, isv_c, sizeof(double) * (size_t)d->n_cqp);
+    }
+    if (nthreads <= 0) {
+        for (int e = o->e0; e < o->e1; ++e)
+            if (assemble_elem(o, e, U, isv_t, func, jac, K, P)) return -1;
+        return 0;
+    }
+    if (!o->colour_ptr && build_colours(o)) return -1;
     int bad = 0;
     for (int c = 0; c < o->ncolour; ++c) {
         const int k0 = o->colour_ptr[c], k1 = o->colour_ptr[c + 1];

@@ -97,6 +97,41 @@ class Oracle:
             pass
 
 
+def assemble_in_ranges(problem, U, isv_committed=None, chunk=2_000_000, threads=0):
+    """The oracle's assembly of the whole mesh into the harness's own 4-block
+    pattern (problem.rp / problem.ci), element range by element range, so that
+    only one range's shape cache is held at a time.  threads = 0 gives the
+    serial element loop's result bit for bit; threads > 0 runs each range
+    colour by colour.  Returns (K, P, trial ISVs)."""
+    L = lib()
+    L.oracle_new_desc_pattern.restype = C.c_void_p
+    L.oracle_new_desc_pattern.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.oracle_assemble_range.restype = C.c_int
+    L.oracle_assemble_range.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    p = problem
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    K = np.zeros(max(p.nnz_total, 1), dtype=np.float64)
+    P = np.zeros(p.n_dof, dtype=np.float64)
+    ic = np.ascontiguousarray(isv_committed if isv_committed is not None else np.zeros(p.n_cqp), dtype=np.float64)
+    it = np.zeros(max(p.n_cqp, 1), dtype=np.float64)
+    for e0 in range(0, p.n_elem, chunk):
+        h = L.oracle_new_desc_pattern(C.addressof(p.desc), e0, min(p.n_elem, e0 + chunk))
+        if not h:
+            raise MemoryError("oracle_new_desc_pattern failed")
+        try:
+            err = L.oracle_error(h)
+            if err:
+                raise RuntimeError("oracle set-up: " + err.decode())
+            rc = L.oracle_assemble_range(h, U.ctypes.data, ic.ctypes.data if p.n_cqp else None, it.ctypes.data, 1, 1,
+                                         K.ctypes.data, P.ctypes.data, int(e0 == 0), int(threads))
+            if rc != 0:
+                raise RuntimeError("oracle_assemble_range: " + L.oracle_error(h).decode())
+        finally:
+            L.oracle_free(h)
+    return K[: p.nnz_total], P, it[: p.n_cqp]
+
+
 def cohesive_law(mat, dim, delta, isv=0.0):
     """The oracle's cohesive constitutive update: returns (delta_used, isv_out, t, D)."""
     L = lib()

@@ -144,6 +144,11 @@ class AssemblerGPU:
                                           self.flags(func, jac), dp(K), dp(P), st)
         check(rc, "pg_asm_assemble_device")
 
+    def check_error(self) -> None:
+        """Waits for the last assembly and raises PgError (PG_E_MESH) if a cut
+        element had a child of negative Jacobian (src/Mesh.cpp:1561)."""
+        check(lib().pg_asm_check_error(self._h), "pg_asm_check_error")
+
     def cohesive_openings(self, U) -> tuple:
         """Cohesive openings of a 2D cohesive mesh (WriterCohesiveOpenings,
         src/WriterCohesiveOpenings.cpp:73-147) from the GPU: U is a host array in

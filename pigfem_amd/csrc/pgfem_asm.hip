@@ -58,7 +58,6 @@ struct DevProb {
     const int32_t* n2e;
     const int64_t* cut_koff;   // per cut element: offset of its ndof^2 block
     const int64_t* cut_poff;   // per cut element: offset of its ndof vector
-    const int64_t* elem_off;   // plain schedule: per element offset of its staged Ke + Pe
     // per group size, NG node entries per group in group-list order (groups
     // without cut incidences first): node, first incidence, first pair,
     // incidences | pairs << 16; entries past the last node count nothing
@@ -93,6 +92,17 @@ __device__ inline int xcd_order(int b, int nb) {
     constexpr int X = 8;
     const int per = nb / X, rem = nb % X, x = b % X;
     return x * per + min(x, rem) + b / X;
+}
+
+// Clears a flag word (a kernel rather than hipMemsetAsync, so that a caller
+// may capture the launch sequence in a HIP graph).
+__global__ void k_fill_int(int* __restrict__ p, int v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+__global__ void k_fill_double(int64_t n, double* __restrict__ p, double v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // --------------------------------------------------------------------------
@@ -236,175 +246,284 @@ __device__ inline void cohesive_point(const M& m, int cnn, const double* N, cons
 }
 
 // --------------------------------------------------------------------------
-// Plain FP64 element kernel (the drop-in schedule, no class binning): one
-// thread per parent element, cut or not, following Assembler::assemble_elem
-// (src/Assembler.cpp:148-368) with KernelVolumetric's B-matrix variant: per qp
-// K_tmp = (B^T D) B, P_tmp = B^T sigma, K_el += (J w) K_tmp.  Uncut linear
-// simplices keep K_el in registers (B, D, sigma are the same at every qp, so each
-// entry accumulates the qp weights in reference order from one K_tmp value);
-// intersected elements accumulate their larger K_el directly in the staging
-// slot.  Staged layout per element: Ke (nd*nd, row-major) then Pe (nd), at
-// elem_off[e].
+// Plain FP64 schedule (the drop-in, no class binning): element kernels that
+// follow Assembler::assemble_elem (src/Assembler.cpp:148-368) with
+// KernelVolumetric's B-matrix variant -- per qp K_tmp = (B^T D) B,
+// P_tmp = B^T sigma, K_el += (J w) K_tmp, P_el += (J w) P_tmp -- stage every
+// element matrix in HBM, and a node-pair kernel sums the staged blocks into the
+// CSR values in element order.
+//
+// Staged layout of uncut elements: blocks of 64 consecutive elements; inside a
+// block, for each local block (la, lb) of the element matrix the 64 elements'
+// d x d blocks one after another (each d x d block contiguous), then for each
+// local node la the 64 elements' d residual entries.  A thread's stores then
+// fill contiguous stretches together with its neighbours', and a node-pair
+// lane reads one d x d block contiguously.  Cut elements stage row-major
+// nd x nd matrices (the reference's K_el) at cut_koff and nd vectors at cut_poff.
 // --------------------------------------------------------------------------
+template <int D> struct PlainStage {
+    static constexpr int NEN = D + 1;
+    static constexpr int64_t KB = int64_t(NEN) * NEN * D * D * 64;  // Ke part of a block of 64 elements
+    static constexpr int64_t W = KB + int64_t(NEN) * D * 64;        // doubles per block of 64 elements
+    __host__ __device__ static int64_t kblk(int64_t e, int la, int lb) {
+        return (e >> 6) * W + (int64_t(la * NEN + lb) * 64 + (e & 63)) * (D * D);
+    }
+    __host__ __device__ static int64_t pvec(int64_t e, int la) {
+        return (e >> 6) * W + KB + (int64_t(la) * 64 + (e & 63)) * D;
+    }
+};
+
+// Stages the D matrices of the materials in LDS (zero past the last one).
+template <int NS>
+__device__ inline void stage_dmats(const DevProb& p, double (*sD)[NS * NS]) {
+    for (int k = threadIdx.x; k < MAX_LDS_MATS * NS * NS; k += blockDim.x) {
+        const int m = k / (NS * NS), j = k % (NS * NS);
+        const double v = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[j];
+        sD[m][j] = m < p.n_mat ? v : 0.0;
+    }
+}
+
+// Uncut elements, one thread each.  A linear simplex has the same B, D and
+// sigma at every qp, so each entry accumulates (J w_q) K_tmp over the qps in
+// the reference order from one K_tmp value.
 template <int D>
-__global__ void k_elem_fp64(DevProb p, const double* __restrict__ Un, const double* __restrict__ isv_c,
-                            double* __restrict__ isv_t, double* __restrict__ stage, int* __restrict__ err) {
-    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5);
-    constexpr int MAXN = CutLim<D>::N;
+__global__ __launch_bounds__(256) void k_elem_plain(DevProb p, const double* __restrict__ Un,
+                                                    double* __restrict__ stage) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5), ND = NEN * D;
+    __shared__ double sD[MAX_LDS_MATS][NS * NS];
+    stage_dmats<NS>(p, sD);
+    __syncthreads();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n_elem) return;
-    double* Ke = stage + p.elem_off[e];
-    const int ci = p.cut_of_elem[e];
-    double px[NEN][3], pdN[NEN][D], pg[NEN][3];
-    int nodes[MAXN];
+    if (e >= p.n_elem || p.cut_of_elem[e] >= 0) return;
+    double px[NEN][3], pdN[NEN][D], pg[NEN][3], u[ND];
+#pragma unroll
     for (int n = 0; n < NEN; ++n) {
-        nodes[n] = p.conn[int64_t(e) * NEN + n];
-        for (int i = 0; i < D; ++i) px[n][i] = p.coords[int64_t(nodes[n]) * D + i];
+        const int node = p.conn[int64_t(e) * NEN + n];
+        for (int i = 0; i < 3; ++i) px[n][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        for (int i = 0; i < D; ++i) u[n * D + i] = Un[int64_t(node) * D + i];
         for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
     }
     const double J = transform<D>(NEN, pdN, px, pg);
-    if (ci < 0) {
-        constexpr int ND = NEN * D;
-        const pg_material& m = p.mats[p.elem_mat[e]];
-        double wq[NQ];
-        for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
-        double eps[NS], sig[NS];
-        for (int k = 0; k < NS; ++k) eps[k] = 0.0;
-        for (int n = 0; n < NEN; ++n)
-            for (int c = 0; c < D; ++c) {
-                double b[NS];
-                bcol<D, double>(pg[n], c, b);
-                const double u = Un[int64_t(nodes[n]) * D + c];
-                for (int k = 0; k < NS; ++k) eps[k] += b[k] * u;
-            }
-        lei_stress<D>(m.D, eps, sig);
-        double* Pe = Ke + ND * ND;
-        for (int i = 0; i < ND; ++i) {
-            double bi[NS], btd[NS];
-            bcol<D, double>(pg[i / D], i % D, bi);
-            for (int l = 0; l < NS; ++l) {
-                double s = 0.0;
-                for (int k = 0; k < NS; ++k) s += bi[k] * m.D[k * NS + l];
-                btd[l] = s;
-            }
-            for (int j = 0; j < ND; ++j) {
-                double bj[NS];
-                bcol<D, double>(pg[j / D], j % D, bj);
-                double kt = 0.0;
-                for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
-                double acc = 0.0;
-                for (int q = 0; q < NQ; ++q) acc += wq[q] * kt;
-                Ke[i * ND + j] = acc;
-            }
-            double pt = 0.0;
-            for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
+    const double* Dm = sD[p.elem_mat[e]];
+    double wq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
+    // strain = B U (dgemv order), stress
+    double eps[NS], sig[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) eps[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        double b[NS];
+        bcol<D, double>(pg[i / D], i % D, b);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) eps[k] += b[k] * u[i];
+    }
+    lei_stress<D>(Dm, eps, sig);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        double bi[NS], btd[NS];
+        bcol<D, double>(pg[i / D], i % D, bi);
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) t += bi[k] * Dm[k * NS + l];
+            btd[l] = t;
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double bj[NS];
+            bcol<D, double>(pg[j / D], j % D, bj);
+            double kt = 0.0;
+#pragma unroll
+            for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
             double acc = 0.0;
-            for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
-            Pe[i] = acc;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc += wq[q] * kt;
+            stage[PlainStage<D>::kblk(e, i / D, j / D) + (i % D) * D + j % D] = acc;
         }
-        return;
+        double pt = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pt += bi[k] * sig[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
+        stage[PlainStage<D>::pvec(e, i / D) + i % D] = acc;
     }
-    // ---- intersected element ----
-    const pg_cut_rec& r = p.cut[ci];
-    const int nloc = NEN + r.n_enr, nd = nloc * D;
-    double* Pe = Ke + int64_t(nd) * nd;
-    double lx[MAXN][3], ue[MAXN * 3];
-    for (int l = 0; l < nloc; ++l) {
-        if (l >= NEN) nodes[l] = r.enr[l - NEN];
-        for (int i = 0; i < D; ++i) {
-            lx[l][i] = p.coords[int64_t(nodes[l]) * D + i];
-            ue[l * D + i] = Un[int64_t(nodes[l]) * D + i];
+}
+
+// Cut elements, one wavefront each (store_shape_functions for its children,
+// src/Mesh.cpp:1521-1641, then assemble_elem's cut and cohesive branches,
+// src/Assembler.cpp:223-365).  Phase 1: one lane per child qp computes the
+// child Jacobian, the gradient of every local node (parent gradient, the child
+// gradient of the child node that is the enrichment node, else 0;
+// src/Mesh.cpp:1596-1614), J w and the stress; one lane per cohesive qp runs
+// the law with the history update.  Phase 2: one lane per entry of K_el sums
+// its volumetric qps in qp order, then adds each cohesive element's K_c (its
+// qps summed first, as the reference folds K_c into K_el); likewise P_el.
+template <int D>
+__global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __restrict__ Un,
+                                                  const double* __restrict__ isv_c, double* __restrict__ isv_t,
+                                                  double* __restrict__ Ks, double* __restrict__ Ps,
+                                                  int* __restrict__ err) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
+    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q, MAXC = PG_MAX_COH * 4;
+    __shared__ double sD[MAX_LDS_MATS][NS * NS];
+    __shared__ double lx[MAXN][3];
+    __shared__ double ue[MAXN * D];
+    __shared__ double gq[MAXQ][MAXN][D];  // gradient of each local node at each volumetric qp
+    __shared__ double wq[MAXQ];            // J w of each volumetric qp
+    __shared__ double sq[MAXQ][NS];        // stress at each volumetric qp
+    __shared__ int mq[MAXQ];               // material of each volumetric qp (its child's)
+    __shared__ double cN[MAXC][3], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
+    __shared__ pg_cut_rec rec;
+    const int c = blockIdx.x, lane = threadIdx.x;
+    stage_dmats<NS>(p, sD);
+    constexpr int RW = int(sizeof(pg_cut_rec) / 4);
+    if (lane < RW) reinterpret_cast<int32_t*>(&rec)[lane] = reinterpret_cast<const int32_t*>(&p.cut[c])[lane];
+    const int e = p.cut_elem[c];
+    __syncthreads();
+    const int nloc = NEN + rec.n_enr, nd = nloc * D;
+    if (lane < nloc) {
+        const int node = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
+        for (int i = 0; i < 3; ++i) lx[lane][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        for (int i = 0; i < D; ++i) ue[lane * D + i] = Un[int64_t(node) * D + i];
+    }
+    __syncthreads();
+    double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
+    for (int n = 0; n < NEN; ++n) {
+        for (int i = 0; i < 3; ++i) px[n][i] = lx[n][i];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    transform<D>(NEN, pdN, px, pgr);
+    const int nvq = rec.n_vqp;
+    for (int t = lane; t < nvq; t += 64) {
+        int k = 0, q = t;
+        for (; k < rec.n_child; ++k) {
+            const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
+            if (q < nq) break;
+            q -= nq;
         }
-    }
-    for (int i = 0; i < nd * nd; ++i) Ke[i] = 0.0;
-    for (int i = 0; i < nd; ++i) Pe[i] = 0.0;
-    for (int k = 0; k < r.n_child; ++k) {
-        const int ct = r.child_type[k], cnn = ct == PG_CHILD_TRI3 ? 3 : 4;
-        const pg_material& m = p.mats[r.child_mat[k]];
-        double cx[4][3], cg[4][3];
-        for (int n = 0; n < cnn; ++n)
-            for (int i = 0; i < D; ++i) cx[n][i] = lx[r.child_nodes[k][n]][i];
-        const int nq = ct == PG_CHILD_TET4 ? 5 : 4;
-        for (int q = 0; q < nq; ++q) {
-            double w;
-            const double Jc = child_qp<D>(ct, q, cnn, cx, cg, w);
-            if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
-            const double wgt = Jc * w;
-            // gradient of local node l at this qp: parent gradient, enrichment
-            // gradient from the child node that is the enrichment node, else 0
-            auto grad = [&](int l, double* g) {
-                if (l < NEN) { for (int a = 0; a < D; ++a) g[a] = pg[l][a]; return; }
-                for (int a = 0; a < D; ++a) g[a] = 0.0;
-                for (int n = 0; n < cnn; ++n)
-                    if (r.child_nodes[k][n] == l)
-                        for (int a = 0; a < D; ++a) g[a] = cg[n][a];
-            };
-            double eps[NS], sig[NS];
-            for (int s = 0; s < NS; ++s) eps[s] = 0.0;
-            for (int l = 0; l < nloc; ++l) {
-                double g[3];
-                grad(l, g);
-                for (int c = 0; c < D; ++c) {
-                    double b[NS];
-                    bcol<D, double>(g, c, b);
-                    for (int s = 0; s < NS; ++s) eps[s] += b[s] * ue[l * D + c];
-                }
+        const int ct = rec.child_type[k], ccn = ct == PG_CHILD_TRI3 ? 3 : 4;
+        double cx[4][3], cg[4][3], w;
+        int cl[4];
+        for (int n = 0; n < 4; ++n) {
+            cl[n] = n < ccn ? rec.child_nodes[k][n] : -1;
+            for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? lx[cl[n]][i] : 0.0;
+        }
+        const double Jc = child_qp<D>(ct, q, ccn, cx, cg, w);
+        if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
+        double eps[NS];
+        for (int s2 = 0; s2 < NS; ++s2) eps[s2] = 0.0;
+        for (int l = 0; l < nloc; ++l) {
+            double gr[3] = {0.0, 0.0, 0.0};
+            if (l < NEN) {
+                for (int a = 0; a < D; ++a) gr[a] = pgr[l][a];
+            } else {
+                for (int n = 0; n < 4; ++n)
+                    if (cl[n] == l)
+                        for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
             }
-            lei_stress<D>(m.D, eps, sig);
-            for (int i = 0; i < nd; ++i) {
-                double gi[3], bi[NS], btd[NS];
-                grad(i / D, gi);
-                bcol<D, double>(gi, i % D, bi);
-                for (int l2 = 0; l2 < NS; ++l2) {
-                    double s = 0.0;
-                    for (int k2 = 0; k2 < NS; ++k2) s += bi[k2] * m.D[k2 * NS + l2];
-                    btd[l2] = s;
-                }
-                for (int j = 0; j < nd; ++j) {
-                    double gj[3], bj[NS];
-                    grad(j / D, gj);
-                    bcol<D, double>(gj, j % D, bj);
-                    double kt = 0.0;
-                    for (int l2 = 0; l2 < NS; ++l2) kt += btd[l2] * bj[l2];
-                    Ke[int64_t(i) * nd + j] += wgt * kt;
-                }
-                double pt = 0.0;
-                for (int s = 0; s < NS; ++s) pt += bi[s] * sig[s];
-                Pe[i] += wgt * pt;
+            for (int a = 0; a < D; ++a) gq[t][l][a] = gr[a];
+            for (int cc = 0; cc < D; ++cc) {
+                double b[NS];
+                bcol<D, double>(gr, cc, b);
+                for (int s2 = 0; s2 < NS; ++s2) eps[s2] += b[s2] * ue[l * D + cc];
             }
         }
+        double sig[NS];
+        lei_stress<D>(sD[rec.child_mat[k]], eps, sig);
+        wq[t] = Jc * w;
+        mq[t] = rec.child_mat[k];
+        for (int s2 = 0; s2 < NS; ++s2) sq[t][s2] = sig[s2];
     }
-    const int cnn = r.coh_nn / 2, cnq = r.coh_nn == 4 ? 2 : 4;
-    for (int k = 0; k < r.n_coh; ++k) {
-        double lo[3][3], ulo[9], uup[9];
-        for (int n = 0; n < cnn; ++n)
+    const int cnn = rec.coh_nn / 2, cnq = rec.coh_nn == 4 ? 2 : 4;
+    const int ncq = rec.n_coh * cnq;
+    for (int cq = lane; cq < ncq; cq += 64) {
+        const int k = cq / cnq, q = cq % cnq;
+        double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
+        for (int n = 0; n < 3; ++n)
             for (int i = 0; i < D; ++i) {
-                lo[n][i] = lx[r.coh_nodes[k][n]][i];
-                ulo[n * D + i] = ue[r.coh_nodes[k][n] * D + i];
-                uup[n * D + i] = ue[r.coh_nodes[k][n + cnn] * D + i];
+                const bool in = n < cnn;
+                lo[n][i] = in ? lx[rec.coh_nodes[k][n]][i] : 0.0;
+                ulo[n * D + i] = in ? ue[rec.coh_nodes[k][n] * D + i] : 0.0;
+                uup[n * D + i] = in ? ue[rec.coh_nodes[k][n + cnn] * D + i] : 0.0;
             }
-        for (int q = 0; q < cnq; ++q) {
-            double N[3], R[D][D], w, RDR[D][D], tR[D];
-            const double dA = cohesive_qp<D>(r.coh_nn, q, lo, N, R, w);
-            const int iq = r.isv_off + k * cnq + q;
-            double hist = isv_c ? isv_c[iq] : 0.0;
-            cohesive_point<D>(p.mats[r.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
-            if (isv_t) isv_t[iq] = hist;
-            const double wgt = dA * w;
-            for (int m1 = 0; m1 < r.coh_nn; ++m1) {
-                const int n1 = m1 % cnn, l1 = r.coh_nodes[k][m1];
-                const double s1 = m1 < cnn ? -1.0 : 1.0;
-                for (int d1 = 0; d1 < D; ++d1) {
-                    Pe[l1 * D + d1] += wgt * (s1 * (N[n1] * tR[d1]));
-                    for (int m2 = 0; m2 < r.coh_nn; ++m2) {
-                        const int n2 = m2 % cnn, l2 = r.coh_nodes[k][m2];
-                        const double sg = ((m1 < cnn) == (m2 < cnn)) ? 1.0 : -1.0;
-                        for (int d2 = 0; d2 < D; ++d2)
-                            Ke[int64_t(l1 * D + d1) * nd + l2 * D + d2] += wgt * (sg * (N[n1] * N[n2] * RDR[d1][d2]));
-                    }
-                }
-            }
+        const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
+        const int iq = rec.isv_off + cq;
+        double hist = isv_c ? isv_c[iq] : 0.0;
+        cohesive_point<D>(p.mats[rec.coh_mat[k]], cnn, N, R, ulo, uup, hist, RDR, tR);
+        if (isv_t) isv_t[iq] = hist;
+        for (int i = 0; i < D; ++i) {
+            for (int j = 0; j < D; ++j) cRDR[cq][i][j] = RDR[i][j];
+            ctR[cq][i] = tR[i];
         }
+        for (int n = 0; n < 3; ++n) cN[cq][n] = N[n];
+        cw[cq] = dA * w;
+    }
+    __syncthreads();
+    double* Ke = Ks + p.cut_koff[c];
+    for (int idx = lane; idx < nd * nd; idx += 64) {
+        const int i = idx / nd, j = idx % nd;
+        const int li = i / D, di = i % D, lj = j / D, dj = j % D;
+        double acc = 0.0;
+        for (int q = 0; q < nvq; ++q) {
+            const double* Dm = sD[mq[q]];
+            double bi[NS], bj[NS], btd[NS];
+            bcol<D, double>(gq[q][li], di, bi);
+            bcol<D, double>(gq[q][lj], dj, bj);
+            for (int l = 0; l < NS; ++l) {
+                double t = 0.0;
+                for (int k = 0; k < NS; ++k) t += bi[k] * Dm[k * NS + l];
+                btd[l] = t;
+            }
+            double kt = 0.0;
+            for (int l = 0; l < NS; ++l) kt += btd[l] * bj[l];
+            acc += wq[q] * kt;
+        }
+        for (int k = 0; k < rec.n_coh; ++k) {
+            int m1 = -1, m2 = -1;
+            for (int m = 0; m < rec.coh_nn; ++m) {
+                if (rec.coh_nodes[k][m] == li) m1 = m;
+                if (rec.coh_nodes[k][m] == lj) m2 = m;
+            }
+            if (m1 < 0 || m2 < 0) continue;
+            const int n1 = m1 % cnn, n2 = m2 % cnn;
+            const double sg = ((m1 < cnn) == (m2 < cnn)) ? 1.0 : -1.0;
+            double kc = 0.0;
+            for (int q = 0; q < cnq; ++q) {
+                const int cq = k * cnq + q;
+                kc += cw[cq] * (sg * (cN[cq][n1] * cN[cq][n2] * cRDR[cq][di][dj]));
+            }
+            acc += kc;
+        }
+        Ke[idx] = acc;
+    }
+    double* Pe = Ps + p.cut_poff[c];
+    for (int i = lane; i < nd; i += 64) {
+        const int l = i / D, di = i % D;
+        double acc = 0.0;
+        for (int q = 0; q < nvq; ++q) {
+            double bi[NS];
+            bcol<D, double>(gq[q][l], di, bi);
+            double pt = 0.0;
+            for (int k = 0; k < NS; ++k) pt += bi[k] * sq[q][k];
+            acc += wq[q] * pt;
+        }
+        for (int k = 0; k < rec.n_coh; ++k)
+            for (int m = 0; m < rec.coh_nn; ++m)
+                if (rec.coh_nodes[k][m] == l) {
+                    const int n = m % cnn;
+                    const double sgn = m < cnn ? -1.0 : 1.0;
+                    double pc = 0.0;
+                    for (int q = 0; q < cnq; ++q) {
+                        const int cq = k * cnq + q;
+                        pc += cw[cq] * (sgn * (cN[cq][n] * ctR[cq][di]));
+                    }
+                    acc += pc;
+                }
+        Pe[i] = acc;
     }
 }
 
@@ -930,8 +1049,9 @@ __global__ void k_pcg_p(int64_t n, double* __restrict__ pv, const double* __rest
 }
 
 template <int D>
-__global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, double* __restrict__ K,
-                              double* __restrict__ P, unsigned flags) {
+__global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const double* __restrict__ Ks,
+                              const double* __restrict__ Ps, double* __restrict__ K, double* __restrict__ P,
+                              unsigned flags) {
     constexpr int NEN = D + 1;
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= p.n_pairs) return;
@@ -946,15 +1066,30 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, doubl
     for (int t = e0; t < e1; ++t) {
         const int e = p.n2e[t];
         const int ci = p.cut_of_elem[e];
+        if (ci < 0) {
+            int la = -1, lb = -1;
+            for (int n = 0; n < NEN; ++n) {
+                const int node = p.conn[int64_t(e) * NEN + n];
+                if (node == a) la = n;
+                if (node == b) lb = n;
+            }
+            if (la < 0 || lb < 0) continue;
+            const double* Kb = stage + PlainStage<D>::kblk(e, la, lb);
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) acc[i][j] += Kb[i * D + j];
+            if (diag)
+                for (int i = 0; i < D; ++i) pa[i] += stage[PlainStage<D>::pvec(e, la) + i];
+            continue;
+        }
         const int la = find_local(p, e, ci, a, NEN);
         const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
         if (la < 0 || lb < 0) continue;
-        const int nd = (ci < 0 ? NEN : NEN + p.cut[ci].n_enr) * D;
-        const double* Ke = stage + p.elem_off[e];
+        const int nd = (NEN + p.cut[ci].n_enr) * D;
+        const double* Ke = Ks + p.cut_koff[ci];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) acc[i][j] += Ke[int64_t(la * D + i) * nd + lb * D + j];
         if (diag)
-            for (int i = 0; i < D; ++i) pa[i] += Ke[int64_t(nd) * nd + la * D + i];
+            for (int i = 0; i < D; ++i) pa[i] += Ps[p.cut_poff[ci] + la * D + i];
     }
     if (flags & PG_JAC) {
         const int32_t* mp = p.npat_map + q * D * D;
@@ -1767,7 +1902,9 @@ struct pg_asm {
     int64_t cut_k_total = 0, cut_p_total = 0;
     std::vector<void*> allocs;
     double* Un = nullptr;        // node-major U
-    double* stage = nullptr;     // plain schedule: per element Ke + Pe (FP64)
+    double* stage = nullptr;     // plain schedule: uncut elements' Ke + Pe (FP64, PlainStage layout)
+    double* stage_k = nullptr;   // plain schedule: cut elements' K_el, row-major, at cut_koff
+    double* stage_p = nullptr;   // plain schedule: cut elements' P_el at cut_poff
     double* Kc64 = nullptr;      // cut tangents (FP64)
     float* Kc32 = nullptr;       // cut tangents (FP32)
     double* Pc = nullptr;        // cut residuals (FP64)
@@ -1780,6 +1917,12 @@ struct pg_asm {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t side = nullptr;     // second stream: node groups without cut elements overlap the cut kernel
     hipEvent_t sync[2] = {nullptr, nullptr};
+    // completion of the handle's last call that used its scratch (Un, staged
+    // tangents, the error flag) and the stream it ran on: a call on another
+    // stream waits for it first
+    hipEvent_t done = nullptr;
+    hipStream_t done_st = nullptr;
+    bool done_valid = false;
     int gopt_ng[3] = {0, 0, 0};     // group sizes with a group list
     int goff[3] = {0, 0, 0};        // first gtab entry of each group list
     int64_t gioff[3] = {0, 0, 0};   // first incg / slotg slot of each group list
@@ -1789,7 +1932,6 @@ struct pg_asm {
     bool timing = false;
     double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
-    int elem_block = 256;
     int group_nodes = 0;   // node-group size of the binned schedule (0: default)
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
     int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap;
@@ -1912,6 +2054,21 @@ int alloc(pg_asm* h, T** p, size_t n) {
     return PG_OK;
 }
 
+// Calls on one handle share its scratch buffers: a call on a stream other than
+// the last call's waits for that call to complete, and every call records its
+// own completion.
+int order_after_last(pg_asm* h, hipStream_t st) {
+    if (h->done_valid && st != h->done_st) HIP_OK(hipStreamWaitEvent(st, h->done, 0));
+    return PG_OK;
+}
+
+int record_done(pg_asm* h, hipStream_t st) {
+    HIP_OK(hipEventRecord(h->done, st));
+    h->done_st = st;
+    h->done_valid = true;
+    return PG_OK;
+}
+
 // Node-group kernel launch: the block-table form for tri3 meshes unless the
 // list form is asked for; tet4 meshes always use the list form.
 template <int D, typename TC, typename TKC, int NG, int TV>
@@ -1939,17 +2096,25 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     const int64_t nu = nall * d.dim;
     const bool fp32 = (flags & PG_FP32) != 0;
     const bool binned = fp32 || (flags & PG_BATCHED) != 0;
+    if (int rc = order_after_last(h, st)) return rc;
     if (h->timing) HIP_OK(hipEventRecord(h->ev[0], st));
+    // the error flag describes this assembly only (pg_asm_check_error)
+    k_fill_int<<<1, 64, 0, st>>>(h->err, 0);
     k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, d.node_dof, dU, h->Un);
     const unsigned gp = unsigned((d.n_pairs + h->pair_block - 1) / h->pair_block);
     if (!binned) {
-        const unsigned gb = unsigned((d.n_elem + h->elem_block - 1) / h->elem_block);
-        if (d.dim == 2) k_elem_fp64<2><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
-        else k_elem_fp64<3><<<gb, h->elem_block, 0, st>>>(d, h->Un, dIc, dIt, h->stage, h->err);
+        const unsigned gb = unsigned((d.n_elem + 255) / 256);
+        if (d.dim == 2) {
+            k_elem_plain<2><<<gb, 256, 0, st>>>(d, h->Un, h->stage);
+            if (d.n_cut) k_cut_plain<2><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err);
+        } else {
+            k_elem_plain<3><<<gb, 256, 0, st>>>(d, h->Un, h->stage);
+            if (d.n_cut) k_cut_plain<3><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err);
+        }
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
-        if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
-        else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, dK, dP, flags);
+        if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, dK, dP, flags);
+        else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, dK, dP, flags);
     } else {
         const int nall = d.n_nodes + d.n_enr;
         const bool cut64 = fp32 && (flags & PG_CUT64) != 0;  // cut elements staged in FP64
@@ -2062,7 +2227,7 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
-    return PG_OK;
+    return record_done(h, st);
 }
 
 }  // namespace
@@ -2098,13 +2263,6 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         delete h;
         g_err = "cut-element dofs exceed 2^31";
         return PG_E_UNSUPPORTED;
-    }
-    std::vector<int64_t> eoff(size_t(s->n_elem) + 1);
-    eoff[0] = 0;
-    for (int e = 0; e < s->n_elem; ++e) {
-        const int c = s->cut_of_elem[e];
-        const int64_t nd = int64_t(s->nen + (c >= 0 ? s->cut[c].n_enr : 0)) * s->dim;
-        eoff[e + 1] = eoff[e] + nd * nd + nd;
     }
     d.n_mat = s->n_mat;
     // incidence records in n2e order (host only; the kernels read them regrouped
@@ -2205,7 +2363,6 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(n2e, s->n2e, s->n2e_ptr[nall], int32_t);
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
-    UP(elem_off, eoff.data(), s->n_elem + 1, int64_t);
     {
         // node table, and for each group size the list of group starts: groups
         // none of whose nodes touches a cut element first (they can run while
@@ -2386,7 +2543,13 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     const int ndp = s->nen * s->dim;
     if ((rc = alloc(h, &h->Un, size_t(nall * s->dim)))) return fail(rc);
     (void)ndp;
-    if ((rc = alloc(h, &h->stage, size_t(eoff[s->n_elem])))) return fail(rc);
+    {
+        const int64_t nblk = (int64_t(s->n_elem) + 63) / 64;
+        const int64_t W = s->dim == 2 ? PlainStage<2>::W : PlainStage<3>::W;
+        if ((rc = alloc(h, &h->stage, size_t(nblk * W)))) return fail(rc);
+        if ((rc = alloc(h, &h->stage_k, size_t(h->cut_k_total)))) return fail(rc);
+        if ((rc = alloc(h, &h->stage_p, size_t(h->cut_p_total)))) return fail(rc);
+    }
     // the node-group kernels read a staged row's first 8 blocks at once: pad
     if ((rc = alloc(h, &h->Kc64, size_t(h->cut_k_total) + KC_PAD))) return fail(rc);
     if ((rc = alloc(h, &h->Kc32, size_t(h->cut_k_total) + KC_PAD))) return fail(rc);
@@ -2402,6 +2565,10 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         if (hipEventCreate(&e) != hipSuccess) { g_err = "hipEventCreate failed"; return fail(PG_E_HIP); }
     for (auto& e : h->sync)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { g_err = "hipEventCreate failed"; return fail(PG_E_HIP); }
+    if (hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess) {
+        g_err = "hipEventCreate failed";
+        return fail(PG_E_HIP);
+    }
     // algorithmic bytes of one K+R assembly
     int64_t B = 4LL * s->nen * s->n_elem + 8LL * s->dim * nall + 8LL * h->n_dof + int64_t(sizeof(pg_cut_rec)) * s->n_cut +
                 16LL * s->n_cqp + 12LL * h->nnz_total + 4LL * (int64_t(s->n_free) + s->n_pres + 2) + 8LL * h->n_dof;
@@ -2421,6 +2588,10 @@ int pg_asm_destroy(pg_asm* h) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->sync)
         if (e) (void)hipEventDestroy(e);
+    if (h->done) {
+        (void)hipEventSynchronize(h->done);
+        (void)hipEventDestroy(h->done);
+    }
     if (h->side) (void)hipStreamDestroy(h->side);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -2454,10 +2625,12 @@ int pg_asm_cohesive_openings(pg_asm* h, const double* dU, const double* centers,
     if (!dU || !centers || !out) { g_err = "missing device buffer"; return PG_E_ARG; }
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
     const int64_t nu = int64_t(h->d.n_nodes + h->d.n_enr) * h->d.dim;
-    HIP_OK(hipMemsetAsync(h->err, 0, sizeof(int), st));
+    if (int rc = order_after_last(h, st)) return rc;
+    k_fill_int<<<1, 64, 0, st>>>(h->err, 0);
     k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, h->d.node_dof, dU, h->Un);
     k_coh_openings<<<dim3(unsigned((h->d.n_cut + 255) / 256)), dim3(256), 0, st>>>(h->d, h->Un, centers, out, h->err);
     HIP_OK(hipGetLastError());
+    if (int rc = record_done(h, st)) return rc;
     int herr = 0;
     HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -2473,10 +2646,11 @@ int pg_asm_cohesive_failure(pg_asm* h, const double* dU, double* out, int64_t n_
     if (!dU || !out) { g_err = "missing device buffer"; return PG_E_ARG; }
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
     const int64_t nu = int64_t(h->d.n_nodes + h->d.n_enr) * h->d.dim;
+    if (int rc = order_after_last(h, st)) return rc;
     k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, h->d.node_dof, dU, h->Un);
     k_coh_failure<<<dim3(unsigned((h->d.n_cut + 255) / 256)), dim3(256), 0, st>>>(h->d, h->Un, out);
     HIP_OK(hipGetLastError());
-    return PG_OK;
+    return record_done(h, st);
 }
 
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant) {
@@ -2551,7 +2725,6 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
         h->dev_bytes += 8 * (2 * h->n_dof + std::max<int64_t>(h->nnz_total, 1) + 2 * std::max<int64_t>(ncq, 1));
     }
     hipStream_t st = h->stream;
-    HIP_OK(hipMemsetAsync(h->err, 0, sizeof(int), st));
     HIP_OK(hipMemcpyAsync(h->dU, U, size_t(h->n_dof) * sizeof(double), hipMemcpyHostToDevice, st));
     if (ncq > 0) HIP_OK(hipMemcpyAsync(h->dIc, Ic, size_t(ncq) * sizeof(double), hipMemcpyHostToDevice, st));
     int rc = pg_asm_assemble_device(h, h->dU, h->dIc, h->dIt, delta_t, flags, h->dK, h->dP, st);
@@ -2577,6 +2750,16 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
     return PG_OK;
 }
 
+int pg_asm_check_error(pg_asm* h) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (!h->done_valid) return PG_OK;
+    HIP_OK(hipEventSynchronize(h->done));
+    int herr = 0;
+    HIP_OK(hipMemcpy(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr & 1) { g_err = "negative child Jacobian in a cut element"; return PG_E_MESH; }
+    return PG_OK;
+}
+
 int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
                double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream) {
     if (n < 0 || max_iter < 0 || !(rtol >= 0.0)) { g_err = "bad size, iteration limit or tolerance"; return PG_E_ARG; }
@@ -2599,26 +2782,38 @@ int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const 
         k_dot2<<<dim3(nb), dim3(PCG_T), 0, st>>>(n, a, bb, c, d, part);
         k_dot_fin<<<dim3(1), dim3(PCG_T), 0, st>>>(part, nb, sc, o0, o1);
     };
-    auto read = [&](int o, double& v) -> int {
-        HIP_OK(hipMemcpyAsync(&v, sc + o, sizeof(double), hipMemcpyDeviceToHost, st));
+    // the scalars sc[0..3] come back together once per iteration
+    double hs[4] = {0.0, 0.0, 0.0, 0.0};
+    auto read = [&]() -> int {
+        HIP_OK(hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         return PG_OK;
     };
     k_csr_diag_inv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, dinv);
     dot2(b, b, b, b, 3, 4);
-    double bb = 0.0, rr = 0.0;
-    if (int e = read(3, bb)) return e;
+    if (int e = read()) return e;
+    const double bb = hs[3];
+    if (!std::isfinite(bb)) { g_err = "right-hand side is not finite"; return PG_E_ARG; }
     if (bb == 0.0) {
-        HIP_OK(hipMemsetAsync(x, 0, nv * sizeof(double), st));
+        k_fill_double<<<g, t, 0, st>>>(n, x, 0.0);
+        HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(st));
         return PG_OK;
     }
     k_csr_spmv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, x, q);
     k_pcg_init<<<g, t, 0, st>>>(n, b, q, dinv, r, z, pv);
     dot2(r, z, r, r, 0, 3);
-    if (int e = read(3, rr)) return e;
+    if (int e = read()) return e;
+    double rr = hs[3];
+    if (!std::isfinite(rr) || !std::isfinite(hs[0])) {
+        g_err = "matrix or initial guess is not finite";
+        return PG_E_ARG;
+    }
     int it = 0, cur = 0;
     while (it < max_iter && std::sqrt(rr) > rtol * std::sqrt(bb)) {
+        // r.z > 0 and p.Ap > 0 hold for a symmetric positive definite matrix
+        // and a positive Jacobi diagonal; anything else is a breakdown
+        if (!(hs[cur] > 0.0)) { g_err = "PCG broke down: r.z <= 0 (matrix not positive definite)"; return PG_E_ARG; }
         k_csr_spmv<<<g, t, 0, st>>>(n, row_ptr, col_idx, vals, pv, q);
         dot2(pv, q, pv, q, 2, 4);
         k_pcg_xr<<<g, t, 0, st>>>(n, x, r, z, pv, q, dinv, sc, cur);
@@ -2627,8 +2822,10 @@ int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const 
         HIP_OK(hipGetLastError());
         cur = 1 - cur;
         ++it;
-        if (int e = read(3, rr)) return e;
-        if (!std::isfinite(rr)) { g_err = "PCG broke down (matrix not positive definite?)"; return PG_E_ARG; }
+        if (int e = read()) return e;
+        if (!(hs[2] > 0.0)) { g_err = "PCG broke down: p.Ap <= 0 (matrix not positive definite)"; return PG_E_ARG; }
+        rr = hs[3];
+        if (!std::isfinite(rr)) { g_err = "PCG broke down: residual is not finite"; return PG_E_ARG; }
     }
     if (iters) *iters = it;
     if (rel_res) *rel_res = std::sqrt(rr / bb);

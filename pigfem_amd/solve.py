@@ -39,7 +39,7 @@ class DeviceCSR:
         mi = 10 * self.n if max_iter is None else int(max_iter)
         check(lib().pg_csr_pcg(self.n, self.rp.ptr, self.ci.ptr, dV.ptr, dB.ptr, dX.ptr, float(rtol), mi,
                                C.byref(it), C.byref(rel), None), "pg_csr_pcg")
-        if rel.value > rtol:
+        if not rel.value <= rtol:
             raise PgError(f"PCG did not converge: relative residual {rel.value:.3e} after {it.value} iterations")
         return dX.download(self.n), it.value, rel.value
 

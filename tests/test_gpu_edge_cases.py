@@ -58,3 +58,37 @@ def test_batch_of_one_instance():
     q = RVEProblem(cfg)
     assert p.n_elem == q.n_elem and p.n_dof == q.n_dof
     _check(p, scale=1e-5)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_inverted_child_is_reported_by_both_apis(mode):
+    """A cut element whose first child lists two nodes swapped has a negative
+    child Jacobian: the reference aborts in store_shape_functions
+    (src/Mesh.cpp:1561); here the host API returns PG_E_MESH and the device API
+    reports it through pg_asm_check_error.  A good problem assembled next on
+    the device reports nothing (the flag belongs to one assembly)."""
+    from pigfem_amd import DeviceBuffer, PgError
+    p = RVEProblem(small_config("c0", n=12, n_incl=1))
+    rec = p.desc.cut[0]
+    c = rec.child_nodes[0]
+    c[1], c[2] = c[2], c[1]
+    assert p.desc.cut[0].child_nodes[0][1] == c[1]
+    U = p.random_u(1)
+    asm = AssemblerGPU(p, mode)
+    K, P = PMatrix(p), PVector(p)
+    print(f"inverted child: {mode}", flush=True)
+    with pytest.raises(PgError, match=r"\(-4\)"):
+        asm.assemble(K, P, 0.0, np.zeros(0), U, True, True)
+    bufs = [DeviceBuffer.from_numpy(U), None, None, DeviceBuffer.zeros(p.nnz_total), DeviceBuffer.zeros(p.n_dof)]
+    asm.assemble_device(*bufs)
+    with pytest.raises(PgError, match=r"\(-4\)"):
+        asm.check_error()
+    q = RVEProblem(small_config("c0", n=12, n_incl=1))
+    good = AssemblerGPU(q, mode)
+    good.assemble_device(*bufs)
+    good.check_error()
+    # and the handle that failed recovers once its flag is cleared by the next
+    # assembly of a good description: a second handle on the bad one fails again
+    asm.assemble_device(*bufs)
+    with pytest.raises(PgError, match=r"\(-4\)"):
+        asm.check_error()

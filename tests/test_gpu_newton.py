@@ -65,3 +65,37 @@ def test_newton_c1_ortiz_pandolfi_to_two_percent():
     ref = runs["oracle"].iteration_counts()
     for mode in ("fp64", "binned64", "mixed", "mixed_cut64"):
         assert runs[mode].iteration_counts() == ref, mode
+
+
+def _implicit(counts):
+    return [tuple(c) for c in counts if c[1] != 0]
+
+
+@pytest.mark.parametrize("case,mode", [("c5x4", "fp64"), ("c5x4", "mixed"), ("c2", "fp64"), ("c2", "mixed")])
+def test_newton_history_matches_the_committed_fixture(case, mode):
+    """The load-step histories of the 500 x 500 bilinear RVE (10 steps to 2 %)
+    and of the four smallest RVEs of the batch, assembled on the GPU, against
+    the oracle's committed histories (tests/golden/newton_history.py).  Every
+    schedule repeats every implicit attempt's outcome and every converged
+    step's iteration count.  An attempt that fails by overflow (a non-finite
+    residual) may end one iteration apart, and the number of explicit steps
+    before the bilinear contact branch overflows may differ: last-bit rounding
+    decides both, and the host solve's rounding already depends on its thread
+    count (the fixture was recorded on 8 threads).  On the 500 x 500 RVE the
+    FP64 drop-in repeats the whole history, explicit steps included."""
+    import json
+    from tests.golden.newton_history import compare, fixture, problem, run
+    from pigfem_amd.newton import gpu_assemble_fn
+    ref = json.load(open(fixture(case)))
+    prob = problem(case)
+    print(f"newton {case} {mode}: {prob.stats()}", flush=True)
+    out = run(prob, gpu_assemble_fn(AssemblerGPU(prob, mode)), mode)
+    print(f"newton {case} {mode}: {out['status']} at t = {out['t_reached']} ({out['seconds']:.0f}s); "
+          f"{compare(ref, out)}", flush=True)
+    assert out["status"].split(":")[0] == ref["status"].split(":")[0]
+    a, b = _implicit(ref["counts"]), _implicit(out["counts"])
+    assert [x[:3] for x in a] == [x[:3] for x in b]
+    assert [x[3] for x in a if x[2]] == [x[3] for x in b if x[2]]
+    if case == "c2" and mode == "fp64":
+        assert compare(ref, out)["identical"]
+        assert out["t_reached"] == pytest.approx(ref["t_reached"], rel=1e-12)

@@ -58,3 +58,31 @@ def test_newton_with_the_gpu_solve_repeats_the_oracle(kind, strain):
     print(kind, ref.iteration_counts(), d.iteration_counts(), flush=True)
     assert d.iteration_counts() == ref.iteration_counts()
     assert np.linalg.norm(d.U - ref.U) <= 1e-8 * np.linalg.norm(ref.U)
+
+
+@pytest.mark.gpu
+def test_pcg_reports_an_indefinite_matrix_at_once():
+    """r.z or p.Ap of the first iterations turns negative on an indefinite
+    matrix: the solve returns PG_E_ARG at once instead of iterating to its limit."""
+    import scipy.sparse as sp
+    n = 64
+    d = np.ones(n)
+    d[1::2] = -1.0
+    A = sp.diags([d, 0.1 * np.ones(n - 1), 0.1 * np.ones(n - 1)], [0, -1, 1], format="csr")
+    b = np.random.default_rng(1).standard_normal(n)
+    with pytest.raises(PgError, match="broke down"):
+        pcg_solve(A, b, max_iter=10 * n)
+
+
+@pytest.mark.gpu
+def test_pcg_rejects_non_finite_input():
+    import scipy.sparse as sp
+    A = sp.identity(8, format="csr")
+    b = np.ones(8)
+    b[3] = np.nan
+    with pytest.raises(PgError, match="not finite"):
+        pcg_solve(A, b)
+    A2 = sp.identity(8, format="csr") * 1.0
+    A2.data[2] = np.inf
+    with pytest.raises(PgError):
+        pcg_solve(A2, np.ones(8))

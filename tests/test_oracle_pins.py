@@ -530,3 +530,23 @@ def test_cohesive_integration_with_a_linear_law(kind):
         assert np.abs(Pp).max() <= 1e-9 * np.abs(expect).max()
         checked += 1
     assert checked >= 5
+
+
+@pytest.mark.parametrize("kind", ["c0", "op", "bl", "tet"])
+def test_range_assembly_in_the_harness_pattern_equals_the_element_loop(kind):
+    """The oracle's range-by-range mode (used at full sizes) writes into the
+    harness's pattern, which equals its own insertion pattern; serially it
+    gives the whole element loop's K, P and ISVs bit for bit, colour by colour
+    to rounding."""
+    from oracle.oracle import assemble_in_ranges
+    p = RVEProblem(small_config(kind, n=20 if kind != "tet" else 6, n_incl=3 if kind != "tet" else 1))
+    o = Oracle(p)
+    U = p.random_u(1, u_scale(p.cfg))
+    ic = np.random.default_rng(0).uniform(0, 1e-4, p.n_cqp)
+    K, P, I = o.assemble(U, ic)
+    assert all(np.array_equal(a, b) for a, b in zip(o.rp, p.rp))
+    assert all(np.array_equal(a, b) for a, b in zip(o.ci, p.ci))
+    K2, P2, I2 = assemble_in_ranges(p, U, ic, chunk=97)
+    assert np.array_equal(K, K2) and np.array_equal(P, P2) and np.array_equal(I, I2)
+    K3, P3, I3 = assemble_in_ranges(p, U, ic, chunk=97, threads=4)
+    assert rel(K3, K) < 1e-14 and rel(P3, P) < 1e-14 and np.array_equal(I3, I)
