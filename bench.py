@@ -6,12 +6,15 @@ One step = one full K (4 CSR blocks) + R assembly with inputs resident in HBM.
 
 Prints ONE JSON line: value = elements/s of the mixed-precision class-binned
 schedule (FP32 tangents, FP64 damage / sums / residual); the plain FP64 schedule,
-the speed-up, K and R relative errors (mixed vs FP64), the roofline of the
-dominant kernel and the CPU baseline are reported beside it.
+the binned FP64 schedule, the speed-ups, K and R relative errors (mixed vs FP64),
+the roofline of the dominant kernel and the CPU baseline are reported beside it.
+The 3D cube (C4) and the batch of 256 RVEs (C5) are then measured the same way
+and reported under their own keys (--extra '' skips them).
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -39,40 +42,35 @@ def workload(cfg, batch, uscale):
             f"{law}, U~U[-{uscale:g},{uscale:g}]")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="C3")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
-    ap.add_argument("--tiles", default=None,
-                    help="group_nodes,cut_variant,group_variant for every schedule instead of autotuning "
-                         "(short profiler runs)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only",
-              file=sys.stderr)
-        sys.exit(2)
-
+def build_problem(name):
     import dataclasses
-    import numpy as np
-    from pigfem_amd import AssemblerGPU, DeviceBuffer, RVEProblem, batch_config, config, synchronize
-
-    batch = args.config.upper() == "C5"
+    from pigfem_amd import RVEProblem, batch_config, config
     t0 = time.time()
-    if batch:
+    if name.upper() == "C5":
         cfgs = batch_config(256)
         prob = RVEProblem.batch(cfgs)
         cfg = dataclasses.replace(cfgs[0], name="C5")
     else:
-        cfg = config(args.config)
+        cfgs = None
+        cfg = config(name)
         prob = RVEProblem(cfg)
     setup_s = time.time() - t0
-    st = prob.stats()
-    log(f"[bench] {cfg.name} built in {setup_s:.1f}s: {st}")
+    log(f"[bench] {cfg.name} built in {setup_s:.1f}s: {prob.stats()}")
+    return prob, cfg, cfgs, setup_s
+
+
+def kernel_name(dim, g, tv, lists):
+    per = (8 if dim == 2 else 16) * (2 if tv else 1) * g
+    bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
+    return f"k_pair_{'lists' if lists else 'group'}<{dim}, float, {g}, {tv}, {bs}>"
+
+
+def measure(prob, cfg, args, full):
+    """Time the schedules on one problem; full adds the FP64-cut variant and the
+    host-buffer API.  Returns a dict of results (the caller frees the problem)."""
+    import numpy as np
+    from pigfem_amd import AssemblerGPU, DeviceBuffer, synchronize
+    from pigfem_amd.autotune import tune
 
     asm = AssemblerGPU(prob, "mixed")
     # bilinear law: openings beyond ~1e-4 in compression overflow the contact
@@ -87,15 +85,15 @@ def main():
     ic = dIc if prob.n_cqp else None
     it = dIt if prob.n_cqp else None
 
-    from pigfem_amd.autotune import tune
+    binned_modes = ("binned64", "mixed", "mixed_cut64") if full else ("binned64", "mixed")
     tuned = {}
-    for mode in ("binned64", "mixed", "mixed_cut64"):
+    for mode in binned_modes:
         asm.mode = mode
         if args.tiles:
             tuned[mode] = tuple(int(x) for x in args.tiles.split(","))
         else:
             tuned[mode] = tune(asm, U, use_cache=False, log=log)[0]
-        log(f"[bench] tuned {mode}: group_nodes, cut_variant, group_variant = {tuned[mode]}")
+        log(f"[bench] {cfg.name} tuned {mode}: group_nodes, cut_variant, group_variant = {tuned[mode]}")
 
     def run(mode, steps):
         asm.mode = mode
@@ -105,15 +103,16 @@ def main():
             asm.assemble_device(dU, ic, it, dK, dP)
 
     results = {}
-    for mode in ("fp64", "binned64", "mixed", "mixed_cut64"):
+    for mode in ("fp64",) + binned_modes:
         run(mode, args.warmup)
         synchronize()
         t = time.perf_counter()
         run(mode, args.steps)
         synchronize()
         ms = (time.perf_counter() - t) * 1e3 / args.steps
+        asm.check_error()
         results[mode] = {"ms": ms, "eps": prob.n_elem / (ms * 1e-3)}
-        log(f"[bench] {mode}: {ms:.3f} ms/step, {results[mode]['eps']:.4g} elements/s")
+        log(f"[bench] {cfg.name} {mode}: {ms:.3f} ms/step, {results[mode]['eps']:.4g} elements/s")
 
     # accuracy of the mixed schedule against the FP64 schedule on the same inputs
     def snapshot(mode):
@@ -126,53 +125,22 @@ def main():
                 dIt.download(prob.n_cqp) if prob.n_cqp else np.zeros(0))
     K64, P64, I64 = snapshot("fp64")
     K32, P32, I32 = snapshot("mixed")
-    relK = float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64))
-    relP = float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64))
-    isv_diff = float(np.abs(I32 - I64).max()) if prob.n_cqp else 0.0
-    log(f"[bench] mixed vs fp64: rel K {relK:.3e}, rel P {relP:.3e}, max |isv diff| {isv_diff:.3e}")
+    acc = {"rel_err_K": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
+           "rel_err_P": float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64)),
+           "isv_max_abs_diff": float(np.abs(I32 - I64).max()) if prob.n_cqp else 0.0}
+    log(f"[bench] {cfg.name} mixed vs fp64: rel K {acc['rel_err_K']:.3e}, rel P {acc['rel_err_P']:.3e}, "
+        f"max |isv diff| {acc['isv_max_abs_diff']:.3e}")
     del K32, P32, I32
-    Kc, Pc_, _ = snapshot("mixed_cut64")
-    relK_c64 = float(np.linalg.norm(Kc - K64) / np.linalg.norm(K64))
-    relP_c64 = float(np.linalg.norm(Pc_ - P64) / np.linalg.norm(P64))
-    log(f"[bench] mixed_cut64 vs fp64: rel K {relK_c64:.3e}, rel P {relP_c64:.3e}")
-    del K64, P64, I64, Kc, Pc_
+    if full:
+        Kc, Pc_, _ = snapshot("mixed_cut64")
+        acc["mixed_cut64_rel_err_K"] = float(np.linalg.norm(Kc - K64) / np.linalg.norm(K64))
+        acc["mixed_cut64_rel_err_P"] = float(np.linalg.norm(Pc_ - P64) / np.linalg.norm(P64))
+        log(f"[bench] {cfg.name} mixed_cut64 vs fp64: rel K {acc['mixed_cut64_rel_err_K']:.3e}, "
+            f"rel P {acc['mixed_cut64_rel_err_P']:.3e}")
+        del Kc, Pc_
+    del K64, P64, I64
 
-    per_instance = None
-    if batch:
-        # each instance assembled on its own (own handle, same schedule and tiles):
-        # what a loop over 256 separate assemblies would achieve
-        inst_ms, inst_el = [], []
-        for i, c in enumerate(cfgs):
-            pi = RVEProblem(c)
-            ai = AssemblerGPU(pi, "mixed")
-            ai.set_tiles(*tuned["mixed"])
-            bu = [DeviceBuffer.from_numpy(pi.random_u(i, uscale)), DeviceBuffer.zeros(max(pi.n_cqp, 1)),
-                  DeviceBuffer.zeros(max(pi.n_cqp, 1)), DeviceBuffer.zeros(pi.nnz_total), DeviceBuffer.zeros(pi.n_dof)]
-            for _ in range(2):
-                ai.assemble_device(*bu)
-            synchronize()
-            t = time.perf_counter()
-            for _ in range(5):
-                ai.assemble_device(*bu)
-            synchronize()
-            inst_ms.append((time.perf_counter() - t) * 1e3 / 5)
-            inst_el.append(pi.n_elem)
-            ai.close()
-            del bu, pi
-        eps_i = np.array(inst_el) / (np.array(inst_ms) * 1e-3)
-        per_instance = {
-            "instances": len(cfgs), "elements_min": int(min(inst_el)), "elements_max": int(max(inst_el)),
-            "elements_per_s_min": float(eps_i.min()), "elements_per_s_median": float(np.median(eps_i)),
-            "elements_per_s_max": float(eps_i.max()),
-            "sum_of_separate_ms": float(sum(inst_ms)),
-            "separate_aggregate_elements_per_s": float(sum(inst_el) / (sum(inst_ms) * 1e-3)),
-            "batched_speedup_vs_separate": float(sum(inst_ms) / results["mixed"]["ms"]),
-        }
-        log(f"[bench] separate instances: {sum(inst_ms):.2f} ms total, per-instance elements/s "
-            f"min {eps_i.min():.3g} median {np.median(eps_i):.3g} max {eps_i.max():.3g}; "
-            f"batched launch {results['mixed']['ms']:.2f} ms")
-
-    # per-kernel durations with HIP events on the launch stream (mixed schedule)
+    # per-kernel durations with HIP events on the launch streams (mixed schedule)
     asm.mode = "mixed"
     asm.set_tiles(*tuned["mixed"])
     asm.set_timing(True)
@@ -189,79 +157,18 @@ def main():
     pair_bytes = B_alg - cut_bytes
     achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
     whole = B_alg / (results["mixed"]["ms"] * 1e-3) / 1e9
-    log(f"[bench] events: permute+cut {pre_ms:.3f} ms, node-pair {pair_ms:.3f} ms (overlapped groups "
+    log(f"[bench] {cfg.name} events: permute+cut {pre_ms:.3f} ms, node-pair {pair_ms:.3f} ms (overlapped groups "
         f"{clean_ms:.3f} + remaining {dirty_ms:.3f}) -> {achieved:.0f} GB/s (algorithmic {pair_bytes/1e9:.3f} GB); "
         f"whole assembly {whole:.0f} GB/s")
 
-    # host-buffer API (PCIe-inclusive: U in, K / P / ISVs out), mixed schedule
-    from pigfem_amd import PMatrix, PVector
-    Kh, Ph = PMatrix(prob), PVector(prob)
-    ith = np.zeros(prob.n_cqp)
-    asm.assemble(Kh, Ph, 0.0, ith, U, True, True)
-    t = time.perf_counter()
-    for _ in range(3):
-        asm.assemble(Kh, Ph, 0.0, ith, U, True, True)
-    host_ms = (time.perf_counter() - t) * 1e3 / 3
-    dev_bytes = asm.device_bytes()
-    log(f"[bench] host-buffer API: {host_ms:.2f} ms per call (PCIe-inclusive); device memory held {dev_bytes/1e9:.2f} GB")
-    del Kh, Ph, ith
-
-    if args.calib_copy:
-        a = DeviceBuffer(1 << 30)
-        b = DeviceBuffer(1 << 30)
-        for _ in range(3):
-            b.copy_from(a)
-        synchronize()
-        log("[bench] calibration copy of 1 GiB done (3x)")
-
-    cpu = cpu_threads = None
-    if not args.skip_cpu:
-        from oracle.oracle import Oracle
-        # size the element range for ~cpu_seconds of work from a short probe
-        probe = min(prob.n_elem, 20000)
-        o = Oracle(prob, 0, probe)
-        t = time.perf_counter()
-        o.assemble(U, np.zeros(prob.n_cqp))
-        rate = probe / (time.perf_counter() - t)
-        o.close()
-        m = int(min(prob.n_elem, 4_000_000, max(probe, rate * args.cpu_seconds)))
-        o = Oracle(prob, 0, m)
-        best = 1e30
-        for _ in range(2):
-            t = time.perf_counter()
-            o.assemble(U, np.zeros(prob.n_cqp))
-            best = min(best, time.perf_counter() - t)
-        cpu = {"value": m / best, "unit": "elements/s", "cores": 1, "kind": "port",
-               "sample": f"first {m} elements (element-loop order) of the {cfg.name} RVE, K+R, FP64, "
-                         f"best of 2 single-thread runs of the oracle restatement"}
-        log(f"[bench] cpu baseline: {cpu['value']:.4g} elements/s over {m} elements")
-        # the same sample on every host thread the run may use, element colour
-        # by colour (elements of one colour share no node)
-        nt = max(1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
-        o.assemble(U, np.zeros(prob.n_cqp), threads=nt)  # colouring + warm-up
-        best_t = 1e30
-        for _ in range(3):
-            t = time.perf_counter()
-            o.assemble(U, np.zeros(prob.n_cqp), threads=nt)
-            best_t = min(best_t, time.perf_counter() - t)
-        o.close()
-        cpu_threads = {"value": m / best_t, "unit": "elements/s", "cores": nt, "kind": "port",
-                       "sample": f"the same {m} elements, colour by colour on {nt} OpenMP threads, best of 3"}
-        log(f"[bench] cpu baseline, {nt} threads: {cpu_threads['value']:.4g} elements/s")
-
-    # HBM traffic of the same kernel instantiation from the committed rocprofv3
-    # counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present
-    ng, tv, lists = tuned["mixed"][0], tuned["mixed"][2] & 1, prob.dim == 3 or bool(tuned["mixed"][2] & 2)
-
-    def kernel_name(g):
-        per = (8 if prob.dim == 2 else 16) * (2 if tv else 1) * g
-        bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
-        return f"k_pair_{'lists' if lists else 'group'}<{prob.dim}, float, {g}, {tv}, {bs}>"
-
+    tm = tuned["mixed"]
+    ng, tv, lists = tm[0], tm[2] & 1, prob.dim == 3 or bool(tm[2] & 2)
     # group variant bits 5-6: the groups after the cut kernels may have another size
-    rsel = (tuned["mixed"][2] >> 5) & 3
+    rsel = (tm[2] >> 5) & 3
     ng_rest = ((8, 16, 32) if prob.dim == 2 else (2, 4, 8))[rsel - 1] if rsel else ng
-    kname = kernel_name(ng) if ng_rest == ng else f"{kernel_name(ng)} + {kernel_name(ng_rest)}"
+    kname = kernel_name(prob.dim, ng, tv, lists)
+    if ng_rest != ng:
+        kname += " + " + kernel_name(prob.dim, ng_rest, tv, lists)
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):
@@ -270,49 +177,216 @@ def main():
                 name, val = line.split(":", 1)[1].rsplit("=", 1)
                 if name.strip() == kname:
                     traffic = int(val.split()[0])
+
+    out = {"results": results, "acc": acc, "tuned": tuned, "uscale": uscale,
+           "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                        "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms,
+                        "kernel_ms_overlapped_groups": clean_ms, "kernel_ms_remaining_groups": dirty_ms},
+           "whole_GBps": whole, "B_alg": B_alg, "cut_kernel_ms": pre_ms, "device_bytes": asm.device_bytes(), "U": U}
+
+    if full:
+        # host-buffer API (PCIe-inclusive: U in, K / P / ISVs out), mixed schedule
+        from pigfem_amd import PMatrix, PVector
+        Kh, Ph = PMatrix(prob), PVector(prob)
+        ith = np.zeros(prob.n_cqp)
+        asm.assemble(Kh, Ph, 0.0, ith, U, True, True)
+        t = time.perf_counter()
+        for _ in range(3):
+            asm.assemble(Kh, Ph, 0.0, ith, U, True, True)
+        out["host_ms"] = (time.perf_counter() - t) * 1e3 / 3
+        log(f"[bench] {cfg.name} host-buffer API: {out['host_ms']:.2f} ms per call (PCIe-inclusive); "
+            f"device memory held {out['device_bytes']/1e9:.2f} GB")
+        del Kh, Ph, ith
+    asm.close()
+    for b in (dU, dIc, dIt, dK, dP):
+        b.free()
+    return out
+
+
+def per_instance_rates(cfgs, tuned, uscale, batched_ms):
+    """Each instance of the batch assembled on its own (own handle, same
+    schedule and tiles): what a loop over 256 separate assemblies achieves."""
+    import numpy as np
+    from pigfem_amd import AssemblerGPU, DeviceBuffer, RVEProblem, synchronize
+    inst_ms, inst_el = [], []
+    for i, c in enumerate(cfgs):
+        pi = RVEProblem(c)
+        ai = AssemblerGPU(pi, "mixed")
+        ai.set_tiles(*tuned)
+        bu = [DeviceBuffer.from_numpy(pi.random_u(i, uscale)), DeviceBuffer.zeros(max(pi.n_cqp, 1)),
+              DeviceBuffer.zeros(max(pi.n_cqp, 1)), DeviceBuffer.zeros(pi.nnz_total), DeviceBuffer.zeros(pi.n_dof)]
+        for _ in range(2):
+            ai.assemble_device(*bu)
+        synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            ai.assemble_device(*bu)
+        synchronize()
+        inst_ms.append((time.perf_counter() - t) * 1e3 / 5)
+        inst_el.append(pi.n_elem)
+        ai.close()
+        del bu, pi
+    eps_i = np.array(inst_el) / (np.array(inst_ms) * 1e-3)
+    log(f"[bench] separate instances: {sum(inst_ms):.2f} ms total, per-instance elements/s "
+        f"min {eps_i.min():.3g} median {np.median(eps_i):.3g} max {eps_i.max():.3g}; batched launch {batched_ms:.2f} ms")
+    return {
+        "instances": len(cfgs), "elements_min": int(min(inst_el)), "elements_max": int(max(inst_el)),
+        "elements_per_s_min": float(eps_i.min()), "elements_per_s_median": float(np.median(eps_i)),
+        "elements_per_s_max": float(eps_i.max()),
+        "sum_of_separate_ms": float(sum(inst_ms)),
+        "separate_aggregate_elements_per_s": float(sum(inst_el) / (sum(inst_ms) * 1e-3)),
+        "batched_speedup_vs_separate": float(sum(inst_ms) / batched_ms),
+    }
+
+
+def cpu_baselines(prob, cfg, U, args):
+    import numpy as np
+    from oracle.oracle import Oracle
+    # size the element range for ~cpu_seconds of work from a short probe
+    probe = min(prob.n_elem, 20000)
+    o = Oracle(prob, 0, probe)
+    t = time.perf_counter()
+    o.assemble(U, np.zeros(prob.n_cqp))
+    rate = probe / (time.perf_counter() - t)
+    o.close()
+    m = int(min(prob.n_elem, 4_000_000, max(probe, rate * args.cpu_seconds)))
+    o = Oracle(prob, 0, m)
+    best = 1e30
+    for _ in range(2):
+        t = time.perf_counter()
+        o.assemble(U, np.zeros(prob.n_cqp))
+        best = min(best, time.perf_counter() - t)
+    cpu = {"value": m / best, "unit": "elements/s", "cores": 1, "kind": "port",
+           "sample": f"first {m} elements (element-loop order) of the {cfg.name} RVE, K+R, FP64, "
+                     f"best of 2 single-thread runs of the oracle restatement"}
+    log(f"[bench] cpu baseline: {cpu['value']:.4g} elements/s over {m} elements")
+    # the same sample on every host thread the run may use, element colour by
+    # colour (elements of one colour share no node)
+    nt = max(1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
+    o.assemble(U, np.zeros(prob.n_cqp), threads=nt)  # colouring + warm-up
+    best_t = 1e30
+    for _ in range(3):
+        t = time.perf_counter()
+        o.assemble(U, np.zeros(prob.n_cqp), threads=nt)
+        best_t = min(best_t, time.perf_counter() - t)
+    o.close()
+    cpu_threads = {"value": m / best_t, "unit": "elements/s", "cores": nt, "kind": "port",
+                   "sample": f"the same {m} elements, colour by colour on {nt} OpenMP threads, best of 3"}
+    log(f"[bench] cpu baseline, {nt} threads: {cpu_threads['value']:.4g} elements/s")
+    return cpu, cpu_threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C3")
+    ap.add_argument("--extra", default="C4,C5", help="further workloads measured after the main one ('' for none)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--per-instance", action="store_true", help="C5 main config: also time the instances one by one")
+    ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--tiles", default=None,
+                    help="group_nodes,cut_variant,group_variant for every schedule instead of autotuning "
+                         "(short profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only",
+              file=sys.stderr)
+        sys.exit(2)
+
+    prob, cfg, cfgs, setup_s = build_problem(args.config)
+    batch = cfgs is not None
+    st = prob.stats()
+    m = measure(prob, cfg, args, full=True)
+    res = m["results"]
+    per_instance = None
+    if batch and args.per_instance:
+        per_instance = per_instance_rates(cfgs, m["tuned"]["mixed"], m["uscale"], res["mixed"]["ms"])
+
+    if args.calib_copy:
+        from pigfem_amd import DeviceBuffer, synchronize
+        a = DeviceBuffer(1 << 30)
+        b = DeviceBuffer(1 << 30)
+        for _ in range(3):
+            b.copy_from(a)
+        synchronize()
+        a.free()
+        b.free()
+        log("[bench] calibration copy of 1 GiB done (3x)")
+
+    cpu = cpu_threads = None
+    if not args.skip_cpu:
+        cpu, cpu_threads = cpu_baselines(prob, cfg, m["U"], args)
+
     out = {
         "metric": "K+R assembled elements/s, % HBM peak; FP32+batched vs FP64 at <=1e-6 rel err",
-        "value": results["mixed"]["eps"],
+        "value": res["mixed"]["eps"],
         "unit": "elements/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": results["mixed"]["ms"],
+        "ms_per_step": res["mixed"]["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": workload(cfg, batch, uscale),
+        "config": {"workload": workload(cfg, batch, m["uscale"]),
                    "elements": prob.n_elem, "cut_elements": prob.n_cut, "cohesive_qps": prob.n_cqp,
                    "dofs": prob.n_dof, "nnz": prob.nnz_total, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms,
-                     "kernel_ms_overlapped_groups": clean_ms, "kernel_ms_remaining_groups": dirty_ms},
+        "roofline": m["roofline"],
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_threads,
-        "fp64_elements_per_s": results["fp64"]["eps"],
-        "fp64_ms_per_step": results["fp64"]["ms"],
-        "speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],
-        "binned64_elements_per_s": results["binned64"]["eps"],
-        "mixed_cut64_elements_per_s": results["mixed_cut64"]["eps"],
-        "mixed_cut64_ms_per_step": results["mixed_cut64"]["ms"],
-        "mixed_cut64_rel_err_K": relK_c64,
-        "mixed_cut64_rel_err_P": relP_c64,
-        "binned64_ms_per_step": results["binned64"]["ms"],
-        "rel_err_K": relK,
-        "rel_err_P": relP,
-        "whole_assembly_GBps": whole,
-        "whole_assembly_frac": whole / HBM_PEAK_GBS,
-        "algorithmic_bytes": B_alg,
+        "fp64_elements_per_s": res["fp64"]["eps"],
+        "fp64_ms_per_step": res["fp64"]["ms"],
+        "speedup_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "speedup_vs_binned64": res["binned64"]["ms"] / res["mixed"]["ms"],
+        "binned64_elements_per_s": res["binned64"]["eps"],
+        "binned64_ms_per_step": res["binned64"]["ms"],
+        "mixed_cut64_elements_per_s": res["mixed_cut64"]["eps"],
+        "mixed_cut64_ms_per_step": res["mixed_cut64"]["ms"],
+        "mixed_cut64_rel_err_K": m["acc"]["mixed_cut64_rel_err_K"],
+        "mixed_cut64_rel_err_P": m["acc"]["mixed_cut64_rel_err_P"],
+        "rel_err_K": m["acc"]["rel_err_K"],
+        "rel_err_P": m["acc"]["rel_err_P"],
+        "isv_max_abs_diff": m["acc"]["isv_max_abs_diff"],
+        "whole_assembly_GBps": m["whole_GBps"],
+        "whole_assembly_frac": m["whole_GBps"] / HBM_PEAK_GBS,
+        "algorithmic_bytes": m["B_alg"],
+        "cut_kernel_ms": m["cut_kernel_ms"],
         "setup_s": setup_s,
-        "host_api_ms_per_call": host_ms,
-        "device_bytes": dev_bytes,
-        "tiles": {"mixed": tuned["mixed"], "binned64": tuned["binned64"], "mixed_cut64": tuned["mixed_cut64"]},
+        "host_api_ms_per_call": m["host_ms"],
+        "device_bytes": m["device_bytes"],
+        "tiles": m["tuned"],
     }
     if per_instance:
         out["per_instance"] = per_instance
+    del prob, m
+    gc.collect()
+
+    # further workloads, measured the same way (mixed, binned FP64, plain FP64)
+    for name in [x.strip().upper() for x in args.extra.split(",") if x.strip()]:
+        if name == args.config.upper():
+            continue
+        p2, c2, cf2, s2 = build_problem(name)
+        m2 = measure(p2, c2, args, full=False)
+        r2 = m2["results"]
+        out[name] = {
+            "workload": workload(c2, cf2 is not None, m2["uscale"]),
+            "elements": p2.n_elem, "cut_elements": p2.n_cut, "nnz": p2.nnz_total,
+            "ms_per_step": r2["mixed"]["ms"], "elements_per_s": r2["mixed"]["eps"],
+            "fp64_ms_per_step": r2["fp64"]["ms"], "binned64_ms_per_step": r2["binned64"]["ms"],
+            "speedup_vs_fp64": r2["fp64"]["ms"] / r2["mixed"]["ms"],
+            "speedup_vs_binned64": r2["binned64"]["ms"] / r2["mixed"]["ms"],
+            "rel_err_K": m2["acc"]["rel_err_K"], "rel_err_P": m2["acc"]["rel_err_P"],
+            "roofline": m2["roofline"], "whole_assembly_frac": m2["whole_GBps"] / HBM_PEAK_GBS,
+            "cut_kernel_ms": m2["cut_kernel_ms"], "setup_s": s2, "tiles": m2["tuned"],
+        }
+        del p2, m2
+        gc.collect()
     print(json.dumps(out), flush=True)
 
 
