@@ -246,7 +246,10 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                take twice the threads per node pair whatever bit 0 says;
  *                bits 5-6 = 0: the groups that run after the cut kernels have
  *                group_nodes nodes too, k = 1..3: they have the k-th size of
- *                the list above (8, 16, 32 in 2D; 2, 4, 8 in 3D).
+ *                the list above (8, 16, 32 in 2D; 2, 4, 8 in 3D); bit 7 set =
+ *                the nodes of a structured tri3 grid (generate_mesh's pattern,
+ *                checked at create) take the generic node-group kernel instead
+ *                of the one that computes their records from the grid.
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 
@@ -259,6 +262,14 @@ int64_t pg_asm_algorithmic_bytes(const pg_asm* h);
  * lists, the staging buffers, and the host-API buffers once pg_asm_assemble
  * has been called. */
 int64_t pg_asm_device_bytes(const pg_asm* h);
+
+/* Nodes that the class-binned schedules assemble from the structured-grid
+ * formulas (tri3 meshes in the pattern of the reference's generate_mesh,
+ * src/Mesh.cpp:418-441): regular nodes none of whose elements is cut and whose
+ * incidences, element node order, pattern columns, scatter indices and dofs
+ * pg_asm_create found equal to those formulas.  No counterpart in the
+ * reference (a diagnostic of the schedule). */
+int64_t pg_asm_grid_nodes(const pg_asm* h);
 
 /* Cohesive openings of a 2D cohesive mesh from device data: the data of
  * WriterCohesiveOpenings::writeFromStream (src/WriterCohesiveOpenings.cpp:73-147)

@@ -98,6 +98,7 @@ SIGNATURES = {
     "pg_asm_cohesive_openings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_cohesive_failure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_check_error": (C.c_int, [C.c_void_p]),
+    "pg_asm_grid_nodes": (C.c_int64, [C.c_void_p]),
     "pg_csr_pcg": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                              C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p]),
     "pg_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),

@@ -211,6 +211,10 @@ class AssemblerGPU:
     def device_bytes(self) -> int:
         return int(lib().pg_asm_device_bytes(self._h))
 
+    def grid_nodes(self) -> int:
+        """Nodes assembled from the structured-grid formulas (pg_asm_grid_nodes)."""
+        return int(lib().pg_asm_grid_nodes(self._h))
+
     def close(self):
         if self._h:
             lib().pg_asm_destroy(self._h)

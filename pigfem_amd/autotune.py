@@ -1,10 +1,10 @@
 """Autotuner of the class-binned schedules.
 
 Sweeps the schedule parameters of `pg_asm_set_tiles` -- nodes per block, threads
-per node and form of the node-group kernel, the size of the groups that run
-after the cut kernels, then the variant of the small-class cut kernel (elements
-per wavefront, register cap) -- timing each kernel with HIP events on the
-assembly stream, and keeps the fastest.  Every choice gives
+per node and form of the node-group kernel, overlap, and the variant of the
+small-class cut kernel (elements per wavefront, register cap) jointly, then the
+size of the groups that run after the cut kernels -- timing the whole assembly
+with HIP events, and keeps the fastest.  Every choice gives
 bit-identical results.  Results are cached in a JSON file keyed by the problem
 class (dimension, cohesive law, size bucket) and the mode.
 """
@@ -69,15 +69,16 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
 
     asm.set_timing(True)
     try:
-        # node-group kernel: nodes per block x variant, timed on the whole
-        # assembly (groups of nodes without cut incidences overlap the cut kernel)
+        # node-group size x group variant x cut-kernel variant, each timed on
+        # the whole assembly (the groups of nodes without cut incidences overlap
+        # the cut kernel, so the choices are coupled)
         for g in GROUPS[p.dim]:
             for v in GROUP_VARIANTS[p.dim]:
-                table[f"group {g},{v}"] = timed(g, 0, v, 0)
-                if log:
-                    log(f"[tune] group_nodes={g} group_variant={v}: {table[f'group {g},{v}']:.3f} ms")
-        gbest = min((k for k in table if k.startswith("group")), key=table.get)
-        g, v = (int(x) for x in gbest.split()[1].split(","))
+                for c in CUT_VARIANTS:
+                    table[f"{g},{c},{v}"] = timed(g, c, v, 0)
+                    if log:
+                        log(f"[tune] group_nodes={g} cut_variant={c} group_variant={v}: {table[f'{g},{c},{v}']:.3f} ms")
+        g, c, v = (int(x) for x in min(table, key=table.get).split(","))
         # the groups that wait for the cut kernels may take another size
         # (group variant bits 5-6: 1 + index of the size)
         if not v & 4:
@@ -85,17 +86,10 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
                 if g2 == g:
                     continue
                 v2 = v | ((k + 1) << 5)
-                table[f"group {g},{v2}"] = timed(g, 0, v2, 0)
+                table[f"{g},{c},{v2}"] = timed(g, c, v2, 0)
                 if log:
-                    log(f"[tune] group_nodes={g} group_variant={v2}: {table[f'group {g},{v2}']:.3f} ms")
-            gbest = min((k for k in table if k.startswith("group")), key=table.get)
-            g, v = (int(x) for x in gbest.split()[1].split(","))
-        # cut-element kernel variants, timed on the whole assembly
-        for c in CUT_VARIANTS:
-            table[f"cut {c}"] = timed(g, c, v, 0)
-            if log:
-                log(f"[tune] cut_variant={c}: {table[f'cut {c}']:.3f} ms")
-        c = int(min((k for k in table if k.startswith("cut")), key=table.get).split()[1])
+                    log(f"[tune] group_nodes={g} cut_variant={c} group_variant={v2}: {table[f'{g},{c},{v2}']:.3f} ms")
+            g, c, v = (int(x) for x in min(table, key=table.get).split(","))
     finally:
         asm.set_timing(False)
         synchronize()

@@ -1657,6 +1657,185 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
     }
 }
 
+// Structured tri3 grid of the reference's generate_mesh (src/Mesh.cpp:418-441):
+// cell (cx, cy) of an n x n grid (node (x, y) = x + y (n + 1), elements
+// 2 (cx + cy n) and 2 (cx + cy n) + 1) is split along (x, y)-(x+1, y+1) when
+// cx + cy is even and along (x+1, y)-(x, y+1) otherwise.  An interior node
+// with x + y even has 8 incident elements and 9 node pairs, one with x + y odd
+// 4 and 5.  GridTpl holds, for both parities, what the node-group kernel
+// otherwise reads from its incidence records, slot words and pair tables: per
+// incidence in ascending element id (the reference's loop order) its cell
+// offset and half, its element's nodes owner first as grid offsets, and the
+// (slot, rank) fields of the two other nodes; per node pair in ascending
+// column node id the column's grid offset.  pg_asm_create accepts a node for
+// this path only after checking its incidences, element node order, pattern
+// columns, scatter indices and dofs against these formulas.
+struct GridTpl {
+    int8_t pdx[2][9], pdy[2][9];           // column node of slot s
+    int8_t ecx[2][8], ecy[2][8], eh[2][8];  // cell offset and half of incidence r
+    int8_t ndx[2][8][3], ndy[2][8][3];      // the element's nodes, owner first
+    int16_t sr[2][8][2];                    // slot | rank << 6 of owner-first nodes 1 and 2
+};
+static_assert(sizeof(GridTpl) % 4 == 0, "the template is copied to LDS as words");
+constexpr int GRID_NI[2] = {8, 4}, GRID_NP[2] = {9, 5}, GRID_DIAG[2] = {4, 2};
+
+// Node-group kernel for the nodes of the structured grid whose incidences are
+// all uncut (tri3).  Each node comes with one record -- node, index of its first
+// K value (row 0, first column; row 1 follows after the 2 np values of row 0),
+// first dof, element id of its cell (x, y) -- and a word with the grid's row
+// stride S = n + 1 and the parity of x + y.  Everything else is arithmetic, so
+// the loads are the records, then the element data; the arithmetic, the
+// summation order and the stores are those of k_pair_group, bit for bit.
+template <typename TC, int NG, int TV, int BS = GroupThreads<2, NG, TV>::value>
+__global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __restrict__ tpl,
+                                                     const int4* __restrict__ srec, const int32_t* __restrict__ sinf,
+                                                     const double* __restrict__ Un, double* __restrict__ K,
+                                                     double* __restrict__ P, unsigned flags) {
+    constexpr int D = 2, NEN = 3, NQ = 4;
+    constexpr int MAXP = 9 * NG;
+    constexpr int CAPO = 2, CAPD = 8;
+    static_assert(NG <= 32, "the first wavefront holds the group's nodes");
+    __shared__ int s_node[NG], s_kb[NG], s_dof[NG], s_eb[NG], s_inf[NG];
+    __shared__ int s_iptr[NG + 1], s_qptr[NG + 1];
+    __shared__ TC s_blk[MAXP][CAPO][D * D];      // off-diagonal blocks by pair and rank
+    __shared__ double s_dblk[NG][CAPD][D * D];   // diagonal blocks by node and incidence
+    __shared__ double s_dP[NG][CAPD][D];         // internal force by node and incidence
+    __shared__ TC sLM[MAX_LDS_MATS][2];          // lambda, mu of each LEI material
+    __shared__ double sLC[MAX_LDS_MATS][LeiCoef<D>::N];  // LEI stress coefficients
+    __shared__ GridTpl sT;
+
+    const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    const int tid = threadIdx.x;
+    const int grp = xcd_order(blockIdx.x, gridDim.x);
+    // set-up loads, all independent: the group's records, the template and
+    // the materials
+    const int64_t k0 = int64_t(grp) * NG + (tid & (NG - 1));
+    const int4 r = srec[k0];
+    const int inf = sinf[k0];
+    constexpr int TW = int(sizeof(GridTpl) / 4);
+    const int tw = tpl[tid < TW ? tid : TW - 1];
+    stage_lm<TC, BS>(p, 3, sLM);
+    for (int k = tid; k < MAX_LDS_MATS * LeiCoef<D>::N; k += BS) {
+        const int m = k / LeiCoef<D>::N, j = k % LeiCoef<D>::N;
+        const double v = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[LeiCoef<D>::index(j)];
+        sLC[m][j] = m < p.n_mat ? v : 0.0;
+    }
+    if (tid < TW) reinterpret_cast<int32_t*>(&sT)[tid] = tw;
+    const bool live = tid < NG && r.x >= 0;
+    const int par = (inf >> 24) & 1;
+    int ci = live ? GRID_NI[par] : 0, cq = live ? GRID_NP[par] : 0;
+#pragma unroll
+    for (int off = 1; off < NG; off <<= 1) {
+        const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+        if ((tid & 63) >= off) { ci += ti; cq += tq; }
+    }
+    if (tid < NG) {
+        s_node[tid] = r.x;
+        s_kb[tid] = r.y;
+        s_dof[tid] = r.z;
+        s_eb[tid] = r.w;
+        s_inf[tid] = inf;
+        s_iptr[tid + 1] = ci;
+        s_qptr[tid + 1] = cq;
+    }
+    if (tid == 0) {
+        s_iptr[0] = 0;
+        s_qptr[0] = 0;
+    }
+    __syncthreads();
+    const int ni = s_iptr[NG], nq = s_qptr[NG];
+    double wsum = 0.0;
+    for (int k = 0; k < NQ; ++k) wsum += TRI_QW[k];
+
+    // ---- phase A: incidences (as k_pair_group, records from the template) ----
+    for (int i = tid; i < ni; i += BS) {
+        int lo = 0, hi = NG;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_iptr[mid] <= i) lo = mid; else hi = mid;
+        }
+        const int rr = i - s_iptr[lo];
+        const int a = s_node[lo], inf0 = s_inf[lo], pr = (inf0 >> 24) & 1, S = inf0 & 0xffffff;
+        const int e = s_eb[lo] + 2 * (sT.ecx[pr][rr] + sT.ecy[pr][rr] * (S - 1)) + sT.eh[pr][rr];
+        int ln[NEN];
+        for (int n = 0; n < NEN; ++n) ln[n] = a + sT.ndx[pr][rr][n] + sT.ndy[pr][rr][n] * S;
+        double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
+        for (int n = 0; n < NEN; ++n)
+            for (int c = 0; c < D; ++c) {
+                x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                u[n][c] = Un[int64_t(ln[n]) * D + c];
+            }
+        const int mi = p.elem_mat[e] & (MAX_LDS_MATS - 1);
+        // every load of the incidence is out before the arithmetic
+        __builtin_amdgcn_sched_barrier(0);
+        double lc[LeiCoef<D>::N];
+        for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = sLC[mi][k];
+        double g[NEN][D];
+        const double J = simplex_grads<D, double>(x, g);
+        const double w = J * wsum;
+        plain_p<D>(g, u, 0, w, lc, s_dP[lo][rr]);
+        if (!jac) continue;
+        TC gt[NEN][D];
+        for (int n = 0; n < NEN; ++n)
+            for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
+        const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+        const int ps0 = s_qptr[lo];
+        double dd[D][D];
+#pragma unroll
+        for (int k = 0; k < NEN - 1; ++k) {
+            TC blk[D][D];
+            iso_block<D, TC>(gt[0], gt[k + 1], wl, wm, blk);
+            diag_acc<D, TC>(k, blk, dd);
+            const int f = sT.sr[pr][rr][k];
+            const int slot = f & 63, rank = f >> 6;
+            for (int ii = 0; ii < D; ++ii)
+                for (int jj = 0; jj < D; ++jj) s_blk[ps0 + slot][rank][ii * D + jj] = blk[ii][jj];
+        }
+        diag_fin<D, TC>(gt[0], wl, wm, dd);
+        for (int ii = 0; ii < D; ++ii)
+            for (int jj = 0; jj < D; ++jj) s_dblk[lo][rr][ii * D + jj] = dd[ii][jj];
+    }
+    __syncthreads();
+
+    // ---- phase B: node pairs ----
+    for (int kq = tid; kq < nq; kq += BS) {
+        int lo = 0, hi = NG;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
+        }
+        const int slot = kq - s_qptr[lo];
+        const int pr = (s_inf[lo] >> 24) & 1, np = GRID_NP[pr];
+        const bool isdiag = slot == GRID_DIAG[pr];
+        double acc[D][D];
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
+        double pa[D] = {0.0, 0.0};
+        if (isdiag) {
+            const int nd = GRID_NI[pr];
+            for (int rr = 0; rr < nd; ++rr) {
+                if (func)
+                    for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][rr][i];
+                if (jac)
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[lo][rr][i * D + j];
+            }
+        } else if (jac) {
+#pragma unroll
+            for (int rk = 0; rk < CAPO; ++rk)
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[kq][rk][i * D + j]);
+        }
+        if (jac) {
+            const int64_t k0v = int64_t(s_kb[lo]) + 2 * slot;
+            *reinterpret_cast<dpair*>(K + k0v) = dpair{acc[0][0], acc[0][1]};
+            *reinterpret_cast<dpair*>(K + k0v + 2 * np) = dpair{acc[1][0], acc[1][1]};
+        }
+        if (isdiag && func)
+            for (int i = 0; i < D; ++i) P[s_dof[lo] + i] = pa[i];
+    }
+}
+
 // The same schedule for tet4 meshes, where the per-pair block tables would not
 // fit in LDS (and a tri3 variant): phase A stores each incidence's TC
 // gradients, w lambda and w mu, its diagonal block and internal force, and
@@ -1929,6 +2108,11 @@ struct pg_asm {
     int ngroups[3] = {0, 0, 0};     // groups of each list
     int nclean[3] = {0, 0, 0};      // groups without cut incidences (listed first)
     int nenrg[3] = {0, 0, 0};       // groups of enrichment nodes (listed last)
+    int nstruct[3] = {0, 0, 0};     // groups of structured-grid nodes (listed first)
+    int64_t n_struct = 0;           // structured-grid nodes (k_pair_struct records)
+    int32_t* stpl = nullptr;        // GridTpl words
+    int4* srec = nullptr;           // per structured node: node, first K value, first dof, cell element
+    int32_t* sinf = nullptr;        // per structured node: row stride | parity << 24
     bool timing = false;
     double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
@@ -1942,6 +2126,118 @@ struct pg_asm {
 };
 
 namespace {
+
+// GridTpl built from the union-jack rule, with each incidence's element nodes
+// in their local (connectivity) order for the host check.
+struct GridTplHost {
+    GridTpl t;
+    int8_t ldx[2][8][3], ldy[2][8][3];
+};
+
+// The two triangles of cell (cx, cy), nodes in the reference's local order
+// (src/Mesh.cpp:418-441).
+void grid_tri(int cx, int cy, int half, int v[3][2]) {
+    static const int E[2][2][3][2] = {{{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {1, 1}, {0, 1}}},
+                                      {{{0, 0}, {1, 0}, {0, 1}}, {{1, 0}, {1, 1}, {0, 1}}}};
+    const int k = ((cx + cy) % 2 + 2) % 2;
+    for (int m = 0; m < 3; ++m) {
+        v[m][0] = cx + E[k][half][m][0];
+        v[m][1] = cy + E[k][half][m][1];
+    }
+}
+
+bool build_grid_tpl(GridTplHost& H) {
+    memset(&H, 0, sizeof H);
+    for (int par = 0; par < 2; ++par) {
+        const int x = 2 + par, y = 2;
+        int r = 0;
+        std::vector<std::pair<int, int>> nb;  // (dy, dx): ascending node id for any row stride > 2
+        for (int dcy = -1; dcy <= 0; ++dcy)
+            for (int dcx = -1; dcx <= 0; ++dcx)
+                for (int hh = 0; hh < 2; ++hh) {
+                    int v[3][2];
+                    grid_tri(x + dcx, y + dcy, hh, v);
+                    int la = -1;
+                    for (int m = 0; m < 3; ++m)
+                        if (v[m][0] == x && v[m][1] == y) la = m;
+                    if (la < 0) continue;
+                    if (r >= GRID_NI[par]) return false;
+                    H.t.ecx[par][r] = int8_t(dcx);
+                    H.t.ecy[par][r] = int8_t(dcy);
+                    H.t.eh[par][r] = int8_t(hh);
+                    for (int m = 0; m < 3; ++m) {
+                        H.ldx[par][r][m] = int8_t(v[m][0] - x);
+                        H.ldy[par][r][m] = int8_t(v[m][1] - y);
+                        const int mo = owner_first(3, la, m);
+                        H.t.ndx[par][r][m] = int8_t(v[mo][0] - x);
+                        H.t.ndy[par][r][m] = int8_t(v[mo][1] - y);
+                        nb.push_back({v[m][1] - y, v[m][0] - x});
+                    }
+                    ++r;
+                }
+        if (r != GRID_NI[par]) return false;
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        if (int(nb.size()) != GRID_NP[par]) return false;
+        for (int sl = 0; sl < GRID_NP[par]; ++sl) {
+            H.t.pdy[par][sl] = int8_t(nb[size_t(sl)].first);
+            H.t.pdx[par][sl] = int8_t(nb[size_t(sl)].second);
+        }
+        if (H.t.pdx[par][GRID_DIAG[par]] != 0 || H.t.pdy[par][GRID_DIAG[par]] != 0) return false;
+        int cnt[9] = {0};
+        for (int ri = 0; ri < r; ++ri)
+            for (int k = 1; k < 3; ++k) {
+                int sl = 0;
+                while (sl < GRID_NP[par] &&
+                       !(H.t.pdx[par][sl] == H.t.ndx[par][ri][k] && H.t.pdy[par][sl] == H.t.ndy[par][ri][k]))
+                    ++sl;
+                if (sl == GRID_NP[par]) return false;
+                H.t.sr[par][ri][k - 1] = int16_t(sl | (cnt[sl]++ << 6));
+            }
+        // every off-diagonal pair gets exactly ranks 0 and 1 (an interior edge
+        // lies in two triangles), so the kernel needs no cleared table
+        for (int sl = 0; sl < GRID_NP[par]; ++sl)
+            if (cnt[sl] != (sl == GRID_DIAG[par] ? 0 : 2)) return false;
+    }
+    return true;
+}
+
+// Whether regular node a is a node of the structured grid whose incidences,
+// element node order, pattern columns, scatter indices and dofs are exactly
+// what GridTpl gives; then its k_pair_struct record and word.
+bool grid_node(const pg_problem_desc* s, const GridTplHost& H, int64_t a, int32_t rec[4], int32_t& inf) {
+    const int64_t t0 = s->n2e_ptr[a], ni = s->n2e_ptr[a + 1] - t0;
+    const int64_t p0 = s->npat_ptr[a], np = s->npat_ptr[a + 1] - p0;
+    const int par = ni == GRID_NI[0] ? 0 : (ni == GRID_NI[1] ? 1 : -1);
+    if (par < 0 || np != GRID_NP[par]) return false;
+    // the first column is (x - 1, y - 1) (even) or (x, y - 1) (odd): the stride
+    const int64_t S = a + H.t.pdx[par][0] - s->npat_col[p0];
+    if (S < 3 || S >= (int64_t(1) << 24)) return false;
+    const int64_t n = S - 1, rl = ni - 1;
+    const int64_t eb = int64_t(s->n2e[t0 + rl]) - (2 * (H.t.ecx[par][rl] + H.t.ecy[par][rl] * n) + H.t.eh[par][rl]);
+    if (eb < 0) return false;
+    for (int r = 0; r < ni; ++r) {
+        const int64_t e = eb + 2 * (H.t.ecx[par][r] + H.t.ecy[par][r] * n) + H.t.eh[par][r];
+        if (s->n2e[t0 + r] != e || s->cut_of_elem[e] >= 0) return false;
+        for (int m = 0; m < 3; ++m)
+            if (s->conn[e * 3 + m] != a + H.ldx[par][r][m] + H.ldy[par][r][m] * S) return false;
+    }
+    const int64_t kb = s->npat_map[p0 * 4];
+    const int32_t dof = s->node_dof[a * 2];
+    if (s->node_dof[a * 2 + 1] != dof + 1) return false;
+    for (int sl = 0; sl < np; ++sl) {
+        if (s->npat_col[p0 + sl] != a + H.t.pdx[par][sl] + H.t.pdy[par][sl] * S) return false;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                if (s->npat_map[(p0 + sl) * 4 + i * 2 + j] != kb + i * 2 * np + 2 * sl + j) return false;
+    }
+    rec[0] = int32_t(a);
+    rec[1] = int32_t(kb);
+    rec[2] = dof;
+    rec[3] = int32_t(eb);
+    inf = int32_t(S | (int64_t(par) << 24));
+    return true;
+}
 
 int check_desc(const pg_problem_desc* s) {
     if (!s) { g_err = "null description"; return PG_E_ARG; }
@@ -2127,6 +2423,26 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const int nclean = h->nclean[o], nrest = h->ngroups[o] - h->nclean[o];
         const bool overlap = !(h->group_variant & 4) && nclean > 0 && nrest > 0;
         const bool tv0 = (h->group_variant & 1) != 0;
+        // structured-grid nodes through k_pair_struct (off with group variant
+        // bit 7, and with bit 3, which sends every node to the global path);
+        // the generic groups of list 0 are then skipped
+        const bool grid = h->n_struct > 0 && !(h->group_variant & (128 | 8));
+        auto skip = [&](int oo) { return grid ? h->nstruct[oo] : 0; };
+        auto grid_groups = [&](int oo, hipStream_t s, bool tv) {
+            const int ng = h->gopt_ng[oo];
+            const unsigned grid_n = unsigned((h->n_struct + ng - 1) / ng);
+#define STRUCT1(TC, NGV, TVV) \
+    k_pair_struct<TC, NGV, TVV><<<grid_n, GroupThreads<2, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->srec, h->sinf, h->Un, dK, dP, flags)
+#define STRUCT2(TC, NGV) \
+    do { if (tv) STRUCT1(TC, NGV, 1); else STRUCT1(TC, NGV, 0); } while (0)
+#define STRUCT3(TC) \
+    do { if (ng == 8) STRUCT2(TC, 8); else if (ng == 16) STRUCT2(TC, 16); else STRUCT2(TC, 32); } while (0)
+            if (fp32) STRUCT3(float);
+            else STRUCT3(double);
+#undef STRUCT3
+#undef STRUCT2
+#undef STRUCT1
+        };
         // LEN node groups on stream S: node-table entries from BASE, incidence slots from ibase
 #define GROUP1(DD, TC, TKC, KC, NGV, TVV, S, BASE, LEN) \
     launch_group<DD, TC, TKC, NGV, TVV>((h->group_variant & 2) != 0, BASE, ibase, LEN, S, d, h->Un, KC, h->Pc, dK, dP, \
@@ -2174,7 +2490,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             HIP_OK(hipEventRecord(h->sync[0], st));
             HIP_OK(hipStreamWaitEvent(h->side, h->sync[0], 0));
             if (h->timing) HIP_OK(hipEventRecord(h->ev[3], h->side));
-            groups(o, h->side, 0, nclean, tv0);
+            if (grid) grid_groups(o, h->side, tv0);
+            groups(o, h->side, skip(o), nclean - skip(o), tv0);
             if (h->timing) HIP_OK(hipEventRecord(h->ev[4], h->side));
             HIP_OK(hipEventRecord(h->sync[1], h->side));
         }
@@ -2220,7 +2537,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         // the groups of enrichment nodes (the tail of the list) may take twice
         // the threads per pair (group variant bit 4): they have few incidences
         // and many pairs
-        const int first = overlap ? h->nclean[orr] : 0, nenr = h->nenrg[orr];
+        if (!overlap && grid) grid_groups(orr, st, tv0);
+        const int first = overlap ? h->nclean[orr] : skip(orr), nenr = h->nenrg[orr];
         const int nreg = h->ngroups[orr] - first - nenr;
         groups(orr, st, first, nreg, tv0);
         groups(orr, st, first + nreg, nenr, tv0 || (h->group_variant & 16) != 0);
@@ -2364,9 +2682,18 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
     {
-        // node table, and for each group size the list of group starts: groups
-        // none of whose nodes touches a cut element first (they can run while
-        // the cut kernels run), then the others; each list keeps mesh order
+        // Node lists of the node-group kernels, each in mesh order: (0) the
+        // nodes of the structured grid whose incidences are all uncut (tri3;
+        // k_pair_struct takes them from their records), (1) the other regular
+        // nodes none of whose elements is cut, (2) the other regular nodes,
+        // (3) the enrichment nodes.  Lists 0 and 1 need no staged cut tangent,
+        // so their groups run while the cut kernels run.  Each list is cut into
+        // blocks of B nodes (B = the smallest group size; the last block
+        // padded) and a group of NG nodes takes NG / B consecutive blocks of one
+        // list, so every group size covers the same node sets, and the groups
+        // that wait for the cut kernels may use another size than the groups
+        // that overlap them.  The generic tables cover list 0 too (the
+        // structured path can be switched off).
         std::vector<char> dirty(size_t(nall), 0);
         for (int64_t a = 0; a < nall; ++a)
             for (int64_t t = s->n2e_ptr[a]; t < s->n2e_ptr[a + 1] && !dirty[a]; ++t)
@@ -2383,28 +2710,36 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             tab[size_t(a) * 4 + 2] = s->npat_ptr[a];
             tab[size_t(a) * 4 + 3] = ic | (qc << 16);
         }
-        const int opts2[3] = {8, 16, 32}, opts3[3] = {2, 4, 8};
-        // blocks of B consecutive nodes (B = the smallest group size) are the
-        // unit of the partition: blocks of regular nodes none of which touches a
-        // cut element, the other blocks of regular nodes, the blocks of
-        // enrichment nodes, each list in mesh order.  A group of NG nodes takes
-        // NG / B consecutive blocks of one list (the last group of a list is
-        // padded), so every group size covers the same three node sets, and the
-        // groups that wait for the cut kernels may use another size than the
-        // groups that overlap them.
-        const int B = s->dim == 2 ? opts2[0] : opts3[0];
-        std::vector<int64_t> blk[3];
-        for (int64_t a0 = 0; a0 < nall; a0 += B) {
-            bool any = false;
-            for (int64_t a = a0; a < std::min<int64_t>(nall, a0 + B) && !any; ++a) any = dirty[a] != 0;
-            blk[!any ? 0 : (a0 >= s->n_nodes ? 2 : 1)].push_back(a0);
+        std::vector<int32_t> srec_v, sinf_v;
+        std::vector<char> grid(size_t(nall), 0);
+        GridTplHost T;
+        if (s->dim == 2 && build_grid_tpl(T)) {
+            for (int64_t a = 0; a < s->n_nodes; ++a) {
+                int32_t rec[4], inf = 0;
+                if (dirty[a] || !grid_node(s, T, a, rec, inf)) continue;
+                grid[a] = 1;
+                srec_v.insert(srec_v.end(), rec, rec + 4);
+                sinf_v.push_back(inf);
+            }
         }
+        h->n_struct = int64_t(sinf_v.size());
+        // padding records (node -1) up to a whole number of the largest group
+        while (sinf_v.size() % 32) {
+            srec_v.insert(srec_v.end(), {-1, 0, 0, 0});
+            sinf_v.push_back(0);
+        }
+        const int opts2[3] = {8, 16, 32}, opts3[3] = {2, 4, 8};
+        const int B = s->dim == 2 ? opts2[0] : opts3[0];
+        std::vector<int64_t> lst[4];
+        for (int64_t a = 0; a < nall; ++a)
+            lst[grid[a] ? 0 : (a >= s->n_nodes ? 3 : (dirty[a] ? 2 : 1))].push_back(a);
         std::vector<int32_t> gl;
         for (int o = 0; o < 3; ++o) {
             const int NGo = s->dim == 2 ? opts2[o] : opts3[o], per = NGo / B;
-            int64_t ngl[3], ngt = 0;
-            for (int l = 0; l < 3; ++l) {
-                ngl[l] = (int64_t(blk[l].size()) + per - 1) / per;
+            int64_t ngl[4], ngt = 0;
+            for (int l = 0; l < 4; ++l) {
+                const int64_t nb = (int64_t(lst[l].size()) + B - 1) / B;
+                ngl[l] = (nb + per - 1) / per;
                 ngt += ngl[l];
             }
             if (int64_t(gl.size()) / 4 + ngt * NGo >= (int64_t(1) << 31)) {
@@ -2414,16 +2749,41 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
             h->gopt_ng[o] = NGo;
             h->goff[o] = int(gl.size() / 4);
             h->ngroups[o] = int(ngt);
-            h->nclean[o] = int(ngl[0]);
-            h->nenrg[o] = int(ngl[2]);
-            for (int l = 0; l < 3; ++l)
-                for (int64_t k = 0; k < ngl[l] * per; ++k) {
-                    const int64_t a0 = k < int64_t(blk[l].size()) ? blk[l][size_t(k)] : nall;  // nall: padding
-                    for (int64_t a = a0; a < a0 + B; ++a) {
-                        if (a < nall) gl.insert(gl.end(), &tab[size_t(a) * 4], &tab[size_t(a) * 4] + 4);
-                        else gl.insert(gl.end(), {0, 0, 0, 0});
+            h->nstruct[o] = int(ngl[0]);
+            h->nclean[o] = int(ngl[0] + ngl[1]);
+            h->nenrg[o] = int(ngl[3]);
+            for (int l = 0; l < 4; ++l)
+                for (int64_t k = 0; k < ngl[l] * NGo; ++k) {
+                    if (k < int64_t(lst[l].size())) {
+                        const int64_t a = lst[l][size_t(k)];
+                        gl.insert(gl.end(), &tab[size_t(a) * 4], &tab[size_t(a) * 4] + 4);
+                    } else {
+                        gl.insert(gl.end(), {0, 0, 0, 0});  // padding: no incidence, no pair
                     }
                 }
+        }
+        {
+            int32_t* t_;
+            if ((rc = alloc(h, &t_, sizeof(GridTpl) / 4))) return fail(rc);
+            if (hipMemcpy(t_, &T.t, sizeof(GridTpl), hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the grid template failed";
+                return fail(PG_E_HIP);
+            }
+            h->stpl = t_;
+            int4* r_;
+            if ((rc = alloc(h, &r_, sinf_v.size()))) return fail(rc);
+            if (!sinf_v.empty() && hipMemcpy(r_, srec_v.data(), srec_v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the grid records failed";
+                return fail(PG_E_HIP);
+            }
+            h->srec = r_;
+            int32_t* i_;
+            if ((rc = alloc(h, &i_, sinf_v.size()))) return fail(rc);
+            if (!sinf_v.empty() && hipMemcpy(i_, sinf_v.data(), sinf_v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the grid records failed";
+                return fail(PG_E_HIP);
+            }
+            h->sinf = i_;
         }
         UP(gtab, gl.data(), gl.size(), int32_t);
         {
@@ -2610,6 +2970,8 @@ int64_t pg_asm_algorithmic_bytes(const pg_asm* h) { return h ? h->alg_bytes : -1
 
 int64_t pg_asm_device_bytes(const pg_asm* h) { return h ? h->dev_bytes : -1; }
 
+int64_t pg_asm_grid_nodes(const pg_asm* h) { return h ? h->n_struct : -1; }
+
 int pg_asm_set_timing(pg_asm* h, int enable) {
     if (!h) { g_err = "null handle"; return PG_E_ARG; }
     h->timing = enable != 0;
@@ -2659,7 +3021,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 127) { g_err = "group variant must be 0..127"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 255) { g_err = "group variant must be 0..255"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;

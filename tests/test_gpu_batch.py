@@ -39,3 +39,22 @@ def test_batch_parity_and_instance_slices(mode):
         assert np.array_equal(np.concatenate([K[a:b] for a, b in kr]), Ki)
         assert np.array_equal(np.concatenate([P[a:b] for a, b in pr]), Pi)
         assert np.array_equal(It[o[5]: q[5]], Ii)
+
+
+@pytest.mark.parametrize("mode", ["binned64", "mixed"])
+def test_batch_structured_path_is_bitwise_the_generic_one(mode):
+    """The instances of a batch have different grid strides: each structured
+    node carries its own.  With the structured path off (group variant bit 7)
+    every node takes the generic kernel; the results agree bit for bit."""
+    cfgs = small_batch()
+    B = RVEProblem.batch(cfgs)
+    U, _ = _instance_u(B, [RVEProblem(c) for c in cfgs])
+    ic = np.random.default_rng(4).uniform(0, 5e-5, B.n_cqp)
+    on = AssemblerGPU(B, mode)
+    assert on.grid_nodes() > 0.5 * B.n_nodes
+    off = AssemblerGPU(B, mode)
+    off.set_tiles(0, 0, 128)
+    a, b = _run(on, B, U, ic), _run(off, B, U, ic)
+    print(mode, "grid nodes", on.grid_nodes(), "of", B.n_nodes, flush=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -167,9 +167,12 @@ def test_every_schedule_variant(kind, n, ni):
     for mode in ("binned64", "mixed", "mixed_cut64"):
         ref = None
         # bit 3: every group on the global-memory fallback path; bit 4: enrichment groups with TV = 1;
-        # bits 5-6: another group size for the groups after the cut kernels
-        variants = ((*range(8), 8, 11, 16, 18, 32, 64, 96, 97, 100) if p.dim == 2
+        # bits 5-6: another group size for the groups after the cut kernels; bit 7: the nodes of the
+        # structured grid through the generic kernel
+        variants = ((*range(8), 8, 11, 16, 18, 32, 64, 96, 97, 100, 128, 129, 130, 132, 133, 228) if p.dim == 2
                     else (0, 1, 4, 5, 8, 13, 16, 20, 32, 64, 97, 101))
+        if p.dim == 2:
+            assert AssemblerGPU(p, mode).grid_nodes() > 0.6 * p.n_nodes
         for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
             asm = AssemblerGPU(p, mode)
             asm.set_tiles(g, c, v)
