@@ -314,6 +314,20 @@ int pg_asm_cohesive_failure(pg_asm* h, const double* U, double* out, int64_t n_o
 int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
                double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream);
 
+/* pg_csr_pcg without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * arithmetic and results (bit for bit) as pg_csr_pcg, but the workspace is a
+ * fixed device array inside the library, one per GPU, held until the process
+ * ends: 5 * 1048576 + 1032 doubles (about 42 MB), so n may be at most 1048576
+ * rows (PG_E_ARG beyond that, before any device work).  Every call on one GPU
+ * uses that GPU's array: calls that overlap on one GPU (from several threads
+ * or streams) race and must be serialised by the caller.  Nothing in the
+ * array needs clearing between calls (each call writes every word it reads). */
+int pg_csr_pcg_noalloc(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                       const double* b, double* x, double rtol, int max_iter, int* iters, double* rel_res,
+                       void* stream);
+
 /* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the
  * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own

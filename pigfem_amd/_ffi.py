@@ -101,6 +101,8 @@ SIGNATURES = {
     "pg_asm_grid_nodes": (C.c_int64, [C.c_void_p]),
     "pg_csr_pcg": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                              C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p]),
+    "pg_csr_pcg_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p]),
     "pg_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "pg_dev_free": (C.c_int, [C.c_void_p]),
     "pg_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

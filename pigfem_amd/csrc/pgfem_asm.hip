@@ -19,6 +19,7 @@
 #include <cmath>
 #include <type_traits>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "elem_math.h"
@@ -3122,20 +3123,19 @@ int pg_asm_check_error(pg_asm* h) {
     return PG_OK;
 }
 
-int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
-               double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream) {
-    if (n < 0 || max_iter < 0 || !(rtol >= 0.0)) { g_err = "bad size, iteration limit or tolerance"; return PG_E_ARG; }
-    if (iters) *iters = 0;
-    if (rel_res) *rel_res = 0.0;
-    if (n == 0) return PG_OK;
-    if (!row_ptr || !col_idx || !vals || !b || !x) { g_err = "missing device buffer"; return PG_E_ARG; }
-    struct Mem {
-        void* p = nullptr;
-        ~Mem() { if (p) (void)hipFree(p); }
-    } mem;
-    const size_t nv = size_t(n), words = 5 * nv + 2 * PCG_B + 8;
-    if (hipMalloc(&mem.p, words * sizeof(double)) != hipSuccess) { g_err = "device allocation failed"; return PG_E_NOMEM; }
-    double* r = static_cast<double*>(mem.p);
+}  // extern "C"
+
+namespace {
+
+// Workspace of the PCG solve in doubles: r, z, p, q, the inverse diagonal, the
+// dot products' per-block partial sums and the scalars.
+int64_t pcg_words(int64_t n) { return 5 * n + 2 * PCG_B + 8; }
+
+// The PCG solve on a caller-given workspace of pcg_words(n) doubles.
+int pcg_solve_ws(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
+                 double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream, double* ws) {
+    const size_t nv = size_t(n);
+    double* r = ws;
     double *z = r + nv, *pv = z + nv, *q = pv + nv, *dinv = q + nv, *part = dinv + nv, *sc = part + 2 * PCG_B;
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : nullptr;
     const dim3 g(unsigned((n + 255) / 256)), t(256);
@@ -3192,6 +3192,70 @@ int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const 
     if (iters) *iters = it;
     if (rel_res) *rel_res = std::sqrt(rr / bb);
     return PG_OK;
+}
+
+// Workspace of pg_csr_pcg_noalloc: a file-scope device array, one per GPU the
+// library runs on, held until the process ends.  Its address is taken once
+// per device with hipGetSymbolAddress; host code uses only that address.
+constexpr int64_t PCG_NOALLOC_MAX_N = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[5 * PCG_NOALLOC_MAX_N + 2 * PCG_B + 8];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_for_current_device(double** out) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) { g_err = "device index beyond the supported range"; return PG_E_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) { g_err = "hipGetSymbolAddress returned no address"; return PG_E_HIP; }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_csr_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals, const double* b,
+               double* x, double rtol, int max_iter, int* iters, double* rel_res, void* stream) {
+    if (n < 0 || max_iter < 0 || !(rtol >= 0.0)) { g_err = "bad size, iteration limit or tolerance"; return PG_E_ARG; }
+    if (iters) *iters = 0;
+    if (rel_res) *rel_res = 0.0;
+    if (n == 0) return PG_OK;
+    if (!row_ptr || !col_idx || !vals || !b || !x) { g_err = "missing device buffer"; return PG_E_ARG; }
+    struct Mem {
+        void* p = nullptr;
+        ~Mem() { if (p) (void)hipFree(p); }
+    } mem;
+    if (hipMalloc(&mem.p, size_t(pcg_words(n)) * sizeof(double)) != hipSuccess) {
+        g_err = "device allocation failed";
+        return PG_E_NOMEM;
+    }
+    return pcg_solve_ws(n, row_ptr, col_idx, vals, b, x, rtol, max_iter, iters, rel_res, stream,
+                        static_cast<double*>(mem.p));
+}
+
+int pg_csr_pcg_noalloc(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                       const double* b, double* x, double rtol, int max_iter, int* iters, double* rel_res,
+                       void* stream) {
+    if (n < 0 || max_iter < 0 || !(rtol >= 0.0)) { g_err = "bad size, iteration limit or tolerance"; return PG_E_ARG; }
+    if (iters) *iters = 0;
+    if (rel_res) *rel_res = 0.0;
+    if (n > PCG_NOALLOC_MAX_N) {
+        g_err = "pg_csr_pcg_noalloc: n exceeds its fixed workspace (at most 1048576 rows)";
+        return PG_E_ARG;
+    }
+    if (n == 0) return PG_OK;
+    if (!row_ptr || !col_idx || !vals || !b || !x) { g_err = "missing device buffer"; return PG_E_ARG; }
+    double* ws = nullptr;
+    if (int rc = scratch_for_current_device(&ws)) return rc;
+    return pcg_solve_ws(n, row_ptr, col_idx, vals, b, x, rtol, max_iter, iters, rel_res, stream, ws);
 }
 
 int pg_dev_alloc(void** ptr, int64_t bytes) {
