@@ -249,7 +249,10 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                the list above (8, 16, 32 in 2D; 2, 4, 8 in 3D); bit 7 set =
  *                the nodes of a structured tri3 grid (generate_mesh's pattern,
  *                checked at create) take the generic node-group kernel instead
- *                of the one that computes their records from the grid.
+ *                of the one that computes their records from the grid; bits
+ *                8-9 = k: each block of that kernel takes 2^k consecutive
+ *                groups, the next group's records loading during the current
+ *                group's work.
  * Every choice gives bit-identical results. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 

@@ -90,6 +90,15 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
                 if log:
                     log(f"[tune] group_nodes={g} cut_variant={c} group_variant={v2}: {table[f'{g},{c},{v2}']:.3f} ms")
             g, c, v = (int(x) for x in min(table, key=table.get).split(","))
+        # consecutive groups per block of the structured-grid kernel (group
+        # variant bits 8-9; tri3 meshes)
+        if p.dim == 2:
+            for k in (1, 2, 3):
+                v2 = (v & 255) | (k << 8)
+                table[f"{g},{c},{v2}"] = timed(g, c, v2, 0)
+                if log:
+                    log(f"[tune] group_nodes={g} cut_variant={c} group_variant={v2}: {table[f'{g},{c},{v2}']:.3f} ms")
+            g, c, v = (int(x) for x in min(table, key=table.get).split(","))
     finally:
         asm.set_timing(False)
         synchronize()

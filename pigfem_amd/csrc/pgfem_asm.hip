@@ -1691,7 +1691,7 @@ template <typename TC, int NG, int TV, int BS = GroupThreads<2, NG, TV>::value>
 __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __restrict__ tpl,
                                                      const int4* __restrict__ srec, const int32_t* __restrict__ sinf,
                                                      const double* __restrict__ Un, double* __restrict__ K,
-                                                     double* __restrict__ P, unsigned flags) {
+                                                     double* __restrict__ P, unsigned flags, int ngroups, int per) {
     constexpr int D = 2, NEN = 3, NQ = 4;
     constexpr int MAXP = 9 * NG;
     constexpr int CAPO = 2, CAPD = 8;
@@ -1707,12 +1707,17 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
     const int tid = threadIdx.x;
-    const int grp = xcd_order(blockIdx.x, gridDim.x);
-    // set-up loads, all independent: the group's records, the template and
-    // the materials
-    const int64_t k0 = int64_t(grp) * NG + (tid & (NG - 1));
-    const int4 r = srec[k0];
-    const int inf = sinf[k0];
+    // each block takes `per` consecutive groups (XCD-aware block order)
+    const int g0 = xcd_order(blockIdx.x, gridDim.x) * per, g1 = min(ngroups, g0 + per);
+    // set-up loads, all independent: the first group's records, the template
+    // and the materials
+    int4 r = make_int4(-1, 0, 0, 0);
+    int inf = 0;
+    if (g0 < g1) {
+        const int64_t k0 = int64_t(g0) * NG + (tid & (NG - 1));
+        r = srec[k0];
+        inf = sinf[k0];
+    }
     constexpr int TW = int(sizeof(GridTpl) / 4);
     const int tw = tpl[tid < TW ? tid : TW - 1];
     stage_lm<TC, BS>(p, 3, sLM);
@@ -1722,118 +1727,127 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
         sLC[m][j] = m < p.n_mat ? v : 0.0;
     }
     if (tid < TW) reinterpret_cast<int32_t*>(&sT)[tid] = tw;
-    const bool live = tid < NG && r.x >= 0;
-    const int par = (inf >> 24) & 1;
-    int ci = live ? GRID_NI[par] : 0, cq = live ? GRID_NP[par] : 0;
-#pragma unroll
-    for (int off = 1; off < NG; off <<= 1) {
-        const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
-        if ((tid & 63) >= off) { ci += ti; cq += tq; }
-    }
-    if (tid < NG) {
-        s_node[tid] = r.x;
-        s_kb[tid] = r.y;
-        s_dof[tid] = r.z;
-        s_eb[tid] = r.w;
-        s_inf[tid] = inf;
-        s_iptr[tid + 1] = ci;
-        s_qptr[tid + 1] = cq;
-    }
-    if (tid == 0) {
-        s_iptr[0] = 0;
-        s_qptr[0] = 0;
-    }
-    __syncthreads();
-    const int ni = s_iptr[NG], nq = s_qptr[NG];
     double wsum = 0.0;
     for (int k = 0; k < NQ; ++k) wsum += TRI_QW[k];
-
-    // ---- phase A: incidences (as k_pair_group, records from the template) ----
-    for (int i = tid; i < ni; i += BS) {
-        int lo = 0, hi = NG;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_iptr[mid] <= i) lo = mid; else hi = mid;
-        }
-        const int rr = i - s_iptr[lo];
-        const int a = s_node[lo], inf0 = s_inf[lo], pr = (inf0 >> 24) & 1, S = inf0 & 0xffffff;
-        const int e = s_eb[lo] + 2 * (sT.ecx[pr][rr] + sT.ecy[pr][rr] * (S - 1)) + sT.eh[pr][rr];
-        int ln[NEN];
-        for (int n = 0; n < NEN; ++n) ln[n] = a + sT.ndx[pr][rr][n] + sT.ndy[pr][rr][n] * S;
-        double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
-        for (int n = 0; n < NEN; ++n)
-            for (int c = 0; c < D; ++c) {
-                x[n][c] = p.coords[int64_t(ln[n]) * D + c];
-                u[n][c] = Un[int64_t(ln[n]) * D + c];
-            }
-        const int mi = p.elem_mat[e] & (MAX_LDS_MATS - 1);
-        // every load of the incidence is out before the arithmetic
-        __builtin_amdgcn_sched_barrier(0);
-        double lc[LeiCoef<D>::N];
-        for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = sLC[mi][k];
-        double g[NEN][D];
-        const double J = simplex_grads<D, double>(x, g);
-        const double w = J * wsum;
-        plain_p<D>(g, u, 0, w, lc, s_dP[lo][rr]);
-        if (!jac) continue;
-        TC gt[NEN][D];
-        for (int n = 0; n < NEN; ++n)
-            for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
-        const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
-        const int ps0 = s_qptr[lo];
-        double dd[D][D];
+    for (int g = g0; g < g1; ++g) {
+        const bool live = tid < NG && r.x >= 0;
+        const int par = (inf >> 24) & 1;
+        int ci = live ? GRID_NI[par] : 0, cq = live ? GRID_NP[par] : 0;
 #pragma unroll
-        for (int k = 0; k < NEN - 1; ++k) {
-            TC blk[D][D];
-            iso_block<D, TC>(gt[0], gt[k + 1], wl, wm, blk);
-            diag_acc<D, TC>(k, blk, dd);
-            const int f = sT.sr[pr][rr][k];
-            const int slot = f & 63, rank = f >> 6;
+        for (int off = 1; off < NG; off <<= 1) {
+            const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
+            if ((tid & 63) >= off) { ci += ti; cq += tq; }
+        }
+        if (tid < NG) {
+            s_node[tid] = r.x;
+            s_kb[tid] = r.y;
+            s_dof[tid] = r.z;
+            s_eb[tid] = r.w;
+            s_inf[tid] = inf;
+            s_iptr[tid + 1] = ci;
+            s_qptr[tid + 1] = cq;
+        }
+        if (tid == 0) {
+            s_iptr[0] = 0;
+            s_qptr[0] = 0;
+        }
+        __syncthreads();
+        // the next group's records go out now and land during this group's work
+        if (g + 1 < g1) {
+            const int64_t k1 = int64_t(g + 1) * NG + (tid & (NG - 1));
+            r = srec[k1];
+            inf = sinf[k1];
+        }
+        const int ni = s_iptr[NG], nq = s_qptr[NG];
+
+        // ---- phase A: incidences (as k_pair_group, records from the template) ----
+        for (int i = tid; i < ni; i += BS) {
+            int lo = 0, hi = NG;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_iptr[mid] <= i) lo = mid; else hi = mid;
+            }
+            const int rr = i - s_iptr[lo];
+            const int a = s_node[lo], inf0 = s_inf[lo], pr = (inf0 >> 24) & 1, S = inf0 & 0xffffff;
+            const int e = s_eb[lo] + 2 * (sT.ecx[pr][rr] + sT.ecy[pr][rr] * (S - 1)) + sT.eh[pr][rr];
+            int ln[NEN];
+            for (int n = 0; n < NEN; ++n) ln[n] = a + sT.ndx[pr][rr][n] + sT.ndy[pr][rr][n] * S;
+            double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
+            for (int n = 0; n < NEN; ++n)
+                for (int c = 0; c < D; ++c) {
+                    x[n][c] = p.coords[int64_t(ln[n]) * D + c];
+                    u[n][c] = Un[int64_t(ln[n]) * D + c];
+                }
+            const int mi = p.elem_mat[e] & (MAX_LDS_MATS - 1);
+            // every load of the incidence is out before the arithmetic
+            __builtin_amdgcn_sched_barrier(0);
+            double lc[LeiCoef<D>::N];
+            for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = sLC[mi][k];
+            double g[NEN][D];
+            const double J = simplex_grads<D, double>(x, g);
+            const double w = J * wsum;
+            plain_p<D>(g, u, 0, w, lc, s_dP[lo][rr]);
+            if (!jac) continue;
+            TC gt[NEN][D];
+            for (int n = 0; n < NEN; ++n)
+                for (int c = 0; c < D; ++c) gt[n][c] = TC(g[n][c]);
+            const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+            const int ps0 = s_qptr[lo];
+            double dd[D][D];
+#pragma unroll
+            for (int k = 0; k < NEN - 1; ++k) {
+                TC blk[D][D];
+                iso_block<D, TC>(gt[0], gt[k + 1], wl, wm, blk);
+                diag_acc<D, TC>(k, blk, dd);
+                const int f = sT.sr[pr][rr][k];
+                const int slot = f & 63, rank = f >> 6;
+                for (int ii = 0; ii < D; ++ii)
+                    for (int jj = 0; jj < D; ++jj) s_blk[ps0 + slot][rank][ii * D + jj] = blk[ii][jj];
+            }
+            diag_fin<D, TC>(gt[0], wl, wm, dd);
             for (int ii = 0; ii < D; ++ii)
-                for (int jj = 0; jj < D; ++jj) s_blk[ps0 + slot][rank][ii * D + jj] = blk[ii][jj];
+                for (int jj = 0; jj < D; ++jj) s_dblk[lo][rr][ii * D + jj] = dd[ii][jj];
         }
-        diag_fin<D, TC>(gt[0], wl, wm, dd);
-        for (int ii = 0; ii < D; ++ii)
-            for (int jj = 0; jj < D; ++jj) s_dblk[lo][rr][ii * D + jj] = dd[ii][jj];
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- phase B: node pairs ----
-    for (int kq = tid; kq < nq; kq += BS) {
-        int lo = 0, hi = NG;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
-        }
-        const int slot = kq - s_qptr[lo];
-        const int pr = (s_inf[lo] >> 24) & 1, np = GRID_NP[pr];
-        const bool isdiag = slot == GRID_DIAG[pr];
-        double acc[D][D];
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
-        double pa[D] = {0.0, 0.0};
-        if (isdiag) {
-            const int nd = GRID_NI[pr];
-            for (int rr = 0; rr < nd; ++rr) {
-                if (func)
-                    for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][rr][i];
-                if (jac)
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[lo][rr][i * D + j];
+        // ---- phase B: node pairs ----
+        for (int kq = tid; kq < nq; kq += BS) {
+            int lo = 0, hi = NG;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
             }
-        } else if (jac) {
+            const int slot = kq - s_qptr[lo];
+            const int pr = (s_inf[lo] >> 24) & 1, np = GRID_NP[pr];
+            const bool isdiag = slot == GRID_DIAG[pr];
+            double acc[D][D];
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
+            double pa[D] = {0.0, 0.0};
+            if (isdiag) {
+                const int nd = GRID_NI[pr];
+                for (int rr = 0; rr < nd; ++rr) {
+                    if (func)
+                        for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][rr][i];
+                    if (jac)
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[lo][rr][i * D + j];
+                }
+            } else if (jac) {
 #pragma unroll
-            for (int rk = 0; rk < CAPO; ++rk)
-                for (int i = 0; i < D; ++i)
-                    for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[kq][rk][i * D + j]);
+                for (int rk = 0; rk < CAPO; ++rk)
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(s_blk[kq][rk][i * D + j]);
+            }
+            if (jac) {
+                const int64_t k0v = int64_t(s_kb[lo]) + 2 * slot;
+                *reinterpret_cast<dpair*>(K + k0v) = dpair{acc[0][0], acc[0][1]};
+                *reinterpret_cast<dpair*>(K + k0v + 2 * np) = dpair{acc[1][0], acc[1][1]};
+            }
+            if (isdiag && func)
+                for (int i = 0; i < D; ++i) P[s_dof[lo] + i] = pa[i];
         }
-        if (jac) {
-            const int64_t k0v = int64_t(s_kb[lo]) + 2 * slot;
-            *reinterpret_cast<dpair*>(K + k0v) = dpair{acc[0][0], acc[0][1]};
-            *reinterpret_cast<dpair*>(K + k0v + 2 * np) = dpair{acc[1][0], acc[1][1]};
-        }
-        if (isdiag && func)
-            for (int i = 0; i < D; ++i) P[s_dof[lo] + i] = pa[i];
+        __syncthreads();  // the next group rewrites the tables
     }
 }
 
@@ -2431,9 +2445,12 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         auto skip = [&](int oo) { return grid ? h->nstruct[oo] : 0; };
         auto grid_groups = [&](int oo, hipStream_t s, bool tv) {
             const int ng = h->gopt_ng[oo];
-            const unsigned grid_n = unsigned((h->n_struct + ng - 1) / ng);
+            const int ngr = int((h->n_struct + ng - 1) / ng);
+            // group variant bits 8-9: each block takes 2^k consecutive groups
+            const int per = 1 << ((h->group_variant >> 8) & 3);
+            const unsigned grid_n = unsigned((ngr + per - 1) / per);
 #define STRUCT1(TC, NGV, TVV) \
-    k_pair_struct<TC, NGV, TVV><<<grid_n, GroupThreads<2, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->srec, h->sinf, h->Un, dK, dP, flags)
+    k_pair_struct<TC, NGV, TVV><<<grid_n, GroupThreads<2, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->srec, h->sinf, h->Un, dK, dP, flags, ngr, per)
 #define STRUCT2(TC, NGV) \
     do { if (tv) STRUCT1(TC, NGV, 1); else STRUCT1(TC, NGV, 0); } while (0)
 #define STRUCT3(TC) \
@@ -3022,7 +3039,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 255) { g_err = "group variant must be 0..255"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 1023) { g_err = "group variant must be 0..1023"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;

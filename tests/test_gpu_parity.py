@@ -168,8 +168,8 @@ def test_every_schedule_variant(kind, n, ni):
         ref = None
         # bit 3: every group on the global-memory fallback path; bit 4: enrichment groups with TV = 1;
         # bits 5-6: another group size for the groups after the cut kernels; bit 7: the nodes of the
-        # structured grid through the generic kernel
-        variants = ((*range(8), 8, 11, 16, 18, 32, 64, 96, 97, 100, 128, 129, 130, 132, 133, 228) if p.dim == 2
+        # structured grid through the generic kernel; bits 8-9: consecutive groups per block of the structured kernel
+        variants = ((*range(8), 8, 11, 16, 18, 32, 64, 96, 97, 100, 128, 129, 130, 132, 133, 228, 256, 516, 768, 897) if p.dim == 2
                     else (0, 1, 4, 5, 8, 13, 16, 20, 32, 64, 97, 101))
         if p.dim == 2:
             assert AssemblerGPU(p, mode).grid_nodes() > 0.6 * p.n_nodes
