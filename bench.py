@@ -59,10 +59,30 @@ def build_problem(name):
     return prob, cfg, cfgs, setup_s
 
 
-def kernel_name(dim, g, tv, lists):
-    per = (8 if dim == 2 else 16) * (2 if tv else 1) * g
-    bs = 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
-    return f"k_pair_{'lists' if lists else 'group'}<{dim}, float, {g}, {tv}, {bs}>"
+def node_pair_kernels(dim, tiles, grid_nodes):
+    """Names of the node-pair kernels one mixed-schedule assembly launches with
+    these tiles: the structured-grid kernel (tri3 nodes checked against the
+    grid formulas) and the generic node-group kernel for the other nodes."""
+    g, _, v = tiles
+    tv, lists = v & 1, dim == 3 or bool(v & 2)
+    sizes = (8, 16, 32) if dim == 2 else (2, 4, 8)
+    rsel = (v >> 5) & 3
+    g_rest = sizes[rsel - 1] if rsel else g
+    overlap = not v & 4
+
+    def bs(ng, t):
+        per = (8 if dim == 2 else 16) * (2 if t else 1) * ng
+        return 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
+    names = []
+    if dim == 2 and grid_nodes > 0 and not v & (128 | 8):
+        gs = g if overlap else g_rest
+        names.append(f"k_pair_struct<float, {gs}, {tv}, {bs(gs, tv)}>")
+    for ng in ((g, g_rest) if overlap else (g_rest,)):
+        name = (f"k_pair_lists<{dim}, float, float, {ng}, {tv}, {bs(ng, tv)}>" if lists
+                else f"k_pair_group<{dim}, float, {ng}, {tv}, {bs(ng, tv)}>")
+        if name not in names:
+            names.append(name)
+    return " + ".join(names)
 
 
 def measure(prob, cfg, args, full):
@@ -161,14 +181,7 @@ def measure(prob, cfg, args, full):
         f"{clean_ms:.3f} + remaining {dirty_ms:.3f}) -> {achieved:.0f} GB/s (algorithmic {pair_bytes/1e9:.3f} GB); "
         f"whole assembly {whole:.0f} GB/s")
 
-    tm = tuned["mixed"]
-    ng, tv, lists = tm[0], tm[2] & 1, prob.dim == 3 or bool(tm[2] & 2)
-    # group variant bits 5-6: the groups after the cut kernels may have another size
-    rsel = (tm[2] >> 5) & 3
-    ng_rest = ((8, 16, 32) if prob.dim == 2 else (2, 4, 8))[rsel - 1] if rsel else ng
-    kname = kernel_name(prob.dim, ng, tv, lists)
-    if ng_rest != ng:
-        kname += " + " + kernel_name(prob.dim, ng_rest, tv, lists)
+    kname = node_pair_kernels(prob.dim, tuned["mixed"], asm.grid_nodes())
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"hbm_traffic_{cfg.name}.md")
     if os.path.exists(tf):
@@ -183,7 +196,8 @@ def measure(prob, cfg, args, full):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "algorithmic_bytes_per_launch": pair_bytes, "kernel_ms": pair_ms,
                         "kernel_ms_overlapped_groups": clean_ms, "kernel_ms_remaining_groups": dirty_ms},
-           "whole_GBps": whole, "B_alg": B_alg, "cut_kernel_ms": pre_ms, "device_bytes": asm.device_bytes(), "U": U}
+           "whole_GBps": whole, "B_alg": B_alg, "cut_kernel_ms": pre_ms, "device_bytes": asm.device_bytes(), "U": U,
+           "grid_nodes": asm.grid_nodes()}
 
     if full:
         # host-buffer API (PCIe-inclusive: U in, K / P / ISVs out), mixed schedule
@@ -357,6 +371,7 @@ def main():
         "whole_assembly_frac": m["whole_GBps"] / HBM_PEAK_GBS,
         "algorithmic_bytes": m["B_alg"],
         "cut_kernel_ms": m["cut_kernel_ms"],
+        "grid_nodes": m["grid_nodes"],
         "setup_s": setup_s,
         "host_api_ms_per_call": m["host_ms"],
         "device_bytes": m["device_bytes"],
@@ -383,7 +398,7 @@ def main():
             "speedup_vs_binned64": r2["binned64"]["ms"] / r2["mixed"]["ms"],
             "rel_err_K": m2["acc"]["rel_err_K"], "rel_err_P": m2["acc"]["rel_err_P"],
             "roofline": m2["roofline"], "whole_assembly_frac": m2["whole_GBps"] / HBM_PEAK_GBS,
-            "cut_kernel_ms": m2["cut_kernel_ms"], "setup_s": s2, "tiles": m2["tuned"],
+            "cut_kernel_ms": m2["cut_kernel_ms"], "grid_nodes": m2["grid_nodes"], "setup_s": s2, "tiles": m2["tuned"],
         }
         del p2, m2
         gc.collect()
