@@ -332,6 +332,35 @@ int pg_csr_pcg_noalloc(int64_t n, const int32_t* row_ptr, const int32_t* col_idx
                        void* stream);
 
 /* ----------------------------------------------------------------------------
+ * Newton's linear solve on the device: K_ff x = b, replacing the KSP solve
+ * (MUMPS through PETSc) of SolverNonlinearKSP::nonlinearStep
+ * (src/SolverNonlinearKSP.cpp:94, src/Solver.cpp:108-160).  The solver object
+ * takes the structure of K_ff (rp[0], ci[0], n_free, nnz[0] of the
+ * description) once; each solve takes the values from a device buffer laid out
+ * as pg_asm_assemble_device writes K ([ff | fp | pf | pp]: the first nnz[0]
+ * values are K_ff), copied device to device, so K never crosses PCIe.  A
+ * multicolour symmetric Gauss-Seidel preconditioner (rocALUTION) serves CG
+ * (symmetric positive definite tangents), FGMRES(50) (indefinite or
+ * non-symmetric ones: softening cohesive laws, the bilinear contact branch) or
+ * BiCGStab; it is built at the first solve and rebuilt numerically at the
+ * others.  An iterative solve: its iteration count grows with the mesh
+ * (hundreds at 100 x 100 cells, thousands at 200 x 200), so it suits meshes
+ * far below the largest workloads, whose direct solve stays on the host.  dx
+ * holds the initial guess on entry and the solution on return; *rel_res is the
+ * true |b - K_ff x| / |b|, which the caller compares with its tolerance.
+ * Synchronous.  PG_E_ARG on non-finite data or divergence.  The messages of
+ * these calls come from pg_ksp_last_error(). */
+#define PG_KSP_CG        0
+#define PG_KSP_FGMRES    1
+#define PG_KSP_BICGSTAB  2
+typedef struct pg_ksp pg_ksp;
+int pg_ksp_create(const pg_problem_desc* desc, int method, pg_ksp** out);
+int pg_ksp_solve(pg_ksp* k, const double* dK, const double* db, double* dx, double rtol, int max_iter,
+                 int* iters, double* rel_res);
+int pg_ksp_destroy(pg_ksp* k);
+const char* pg_ksp_last_error(void);
+
+/* ----------------------------------------------------------------------------
  * Device buffers for callers that have no HIP allocator of their own (the
  * Python host layer uses these; a PETSc/hipSPARSE caller would pass its own
  * device pointers).  No counterpart in the reference (CPU-only).  All act on
@@ -344,6 +373,11 @@ int pg_dev_upload(void* dst_device, const void* src_host, int64_t bytes);
 int pg_dev_download(void* dst_host, const void* src_device, int64_t bytes);
 int pg_dev_copy(void* dst_device, const void* src_device, int64_t bytes);
 int pg_dev_synchronize(void);
+/* y += a x on n doubles (device pointers; synchronous). */
+int pg_dev_axpy(int64_t n, double a, const double* x, double* y);
+/* *out = x . y over n doubles (device pointers; synchronous, fixed summation
+ * order, so bitwise repeatable). */
+int pg_dev_dot(int64_t n, const double* x, const double* y, double* out);
 
 #ifdef __cplusplus
 }

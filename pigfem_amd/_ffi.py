@@ -109,6 +109,13 @@ SIGNATURES = {
     "pg_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "pg_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "pg_dev_synchronize": (C.c_int, []),
+    "pg_dev_axpy": (C.c_int, [C.c_int64, C.c_double, C.c_void_p, C.c_void_p]),
+    "pg_dev_dot": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    "pg_ksp_create": (C.c_int, [C.POINTER(PgProblemDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "pg_ksp_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "pg_ksp_destroy": (C.c_int, [C.c_void_p]),
+    "pg_ksp_last_error": (C.c_char_p, []),
     # pgfem_harness.h
     "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
     "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),
@@ -151,5 +158,6 @@ class PgError(RuntimeError):
 def check(rc: int, where: str) -> None:
     if rc != 0:
         L = lib()
-        msg = (L.pgh_last_error() if where.startswith("pgh") else L.pg_asm_last_error()) or b""
+        msg = (L.pgh_last_error() if where.startswith("pgh") else
+               L.pg_ksp_last_error() if where.startswith("pg_ksp") else L.pg_asm_last_error()) or b""
         raise PgError(f"{where} failed ({rc}): {msg.decode(errors='replace')}")

@@ -27,6 +27,7 @@ HIP_FLAGS = [
     "-I" + INC, "-I" + CSRC,
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-I" + INC, "-I" + CSRC]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
 def _run(cmd):
@@ -40,11 +41,14 @@ def _run(cmd):
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     objs = []
-    for src, kind in (("harness.cpp", "cxx"), ("pgfem_asm.hip", "hip")):
+    for src, kind in (("harness.cpp", "cxx"), ("ksp.cpp", "rocalution"), ("pgfem_asm.hip", "hip")):
         s = os.path.join(CSRC, src)
         o = os.path.join(OUT, src.rsplit(".", 1)[0] + ".o")
         if kind == "cxx":
             cmd = ["g++", *CXX_FLAGS, "-c", s, "-o", o]
+        elif kind == "rocalution":
+            # the Newton linear solve on the device links AMD's rocALUTION (Krylov solvers)
+            cmd = ["g++", *CXX_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), "-c", s, "-o", o]
         else:
             cmd = [HIPCC, *HIP_FLAGS, "-c", s, "-o", o]
         if verbose:
@@ -53,7 +57,8 @@ def build(verbose: bool = False) -> str:
         objs.append(o)
     so = os.path.join(OUT, "libpgfem_asm.so")
     tmp = so + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-pthread", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-pthread", "-o", tmp, *objs,
+          "-L" + os.path.join(ROCM, "lib"), "-lrocalution", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, so)
     for o in objs:
         os.remove(o)

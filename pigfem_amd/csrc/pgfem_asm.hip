@@ -1041,6 +1041,12 @@ __global__ void k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__
     z[i] = dinv[i] * ri;
 }
 
+// y += a x
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = fma(a, x[i], y[i]);
+}
+
 // beta = sc[inew] / sc[iold]; p = z + beta p
 __global__ void k_pcg_p(int64_t n, double* __restrict__ pv, const double* __restrict__ z,
                         const double* __restrict__ sc, int inew, int iold) {
@@ -3307,6 +3313,34 @@ int pg_dev_copy(void* dst, const void* src, int64_t bytes) {
 
 int pg_dev_synchronize(void) {
     HIP_OK(hipDeviceSynchronize());
+    return PG_OK;
+}
+
+int pg_dev_axpy(int64_t n, double a, const double* x, double* y) {
+    if (n < 0 || (n > 0 && (!x || !y))) { g_err = "bad argument"; return PG_E_ARG; }
+    if (n == 0) return PG_OK;
+    k_axpy<<<dim3(unsigned((n + 255) / 256)), dim3(256)>>>(n, a, x, y);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    return PG_OK;
+}
+
+int pg_dev_dot(int64_t n, const double* x, const double* y, double* out) {
+    if (n < 0 || !out || (n > 0 && (!x || !y))) { g_err = "bad argument"; return PG_E_ARG; }
+    *out = 0.0;
+    if (n == 0) return PG_OK;
+    struct Mem {
+        void* p = nullptr;
+        ~Mem() { if (p) (void)hipFree(p); }
+    } mem;
+    HIP_OK(hipMalloc(&mem.p, (2 * PCG_B + 8) * sizeof(double)));
+    double* part = static_cast<double*>(mem.p);
+    double* sc = part + 2 * PCG_B;
+    const int nb = int(std::min<int64_t>(PCG_B, (n + PCG_T - 1) / PCG_T));
+    k_dot2<<<dim3(nb), dim3(PCG_T)>>>(n, x, y, x, y, part);
+    k_dot_fin<<<dim3(1), dim3(PCG_T)>>>(part, nb, sc, 0, 1);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(out, sc, sizeof(double), hipMemcpyDeviceToHost));
     return PG_OK;
 }
 

@@ -115,6 +115,9 @@ class LinearSolver:
     def __init__(self, problem, prefer="pardiso"):
         self.p = problem
         self._gpu = None
+        if prefer == "none":  # the caller solves by itself (DeviceNewtonDriver)
+            self.kind, self._pardiso = "none", None
+            return
         if prefer == "gpu-pcg":
             from .solve import DeviceCSR
             self.kind, self._pardiso = "gpu-pcg", None
@@ -261,6 +264,120 @@ class NewtonDriver:
         """Newton iterations of every attempted load step (repeats included;
         an explicit step counts as one iteration)."""
         return [(r.step, r.repeats, r.converged, r.iterations) for r in self.history]
+
+
+class DeviceNewtonDriver(NewtonDriver):
+    """The same load stepping and Newton loop with the assembly and the linear
+    solve on the device: U, K, P and the trial ISVs stay in HBM; K_ff is solved
+    in place by preconditioned Krylov (DeviceKSP, iterative: its cost grows
+    with the mesh); the residual norm is a
+    device dot product.  Only scalars, the prescribed values at the start of an
+    attempt and U / ISV snapshots of converged steps cross PCIe.  `asm` is an
+    AssemblerGPU (any mode)."""
+
+    def __init__(self, problem, asm, method="fgmres", lin_rtol=1e-10, lin_max_iter=20000, **kw):
+        from .device import DeviceBuffer
+        from .solve import DeviceKSP
+        kw["solver"] = "none"
+        super().__init__(problem, None, **kw)
+        self.lin.kind = f"gpu-{method}"
+        self.asm = asm
+        self.ksp = DeviceKSP(problem, method)
+        self.lin_rtol, self.lin_max_iter = lin_rtol, lin_max_iter
+        p = problem
+        self.dU = DeviceBuffer.zeros(p.n_dof)
+        self.dK = DeviceBuffer.zeros(max(p.nnz_total, 1))
+        self.dP = DeviceBuffer.zeros(p.n_dof)
+        self.dIc = DeviceBuffer.zeros(max(p.n_cqp, 1))
+        self.dIt = DeviceBuffer.zeros(max(p.n_cqp, 1))
+        self.ddu = DeviceBuffer.zeros(p.n_free)
+        self.dzero = DeviceBuffer.zeros(p.n_free)
+        self.lin_iters = []  # Krylov iterations of every solve
+
+    def _assemble(self):
+        """K, P at the device U; returns |P_f| (the residual norm, F_ext = 0)."""
+        from .solve import dev_dot
+        p = self.p
+        c = self.dIc if p.n_cqp else None
+        t = self.dIt if p.n_cqp else None
+        self.asm.assemble_device(self.dU, c, t, self.dK, self.dP)
+        self.asm.check_error()
+        return math.sqrt(max(dev_dot(p.n_free, self.dP, self.dP), 0.0))
+
+    def _solve_update(self):
+        """U_f -= du with K_ff du = P_f (the Newton update du = K^-1 (F_ext - P))."""
+        from .solve import dev_axpy
+        p = self.p
+        self.ddu.copy_from(self.dzero)
+        it, _ = self.ksp.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.lin_max_iter)
+        self.lin_iters.append(it)
+        dev_axpy(p.n_free, -1.0, self.ddu, self.dU)
+
+    def _begin(self, U):
+        p = self.p
+        self.dU.upload(np.ascontiguousarray(U, dtype=np.float64))
+        if p.n_cqp:
+            self.dIc.upload(np.ascontiguousarray(self.isv, dtype=np.float64))
+
+    def _trial_isv(self):
+        p = self.p
+        return self.dIt.download(p.n_cqp) if p.n_cqp else np.zeros(0)
+
+    # SolverNonlinearKSP::nonlinearStep, device-resident
+    def _newton(self, U):
+        from .device import synchronize
+        self._begin(U)
+        r = self._assemble()
+        r0 = r
+        hist = [r]
+        nr, inc = 0, 0
+        while True:
+            try:
+                self._solve_update()
+            except Exception:                        # breakdown: treat as divergence
+                synchronize()
+                return False, nr, hist, self._trial_isv(), self.dU.download(self.p.n_dof)
+            prev = r
+            r = self._assemble()
+            nr += 1
+            hist.append(r)
+            done = None
+            if r < self.rel_tol * r0 or r < self.abs_tol:
+                done = True
+            elif not math.isfinite(r) or nr >= self.max_iter:
+                done = False
+            else:
+                inc = inc + 1 if r > prev else 0
+                if inc >= 3:
+                    done = False
+            if done is not None:
+                return done, nr, hist, self._trial_isv(), self.dU.download(self.p.n_dof)
+
+    # SolverNonlinear::explicit_solve, device-resident
+    def _explicit(self, t, step):
+        p = self.p
+        dt = self.min_dt
+        n = self.explicit_steps
+        n_till_end = int((self.T_final - t) / self.min_dt + 0.5)
+        cont = True
+        if n_till_end < n:
+            n, cont = n_till_end, False
+        bc = np.asarray(p.bc_values)
+        for k in range(n):
+            U = self.U.copy()
+            U[p.n_free:] = ((t + dt) / self.T_final) * bc
+            self._begin(U)
+            r = self._assemble()
+            if not math.isfinite(r):
+                raise NewtonError("invalid residual norm during an explicit step")
+            self._solve_update()
+            self.U = self.dU.download(p.n_dof)
+            self.isv = self._trial_isv().copy()
+            t += dt
+            self.history.append(StepRecord(step, t, dt, 0, True, 1, [r], explicit=True))
+        if self.log:
+            self.log(f"[newton] explicit: {n} steps of dt={dt:.1e} to t={t:.6f}")
+        return t, cont
 
 
 def gpu_assemble_fn(asm):

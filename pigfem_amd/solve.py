@@ -48,3 +48,50 @@ def pcg_solve(A, b, x0=None, rtol=1e-12, max_iter=None):
     """Solve A x = b on the GPU for a symmetric positive definite scipy matrix A."""
     A = sp.csr_matrix(A)
     return DeviceCSR(A.indptr, A.indices).solve(A.data, b, x0, rtol, max_iter)
+
+
+class DeviceKSP:
+    """Newton's linear solve on the device (pg_ksp_*): K_ff x = b for the K_ff
+    block of a device K buffer laid out as pg_asm_assemble_device writes it,
+    preconditioned by multicolour symmetric Gauss-Seidel.  method: "cg",
+    "fgmres" or "bicgstab"."""
+
+    METHODS = {"cg": 0, "fgmres": 1, "bicgstab": 2}
+
+    def __init__(self, problem, method="fgmres"):
+        self.n = problem.n_free
+        h = C.c_void_p()
+        check(lib().pg_ksp_create(C.byref(problem.desc), self.METHODS[method], C.byref(h)), "pg_ksp_create")
+        self._h = h
+
+    def solve(self, dK, db, dx, rtol=1e-10, max_iter=20000):
+        """dK, db, dx: DeviceBuffers (dx: initial guess in, solution out).
+        Returns (iterations, true relative residual); raises PgError if the
+        solve breaks down or stops above rtol."""
+        it, rel = C.c_int(0), C.c_double(0.0)
+        check(lib().pg_ksp_solve(self._h, dK.ptr, db.ptr, dx.ptr, float(rtol), int(max_iter), C.byref(it),
+                                 C.byref(rel)), "pg_ksp_solve")
+        if not rel.value <= 10 * rtol:
+            raise PgError(f"device solve stopped at relative residual {rel.value:.3e} after {it.value} iterations")
+        return it.value, rel.value
+
+    def close(self):
+        if self._h:
+            lib().pg_ksp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dev_dot(n, x, y):
+    out = C.c_double(0.0)
+    check(lib().pg_dev_dot(int(n), x.ptr, y.ptr, C.byref(out)), "pg_dev_dot")
+    return out.value
+
+
+def dev_axpy(n, a, x, y):
+    check(lib().pg_dev_axpy(int(n), float(a), x.ptr, y.ptr), "pg_dev_axpy")

@@ -1,0 +1,60 @@
+"""Newton's linear solve on the device with multicolour-SGS-preconditioned
+Krylov (pg_ksp_*): K_ff is solved in place from the device K buffer.  Pinned by the
+true residual, by scipy's direct solve, and by Newton histories that repeat the
+oracle's (with its direct solver) when U, K, P and the solve stay on the
+device (DeviceNewtonDriver)."""
+import dataclasses
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+from pigfem_amd import AssemblerGPU, DeviceBuffer, RVEProblem, config, small_config
+from pigfem_amd.newton import DeviceNewtonDriver, NewtonDriver, _kff
+from pigfem_amd.solve import DeviceKSP
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind,method", [("c0", "cg"), ("c0", "fgmres"), ("op", "fgmres"), ("bl", "bicgstab")])
+def test_device_solve_of_the_tangent_in_place(kind, method):
+    p = RVEProblem(small_config(kind, n=24, n_incl=3))
+    asm = AssemblerGPU(p, "fp64")
+    U = np.zeros(p.n_dof)
+    U[p.n_free:] = 0.2 * p.bc_values
+    dU, dK, dP = DeviceBuffer.from_numpy(U), DeviceBuffer.zeros(p.nnz_total), DeviceBuffer.zeros(p.n_dof)
+    dI = DeviceBuffer.zeros(max(p.n_cqp, 1)) if p.n_cqp else None
+    dT = DeviceBuffer.zeros(max(p.n_cqp, 1)) if p.n_cqp else None
+    asm.assemble_device(dU, dI, dT, dK, dP)
+    K = dK.download(p.nnz_total)
+    A = _kff(p, K)
+    b = np.random.default_rng(3).standard_normal(p.n_free)
+    db, dx = DeviceBuffer.from_numpy(b), DeviceBuffer.zeros(p.n_free)
+    ksp = DeviceKSP(p, method)
+    print(f"ksp: {kind} {method} n = {p.n_free}", flush=True)
+    it, rel = ksp.solve(dK, db, dx, rtol=1e-10)
+    x = dx.download(p.n_free)
+    assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b)
+    xr = spl.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
+    # a second solve with new values rebuilds the preconditioner numerically
+    dK2 = DeviceBuffer.from_numpy(K * 1.5)
+    dx2 = DeviceBuffer.zeros(p.n_free)
+    ksp.solve(dK2, db, dx2, rtol=1e-10)
+    assert np.linalg.norm(dx2.download(p.n_free) - xr / 1.5) <= 1e-6 * np.linalg.norm(xr)
+    print(f"ksp: {kind} {method}: {it} iterations, rel {rel:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("kind,strain,mode", [("c0", 0.01, "fp64"), ("op", 0.005, "fp64"), ("op", 0.005, "mixed"),
+                                              ("bl", 0.005, "mixed")])
+def test_device_newton_repeats_the_oracle(kind, strain, mode):
+    p = RVEProblem(dataclasses.replace(small_config(kind, n=30, n_incl=3, seed=4), strain=strain))
+    o = Oracle(p)
+    ref = NewtonDriver(p, lambda U, isv: o.assemble(U, isv))
+    ref.solve()
+    d = DeviceNewtonDriver(p, AssemblerGPU(p, mode), method="fgmres")
+    d.solve()
+    print(kind, mode, ref.iteration_counts(), d.iteration_counts(), "krylov", d.lin_iters[:8], flush=True)
+    assert d.iteration_counts() == ref.iteration_counts()
+    assert np.linalg.norm(d.U - ref.U) <= (1e-8 if mode == "fp64" else 1e-5) * np.linalg.norm(ref.U)
