@@ -46,7 +46,7 @@ typedef struct pgh_rve_params {
     double  sigma_c, delta_c, delta_c1, delta_c2, beta, alpha;
     double  strain;         /* applied u_x on RIGHT = strain * width            */
     int32_t max_tries;      /* rejection-sampling budget (0 = default)          */
-    int32_t flags;          /* PGH_PLACE_*                                      */
+    int32_t flags;          /* PGH_PLACE_* | PGH_PATTERN_DEVICE                 */
 } pgh_rve_params;
 
 /* Stop placing inclusions when the rejection budget is spent and keep those
@@ -56,6 +56,11 @@ typedef struct pgh_rve_params {
  * the elements around that node are cut, each with one node inside (in 3D the
  * interface is then made of CohTri3 triangles only). */
 #define PGH_PLACE_AT_NODES 2
+/* Build the pattern stage (node incidence, node-pair pattern, 4-block AIJ
+ * pattern, scatter indices) on the GPU with radix sorts of packed keys instead
+ * of on the host; the result is identical.  For a batch, the flag is read from
+ * params[0]. */
+#define PGH_PATTERN_DEVICE 4
 
 typedef struct pgh_problem pgh_problem;
 
@@ -71,6 +76,8 @@ int  pgh_get_desc(const pgh_problem* h, pg_problem_desc* out);
  * node-pair pattern, the scatter indices and the node incidence.  The full
  * description is then read with pgh_get_desc; the input arrays are copied. */
 int  pgh_build_pattern(const pg_problem_desc* mesh, pgh_problem** out);
+/* The same with flags (PGH_PATTERN_DEVICE; any other bit is rejected). */
+int  pgh_build_pattern_flags(const pg_problem_desc* mesh, int32_t flags, pgh_problem** out);
 void pgh_free(pgh_problem* h);
 const char* pgh_last_error(void);
 
@@ -95,6 +102,12 @@ const int32_t* pgh_node_detect(const pgh_problem* h);
 /* Prescribed value at full load of each prescribed dof (length n_pres); the
  * load-step value is time_fraction * value (src/Assembler.cpp:394-423). */
 const double* pgh_bc_values(const pgh_problem* h);
+/* Wall seconds of the set-up stages, PGH_N_STAGES values: mesh, placement,
+ * detection, materials + cut analysis + enrichment, dof numbering, pattern,
+ * and (batches only, where stages 0-4 stay 0) the parallel per-instance build
+ * plus the merge.  n must be at least PGH_N_STAGES. */
+#define PGH_N_STAGES 7
+int  pgh_stage_seconds(const pgh_problem* h, double* out, int32_t n);
 /* Per element: number of volumetric + cohesive quadrature points. */
 int  pgh_elem_qps(const pgh_problem* h, int32_t e, int32_t* n_vqp, int32_t* n_cqp);
 

@@ -24,7 +24,9 @@ PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
 PGH_PLACE_BEST_EFFORT = 1
 PGH_PLACE_AT_NODES = 2
+PGH_PATTERN_DEVICE = 4
 PGH_BATCH_COLS = 8
+PGH_N_STAGES = 7
 PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4 = range(3)
 PG_MAX_ENR, PG_MAX_CHILD, PG_MAX_COH = 12, 8, 3
 
@@ -120,6 +122,8 @@ SIGNATURES = {
     "pgh_build_rve": (C.c_int, [C.POINTER(PghRveParams), C.POINTER(C.c_void_p)]),
     "pgh_get_desc": (C.c_int, [C.c_void_p, C.POINTER(PgProblemDesc)]),
     "pgh_build_pattern": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
+    "pgh_build_pattern_flags": (C.c_int, [C.POINTER(PgProblemDesc), C.c_int32, C.POINTER(C.c_void_p)]),
+    "pgh_stage_seconds": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32]),
     "pgh_build_batch": (C.c_int, [C.POINTER(PghRveParams), C.c_int32, C.POINTER(C.c_void_p)]),
     "pgh_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.POINTER(C.c_int64))]),
     "pgh_free": (None, [C.c_void_p]),

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +14,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include "pattern_dev.h"
 
 namespace {
 
@@ -81,6 +84,7 @@ struct pgh_problem {
     // plus one closing row (PGH_BATCH_COLS columns)
     int32_t n_inst = 0;
     std::vector<int64_t> inst_off;
+    double stage_s[PGH_N_STAGES] = {};
 };
 
 namespace {
@@ -889,6 +893,55 @@ void build_pattern(pgh_problem& P) {
     }
 }
 
+// The pattern stage on the GPU (pattern_dev.hip): same outputs as build_pattern.
+void build_pattern_device(pgh_problem& P) {
+    pgh::PatternIn in;
+    in.dim = P.dim;
+    in.nen = P.nen;
+    in.n_nodes = P.n_nodes;
+    in.n_enr = P.n_enr;
+    in.n_elem = P.n_elem;
+    in.n_cut = int32_t(P.cut.size());
+    in.n_free = P.n_free;
+    in.n_pres = P.n_pres;
+    in.conn = P.conn.data();
+    in.cut_of_elem = P.cut_of_elem.data();
+    in.cut = P.cut.data();
+    in.node_dof = P.node_dof.data();
+    pgh::PatternOut o;
+    std::string err;
+    if (int rc = pgh::build_pattern_device(in, o, err)) fail(rc, "device pattern build: " + err);
+    P.n2e_ptr = std::move(o.n2e_ptr);
+    P.n2e = std::move(o.n2e);
+    P.npat_ptr = std::move(o.npat_ptr);
+    P.npat_col = std::move(o.npat_col);
+    P.npat_row = std::move(o.npat_row);
+    P.npat_map = std::move(o.npat_map);
+    for (int b = 0; b < 4; ++b) {
+        P.rp[b] = std::move(o.rp[b]);
+        P.ci[b] = std::move(o.ci[b]);
+    }
+}
+
+void build_pattern_any(pgh_problem& P, bool device) {
+    if (device) build_pattern_device(P);
+    else build_pattern(P);
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// runs one set-up stage and adds its wall time to P.stage_s[k]
+template <typename F>
+void timed(pgh_problem& P, int k, F&& f) {
+    const double t0 = now_s();
+    f();
+    P.stage_s[k] += now_s() - t0;
+}
+
+constexpr int32_t kKnownFlags = PGH_PLACE_BEST_EFFORT | PGH_PLACE_AT_NODES | PGH_PATTERN_DEVICE;
+
 }  // namespace
 
 extern "C" {
@@ -907,14 +960,18 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
         if (p->n < 1) fail(PG_E_ARG, "n must be >= 1");
         if (p->n_incl < 0 || p->r_min <= 0 || p->r_max < p->r_min || 2 * p->r_max >= 1.0)
             fail(PG_E_ARG, "bad inclusion parameters");
+        if (p->flags & ~kKnownFlags) fail(PG_E_ARG, "unknown flag bits");
         P->cohesive = p->cohesive != PGH_COH_NONE;
-        generate_mesh(*P);
-        place_inclusions(*P);
-        detect_nodes(*P);
-        build_materials(*P);
-        analyze_and_enrich(*P);
-        number_dofs(*P);
-        build_pattern(*P);
+        pgh_problem& Q = *P;
+        timed(Q, 0, [&] { generate_mesh(Q); });
+        timed(Q, 1, [&] { place_inclusions(Q); });
+        timed(Q, 2, [&] { detect_nodes(Q); });
+        timed(Q, 3, [&] {
+            build_materials(Q);
+            analyze_and_enrich(Q);
+        });
+        timed(Q, 4, [&] { number_dofs(Q); });
+        timed(Q, 5, [&] { build_pattern_any(Q, (p->flags & PGH_PATTERN_DEVICE) != 0); });
     } catch (const HarnessError& e) {
         g_err = e.what();
         delete P;
@@ -928,9 +985,12 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
     return PG_OK;
 }
 
-int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) {
+int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) { return pgh_build_pattern_flags(m, 0, out); }
+
+int pgh_build_pattern_flags(const pg_problem_desc* m, int32_t flags, pgh_problem** out) {
     if (!m || !out) { g_err = "null argument"; return PG_E_ARG; }
     *out = nullptr;
+    if (flags & ~PGH_PATTERN_DEVICE) { g_err = "unknown flag bits"; return PG_E_ARG; }
     if ((m->dim != 2 && m->dim != 3) || m->nen != m->dim + 1 || m->n_nodes <= 0 || m->n_elem <= 0 || m->n_enr < 0 ||
         m->n_cut < 0 || m->n_mat <= 0 || m->n_free < 0 || m->n_pres < 0 ||
         m->n_free + m->n_pres != (m->n_nodes + m->n_enr) * m->dim) {
@@ -972,9 +1032,17 @@ int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) {
             for (int k = 0; k < R.n_enr; ++k)
                 if (R.enr[k] < m->n_nodes || R.enr[k] >= nall) fail(PG_E_ARG, "enrichment node out of range");
         }
-        for (int64_t k = 0; k < nall * m->dim; ++k)
-            if (P->node_dof[k] < 0 || P->node_dof[k] >= m->n_free + m->n_pres) fail(PG_E_ARG, "dof out of range");
-        build_pattern(*P);
+        {
+            // every dof belongs to exactly one (node, component)
+            std::vector<uint8_t> seen(size_t(m->n_free) + m->n_pres, 0);
+            for (int64_t k = 0; k < nall * m->dim; ++k) {
+                const int32_t g = P->node_dof[k];
+                if (g < 0 || g >= m->n_free + m->n_pres) fail(PG_E_ARG, "dof out of range");
+                if (seen[g]++) fail(PG_E_ARG, "dof numbered twice");
+            }
+        }
+        pgh_problem& Q = *P;
+        timed(Q, 5, [&] { build_pattern_any(Q, (flags & PGH_PATTERN_DEVICE) != 0); });
     } catch (const HarnessError& e) {
         g_err = e.what();
         delete P;
@@ -991,6 +1059,9 @@ int pgh_build_pattern(const pg_problem_desc* m, pgh_problem** out) {
 int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) {
     if (!params || !out || n <= 0) { g_err = "bad argument"; return PG_E_ARG; }
     *out = nullptr;
+    for (int32_t i = 0; i < n; ++i)
+        if (params[i].flags & ~kKnownFlags) { g_err = "unknown flag bits"; return PG_E_ARG; }
+    const double t_start = now_s();
     std::vector<pgh_problem*> inst(size_t(n), nullptr);
     std::vector<int> code(size_t(n), 0);
     std::vector<std::string> msg((size_t)n);
@@ -1127,7 +1198,9 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
         }
         cleanup();
         inst.clear();
-        build_pattern(*B);
+        B->stage_s[6] = now_s() - t_start;
+        pgh_problem& Q = *B;
+        timed(Q, 5, [&] { build_pattern_any(Q, (params[0].flags & PGH_PATTERN_DEVICE) != 0); });
     } catch (const HarnessError& e) {
         g_err = e.what();
         cleanup();
@@ -1200,6 +1273,12 @@ int pgh_inclusions(const pgh_problem* h, const double** centers, const double** 
 const int32_t* pgh_node_detect(const pgh_problem* h) { return h ? h->node_detect.data() : nullptr; }
 
 const double* pgh_bc_values(const pgh_problem* h) { return h ? h->bc_val.data() : nullptr; }
+
+int pgh_stage_seconds(const pgh_problem* h, double* out, int32_t n) {
+    if (!h || !out || n < PGH_N_STAGES) { g_err = "bad argument"; return PG_E_ARG; }
+    std::copy(h->stage_s, h->stage_s + PGH_N_STAGES, out);
+    return PG_OK;
+}
 
 int pgh_elem_qps(const pgh_problem* h, int32_t e, int32_t* n_vqp, int32_t* n_cqp) {
     if (!h || e < 0 || e >= h->n_elem) return PG_E_ARG;

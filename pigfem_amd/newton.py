@@ -273,16 +273,28 @@ class DeviceNewtonDriver(NewtonDriver):
     with the mesh); the residual norm is a
     device dot product.  Only scalars, the prescribed values at the start of an
     attempt and U / ISV snapshots of converged steps cross PCIe.  `asm` is an
-    AssemblerGPU (any mode)."""
+    AssemblerGPU (any mode).
 
-    def __init__(self, problem, asm, method="fgmres", lin_rtol=1e-10, lin_max_iter=20000, **kw):
+    method: "cg", "fgmres", "bicgstab" or "auto".  "auto" solves with CG
+    (at most cg_max_iter iterations) while the tangent lets it converge --
+    a tangent without softening is symmetric positive definite -- and from the
+    first CG failure on (an indefinite tangent after the onset of cohesive
+    softening) with FGMRES; the failed attempt restarts from zero."""
+
+    def __init__(self, problem, asm, method="auto", lin_rtol=1e-10, lin_max_iter=20000, cg_max_iter=5000, **kw):
         from .device import DeviceBuffer
         from .solve import DeviceKSP
         kw["solver"] = "none"
         super().__init__(problem, None, **kw)
         self.lin.kind = f"gpu-{method}"
         self.asm = asm
-        self.ksp = DeviceKSP(problem, method)
+        if method == "auto":
+            self.ksp = DeviceKSP(problem, "cg")
+            self.ksp_fallback = DeviceKSP(problem, "fgmres")
+        else:
+            self.ksp = DeviceKSP(problem, method)
+            self.ksp_fallback = None
+        self.cg_max_iter = cg_max_iter
         self.lin_rtol, self.lin_max_iter = lin_rtol, lin_max_iter
         p = problem
         self.dU = DeviceBuffer.zeros(p.n_dof)
@@ -308,8 +320,19 @@ class DeviceNewtonDriver(NewtonDriver):
         """U_f -= du with K_ff du = P_f (the Newton update du = K^-1 (F_ext - P))."""
         from .solve import dev_axpy
         p = self.p
+        from ._ffi import PgError
         self.ddu.copy_from(self.dzero)
-        it, _ = self.ksp.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.lin_max_iter)
+        if self.ksp_fallback is not None and self.ksp is not None:
+            try:
+                it, _ = self.ksp.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.cg_max_iter)
+            except PgError:
+                self.ksp.close()
+                self.ksp = None
+                self.ddu.copy_from(self.dzero)
+                it, _ = self.ksp_fallback.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.lin_max_iter)
+        else:
+            ksp = self.ksp if self.ksp is not None else self.ksp_fallback
+            it, _ = ksp.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.lin_max_iter)
         self.lin_iters.append(it)
         dev_axpy(p.n_free, -1.0, self.ddu, self.dU)
 

@@ -130,18 +130,22 @@ class RVEProblem:
     """A built RVE (or a batch of RVEs merged into one block-diagonal problem):
     the static problem description plus numpy views of it."""
 
-    def __init__(self, cfg: RVEConfig, _cfgs=None):
+    def __init__(self, cfg: RVEConfig, _cfgs=None, device_pattern: bool = False):
+        """device_pattern: build the pattern stage (incidence, node-pair and
+        AIJ patterns, scatter indices) on the GPU; the result is identical."""
         self.cfg = cfg
         L = lib()
         h = C.c_void_p()
+        dev = _ffi.PGH_PATTERN_DEVICE if device_pattern else 0
         if _cfgs is None:
             self._params = cfg.params()
+            self._params.flags |= dev
             check(L.pgh_build_rve(C.byref(self._params), C.byref(h)), "pgh_build_rve")
         else:
             arr = (PghRveParams * len(_cfgs))()
             for i, c in enumerate(_cfgs):
                 arr[i] = c.params()
-                arr[i].flags = c.place | _ffi.PGH_PLACE_BEST_EFFORT
+                arr[i].flags = c.place | _ffi.PGH_PLACE_BEST_EFFORT | dev
                 arr[i].max_tries = 2_000_000
             self._params = arr
             check(L.pgh_build_batch(arr, len(_cfgs), C.byref(h)), "pgh_build_batch")
@@ -186,9 +190,18 @@ class RVEProblem:
                          .reshape(ni.value + 1, _ffi.PGH_BATCH_COLS) if ni.value else None)
 
     @classmethod
-    def batch(cls, cfgs):
+    def batch(cls, cfgs, device_pattern: bool = False):
         """Merge independent RVEs into one problem (pgh_build_batch)."""
-        return cls(cfgs[0], _cfgs=list(cfgs))
+        return cls(cfgs[0], _cfgs=list(cfgs), device_pattern=device_pattern)
+
+    STAGES = ("mesh", "place", "detect", "enrich", "dofs", "pattern", "instances")
+
+    def stage_seconds(self) -> dict:
+        """Wall seconds of each set-up stage (pgh_stage_seconds); 'instances'
+        is the parallel per-instance build of a batch."""
+        out = (C.c_double * _ffi.PGH_N_STAGES)()
+        check(lib().pgh_stage_seconds(self._h, out, _ffi.PGH_N_STAGES), "pgh_stage_seconds")
+        return {k: float(out[i]) for i, k in enumerate(self.STAGES)}
 
     def instance_ranges(self, i: int):
         """Value ranges of instance i in the concatenated K [ff|fp|pf|pp] and P [f|p]."""

@@ -46,15 +46,16 @@ def test_device_solve_of_the_tangent_in_place(kind, method):
     print(f"ksp: {kind} {method}: {it} iterations, rel {rel:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("kind,strain,mode", [("c0", 0.01, "fp64"), ("op", 0.005, "fp64"), ("op", 0.005, "mixed"),
-                                              ("bl", 0.005, "mixed")])
-def test_device_newton_repeats_the_oracle(kind, strain, mode):
+@pytest.mark.parametrize("kind,strain,mode,method", [("c0", 0.01, "fp64", "auto"), ("op", 0.005, "fp64", "auto"),
+                                                     ("op", 0.005, "mixed", "auto"), ("bl", 0.005, "mixed", "fgmres")])
+def test_device_newton_repeats_the_oracle(kind, strain, mode, method):
     p = RVEProblem(dataclasses.replace(small_config(kind, n=30, n_incl=3, seed=4), strain=strain))
     o = Oracle(p)
     ref = NewtonDriver(p, lambda U, isv: o.assemble(U, isv))
     ref.solve()
-    d = DeviceNewtonDriver(p, AssemblerGPU(p, mode), method="fgmres")
+    d = DeviceNewtonDriver(p, AssemblerGPU(p, mode), method=method)
     d.solve()
-    print(kind, mode, ref.iteration_counts(), d.iteration_counts(), "krylov", d.lin_iters[:8], flush=True)
+    print(kind, mode, method, ref.iteration_counts(), d.iteration_counts(), "krylov", d.lin_iters[:8],
+          "cg" if d.ksp is not None else "fgmres", flush=True)
     assert d.iteration_counts() == ref.iteration_counts()
     assert np.linalg.norm(d.U - ref.U) <= (1e-8 if mode == "fp64" else 1e-5) * np.linalg.norm(ref.U)

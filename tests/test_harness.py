@@ -214,3 +214,26 @@ def test_build_pattern_from_external_mesh(kind):
         assert L.pgh_build_pattern(C.byref(bad), C.byref(h2)) != 0
     finally:
         L.pgh_free(h)
+
+
+def test_pattern_flags_and_stage_times():
+    """Unknown flag bits and a dof numbered twice are rejected; the stage
+    timer reports every stage of a build."""
+    import ctypes as C
+    from pigfem_amd import _ffi
+    from pigfem_amd.problem import RVEConfig
+    with pytest.raises(Exception, match="unknown flag"):
+        RVEProblem(RVEConfig(dim=2, n=8, n_incl=1, r_min=0.1, r_max=0.1, place=64))
+    p = RVEProblem(small_config("op", n=24, n_incl=3))
+    st = p.stage_seconds()
+    assert set(st) == set(RVEProblem.STAGES) and st["pattern"] > 0 and st["instances"] == 0
+    L = _ffi.lib()
+    m = _ffi.PgProblemDesc()
+    C.memmove(C.addressof(m), C.addressof(p.desc), C.sizeof(m))
+    h = C.c_void_p()
+    assert L.pgh_build_pattern_flags(C.byref(m), 8, C.byref(h)) == _ffi.PG_E_ARG
+    dup = np.array(p.node_dof, copy=True)
+    dup[1, 0] = dup[0, 0]
+    m.node_dof = dup.ctypes.data_as(C.POINTER(C.c_int32))
+    assert L.pgh_build_pattern(C.byref(m), C.byref(h)) == _ffi.PG_E_ARG
+    assert b"twice" in L.pgh_last_error()
