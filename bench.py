@@ -42,21 +42,25 @@ def workload(cfg, batch, uscale):
             f"{law}, U~U[-{uscale:g},{uscale:g}]")
 
 
-def build_problem(name):
+def build_problem(name, device_pattern=True):
+    """The workload's set-up; the pattern stage runs on the GPU unless
+    device_pattern is False.  Returns the problem, its config, the batch's
+    configs (or None) and the set-up wall seconds (total and per stage)."""
     import dataclasses
     from pigfem_amd import RVEProblem, batch_config, config
     t0 = time.time()
     if name.upper() == "C5":
         cfgs = batch_config(256)
-        prob = RVEProblem.batch(cfgs)
+        prob = RVEProblem.batch(cfgs, device_pattern=device_pattern)
         cfg = dataclasses.replace(cfgs[0], name="C5")
     else:
         cfgs = None
         cfg = config(name)
-        prob = RVEProblem(cfg)
-    setup_s = time.time() - t0
-    log(f"[bench] {cfg.name} built in {setup_s:.1f}s: {prob.stats()}")
-    return prob, cfg, cfgs, setup_s
+        prob = RVEProblem(cfg, device_pattern=device_pattern)
+    setup = {"total": time.time() - t0, "pattern_on": "gpu" if device_pattern else "host"}
+    setup.update({k: round(v, 4) for k, v in prob.stage_seconds().items()})
+    log(f"[bench] {cfg.name} built in {setup['total']:.1f}s ({setup}): {prob.stats()}")
+    return prob, cfg, cfgs, setup
 
 
 def node_pair_kernels(dim, tiles, grid_nodes):
@@ -302,6 +306,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--per-instance", action="store_true", help="C5 main config: also time the instances one by one")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--host-pattern", action="store_true",
+                    help="build the pattern stage of the set-up on the host instead of the GPU")
     ap.add_argument("--tiles", default=None,
                     help="group_nodes,cut_variant,group_variant for every schedule instead of autotuning "
                          "(short profiler runs)")
@@ -311,7 +317,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    prob, cfg, cfgs, setup_s = build_problem(args.config)
+    prob, cfg, cfgs, setup = build_problem(args.config, not args.host_pattern)
     batch = cfgs is not None
     st = prob.stats()
     m = measure(prob, cfg, args, full=True)
@@ -372,7 +378,8 @@ def main():
         "algorithmic_bytes": m["B_alg"],
         "cut_kernel_ms": m["cut_kernel_ms"],
         "grid_nodes": m["grid_nodes"],
-        "setup_s": setup_s,
+        "setup_s": setup["total"],
+        "setup_stages_s": setup,
         "host_api_ms_per_call": m["host_ms"],
         "device_bytes": m["device_bytes"],
         "tiles": m["tuned"],
@@ -386,7 +393,7 @@ def main():
     for name in [x.strip().upper() for x in args.extra.split(",") if x.strip()]:
         if name == args.config.upper():
             continue
-        p2, c2, cf2, s2 = build_problem(name)
+        p2, c2, cf2, s2 = build_problem(name, not args.host_pattern)
         m2 = measure(p2, c2, args, full=False)
         r2 = m2["results"]
         out[name] = {
@@ -398,7 +405,8 @@ def main():
             "speedup_vs_binned64": r2["binned64"]["ms"] / r2["mixed"]["ms"],
             "rel_err_K": m2["acc"]["rel_err_K"], "rel_err_P": m2["acc"]["rel_err_P"],
             "roofline": m2["roofline"], "whole_assembly_frac": m2["whole_GBps"] / HBM_PEAK_GBS,
-            "cut_kernel_ms": m2["cut_kernel_ms"], "grid_nodes": m2["grid_nodes"], "setup_s": s2, "tiles": m2["tuned"],
+            "cut_kernel_ms": m2["cut_kernel_ms"], "grid_nodes": m2["grid_nodes"], "setup_s": s2["total"],
+            "setup_stages_s": s2, "tiles": m2["tuned"],
         }
         del p2, m2
         gc.collect()
