@@ -80,15 +80,28 @@ typedef struct pg_material {
  * the reference: 0..nen-1 parent nodes, then enrichment nodes in cut-edge order,
  * then (cohesive meshes) their mirror nodes (src/tri3.cpp:287-596,
  * src/tet4.cpp:228-520, src/elem.cpp:428-559).  A child is an integration
- * sub-element (tri3 / quad4 / tet4) listed by local node index; a cohesive
- * child is a CohEdge2 (4 local nodes: 2 lower then 2 upper) or CohTri3 (6).
+ * sub-element listed by local node index: tri3, quad4 or a 5- / 6-node polygon
+ * (a tri3 cut by two or three inclusions, src/tri3.cpp:376-452,
+ * src/Polygon.cpp:51-228) in 2D, tet4 in 3D.  Child k's nodes start at
+ * child_node[PG_CHILD_STRIDE(dim) * k]: at most 4 children of at most 6 nodes
+ * in 2D, at most 8 tet4 children in 3D.  A cohesive child is a CohEdge2
+ * (4 local nodes: 2 lower then 2 upper) or CohTri3 (6).
  * ------------------------------------------------------------------------- */
 #define PG_CHILD_TRI3   0
 #define PG_CHILD_QUAD4  1
 #define PG_CHILD_TET4   2
+#define PG_CHILD_POLY5  3   /* pentagon: Wachspress shapes, 5 x 4 qps */
+#define PG_CHILD_POLY6  4   /* hexagon: 6 x 4 qps */
 #define PG_MAX_ENR     12
 #define PG_MAX_CHILD    8
 #define PG_MAX_COH      3
+#define PG_CHILD_STRIDE(dim) ((dim) == 2 ? 8 : 4)
+/* nodes and volumetric qps of a child type.  A polygon is integrated with the
+ * tri3 order-3 rule on each of its sub-triangles: the reference calls
+ * n_q_points() / q_point() through Elem*, whose default order (3) applies,
+ * not Polygon's own default of 5 (include/elem.h:66-67, include/Polygon.h:42-43). */
+#define PG_CHILD_NN(t)  ((t) == PG_CHILD_TRI3 ? 3 : (t) == PG_CHILD_POLY5 ? 5 : (t) == PG_CHILD_POLY6 ? 6 : 4)
+#define PG_CHILD_NQP(t) ((t) == PG_CHILD_TET4 ? 5 : (t) == PG_CHILD_POLY5 ? 20 : (t) == PG_CHILD_POLY6 ? 24 : 4)
 
 typedef struct pg_cut_rec {
     int32_t enr[PG_MAX_ENR];        /* global node index of each local enrichment node */
@@ -99,7 +112,7 @@ typedef struct pg_cut_rec {
     uint8_t n_vqp;                  /* volumetric qps (sum over children)               */
     uint8_t child_type[PG_MAX_CHILD];
     uint8_t child_mat[PG_MAX_CHILD];
-    uint8_t child_nodes[PG_MAX_CHILD][4];
+    uint8_t child_node[32];         /* child k's nodes from PG_CHILD_STRIDE(dim) * k    */
     uint8_t coh_nodes[PG_MAX_COH][6];
     uint8_t coh_mat[PG_MAX_COH];
     uint8_t coh_nn;                 /* nodes per cohesive child: 4 (CohEdge2) or 6 (CohTri3) */
@@ -278,7 +291,8 @@ int64_t pg_asm_grid_nodes(const pg_asm* h);
  * WriterCohesiveOpenings::writeFromStream (src/WriterCohesiveOpenings.cpp:73-147)
  * before its sort.  For cohesive edge k of cut record c, at row isv_off / 2 + k
  * of out (3 doubles): the angle of the edge midpoint (its lower nodes) about
- * centers[c] (dim doubles: the centre of the inclusion that cuts the element),
+ * centers[isv_off / 2 + k] (2 doubles: the centre of the edge's inclusion, the
+ * one of its first lower node's cut-edge slot, src/WriterCohesiveOpenings.cpp:128),
  * then the opening at the midpoint in the edge frame, tangential and normal,
  * each divided by the law's delta_c.  U: global-dof order.  All pointers are
  * device pointers; asynchronous on stream (0: the handle's stream).

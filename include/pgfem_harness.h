@@ -97,6 +97,11 @@ int  pgh_batch_info(const pgh_problem* h, int32_t* n_inst, const int64_t** offse
 /* Inclusions actually placed: centers (dim each) and radii. */
 int  pgh_inclusions(const pgh_problem* h, const double** centers,
                     const double** radii, int32_t* n);
+/* Inclusion of each enrichment node (n_enr values, mirrors included: the
+ * inclusion of the cut-edge slot it was made for, the reference's
+ * Elem::get_inclusion_from_enrich); NULL for a pattern built from an external
+ * mesh, whose inclusions the harness does not know. */
+const int32_t* pgh_enrich_inclusion(const pgh_problem* h);
 /* Nodal detection value per regular node (-1 matrix, else inclusion index). */
 const int32_t* pgh_node_detect(const pgh_problem* h);
 /* Prescribed value at full load of each prescribed dof (length n_pres); the

@@ -90,10 +90,82 @@ static const double TET_X[5][3] = {{0.25, 0.25, 0.25},
 static const double TET_W[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
 
 /* q_point + compute_shape_grad for the element types on the path */
-static int nqp_of(int type) { return type == PG_CHILD_TET4 ? 5 : 4; }
+static int nqp_of(int type) { return PG_CHILD_NQP(type); }
+
+/* PI of include/common.h:215 */
+static const double PG_PI = 3.14159265358979323846264338327950;
+
+/* Polygon::PolyTriangulate (src/Polygon.cpp:51-63): vertex n of the regular
+ * reference polygon at angle 2 pi (n + 1) / nn; sub-triangle t is (centre,
+ * vertex t, vertex t + 1), the last one closing on vertex 0. */
+static void poly_vertex(int nn, int n, double* p) {
+    if (n == nn) n = 0;
+    p[0] = cos(2.0 * PG_PI * (n + 1) / nn);
+    p[1] = sin(2.0 * PG_PI * (n + 1) / nn);
+}
+
+/* Polygon::q_point (src/Polygon.cpp:84-131) with the tri3 rule of order 3
+ * (the Elem* default): qp q is point q % 4 of sub-triangle q / 4 around the
+ * origin; the weight is the tri3 weight times the sub-triangle's Jacobian. */
+static void poly_qpoint(int nn, int q, double* xi, double* w) {
+    const int t = q / 4, qq = q % 4;
+    const double tN[3] = {1.0 - TRI_X[qq] - TRI_Y[qq], TRI_X[qq], TRI_Y[qq]};
+    const double tdN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    double pT[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    poly_vertex(nn, t, pT[1]);
+    poly_vertex(nn, t + 1, pT[2]);
+    double J0[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int n = 0; n < 3; ++n) J0[i][j] += pT[n][i] * tdN[n][j];
+    const double det_J0 = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
+    *w = det_J0 * TRI_W[qq];
+    xi[0] = 0.0;
+    xi[1] = 0.0;
+    for (int n = 0; n < 3; ++n) {
+        xi[0] += tN[n] * pT[n][0];
+        xi[1] += tN[n] * pT[n][1];
+    }
+}
+
+/* Polygon::compute_shape / compute_shape_grad (src/Polygon.cpp:133-222):
+ * with A_n the signed area of (x, vertex n - 1, vertex n) (A_0 = A_nn),
+ * alpha_n = 1 / (A_n A_{n+1}), N_n = alpha_n / sum alpha and its gradient
+ * from dA / dx. */
+static void poly_shape(int nn, const double* xi, double* N, double dN[][3]) {
+    double A[7], dA[7][2], alpha[6], dalpha[6][2];
+    for (int n = 0; n < nn; ++n) {
+        double p1[2], p2[2];
+        poly_vertex(nn, n, p1);
+        poly_vertex(nn, n + 1, p2);
+        A[n + 1] = 0.5 * ((p1[0] * p2[1] - p1[1] * p2[0]) - (xi[0] * p2[1] - xi[1] * p2[0]) +
+                          (xi[0] * p1[1] - xi[1] * p1[0]));
+        dA[n + 1][0] = 0.5 * (p2[1] - p1[1]);
+        dA[n + 1][1] = 0.5 * (p1[0] - p2[0]);
+    }
+    A[0] = A[nn];
+    dA[0][0] = dA[nn][0];
+    dA[0][1] = dA[nn][1];
+    double sum_alpha = 0.0, sum_dalpha[2] = {0.0, 0.0};
+    for (int n = 0; n < nn; ++n) {
+        alpha[n] = 1.0 / (A[n] * A[n + 1]);
+        dalpha[n][0] = -1.0 * alpha[n] * (dA[n][0] / A[n] + dA[n + 1][0] / A[n + 1]);
+        dalpha[n][1] = -1.0 * alpha[n] * (dA[n][1] / A[n] + dA[n + 1][1] / A[n + 1]);
+        sum_alpha += alpha[n];
+        sum_dalpha[0] += dalpha[n][0];
+        sum_dalpha[1] += dalpha[n][1];
+    }
+    for (int n = 0; n < nn; ++n) {
+        const double Nn = alpha[n] / sum_alpha;
+        N[n] = Nn;
+        dN[n][0] = (dalpha[n][0] - Nn * sum_dalpha[0]) / sum_alpha;
+        dN[n][1] = (dalpha[n][1] - Nn * sum_dalpha[1]) / sum_alpha;
+    }
+}
 
 static void qpoint(int type, int q, double* xi, double* w) {
-    if (type == PG_CHILD_TRI3) { xi[0] = TRI_X[q]; xi[1] = TRI_Y[q]; *w = TRI_W[q]; }
+    if (type == PG_CHILD_POLY5 || type == PG_CHILD_POLY6) poly_qpoint(PG_CHILD_NN(type), q, xi, w);
+    else if (type == PG_CHILD_TRI3) { xi[0] = TRI_X[q]; xi[1] = TRI_Y[q]; *w = TRI_W[q]; }
     else if (type == PG_CHILD_QUAD4) {
         /* Quad4::q_point: edge qp (q % 2) then (q / 2), w = w0 * w1 */
         xi[0] = (q % 2 == 0) ? -G2 : G2;
@@ -103,7 +175,9 @@ static void qpoint(int type, int q, double* xi, double* w) {
 }
 
 static void shape(int type, const double* xi, double* N, double dN[][3]) {
-    if (type == PG_CHILD_TRI3) {
+    if (type == PG_CHILD_POLY5 || type == PG_CHILD_POLY6) {
+        poly_shape(PG_CHILD_NN(type), xi, N, dN);
+    } else if (type == PG_CHILD_TRI3) {
         double r = xi[0], s = xi[1];
         N[0] = 1.0 - r - s; N[1] = r; N[2] = s;
         dN[0][0] = -1.0; dN[0][1] = -1.0; dN[1][0] = 1.0; dN[1][1] = 0.0; dN[2][0] = 0.0; dN[2][1] = 1.0;
@@ -123,7 +197,7 @@ static void shape(int type, const double* xi, double* N, double dN[][3]) {
     }
 }
 
-static int nn_of(int type) { return type == PG_CHILD_TRI3 ? 3 : 4; }
+static int nn_of(int type) { return PG_CHILD_NN(type); }
 
 /* Elem::transform_gradients + Utilities::det / inverse */
 static double transform(int dim, int nn, double dNdr[][3], double x[][3], double dNdx[][3]) {
@@ -234,9 +308,10 @@ static int store_shapes(oracle* o, int e, double* out) {
     int nd = nl * dim;
     for (int k = 0; k < r->n_child; ++k) {
         int ct = r->child_type[k], cnn = nn_of(ct);
-        double cx[4][3], cg[4][3], cN[4];
+        const uint8_t* cl = &r->child_node[PG_CHILD_STRIDE(dim) * k];
+        double cx[6][3], cg[6][3], cN[6];
         for (int n = 0; n < cnn; ++n)
-            for (int i = 0; i < 3; ++i) cx[n][i] = x[r->child_nodes[k][n]][i];
+            for (int i = 0; i < 3; ++i) cx[n][i] = x[cl[n]][i];
         for (int q = 0; q < nqp_of(ct); ++q) {
             qpoint(ct, q, xi, &w);
             shape(ct, xi, cN, dN);
@@ -294,7 +369,7 @@ static int store_shapes(oracle* o, int e, double* out) {
                 for (int a = 0; a < 3; ++a) g[n][a] = n < nen ? pg_[n][a] : 0.0;
             for (int en = 0; en < r->n_enr; ++en)
                 for (int n = 0; n < cnn; ++n)
-                    if (r->child_nodes[k][n] == nen + en) {
+                    if (cl[n] == nen + en) {
                         for (int a = 0; a < dim; ++a) g[nen + en][a] = 1.0 * cg[n][a];
                         break;
                     }
@@ -882,4 +957,33 @@ int64_t oracle_elem_cache(const oracle* o, int e, const double** out) {
     if (e < o->e0 || e >= o->e1) return -1;
     *out = o->cache + o->off[e - o->e0];
     return o->off[e - o->e0 + 1] - o->off[e - o->e0];
+}
+
+/* Test access to the child shape data: for child type `type` with physical
+ * node coordinates cx (nn x 3, row-major), qp q gives the reference point xi,
+ * the weight w, the shape values N, the physical gradients grad (nn x 3) and
+ * the Jacobian J (Elem::q_point + Elem::ShapeFunctions). */
+int oracle_child_qp(int dim, int type, const double* cx, int q, double* xi, double* w, double* N, double* grad,
+                    double* J) {
+    const int nn = nn_of(type);
+    if (q < 0 || q >= nqp_of(type)) return -1;
+    double x[6][3], dN[6][3] = {{0}}, g[6][3];
+    for (int n = 0; n < nn; ++n)
+        for (int i = 0; i < 3; ++i) x[n][i] = cx[n * 3 + i];
+    qpoint(type, q, xi, w);
+    shape(type, xi, N, dN);
+    *J = transform(dim, nn, dN, x, g);
+    for (int n = 0; n < nn; ++n)
+        for (int i = 0; i < 3; ++i) grad[n * 3 + i] = i < dim ? g[n][i] : 0.0;
+    return 0;
+}
+
+/* Polygon shape values and reference gradients (nn x 2) at reference point xi. */
+void oracle_polygon_shape(int nn, const double* xi, double* N, double* dN) {
+    double d[6][3];
+    poly_shape(nn, xi, N, d);
+    for (int n = 0; n < nn; ++n) {
+        dN[n * 2] = d[n][0];
+        dN[n * 2 + 1] = d[n][1];
+    }
 }

@@ -144,6 +144,34 @@ def cohesive_law(mat, dim, delta, isv=0.0):
     return d, float(h[0]), t, D.reshape(dim, dim)
 
 
+def child_qp(dim, ctype, X, q):
+    """Reference point, weight, shape values, physical gradients and Jacobian
+    of qp q of a child of type ctype with node coordinates X (nn x dim)."""
+    L = lib()
+    L.oracle_child_qp.restype = C.c_int
+    L.oracle_child_qp.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 5
+    nn = len(X)
+    cx = np.zeros((nn, 3))
+    cx[:, :dim] = X
+    xi, w, N, g, J = np.zeros(3), np.zeros(1), np.zeros(nn), np.zeros((nn, 3)), np.zeros(1)
+    rc = L.oracle_child_qp(dim, ctype, cx.ctypes.data, q, xi.ctypes.data, w.ctypes.data, N.ctypes.data,
+                           g.ctypes.data, J.ctypes.data)
+    if rc:
+        raise ValueError("qp out of range")
+    return xi[:dim], float(w[0]), N, g[:, :dim], float(J[0])
+
+
+def polygon_shape(nn, xi):
+    """Polygon shape values and reference gradients at reference point xi."""
+    L = lib()
+    L.oracle_polygon_shape.restype = None
+    L.oracle_polygon_shape.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    x = np.ascontiguousarray(xi, dtype=np.float64)
+    N, dN = np.zeros(nn), np.zeros((nn, 2))
+    L.oracle_polygon_shape(nn, x.ctypes.data, N.ctypes.data, dN.ctypes.data)
+    return N, dN
+
+
 def elem_cache(orc, e):
     L = lib()
     L.oracle_elem_cache.restype = C.c_int64
@@ -164,7 +192,6 @@ def cohesive_openings(problem, U):
     src/Utilities.cpp:528-545) in the edge frame of CohEdge2
     (src/CohesiveElem.cpp:145-238: t = dx/dr over |t|, n = (-t_y, t_x) over |n|),
     tangential then normal, each over delta_c of the Ortiz-Pandolfi law."""
-    det = problem.node_detect
     Un = np.asarray(U, np.float64)[problem.node_dof]
     rows = np.zeros((problem.n_cqp // 2, 3))
     inc = np.zeros(problem.n_cqp // 2, np.int64)
@@ -172,8 +199,10 @@ def cohesive_openings(problem, U):
         r = problem.cut_record(c)
         e = int(problem.cut_elem[c])
         gl = [int(x) for x in problem.conn[e]] + [int(r.enr[k]) for k in range(r.n_enr)]
-        ic = int(det[problem.conn[e]].max())
         for k in range(r.n_coh):
+            # the inclusion of the edge's first lower (enrichment) node's cut-edge
+            # slot, el->get_inclusion_from_enrich (src/WriterCohesiveOpenings.cpp:128)
+            ic = int(problem.enr_incl[gl[r.coh_nodes[k][0]] - problem.n_nodes])
             dc = problem.desc.mats[r.coh_mat[k]].delta_c
             lo = [gl[r.coh_nodes[k][n]] for n in range(2)]
             up = [gl[r.coh_nodes[k][n + 2]] for n in range(2)]

@@ -27,8 +27,10 @@ PGH_PLACE_AT_NODES = 2
 PGH_PATTERN_DEVICE = 4
 PGH_BATCH_COLS = 8
 PGH_N_STAGES = 7
-PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4 = range(3)
+PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4, PG_CHILD_POLY5, PG_CHILD_POLY6 = range(5)
 PG_MAX_ENR, PG_MAX_CHILD, PG_MAX_COH = 12, 8, 3
+PG_CHILD_NN = {PG_CHILD_TRI3: 3, PG_CHILD_QUAD4: 4, PG_CHILD_TET4: 4, PG_CHILD_POLY5: 5, PG_CHILD_POLY6: 6}
+PG_CHILD_NQP = {PG_CHILD_TRI3: 4, PG_CHILD_QUAD4: 4, PG_CHILD_TET4: 5, PG_CHILD_POLY5: 20, PG_CHILD_POLY6: 24}
 
 
 class PgMaterial(C.Structure):
@@ -44,13 +46,19 @@ class PgCutRec(C.Structure):
         ("enr", C.c_int32 * PG_MAX_ENR), ("isv_off", C.c_int32),
         ("n_enr", C.c_uint8), ("n_child", C.c_uint8), ("n_coh", C.c_uint8), ("n_vqp", C.c_uint8),
         ("child_type", C.c_uint8 * PG_MAX_CHILD), ("child_mat", C.c_uint8 * PG_MAX_CHILD),
-        ("child_nodes", (C.c_uint8 * 4) * PG_MAX_CHILD),
+        ("child_node", C.c_uint8 * 32),
         ("coh_nodes", (C.c_uint8 * 6) * PG_MAX_COH), ("coh_mat", C.c_uint8 * PG_MAX_COH),
         ("coh_nn", C.c_uint8), ("pad", C.c_uint8 * 2),
     ]
 
 
 assert C.sizeof(PgCutRec) == 128
+
+
+def child_nodes(rec, dim: int, k: int) -> list:
+    """Element-local nodes of child k of a cut record (PG_CHILD_STRIDE layout)."""
+    st = 8 if dim == 2 else 4
+    return [int(rec.child_node[st * k + n]) for n in range(PG_CHILD_NN[int(rec.child_type[k])])]
 
 
 class PgProblemDesc(C.Structure):
@@ -131,6 +139,7 @@ SIGNATURES = {
     "pgh_inclusions": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
                                  C.POINTER(C.c_int32)]),
     "pgh_node_detect": (C.POINTER(C.c_int32), [C.c_void_p]),
+    "pgh_enrich_inclusion": (C.POINTER(C.c_int32), [C.c_void_p]),
     "pgh_bc_values": (C.POINTER(C.c_double), [C.c_void_p]),
     "pgh_elem_qps": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 }

@@ -158,21 +158,16 @@ class AssemblerGPU:
         from .device import DeviceBuffer, synchronize
         p = self.problem
         n_out = p.n_cqp // 2
-        incl = cut_inclusions(p)
-        centers = np.ascontiguousarray(p.incl_centers[np.maximum(incl, 0)], dtype=np.float64)
         rows = np.zeros((n_out, 3))
         if n_out == 0:
             return np.zeros(0, np.int64), rows
+        inc = p.edge_inclusions()
+        centers = np.ascontiguousarray(p.incl_centers[inc], dtype=np.float64)
         dU, dC, dO = DeviceBuffer.from_numpy(np.asarray(U, np.float64)), DeviceBuffer.from_numpy(centers), \
             DeviceBuffer.zeros(n_out * 3)
         check(lib().pg_asm_cohesive_openings(self._h, dU.ptr, dC.ptr, dO.ptr, n_out, None), "pg_asm_cohesive_openings")
         synchronize()
         rows = dO.download(n_out * 3).reshape(n_out, 3)
-        inc = np.zeros(n_out, np.int64)
-        for c in range(p.n_cut):
-            r = p.cut_record(c)
-            for k in range(r.n_coh):
-                inc[r.isv_off // 2 + k] = incl[c]
         return inc, rows
 
     def cohesive_failure(self, U) -> np.ndarray:

@@ -79,6 +79,7 @@ struct pgh_problem {
     std::vector<int32_t> rp[4], ci[4];
     std::vector<int32_t> npat_ptr, npat_col, npat_row, npat_map;
     std::vector<int32_t> n2e_ptr, n2e;
+    std::vector<int32_t> enr_incl;        // inclusion of each enrichment node (mirrors included)
     // batch of independent RVEs: per instance the first element, regular node,
     // enrichment node, free dof, prescribed dof, cohesive qp and cut element,
     // plus one closing row (PGH_BATCH_COLS columns)
@@ -316,9 +317,11 @@ void detect_nodes(pgh_problem& P) {
 }
 
 // --------------------------------------------------------------------------
-// Element detection tables (single inclusion per element).
-//   Tri3::detection, polygon-decomposition branch, 2 unique values with the
-//   matrix present: src/tri3.cpp:323-356.
+// Element detection tables.
+//   Tri3::detection, polygon-decomposition branch (src/tri3.cpp:283-452): one
+//   inclusion against the matrix (two children), two inclusions with or
+//   without matrix nodes, three inclusions (three triangles around a
+//   hexagon of matrix); children of 5 or 6 nodes are polygons.
 //   Tet4::detection, prism-decomposition branch: src/tet4.cpp:493-558, then
 //   Tet4::decomposePrisms / partitionPrism: src/tet4.cpp:673-870.
 // Output uses the cohesive (C-1) numbering; transformDetectionCohToReg folds
@@ -328,45 +331,83 @@ void detect_nodes(pgh_problem& P) {
 struct Detect {
     int ncut = 0;
     uint8_t cut_edge[PG_MAX_ENR];
+    int32_t slot_incl[PG_MAX_ENR];  // inclusion of each cut-edge slot
     std::vector<std::vector<int>> ints;
     std::vector<int> int_mat;  // detection value whose material the child takes
     std::vector<std::vector<int>> cohs;
-    int incl = -1;
 };
 
-void set_tri3(Detect& D, int ce0, int ce1, int sep, int other, int sep_node,
-              const int a[3], const int b[3], bool sep_lower) {
-    (void)sep_node;
-    D.ncut = 2;
-    D.cut_edge[0] = uint8_t(ce0);
-    D.cut_edge[1] = uint8_t(ce1);
+void set_slots(Detect& D, std::initializer_list<int> edges, std::initializer_list<int> incl) {
+    D.ncut = int(edges.size());
+    int s = 0;
+    for (int e : edges) D.cut_edge[s++] = uint8_t(e);
+    s = 0;
+    for (int i : incl) D.slot_incl[s++] = i;
+}
+
+void set_tri3(Detect& D, int ce0, int ce1, int sep, int other, const int a[3], bool sep_lower) {
+    set_slots(D, {ce0, ce1}, {std::max(sep, other), std::max(sep, other)});
     D.int_mat = {sep, other};
     if (sep_lower) {
         D.ints = {{a[0], 4, 3}, {a[1], a[2], 5, 6}};
         D.cohs = {{3, 4, 5, 6}};
     } else {
-        D.ints = {{b[0], 6, 5}, {b[1], b[2], 3, 4}};
+        D.ints = {{a[0], 6, 5}, {a[1], a[2], 3, 4}};
         D.cohs = {{4, 3, 6, 5}};
     }
 }
 
 bool detect_tri3(const int nd[3], Detect& D) {
-    int n0 = nd[0], n1 = nd[1], n2 = nd[2];
+    const int n0 = nd[0], n1 = nd[1], n2 = nd[2];
     if (n0 == n1 && n1 == n2) return false;
-    int vmin = std::min(n0, std::min(n1, n2)), vmax = std::max(n0, std::max(n1, n2));
-    int uniq = 1 + (n0 != n1 ? 1 : 0) + ((n2 != n0 && n2 != n1) ? 1 : 0);
-    if (uniq != 2 || vmin >= 0)
-        fail(PG_E_UNSUPPORTED, "tri3 element cut by more than one inclusion (polygon children) is not supported");
-    D.incl = vmax;
-    if (n1 == n2) {  // node 0 separate
-        const int a[3] = {0, 1, 2};
-        set_tri3(D, 2, 0, n0, n1, 0, a, a, n0 < n1);
-    } else if (n0 == n2) {  // node 1 separate
-        const int a[3] = {1, 2, 0};
-        set_tri3(D, 0, 1, n1, n0, 1, a, a, n1 < n0);
-    } else {  // node 2 separate
-        const int a[3] = {2, 0, 1};
-        set_tri3(D, 1, 2, n2, n0, 2, a, a, n2 < n0);
+    const int vmin = std::min(n0, std::min(n1, n2));
+    const int uniq = 1 + (n0 != n1 ? 1 : 0) + ((n2 != n0 && n2 != n1) ? 1 : 0);
+    if (uniq == 2 && vmin < 0) {  // one inclusion against the matrix, src/tri3.cpp:336-367
+        if (n1 == n2) {  // node 0 separate
+            const int a[3] = {0, 1, 2};
+            set_tri3(D, 2, 0, n0, n1, a, n0 < n1);
+        } else if (n0 == n2) {  // node 1 separate
+            const int a[3] = {1, 2, 0};
+            set_tri3(D, 0, 1, n1, n0, a, n1 < n0);
+        } else {  // node 2 separate
+            const int a[3] = {2, 0, 1};
+            set_tri3(D, 1, 2, n2, n0, a, n2 < n0);
+        }
+    } else if (uniq == 2) {  // two inclusions, no matrix node, src/tri3.cpp:370-398
+        if (n1 == n2) {
+            set_slots(D, {2, 0, 2, 0}, {n0, n0, n1, n1});
+            D.int_mat = {n0, -1, n1};
+            D.ints = {{0, 8, 7}, {4, 6, 5, 3}, {1, 2, 9, 10}};
+        } else if (n0 == n2) {
+            set_slots(D, {0, 1, 0, 1}, {n1, n1, n2, n2});
+            D.int_mat = {n1, -1, n2};
+            D.ints = {{1, 8, 7}, {4, 6, 5, 3}, {2, 0, 9, 10}};
+        } else {
+            set_slots(D, {1, 2, 1, 2}, {n2, n2, n0, n0});
+            D.int_mat = {n2, -1, n0};
+            D.ints = {{2, 8, 7}, {4, 6, 5, 3}, {0, 1, 9, 10}};
+        }
+        D.cohs = {{4, 3, 8, 7}, {5, 6, 9, 10}};
+    } else if (vmin < 0) {  // two inclusions and a matrix node: a pentagon, src/tri3.cpp:407-433
+        if (n0 < 0) {
+            set_slots(D, {1, 0, 2, 1}, {n1, n1, n2, n2});
+            D.int_mat = {n1, -1, n2};
+            D.ints = {{1, 7, 8}, {0, 4, 3, 6, 5}, {2, 9, 10}};
+        } else if (n1 < 0) {
+            set_slots(D, {2, 1, 0, 2}, {n2, n2, n0, n0});
+            D.int_mat = {n2, -1, n0};
+            D.ints = {{2, 7, 8}, {1, 4, 3, 6, 5}, {0, 9, 10}};
+        } else {
+            set_slots(D, {0, 2, 1, 0}, {n0, n0, n1, n1});
+            D.int_mat = {n0, -1, n1};
+            D.ints = {{0, 7, 8}, {2, 4, 3, 6, 5}, {1, 9, 10}};
+        }
+        D.cohs = {{3, 4, 7, 8}, {5, 6, 9, 10}};
+    } else {  // three inclusions: three triangles and a hexagon, src/tri3.cpp:437-444
+        set_slots(D, {0, 2, 2, 1, 1, 0}, {n0, n0, n2, n2, n1, n1});
+        D.int_mat = {n0, n2, n1, -1};
+        D.ints = {{0, 9, 10}, {2, 11, 12}, {1, 13, 14}, {8, 7, 6, 5, 4, 3}};
+        D.cohs = {{3, 4, 9, 10}, {5, 6, 11, 12}, {7, 8, 13, 14}};
     }
     return true;
 }
@@ -464,7 +505,7 @@ bool detect_tet4(const int nd[4], const int32_t gid[4], bool cohesive, Detect& D
                                    "side) are not supported; node-centred inclusions (PGH_PLACE_AT_NODES) cut "
                                    "tets one node against three only");
     }
-    D.incl = vec[1];
+    for (int k = 0; k < D.ncut; ++k) D.slot_incl[k] = vec[1];
     // decomposePrisms, src/tet4.cpp:673-735
     for (size_t o = 0; o < ints.size(); ++o) {
         if (ints[o].size() == 6) {
@@ -620,9 +661,9 @@ void analyze_and_enrich(pgh_problem& P) {
         CutInfo ci{};
         ci.elem = e;
         ci.ncut = D.ncut;
-        for (int s = 0; s < D.ncut; ++s) { ci.cut_edge[s] = D.cut_edge[s]; ci.slot_incl[s] = D.incl; }
+        for (int s = 0; s < D.ncut; ++s) { ci.cut_edge[s] = D.cut_edge[s]; ci.slot_incl[s] = D.slot_incl[s]; }
         ci.nchild = int(D.ints.size());
-        if (ci.nchild > PG_MAX_CHILD) fail(PG_E_UNSUPPORTED, "too many integration children");
+        if (ci.nchild > (P.dim == 2 ? 4 : PG_MAX_CHILD)) fail(PG_E_UNSUPPORTED, "too many integration children");
         for (int k = 0; k < ci.nchild; ++k) {
             ci.child_nn[k] = uint8_t(D.ints[k].size());
             for (size_t j = 0; j < D.ints[k].size(); ++j) {
@@ -669,6 +710,8 @@ void analyze_and_enrich(pgh_problem& P) {
                keys.end());
     const int32_t n_init = int32_t(keys.size());
     P.n_enr = coh ? 2 * n_init : n_init;
+    P.enr_incl.resize(size_t(P.n_enr));
+    for (int32_t k = 0; k < P.n_enr; ++k) P.enr_incl[k] = keys[k % n_init].inc;
     P.coords.resize(size_t(P.n_nodes + P.n_enr) * P.dim);
     for (int32_t k = 0; k < n_init; ++k) {
         const double* x0 = &P.coords[size_t(keys[k].lo) * P.dim];
@@ -704,10 +747,15 @@ void analyze_and_enrich(pgh_problem& P) {
         int nvqp = 0;
         for (int k = 0; k < ci.nchild; ++k) {
             int nn = ci.child_nn[k];
-            R.child_type[k] = uint8_t(nn == 3 ? PG_CHILD_TRI3 : (P.dim == 2 ? PG_CHILD_QUAD4 : PG_CHILD_TET4));
+            if (P.dim == 2) {
+                const int types[7] = {-1, -1, -1, PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_POLY5, PG_CHILD_POLY6};
+                R.child_type[k] = uint8_t(types[nn]);
+            } else {
+                R.child_type[k] = uint8_t(PG_CHILD_TET4);
+            }
             R.child_mat[k] = uint8_t(ci.child_mat[k]);
-            for (int j = 0; j < nn; ++j) R.child_nodes[k][j] = ci.child_nodes[k][j];
-            nvqp += (R.child_type[k] == PG_CHILD_TET4) ? 5 : 4;
+            for (int j = 0; j < nn; ++j) R.child_node[PG_CHILD_STRIDE(P.dim) * k + j] = ci.child_nodes[k][j];
+            nvqp += PG_CHILD_NQP(R.child_type[k]);
         }
         R.n_vqp = uint8_t(nvqp);
         R.n_coh = uint8_t(coh ? ci.ncoh : 0);
@@ -1160,6 +1208,7 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
         B->node_dof.resize(size_t(nall) * 2);
         B->bc_val.resize(size_t(T[4]));
         B->node_detect.resize(size_t(T[1]));
+        B->enr_incl.resize(size_t(T[2]));
         for (int i = 0; i < n; ++i) {
             const pgh_problem& P = *inst[i];
             const int64_t* o = &B->inst_off[size_t(i) * C];
@@ -1177,6 +1226,7 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
             }
             for (int32_t a = 0; a < P.n_nodes; ++a)
                 B->node_detect[size_t(r0) + a] = P.node_detect[a] < 0 ? -1 : P.node_detect[a];
+            for (int32_t k = 0; k < P.n_enr; ++k) B->enr_incl[size_t(o[2]) + k] = int32_t(o[7] + P.enr_incl[k]);
             for (int32_t k = 0; k < P.n_pres; ++k) B->bc_val[size_t(o[4]) + k] = P.bc_val[k];
             for (int32_t e = 0; e < P.n_elem; ++e) {
                 for (int k = 0; k < 3; ++k) B->conn[size_t(e0 + e) * 3 + k] = r0 + P.conn[size_t(e) * 3 + k];
@@ -1273,6 +1323,10 @@ int pgh_inclusions(const pgh_problem* h, const double** centers, const double** 
 const int32_t* pgh_node_detect(const pgh_problem* h) { return h ? h->node_detect.data() : nullptr; }
 
 const double* pgh_bc_values(const pgh_problem* h) { return h ? h->bc_val.data() : nullptr; }
+
+const int32_t* pgh_enrich_inclusion(const pgh_problem* h) {
+    return h && !h->enr_incl.empty() ? h->enr_incl.data() : nullptr;
+}
 
 int pgh_stage_seconds(const pgh_problem* h, double* out, int32_t n) {
     if (!h || !out || n < PGH_N_STAGES) { g_err = "bad argument"; return PG_E_ARG; }

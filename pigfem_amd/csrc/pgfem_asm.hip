@@ -116,13 +116,13 @@ __global__ void k_permute_u(int64_t n, const int32_t* __restrict__ node_dof, con
     if (k < n) Un[k] = U[node_dof[k]];
 }
 
-constexpr int MAXN2 = 3 + 8;             // 2D local nodes (tri3 + 2x4 enrichment slots)
+constexpr int MAXN2 = 3 + PG_MAX_ENR;    // 2D local nodes (tri3 + 6 cut-edge slots + mirrors)
 constexpr int CUT_NODES = 4 + PG_MAX_ENR;  // words per local node list of a cut element
 // internal launch flag: every node group takes the global-memory path (the
 // fallback of groups that overflow the LDS tables; group variant bit 3)
 constexpr unsigned PG_GLOBAL_PATH = 1u << 31;
 constexpr int MAXN3 = 4 + PG_MAX_ENR;    // 3D local nodes
-constexpr int MAXQ2 = 16;                // 2D volumetric qps per cut element
+constexpr int MAXQ2 = 36;                // 2D volumetric qps per cut element (3 tri3 + hexagon)
 constexpr int MAXQ3 = 32;                // 3D volumetric qps per cut element
 constexpr int CUT_MAXC = PG_MAX_COH * 4; // cohesive qps per cut element
 constexpr int MAX_LDS_MATS = 8;          // materials staged in LDS
@@ -141,10 +141,92 @@ template <int D> struct CutLim;
 template <> struct CutLim<2> { static constexpr int N = MAXN2, Q = MAXQ2; };
 template <> struct CutLim<3> { static constexpr int N = MAXN3, Q = MAXQ3; };
 
+// Regular reference polygon of Polygon::PolyTriangulate (src/Polygon.cpp:51-63):
+// vertex n at angle 2 pi (n + 1) / nn, as cos / sin of (2.0 * PI * (n + 1)) / nn
+// give them on the host (PI of include/common.h:215); pentagon, then hexagon.
+__constant__ const double POLY_VX[11] = {
+    0x1.3c6ef372fe950p-2, -0x1.9e3779b97f4a7p-1, -0x1.9e3779b97f4a9p-1, 0x1.3c6ef372fe94cp-2, 0x1.0000000000000p+0,
+    0x1.0000000000001p-1, -0x1.ffffffffffffcp-2, -0x1.0000000000000p+0, -0x1.0000000000004p-1, 0x1.0000000000001p-1,
+    0x1.0000000000000p+0};
+__constant__ const double POLY_VY[11] = {
+    0x1.e6f0e134454ffp-1, 0x1.2cf2304755a5fp-1, -0x1.2cf2304755a5dp-1, -0x1.e6f0e13445500p-1, -0x1.1a62633145c07p-52,
+    0x1.bb67ae8584caap-1, 0x1.bb67ae8584cabp-1, 0x1.1a62633145c07p-53, -0x1.bb67ae8584ca8p-1, -0x1.bb67ae8584caap-1,
+    -0x1.1a62633145c07p-52};
+
+// Polygon child qp q (Polygon::q_point + compute_shape_grad, src/Polygon.cpp:84-222):
+// point q % 4 of the tri3 order-3 rule on sub-triangle t = q / 4 (origin,
+// vertex t, vertex t + 1), weight = tri3 weight x sub-triangle Jacobian;
+// Wachspress gradients at that point with A_n the signed area of (x,
+// vertex n - 1, vertex n), alpha_n = 1 / (A_n A_{n+1}), N_n = alpha_n / sum.
+// As in the reference, dA is minus the derivative of A (src/Polygon.cpp:199-200),
+// so these are -grad N; in 2D the Jacobian built from them flips too, and J and
+// the physical gradients come out right.  NN is a template parameter so that
+// every array index is a constant.
+template <int NN>
+__device__ inline double poly_qp(int q, double (&dN)[NN][2]) {
+    constexpr int V0 = NN == 5 ? 0 : 5;
+    const int t = q / 4, qq = q % 4;
+    const double qx = pick(TRI_QX, qq), qy = pick(TRI_QY, qq);
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        if (n == t) { ax = POLY_VX[V0 + n]; ay = POLY_VY[V0 + n]; }
+        if (n == (t + 1 == NN ? 0 : t + 1)) { bx = POLY_VX[V0 + n]; by = POLY_VY[V0 + n]; }
+    }
+    // sub-triangle Jacobian (columns: vertex t, vertex t + 1) and the point
+    const double w = (ax * by - bx * ay) * pick(TRI_QW, qq);
+    const double n0 = 1.0 - qx - qy;
+    const double xi0 = ((0.0 + n0 * 0.0) + qx * ax) + qy * bx;
+    const double xi1 = ((0.0 + n0 * 0.0) + qx * ay) + qy * by;
+    double A[NN + 1], dA0[NN + 1], dA1[NN + 1];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        const int m = n + 1 == NN ? 0 : n + 1;
+        const double p1x = POLY_VX[V0 + n], p1y = POLY_VY[V0 + n], p2x = POLY_VX[V0 + m], p2y = POLY_VY[V0 + m];
+        A[n + 1] = 0.5 * (((p1x * p2y - p1y * p2x) - (xi0 * p2y - xi1 * p2x)) + (xi0 * p1y - xi1 * p1x));
+        dA0[n + 1] = 0.5 * (p2y - p1y);
+        dA1[n + 1] = 0.5 * (p1x - p2x);
+    }
+    A[0] = A[NN];
+    dA0[0] = dA0[NN];
+    dA1[0] = dA1[NN];
+    double alpha[NN], da0[NN], da1[NN], sa = 0.0, sd0 = 0.0, sd1 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        alpha[n] = 1.0 / (A[n] * A[n + 1]);
+        da0[n] = (-1.0 * alpha[n]) * (dA0[n] / A[n] + dA0[n + 1] / A[n + 1]);
+        da1[n] = (-1.0 * alpha[n]) * (dA1[n] / A[n] + dA1[n + 1] / A[n + 1]);
+        sa += alpha[n];
+        sd0 += da0[n];
+        sd1 += da1[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        const double N = alpha[n] / sa;
+        dN[n][0] = (da0[n] - N * sd0) / sa;
+        dN[n][1] = (da1[n] - N * sd1) / sa;
+    }
+    return w;
+}
+
+// A polygon child's qp: reference gradients, then the physical ones.
+template <int NN>
+__device__ inline double poly_child_qp(int q, const double (*cx)[3], double (*cg)[3], double& w) {
+    double dN[NN][2];
+    w = poly_qp<NN>(q, dN);
+    return transform<2>(NN, dN, cx, cg);
+}
+
 // Geometry of one volumetric qp of a child: child gradients, child J and weight
 // (Elem::q_point + Elem::ShapeFunctions on the child, src/Mesh.cpp:1535-1562).
-template <int D>
+// POLY: polygon children may occur (2D only; the kernels that never meet one
+// leave the polygon code out).
+template <int D, bool POLY = false>
 __device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3], double (*cg)[3], double& w) {
+    if constexpr (POLY && D == 2) {
+        if (ct == PG_CHILD_POLY5) return poly_child_qp<5>(q, cx, cg, w);
+        if (ct == PG_CHILD_POLY6) return poly_child_qp<6>(q, cx, cg, w);
+    }
     double cdN[4][D];
     if (ct == PG_CHILD_QUAD4) {
         const double r = (q % 2 == 0) ? -PG_GAUSS2 : PG_GAUSS2;
@@ -363,11 +445,15 @@ __global__ __launch_bounds__(256) void k_elem_plain(DevProb p, const double* __r
 // the law with the history update.  Phase 2: one lane per entry of K_el sums
 // its volumetric qps in qp order, then adds each cohesive element's K_c (its
 // qps summed first, as the reference folds K_c into K_el); likewise P_el.
-template <int D>
+//
+// POLY: the launch over the cut elements with polygon children (list: their
+// cut indices); the other launch (list null, one block per cut element) leaves
+// those elements to it, so that its kernel carries no polygon code.
+template <int D, bool POLY>
 __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __restrict__ Un,
                                                   const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                                   double* __restrict__ Ks, double* __restrict__ Ps,
-                                                  int* __restrict__ err) {
+                                                  int* __restrict__ err, const int32_t* __restrict__ list) {
     constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
     constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q, MAXC = PG_MAX_COH * 4;
     __shared__ double sD[MAX_LDS_MATS][NS * NS];
@@ -379,12 +465,18 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     __shared__ int mq[MAXQ];               // material of each volumetric qp (its child's)
     __shared__ double cN[MAXC][3], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
     __shared__ pg_cut_rec rec;
-    const int c = blockIdx.x, lane = threadIdx.x;
+    const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
     stage_dmats<NS>(p, sD);
     constexpr int RW = int(sizeof(pg_cut_rec) / 4);
     if (lane < RW) reinterpret_cast<int32_t*>(&rec)[lane] = reinterpret_cast<const int32_t*>(&p.cut[c])[lane];
     const int e = p.cut_elem[c];
     __syncthreads();
+    if constexpr (!POLY && D == 2) {
+        bool poly = false;
+        for (int k = 0; k < rec.n_child; ++k)
+            poly = poly || rec.child_type[k] == PG_CHILD_POLY5 || rec.child_type[k] == PG_CHILD_POLY6;
+        if (poly) return;  // the polygon launch takes it (the whole block returns)
+    }
     const int nloc = NEN + rec.n_enr, nd = nloc * D;
     if (lane < nloc) {
         const int node = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
@@ -402,18 +494,19 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     for (int t = lane; t < nvq; t += 64) {
         int k = 0, q = t;
         for (; k < rec.n_child; ++k) {
-            const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
+            const int nq = PG_CHILD_NQP(rec.child_type[k]);
             if (q < nq) break;
             q -= nq;
         }
-        const int ct = rec.child_type[k], ccn = ct == PG_CHILD_TRI3 ? 3 : 4;
-        double cx[4][3], cg[4][3], w;
-        int cl[4];
-        for (int n = 0; n < 4; ++n) {
-            cl[n] = n < ccn ? rec.child_nodes[k][n] : -1;
+        constexpr int CN = (D == 2 && POLY) ? 6 : 4;  // nodes of the largest child
+        const int ct = rec.child_type[k], ccn = PG_CHILD_NN(ct);
+        double cx[CN][3], cg[CN][3], w;
+        int cl[CN];
+        for (int n = 0; n < CN; ++n) {
+            cl[n] = n < ccn ? rec.child_node[PG_CHILD_STRIDE(D) * k + n] : -1;
             for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? lx[cl[n]][i] : 0.0;
         }
-        const double Jc = child_qp<D>(ct, q, ccn, cx, cg, w);
+        const double Jc = child_qp<D, POLY>(ct, q, ccn, cx, cg, w);
         if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
         double eps[NS];
         for (int s2 = 0; s2 < NS; ++s2) eps[s2] = 0.0;
@@ -422,7 +515,7 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
             if (l < NEN) {
                 for (int a = 0; a < D; ++a) gr[a] = pgr[l][a];
             } else {
-                for (int n = 0; n < 4; ++n)
+                for (int n = 0; n < CN; ++n)
                     if (cl[n] == l)
                         for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
             }
@@ -553,8 +646,12 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double qg[EPW][MAXQ][MAXN - NEN][D];
     __shared__ double qw[EPW][MAXQ];
     __shared__ TK qwl[EPW][MAXQ][2];        // w lambda, w mu of each qp (its child's material)
-    __shared__ uint32_t qmask[EPW][MAXN];   // qps at which a local node's gradient is not zero
-    static_assert(MAXQ <= 32, "one mask bit per qp");
+    // qps at which a local node's gradient is not zero (one bit per qp)
+    using QMask = typename std::conditional<(MAXQ > 32), unsigned long long, uint32_t>::type;
+    static_assert(MAXQ <= 64, "one mask bit per qp");
+    __shared__ QMask qmask[EPW][MAXN];
+    // nodes of the largest child
+    constexpr int CN = (D == 2 && MAXQ > 8) ? 6 : 4;
     __shared__ double qs[EPW][MAXQ][NS];
     __shared__ double cN[EPW][CUT_MAXC][3];
     __shared__ double cw[EPW][CUT_MAXC];
@@ -610,7 +707,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         if (live && kk < RW) reinterpret_cast<int32_t*>(&recs[g])[kk] = rw[k];
     }
     if (gl < MAXN) {
-        qmask[g][gl] = 0u;
+        qmask[g][gl] = QMask(0);
         if (live && nodeid >= 0)
             for (int i = 0; i < D; ++i) {
                 xs[g][gl][i] = nx[i];
@@ -635,19 +732,19 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             // ---- volumetric qp ----
             int k = 0, q = t;
             for (; k < rec.n_child; ++k) {
-                const int nq = rec.child_type[k] == PG_CHILD_TET4 ? 5 : 4;
+                const int nq = PG_CHILD_NQP(rec.child_type[k]);
                 if (q < nq) break;
                 q -= nq;
             }
             const int ct = rec.child_type[k];
-            const int ccn = ct == PG_CHILD_TRI3 ? 3 : 4;
-            double cx[4][3], cg[4][3], w;
-            int cl[4];
-            for (int n = 0; n < 4; ++n) {
-                cl[n] = n < ccn ? rec.child_nodes[k][n] : -1;
+            const int ccn = PG_CHILD_NN(ct);
+            double cx[CN][3], cg[CN][3], w;
+            int cl[CN];
+            for (int n = 0; n < CN; ++n) {
+                cl[n] = n < ccn ? rec.child_node[PG_CHILD_STRIDE(D) * k + n] : -1;
                 for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? xs[g][cl[n]][i] : 0.0;
             }
-            const double Jc = child_qp<D>(ct, q, ccn, cx, cg, w);
+            const double Jc = child_qp<D, CN == 6>(ct, q, ccn, cx, cg, w);
             if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
             double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
             for (int n = 0; n < NEN; ++n) {
@@ -664,7 +761,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 } else {
                     for (int a = 0; a < D; ++a) gr[a] = 0.0;
 #pragma unroll
-                    for (int n = 0; n < 4; ++n)
+                    for (int n = 0; n < CN; ++n)
                         if (cl[n] == l)
                             for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
                 }
@@ -674,7 +771,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 }
                 bool nz = false;
                 for (int a = 0; a < D; ++a) nz = nz || gr[a] != 0.0;
-                if (nz) atomicOr(&qmask[g][l], 1u << t);
+                if (nz) atomicOr(&qmask[g][l], QMask(1) << t);
                 for (int c = 0; c < D; ++c) {
                     double b[NS];
                     bcol<D, double>(gr, c, b);
@@ -743,8 +840,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             for (int j = 0; j < D; ++j) vol[i][j] = TK(0);
         // the qps at which both gradients are not zero, ascending (the others
         // contribute nothing)
-        for (uint32_t qm = qmask[g][li] & qmask[g][lj]; qm; qm &= qm - 1u) {
-            const int q = __builtin_ctz(qm);
+        for (QMask qm = qmask[g][li] & qmask[g][lj]; qm; qm &= qm - QMask(1)) {
+            const int q = sizeof(QMask) == 8 ? __builtin_ctzll(qm) : __builtin_ctz(uint32_t(qm));
             double gid[D], gjd[D];
             grad(q, li, gid);
             grad(q, lj, gjd);
@@ -787,8 +884,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     for (int i = gl; i < nd; i += GS) {
         const int l = i / D, c = i % D;
         double a = 0.0;
-        for (uint32_t qm = qmask[g][l]; qm; qm &= qm - 1u) {
-            const int q = __builtin_ctz(qm);
+        for (QMask qm = qmask[g][l]; qm; qm &= qm - QMask(1)) {
+            const int q = sizeof(QMask) == 8 ? __builtin_ctzll(qm) : __builtin_ctz(uint32_t(qm));
             double gr[D], v[D];
             int r[D];
             grad(q, l, gr);
@@ -878,7 +975,8 @@ __global__ void k_coh_openings(DevProb p, const double* __restrict__ Un, const d
             del[i] = s;
         }
         double* o = out + (int64_t(rec.isv_off) / 2 + k) * 3;
-        o[0] = atan2(xm[1] - centers[int64_t(c) * 2 + 1], xm[0] - centers[int64_t(c) * 2]);
+        const int64_t row = int64_t(rec.isv_off) / 2 + k;
+        o[0] = atan2(xm[1] - centers[row * 2 + 1], xm[0] - centers[row * 2]);
         o[1] = del[0] / m.delta_c;
         o[2] = del[1] / m.delta_c;
     }
@@ -2134,6 +2232,8 @@ struct pg_asm {
     int32_t* stpl = nullptr;        // GridTpl words
     int4* srec = nullptr;           // per structured node: node, first K value, first dof, cell element
     int32_t* sinf = nullptr;        // per structured node: row stride | parity << 24
+    int32_t n_poly = 0;             // cut elements with polygon children (2D)
+    int32_t* poly_list = nullptr;   // their cut indices (the FP64 drop-in's polygon launch)
     bool timing = false;
     double last_ms[5] = {0, 0, 0, 0, 0};
     int pair_block = 256;
@@ -2302,12 +2402,16 @@ int check_desc(const pg_problem_desc* s) {
             g_err = "cut record exceeds the supported class sizes";
             return PG_E_UNSUPPORTED;
         }
+        if (s->dim == 2 && r.n_child > 4) { g_err = "more than 4 children in a 2D cut record"; return PG_E_UNSUPPORTED; }
         int nq = 0;
         for (int k = 0; k < r.n_child; ++k) {
             int t = r.child_type[k];
-            if (s->dim == 2 && t != PG_CHILD_TRI3 && t != PG_CHILD_QUAD4) { g_err = "bad 2D child"; return PG_E_UNSUPPORTED; }
+            if (s->dim == 2 && t != PG_CHILD_TRI3 && t != PG_CHILD_QUAD4 && t != PG_CHILD_POLY5 && t != PG_CHILD_POLY6) {
+                g_err = "bad 2D child";
+                return PG_E_UNSUPPORTED;
+            }
             if (s->dim == 3 && t != PG_CHILD_TET4) { g_err = "bad 3D child"; return PG_E_UNSUPPORTED; }
-            nq += t == PG_CHILD_TET4 ? 5 : 4;
+            nq += PG_CHILD_NQP(t);
         }
         if (nq != r.n_vqp) { g_err = "cut record qp count mismatch"; return PG_E_ARG; }
         if (r.n_coh && r.coh_nn != (s->dim == 2 ? 4 : 6)) { g_err = "bad cohesive child"; return PG_E_UNSUPPORTED; }
@@ -2317,8 +2421,11 @@ int check_desc(const pg_problem_desc* s) {
         }
         int nloc = s->nen + r.n_enr;
         for (int k = 0; k < r.n_child; ++k)
-            for (int n = 0; n < (r.child_type[k] == PG_CHILD_TRI3 ? 3 : 4); ++n)
-                if (r.child_nodes[k][n] >= nloc) { g_err = "child node out of range"; return PG_E_ARG; }
+            for (int n = 0; n < PG_CHILD_NN(r.child_type[k]); ++n)
+                if (r.child_node[PG_CHILD_STRIDE(s->dim) * k + n] >= nloc) {
+                    g_err = "child node out of range";
+                    return PG_E_ARG;
+                }
         for (int k = 0; k < r.n_coh; ++k)
             for (int n = 0; n < r.coh_nn; ++n)
                 if (r.coh_nodes[k][n] >= nloc) { g_err = "cohesive node out of range"; return PG_E_ARG; }
@@ -2423,10 +2530,11 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const unsigned gb = unsigned((d.n_elem + 255) / 256);
         if (d.dim == 2) {
             k_elem_plain<2><<<gb, 256, 0, st>>>(d, h->Un, h->stage);
-            if (d.n_cut) k_cut_plain<2><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err);
+            if (d.n_cut) k_cut_plain<2, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err, nullptr);
+            if (h->n_poly) k_cut_plain<2, true><<<unsigned(h->n_poly), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err, h->poly_list);
         } else {
             k_elem_plain<3><<<gb, 256, 0, st>>>(d, h->Un, h->stage);
-            if (d.n_cut) k_cut_plain<3><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err);
+            if (d.n_cut) k_cut_plain<3, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, dIc, dIt, h->stage_k, h->stage_p, h->err, nullptr);
         }
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
@@ -2541,8 +2649,8 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
                 }
             }
             if (n1) {
-                if (fp32 && !cut64) k_cut<2, float, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<2, double, 4, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 3) / 4, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32 && !cut64) k_cut<2, float, 2, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 1) / 2, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
+                else k_cut<2, double, 2, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 1) / 2, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         } else {
             if (n0) {
@@ -2705,6 +2813,23 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     UP(n2e, s->n2e, s->n2e_ptr[nall], int32_t);
     UP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
     UP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
+    {
+        std::vector<int32_t> pl;
+        for (int32_t c = 0; c < s->n_cut; ++c)
+            for (int k = 0; k < s->cut[c].n_child; ++k)
+                if (s->cut[c].child_type[k] == PG_CHILD_POLY5 || s->cut[c].child_type[k] == PG_CHILD_POLY6) {
+                    pl.push_back(c);
+                    break;
+                }
+        h->n_poly = int32_t(pl.size());
+        if (!pl.empty()) {
+            if ((rc = alloc(h, &h->poly_list, pl.size()))) return fail(rc);
+            if (hipMemcpy(h->poly_list, pl.data(), pl.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the polygon list failed";
+                return fail(PG_E_HIP);
+            }
+        }
+    }
     {
         // Node lists of the node-group kernels, each in mesh order: (0) the
         // nodes of the structured grid whose incidences are all uncut (tri3;

@@ -92,6 +92,14 @@ def small_config(kind: str = "c0", n: int = 24, n_incl: int = 3, seed: int = 11)
                          cohesive=_ffi.PGH_COH_OP, name=f"small-op-{n}")
     if kind == "tet":
         return RVEConfig(dim=3, n=n, n_incl=n_incl, r_min=0.12, r_max=0.2, seed=seed, name=f"small-tet-{n}")
+    if kind in ("multi", "multi-op", "multi-bl"):
+        # 120 small circles packed with a gap of 0.05 h: tri3 elements cut by
+        # two inclusions (a matrix pentagon, or a matrix band between two
+        # children) and by three (three triangles around a matrix hexagon)
+        coh = {"multi": _ffi.PGH_COH_NONE, "multi-op": _ffi.PGH_COH_OP, "multi-bl": _ffi.PGH_COH_BL}[kind]
+        return RVEConfig(dim=2, n=n, n_incl=120 if n_incl == 3 else n_incl, r_min=0.02, r_max=0.035, gap_h=0.05,
+                         seed=1 if seed == 11 else seed, cohesive=coh, place=_ffi.PGH_PLACE_BEST_EFFORT,
+                         name=f"small-{kind}-{n}")
     if kind in ("tet-op", "tet-bl"):
         # spheres centred at mesh nodes, r < h / 2: CohTri3 interfaces only
         h = 1.0 / n
@@ -177,6 +185,8 @@ class RVEProblem:
         self.n2e_ptr = _arr(d.n2e_ptr, nall + 1, np.int32)
         self.n2e = _arr(d.n2e, int(self.n2e_ptr[-1]), np.int32)
         self.node_detect = _arr(L.pgh_node_detect(h), self.n_nodes, np.int32)
+        ei = L.pgh_enrich_inclusion(h)
+        self.enr_incl = _arr(ei, self.n_enr, np.int32) if ei else None
         self.bc_values = _arr(L.pgh_bc_values(h), self.n_pres, np.float64)
         cptr, rptr, ni = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.c_int32()
         check(L.pgh_inclusions(h, C.byref(cptr), C.byref(rptr), C.byref(ni)), "pgh_inclusions")
@@ -218,6 +228,21 @@ class RVEProblem:
 
     def cut_record(self, i: int):
         return self.desc.cut[i]
+
+    def edge_inclusions(self) -> np.ndarray:
+        """Inclusion of each 2D cohesive edge, at row isv_off / 2 + k for edge k
+        of a cut record: the inclusion of the enrichment node that is the edge's
+        first lower node (Elem::get_inclusion_from_enrich, as
+        src/WriterCohesiveOpenings.cpp:128 takes it)."""
+        out = np.zeros(self.n_cqp // 2, np.int64)
+        if self.enr_incl is None:
+            raise ValueError("the inclusions of a mesh set up elsewhere are not known")
+        for c in range(self.n_cut):
+            r = self.desc.cut[c]
+            for k in range(r.n_coh):
+                g = int(r.enr[r.coh_nodes[k][0] - self.nen])
+                out[r.isv_off // 2 + k] = int(self.enr_incl[g - self.n_nodes])
+        return out
 
     def random_u(self, seed: int = 0, scale: float = 1e-3) -> np.ndarray:
         """U ~ U[-scale, scale] on every dof (free and prescribed, regular and enrichment)."""

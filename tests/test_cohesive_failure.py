@@ -14,7 +14,6 @@ import numpy as np
 import pytest
 
 from pigfem_amd import AssemblerGPU, PgError, RVEProblem, _ffi, small_config, write_cohesive_failure_lengths
-from pigfem_amd.assembler import cut_inclusions
 from oracle.oracle import cohesive_failure
 
 
@@ -75,10 +74,10 @@ def test_zero_opening_is_clamped():
 def test_lengths_are_chords_of_their_inclusions():
     p = _problem("op")
     rows = cohesive_failure(p, np.zeros(p.n_dof))
-    incl = cut_inclusions(p)
+    incl = p.edge_inclusions()
     tot = np.zeros(len(p.incl_radii))
     for c, row, _, _ in _edges(p):
-        tot[incl[c]] += rows[row, 0]
+        tot[incl[row]] += rows[row, 0]
     for i, r in enumerate(p.incl_radii):
         if tot[i] > 0:
             assert 0.9 * 2 * np.pi * r < tot[i] <= 2 * np.pi * r * (1 + 1e-6)

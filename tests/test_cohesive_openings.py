@@ -3,7 +3,7 @@
 CPU: the restatement in oracle/oracle.py is pinned by invariants that a wrong
 frame, index or sign breaks (a rigid translation opens nothing; a uniform jump v
 of the upper nodes gives openings of norm |v| / delta_c in every edge frame, with
-the normal part v . n; the edge midpoints lie on their inclusion's circle), and
+the normal part v . n; each edge is a chord of its own inclusion's circle), and
 the writer's sort and format are checked on a hand-made input.  GPU: the device
 kernel against the restatement."""
 
@@ -14,9 +14,11 @@ from pigfem_amd import AssemblerGPU, PgError, RVEProblem, small_config, write_co
 from oracle.oracle import cohesive_openings
 
 
-@pytest.fixture(scope="module")
-def op_problem():
-    return RVEProblem(small_config("op", n=24, n_incl=3))
+@pytest.fixture(scope="module", params=["op", "multi-op"])
+def op_problem(request):
+    # multi-op: elements cut by two or three inclusions, whose cohesive edges
+    # belong to different inclusions
+    return RVEProblem(small_config(request.param, n=24, n_incl=3))
 
 
 def _upper_nodes(p):
@@ -72,8 +74,11 @@ def test_midpoints_lie_on_their_inclusion(op_problem):
             xm = 0.5 * (p.coords[gl[r.coh_nodes[k][0]]] + p.coords[gl[r.coh_nodes[k][1]]])
             d = xm - p.incl_centers[inc[row]]
             rad = p.incl_radii[inc[row]]
-            # a chord of the circle: its midpoint lies within the radius, close to it
-            assert 0.9 * rad < np.linalg.norm(d) <= rad * (1 + 1e-6)
+            # a chord of the circle: both ends on it, the midpoint within it
+            for n in range(2):
+                x = p.coords[gl[r.coh_nodes[k][n]]]
+                assert abs(np.linalg.norm(x - p.incl_centers[inc[row]]) - rad) <= 1e-6 * rad
+            assert np.linalg.norm(d) <= rad * (1 + 1e-6)
             assert abs(np.arctan2(d[1], d[0]) - rows[row, 0]) < 1e-15
 
 

@@ -70,9 +70,9 @@ def test_inverted_child_is_reported_by_both_apis(mode):
     from pigfem_amd import DeviceBuffer, PgError
     p = RVEProblem(small_config("c0", n=12, n_incl=1))
     rec = p.desc.cut[0]
-    c = rec.child_nodes[0]
+    c = rec.child_node  # child 0's nodes lead the list
     c[1], c[2] = c[2], c[1]
-    assert p.desc.cut[0].child_nodes[0][1] == c[1]
+    assert p.desc.cut[0].child_node[1] == c[1]
     U = p.random_u(1)
     asm = AssemblerGPU(p, mode)
     K, P = PMatrix(p), PVector(p)

@@ -113,7 +113,7 @@ def test_hand_enumerated_cut_tables():
             else:
                 want = {0: [[0, 4, 3], [1, 2, 3, 4]], 1: [[1, 4, 3], [2, 0, 3, 4]], 2: [[2, 4, 3], [0, 1, 3, 4]]}[sep]
                 assert r.n_coh == 0
-            got = [list(r.child_nodes[k][: (3 if r.child_type[k] == _ffi.PG_CHILD_TRI3 else 4)]) for k in range(r.n_child)]
+            got = [_ffi.child_nodes(r, 2, k) for k in range(r.n_child)]
             assert got == want
             assert list(r.child_mat[:2]) == [1, 0]      # separate (fiber) child first
             assert r.n_vqp == 8
@@ -237,3 +237,52 @@ def test_pattern_flags_and_stage_times():
     m.node_dof = dup.ctypes.data_as(C.POINTER(C.c_int32))
     assert L.pgh_build_pattern(C.byref(m), C.byref(h)) == _ffi.PG_E_ARG
     assert b"twice" in L.pgh_last_error()
+
+
+@pytest.mark.parametrize("kind", ["multi", "multi-op"])
+def test_multi_inclusion_cut_topology(kind):
+    """Tri3 elements cut by two or three inclusions (src/tri3.cpp:370-444): the
+    children are counter-clockwise, tile the parent exactly (areas sum to it),
+    polygons of 5 and 6 nodes appear, an inclusion child's parent nodes detect
+    its inclusion and the matrix child's none, every enrichment node lies on the
+    circle of its own inclusion, and each cohesive edge joins two enrichment
+    nodes of one inclusion to their mirrors."""
+    from pigfem_amd import _ffi
+
+    def cross(a, b):
+        return a[0] * b[1] - a[1] * b[0]
+    p = RVEProblem(small_config(kind))
+    seen = set()
+    multi = 0
+    for c in range(p.n_cut):
+        r = p.cut_record(c)
+        e = int(p.cut_elem[c])
+        local = [int(x) for x in p.conn[e]] + [int(r.enr[k]) for k in range(r.n_enr)]
+        X = p.coords[local]
+        parea = 0.5 * cross(X[1] - X[0], X[2] - X[0])
+        tot = 0.0
+        for k in range(r.n_child):
+            cn = _ffi.child_nodes(r, 2, k)
+            seen.add(len(cn))
+            P = X[cn]
+            a = 0.5 * sum(cross(P[i], P[(i + 1) % len(P)]) for i in range(len(P)))
+            assert a > 0
+            tot += a
+            det = {int(p.node_detect[local[l]]) for l in cn if l < 3}
+            if r.child_mat[k] == 0:
+                assert det <= {-1}
+            else:
+                assert len(det) == 1 and det.pop() >= 0
+        assert abs(tot - parea) <= 1e-12 * parea
+        for k in range(r.n_enr):
+            g = local[3 + k]
+            inc = int(p.enr_incl[g - p.n_nodes])
+            assert abs(np.linalg.norm(p.coords[g] - p.incl_centers[inc]) - p.incl_radii[inc]) <= 1e-6 * p.incl_radii[inc]
+        if r.n_child > 2:
+            multi += 1
+        for k in range(r.n_coh):
+            lo = [local[r.coh_nodes[k][n]] for n in range(2)]
+            up = [local[r.coh_nodes[k][n + 2]] for n in range(2)]
+            assert len({int(p.enr_incl[g - p.n_nodes]) for g in lo + up}) == 1
+            assert all(np.array_equal(p.coords[a], p.coords[b]) for a, b in zip(lo, up))
+    assert multi > 50 and {3, 4, 5, 6} <= seen

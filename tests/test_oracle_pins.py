@@ -368,44 +368,64 @@ def _polygon_grad_integrals(X):
     return area, I
 
 
-@pytest.mark.parametrize("kind", ["c0", "tet"])
+@pytest.mark.parametrize("kind", ["c0", "multi", "tet"])
 def test_enrichment_patch_test_on_cut_elements(kind):
     """IGFEM patch test on each cut element alone (pins the enrichment shape
     gradients of src/Mesh.cpp:1596-1614: scale factor 1, child node -> local
-    enrichment node).  A displacement field v that is continuous and linear on
-    every child (2D: kinked along the element's interface chord; 3D: arbitrary
-    values at the parent and enrichment nodes of the child tets) is reproduced
-    exactly by regular dofs v(x_i) and enrichment dofs v(x_j) - (parent
-    interpolation of v at x_j).  Then U^T K_e U = sum_children |c| eps_c^T D_c eps_c
-    and P_e[l] = sum_children B(int_c grad phi_l)^T sigma_c with the child-wise
-    constant strain, in closed form (3D: 6 |c|, the tet weights sum to 1)."""
-    cfg = small_config(kind, n=12 if kind == "c0" else 6, n_incl=2 if kind == "c0" else 1)
+    enrichment node; and, for elements cut by two or three inclusions, the
+    Wachspress shapes and quadrature of the polygon children,
+    src/Polygon.cpp:51-222).  A displacement field v that is continuous and
+    linear on every child (2D: kinked along the chord of each inclusion child;
+    3D: arbitrary values at the parent and enrichment nodes of the child tets)
+    is reproduced exactly by regular dofs v(x_i) and enrichment dofs v(x_j) -
+    (parent interpolation of v at x_j) when the child shapes reproduce linear
+    fields.  Then U^T K_e U = sum_children |c| eps_c^T D_c eps_c and P_e[l] =
+    sum_children B(int_c grad phi_l)^T sigma_c with the child-wise constant
+    strain, in closed form (3D: 6 |c|, the tet weights sum to 1; the integral
+    of a shape gradient over a polygon child is its boundary integral, since
+    the shapes are linear along the edges)."""
+    cfg = small_config(kind, n=12 if kind == "c0" else (24 if kind == "multi" else 6),
+                       n_incl=2 if kind == "c0" else (3 if kind == "multi" else 1))
     p = RVEProblem(cfg)
     dim = p.dim
     rng = np.random.default_rng(17)
     scale = 1.0 if dim == 2 else 6.0
     checked = 0
-    for ci in range(0, p.n_cut, max(1, p.n_cut // 12)):
+    sample = list(range(0, p.n_cut, max(1, p.n_cut // 12)))
+    if kind == "multi":  # every element cut by more than one inclusion
+        sample += [c for c in range(p.n_cut) if p.cut[c].n_child > 2]
+        assert any(p.cut[c].n_child == 4 for c in range(p.n_cut))
+    for ci in sample:
         e = int(p.cut_elem[ci])
         rec = p.cut[ci]
         local = [int(x) for x in p.conn[e]] + [int(rec.enr[k]) for k in range(rec.n_enr)]
         X = p.coords[local]
         nl = len(local)
-        Np, _ = _simplex_grads(X[:dim + 1])  # parent gradients (rows)
+        Np, parea = _simplex_grads(X[:dim + 1])  # parent gradients (rows), parent measure
         if dim == 2:
-            # kinked field: v = A x + c max(0, n . (x - x_e0)), chord through the
-            # two enrichment nodes
-            assert rec.n_enr == 2
+            # kinked field: v = A x + sum_k c_k max(0, n_k . (x - x_k)) over the
+            # inclusion children k, n_k normal to the chord through the child's
+            # two enrichment nodes, pointing into the child (the half-plane
+            # meets the parent exactly in that child)
             A = rng.uniform(-1e-3, 1e-3, (2, 2))
-            cvec = rng.uniform(-1e-3, 1e-3, 2)
-            t = X[dim + 1] - X[dim + 2]
-            nrm = np.array([-t[1], t[0]]) / np.linalg.norm(t)
+            kinks = []
+            for k in range(rec.n_child):
+                if rec.child_mat[k] != 1:
+                    continue
+                cn = _ffi.child_nodes(rec, dim, k)
+                en = [l for l in cn if l > dim]
+                assert len(en) == 2
+                t = X[en[0]] - X[en[1]]
+                nrm = np.array([-t[1], t[0]]) / np.linalg.norm(t)
+                if nrm @ (X[cn].mean(axis=0) - X[en[0]]) < 0:
+                    nrm = -nrm
+                kinks.append((X[en[0]], nrm, rng.uniform(-1e-3, 1e-3, 2)))
 
             def field(x):
-                return A @ x + cvec * max(0.0, float(nrm @ (x - X[dim + 1])))
+                return A @ x + sum(c * max(0.0, float(n @ (x - x0))) for x0, n, c in kinks)
 
             def grad_on(xc):
-                return A + (np.outer(cvec, nrm) if nrm @ (xc - X[dim + 1]) > 0 else 0.0)
+                return A + sum((np.outer(c, n) for x0, n, c in kinks if n @ (xc - x0) > 0), np.zeros((2, 2)))
             V = np.array([field(x) for x in X])
         else:
             V = rng.uniform(-1e-3, 1e-3, (nl, dim))
@@ -420,9 +440,15 @@ def test_enrichment_patch_test_on_cut_elements(kind):
             U[p.node_dof[local[l]]] = Uloc[l]
         energy = 0.0
         Pexp = np.zeros((nl, dim))
+        o = Oracle(p, e, e + 1)
+        cache = elem_cache(o, e)
+        nst = 3 if dim == 2 else 6
+        qoff = 0
+        poly = False
         for k in range(rec.n_child):
-            cn = [int(rec.child_nodes[k][n]) for n in range(3 if rec.child_type[k] == _ffi.PG_CHILD_TRI3 else 4)]
-            Dm = np.array(p.desc.mats[rec.child_mat[k]].D[:(3 if dim == 2 else 6) ** 2]).reshape((3, 3) if dim == 2 else (6, 6))
+            ct = int(rec.child_type[k])
+            cn = _ffi.child_nodes(rec, dim, k)
+            Dm = np.array(p.desc.mats[rec.child_mat[k]].D[:nst ** 2]).reshape(nst, nst)
             Xc = X[cn]
             if dim == 2:
                 area, I = _polygon_grad_integrals(Xc)
@@ -432,22 +458,133 @@ def test_enrichment_patch_test_on_cut_elements(kind):
                 Gb, area = _simplex_grads(Xc)
                 I = area * Gb
                 G = V[cn].T @ Gb  # du_i/dx_j of the linear interpolant on the child
+            # the child's quadrature: sum of J w over its qps (oracle cache:
+            # per qp W, J, B).  Exact for tri3, quad4 and tet4 children; a
+            # polygon's Jacobian is rational, so its rule is exact only for an
+            # affine image of the regular polygon
+            nq = _ffi.PG_CHILD_NQP[ct]
+            jw = sum(cache[(qoff + q) * (2 + nst * nl * dim)] * cache[(qoff + q) * (2 + nst * nl * dim) + 1]
+                     for q in range(nq))
+            qoff += nq
+            if ct in (_ffi.PG_CHILD_POLY5, _ffi.PG_CHILD_POLY6):
+                poly = True
+                assert abs(jw - area) <= 2e-3 * area
+            else:
+                assert abs(jw / scale - area) <= 1e-12 * parea  # (slivers: rounding of the parent's scale)
             eps = _voigt(G)
             sig = Dm @ eps
-            energy += scale * area * eps @ Dm @ eps
+            # the strain is the field's at every qp (linear precision of the
+            # child shapes), so the energy is exact given the child's J w sum
+            energy += jw * eps @ Dm @ eps
             for m, l in enumerate(cn):
                 if l > dim:
                     Pexp[l] += scale * _bt(I[m], sig)  # enrichment node of this child
             for i in range(dim + 1):
-                Pexp[i] += scale * area * _bt(Np[i], sig)  # parent node over the child
-        o = Oracle(p, e, e + 1)
+                Pexp[i] += jw * _bt(Np[i], sig)  # parent node over the child
         K, P, _ = o.assemble(U)
         Kf = full_matrix(p, K, o.rp, o.ci)
         assert abs(U @ (Kf @ U) - energy) <= 1e-12 * energy
         Pl = np.array([P[p.node_dof[local[l]]] for l in range(nl)])
-        assert np.abs(Pl - Pexp).max() <= 1e-12 * np.abs(Pexp).max()
+        # the enrichment rows integrate a child gradient: exact but for the
+        # polygons' quadrature error
+        assert np.abs(Pl - Pexp).max() <= (2e-3 if poly else 1e-12) * np.abs(Pexp).max()
         checked += 1
     assert checked >= 10
+
+
+def _regular_polygon(nn):
+    """Vertices of the reference polygon: vertex n at angle 2 pi (n + 1) / nn."""
+    a = 2.0 * np.pi * (np.arange(nn) + 1) / nn
+    return np.stack([np.cos(a), np.sin(a)], axis=1)
+
+
+def _fan_moments(P):
+    """Exact area, first and second moments (int x, int y, int xx, int xy,
+    int yy) of the polygon P, as sums over the triangles (0, P_n, P_n+1)."""
+    m = np.zeros(6)
+    for n in range(len(P)):
+        a, b = P[n], P[(n + 1) % len(P)]
+        A = 0.5 * (a[0] * b[1] - a[1] * b[0])
+        m[0] += A
+        m[1] += A * (a[0] + b[0]) / 3.0
+        m[2] += A * (a[1] + b[1]) / 3.0
+        m[3] += A * (a[0] ** 2 + b[0] ** 2 + a[0] * b[0]) / 6.0
+        m[4] += A * (2 * a[0] * a[1] + 2 * b[0] * b[1] + a[0] * b[1] + b[0] * a[1]) / 12.0
+        m[5] += A * (a[1] ** 2 + b[1] ** 2 + a[1] * b[1]) / 6.0
+    return m
+
+
+@pytest.mark.parametrize("nn", [5, 6])
+def test_polygon_shapes_and_quadrature(nn):
+    """Polygon children (Polygon::PolyTriangulate / q_point / compute_shape /
+    compute_shape_grad, src/Polygon.cpp:51-222), pinned by what the mathematics
+    fixes.  Reference polygon (vertex n at angle 2 pi (n + 1) / nn): the shapes
+    are a partition of unity with linear precision (sum N_n p_n = xi), equal
+    1 / nn at the centre, tend to the linear hat along an edge (the other
+    vertices' shapes vanish there), and the gradients are the derivatives of
+    the values -- with the sign flipped: compute_shape_grad's dA is minus the
+    derivative of the sub-triangle areas (src/Polygon.cpp:199-200), so it
+    returns -grad N.  In 2D that cancels in Elem::ShapeFunctions (the Jacobian
+    flips too, det(-M) = det M, and (-M)^-1 (-g) = M^-1 g), so J and the
+    physical gradients are right, which the physical checks below pin.  Quadrature (the tri3 order-3 rule on each of the nn
+    sub-triangles, nn x 4 points): area, first and second moments of the
+    regular polygon exactly.  Physical polygon: at every qp the gradients
+    reproduce a linear field (sum_n X_n (x) grad N_n = I); on an affine image
+    of the regular polygon sum J w is its area exactly, on a distorted convex
+    one within the rule's error."""
+    from oracle.oracle import child_qp, polygon_shape
+    ct = _ffi.PG_CHILD_POLY5 if nn == 5 else _ffi.PG_CHILD_POLY6
+    Pv = _regular_polygon(nn)
+    rng = np.random.default_rng(nn)
+    for _ in range(20):
+        lam = rng.dirichlet(np.ones(nn))
+        xi = lam @ Pv
+        N, dN = polygon_shape(nn, xi)
+        assert abs(N.sum() - 1.0) <= 1e-14 and np.abs(N @ Pv - xi).max() <= 1e-14
+        assert np.all(N > 0)
+        h = 1e-6
+        for a in range(2):
+            e = np.zeros(2)
+            e[a] = h
+            fd = (polygon_shape(nn, xi + e)[0] - polygon_shape(nn, xi - e)[0]) / (2 * h)
+            assert np.abs(fd + dN[:, a]).max() <= 1e-7 * max(1.0, np.abs(dN).max())
+        assert np.abs(dN.sum(axis=0)).max() <= 1e-13 and np.abs(dN.T @ Pv + np.eye(2)).max() <= 1e-13
+    N0, _ = polygon_shape(nn, np.zeros(2))
+    assert np.abs(N0 - 1.0 / nn).max() <= 1e-15
+    for k in range(nn):  # near the middle of edge (k, k+1)
+        mid = 0.5 * (Pv[k] + Pv[(k + 1) % nn])
+        N, _ = polygon_shape(nn, mid * (1.0 - 1e-9))
+        want = np.zeros(nn)
+        want[k] = want[(k + 1) % nn] = 0.5
+        assert np.abs(N - want).max() <= 1e-7
+    # quadrature on the reference polygon (identity map): exact moments
+    mom = np.zeros(6)
+    for q in range(4 * nn):
+        xi, w, _, _, J = child_qp(2, ct, Pv, q)
+        assert abs(J - 1.0) <= 1e-14
+        mom += w * np.array([1.0, xi[0], xi[1], xi[0] ** 2, xi[0] * xi[1], xi[1] ** 2])
+    assert np.abs(mom - _fan_moments(Pv)).max() <= 1e-14
+    with pytest.raises(ValueError):
+        child_qp(2, ct, Pv, 4 * nn)
+    # physical polygons: affine image, then a convex distortion
+    M = np.array([[0.03, 0.011], [-0.004, 0.025]])
+    c = np.array([0.4, 0.7])
+    for distort in (False, True):
+        X = Pv @ M.T + c
+        if distort:
+            X = X + rng.uniform(-0.002, 0.002, X.shape)
+        area = _fan_moments(X - X.mean(axis=0))[0]
+        s = 0.0
+        for q in range(4 * nn):
+            xi, w, N, g, J = child_qp(2, ct, X, q)
+            assert J > 0
+            assert np.abs(X.T @ g - np.eye(2)).max() <= 1e-11
+            assert abs(N.sum() - 1.0) <= 1e-14
+            s += J * w
+        if distort:
+            assert abs(s - area) <= 2e-3 * area and abs(s - area) > 1e-12 * area
+        else:
+            assert abs(s - area) <= 1e-13 * area
 
 
 @pytest.mark.parametrize("kind", ["bl", "tet-bl"])
@@ -504,7 +641,7 @@ def test_cohesive_integration_with_a_linear_law(kind):
         v = 1e-5 * (R[-1] + 0.3 * R[0])
         upper_side = set()
         for k in range(rec.n_child):
-            cn = [int(rec.child_nodes[k][n]) for n in range(3 if rec.child_type[k] == _ffi.PG_CHILD_TRI3 else 4)]
+            cn = _ffi.child_nodes(rec, dim, k)
             if any(c in up for c in cn):
                 upper_side.update(cn)
         assert not upper_side & set(lo)
