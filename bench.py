@@ -65,8 +65,8 @@ def build_problem(name, device_pattern=True):
 
 def node_pair_kernels(dim, tiles, grid_nodes):
     """Names of the node-pair kernels one mixed-schedule assembly launches with
-    these tiles: the structured-grid kernel (tri3 nodes checked against the
-    grid formulas) and the generic node-group kernel for the other nodes."""
+    these tiles: the structured-grid kernel (tri3 / tet4 nodes checked against
+    the grid formulas) and the generic node-group kernel for the other nodes."""
     g, _, v = tiles
     tv, lists = v & 1, dim == 3 or bool(v & 2)
     sizes = (8, 16, 32) if dim == 2 else (2, 4, 8)
@@ -78,9 +78,9 @@ def node_pair_kernels(dim, tiles, grid_nodes):
         per = (8 if dim == 2 else 16) * (2 if t else 1) * ng
         return 64 if per < 64 else (512 if per > 512 else (per + 63) // 64 * 64)
     names = []
-    if dim == 2 and grid_nodes > 0 and not v & (128 | 8):
+    if grid_nodes > 0 and not v & (128 | 8):
         gs = g if overlap else g_rest
-        names.append(f"k_pair_struct<float, {gs}, {tv}, {bs(gs, tv)}>")
+        names.append(f"k_pair_struct{'' if dim == 2 else '3'}<float, {gs}, {tv}, {bs(gs, tv)}>")
     for ng in ((g, g_rest) if overlap else (g_rest,)):
         name = (f"k_pair_lists<{dim}, float, float, {ng}, {tv}, {bs(ng, tv)}>" if lists
                 else f"k_pair_group<{dim}, float, {ng}, {tv}, {bs(ng, tv)}>")

@@ -20,7 +20,8 @@ from .device import DeviceBuffer, synchronize
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 GROUPS = {2: (8, 16, 32), 3: (2, 4, 8)}
-GROUP_VARIANTS = {2: (0, 1, 2, 3, 4, 6), 3: (0, 1, 4, 5)}
+# (bit 7: the structured-grid nodes through the generic kernels instead)
+GROUP_VARIANTS = {2: (0, 1, 2, 3, 4, 6), 3: (0, 1, 4, 5, 128, 132)}
 CUT_VARIANTS = (0, 1, 2, 3)
 
 
@@ -90,9 +91,9 @@ def tune(asm, U=None, reps: int = 5, use_cache: bool = True, log=None):
                 if log:
                     log(f"[tune] group_nodes={g} cut_variant={c} group_variant={v2}: {table[f'{g},{c},{v2}']:.3f} ms")
             g, c, v = (int(x) for x in min(table, key=table.get).split(","))
-        # consecutive groups per block of the structured-grid kernel (group
-        # variant bits 8-9; tri3 meshes)
-        if p.dim == 2:
+        # consecutive groups per block of the structured-grid kernels (group
+        # variant bits 8-9)
+        if asm.grid_nodes() > 0:
             for k in (1, 2, 3):
                 v2 = (v & 255) | (k << 8)
                 table[f"{g},{c},{v2}"] = timed(g, c, v2, 0)

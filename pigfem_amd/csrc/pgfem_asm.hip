@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <type_traits>
 #include <string>
@@ -1955,6 +1956,180 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
     }
 }
 
+// Structured tet4 grid of generate_mesh (src/Mesh.cpp:443-630): cube
+// (cx, cy, cz) of the n^3 grid (node (x, y, z) = x + y S + z S^2, S = n + 1)
+// holds elements 6 (cx + cy n + cz n^2) + t, t = 0 .. 5, the same six tets in
+// every cube around its diagonal from corner (0, 0, 0) to (1, 1, 1).  Every
+// interior node has 24 incident elements and 15 node pairs; an off-diagonal
+// pair gets 4 or 6 contributions.  GridTpl3 holds what k_pair_lists reads from
+// its records for such a node: per incidence in ascending element id the cube
+// offset, the tet, the owner's local index and the element's nodes owner first
+// (la ^ n) as grid offsets; per node pair in ascending column id the column's
+// offset and its contributions by rank (incidence << 2 | the column's
+// owner-first position).  pg_asm_create accepts a node only after checking its
+// incidences, element nodes, pattern columns, scatter indices and dofs.
+constexpr int G3_NI = 24, G3_NP = 15, G3_CAP = 6;
+struct GridTpl3 {
+    int8_t ec[G3_NI][4];        // cube offset dx, dy, dz and tet t of incidence r
+    int8_t la[G3_NI];           // the owner's local index in the element
+    int8_t nd[G3_NI][4][3];     // the element's nodes, owner first, as grid offsets
+    int8_t pd[G3_NP][3];        // column node of slot s
+    uint8_t pc[G3_NP][G3_CAP];  // contributions to slot s by rank (0xff: none)
+    int8_t diag;                // slot of the diagonal pair
+};
+static_assert(sizeof(GridTpl3) % 4 == 0, "the template is copied to LDS as words");
+
+// Node-group kernel for the structured tet4 nodes whose incidences are all
+// uncut.  Each node comes with one record -- node, index of its first K value
+// (row 0, first column; rows 1 and 2 follow after 3 np values each), first
+// dof, element id of its cube's first tet -- and the row stride S.  The
+// arithmetic, the summation order and the values are those of k_pair_lists,
+// bit for bit; the loads are the records, then the element data.
+template <typename TC, int NG, int TV, int BS = GroupThreads<3, NG, TV>::value>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_struct3(
+    DevProb p, const int32_t* __restrict__ tpl, const int4* __restrict__ srec, const int32_t* __restrict__ sinf,
+    const double* __restrict__ Un, double* __restrict__ K, double* __restrict__ P, unsigned flags, int ngroups,
+    int per) {
+    constexpr int D = 3, NEN = 4, NQ = 5;
+    constexpr int MAXI = G3_NI * NG, MAXP = G3_NP * NG;
+    static_assert(NG <= 32, "the first wavefront holds the group's nodes");
+    __shared__ int s_node[NG], s_kb[NG], s_dof[NG], s_eb[NG], s_S[NG];
+    __shared__ TC s_g[MAXI][NEN][D];
+    __shared__ TC s_wlm[MAXI][2];           // w lambda, w mu of each incidence
+    __shared__ double s_P[MAXI][D];
+    __shared__ double s_dblk[MAXI][D * D];   // diagonal block (la, la) of each incidence
+    __shared__ TC sLM[MAX_LDS_MATS][2];      // lambda, mu of each LEI material
+    __shared__ double sLC[MAX_LDS_MATS][LeiCoef<D>::N];  // LEI stress coefficients
+    __shared__ GridTpl3 sT;
+
+    const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
+    const int tid = threadIdx.x;
+    // each block takes `per` consecutive groups (XCD-aware block order)
+    const int g0 = xcd_order(blockIdx.x, gridDim.x) * per, g1 = min(ngroups, g0 + per);
+    int4 r = make_int4(-1, 0, 0, 0);
+    int inf = 0;
+    if (g0 < g1) {
+        const int64_t k0 = int64_t(g0) * NG + (tid & (NG - 1));
+        r = srec[k0];
+        inf = sinf[k0];
+    }
+    constexpr int TW = int(sizeof(GridTpl3) / 4);
+    int tw[(TW + BS - 1) / BS];
+#pragma unroll
+    for (int k = 0; k < (TW + BS - 1) / BS; ++k) {
+        const int w = tid + k * BS;
+        tw[k] = tpl[w < TW ? w : TW - 1];
+    }
+    stage_lm<TC, BS>(p, 6, sLM);
+    for (int k = tid; k < MAX_LDS_MATS * LeiCoef<D>::N; k += BS) {
+        const int m = k / LeiCoef<D>::N, j = k % LeiCoef<D>::N;
+        const double v = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[LeiCoef<D>::index(j)];
+        sLC[m][j] = m < p.n_mat ? v : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < (TW + BS - 1) / BS; ++k) {
+        const int w = tid + k * BS;
+        if (w < TW) reinterpret_cast<int32_t*>(&sT)[w] = tw[k];
+    }
+    double wsum = 0.0;
+    for (int k = 0; k < NQ; ++k) wsum += TET_QW[k];
+    for (int g = g0; g < g1; ++g) {
+        if (tid < NG) {
+            s_node[tid] = r.x;
+            s_kb[tid] = r.y;
+            s_dof[tid] = r.z;
+            s_eb[tid] = r.w;
+            s_S[tid] = inf;
+        }
+        __syncthreads();
+        // the next group's records go out now and land during this group's work
+        if (g + 1 < g1) {
+            const int64_t k1 = int64_t(g + 1) * NG + (tid & (NG - 1));
+            r = srec[k1];
+            inf = sinf[k1];
+        }
+
+        // ---- phase A: incidences (as k_pair_lists, records from the template) ----
+        for (int i = tid; i < MAXI; i += BS) {
+            const int lo = i / G3_NI, rr = i - lo * G3_NI;
+            const int a = s_node[lo];
+            if (a < 0) continue;  // padding past the last node
+            const int64_t S = s_S[lo], n = S - 1;
+            const int64_t e = int64_t(s_eb[lo]) + 6 * (sT.ec[rr][0] + sT.ec[rr][1] * n + sT.ec[rr][2] * n * n) +
+                              sT.ec[rr][3];
+            int64_t ln[NEN];
+            for (int k = 0; k < NEN; ++k) ln[k] = a + sT.nd[rr][k][0] + sT.nd[rr][k][1] * S + sT.nd[rr][k][2] * S * S;
+            double x[NEN][3], u[NEN][D];  // U goes out with the coordinates
+            for (int k = 0; k < NEN; ++k)
+                for (int c = 0; c < D; ++c) {
+                    x[k][c] = p.coords[ln[k] * D + c];
+                    u[k][c] = Un[ln[k] * D + c];
+                }
+            const int mi = p.elem_mat[e] & (MAX_LDS_MATS - 1);
+            // every load of the incidence is out before the arithmetic
+            __builtin_amdgcn_sched_barrier(0);
+            double lc[LeiCoef<D>::N];
+            for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = sLC[mi][k];
+            double gr[NEN][D];
+            const double J = simplex_grads<D, double>(x, gr);
+            const double w = J * wsum;
+            TC gt[NEN][D];
+            for (int k = 0; k < NEN; ++k)
+                for (int c = 0; c < D; ++c) { gt[k][c] = TC(gr[k][c]); s_g[i][k][c] = gt[k][c]; }
+            plain_p<D>(gr, u, 0, w, lc, s_P[i]);
+            if (jac) {
+                const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
+                s_wlm[i][0] = wl;
+                s_wlm[i][1] = wm;
+                diag_block<D, TC>(gt, 0, wl, wm, reinterpret_cast<double (*)[D]>(s_dblk[i]));
+            }
+        }
+        __syncthreads();
+
+        // ---- phase B: node pairs ----
+        for (int kq = tid; kq < MAXP; kq += BS) {
+            const int lo = kq / G3_NP, slot = kq - lo * G3_NP;
+            if (s_node[lo] < 0) continue;
+            double acc[D][D];
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
+            double pa[D] = {0.0, 0.0, 0.0};
+            const bool isdiag = slot == sT.diag;
+            if (isdiag) {
+                // every incidence of the node, in ascending element order
+                const int t0 = lo * G3_NI;
+#pragma unroll 4
+                for (int t = t0; t < t0 + G3_NI; ++t) {
+                    if (func)
+                        for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
+                    if (jac)
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[t][i * D + j];
+                }
+            } else if (jac) {
+#pragma unroll
+                for (int k = 0; k < G3_CAP; ++k) {
+                    const unsigned code = sT.pc[slot][k];
+                    if (code == 0xffu) continue;
+                    const int t = lo * G3_NI + int(code >> 2), lb = int(code & 3);
+                    TC blk[D][D];
+                    iso_block<D, TC>(s_g[t][0], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
+                    for (int i = 0; i < D; ++i)
+                        for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                }
+            }
+            if (jac) {
+                const int64_t k0v = int64_t(s_kb[lo]) + 3 * slot;
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) K[k0v + int64_t(i) * 3 * G3_NP + j] = acc[i][j];
+            }
+            if (isdiag && func)
+                for (int i = 0; i < D; ++i) P[s_dof[lo] + i] = pa[i];
+        }
+        __syncthreads();  // the next group rewrites the tables
+    }
+}
+
 // The same schedule for tet4 meshes, where the per-pair block tables would not
 // fit in LDS (and a tri3 variant): phase A stores each incidence's TC
 // gradients, w lambda and w mu, its diagonal block and internal force, and
@@ -2360,6 +2535,111 @@ bool grid_node(const pg_problem_desc* s, const GridTplHost& H, int64_t a, int32_
     return true;
 }
 
+// GridTpl3 from the tet4 cube split of generate_mesh (src/Mesh.cpp:540-583:
+// corners 0 .. 7 = (0,0,0), (1,0,0), (1,1,0), (0,1,0), then the same at z + 1;
+// the six tets around the diagonal 0 - 6), for a node at the origin.
+bool build_grid_tpl3(GridTpl3& T) {
+    static const int C[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    static const int TT[6][4] = {{0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 5, 6, 4}, {1, 5, 6, 0}, {1, 6, 2, 0}};
+    memset(&T, 0, sizeof T);
+    struct Inc {
+        int64_t key;
+        int c[4], la;
+        int v[4][3];
+    };
+    std::vector<Inc> inc;
+    const int64_t N = 1000;  // element ids of any mesh with n > 2 sort the same way
+    for (int dz = -1; dz <= 0; ++dz)
+        for (int dy = -1; dy <= 0; ++dy)
+            for (int dx = -1; dx <= 0; ++dx)
+                for (int t = 0; t < 6; ++t) {
+                    Inc I{};
+                    I.la = -1;
+                    for (int k = 0; k < 4; ++k) {
+                        I.v[k][0] = dx + C[TT[t][k]][0];
+                        I.v[k][1] = dy + C[TT[t][k]][1];
+                        I.v[k][2] = dz + C[TT[t][k]][2];
+                        if (I.v[k][0] == 0 && I.v[k][1] == 0 && I.v[k][2] == 0) I.la = k;
+                    }
+                    if (I.la < 0) continue;
+                    I.key = 6 * (dx + dy * N + dz * N * N) + t;
+                    I.c[0] = dx; I.c[1] = dy; I.c[2] = dz; I.c[3] = t;
+                    inc.push_back(I);
+                }
+    if (int(inc.size()) != G3_NI) return false;
+    std::sort(inc.begin(), inc.end(), [](const Inc& A, const Inc& B) { return A.key < B.key; });
+    // the columns in ascending node id: (dz, dy, dx) for any row stride > 2
+    std::vector<std::array<int, 3>> nb;
+    for (const Inc& I : inc)
+        for (int k = 0; k < 4; ++k) nb.push_back({I.v[k][2], I.v[k][1], I.v[k][0]});
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    if (int(nb.size()) != G3_NP) return false;
+    T.diag = -1;
+    for (int sl = 0; sl < G3_NP; ++sl) {
+        T.pd[sl][0] = int8_t(nb[size_t(sl)][2]);
+        T.pd[sl][1] = int8_t(nb[size_t(sl)][1]);
+        T.pd[sl][2] = int8_t(nb[size_t(sl)][0]);
+        if (nb[size_t(sl)][0] == 0 && nb[size_t(sl)][1] == 0 && nb[size_t(sl)][2] == 0) T.diag = int8_t(sl);
+    }
+    if (T.diag < 0) return false;
+    memset(T.pc, 0xff, sizeof T.pc);
+    int cnt[G3_NP] = {0};
+    for (int r = 0; r < G3_NI; ++r) {
+        const Inc& I = inc[size_t(r)];
+        for (int k = 0; k < 4; ++k) T.ec[r][k] = int8_t(I.c[k]);
+        T.la[r] = int8_t(I.la);
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) T.nd[r][k][c] = int8_t(I.v[owner_first(4, I.la, k)][c]);
+        for (int m = 1; m < 4; ++m) {
+            int sl = 0;
+            while (sl < G3_NP && !(T.pd[sl][0] == T.nd[r][m][0] && T.pd[sl][1] == T.nd[r][m][1] &&
+                                   T.pd[sl][2] == T.nd[r][m][2]))
+                ++sl;
+            if (sl == G3_NP || cnt[sl] >= G3_CAP) return false;
+            T.pc[sl][cnt[sl]++] = uint8_t(r << 2 | m);
+        }
+    }
+    return true;
+}
+
+// Whether regular node a of a tet4 mesh with row stride S is a node of the
+// structured grid whose incidences, element nodes, pattern columns, scatter
+// indices and dofs are exactly what GridTpl3 gives; then its k_pair_struct3
+// record and word.
+bool grid_node3(const pg_problem_desc* s, const GridTpl3& T, int64_t S, int64_t a, int32_t rec[4], int32_t& inf) {
+    const int64_t t0 = s->n2e_ptr[a], ni = s->n2e_ptr[a + 1] - t0;
+    const int64_t p0 = s->npat_ptr[a], np = s->npat_ptr[a + 1] - p0;
+    if (ni != G3_NI || np != G3_NP) return false;
+    const int64_t n = S - 1;
+    auto rel = [&](int r) { return 6 * (T.ec[r][0] + T.ec[r][1] * n + T.ec[r][2] * n * n) + T.ec[r][3]; };
+    auto off = [&](const int8_t* o) { return o[0] + o[1] * S + o[2] * S * S; };
+    const int64_t eb = int64_t(s->n2e[t0 + G3_NI - 1]) - rel(G3_NI - 1);
+    if (eb < 0 || eb >= (int64_t(1) << 31)) return false;
+    for (int r = 0; r < G3_NI; ++r) {
+        const int64_t e = eb + rel(r);
+        if (s->n2e[t0 + r] != e || s->cut_of_elem[e] >= 0) return false;
+        for (int k = 0; k < 4; ++k)
+            if (s->conn[e * 4 + owner_first(4, T.la[r], k)] != a + off(T.nd[r][k])) return false;
+    }
+    const int64_t kb = s->npat_map[p0 * 9];
+    const int32_t dof = s->node_dof[a * 3];
+    for (int i = 1; i < 3; ++i)
+        if (s->node_dof[a * 3 + i] != dof + i) return false;
+    for (int sl = 0; sl < G3_NP; ++sl) {
+        if (s->npat_col[p0 + sl] != a + off(T.pd[sl])) return false;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (s->npat_map[(p0 + sl) * 9 + i * 3 + j] != kb + i * 3 * np + 3 * sl + j) return false;
+    }
+    rec[0] = int32_t(a);
+    rec[1] = int32_t(kb);
+    rec[2] = dof;
+    rec[3] = int32_t(eb);
+    inf = int32_t(S);
+    return true;
+}
+
 int check_desc(const pg_problem_desc* s) {
     if (!s) { g_err = "null description"; return PG_E_ARG; }
     if (s->dim != 2 && s->dim != 3) { g_err = "dim must be 2 or 3"; return PG_E_ARG; }
@@ -2569,8 +2849,22 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
     do { if (tv) STRUCT1(TC, NGV, 1); else STRUCT1(TC, NGV, 0); } while (0)
 #define STRUCT3(TC) \
     do { if (ng == 8) STRUCT2(TC, 8); else if (ng == 16) STRUCT2(TC, 16); else STRUCT2(TC, 32); } while (0)
-            if (fp32) STRUCT3(float);
-            else STRUCT3(double);
+#define STRUCT1T(TC, NGV, TVV) \
+    k_pair_struct3<TC, NGV, TVV><<<grid_n, GroupThreads<3, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->srec, h->sinf, h->Un, dK, dP, flags, ngr, per)
+#define STRUCT2T(TC, NGV) \
+    do { if (tv) STRUCT1T(TC, NGV, 1); else STRUCT1T(TC, NGV, 0); } while (0)
+#define STRUCT3T(TC) \
+    do { if (ng == 2) STRUCT2T(TC, 2); else if (ng == 4) STRUCT2T(TC, 4); else STRUCT2T(TC, 8); } while (0)
+            if (d.dim == 2) {
+                if (fp32) STRUCT3(float);
+                else STRUCT3(double);
+            } else {
+                if (fp32) STRUCT3T(float);
+                else STRUCT3T(double);
+            }
+#undef STRUCT3T
+#undef STRUCT2T
+#undef STRUCT1T
 #undef STRUCT3
 #undef STRUCT2
 #undef STRUCT1
@@ -2862,6 +3156,7 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         std::vector<int32_t> srec_v, sinf_v;
         std::vector<char> grid(size_t(nall), 0);
         GridTplHost T;
+        GridTpl3 T3;
         if (s->dim == 2 && build_grid_tpl(T)) {
             for (int64_t a = 0; a < s->n_nodes; ++a) {
                 int32_t rec[4], inf = 0;
@@ -2870,6 +3165,18 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
                 srec_v.insert(srec_v.end(), rec, rec + 4);
                 sinf_v.push_back(inf);
             }
+        }
+        if (s->dim == 3 && build_grid_tpl3(T3)) {
+            // the generator's cube has S^3 nodes, S = n + 1
+            int64_t S = int64_t(std::llround(std::cbrt(double(s->n_nodes))));
+            if (S >= 3 && S * S * S == s->n_nodes && S < (int64_t(1) << 24))
+                for (int64_t a = 0; a < s->n_nodes; ++a) {
+                    int32_t rec[4], inf = 0;
+                    if (dirty[a] || !grid_node3(s, T3, S, a, rec, inf)) continue;
+                    grid[a] = 1;
+                    srec_v.insert(srec_v.end(), rec, rec + 4);
+                    sinf_v.push_back(inf);
+                }
         }
         h->n_struct = int64_t(sinf_v.size());
         // padding records (node -1) up to a whole number of the largest group
@@ -2913,8 +3220,10 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         }
         {
             int32_t* t_;
-            if ((rc = alloc(h, &t_, sizeof(GridTpl) / 4))) return fail(rc);
-            if (hipMemcpy(t_, &T.t, sizeof(GridTpl), hipMemcpyHostToDevice) != hipSuccess) {
+            const size_t tb = s->dim == 2 ? sizeof(GridTpl) : sizeof(GridTpl3);
+            const void* ts = s->dim == 2 ? static_cast<const void*>(&T.t) : static_cast<const void*>(&T3);
+            if ((rc = alloc(h, &t_, tb / 4))) return fail(rc);
+            if (hipMemcpy(t_, ts, tb, hipMemcpyHostToDevice) != hipSuccess) {
                 g_err = "hipMemcpy of the grid template failed";
                 return fail(PG_E_HIP);
             }

@@ -173,8 +173,8 @@ def test_every_schedule_variant(kind, n, ni):
         # bits 5-6: another group size for the groups after the cut kernels; bit 7: the nodes of the
         # structured grid through the generic kernel; bits 8-9: consecutive groups per block of the structured kernel
         variants = ((*range(8), 8, 11, 16, 18, 32, 64, 96, 97, 100, 128, 129, 130, 132, 133, 228, 256, 516, 768, 897) if p.dim == 2
-                    else (0, 1, 4, 5, 8, 13, 16, 20, 32, 64, 97, 101))
-        if p.dim == 2:
+                    else (0, 1, 4, 5, 8, 13, 16, 20, 32, 64, 97, 101, 128, 133, 256, 517, 768))
+        if kind in ("op", "multi-op", "tet"):
             assert AssemblerGPU(p, mode).grid_nodes() > (0.6 if kind == "op" else 0.05) * p.n_nodes
         for g, c, v in [(g, c, v) for g in groups for c in range(4) for v in variants]:
             asm = AssemblerGPU(p, mode)
