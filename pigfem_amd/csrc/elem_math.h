@@ -161,6 +161,17 @@ __device__ inline void iso_block(const T* ga, const T* gb, T wl, T wm, T (*out)[
     }
 }
 
+// The sum over qps of iso_block(ga, gb_q, wl_q, wm_q) for a constant ga (a
+// parent node's gradient): out[i][j] = ga_i vl_j + ga_j vm_i + delta_ij ga . vm
+// with vl = sum_q wl_q gb_q and vm = sum_q wm_q gb_q.
+template <int D, typename T>
+__device__ inline void iso_block_v(const T* ga, const T* vl, const T* vm, T (*out)[D]) {
+    T dot = ga[0] * vm[0];
+    for (int k = 1; k < D; ++k) dot = fused(ga[k], vm[k], dot);
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) out[i][j] = fused(ga[i], vl[j], fused(ga[j], vm[i], i == j ? dot : T(0)));
+}
+
 // LinearElasticIsotropicMaterial::computeStress, src/material_lei.cpp:232-261.
 template <int D>
 __device__ inline void lei_stress(const double* Dm, const double* e, double* s) {

@@ -659,6 +659,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     __shared__ double cRDR[EPW][CUT_MAXC][D][D];
     __shared__ double ctR[EPW][CUT_MAXC][D];
     __shared__ pg_cut_rec recs[EPW];
+    // 3D: sums over the qps of w lambda and w mu, and of w lambda g_l and
+    // w mu g_l for each enrichment node l (the blocks of a parent node, whose
+    // gradient is the same at every qp, follow from them)
+    __shared__ TK sS[EPW][2];
+    __shared__ TK sV[EPW][D == 3 ? MAXN - NEN : 1][2][D];
     __shared__ TK sLM[MAX_LDS_MATS][2];  // lambda, mu of each LEI material
     __shared__ double sLC[MAX_LDS_MATS][LeiCoef<D>::N];  // LEI stress coefficients
     __shared__ CohMat sCM[MAX_LDS_MATS];                  // cohesive law parameters
@@ -816,6 +821,27 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         }
     }
     __syncthreads();
+    if constexpr (D == 3) {
+        // one lane per sum: S (2), then V (2 x D per enrichment node), each over
+        // its qps in ascending order in FP64, rounded once to TK
+        const int ne = nloc > NEN ? nloc - NEN : 0;
+        for (int task = gl; task < 2 + 2 * D * ne; task += GS) {
+            if (task < 2) {
+                double a = 0.0;
+                for (int q = 0; q < nvq; ++q) a += double(qwl[g][q][task]);
+                sS[g][task] = TK(a);
+            } else {
+                const int k = task - 2, l = k / (2 * D), kind = (k / D) & 1, c = k % D;
+                double a = 0.0;
+                for (QMask qm = qmask[g][NEN + l]; qm; qm &= qm - QMask(1)) {
+                    const int q = sizeof(QMask) == 8 ? __builtin_ctzll(qm) : __builtin_ctz(uint32_t(qm));
+                    a += double(qwl[g][q][kind]) * qg[g][q][l][c];
+                }
+                sV[g][l][kind][c] = TK(a);
+            }
+        }
+        __syncthreads();
+    }
     if (!live) return;
 
     // gradient of local node l at qp q (parent gradient, the child gradient of the
@@ -839,8 +865,26 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         TK vol[D][D];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) vol[i][j] = TK(0);
+        bool summed = false;
+        if constexpr (D == 3) {
+            if (li < NEN) {
+                // a parent row: its gradient is the same at every qp, so the sum
+                // over the qps collapses onto the sums S (parent column) or V
+                TK gi[D];
+                for (int a = 0; a < D; ++a) gi[a] = TK(pg[g][li][a]);
+                if (lj < NEN) {
+                    TK gj[D];
+                    for (int a = 0; a < D; ++a) gj[a] = TK(pg[g][lj][a]);
+                    iso_block<D, TK>(gi, gj, sS[g][0], sS[g][1], vol);
+                } else {
+                    iso_block_v<D, TK>(gi, sV[g][lj - NEN][0], sV[g][lj - NEN][1], vol);
+                }
+                summed = true;
+            }
+        }
         // the qps at which both gradients are not zero, ascending (the others
         // contribute nothing)
+        if (!summed)
         for (QMask qm = qmask[g][li] & qmask[g][lj]; qm; qm &= qm - QMask(1)) {
             const int q = sizeof(QMask) == 8 ? __builtin_ctzll(qm) : __builtin_ctz(uint32_t(qm));
             double gid[D], gjd[D];
@@ -2947,10 +2991,24 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
                 else k_cut<2, double, 2, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 1) / 2, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
             }
         } else {
+            // small class: elements per wave and the register cap (waves per
+            // EU) by the cut variant
+#define CUT_SMALL3(TK, KST, EPW, OCC) \
+    k_cut<3, TK, EPW, 8, 30, 4, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
             if (n0) {
-                if (fp32 && !cut64) k_cut<3, float, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<3, double, 2, 8, 30, 4, 1><<<(n0 + 1) / 2, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32 && !cut64) {
+                    if (cv == 1) CUT_SMALL3(float, h->Kc32, 2, 3);
+                    else if (cv == 2) CUT_SMALL3(float, h->Kc32, 1, 1);
+                    else if (cv == 3) CUT_SMALL3(float, h->Kc32, 2, 2);
+                    else CUT_SMALL3(float, h->Kc32, 2, 1);
+                } else {
+                    if (cv == 1) CUT_SMALL3(double, h->Kc64, 2, 3);
+                    else if (cv == 2) CUT_SMALL3(double, h->Kc64, 1, 1);
+                    else if (cv == 3) CUT_SMALL3(double, h->Kc64, 2, 2);
+                    else CUT_SMALL3(double, h->Kc64, 2, 1);
+                }
             }
+#undef CUT_SMALL3
             if (n1) {
                 if (fp32 && !cut64) k_cut<3, float, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
                 else k_cut<3, double, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
