@@ -279,7 +279,11 @@ class DeviceNewtonDriver(NewtonDriver):
     (at most cg_max_iter iterations) while the tangent lets it converge --
     a tangent without softening is symmetric positive definite -- and from the
     first CG failure on (an indefinite tangent after the onset of cohesive
-    softening) with FGMRES; the failed attempt restarts from zero."""
+    softening) with FGMRES; the failed attempt restarts from zero.  "cg" and
+    "bicgstab" keep their method, but a solve of theirs that breaks down or
+    stalls (at most cg_max_iter iterations) is done again from zero by FGMRES:
+    the reference's direct solve has no Krylov breakdown, so one must not
+    count as a failed Newton iteration."""
 
     def __init__(self, problem, asm, method="auto", lin_rtol=1e-10, lin_max_iter=20000, cg_max_iter=5000, **kw):
         from .device import DeviceBuffer
@@ -288,12 +292,14 @@ class DeviceNewtonDriver(NewtonDriver):
         super().__init__(problem, None, **kw)
         self.lin.kind = f"gpu-{method}"
         self.asm = asm
-        if method == "auto":
-            self.ksp = DeviceKSP(problem, "cg")
-            self.ksp_fallback = DeviceKSP(problem, "fgmres")
-        else:
+        # sticky: after the first failure of the primary method, FGMRES only
+        self.sticky = method == "auto"
+        if method == "fgmres":
             self.ksp = DeviceKSP(problem, method)
             self.ksp_fallback = None
+        else:
+            self.ksp = DeviceKSP(problem, "cg" if method == "auto" else method)
+            self.ksp_fallback = DeviceKSP(problem, "fgmres")
         self.cg_max_iter = cg_max_iter
         self.lin_rtol, self.lin_max_iter = lin_rtol, lin_max_iter
         p = problem
@@ -326,8 +332,9 @@ class DeviceNewtonDriver(NewtonDriver):
             try:
                 it, _ = self.ksp.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.cg_max_iter)
             except PgError:
-                self.ksp.close()
-                self.ksp = None
+                if self.sticky:
+                    self.ksp.close()
+                    self.ksp = None
                 self.ddu.copy_from(self.dzero)
                 it, _ = self.ksp_fallback.solve(self.dK, self.dP, self.ddu, self.lin_rtol, self.lin_max_iter)
         else:
