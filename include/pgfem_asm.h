@@ -73,6 +73,12 @@ typedef struct pg_material {
     double  delta_c2;   /* BL: opening at full separation                     */
     double  beta2;      /* cohesive: beta^2 (tangential weight)               */
     double  alpha;      /* BL: contact penalty sharpness                      */
+    double  thermal_exp;     /* LEI: thermal expansion ratio (set_parameter
+                                "THERMAL_EXPANSION", material_lei.cpp:384,690)  */
+    int32_t thermal_exp_set; /* LEI: nonzero once thermal_exp is set; only then
+                                does a temperature change strain the material
+                                (material_lei.cpp:272-291)                      */
+    int32_t pad1;
 } pg_material;
 
 /* ----------------------------------------------------------------------------
@@ -219,6 +225,19 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
                            const double* disv_committed, double* disv_trial,
                            double delta_t, unsigned flags,
                            double* dK, double* dP, void* stream);
+
+/* Thermal pre-load (AssemblerSmallStrainStructural_ThermalPreLoad,
+ * src/AssemblerSmallStrainStructural_ThermalPreLoad.cpp:40-98,213-227): the
+ * temperature change that the following assemblies of this handle apply, i.e.
+ * the assembler's current delta T that its assemble_new_load_step sets (the
+ * full delta T once relaxation is complete, else delta T times the load-step
+ * time fraction) and KernelVolumetric hands to the material as
+ * input.temp_change.  Every LEI material whose thermal_exp_set is nonzero then
+ * adds the thermal strain -temp_change * thermal_exp to its normal strains
+ * before its stress (src/material_lei.cpp:272-291): P changes, K does not.
+ * 0 (the default) is the plain structural assembler.  Applies to every
+ * schedule and to calls made after it returns; PG_E_ARG if not finite. */
+int pg_asm_set_temperature_change(pg_asm* h, double temp_change);
 
 /* Status of the handle's last assembly (either API): waits for it to complete
  * and returns PG_E_MESH if a cut element had a child of negative Jacobian (the

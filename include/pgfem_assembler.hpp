@@ -26,6 +26,7 @@
 
 #include <stdint.h>
 
+#include <cctype>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -187,13 +188,65 @@ public:
         return PG_OK;
     }
 
-private:
+protected:
     const pg_problem_desc& _d;
     Schedule _sched;
     pg_asm* _h = nullptr;
     InternalVars* _iv = nullptr;
     int64_t _n_dof = 0;
     std::vector<double> _U, _ic, _it;
+};
+
+/* AssemblerSmallStrainStructural_ThermalPreLoad
+ * (src/AssemblerSmallStrainStructural_ThermalPreLoad.cpp): the structural
+ * assembler with a temperature change that strains the LEI materials whose
+ * thermal_exp_set is nonzero.  set_parameter / get_parameter / clear and the
+ * load-step rule of assemble_new_load_step follow :25-33,197-267; the current
+ * change reaches the kernels through pg_asm_set_temperature_change. */
+class AssemblerGPU_ThermalPreLoad : public AssemblerGPU {
+public:
+    using AssemblerGPU::AssemblerGPU;
+
+    void clear() {
+        _delta_T = _curr_delta_T = 0.0;
+        _temp_set = _relaxation_complete = false;
+        pg_asm_set_temperature_change(_h, 0.0);
+    }
+    /* returns PG_E_ARG for an unknown name (the reference calls err_message) */
+    int set_parameter(std::string name, double val) {
+        for (auto& c : name) c = char(std::toupper(static_cast<unsigned char>(c)));
+        if (name == "TEMP" || name == "DELTA_T" || name == "DELTA_TEMP") {
+            _delta_T = val;
+            _curr_delta_T = 0.0;
+            _temp_set = true;
+        } else if (name == "COMPLETE" || name == "CONVERGED") {
+            _relaxation_complete = val != 0.0;
+        } else {
+            return PG_E_ARG;
+        }
+        return pg_asm_set_temperature_change(_h, _curr_delta_T);
+    }
+    double get_parameter(std::string name) const {
+        for (auto& c : name) c = char(std::toupper(static_cast<unsigned char>(c)));
+        if (name == "TEMP" || name == "DELTA_T" || name == "DELTA_TEMP") {
+            if (!_temp_set) throw std::runtime_error("Please set a temperature change value before attempting to get it.");
+            return _delta_T;
+        }
+        if (name == "COMPLETE" || name == "CONVERGED") return _relaxation_complete ? 1.0 : 0.0;
+        if (name == "CURRENT TEMP" || name == "CURRENT_TEMP") return _curr_delta_T;
+        throw std::runtime_error(name + " is not a valid parameter name");
+    }
+    /* the temperature update of assemble_new_load_step; the prescribed
+     * displacements and loads of the base class stay with the caller */
+    int assemble_new_load_step(double current_time_frac) {
+        if (!_temp_set) return PG_E_ARG;
+        _curr_delta_T = _relaxation_complete ? _delta_T : _delta_T * current_time_frac;
+        return pg_asm_set_temperature_change(_h, _curr_delta_T);
+    }
+
+private:
+    double _delta_T = 0.0, _curr_delta_T = 0.0;
+    bool _temp_set = false, _relaxation_complete = false;
 };
 
 }  // namespace pgfem

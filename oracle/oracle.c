@@ -71,6 +71,10 @@ typedef struct oracle {
     int32_t* colour_ptr;  /* [ncolour + 1] */
     int32_t* colour_elem; /* elements grouped by colour, ascending within one */
     int own_pattern;      /* 0: rp / ci are the description's (harness) pattern */
+    /* thermal pre-load: input.temp_change of every volumetric qp
+     * (AssemblerSmallStrainStructural_ThermalPreLoad::KernelVolumetric,
+     * src/AssemblerSmallStrainStructural_ThermalPreLoad.cpp:58-59) */
+    double temp_change;
     char err[256];
 } oracle;
 
@@ -716,6 +720,13 @@ static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int f
                 for (int j = 0; j < nd; ++j) t += U[dofs[j]] * B[s * nd + j];
                 eps[s] = 0.0 + 1.0 * t;
             }
+            if (m->thermal_exp_set) {
+                /* LinearElasticIsotropicMaterial::Constitutive with the thermal
+                 * expansion set (src/material_lei.cpp:272-291): the normal
+                 * strains take the thermal strain, strain = 1.0 * thermal + strain */
+                double strain_val = -1.0 * o->temp_change * m->thermal_exp;
+                for (int i = 0; i < dim; ++i) eps[i] = 1.0 * strain_val + eps[i];
+            }
             lei(m, dim, eps, sig);
             if (jac) {
                 for (int i = 0; i < nd; ++i)
@@ -937,6 +948,8 @@ int oracle_assemble_range(oracle* o, const double* U, const double* isv_c, doubl
     }
     return bad ? -1 : 0;
 }
+
+void oracle_set_temperature_change(oracle* o, double temp_change) { o->temp_change = temp_change; }
 
 /* ------------------------------------------------------------------------ */
 /* Test hooks: the oracle's constitutive routines, exposed so that tests can pin

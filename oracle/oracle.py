@@ -68,11 +68,13 @@ class Oracle:
             self.rp.append(np.ctypeslib.as_array(rp, shape=(rows[b] + 1,)).copy())
             self.ci.append(np.ctypeslib.as_array(ci, shape=(max(self.nnz[b], 1),))[: self.nnz[b]].copy())
 
-    def assemble(self, U, isv_committed=None, func=True, jac=True, threads=0):
+    def assemble(self, U, isv_committed=None, func=True, jac=True, threads=0, temp_change=0.0):
         """threads = 0: the element loop in element order (the checker);
         threads > 0: colour by colour on that many OpenMP threads (the threaded
-        CPU baseline; equal to the serial result up to rounding)."""
+        CPU baseline; equal to the serial result up to rounding).
+        temp_change: the thermal pre-load's current temperature change."""
         p = self.problem
+        _set_temp(self._h, temp_change)
         U = np.ascontiguousarray(U, dtype=np.float64)
         K = np.zeros(sum(self.nnz), dtype=np.float64)
         P = np.zeros(p.n_dof, dtype=np.float64)
@@ -97,7 +99,14 @@ class Oracle:
             pass
 
 
-def assemble_in_ranges(problem, U, isv_committed=None, chunk=2_000_000, threads=0):
+def _set_temp(h, temp_change):
+    L = lib()
+    L.oracle_set_temperature_change.restype = None
+    L.oracle_set_temperature_change.argtypes = [C.c_void_p, C.c_double]
+    L.oracle_set_temperature_change(h, float(temp_change))
+
+
+def assemble_in_ranges(problem, U, isv_committed=None, chunk=2_000_000, threads=0, temp_change=0.0):
     """The oracle's assembly of the whole mesh into the harness's own 4-block
     pattern (problem.rp / problem.ci), element range by element range, so that
     only one range's shape cache is held at a time.  threads = 0 gives the
@@ -123,6 +132,7 @@ def assemble_in_ranges(problem, U, isv_committed=None, chunk=2_000_000, threads=
             err = L.oracle_error(h)
             if err:
                 raise RuntimeError("oracle set-up: " + err.decode())
+            _set_temp(h, temp_change)
             rc = L.oracle_assemble_range(h, U.ctypes.data, ic.ctypes.data if p.n_cqp else None, it.ctypes.data, 1, 1,
                                          K.ctypes.data, P.ctypes.data, int(e0 == 0), int(threads))
             if rc != 0:

@@ -6,10 +6,11 @@ description, and thin Python mirrors of the reference's Assembler API.
 """
 from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PgError, lib  # noqa: F401
 from .problem import RVEConfig, RVEProblem, batch_config, config, small_config  # noqa: F401
-from .assembler import AssemblerGPU, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
+from .assembler import AssemblerGPU, AssemblerGPUThermalPreLoad, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
     write_cohesive_failure_lengths  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
 
-__all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU", "PMatrix", "PVector",
+__all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU",
+           "AssemblerGPUThermalPreLoad", "PMatrix", "PVector",
            "cut_inclusions", "write_cohesive_openings", "write_cohesive_failure_lengths",
            "DeviceBuffer", "synchronize", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

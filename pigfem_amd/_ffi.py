@@ -38,6 +38,7 @@ class PgMaterial(C.Structure):
         ("type", C.c_int32), ("nstr", C.c_int32), ("D", C.c_double * 36),
         ("sigma_c", C.c_double), ("delta_c", C.c_double), ("delta_c1", C.c_double),
         ("delta_c2", C.c_double), ("beta2", C.c_double), ("alpha", C.c_double),
+        ("thermal_exp", C.c_double), ("thermal_exp_set", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
@@ -108,6 +109,7 @@ SIGNATURES = {
     "pg_asm_cohesive_openings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_cohesive_failure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_check_error": (C.c_int, [C.c_void_p]),
+    "pg_asm_set_temperature_change": (C.c_int, [C.c_void_p, C.c_double]),
     "pg_asm_grid_nodes": (C.c_int64, [C.c_void_p]),
     "pg_csr_pcg": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                              C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p]),

@@ -144,6 +144,11 @@ class AssemblerGPU:
                                           self.flags(func, jac), dp(K), dp(P), st)
         check(rc, "pg_asm_assemble_device")
 
+    def set_temperature_change(self, temp_change: float) -> None:
+        """Temperature change applied by the following assemblies (the thermal
+        pre-load's input.temp_change; pg_asm_set_temperature_change)."""
+        check(lib().pg_asm_set_temperature_change(self._h, float(temp_change)), "pg_asm_set_temperature_change")
+
     def check_error(self) -> None:
         """Waits for the last assembly and raises PgError (PG_E_MESH) if a cut
         element had a child of negative Jacobian (src/Mesh.cpp:1561)."""
@@ -220,6 +225,61 @@ class AssemblerGPU:
             self.close()
         except Exception:
             pass
+
+
+class AssemblerGPUThermalPreLoad(AssemblerGPU):
+    """Drop-in for AssemblerSmallStrainStructural_ThermalPreLoad
+    (src/AssemblerSmallStrainStructural_ThermalPreLoad.cpp): the structural
+    assembler with a temperature change that strains every LEI material whose
+    thermal expansion ratio is set (pg_material.thermal_exp_set).  The parameter
+    interface and the load-step rule are the reference's (:213-267):
+    set_parameter("DELTA_T" | "TEMP" | "DELTA_TEMP", v) sets the target change
+    and resets the current one to 0; set_parameter("COMPLETE" | "CONVERGED", v)
+    marks the relaxation complete (v != 0); assemble_new_load_step(frac) sets the
+    current change to delta_T once relaxation is complete, else delta_T * frac
+    (:197-211).  Assemblies use the current change."""
+
+    def __init__(self, problem, mode: str = "fp64"):
+        super().__init__(problem, mode)
+        self._delta_T = 0.0
+        self._curr_delta_T = 0.0
+        self._temp_set = False
+        self._relaxation_complete = False
+
+    def set_parameter(self, name: str, val: float) -> None:
+        name = name.upper()
+        if name in ("TEMP", "DELTA_T", "DELTA_TEMP"):
+            self._delta_T = float(val)
+            self._curr_delta_T = 0.0
+            self._temp_set = True
+        elif name in ("COMPLETE", "CONVERGED"):
+            self._relaxation_complete = val != 0.0
+        else:
+            raise ValueError(f"{name} is not a valid parameter name")
+        self.set_temperature_change(self._curr_delta_T)
+
+    def get_parameter(self, name: str) -> float:
+        name = name.upper()
+        if name in ("TEMP", "DELTA_T", "DELTA_TEMP"):
+            if not self._temp_set:
+                raise ValueError("Please set a temperature change value before attempting to get it.")
+            return self._delta_T
+        if name in ("COMPLETE", "CONVERGED"):
+            return 1.0 if self._relaxation_complete else 0.0
+        if name in ("CURRENT TEMP", "CURRENT_TEMP"):
+            return self._curr_delta_T
+        raise ValueError(f"{name} is not a valid parameter name")
+
+    def assemble_new_load_step(self, current_time_frac: float) -> None:
+        if not self._temp_set:
+            raise ValueError("Please set a temperature change value before attemping to solve a residual strain problem.")
+        self._curr_delta_T = self._delta_T if self._relaxation_complete else self._delta_T * current_time_frac
+        self.set_temperature_change(self._curr_delta_T)
+
+    def clear(self) -> None:
+        self._delta_T = self._curr_delta_T = 0.0
+        self._temp_set = self._relaxation_complete = False
+        self.set_temperature_change(0.0)
 
 
 def cut_inclusions(problem) -> np.ndarray:

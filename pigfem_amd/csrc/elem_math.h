@@ -172,6 +172,18 @@ __device__ inline void iso_block_v(const T* ga, const T* vl, const T* vm, T (*ou
         for (int j = 0; j < D; ++j) out[i][j] = fused(ga[i], vl[j], fused(ga[j], vm[i], i == j ? dot : T(0)));
 }
 
+// Thermal strain of LinearElasticIsotropicMaterial::Constitutive
+// (src/material_lei.cpp:272-291): with its thermal expansion set, the material
+// adds th = -temp_change * thermal_exp to the normal strains (the first D Voigt
+// components) before computing the stress, strain = 1.0 * thermal + strain.
+// th = 0 leaves the strain as it is.
+template <int D>
+__device__ inline void add_thermal(double* e, double th) {
+    e[0] = th + e[0];
+    e[1] = th + e[1];
+    if constexpr (D == 3) e[2] = th + e[2];
+}
+
 // LinearElasticIsotropicMaterial::computeStress, src/material_lei.cpp:232-261.
 template <int D>
 __device__ inline void lei_stress(const double* Dm, const double* e, double* s) {
@@ -194,15 +206,19 @@ __device__ inline void lei_stress(const double* Dm, const double* e, double* s) 
 // LinearElasticIsotropicMaterial builds (src/material_lei.cpp:52-97), whose
 // equal entries are equal to the bit, this is lei_stress's arithmetic with its
 // exact-zero products skipped; the node-group kernels load these three values
-// instead of D.
+// instead of D.  The fourth coefficient is the material's thermal strain
+// (add_thermal), 0 without a temperature change.
 template <int D> struct LeiCoef {
-    static constexpr int N = 3;
+    static constexpr int N = 4;
     __host__ __device__ static constexpr int index(int k) {
         return k == 0 ? 0 : (k == 1 ? 1 : (D == 2 ? 8 : 35));
     }
 };
 template <int D>
-__device__ inline void lei_stress_c(const double* c, const double* e, double* s) {
+__device__ inline void lei_stress_c(const double* c, const double* e_in, double* s) {
+    double e[D == 2 ? 3 : 6];
+    for (int k = 0; k < (D == 2 ? 3 : 6); ++k) e[k] = e_in[k];
+    add_thermal<D>(e, c[3]);
     if constexpr (D == 2) {
         s[0] = c[0] * e[0] + c[1] * e[1];
         s[1] = c[1] * e[0] + c[0] * e[1];

@@ -84,6 +84,10 @@ struct DevProb {
     const int64_t* ckoff;
     const int64_t* cpoff;
     int n_mat;
+    // thermal strain of each material for the handle's current temperature
+    // change: -temp_change * thermal_exp where the material's expansion is
+    // set, else 0 (src/material_lei.cpp:272-291); indexed by material & 7
+    double eth[8];
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
@@ -136,6 +140,13 @@ constexpr int64_t KC_PAD = 8 * 9;        // staged cut tangents: padding past th
 // kernels compute an incidence's element in this order, owner first.
 __host__ __device__ inline int owner_first(int nen, int la, int n) {
     return nen == 3 ? (la + n) % 3 : (la ^ n);
+}
+
+// Coefficient k of material m's LEI stress (LeiCoef): three entries of D, then
+// the thermal strain of the handle's current temperature change.
+template <int D>
+__device__ inline double lei_coef(const DevProb& p, int m, int k) {
+    return k < 3 ? p.mats[m].D[LeiCoef<D>::index(k)] : p.eth[m & 7];
 }
 
 template <int D> struct CutLim;
@@ -403,6 +414,7 @@ __global__ __launch_bounds__(256) void k_elem_plain(DevProb p, const double* __r
 #pragma unroll
         for (int k = 0; k < NS; ++k) eps[k] += b[k] * u[i];
     }
+    add_thermal<D>(eps, p.eth[p.elem_mat[e] & 7]);
     lei_stress<D>(Dm, eps, sig);
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
@@ -528,6 +540,7 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
             }
         }
         double sig[NS];
+        add_thermal<D>(eps, p.eth[rec.child_mat[k] & 7]);
         lei_stress<D>(sD[rec.child_mat[k]], eps, sig);
         wq[t] = Jc * w;
         mq[t] = rec.child_mat[k];
@@ -676,8 +689,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     // mu and stress coefficients, the record and the local node list (both
     // indexed by the class position); the node data follows the node list.
     // Loads use clamped indices so that none sits behind a branch.
-    const int mm = threadIdx.x >> 2, mj = threadIdx.x & 3;  // 64 lanes: 16 x 4 >= 8 materials x 3
-    const double mv0 = p.mats[mm < p.n_mat ? mm : p.n_mat - 1].D[mj < 3 ? LeiCoef<D>::index(mj) : 0];
+    const int mm = threadIdx.x >> 2, mj = threadIdx.x & 3;  // 64 lanes: 16 x 4 >= 8 materials x 4
+    const double mv0 = lei_coef<D>(p, mm < p.n_mat ? mm : p.n_mat - 1, mj);
     // cohesive parameters: lane t < 48 loads field t % 6 (sigma_c .. alpha,
     // contiguous) of material t / 6, and every lane the type of material t % 8
     const int cm = threadIdx.x < 48 ? threadIdx.x / 6 : 0, cf = threadIdx.x < 48 ? threadIdx.x % 6 : 0;
@@ -701,7 +714,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     }
     {
         const double v = mm < p.n_mat ? mv0 : 0.0;
-        if (mm < MAX_LDS_MATS && mj < 3) sLC[mm][mj] = v;
+        if (mm < MAX_LDS_MATS) sLC[mm][mj] = v;
         if (mm < MAX_LDS_MATS && mj == 1) sLM[mm][0] = TK(v);  // D[0][1] = lambda
         if (mm < MAX_LDS_MATS && mj == 2) sLM[mm][1] = TK(v);  // D[last][last] = mu
         if (threadIdx.x < 48) (&sCM[cm].sigma_c)[cf] = cv0;
@@ -1573,7 +1586,7 @@ __device__ inline void pair_from_global(const DevProb& p, const double* __restri
             }
             if (diag) {
                 double pp[D], lc[LeiCoef<D>::N];
-                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = lei_coef<D>(p, mi, k);
                 plain_p<D>(g, u, 0, w, lc, pp);
                 for (int i = 0; i < D; ++i) pa[i] += pp[i];
             }
@@ -1692,7 +1705,7 @@ __global__ __launch_bounds__(BS) void k_pair_group(DevProb p, const double* __re
                     }
                 const int mi = tag & 7;
                 double lc[LeiCoef<D>::N];
-                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = lei_coef<D>(p, mi, k);
                 // every load of the incidence is out before the arithmetic
                 __builtin_amdgcn_sched_barrier(0);
                 double g[NEN][D];
@@ -1872,7 +1885,7 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
     stage_lm<TC, BS>(p, 3, sLM);
     for (int k = tid; k < MAX_LDS_MATS * LeiCoef<D>::N; k += BS) {
         const int m = k / LeiCoef<D>::N, j = k % LeiCoef<D>::N;
-        const double v = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[LeiCoef<D>::index(j)];
+        const double v = lei_coef<D>(p, m < p.n_mat ? m : p.n_mat - 1, j);
         sLC[m][j] = m < p.n_mat ? v : 0.0;
     }
     if (tid < TW) reinterpret_cast<int32_t*>(&sT)[tid] = tw;
@@ -2067,7 +2080,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     stage_lm<TC, BS>(p, 6, sLM);
     for (int k = tid; k < MAX_LDS_MATS * LeiCoef<D>::N; k += BS) {
         const int m = k / LeiCoef<D>::N, j = k % LeiCoef<D>::N;
-        const double v = p.mats[m < p.n_mat ? m : p.n_mat - 1].D[LeiCoef<D>::index(j)];
+        const double v = lei_coef<D>(p, m < p.n_mat ? m : p.n_mat - 1, j);
         sLC[m][j] = m < p.n_mat ? v : 0.0;
     }
 #pragma unroll
@@ -2279,7 +2292,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     }
                 const int mi = tag & 7;
                 double lc[LeiCoef<D>::N];
-                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = p.mats[mi].D[LeiCoef<D>::index(k)];
+                for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = lei_coef<D>(p, mi, k);
                 // every load of the incidence is out before the arithmetic
                 __builtin_amdgcn_sched_barrier(0);
                 double g[NEN][D];
@@ -2463,6 +2476,11 @@ struct pg_asm {
                            // bits 5-6: 1 + option index of the group size of the groups after the cut kernels
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
+    // thermal pre-load: each material's expansion ratio (0 where not set) and
+    // the temperature change of the following assemblies
+    double thermal_exp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool thermal_set[8] = {false, false, false, false, false, false, false, false};
+    double temp_change = 0.0;
 };
 
 namespace {
@@ -2704,6 +2722,10 @@ int check_desc(const pg_problem_desc* s) {
         // binned kernels use the closed form of B_a^T D B_b in lambda = D[0][1]
         // and mu = D[last][last], so D must have the isotropic pattern
         if (s->mats[m].type != PG_MAT_LEI) continue;
+        if (s->mats[m].thermal_exp_set && !std::isfinite(s->mats[m].thermal_exp)) {
+            g_err = "thermal expansion ratio must be finite";
+            return PG_E_ARG;
+        }
         const int ns = s->dim == 2 ? 3 : 6, nd = s->dim;
         const double* Dm = s->mats[m].D;
         const double lam = Dm[1], mu = Dm[ns * ns - 1];
@@ -3067,6 +3089,11 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         return PG_E_UNSUPPORTED;
     }
     d.n_mat = s->n_mat;
+    for (int m = 0; m < s->n_mat; ++m) {
+        h->thermal_set[m] = s->mats[m].type == PG_MAT_LEI && s->mats[m].thermal_exp_set != 0;
+        h->thermal_exp[m] = h->thermal_set[m] ? s->mats[m].thermal_exp : 0.0;
+        d.eth[m] = 0.0;
+    }
     // incidence records in n2e order (host only; the kernels read them regrouped
     // per node group below), nen + 1 words, the last a tag.  Uncut
     // element: its nodes, tag -1 - material.  Cut element: the staged row of
@@ -3625,6 +3652,17 @@ int pg_asm_assemble(pg_asm* h, const double* U, const double* Ic, double* It, do
     HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (herr) { g_err = "negative child Jacobian in a cut element"; return PG_E_MESH; }
+    return PG_OK;
+}
+
+int pg_asm_set_temperature_change(pg_asm* h, double temp_change) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (!std::isfinite(temp_change)) { g_err = "temperature change must be finite"; return PG_E_ARG; }
+    h->temp_change = temp_change;
+    // LinearElasticIsotropicMaterial::Constitutive: strain_val = -1.0 *
+    // input.temp_change * _thermal_exp (src/material_lei.cpp:276)
+    for (int m = 0; m < h->d.n_mat; ++m)
+        h->d.eth[m] = h->thermal_set[m] ? -1.0 * temp_change * h->thermal_exp[m] : 0.0;
     return PG_OK;
 }
 

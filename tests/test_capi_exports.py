@@ -37,4 +37,4 @@ def test_ffi_signatures_cover_header():
 
 def test_struct_layouts():
     assert ctypes.sizeof(_ffi.PgCutRec) == 128
-    assert ctypes.sizeof(_ffi.PgMaterial) == 8 + 36 * 8 + 6 * 8
+    assert ctypes.sizeof(_ffi.PgMaterial) == 8 + 36 * 8 + 7 * 8 + 8
