@@ -239,6 +239,60 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
  * schedule and to calls made after it returns; PG_E_ARG if not finite. */
 int pg_asm_set_temperature_change(pg_asm* h, double temp_change);
 
+/* ----------------------------------------------------------------------------
+ * Material-parameter sensitivity: the partial derivative of the internal force
+ * with respect to one material parameter d, dP/dd, replacing
+ * SensitivityParameter::assemble_dPdd (src/SensitivityParameter.cpp:108-151)
+ * with SensitivityMaterialParameter::assemble_elem_dPdd
+ * (src/SensitivityMaterialParameter.cpp:54-264) and the structural kernels of
+ * SensitivityMaterialParameterStructural (src/SensitivityMaterialParameterStructural.cpp:17-121).
+ *
+ *   kind PG_SENS_LEI: a parameter of LEI material `mat` (E, lambda, mu, K or
+ *       nu); dD holds dD/dd (row-major nstr x nstr, the matrix of
+ *       LinearElasticIsotropicMaterial::computeDmatSensitivities,
+ *       src/material_lei.cpp:106-210, which the caller computes from E and nu).
+ *       Each qp of that material adds (J w) B^T (dD . B U) (SensitivityConstitutive,
+ *       src/material_lei.cpp:300-350); no thermal strain enters.
+ *   kind PG_SENS_SIGMA_C / PG_SENS_DELTA_C: sigma_c or delta_c of the
+ *       Ortiz-Pandolfi law `mat` (PG_MAT_OP or PG_MAT_OP_NU): each cohesive qp adds
+ *       the fold of (J w) -+N (R^T dt), dt the traction derivative of
+ *       OPCohesiveMaterial / OPCohesiveNUMaterial::SensitivityConstitutive
+ *       (src/Material_OPCohesive.cpp:104-180, src/Material_OPCohesiveNU.cpp:117-170).
+ * Cohesive qps of a PG_MAT_OP law that is not `mat` still add the
+ * history term of OPCohesiveMaterial (unloading: d delta_max / dd times the
+ * traction's derivative in delta_max), as the reference visits every element
+ * whose materials hold internal variables.
+ *
+ * As in the reference (src/SensitivityMaterialParameter.cpp:138-150), the qps of
+ * a cut element's child k take the B matrix of the element's qp numbered by the
+ * child-local qp index (the first children's B) while J and w are their own;
+ * flag PG_SENS_OWN_CHILD_B gives every child qp its own B instead (the
+ * integral the reference's formula states).
+ *
+ *   dU_all      global-dof-ordered solution (device)
+ *   disv        delta_max per cohesive qp (the committed internal variables,
+ *               device; may be NULL without PG_MAT_OP cohesive children)
+ *   disv_sens   d delta_max / dd per cohesive qp for this parameter (the
+ *               sensitivity internal variables, device; same rule)
+ *   ddPdd       out: [dP_f | dP_p], n_free + n_pres values, overwritten
+ * Asynchronous on stream (NULL: the handle's stream), ordered after the
+ * handle's previous call; uses the FP64 drop-in's staging buffers.
+ * PG_E_UNSUPPORTED where the reference stops with an error: a bilinear cohesive
+ * law on the mesh (its sensitivity is not implemented,
+ * src/Material_BilinearCohesive.cpp:131-134), a cohesive kind on a non-OP
+ * material, PG_SENS_LEI on a non-LEI one. */
+#define PG_SENS_LEI          0
+#define PG_SENS_SIGMA_C      1
+#define PG_SENS_DELTA_C      2
+#define PG_SENS_OWN_CHILD_B  1u
+typedef struct pg_sens_param {
+    int32_t mat;      /* index of the material differentiated against */
+    int32_t kind;     /* PG_SENS_* */
+    double  dD[36];   /* PG_SENS_LEI: dD/dd, row-major nstr x nstr */
+} pg_sens_param;
+int pg_asm_sensitivity_dPdd(pg_asm* h, const pg_sens_param* prm, unsigned flags, const double* dU_all,
+                            const double* disv, const double* disv_sens, double* ddPdd, void* stream);
+
 /* Status of the handle's last assembly (either API): waits for it to complete
  * and returns PG_E_MESH if a cut element had a child of negative Jacobian (the
  * reference aborts there, src/Mesh.cpp:1561), PG_OK otherwise.  The host API

@@ -600,6 +600,18 @@ static void lei(const pg_material* m, int dim, const double* e, double* s) {
 
 static double kron(int i, int j) { return i == j ? 1.0 : 0.0; }
 
+/* OPCohesiveNUMaterial::getDeltaEff (src/Material_OPCohesiveNU.cpp:43-61): an
+ * opening of squared norm below 1e-40 becomes 1e-20 in every component. */
+static double op_delta_eff(double* delta, int dim, double b2) {
+    double norm = 0;
+    for (int i = 0; i < dim; ++i) norm += delta[i] * delta[i];
+    if (norm < 1e-40)
+        for (int i = 0; i < dim; ++i) delta[i] = 1e-20;
+    double de = pow(delta[dim - 1], 2);
+    for (int i = 0; i < dim - 1; i++) de += b2 * pow(delta[i], 2);
+    return sqrt(de);
+}
+
 static void cohesive(const pg_material* m, int dim, double* delta, double* dmax_io, double* t, double Dm[3][3]) {
     double norm = 0;
     for (int i = 0; i < dim; ++i) norm += delta[i] * delta[i];
@@ -950,6 +962,187 @@ int oracle_assemble_range(oracle* o, const double* U, const double* isv_c, doubl
 }
 
 void oracle_set_temperature_change(oracle* o, double temp_change) { o->temp_change = temp_change; }
+
+/* ------------------------------------------------------------------------ */
+/* Material-parameter sensitivity dP/dd */
+
+/* OPCohesiveMaterial::SensitivityConstitutive (src/Material_OPCohesive.cpp:104-180)
+ * and OPCohesiveNUMaterial::SensitivityConstitutive (src/Material_OPCohesiveNU.cpp:117-170):
+ * the partial derivative of the traction, dt, at opening delta (ttn frame, modified
+ * in place by getDeltaEff as the reference does); right: this law is the one
+ * differentiated; dmax / ddmax: delta_max and its sensitivity (OP only). */
+static void cohesive_sens(const pg_material* m, int dim, double* delta, int right, int kind, double dmax,
+                          double ddmax, double* dt) {
+    for (int i = 0; i < dim; ++i) dt[i] = 0.0;
+    const double sc = m->sigma_c, dc = m->delta_c, b2 = m->beta2;
+    double br[3];
+    if (m->type == PG_MAT_OP) {
+        double de = op_delta_eff(delta, dim, b2);
+        for (int i = 0; i < dim; ++i) br[i] = delta[dim - 1];
+        for (int i = 0; i < dim - 1; ++i) br[i] = b2 * delta[i];
+        int loading = de >= dmax;
+        if (right) {
+            double dte = 0.0;
+            if (kind == PG_SENS_SIGMA_C)
+                dte = loading ? de / dc * exp(1 - de / dc) : de / dc * exp(1 - dmax / dc);
+            else if (kind == PG_SENS_DELTA_C)
+                dte = loading ? (sc * de / (dc * dc) * exp(1 - de / dc)) * (de / dc - 1)
+                              : (sc * de / (dc * dc) * exp(1 - dmax / dc)) * (dmax / dc - 1);
+            for (int i = 0; i < dim; ++i) dt[i] = (dte / de) * br[i];
+        }
+        if (!loading) {
+            double dtm = (sc * de / dc) * exp(1 - dmax / dc) * (-1.0 / dc);
+            for (int i = 0; i < dim; ++i) dt[i] += ddmax * (dtm / de) * br[i];
+        }
+    } else if (m->type == PG_MAT_OP_NU && right) {
+        double de = op_delta_eff(delta, dim, b2);
+        for (int i = 0; i < dim; ++i) br[i] = delta[dim - 1];
+        for (int i = 0; i < dim - 1; ++i) br[i] = b2 * delta[i];
+        double dte = 0.0;
+        if (kind == PG_SENS_SIGMA_C) dte = de / dc * exp(1 - de / dc);
+        else if (kind == PG_SENS_DELTA_C) dte = (sc * de / dc / dc * exp(1 - de / dc)) * (de / dc - 1);
+        for (int i = 0; i < dim; ++i) dt[i] = (dte / de) * br[i];
+    }
+}
+
+/* SensitivityParameter::assemble_dPdd (src/SensitivityParameter.cpp:108-151) with
+ * SensitivityMaterialParameter::assemble_elem_dPdd (src/SensitivityMaterialParameter.cpp:54-264)
+ * and the B-matrix kernels of SensitivityMaterialParameterStructural
+ * (src/SensitivityMaterialParameterStructural.cpp:40-121): dP = [dP_f | dP_p],
+ * overwritten.  Returns -2 where the reference stops with an error. */
+int oracle_sens_dPdd(oracle* o, const pg_sens_param* prm, unsigned flags, const double* U, const double* isv,
+                     const double* isv_sens, double* dP) {
+    const pg_problem_desc* d = o->d;
+    const int dim = d->dim, nstr = o->nstr, nen = d->nen;
+    const int tgt = prm->mat;
+    if (tgt < 0 || tgt >= d->n_mat) { strcpy(o->err, "bad material"); return -1; }
+    const pg_material* tm = &d->mats[tgt];
+    if (prm->kind == PG_SENS_LEI ? tm->type != PG_MAT_LEI
+                                 : !(tm->type == PG_MAT_OP || tm->type == PG_MAT_OP_NU)) {
+        strcpy(o->err, "parameter kind does not fit the material");
+        return -2;
+    }
+    memset(dP, 0, sizeof(double) * (size_t)(d->n_free + d->n_pres));
+    for (int e = o->e0; e < o->e1; ++e) {
+        int c = d->cut_of_elem[e];
+        const pg_cut_rec* r = c >= 0 ? &d->cut[c] : NULL;
+        /* _assemble_elem (set_material_element_assemblies) || elemHasISVs */
+        int visit = 0;
+        if (!r) visit = d->elem_mat[e] == tgt;
+        else {
+            for (int k = 0; k < r->n_child; ++k) visit |= r->child_mat[k] == tgt;
+            if (d->cohesive)
+                for (int k = 0; k < r->n_coh; ++k) {
+                    int t = d->mats[r->coh_mat[k]].type;
+                    visit |= r->coh_mat[k] == tgt || t == PG_MAT_OP || t == PG_MAT_BL;
+                }
+        }
+        if (!visit) continue;
+        int32_t dofs[MAXND];
+        double Ue[MAXND], dPe[MAXND], tmp[MAXND];
+        int nd = elem_dofs(d, e, dofs);
+        for (int i = 0; i < nd; ++i) Ue[i] = U[dofs[i]];
+        memset(dPe, 0, sizeof(double) * nd);
+        const double* cc = o->cache + o->off[e - o->e0];
+        const int vstride = 2 + nstr * nd;
+        /* one volumetric qp: strain = B U, dstress = dD strain, temp = B^T dstress */
+        #define SENS_VQP(Bq, wgt)                                                        \
+            do {                                                                         \
+                double eps_[6], ds_[6];                                                  \
+                for (int s_ = 0; s_ < nstr; ++s_) {                                      \
+                    double t_ = 0;                                                       \
+                    for (int j_ = 0; j_ < nd; ++j_) t_ += Ue[j_] * (Bq)[s_ * nd + j_];   \
+                    eps_[s_] = 0.0 + 1.0 * t_;                                           \
+                }                                                                        \
+                pg_material sm_ = *tm;                                                   \
+                memcpy(sm_.D, prm->dD, sizeof(sm_.D));                                   \
+                lei(&sm_, dim, eps_, ds_);                                               \
+                for (int i_ = 0; i_ < nd; ++i_) {                                        \
+                    double t_ = 0;                                                       \
+                    for (int k_ = 0; k_ < nstr; ++k_) t_ += (Bq)[k_ * nd + i_] * ds_[k_]; \
+                    tmp[i_] = t_;                                                        \
+                }                                                                        \
+                for (int i_ = 0; i_ < nd; ++i_) dPe[i_] += (wgt) * tmp[i_];              \
+            } while (0)
+        if (!r) {
+            if (d->elem_mat[e] == tgt)
+                for (int q = 0; q < (dim == 2 ? 4 : 5); ++q) {
+                    const double* q0 = cc + q * vstride;
+                    SENS_VQP(q0 + 2, q0[1] * q0[0]);
+                }
+        } else {
+            int cq = 0; /* curr_qp */
+            for (int k = 0; k < r->n_child; ++k) {
+                int nq = nqp_of(r->child_type[k]);
+                if (r->child_mat[k] != tgt) { cq += nq; continue; }
+                for (int q = 0; q < nq; ++q, ++cq) {
+                    /* getBmat(l_elem, qp): the child-local index (the reference's
+                     * call); J and W at curr_qp */
+                    const double* Bq = cc + (size_t)((flags & PG_SENS_OWN_CHILD_B) ? cq : q) * vstride + 2;
+                    const double* jw = cc + (size_t)cq * vstride;
+                    SENS_VQP(Bq, jw[1] * jw[0]);
+                }
+            }
+            if (d->cohesive) {
+                const double* ccq = cc + (size_t)r->n_vqp * vstride;
+                int cnn = r->coh_nn / 2, cnq = r->coh_nn == 4 ? 2 : 4, ncd = r->coh_nn * dim;
+                for (int k = 0; k < r->n_coh; ++k) {
+                    const pg_material* cm = &d->mats[r->coh_mat[k]];
+                    int right = r->coh_mat[k] == tgt;
+                    int has_isv = cm->type == PG_MAT_OP || cm->type == PG_MAT_BL;
+                    if (!(has_isv || right)) continue;
+                    if (cm->type == PG_MAT_BL || cm->type == PG_MAT_BL_NU) {
+                        strcpy(o->err, "Sensitivity is not implemented for the bilinear cohesive model");
+                        return -2;
+                    }
+                    double dPc[18], Uc[18];
+                    memset(dPc, 0, sizeof(double) * ncd);
+                    for (int n = 0; n < r->coh_nn; ++n)
+                        for (int j = 0; j < dim; ++j) Uc[n * dim + j] = Ue[r->coh_nodes[k][n] * dim + j];
+                    for (int q = 0; q < cnq; ++q) {
+                        const double* c0 = ccq + (size_t)(k * cnq + q) * (2 + 3 + dim * dim);
+                        const double W = c0[0], J = c0[1];
+                        const double* N = c0 + 2;
+                        double R[3][3];
+                        for (int i = 0; i < dim; ++i)
+                            for (int j = 0; j < dim; ++j) R[i][j] = c0[5 + i * dim + j];
+                        double dl[3] = {0, 0, 0}, del[3], dt[3], tR[3];
+                        for (int i = 0; i < dim; ++i)
+                            for (int n = 0; n < cnn; ++n) dl[i] += N[n] * (Uc[(n + cnn) * dim + i] - Uc[n * dim + i]);
+                        for (int i = 0; i < dim; ++i) {
+                            double t = 0;
+                            for (int j = 0; j < dim; ++j) t += R[i][j] * dl[j];
+                            del[i] = t;
+                        }
+                        int iq = r->isv_off + k * cnq + q;
+                        cohesive_sens(cm, dim, del, right, prm->kind, isv ? isv[iq] : 0.0,
+                                      isv_sens ? isv_sens[iq] : 0.0, dt);
+                        for (int i = 0; i < dim; ++i) {
+                            double t = 0;
+                            for (int l = 0; l < dim; ++l) t += R[l][i] * dt[l];
+                            tR[i] = t;
+                        }
+                        for (int n = 0; n < cnn; ++n)
+                            for (int dd = 0; dd < dim; ++dd) {
+                                double v = N[n] * tR[dd];
+                                tmp[n * dim + dd] = -v;
+                                tmp[(n + cnn) * dim + dd] = v;
+                            }
+                        for (int i = 0; i < ncd; ++i) dPc[i] += (J * W) * tmp[i];
+                    }
+                    for (int n1 = 0; n1 < r->coh_nn; ++n1)
+                        for (int d1 = 0; d1 < dim; ++d1) dPe[r->coh_nodes[k][n1] * dim + d1] += dPc[n1 * dim + d1];
+                }
+            }
+        }
+        #undef SENS_VQP
+        /* set_dP: nonzero entries added */
+        for (int i = 0; i < nd; ++i)
+            if (dPe[i] != 0.0) dP[dofs[i]] += dPe[i];
+    }
+    (void)nen;
+    return 0;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Test hooks: the oracle's constitutive routines, exposed so that tests can pin

@@ -87,6 +87,27 @@ class Oracle:
             raise RuntimeError("oracle_assemble: " + lib().oracle_error(self._h).decode())
         return K, P, it[: p.n_cqp]
 
+    def sens_dPdd(self, prm, U, isv=None, isv_sens=None, flags=0):
+        """dP/dd of one material parameter (prm: a PgSensParam), the oracle's
+        restatement of SensitivityParameter::assemble_dPdd.  Raises
+        NotImplementedError where the reference stops with an error."""
+        p = self.problem
+        L = lib()
+        L.oracle_sens_dPdd.restype = C.c_int
+        L.oracle_sens_dPdd.argtypes = [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        ic = np.ascontiguousarray(isv if isv is not None else np.zeros(max(p.n_cqp, 1)), dtype=np.float64)
+        iS = np.ascontiguousarray(isv_sens if isv_sens is not None else np.zeros(max(p.n_cqp, 1)), dtype=np.float64)
+        dP = np.zeros(p.n_dof)
+        rc = L.oracle_sens_dPdd(self._h, C.addressof(prm), int(flags), U.ctypes.data, ic.ctypes.data,
+                                iS.ctypes.data, dP.ctypes.data)
+        if rc == -2:
+            raise NotImplementedError(L.oracle_error(self._h).decode())
+        if rc != 0:
+            raise RuntimeError("oracle_sens_dPdd: " + L.oracle_error(self._h).decode())
+        return dP
+
     def close(self):
         if self._h:
             lib().oracle_free(self._h)

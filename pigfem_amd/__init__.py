@@ -9,8 +9,9 @@ from .problem import RVEConfig, RVEProblem, batch_config, config, small_config  
 from .assembler import AssemblerGPU, AssemblerGPUThermalPreLoad, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
     write_cohesive_failure_lengths  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
+from .sensitivity import SensitivityMaterialParameterGPU, lei_dmat_sensitivity  # noqa: F401
 
 __all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU",
            "AssemblerGPUThermalPreLoad", "PMatrix", "PVector",
            "cut_inclusions", "write_cohesive_openings", "write_cohesive_failure_lengths",
-           "DeviceBuffer", "synchronize", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]
+           "DeviceBuffer", "synchronize", "SensitivityMaterialParameterGPU", "lei_dmat_sensitivity", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

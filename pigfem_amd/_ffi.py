@@ -42,6 +42,14 @@ class PgMaterial(C.Structure):
     ]
 
 
+PG_SENS_LEI, PG_SENS_SIGMA_C, PG_SENS_DELTA_C = 0, 1, 2
+PG_SENS_OWN_CHILD_B = 1
+
+
+class PgSensParam(C.Structure):
+    _fields_ = [("mat", C.c_int32), ("kind", C.c_int32), ("dD", C.c_double * 36)]
+
+
 class PgCutRec(C.Structure):
     _fields_ = [
         ("enr", C.c_int32 * PG_MAX_ENR), ("isv_off", C.c_int32),
@@ -110,6 +118,8 @@ SIGNATURES = {
     "pg_asm_cohesive_failure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_check_error": (C.c_int, [C.c_void_p]),
     "pg_asm_set_temperature_change": (C.c_int, [C.c_void_p, C.c_double]),
+    "pg_asm_sensitivity_dPdd": (C.c_int, [C.c_void_p, C.POINTER(PgSensParam), C.c_uint, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg_asm_grid_nodes": (C.c_int64, [C.c_void_p]),
     "pg_csr_pcg": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                              C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p]),

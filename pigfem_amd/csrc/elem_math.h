@@ -315,4 +315,45 @@ __device__ inline void cohesive_law(const M& m, double* delta, double& isv, doub
     }
 }
 
+// Traction sensitivity of the Ortiz-Pandolfi laws: the partial derivative dt of
+// the traction with respect to sigma_c or delta_c (kind 1 / 2) at opening delta
+// (ttn frame; getDeltaEff's 1e-20 replacement applied in place).
+// OPCohesiveMaterial::SensitivityConstitutive (src/Material_OPCohesive.cpp:104-180):
+// the parameter's term where this law is the one differentiated (right), plus,
+// on unloading, the history term ddmax * d t / d delta_max for every OP law;
+// OPCohesiveNUMaterial::SensitivityConstitutive (src/Material_OPCohesiveNU.cpp:117-170):
+// the parameter's term only.
+template <int D, typename M>
+__device__ inline void cohesive_sens(const M& m, double* delta, bool right, int kind, double dmax, double ddmax,
+                                     double* dt) {
+    for (int i = 0; i < D; ++i) dt[i] = 0.0;
+    const bool op = m.type == PG_MAT_OP;
+    if (!op && !(m.type == PG_MAT_OP_NU && right)) return;
+    double norm = 0;
+    for (int i = 0; i < D; ++i) norm += delta[i] * delta[i];
+    if (norm < 1e-40)
+        for (int i = 0; i < D; ++i) delta[i] = 1e-20;
+    const double b2 = m.beta2, sc = m.sigma_c, dc = m.delta_c;
+    double de = delta[D - 1] * delta[D - 1];
+    for (int i = 0; i < D - 1; ++i) de += b2 * (delta[i] * delta[i]);
+    de = sqrt(de);
+    double br[D];
+    for (int i = 0; i < D; ++i) br[i] = i < D - 1 ? b2 * delta[i] : delta[D - 1];
+    const bool loading = !op || de >= dmax;  // the NU law has no unloading branch here
+    if (right) {
+        double dte = 0.0;
+        if (kind == 1)
+            dte = loading ? de / dc * exp(1 - de / dc) : de / dc * exp(1 - dmax / dc);
+        else if (kind == 2)
+            dte = op ? (loading ? (sc * de / (dc * dc) * exp(1 - de / dc)) * (de / dc - 1)
+                                : (sc * de / (dc * dc) * exp(1 - dmax / dc)) * (dmax / dc - 1))
+                     : (sc * de / dc / dc * exp(1 - de / dc)) * (de / dc - 1);
+        for (int i = 0; i < D; ++i) dt[i] = (dte / de) * br[i];
+    }
+    if (!loading) {
+        const double dtm = (sc * de / dc) * exp(1 - dmax / dc) * (-1.0 / dc);
+        for (int i = 0; i < D; ++i) dt[i] += ddmax * (dtm / de) * br[i];
+    }
+}
+
 }  // namespace pg

@@ -636,6 +636,241 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
 }
 
 // --------------------------------------------------------------------------
+// Material-parameter sensitivity dP/dd (SensitivityParameter::assemble_dPdd,
+// src/SensitivityParameter.cpp:108-151; SensitivityMaterialParameter::
+// assemble_elem_dPdd, src/SensitivityMaterialParameter.cpp:54-264).  The element
+// vectors go to the FP64 drop-in's staging buffers (uncut: PlainStage residual
+// entries; cut: cut_poff), and k_pair_staged's residual path sums them per node
+// in element order.
+// --------------------------------------------------------------------------
+struct SensDev {
+    int mat;       // material differentiated against
+    int kind;      // PG_SENS_*
+    int own_b;     // PG_SENS_OWN_CHILD_B
+    double dD[36]; // PG_SENS_LEI: dD/dd
+};
+
+// Uncut elements, one thread each: (J w_q) B^T (dD . B U) summed over the qps
+// for elements of the LEI material differentiated against
+// (SensitivityMaterialParameterStructural::KernelVolumetric, B variant,
+// src/SensitivityMaterialParameterStructural.cpp:40-60); 0 for the others,
+// which the reference does not visit.
+template <int D>
+__global__ __launch_bounds__(256) void k_sens_elem(DevProb p, const double* __restrict__ Un, SensDev s,
+                                                   double* __restrict__ stage) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5), ND = NEN * D;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.n_elem || p.cut_of_elem[e] >= 0) return;
+    if (s.kind != PG_SENS_LEI || p.elem_mat[e] != s.mat) {
+        for (int i = 0; i < ND; ++i) stage[PlainStage<D>::pvec(e, i / D) + i % D] = 0.0;
+        return;
+    }
+    double px[NEN][3], pdN[NEN][D], pg[NEN][3], u[ND];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) {
+        const int node = p.conn[int64_t(e) * NEN + n];
+        for (int i = 0; i < 3; ++i) px[n][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        for (int i = 0; i < D; ++i) u[n * D + i] = Un[int64_t(node) * D + i];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    const double J = transform<D>(NEN, pdN, px, pg);
+    double wq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
+    double eps[NS], ds[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) eps[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        double b[NS];
+        bcol<D, double>(pg[i / D], i % D, b);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) eps[k] += b[k] * u[i];
+    }
+    lei_stress<D>(s.dD, eps, ds);  // computeStress with the D sensitivity
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        double bi[NS];
+        bcol<D, double>(pg[i / D], i % D, bi);
+        double pt = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pt += bi[k] * ds[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
+        stage[PlainStage<D>::pvec(e, i / D) + i % D] = acc;
+    }
+}
+
+// Cut elements, one wavefront each.  Phase 1: one lane per child qp computes
+// the gradients of the local nodes and J w (as k_cut_plain does).  Phase 2: the
+// qp's strain and dstress = dD . B U where its child is of the material
+// differentiated against, with the B of the element's qp numbered by the
+// child-local index (getBmat(l_elem, qp), src/SensitivityMaterialParameter.cpp:
+// 138-150) unless own_b; one lane per cohesive qp of a visited cohesive child
+// (a law with internal variables, or the one differentiated) computes R^T dt.
+// Phase 3: one lane per element dof sums (J w) B^T dstress over the qps in
+// order, then folds each cohesive child's sum (:165-258).
+template <int D, bool POLY>
+__global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __restrict__ Un, SensDev s,
+                                                 const double* __restrict__ isv, const double* __restrict__ isvs,
+                                                 double* __restrict__ Ps, const int32_t* __restrict__ list) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6);
+    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q, MAXC = PG_MAX_COH * 4;
+    __shared__ double lx[MAXN][3];
+    __shared__ double ue[MAXN * D];
+    __shared__ double gq[MAXQ][MAXN][D];  // gradient of each local node at each volumetric qp
+    __shared__ double wq[MAXQ];            // J w of each volumetric qp
+    __shared__ double sq[MAXQ][NS];        // dstress at each volumetric qp (0 where not visited)
+    __shared__ int bq[MAXQ];               // the qp whose B each qp uses
+    __shared__ double cN[MAXC][3], cw[MAXC], ctR[MAXC][D];
+    __shared__ pg_cut_rec rec;
+    const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
+    constexpr int RW = int(sizeof(pg_cut_rec) / 4);
+    if (lane < RW) reinterpret_cast<int32_t*>(&rec)[lane] = reinterpret_cast<const int32_t*>(&p.cut[c])[lane];
+    const int e = p.cut_elem[c];
+    __syncthreads();
+    if constexpr (!POLY && D == 2) {
+        bool poly = false;
+        for (int k = 0; k < rec.n_child; ++k)
+            poly = poly || rec.child_type[k] == PG_CHILD_POLY5 || rec.child_type[k] == PG_CHILD_POLY6;
+        if (poly) return;  // the polygon launch takes it (the whole block returns)
+    }
+    const int nloc = NEN + rec.n_enr, nd = nloc * D;
+    if (lane < nloc) {
+        const int node = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
+        for (int i = 0; i < 3; ++i) lx[lane][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        for (int i = 0; i < D; ++i) ue[lane * D + i] = Un[int64_t(node) * D + i];
+    }
+    __syncthreads();
+    double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
+    for (int n = 0; n < NEN; ++n) {
+        for (int i = 0; i < 3; ++i) px[n][i] = lx[n][i];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    transform<D>(NEN, pdN, px, pgr);
+    const int nvq = rec.n_vqp;
+    for (int t = lane; t < nvq; t += 64) {
+        int k = 0, q = t;
+        for (; k < rec.n_child; ++k) {
+            const int nq = PG_CHILD_NQP(rec.child_type[k]);
+            if (q < nq) break;
+            q -= nq;
+        }
+        constexpr int CN = (D == 2 && POLY) ? 6 : 4;
+        const int ct = rec.child_type[k], ccn = PG_CHILD_NN(ct);
+        double cx[CN][3], cg[CN][3], w;
+        int cl[CN];
+        for (int n = 0; n < CN; ++n) {
+            cl[n] = n < ccn ? rec.child_node[PG_CHILD_STRIDE(D) * k + n] : -1;
+            for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? lx[cl[n]][i] : 0.0;
+        }
+        const double Jc = child_qp<D, POLY>(ct, q, ccn, cx, cg, w);
+        for (int l = 0; l < nloc; ++l) {
+            double gr[3] = {0.0, 0.0, 0.0};
+            if (l < NEN) {
+                for (int a = 0; a < D; ++a) gr[a] = pgr[l][a];
+            } else {
+                for (int n = 0; n < CN; ++n)
+                    if (cl[n] == l)
+                        for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
+            }
+            for (int a = 0; a < D; ++a) gq[t][l][a] = gr[a];
+        }
+        wq[t] = Jc * w;
+        bq[t] = s.own_b ? t : q;
+    }
+    __syncthreads();
+    for (int t = lane; t < nvq; t += 64) {
+        int k = 0, q = t;
+        for (; k < rec.n_child; ++k) {
+            const int nq = PG_CHILD_NQP(rec.child_type[k]);
+            if (q < nq) break;
+            q -= nq;
+        }
+        double ds[NS];
+        for (int s2 = 0; s2 < NS; ++s2) ds[s2] = 0.0;
+        if (s.kind == PG_SENS_LEI && rec.child_mat[k] == s.mat) {
+            const int tb = bq[t];
+            double eps[NS];
+            for (int s2 = 0; s2 < NS; ++s2) eps[s2] = 0.0;
+            for (int l = 0; l < nloc; ++l)
+                for (int cc = 0; cc < D; ++cc) {
+                    double b[NS];
+                    bcol<D, double>(gq[tb][l], cc, b);
+                    for (int s2 = 0; s2 < NS; ++s2) eps[s2] += b[s2] * ue[l * D + cc];
+                }
+            lei_stress<D>(s.dD, eps, ds);
+        }
+        for (int s2 = 0; s2 < NS; ++s2) sq[t][s2] = ds[s2];
+    }
+    const int cnn = rec.coh_nn / 2, cnq = rec.coh_nn == 4 ? 2 : 4;
+    const int ncq = p.cohesive ? rec.n_coh * cnq : 0;
+    for (int cq = lane; cq < ncq; cq += 64) {
+        const int k = cq / cnq, q = cq % cnq;
+        const pg_material& m = p.mats[rec.coh_mat[k]];
+        const bool right = rec.coh_mat[k] == s.mat;
+        const bool visit = right || m.type == PG_MAT_OP || m.type == PG_MAT_BL;
+        double lo[3][3], N[3], R[D][D], w, tR[D];
+        for (int n = 0; n < 3; ++n)
+            for (int i = 0; i < D; ++i) lo[n][i] = n < cnn ? lx[rec.coh_nodes[k][n]][i] : 0.0;
+        const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
+        for (int i = 0; i < D; ++i) tR[i] = 0.0;
+        if (visit) {
+            // cohesiveDeltaFast (src/Utilities.cpp:528-545)
+            double dl[D], del[D], dt[D];
+            for (int i = 0; i < D; ++i) {
+                double sm = 0;
+                for (int n = 0; n < cnn; ++n)
+                    sm += N[n] * (ue[rec.coh_nodes[k][n + cnn] * D + i] - ue[rec.coh_nodes[k][n] * D + i]);
+                dl[i] = sm;
+            }
+            for (int i = 0; i < D; ++i) {
+                double sm = 0;
+                for (int j = 0; j < D; ++j) sm += R[i][j] * dl[j];
+                del[i] = sm;
+            }
+            const int iq = rec.isv_off + cq;
+            cohesive_sens<D>(m, del, right, s.kind, isv ? isv[iq] : 0.0, isvs ? isvs[iq] : 0.0, dt);
+            for (int i = 0; i < D; ++i) {
+                double sm = 0;
+                for (int l = 0; l < D; ++l) sm += R[l][i] * dt[l];
+                tR[i] = sm;
+            }
+        }
+        for (int i = 0; i < D; ++i) ctR[cq][i] = tR[i];
+        for (int n = 0; n < 3; ++n) cN[cq][n] = N[n];
+        cw[cq] = dA * w;
+    }
+    __syncthreads();
+    double* Pe = Ps + p.cut_poff[c];
+    for (int i = lane; i < nd; i += 64) {
+        const int l = i / D, di = i % D;
+        double acc = 0.0;
+        for (int t = 0; t < nvq; ++t) {
+            double bi[NS];
+            bcol<D, double>(gq[bq[t]][l], di, bi);
+            double pt = 0.0;
+            for (int k = 0; k < NS; ++k) pt += bi[k] * sq[t][k];
+            acc += wq[t] * pt;
+        }
+        for (int k = 0; k < (p.cohesive ? int(rec.n_coh) : 0); ++k)
+            for (int m = 0; m < rec.coh_nn; ++m)
+                if (rec.coh_nodes[k][m] == l) {
+                    const int n = m % cnn;
+                    const double sgn = m < cnn ? -1.0 : 1.0;
+                    double pc = 0.0;
+                    for (int q = 0; q < cnq; ++q) {
+                        const int cq = k * cnq + q;
+                        pc += cw[cq] * (sgn * (cN[cq][n] * ctR[cq][di]));
+                    }
+                    acc += pc;
+                }
+        Pe[i] = acc;
+    }
+}
+
+// --------------------------------------------------------------------------
 // Cut-element kernel of the class-binned schedule (src/Mesh.cpp:1521-1641 +
 // src/Assembler.cpp:223-365).  EPW intersected elements share one wavefront, a
 // group of 64/EPW lanes each:
@@ -1225,6 +1460,8 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
     double pa[D];
     for (int i = 0; i < D; ++i) pa[i] = 0.0;
     const bool diag = (a == b) && (flags & PG_FUNC);
+    const bool jac = (flags & PG_JAC) != 0;
+    if (!jac && !diag) return;  // a residual-only call needs the diagonal pairs only
     const int e0 = p.n2e_ptr[a], e1 = p.n2e_ptr[a + 1];
     for (int t = e0; t < e1; ++t) {
         const int e = p.n2e[t];
@@ -1237,9 +1474,11 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
                 if (node == b) lb = n;
             }
             if (la < 0 || lb < 0) continue;
-            const double* Kb = stage + PlainStage<D>::kblk(e, la, lb);
-            for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) acc[i][j] += Kb[i * D + j];
+            if (jac) {
+                const double* Kb = stage + PlainStage<D>::kblk(e, la, lb);
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] += Kb[i * D + j];
+            }
             if (diag)
                 for (int i = 0; i < D; ++i) pa[i] += stage[PlainStage<D>::pvec(e, la) + i];
             continue;
@@ -1247,14 +1486,16 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
         const int la = find_local(p, e, ci, a, NEN);
         const int lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
         if (la < 0 || lb < 0) continue;
-        const int nd = (NEN + p.cut[ci].n_enr) * D;
-        const double* Ke = Ks + p.cut_koff[ci];
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) acc[i][j] += Ke[int64_t(la * D + i) * nd + lb * D + j];
+        if (jac) {
+            const int nd = (NEN + p.cut[ci].n_enr) * D;
+            const double* Ke = Ks + p.cut_koff[ci];
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j < D; ++j) acc[i][j] += Ke[int64_t(la * D + i) * nd + lb * D + j];
+        }
         if (diag)
             for (int i = 0; i < D; ++i) pa[i] += Ps[p.cut_poff[ci] + la * D + i];
     }
-    if (flags & PG_JAC) {
+    if (jac) {
         const int32_t* mp = p.npat_map + q * D * D;
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) K[mp[i * D + j]] = acc[i][j];
@@ -2481,6 +2722,8 @@ struct pg_asm {
     double thermal_exp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool thermal_set[8] = {false, false, false, false, false, false, false, false};
     double temp_change = 0.0;
+    int mat_type[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // PG_MAT_* of each material
+    unsigned coh_law_mask = 0;                   // bit PG_MAT_* set for each law of a cohesive child
 };
 
 namespace {
@@ -3093,7 +3336,11 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         h->thermal_set[m] = s->mats[m].type == PG_MAT_LEI && s->mats[m].thermal_exp_set != 0;
         h->thermal_exp[m] = h->thermal_set[m] ? s->mats[m].thermal_exp : 0.0;
         d.eth[m] = 0.0;
+        h->mat_type[m] = s->mats[m].type;
     }
+    if (s->cohesive)
+        for (int c = 0; c < s->n_cut; ++c)
+            for (int k = 0; k < s->cut[c].n_coh; ++k) h->coh_law_mask |= 1u << s->mats[s->cut[c].coh_mat[k]].type;
     // incidence records in n2e order (host only; the kernels read them regrouped
     // per node group below), nen + 1 words, the last a tag.  Uncut
     // element: its nodes, tag -1 - material.  Cut element: the staged row of
@@ -3664,6 +3911,67 @@ int pg_asm_set_temperature_change(pg_asm* h, double temp_change) {
     for (int m = 0; m < h->d.n_mat; ++m)
         h->d.eth[m] = h->thermal_set[m] ? -1.0 * temp_change * h->thermal_exp[m] : 0.0;
     return PG_OK;
+}
+
+int pg_asm_sensitivity_dPdd(pg_asm* h, const pg_sens_param* prm, unsigned flags, const double* dU,
+                            const double* disv, const double* disv_sens, double* ddPdd, void* stream) {
+    if (!h || !prm) { g_err = "null argument"; return PG_E_ARG; }
+    if (flags & ~PG_SENS_OWN_CHILD_B) { g_err = "unknown flag bits"; return PG_E_ARG; }
+    const DevProb& d = h->d;
+    if (prm->mat < 0 || prm->mat >= d.n_mat) { g_err = "material index out of range"; return PG_E_ARG; }
+    const int mt = h->mat_type[prm->mat];
+    if (prm->kind == PG_SENS_LEI) {
+        if (mt != PG_MAT_LEI) { g_err = "an LEI parameter needs an LEI material"; return PG_E_UNSUPPORTED; }
+        const int ns = d.dim == 2 ? 3 : 6;
+        for (int k = 0; k < ns * ns; ++k)
+            if (!std::isfinite(prm->dD[k])) { g_err = "dD/dd must be finite"; return PG_E_ARG; }
+    } else if (prm->kind == PG_SENS_SIGMA_C || prm->kind == PG_SENS_DELTA_C) {
+        if (mt == PG_MAT_BL || mt == PG_MAT_BL_NU) {
+            g_err = "Sensitivity is not implemented for the bilinear cohesive model";
+            return PG_E_UNSUPPORTED;
+        }
+        if (mt != PG_MAT_OP && mt != PG_MAT_OP_NU) {
+            g_err = "a cohesive parameter needs an Ortiz-Pandolfi material";
+            return PG_E_UNSUPPORTED;
+        }
+    } else {
+        g_err = "unknown parameter kind";
+        return PG_E_ARG;
+    }
+    if (h->coh_law_mask & (1u << PG_MAT_BL)) {
+        // every element with a bilinear cohesive child holds internal variables and is
+        // visited; the law's SensitivityConstitutive stops there
+        g_err = "Sensitivity is not implemented for the bilinear cohesive model";
+        return PG_E_UNSUPPORTED;
+    }
+    if (!dU || !ddPdd) { g_err = "missing device buffer"; return PG_E_ARG; }
+    if ((h->coh_law_mask & (1u << PG_MAT_OP)) && (!disv || !disv_sens)) {
+        g_err = "Ortiz-Pandolfi cohesive children need the internal variables and their sensitivities";
+        return PG_E_ARG;
+    }
+    SensDev sd{};
+    sd.mat = prm->mat;
+    sd.kind = prm->kind;
+    sd.own_b = (flags & PG_SENS_OWN_CHILD_B) ? 1 : 0;
+    for (int k = 0; k < 36; ++k) sd.dD[k] = prm->kind == PG_SENS_LEI ? prm->dD[k] : 0.0;
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const int64_t nu = int64_t(d.n_nodes + d.n_enr) * d.dim;
+    if (int rc = order_after_last(h, st)) return rc;
+    k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, d.node_dof, dU, h->Un);
+    const unsigned gb = unsigned((d.n_elem + 255) / 256);
+    if (d.dim == 2) {
+        k_sens_elem<2><<<gb, 256, 0, st>>>(d, h->Un, sd, h->stage);
+        if (d.n_cut) k_sens_cut<2, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, sd, disv, disv_sens, h->stage_p, nullptr);
+        if (h->n_poly) k_sens_cut<2, true><<<unsigned(h->n_poly), 64, 0, st>>>(d, h->Un, sd, disv, disv_sens, h->stage_p, h->poly_list);
+    } else {
+        k_sens_elem<3><<<gb, 256, 0, st>>>(d, h->Un, sd, h->stage);
+        if (d.n_cut) k_sens_cut<3, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Un, sd, disv, disv_sens, h->stage_p, nullptr);
+    }
+    const unsigned gp = unsigned((d.n_pairs + h->pair_block - 1) / h->pair_block);
+    if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, nullptr, ddPdd, PG_FUNC);
+    else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, nullptr, ddPdd, PG_FUNC);
+    HIP_OK(hipGetLastError());
+    return record_done(h, st);
 }
 
 int pg_asm_check_error(pg_asm* h) {
