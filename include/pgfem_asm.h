@@ -62,6 +62,9 @@ extern "C" {
 #define PG_MAT_BL_NU      2  /* BilinearCohesiveNU                           */
 #define PG_MAT_OP         3  /* OPCohesive (delta_max history)               */
 #define PG_MAT_OP_NU      4  /* OPCohesiveNU                                 */
+#define PG_MAT_LT         5  /* LinearThermalMaterial (material_lt.cpp:36-60):
+                                D = the dim x dim conductivity (kappa I),
+                                row-major with stride dim; nstr = dim        */
 
 typedef struct pg_material {
     int32_t type;       /* PG_MAT_*                                           */
@@ -157,7 +160,10 @@ typedef struct pg_problem_desc {
     int32_t n_free;
     int32_t n_pres;
     int32_t cohesive;       /* mesh has cohesive interfaces */
-    int32_t pad0;
+    int32_t nndof;          /* dofs per node: 0 (= dim, the structural assemblers
+                               and pg_asm_*) or 1 (a scalar field, the thermal
+                               assembler pg_therm_*); node_dof then holds nndof
+                               entries per node and npat_map nndof^2 per pair */
     int64_t nnz[4];         /* ff, fp, pf, pp */
     int64_t n_pairs;        /* length of npat_col */
     const double*   coords;     /* (n_nodes+n_enr)*dim */
@@ -238,6 +244,36 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
  * 0 (the default) is the plain structural assembler.  Applies to every
  * schedule and to calls made after it returns; PG_E_ARG if not finite. */
 int pg_asm_set_temperature_change(pg_asm* h, double temp_change);
+
+/* ----------------------------------------------------------------------------
+ * Heat conduction: AssemblerThermal (include/AssemblerThermal.h:16-45,
+ * src/AssemblerThermal.cpp:16-107), the sibling plug-in of the structural
+ * assembler behind the same Assembler::assemble.  One dof per node
+ * (desc->nndof == 1, ProblemLinearThermal::nndof, src/ProblemLinearThermal.cpp:24-28),
+ * the same mesh, cut records and IGFEM enrichment; PG_MAT_LT materials.  Per qp
+ * B = the shape gradients (dim x nodes, Assemble_Thermal_B_Mat,
+ * src/Utilities.cpp:147-160), K_el += (J w) (B^T D) B and P_el += (J w) B^T q,
+ * summed in the reference's order (Assembler::assemble_elem, FP64).
+ * LinearThermalMaterial::Constitutive returns D but never writes the flux
+ * (src/material_lt.cpp:51-60), so the reference's Bt_F reads an empty vector;
+ * here q = D . (B T), the flux of the linear law, which makes P = K T.
+ * Cohesive meshes are refused (PG_E_UNSUPPORTED): AssemblerThermal::KernelCohesive
+ * returns a K_coh smaller than the fold that reads it (:90-99).
+ *   pg_therm_create      upload the description (nndof == 1) to HBM
+ *   pg_therm_assemble_device  T (global-dof order), K [ff | fp | pf | pp], P
+ *                        [f | p], device pointers; PG_FUNC / PG_JAC as in
+ *                        pg_asm_assemble; asynchronous on stream (NULL: the
+ *                        handle's own), no allocation
+ *   pg_therm_assemble    the same on host buffers (K blocks, Pf, Pp as in
+ *                        pg_asm_assemble); synchronous
+ * Errors: pg_asm_last_error().  Results are bitwise repeatable. */
+typedef struct pg_therm pg_therm;
+int pg_therm_create(const pg_problem_desc* desc, pg_therm** out);
+int pg_therm_assemble_device(pg_therm* h, const double* dT_all, unsigned flags, double* dK, double* dP,
+                             void* stream);
+int pg_therm_assemble(pg_therm* h, const double* T_all, unsigned flags, double* Kff, double* Kfp, double* Kpf,
+                      double* Kpp, double* Pf, double* Pp);
+int pg_therm_destroy(pg_therm* h);
 
 /* ----------------------------------------------------------------------------
  * Material-parameter sensitivity: the partial derivative of the internal force

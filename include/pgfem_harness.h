@@ -46,7 +46,8 @@ typedef struct pgh_rve_params {
     double  sigma_c, delta_c, delta_c1, delta_c2, beta, alpha;
     double  strain;         /* applied u_x on RIGHT = strain * width            */
     int32_t max_tries;      /* rejection-sampling budget (0 = default)          */
-    int32_t flags;          /* PGH_PLACE_* | PGH_PATTERN_DEVICE                 */
+    int32_t flags;          /* PGH_PLACE_* | PGH_PATTERN_DEVICE | PGH_SCALAR_DOFS */
+    double  kappa_matrix, kappa_fiber;  /* PGH_SCALAR_DOFS: conductivities      */
 } pgh_rve_params;
 
 /* Stop placing inclusions when the rejection budget is spent and keep those
@@ -61,6 +62,11 @@ typedef struct pgh_rve_params {
  * of on the host; the result is identical.  For a batch, the flag is read from
  * params[0]. */
 #define PGH_PATTERN_DEVICE 4
+/* A scalar field (heat conduction, ProblemLinearThermal): one dof per node,
+ * LinearThermalMaterial materials (kappa_matrix, kappa_fiber), T prescribed on
+ * the LEFT (0) and RIGHT (strain) sides, perfect bond only (no cohesive law).
+ * The description then has nndof = 1 and is for pg_therm_*. */
+#define PGH_SCALAR_DOFS 16
 
 typedef struct pgh_problem pgh_problem;
 

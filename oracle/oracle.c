@@ -54,6 +54,7 @@ typedef struct oracle {
     const pg_problem_desc* d;
     int e0, e1;
     int nstr;
+    int nndof; /* dofs per node: dim (structural) or 1 (heat conduction) */
     /* own AIJ pattern per block, built by insertion (PETSc MatSetValue analogue) */
     int32_t nrows[4];
     row_t* rows[4];
@@ -239,6 +240,13 @@ static double transform(int dim, int nn, double dNdr[][3], double x[][3], double
     return J;
 }
 
+/* Assemble_Thermal_B_Mat (src/Utilities.cpp:147-160): B (dim x nn), B(j, n) =
+ * the j-th gradient component of node n */
+static void fill_B_thermal(int dim, int nn, double g[][3], double* B) {
+    for (int j = 0; j < dim; ++j)
+        for (int n = 0; n < nn; ++n) B[j * nn + n] = g[n][j];
+}
+
 /* Assemble_Small_Strain_B_Mat: B (nstr x dim*nn), row-major, zeros included */
 static void fill_B(int dim, int nn, double g[][3], double* B) {
     int nstr = dim == 2 ? 3 : 6, nc = dim * nn;
@@ -271,15 +279,16 @@ static void pt(const pg_problem_desc* d, int node, double* x) {
 }
 
 /* per-element cache size (doubles) */
-static int64_t cache_size(const pg_problem_desc* d, int e) {
-    int dim = d->dim, nstr = dim == 2 ? 3 : 6;
+static int64_t cache_size(const oracle* o, int e) {
+    const pg_problem_desc* d = o->d;
+    int dim = d->dim, nstr = o->nstr;
     int c = d->cut_of_elem[e];
     if (c < 0) {
         int nq = dim == 2 ? 4 : 5;
-        return (int64_t)nq * (2 + nstr * d->nen * dim);
+        return (int64_t)nq * (2 + nstr * d->nen * o->nndof);
     }
     const pg_cut_rec* r = &d->cut[c];
-    int nd = (d->nen + r->n_enr) * dim;
+    int nd = (d->nen + r->n_enr) * o->nndof;
     int ncq = r->n_coh * (r->coh_nn == 4 ? 2 : 4);
     return (int64_t)r->n_vqp * (2 + nstr * nd) + (int64_t)ncq * (2 + 3 + dim * dim);
 }
@@ -303,13 +312,14 @@ static int store_shapes(oracle* o, int e, double* out) {
             double J = transform(dim, nen, dN, x, g);
             if (J < -1e-16) { strcpy(o->err, "negative element Jacobian"); return -1; }
             out[0] = w; out[1] = J;
-            fill_B(dim, nen, g, out + 2);
-            out += 2 + nstr * nen * dim;
+            if (o->nndof == 1) fill_B_thermal(dim, nen, g, out + 2);
+            else fill_B(dim, nen, g, out + 2);
+            out += 2 + nstr * nen * o->nndof;
         }
         return 0;
     }
     const pg_cut_rec* r = &d->cut[c];
-    int nd = nl * dim;
+    int nd = nl * o->nndof;
     for (int k = 0; k < r->n_child; ++k) {
         int ct = r->child_type[k], cnn = nn_of(ct);
         const uint8_t* cl = &r->child_node[PG_CHILD_STRIDE(dim) * k];
@@ -378,7 +388,8 @@ static int store_shapes(oracle* o, int e, double* out) {
                         break;
                     }
             out[0] = w; out[1] = Jc;
-            fill_B(dim, nl, g, out + 2);
+            if (o->nndof == 1) fill_B_thermal(dim, nl, g, out + 2);
+            else fill_B(dim, nl, g, out + 2);
             out += 2 + nstr * nd;
         }
     }
@@ -457,9 +468,10 @@ static int row_insert(row_t* r, int32_t c) {
 static int elem_dofs(const pg_problem_desc* d, int e, int32_t* dofs) {
     int32_t ln[16];
     int nl = local_list(d, e, ln);
+    const int nn = d->nndof ? d->nndof : d->dim;
     for (int n = 0; n < nl; ++n)
-        for (int j = 0; j < d->dim; ++j) dofs[n * d->dim + j] = d->node_dof[(int64_t)ln[n] * d->dim + j];
-    return nl * d->dim;
+        for (int j = 0; j < nn; ++j) dofs[n * nn + j] = d->node_dof[(int64_t)ln[n] * nn + j];
+    return nl * nn;
 }
 
 static void oracle_free_rows(oracle* o) {
@@ -501,7 +513,8 @@ static oracle* oracle_create(const pg_problem_desc* d, int e0, int e1, int own_p
     o->d = d;
     o->e0 = e0 < 0 ? 0 : e0;
     o->e1 = (e1 < 0 || e1 > d->n_elem) ? d->n_elem : e1;
-    o->nstr = d->dim == 2 ? 3 : 6;
+    o->nndof = d->nndof ? d->nndof : d->dim;
+    o->nstr = o->nndof == 1 ? d->dim : (d->dim == 2 ? 3 : 6);
     o->own_pattern = own_pattern;
     int nf = d->n_free, np = d->n_pres;
     o->nrows[0] = nf; o->nrows[1] = nf; o->nrows[2] = np; o->nrows[3] = np;
@@ -554,7 +567,7 @@ cache:
     o->off = (int64_t*)malloc(sizeof(int64_t) * ((size_t)(o->e1 - o->e0) + 1));
     if (!o->off) { oracle_free(o); return NULL; }
     o->off[0] = 0;
-    for (int e = o->e0; e < o->e1; ++e) o->off[e - o->e0 + 1] = o->off[e - o->e0] + cache_size(d, e);
+    for (int e = o->e0; e < o->e1; ++e) o->off[e - o->e0 + 1] = o->off[e - o->e0] + cache_size(o, e);
     o->cache = (double*)malloc(sizeof(double) * (size_t)(o->off[o->e1 - o->e0] + 1));
     if (!o->cache) { oracle_free(o); return NULL; }
     /* elements fill disjoint cache slots, so they run in parallel; the first
@@ -739,7 +752,18 @@ static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int f
                 double strain_val = -1.0 * o->temp_change * m->thermal_exp;
                 for (int i = 0; i < dim; ++i) eps[i] = 1.0 * strain_val + eps[i];
             }
-            lei(m, dim, eps, sig);
+            if (o->nndof == 1) {
+                /* heat conduction: the flux q = D . grad T of the linear law (the
+                 * reference's LinearThermalMaterial leaves its output flux unset,
+                 * src/material_lt.cpp:51-60) */
+                for (int k = 0; k < nstr; ++k) {
+                    double t = 0;
+                    for (int l = 0; l < nstr; ++l) t += m->D[k * nstr + l] * eps[l];
+                    sig[k] = t;
+                }
+            } else {
+                lei(m, dim, eps, sig);
+            }
             if (jac) {
                 for (int i = 0; i < nd; ++i)
                     for (int l = 0; l < nstr; ++l) {

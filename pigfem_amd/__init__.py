@@ -6,12 +6,12 @@ description, and thin Python mirrors of the reference's Assembler API.
 """
 from ._ffi import PG_FUNC, PG_JAC, PG_FP32, PG_BATCHED, PgError, lib  # noqa: F401
 from .problem import RVEConfig, RVEProblem, batch_config, config, small_config  # noqa: F401
-from .assembler import AssemblerGPU, AssemblerGPUThermalPreLoad, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
+from .assembler import AssemblerGPU, AssemblerGPUThermal, AssemblerGPUThermalPreLoad, PMatrix, PVector, cut_inclusions, write_cohesive_openings, \
     write_cohesive_failure_lengths  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
 from .sensitivity import SensitivityMaterialParameterGPU, lei_dmat_sensitivity  # noqa: F401
 
 __all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU",
-           "AssemblerGPUThermalPreLoad", "PMatrix", "PVector",
+           "AssemblerGPUThermal", "AssemblerGPUThermalPreLoad", "PMatrix", "PVector",
            "cut_inclusions", "write_cohesive_openings", "write_cohesive_failure_lengths",
            "DeviceBuffer", "synchronize", "SensitivityMaterialParameterGPU", "lei_dmat_sensitivity", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

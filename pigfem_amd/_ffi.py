@@ -20,11 +20,12 @@ PG_CUT64 = 16
 
 PG_OK, PG_E_ARG, PG_E_HIP, PG_E_UNSUPPORTED, PG_E_MESH, PG_E_NOMEM = 0, -1, -2, -3, -4, -5
 
-PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU = range(5)
+PG_MAT_LEI, PG_MAT_BL, PG_MAT_BL_NU, PG_MAT_OP, PG_MAT_OP_NU, PG_MAT_LT = range(6)
 PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
 PGH_PLACE_BEST_EFFORT = 1
 PGH_PLACE_AT_NODES = 2
 PGH_PATTERN_DEVICE = 4
+PGH_SCALAR_DOFS = 16
 PGH_BATCH_COLS = 8
 PGH_N_STAGES = 7
 PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_TET4, PG_CHILD_POLY5, PG_CHILD_POLY6 = range(5)
@@ -74,7 +75,7 @@ class PgProblemDesc(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("nen", C.c_int32), ("n_nodes", C.c_int32), ("n_enr", C.c_int32),
         ("n_elem", C.c_int32), ("n_cut", C.c_int32), ("n_mat", C.c_int32), ("n_cqp", C.c_int32),
-        ("n_free", C.c_int32), ("n_pres", C.c_int32), ("cohesive", C.c_int32), ("pad0", C.c_int32),
+        ("n_free", C.c_int32), ("n_pres", C.c_int32), ("cohesive", C.c_int32), ("nndof", C.c_int32),
         ("nnz", C.c_int64 * 4), ("n_pairs", C.c_int64),
         ("coords", C.POINTER(C.c_double)), ("conn", C.POINTER(C.c_int32)),
         ("elem_mat", C.POINTER(C.c_int32)), ("cut_of_elem", C.POINTER(C.c_int32)),
@@ -95,6 +96,7 @@ class PghRveParams(C.Structure):
         ("sigma_c", C.c_double), ("delta_c", C.c_double), ("delta_c1", C.c_double), ("delta_c2", C.c_double),
         ("beta", C.c_double), ("alpha", C.c_double), ("strain", C.c_double),
         ("max_tries", C.c_int32), ("flags", C.c_int32),
+        ("kappa_matrix", C.c_double), ("kappa_fiber", C.c_double),
     ]
 
 
@@ -118,6 +120,11 @@ SIGNATURES = {
     "pg_asm_cohesive_failure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_check_error": (C.c_int, [C.c_void_p]),
     "pg_asm_set_temperature_change": (C.c_int, [C.c_void_p, C.c_double]),
+    "pg_therm_create": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
+    "pg_therm_assemble_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pg_therm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pg_therm_destroy": (C.c_int, [C.c_void_p]),
     "pg_asm_sensitivity_dPdd": (C.c_int, [C.c_void_p, C.POINTER(PgSensParam), C.c_uint, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg_asm_grid_nodes": (C.c_int64, [C.c_void_p]),

@@ -282,6 +282,58 @@ class AssemblerGPUThermalPreLoad(AssemblerGPU):
         self.set_temperature_change(0.0)
 
 
+class AssemblerGPUThermal:
+    """Drop-in for AssemblerThermal (include/AssemblerThermal.h:16-45,
+    src/AssemblerThermal.cpp): heat conduction on a scalar-field problem
+    (RVEConfig(thermal=True), one dof per node) through pg_therm_*.  The
+    Assembler::assemble signature is kept; there are no internal variables
+    (update_ISVs is ignored, as the thermal kernels have none).  P_int is
+    B^T (D grad T), i.e. K T for the linear law."""
+
+    def __init__(self, problem):
+        if problem.nndof != 1:
+            raise ValueError("the thermal assembler needs a scalar-field problem (RVEConfig(thermal=True))")
+        self.problem = problem
+        h = C.c_void_p()
+        check(lib().pg_therm_create(C.byref(problem.desc), C.byref(h)), "pg_therm_create")
+        self._h = h
+
+    def assemble(self, K: PMatrix, P_int: PVector, delta_t: float, update_ISVs, solution: np.ndarray,
+                 assembleFunc: bool = True, assembleJac: bool = True) -> int:
+        if not assembleFunc and not assembleJac:
+            return 0
+        p = self.problem
+        T = np.ascontiguousarray(solution, dtype=np.float64)
+        if T.shape != (p.n_dof,):
+            raise ValueError(f"solution must have {p.n_dof} entries")
+        Kb = [_ptr(K.block(b)) if assembleJac else None for b in range(4)]
+        Pb = [_ptr(P_int.f), _ptr(P_int.p)] if assembleFunc else [None, None]
+        flags = (PG_FUNC if assembleFunc else 0) | (PG_JAC if assembleJac else 0)
+        check(lib().pg_therm_assemble(self._h, _ptr(T), flags, *Kb, *Pb), "pg_therm_assemble")
+        return 0
+
+    def assemble_device(self, T, K, P, func=True, jac=True, stream=None) -> None:
+        """Device buffers (DeviceBuffer or raw addresses); asynchronous on stream."""
+        def dp(t):
+            if t is None:
+                return None
+            return t.ptr if hasattr(t, "ptr") else C.c_void_p(int(t))
+        flags = (PG_FUNC if func else 0) | (PG_JAC if jac else 0)
+        st = None if stream is None else C.c_void_p(int(stream))
+        check(lib().pg_therm_assemble_device(self._h, dp(T), flags, dp(K), dp(P), st), "pg_therm_assemble_device")
+
+    def close(self):
+        if self._h:
+            lib().pg_therm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def cut_inclusions(problem) -> np.ndarray:
     """Inclusion that cuts each cut record: the detection value of its parent
     nodes inside an inclusion (single-inclusion cuts; -1 if none is inside)."""

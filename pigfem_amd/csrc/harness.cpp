@@ -63,6 +63,7 @@ struct CutInfo {
 struct pgh_problem {
     pgh_rve_params prm{};
     int dim = 0, nen = 0;
+    int nndof = 0;                       // dofs per node: dim, or 1 for a scalar field
     int cohesive = 0;
     int32_t n_nodes = 0, n_enr = 0, n_elem = 0;
     std::vector<double> coords;          // (n_nodes + n_enr) * dim
@@ -616,6 +617,19 @@ void build_materials(pgh_problem& P) {
         }
         return m;
     };
+    if (P.nndof == 1) {
+        // LinearThermalMaterial::computeLinearDmat, src/material_lt.cpp:36-47
+        auto lt = [&](double kappa) {
+            pg_material m{};
+            m.type = PG_MAT_LT;
+            m.nstr = P.dim;
+            for (int i = 0; i < P.dim; ++i) m.D[i * P.dim + i] = kappa;
+            return m;
+        };
+        P.mats.push_back(lt(p.kappa_matrix));  // 0: matrix
+        P.mats.push_back(lt(p.kappa_fiber));   // 1: fiber
+        return;
+    }
     P.mats.push_back(lei(p.E_matrix, p.nu_matrix));  // 0: matrix
     P.mats.push_back(lei(p.E_fiber, p.nu_fiber));    // 1: fiber
     if (p.cohesive != PGH_COH_NONE) {
@@ -780,7 +794,7 @@ void analyze_and_enrich(pgh_problem& P) {
 // src/DofObject.cpp:336-388, 498-538).
 // --------------------------------------------------------------------------
 void number_dofs(pgh_problem& P) {
-    const int d = P.dim;
+    const int d = P.nndof;
     const int n = P.prm.n;
     const int32_t nall = P.n_nodes + P.n_enr;
     std::vector<int8_t> has_bc(size_t(nall) * d, 0);
@@ -826,7 +840,7 @@ inline int local_nodes(const pgh_problem& P, int32_t e, int32_t* out) {
 // included, src/Assembler.cpp:659-679) and the scatter indices.
 // --------------------------------------------------------------------------
 void build_pattern(pgh_problem& P) {
-    const int d = P.dim;
+    const int d = P.nndof;
     const int32_t nall = P.n_nodes + P.n_enr;
     int32_t ln[PG_MAX_ENR + 4];
     P.n2e_ptr.assign(size_t(nall) + 1, 0);
@@ -945,6 +959,7 @@ void build_pattern(pgh_problem& P) {
 void build_pattern_device(pgh_problem& P) {
     pgh::PatternIn in;
     in.dim = P.dim;
+    in.nndof = P.nndof;
     in.nen = P.nen;
     in.n_nodes = P.n_nodes;
     in.n_enr = P.n_enr;
@@ -988,7 +1003,7 @@ void timed(pgh_problem& P, int k, F&& f) {
     P.stage_s[k] += now_s() - t0;
 }
 
-constexpr int32_t kKnownFlags = PGH_PLACE_BEST_EFFORT | PGH_PLACE_AT_NODES | PGH_PATTERN_DEVICE;
+constexpr int32_t kKnownFlags = PGH_PLACE_BEST_EFFORT | PGH_PLACE_AT_NODES | PGH_PATTERN_DEVICE | PGH_SCALAR_DOFS;
 
 }  // namespace
 
@@ -1010,6 +1025,9 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
             fail(PG_E_ARG, "bad inclusion parameters");
         if (p->flags & ~kKnownFlags) fail(PG_E_ARG, "unknown flag bits");
         P->cohesive = p->cohesive != PGH_COH_NONE;
+        P->nndof = (p->flags & PGH_SCALAR_DOFS) ? 1 : p->dim;
+        if (P->nndof == 1 && P->cohesive)
+            fail(PG_E_UNSUPPORTED, "a scalar field takes no cohesive law (AssemblerThermal::KernelCohesive)");
         pgh_problem& Q = *P;
         timed(Q, 0, [&] { generate_mesh(Q); });
         timed(Q, 1, [&] { place_inclusions(Q); });
@@ -1039,9 +1057,10 @@ int pgh_build_pattern_flags(const pg_problem_desc* m, int32_t flags, pgh_problem
     if (!m || !out) { g_err = "null argument"; return PG_E_ARG; }
     *out = nullptr;
     if (flags & ~PGH_PATTERN_DEVICE) { g_err = "unknown flag bits"; return PG_E_ARG; }
+    const int nn = m->nndof ? m->nndof : m->dim;  // dofs per node
     if ((m->dim != 2 && m->dim != 3) || m->nen != m->dim + 1 || m->n_nodes <= 0 || m->n_elem <= 0 || m->n_enr < 0 ||
-        m->n_cut < 0 || m->n_mat <= 0 || m->n_free < 0 || m->n_pres < 0 ||
-        m->n_free + m->n_pres != (m->n_nodes + m->n_enr) * m->dim) {
+        m->n_cut < 0 || m->n_mat <= 0 || m->n_free < 0 || m->n_pres < 0 || (nn != 1 && nn != m->dim) ||
+        m->n_free + m->n_pres != (m->n_nodes + m->n_enr) * nn) {
         g_err = "inconsistent mesh description";
         return PG_E_ARG;
     }
@@ -1055,6 +1074,7 @@ int pgh_build_pattern_flags(const pg_problem_desc* m, int32_t flags, pgh_problem
     try {
         const int64_t nall = int64_t(m->n_nodes) + m->n_enr;
         P->dim = m->dim;
+        P->nndof = nn;
         P->nen = m->nen;
         P->cohesive = m->cohesive != 0;
         P->n_nodes = m->n_nodes;
@@ -1070,7 +1090,7 @@ int pgh_build_pattern_flags(const pg_problem_desc* m, int32_t flags, pgh_problem
         P->cut_elem.assign(m->cut_elem, m->cut_elem + m->n_cut);
         P->cut.assign(m->cut, m->cut + m->n_cut);
         P->mats.assign(m->mats, m->mats + m->n_mat);
-        P->node_dof.assign(m->node_dof, m->node_dof + nall * m->dim);
+        P->node_dof.assign(m->node_dof, m->node_dof + nall * nn);
         for (int64_t k = 0; k < int64_t(m->n_elem) * m->nen; ++k)
             if (P->conn[k] < 0 || P->conn[k] >= m->n_nodes) fail(PG_E_ARG, "connectivity out of range");
         for (int32_t c = 0; c < m->n_cut; ++c) {
@@ -1083,7 +1103,7 @@ int pgh_build_pattern_flags(const pg_problem_desc* m, int32_t flags, pgh_problem
         {
             // every dof belongs to exactly one (node, component)
             std::vector<uint8_t> seen(size_t(m->n_free) + m->n_pres, 0);
-            for (int64_t k = 0; k < nall * m->dim; ++k) {
+            for (int64_t k = 0; k < nall * nn; ++k) {
                 const int32_t g = P->node_dof[k];
                 if (g < 0 || g >= m->n_free + m->n_pres) fail(PG_E_ARG, "dof out of range");
                 if (seen[g]++) fail(PG_E_ARG, "dof numbered twice");
@@ -1122,7 +1142,9 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
                 const pgh_rve_params& p = params[i];
                 P->prm = p;
                 P->dim = p.dim;
+                P->nndof = p.dim;
                 if (p.dim != 2) fail(PG_E_UNSUPPORTED, "batches are 2D");
+                if (p.flags & PGH_SCALAR_DOFS) fail(PG_E_UNSUPPORTED, "batches are structural");
                 if (p.n < 1 || p.n_incl < 0 || p.r_min <= 0 || p.r_max < p.r_min || 2 * p.r_max >= 1.0)
                     fail(PG_E_ARG, "bad instance parameters");
                 P->cohesive = p.cohesive != PGH_COH_NONE;
@@ -1188,6 +1210,7 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
         if (T[0] >= (int64_t(1) << 31) || T[1] + T[2] >= (int64_t(1) << 31) || (T[3] + T[4]) >= (int64_t(1) << 31))
             fail(PG_E_UNSUPPORTED, "batch exceeds 32-bit indices");
         B->dim = 2;
+        B->nndof = 2;
         B->nen = 3;
         B->cohesive = inst[0]->cohesive;
         B->prm = inst[0]->prm;
@@ -1287,6 +1310,7 @@ int pgh_get_desc(const pgh_problem* h, pg_problem_desc* o) {
     o->n_free = h->n_free;
     o->n_pres = h->n_pres;
     o->cohesive = h->cohesive;
+    o->nndof = h->nndof == h->dim ? 0 : h->nndof;
     for (int b = 0; b < 4; ++b) {
         o->nnz[b] = int64_t(h->ci[b].size());
         o->rp[b] = h->rp[b].data();

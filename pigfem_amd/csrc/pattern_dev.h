@@ -12,11 +12,12 @@ namespace pgh {
 
 struct PatternIn {
     int dim = 0, nen = 0;
+    int nndof = 0;                         // dofs per node (dim, or 1 for a scalar field)
     int32_t n_nodes = 0, n_enr = 0, n_elem = 0, n_cut = 0, n_free = 0, n_pres = 0;
     const int32_t* conn = nullptr;         // n_elem * nen
     const int32_t* cut_of_elem = nullptr;  // n_elem
     const pg_cut_rec* cut = nullptr;       // n_cut
-    const int32_t* node_dof = nullptr;     // (n_nodes + n_enr) * dim
+    const int32_t* node_dof = nullptr;     // (n_nodes + n_enr) * nndof
 };
 
 struct PatternOut {

@@ -167,9 +167,12 @@ int down(std::vector<T>& v, const T* src, int64_t n, std::string& err) {
 namespace pgh {
 
 int build_pattern_device(const PatternIn& in, PatternOut& out, std::string& err) {
-    const int d = in.dim;
+    const int d = in.nndof ? in.nndof : in.dim;  // dofs per node
     const int64_t nall = int64_t(in.n_nodes) + in.n_enr, ndof = int64_t(in.n_free) + in.n_pres;
-    if (in.n_elem <= 0 || nall <= 0 || (d != 2 && d != 3)) { err = "empty or malformed mesh"; return PG_E_ARG; }
+    if (in.n_elem <= 0 || nall <= 0 || (in.dim != 2 && in.dim != 3) || (d != 1 && d != in.dim)) {
+        err = "empty or malformed mesh";
+        return PG_E_ARG;
+    }
     if (ndof >= (int64_t(1) << 31)) { err = "dofs exceed 2^31"; return PG_E_UNSUPPORTED; }
     Pool pool;
     // the mesh on the device

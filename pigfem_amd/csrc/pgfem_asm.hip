@@ -1505,6 +1505,252 @@ __global__ void k_pair_staged(DevProb p, const double* __restrict__ stage, const
 }
 
 // --------------------------------------------------------------------------
+// Heat conduction (AssemblerThermal, src/AssemblerThermal.cpp:24-85): one dof
+// per node, B = the shape gradients (Assemble_Thermal_B_Mat,
+// src/Utilities.cpp:147-160), per qp K_tmp = (B^T D) B and P_tmp = B^T q with
+// the flux q = D . B T, K_el += (J w) K_tmp, P_el += (J w) P_tmp (the
+// reference's order, src/Assembler.cpp:148-368), staged per element and summed
+// per node pair in element order as the FP64 drop-in does.
+// Staged layout of uncut elements: blocks of 64 elements; per local entry
+// (la, lb) the 64 elements' values, then per la the 64 residuals.
+// --------------------------------------------------------------------------
+template <int NEN> struct ThermStage {
+    static constexpr int64_t KB = int64_t(NEN) * NEN * 64;
+    static constexpr int64_t W = KB + int64_t(NEN) * 64;
+    __host__ __device__ static int64_t kval(int64_t e, int la, int lb) {
+        return (e >> 6) * W + int64_t(la * NEN + lb) * 64 + (e & 63);
+    }
+    __host__ __device__ static int64_t pval(int64_t e, int la) { return (e >> 6) * W + KB + int64_t(la) * 64 + (e & 63); }
+};
+
+// Uncut elements, one thread each.  Tn: T per node (node order).
+template <int D>
+__global__ __launch_bounds__(256) void k_therm_elem(DevProb p, const double* __restrict__ Tn,
+                                                    double* __restrict__ stage) {
+    constexpr int NEN = D + 1, NQ = (D == 2 ? 4 : 5);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.n_elem || p.cut_of_elem[e] >= 0) return;
+    double px[NEN][3], pdN[NEN][D], pg[NEN][3], t[NEN];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) {
+        const int node = p.conn[int64_t(e) * NEN + n];
+        for (int i = 0; i < 3; ++i) px[n][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        t[n] = Tn[node];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    const double J = transform<D>(NEN, pdN, px, pg);
+    const double* Dm = p.mats[p.elem_mat[e]].D;  // D x D, row-major
+    double Dl[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        for (int l = 0; l < D; ++l) Dl[k][l] = Dm[k * D + l];
+    double wq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) wq[q] = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
+    // grad T = B T (dgemv order: nodes ascending), flux q = D grad T
+    double gT[D], fl[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) v += pg[a][k] * t[a];
+        gT[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int l = 0; l < D; ++l) v += Dl[k][l] * gT[l];
+        fl[k] = v;
+    }
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) {
+        double btd[D];
+#pragma unroll
+        for (int l = 0; l < D; ++l) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) v += pg[a][k] * Dl[k][l];
+            btd[l] = v;
+        }
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) {
+            double kt = 0.0;
+#pragma unroll
+            for (int l = 0; l < D; ++l) kt += btd[l] * pg[b][l];
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc += wq[q] * kt;
+            stage[ThermStage<NEN>::kval(e, a, b)] = acc;
+        }
+        double pt = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pt += pg[a][k] * fl[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc += wq[q] * pt;
+        stage[ThermStage<NEN>::pval(e, a)] = acc;
+    }
+}
+
+// Cut elements, one wavefront each: one lane per child qp computes the
+// gradients of the local nodes (parent, and a child's own gradient for its
+// enrichment nodes, src/Mesh.cpp:1596-1614), J w and the flux; then one lane
+// per entry of K_el (and of P_el) sums its qps in order.  Ks / Ps: row-major
+// nloc x nloc matrices and nloc vectors at cut_koff / cut_poff.  POLY as in
+// k_cut_plain.
+template <int D, bool POLY>
+__global__ __launch_bounds__(64) void k_therm_cut(DevProb p, const double* __restrict__ Tn, double* __restrict__ Ks,
+                                                  double* __restrict__ Ps, int* __restrict__ err,
+                                                  const int32_t* __restrict__ list) {
+    constexpr int NEN = D + 1;
+    constexpr int MAXN = CutLim<D>::N, MAXQ = CutLim<D>::Q;
+    __shared__ double lx[MAXN][3];
+    __shared__ double te[MAXN];
+    __shared__ double gq[MAXQ][MAXN][D];
+    __shared__ double wq[MAXQ];
+    __shared__ double fq[MAXQ][D];   // flux at each qp
+    __shared__ int mq[MAXQ];
+    __shared__ pg_cut_rec rec;
+    const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
+    constexpr int RW = int(sizeof(pg_cut_rec) / 4);
+    if (lane < RW) reinterpret_cast<int32_t*>(&rec)[lane] = reinterpret_cast<const int32_t*>(&p.cut[c])[lane];
+    const int e = p.cut_elem[c];
+    __syncthreads();
+    if constexpr (!POLY && D == 2) {
+        bool poly = false;
+        for (int k = 0; k < rec.n_child; ++k)
+            poly = poly || rec.child_type[k] == PG_CHILD_POLY5 || rec.child_type[k] == PG_CHILD_POLY6;
+        if (poly) return;  // the polygon launch takes it (the whole block returns)
+    }
+    const int nloc = NEN + rec.n_enr;
+    if (lane < nloc) {
+        const int node = lane < NEN ? p.conn[int64_t(e) * NEN + lane] : rec.enr[lane - NEN];
+        for (int i = 0; i < 3; ++i) lx[lane][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        te[lane] = Tn[node];
+    }
+    __syncthreads();
+    double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
+    for (int n = 0; n < NEN; ++n) {
+        for (int i = 0; i < 3; ++i) px[n][i] = lx[n][i];
+        for (int a = 0; a < D; ++a) pdN[n][a] = simplex_dNdr(n, a);
+    }
+    transform<D>(NEN, pdN, px, pgr);
+    const int nvq = rec.n_vqp;
+    for (int t = lane; t < nvq; t += 64) {
+        int k = 0, q = t;
+        for (; k < rec.n_child; ++k) {
+            const int nq = PG_CHILD_NQP(rec.child_type[k]);
+            if (q < nq) break;
+            q -= nq;
+        }
+        constexpr int CN = (D == 2 && POLY) ? 6 : 4;
+        const int ct = rec.child_type[k], ccn = PG_CHILD_NN(ct);
+        double cx[CN][3], cg[CN][3], w;
+        int cl[CN];
+        for (int n = 0; n < CN; ++n) {
+            cl[n] = n < ccn ? rec.child_node[PG_CHILD_STRIDE(D) * k + n] : -1;
+            for (int i = 0; i < 3; ++i) cx[n][i] = (n < ccn && i < D) ? lx[cl[n]][i] : 0.0;
+        }
+        const double Jc = child_qp<D, POLY>(ct, q, ccn, cx, cg, w);
+        if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
+        double gT[D];
+        for (int a = 0; a < D; ++a) gT[a] = 0.0;
+        for (int l = 0; l < nloc; ++l) {
+            double gr[3] = {0.0, 0.0, 0.0};
+            if (l < NEN) {
+                for (int a = 0; a < D; ++a) gr[a] = pgr[l][a];
+            } else {
+                for (int n = 0; n < CN; ++n)
+                    if (cl[n] == l)
+                        for (int a = 0; a < D; ++a) gr[a] = cg[n][a];
+            }
+            for (int a = 0; a < D; ++a) {
+                gq[t][l][a] = gr[a];
+                gT[a] += gr[a] * te[l];
+            }
+        }
+        const double* Dm = p.mats[rec.child_mat[k]].D;
+        for (int a = 0; a < D; ++a) {
+            double v = 0.0;
+            for (int b = 0; b < D; ++b) v += Dm[a * D + b] * gT[b];
+            fq[t][a] = v;
+        }
+        wq[t] = Jc * w;
+        mq[t] = rec.child_mat[k];
+    }
+    __syncthreads();
+    double* Ke = Ks + p.cut_koff[c];
+    for (int idx = lane; idx < nloc * nloc; idx += 64) {
+        const int i = idx / nloc, j = idx % nloc;
+        double acc = 0.0;
+        for (int q = 0; q < nvq; ++q) {
+            const double* Dm = p.mats[mq[q]].D;
+            double kt = 0.0;
+            for (int l = 0; l < D; ++l) {
+                double btd = 0.0;
+                for (int k = 0; k < D; ++k) btd += gq[q][i][k] * Dm[k * D + l];
+                kt += btd * gq[q][j][l];
+            }
+            acc += wq[q] * kt;
+        }
+        Ke[idx] = acc;
+    }
+    double* Pe = Ps + p.cut_poff[c];
+    for (int i = lane; i < nloc; i += 64) {
+        double acc = 0.0;
+        for (int q = 0; q < nvq; ++q) {
+            double pt = 0.0;
+            for (int k = 0; k < D; ++k) pt += gq[q][i][k] * fq[q][k];
+            acc += wq[q] * pt;
+        }
+        Pe[i] = acc;
+    }
+}
+
+// Node pair (a, b) of the scalar pattern: the staged entries of every element
+// incident to a that contains b, summed in element order, written once through
+// the pair's scatter index; the diagonal pair also sums P (set_matrix /
+// set_vec, src/Assembler.cpp:659-701).
+template <int D>
+__global__ void k_therm_pair(DevProb p, const double* __restrict__ stage, const double* __restrict__ Ks,
+                             const double* __restrict__ Ps, double* __restrict__ K, double* __restrict__ P,
+                             unsigned flags) {
+    constexpr int NEN = D + 1;
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= p.n_pairs) return;
+    const int a = p.npat_row[q], b = p.npat_col[q];
+    const bool diag = (a == b) && (flags & PG_FUNC);
+    const bool jac = (flags & PG_JAC) != 0;
+    if (!jac && !diag) return;
+    double acc = 0.0, pa = 0.0;
+    for (int t = p.n2e_ptr[a]; t < p.n2e_ptr[a + 1]; ++t) {
+        const int e = p.n2e[t];
+        const int ci = p.cut_of_elem[e];
+        int la = -1, lb = -1;
+        if (ci < 0) {
+            for (int n = 0; n < NEN; ++n) {
+                const int node = p.conn[int64_t(e) * NEN + n];
+                if (node == a) la = n;
+                if (node == b) lb = n;
+            }
+            if (la < 0 || lb < 0) continue;
+            if (jac) acc += stage[ThermStage<NEN>::kval(e, la, lb)];
+            if (diag) pa += stage[ThermStage<NEN>::pval(e, la)];
+            continue;
+        }
+        la = find_local(p, e, ci, a, NEN);
+        lb = (a == b) ? la : find_local(p, e, ci, b, NEN);
+        if (la < 0 || lb < 0) continue;
+        const int nl = NEN + p.cut[ci].n_enr;
+        if (jac) acc += Ks[p.cut_koff[ci] + int64_t(la) * nl + lb];
+        if (diag) pa += Ps[p.cut_poff[ci] + la];
+    }
+    if (jac) K[p.npat_map[q]] = acc;
+    if (diag) P[p.node_dof[a]] = pa;
+}
+
+// --------------------------------------------------------------------------
 // Node-group kernels of the class-binned schedule.  A block owns NG nodes of a
 // group list (rows of the node-pair pattern).  Phase A takes one lane per
 // (node a, incident element e) incidence: for an uncut element it computes the
@@ -2949,6 +3195,10 @@ int check_desc(const pg_problem_desc* s) {
     if (!s) { g_err = "null description"; return PG_E_ARG; }
     if (s->dim != 2 && s->dim != 3) { g_err = "dim must be 2 or 3"; return PG_E_ARG; }
     if (s->nen != s->dim + 1) { g_err = "only tri3 (2D) and tet4 (3D) parents are supported"; return PG_E_UNSUPPORTED; }
+    if (s->nndof != 0 && s->nndof != s->dim) {
+        g_err = "a description with one dof per node is the thermal assembler's (pg_therm_create)";
+        return PG_E_UNSUPPORTED;
+    }
     if (s->n_nodes <= 0 || s->n_elem <= 0 || s->n_free + s->n_pres != (s->n_nodes + s->n_enr) * s->dim) {
         g_err = "inconsistent node / dof counts";
         return PG_E_ARG;
@@ -4179,6 +4429,273 @@ int pg_dev_dot(int64_t n, const double* x, const double* y, double* out) {
     k_dot_fin<<<dim3(1), dim3(PCG_T)>>>(part, nb, sc, 0, 1);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpy(out, sc, sizeof(double), hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// Heat-conduction handle (AssemblerThermal): the scalar field's description in
+// HBM, the FP64 element and node-pair kernels above.
+// --------------------------------------------------------------------------
+struct pg_therm {
+    DevProb d{};
+    int64_t n_dof = 0, nnz[4] = {0, 0, 0, 0}, nnz_total = 0;
+    std::vector<void*> allocs;
+    double* Tn = nullptr;       // T per node
+    double* stage = nullptr;    // uncut elements' K_el + P_el (ThermStage)
+    double* stage_k = nullptr;  // cut elements' K_el at cut_koff
+    double* stage_p = nullptr;  // cut elements' P_el at cut_poff
+    int* err = nullptr;
+    int32_t n_poly = 0;
+    int32_t* poly_list = nullptr;
+    double *dT = nullptr, *dK = nullptr, *dP = nullptr;  // host-API buffers
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int talloc(pg_therm* h, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIP_OK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+    h->allocs.push_back(*p);
+    return PG_OK;
+}
+
+int therm_launch(pg_therm* h, const double* dT, unsigned flags, double* dK, double* dP, hipStream_t st) {
+    const DevProb& d = h->d;
+    const int64_t nall = int64_t(d.n_nodes) + d.n_enr;
+    k_fill_int<<<1, 64, 0, st>>>(h->err, 0);
+    k_permute_u<<<dim3(unsigned((nall + 255) / 256)), dim3(256), 0, st>>>(nall, d.node_dof, dT, h->Tn);
+    const unsigned gb = unsigned((d.n_elem + 255) / 256);
+    if (d.dim == 2) {
+        k_therm_elem<2><<<gb, 256, 0, st>>>(d, h->Tn, h->stage);
+        if (d.n_cut) k_therm_cut<2, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Tn, h->stage_k, h->stage_p, h->err, nullptr);
+        if (h->n_poly) k_therm_cut<2, true><<<unsigned(h->n_poly), 64, 0, st>>>(d, h->Tn, h->stage_k, h->stage_p, h->err, h->poly_list);
+    } else {
+        k_therm_elem<3><<<gb, 256, 0, st>>>(d, h->Tn, h->stage);
+        if (d.n_cut) k_therm_cut<3, false><<<unsigned(d.n_cut), 64, 0, st>>>(d, h->Tn, h->stage_k, h->stage_p, h->err, nullptr);
+    }
+    const unsigned gp = unsigned((d.n_pairs + 255) / 256);
+    if (d.dim == 2) k_therm_pair<2><<<gp, 256, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, dK, dP, flags);
+    else k_therm_pair<3><<<gp, 256, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, dK, dP, flags);
+    HIP_OK(hipGetLastError());
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_therm_destroy(pg_therm* h) {
+    if (!h) return PG_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : {(void*)h->dT, (void*)h->dK, (void*)h->dP})
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PG_OK;
+}
+
+int pg_therm_create(const pg_problem_desc* s, pg_therm** out) {
+    if (!out) { g_err = "null output handle"; return PG_E_ARG; }
+    *out = nullptr;
+    if (!s) { g_err = "null description"; return PG_E_ARG; }
+    if (s->nndof != 1) { g_err = "the thermal assembler takes one dof per node (nndof = 1)"; return PG_E_ARG; }
+    if (s->dim != 2 && s->dim != 3) { g_err = "dim must be 2 or 3"; return PG_E_ARG; }
+    if (s->nen != s->dim + 1) { g_err = "only tri3 (2D) and tet4 (3D) parents are supported"; return PG_E_UNSUPPORTED; }
+    if (s->cohesive) {
+        g_err = "cohesive meshes are not supported by the thermal assembler (AssemblerThermal::KernelCohesive)";
+        return PG_E_UNSUPPORTED;
+    }
+    const int64_t nall = int64_t(s->n_nodes) + s->n_enr;
+    if (s->n_nodes <= 0 || s->n_elem <= 0 || s->n_enr < 0 || s->n_cut < 0 || int64_t(s->n_free) + s->n_pres != nall) {
+        g_err = "inconsistent node / dof counts";
+        return PG_E_ARG;
+    }
+    if (!s->coords || !s->conn || !s->elem_mat || !s->cut_of_elem || !s->mats || !s->node_dof || !s->npat_row ||
+        !s->npat_col || !s->npat_map || !s->n2e_ptr || !s->n2e || (s->n_cut > 0 && (!s->cut || !s->cut_elem))) {
+        g_err = "missing array in description";
+        return PG_E_ARG;
+    }
+    if (s->n_mat <= 0 || s->n_mat > MAX_LDS_MATS) { g_err = "1 to 8 materials are supported"; return PG_E_UNSUPPORTED; }
+    for (int m = 0; m < s->n_mat; ++m)
+        if (s->mats[m].type != PG_MAT_LT || s->mats[m].nstr != s->dim) {
+            g_err = "the thermal assembler takes LinearThermalMaterial (PG_MAT_LT, nstr = dim) materials";
+            return PG_E_ARG;
+        }
+    for (int32_t e = 0; e < s->n_elem; ++e) {
+        if (s->elem_mat[e] < 0 || s->elem_mat[e] >= s->n_mat) { g_err = "element material out of range"; return PG_E_ARG; }
+        if (s->cut_of_elem[e] >= s->n_cut) { g_err = "cut index out of range"; return PG_E_ARG; }
+        for (int n = 0; n < s->nen; ++n) {
+            const int32_t v = s->conn[int64_t(e) * s->nen + n];
+            if (v < 0 || v >= s->n_nodes) { g_err = "connectivity out of range"; return PG_E_ARG; }
+        }
+    }
+    std::vector<int64_t> koff(size_t(s->n_cut) + 1, 0), poff(size_t(s->n_cut) + 1, 0);
+    for (int32_t c = 0; c < s->n_cut; ++c) {
+        const pg_cut_rec& r = s->cut[c];
+        if (s->cut_elem[c] < 0 || s->cut_elem[c] >= s->n_elem || r.n_enr > PG_MAX_ENR || r.n_coh != 0 ||
+            r.n_child == 0 || r.n_child > (s->dim == 2 ? 4 : PG_MAX_CHILD)) {
+            g_err = "bad cut record";
+            return PG_E_ARG;
+        }
+        int nq = 0;
+        for (int k = 0; k < r.n_child; ++k) {
+            const int t = r.child_type[k];
+            const bool ok = s->dim == 2 ? (t == PG_CHILD_TRI3 || t == PG_CHILD_QUAD4 || t == PG_CHILD_POLY5 ||
+                                           t == PG_CHILD_POLY6)
+                                        : t == PG_CHILD_TET4;
+            if (!ok || r.child_mat[k] >= s->n_mat) { g_err = "bad child type or material"; return PG_E_ARG; }
+            for (int n = 0; n < PG_CHILD_NN(t); ++n)
+                if (r.child_node[PG_CHILD_STRIDE(s->dim) * k + n] >= s->nen + r.n_enr) {
+                    g_err = "child node out of range";
+                    return PG_E_ARG;
+                }
+            nq += PG_CHILD_NQP(t);
+        }
+        if (nq != r.n_vqp || nq > (s->dim == 2 ? MAXQ2 : MAXQ3)) { g_err = "cut record qp count"; return PG_E_ARG; }
+        for (int k = 0; k < r.n_enr; ++k)
+            if (r.enr[k] < s->n_nodes || r.enr[k] >= nall) { g_err = "enrichment node out of range"; return PG_E_ARG; }
+        const int nl = s->nen + r.n_enr;
+        koff[c + 1] = koff[c] + int64_t(nl) * nl;
+        poff[c + 1] = poff[c] + nl;
+    }
+    for (int64_t k = 0; k < nall; ++k)
+        if (s->node_dof[k] < 0 || s->node_dof[k] >= s->n_free + s->n_pres) { g_err = "dof out of range"; return PG_E_ARG; }
+    int64_t nnz_tot = 0;
+    for (int b = 0; b < 4; ++b) nnz_tot += s->nnz[b];
+    for (int64_t q = 0; q < s->n_pairs; ++q) {
+        if (s->npat_row[q] < 0 || s->npat_row[q] >= nall || s->npat_col[q] < 0 || s->npat_col[q] >= nall ||
+            s->npat_map[q] < 0 || s->npat_map[q] >= nnz_tot) {
+            g_err = "node-pair pattern out of range";
+            return PG_E_ARG;
+        }
+    }
+    if (s->n2e_ptr[0] != 0) { g_err = "incidence pointer"; return PG_E_ARG; }
+    for (int64_t a = 0; a < nall; ++a)
+        if (s->n2e_ptr[a + 1] < s->n2e_ptr[a]) { g_err = "incidence pointer"; return PG_E_ARG; }
+    for (int64_t k = 0; k < s->n2e_ptr[nall]; ++k)
+        if (s->n2e[k] < 0 || s->n2e[k] >= s->n_elem) { g_err = "incidence out of range"; return PG_E_ARG; }
+
+    pg_therm* h = new (std::nothrow) pg_therm();
+    if (!h) { g_err = "out of memory"; return PG_E_NOMEM; }
+    auto fail = [&](int code) { pg_therm_destroy(h); return code; };
+    DevProb& d = h->d;
+    d.dim = s->dim; d.nen = s->nen; d.n_nodes = s->n_nodes; d.n_enr = s->n_enr; d.n_elem = s->n_elem;
+    d.n_cut = s->n_cut; d.n_free = s->n_free; d.n_pres = s->n_pres; d.cohesive = 0; d.n_cqp = 0;
+    d.n_pairs = s->n_pairs;
+    d.n_mat = s->n_mat;
+    h->n_dof = int64_t(s->n_free) + s->n_pres;
+    for (int b = 0; b < 4; ++b) {
+        h->nnz[b] = s->nnz[b];
+        h->nnz_total += s->nnz[b];
+    }
+    int rc;
+#define TUP(field, src, n, T)                                   \
+    do {                                                        \
+        T* tmp_;                                                \
+        if ((rc = talloc(h, &tmp_, size_t(n)))) return fail(rc); \
+        if (size_t(n) > 0 && hipMemcpy(tmp_, src, size_t(n) * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) { \
+            g_err = "hipMemcpy of " #field " failed"; return fail(PG_E_HIP); } \
+        d.field = tmp_;                                         \
+    } while (0)
+    TUP(coords, s->coords, nall * s->dim, double);
+    TUP(conn, s->conn, int64_t(s->n_elem) * s->nen, int32_t);
+    TUP(elem_mat, s->elem_mat, s->n_elem, int32_t);
+    TUP(cut_of_elem, s->cut_of_elem, s->n_elem, int32_t);
+    TUP(cut_elem, s->cut_elem, s->n_cut, int32_t);
+    TUP(cut, s->cut, s->n_cut, pg_cut_rec);
+    TUP(mats, s->mats, s->n_mat, pg_material);
+    TUP(node_dof, s->node_dof, nall, int32_t);
+    TUP(npat_row, s->npat_row, s->n_pairs, int32_t);
+    TUP(npat_col, s->npat_col, s->n_pairs, int32_t);
+    TUP(npat_map, s->npat_map, s->n_pairs, int32_t);
+    TUP(n2e_ptr, s->n2e_ptr, nall + 1, int32_t);
+    TUP(n2e, s->n2e, s->n2e_ptr[nall], int32_t);
+    TUP(cut_koff, koff.data(), s->n_cut + 1, int64_t);
+    TUP(cut_poff, poff.data(), s->n_cut + 1, int64_t);
+#undef TUP
+    {
+        std::vector<int32_t> pl;
+        for (int32_t c = 0; c < s->n_cut; ++c)
+            for (int k = 0; k < s->cut[c].n_child; ++k)
+                if (s->cut[c].child_type[k] == PG_CHILD_POLY5 || s->cut[c].child_type[k] == PG_CHILD_POLY6) {
+                    pl.push_back(c);
+                    break;
+                }
+        h->n_poly = int32_t(pl.size());
+        if (!pl.empty()) {
+            if ((rc = talloc(h, &h->poly_list, pl.size()))) return fail(rc);
+            if (hipMemcpy(h->poly_list, pl.data(), pl.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                g_err = "hipMemcpy of the polygon list failed";
+                return fail(PG_E_HIP);
+            }
+        }
+    }
+    const int64_t nblk = (int64_t(s->n_elem) + 63) / 64;
+    const int64_t W = s->dim == 2 ? ThermStage<3>::W : ThermStage<4>::W;
+    if ((rc = talloc(h, &h->Tn, size_t(nall)))) return fail(rc);
+    if ((rc = talloc(h, &h->stage, size_t(nblk * W)))) return fail(rc);
+    if ((rc = talloc(h, &h->stage_k, size_t(koff.back())))) return fail(rc);
+    if ((rc = talloc(h, &h->stage_p, size_t(poff.back())))) return fail(rc);
+    if ((rc = talloc(h, &h->err, 1))) return fail(rc);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "hipStreamCreate failed";
+        return fail(PG_E_HIP);
+    }
+    *out = h;
+    return PG_OK;
+}
+
+int pg_therm_assemble_device(pg_therm* h, const double* dT, unsigned flags, double* dK, double* dP, void* stream) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (flags & ~(PG_FUNC | PG_JAC)) { g_err = "the thermal assembler takes PG_FUNC and PG_JAC only"; return PG_E_ARG; }
+    if (!(flags & (PG_FUNC | PG_JAC))) return PG_OK;
+    if (!dT || ((flags & PG_JAC) && !dK) || ((flags & PG_FUNC) && !dP)) { g_err = "missing device buffer"; return PG_E_ARG; }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    return therm_launch(h, dT, flags, dK, dP, st);
+}
+
+int pg_therm_assemble(pg_therm* h, const double* T, unsigned flags, double* Kff, double* Kfp, double* Kpf,
+                      double* Kpp, double* Pf, double* Pp) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    if (flags & ~(PG_FUNC | PG_JAC)) { g_err = "the thermal assembler takes PG_FUNC and PG_JAC only"; return PG_E_ARG; }
+    if (!(flags & (PG_FUNC | PG_JAC))) return PG_OK;
+    double* Kb[4] = {Kff, Kfp, Kpf, Kpp};
+    bool missing = !T;
+    if (flags & PG_JAC)
+        for (int b = 0; b < 4; ++b) missing |= h->nnz[b] > 0 && !Kb[b];
+    if (flags & PG_FUNC) missing |= (h->d.n_free > 0 && !Pf) || (h->d.n_pres > 0 && !Pp);
+    if (missing) { g_err = "missing host buffer"; return PG_E_ARG; }
+    if (!h->dT) {
+        HIP_OK(hipMalloc(&h->dT, size_t(h->n_dof) * sizeof(double)));
+        HIP_OK(hipMalloc(&h->dK, size_t(std::max<int64_t>(h->nnz_total, 1)) * sizeof(double)));
+        HIP_OK(hipMalloc(&h->dP, size_t(std::max<int64_t>(h->n_dof, 1)) * sizeof(double)));
+    }
+    hipStream_t st = h->stream;
+    HIP_OK(hipMemcpyAsync(h->dT, T, size_t(h->n_dof) * sizeof(double), hipMemcpyHostToDevice, st));
+    if (int rc = therm_launch(h, h->dT, flags, h->dK, h->dP, st)) return rc;
+    int herr = 0;
+    HIP_OK(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (flags & PG_JAC) {
+        int64_t off = 0;
+        for (int b = 0; b < 4; ++b) {
+            if (h->nnz[b])
+                HIP_OK(hipMemcpyAsync(Kb[b], h->dK + off, size_t(h->nnz[b]) * sizeof(double), hipMemcpyDeviceToHost, st));
+            off += h->nnz[b];
+        }
+    }
+    if (flags & PG_FUNC) {
+        if (h->d.n_free) HIP_OK(hipMemcpyAsync(Pf, h->dP, size_t(h->d.n_free) * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (h->d.n_pres)
+            HIP_OK(hipMemcpyAsync(Pp, h->dP + h->d.n_free, size_t(h->d.n_pres) * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    if (herr & 1) { g_err = "negative child Jacobian in a cut element"; return PG_E_MESH; }
     return PG_OK;
 }
 

@@ -43,6 +43,9 @@ class RVEConfig:
     alpha: float = 50.0
     strain: float = 0.02
     place: int = 0               # PGH_PLACE_* flags of the inclusion placement
+    thermal: bool = False        # a scalar field (heat conduction): one dof per node
+    kappa_matrix: float = 0.2    # conductivities of the thermal problem
+    kappa_fiber: float = 1.0
     name: str = "rve"
 
     def params(self) -> PghRveParams:
@@ -60,7 +63,8 @@ class RVEConfig:
         p.delta_c2 = self.delta_c
         p.beta, p.alpha, p.strain = self.beta, self.alpha, self.strain
         p.max_tries = 0
-        p.flags = self.place
+        p.flags = self.place | (_ffi.PGH_SCALAR_DOFS if self.thermal else 0)
+        p.kappa_matrix, p.kappa_fiber = self.kappa_matrix, self.kappa_fiber
         return p
 
 
@@ -162,6 +166,7 @@ class RVEProblem:
         check(L.pgh_get_desc(h, C.byref(self.desc)), "pgh_get_desc")
         d = self.desc
         self.dim, self.nen = d.dim, d.nen
+        self.nndof = d.nndof if d.nndof else d.dim   # dofs per node
         self.n_nodes, self.n_enr, self.n_elem, self.n_cut = d.n_nodes, d.n_enr, d.n_elem, d.n_cut
         self.n_free, self.n_pres, self.n_cqp = d.n_free, d.n_pres, d.n_cqp
         self.n_dof = self.n_free + self.n_pres
@@ -173,7 +178,7 @@ class RVEProblem:
         self.elem_mat = _arr(d.elem_mat, self.n_elem, np.int32)
         self.cut_of_elem = _arr(d.cut_of_elem, self.n_elem, np.int32)
         self.cut_elem = _arr(d.cut_elem, self.n_cut, np.int32)
-        self.node_dof = _arr(d.node_dof, nall * self.dim, np.int32).reshape(nall, self.dim)
+        self.node_dof = _arr(d.node_dof, nall * self.nndof, np.int32).reshape(nall, self.nndof)
         rows = [self.n_free, self.n_free, self.n_pres, self.n_pres]
         self.rp = [_arr(d.rp[b], rows[b] + 1, np.int32) for b in range(4)]
         self.ci = [_arr(d.ci[b], self.nnz[b], np.int32) for b in range(4)]
@@ -181,7 +186,7 @@ class RVEProblem:
         self.npat_ptr = _arr(d.npat_ptr, nall + 1, np.int32)
         self.npat_col = _arr(d.npat_col, self.n_pairs, np.int32)
         self.npat_row = _arr(d.npat_row, self.n_pairs, np.int32)
-        self.npat_map = _arr(d.npat_map, self.n_pairs * self.dim * self.dim, np.int32)
+        self.npat_map = _arr(d.npat_map, self.n_pairs * self.nndof * self.nndof, np.int32)
         self.n2e_ptr = _arr(d.n2e_ptr, nall + 1, np.int32)
         self.n2e = _arr(d.n2e, int(self.n2e_ptr[-1]), np.int32)
         self.node_detect = _arr(L.pgh_node_detect(h), self.n_nodes, np.int32)
