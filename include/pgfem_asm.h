@@ -246,6 +246,37 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU_all,
 int pg_asm_set_temperature_change(pg_asm* h, double temp_change);
 
 /* ----------------------------------------------------------------------------
+ * Output fields of the VTK writers from device data: what
+ * Writer_VTK::gather_nodal_data and gather_element_data compute
+ * (src/Writer_VTK.cpp:642-1000) before Writer_ASCII_VTK / Writer_Binary_VTK
+ * print them (the host writer in pigfem_amd/vtk.py keeps their formats).
+ *   pg_asm_plot_cells   number of VTK cells: every uncut element, and every
+ *                       integration child of a cut element (store_elements,
+ *                       src/Writer_VTK.cpp:221-312; cohesive cells, the
+ *                       writer's "COHESIVE" option, are not produced)
+ *   pg_asm_plot_fields  dnodes: per node (regular, then enrichment) its
+ *                       displacement, an enrichment node's being
+ *                       U(n1)(1 - alpha) + U(n2) alpha + U_enr
+ *                       (Problem::interpolate_nodal_enrichment,
+ *                       src/Problem.cpp:538-570) with denr_edge (n_enr pairs
+ *                       n1, n2) and denr_alpha (n_enr) from the set-up
+ *                       (pgh_enrich_edges); dcells: per cell, in store_elements
+ *                       order, [volume | integral of the strain (nstr) |
+ *                       integral of the stress (nstr)], each the sum over the
+ *                       cell's qps of (J w) times the volume functor, the
+ *                       strain functor (SmallStrainFast) or the stress
+ *                       functor (the material's stress at that strain,
+ *                       include/AssemblerSmallStrainStructural.h:23-79,
+ *                       include/Integrator.h:151-228); the writer divides by
+ *                       the volume.  U: global-dof order.  Device pointers;
+ *                       asynchronous on stream (NULL: the handle's), ordered
+ *                       after the handle's previous call.  The first call
+ *                       allocates the cell list (n_cells int2 + int64 + rows). */
+int pg_asm_plot_cells(const pg_asm* h, int64_t* n_cells);
+int pg_asm_plot_fields(pg_asm* h, const double* dU, const int32_t* denr_edge, const double* denr_alpha,
+                       double* dnodes, double* dcells, void* stream);
+
+/* ----------------------------------------------------------------------------
  * Heat conduction: AssemblerThermal (include/AssemblerThermal.h:16-45,
  * src/AssemblerThermal.cpp:16-107), the sibling plug-in of the structural
  * assembler behind the same Assembler::assemble.  One dof per node

@@ -108,6 +108,12 @@ int  pgh_inclusions(const pgh_problem* h, const double** centers,
  * Elem::get_inclusion_from_enrich); NULL for a pattern built from an external
  * mesh, whose inclusions the harness does not know. */
 const int32_t* pgh_enrich_inclusion(const pgh_problem* h);
+/* The cut edge of each enrichment node (n_enr pairs of regular node ids,
+ * ascending, mirrors included) and its fraction alpha along that edge: the
+ * node sits at x_0 + alpha (x_1 - x_0) (Mesh::_enrich_node_to_edge and
+ * getEnrichmentEdgeDistance, src/Mesh.cpp:3178-3217,3723-3735).  PG_E_ARG for
+ * a pattern built from an external mesh. */
+int pgh_enrich_edges(const pgh_problem* h, const int32_t** edges, const double** alpha);
 /* Nodal detection value per regular node (-1 matrix, else inclusion index). */
 const int32_t* pgh_node_detect(const pgh_problem* h);
 /* Prescribed value at full load of each prescribed dof (length n_pres); the

@@ -287,3 +287,54 @@ def cohesive_failure(problem, U):
                 f += de / m.delta_c if m.type == 4 else de / m.delta_c1 if m.type == 2 else 0.0
             rows[r.isv_off // 2 + k] = (np.linalg.norm(x1 - x0), f / 2)
     return rows
+
+
+def plot_fields(problem, U):
+    """The fields of the VTK writers, restated in numpy (the checker of
+    pg_asm_plot_fields): Writer_VTK::gather_nodal_data with
+    Problem::interpolate_nodal_enrichment (src/Writer_VTK.cpp:642-727,
+    src/Problem.cpp:538-570) for the nodes, and per VTK cell (each uncut element,
+    each integration child) the sums over its qps of J w, J w strain and
+    J w stress (Integrator::integrateElem, include/Integrator.h:151-228) from the
+    oracle's cached B matrices; the stress is the LEI law at that strain."""
+    p = problem
+    dim, nstr = p.dim, (3 if p.dim == 2 else 6)
+    Un = np.asarray(U, np.float64)[p.node_dof]
+    nodes = Un.copy()
+    for k in range(p.n_enr):
+        n1, n2 = (int(x) for x in p.enr_edge[k])
+        a = float(p.enr_alpha[k])
+        nodes[p.n_nodes + k] = Un[n1] * (1.0 - a) + Un[n2] * a + Un[p.n_nodes + k] * 1.0
+    o = Oracle(p)
+    rows = []
+    for e in range(p.n_elem):
+        c = int(p.cut_of_elem[e])
+        if c < 0:
+            local = [int(x) for x in p.conn[e]]
+            groups = [(int(p.elem_mat[e]), list(range(4 if dim == 2 else 5)))]
+        else:
+            r = p.cut_record(c)
+            local = [int(x) for x in p.conn[e]] + [int(r.enr[k]) for k in range(r.n_enr)]
+            groups, q0 = [], 0
+            for k in range(r.n_child):
+                nq = {0: 4, 1: 4, 2: 5, 3: 20, 4: 24}[int(r.child_type[k])]
+                groups.append((int(r.child_mat[k]), list(range(q0, q0 + nq))))
+                q0 += nq
+        Ue = Un[local].ravel()
+        nd = len(Ue)
+        cache = elem_cache(o, e)
+        stride = 2 + nstr * nd
+        for m, qps in groups:
+            D = np.array(p.desc.mats[m].D[: nstr * nstr]).reshape(nstr, nstr)
+            vol, ie, is_ = 0.0, np.zeros(nstr), np.zeros(nstr)
+            for q in qps:
+                W, J = cache[q * stride], cache[q * stride + 1]
+                B = cache[q * stride + 2: (q + 1) * stride].reshape(nstr, nd)
+                eps = B @ Ue
+                wgt = J * W
+                vol += wgt * 1.0
+                ie += wgt * eps
+                is_ += wgt * (D @ eps)
+            rows.append(np.concatenate([[vol], ie, is_]))
+    o.close()
+    return nodes, np.array(rows)

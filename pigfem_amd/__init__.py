@@ -10,8 +10,9 @@ from .assembler import AssemblerGPU, AssemblerGPUThermal, AssemblerGPUThermalPre
     write_cohesive_failure_lengths  # noqa: F401
 from .device import DeviceBuffer, synchronize  # noqa: F401
 from .sensitivity import SensitivityMaterialParameterGPU, lei_dmat_sensitivity  # noqa: F401
+from .vtk import write_vtk  # noqa: F401
 
 __all__ = ["RVEConfig", "RVEProblem", "batch_config", "config", "small_config", "AssemblerGPU",
            "AssemblerGPUThermal", "AssemblerGPUThermalPreLoad", "PMatrix", "PVector",
            "cut_inclusions", "write_cohesive_openings", "write_cohesive_failure_lengths",
-           "DeviceBuffer", "synchronize", "SensitivityMaterialParameterGPU", "lei_dmat_sensitivity", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]
+           "DeviceBuffer", "synchronize", "SensitivityMaterialParameterGPU", "lei_dmat_sensitivity", "write_vtk", "PG_FUNC", "PG_JAC", "PG_FP32", "PG_BATCHED", "PgError", "lib"]

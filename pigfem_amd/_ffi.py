@@ -120,6 +120,10 @@ SIGNATURES = {
     "pg_asm_cohesive_failure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "pg_asm_check_error": (C.c_int, [C.c_void_p]),
     "pg_asm_set_temperature_change": (C.c_int, [C.c_void_p, C.c_double]),
+    "pg_asm_plot_cells": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "pg_asm_plot_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "pgh_enrich_edges": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double))]),
     "pg_therm_create": (C.c_int, [C.POINTER(PgProblemDesc), C.POINTER(C.c_void_p)]),
     "pg_therm_assemble_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg_therm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,

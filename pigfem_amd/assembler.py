@@ -190,6 +190,32 @@ class AssemblerGPU:
         synchronize()
         return dO.download(n_out * 2).reshape(n_out, 2)
 
+    def plot_fields(self, U, enr_edge=None, enr_alpha=None):
+        """The fields of the VTK writers from the GPU (pg_asm_plot_fields): U is a
+        host array in global-dof order.  Returns (nodes, cells): per node its
+        plotted displacement (n_nodes + n_enr rows of dim), per VTK cell in the
+        writer's order [volume, strain integral (nstr), stress integral (nstr)].
+        The enrichment edges default to the problem's (pgh_enrich_edges)."""
+        from .device import DeviceBuffer, synchronize
+        p = self.problem
+        edge = p.enr_edge if enr_edge is None else np.asarray(enr_edge, np.int32)
+        alpha = p.enr_alpha if enr_alpha is None else np.asarray(enr_alpha, np.float64)
+        if p.n_enr and (edge is None or alpha is None):
+            raise ValueError("the enrichment edges of this mesh are not known: pass enr_edge and enr_alpha")
+        nc = C.c_int64()
+        check(lib().pg_asm_plot_cells(self._h, C.byref(nc)), "pg_asm_plot_cells")
+        nall = p.n_nodes + p.n_enr
+        ns = 3 if p.dim == 2 else 6
+        S = 1 + 2 * ns
+        dU = DeviceBuffer.from_numpy(np.ascontiguousarray(U, np.float64))
+        dE = DeviceBuffer.from_numpy(np.ascontiguousarray(edge, np.int32).ravel()) if p.n_enr else None
+        dA = DeviceBuffer.from_numpy(np.ascontiguousarray(alpha, np.float64)) if p.n_enr else None
+        dN, dC = DeviceBuffer.zeros(nall * p.dim), DeviceBuffer.zeros(nc.value * S)
+        check(lib().pg_asm_plot_fields(self._h, dU.ptr, dE.ptr if dE else None, dA.ptr if dA else None, dN.ptr,
+                                       dC.ptr, None), "pg_asm_plot_fields")
+        synchronize()
+        return dN.download(nall * p.dim).reshape(nall, p.dim), dC.download(nc.value * S).reshape(nc.value, S)
+
     def set_timing(self, on: bool) -> None:
         check(lib().pg_asm_set_timing(self._h, 1 if on else 0), "pg_asm_set_timing")
 

@@ -81,6 +81,8 @@ struct pgh_problem {
     std::vector<int32_t> npat_ptr, npat_col, npat_row, npat_map;
     std::vector<int32_t> n2e_ptr, n2e;
     std::vector<int32_t> enr_incl;        // inclusion of each enrichment node (mirrors included)
+    std::vector<int32_t> enr_edge;        // per enrichment node: the cut edge's two nodes (ascending)
+    std::vector<double> enr_alpha;        // per enrichment node: its fraction along that edge
     // batch of independent RVEs: per instance the first element, regular node,
     // enrichment node, free dof, prescribed dof, cohesive qp and cut element,
     // plus one closing row (PGH_BATCH_COLS columns)
@@ -726,12 +728,20 @@ void analyze_and_enrich(pgh_problem& P) {
     P.n_enr = coh ? 2 * n_init : n_init;
     P.enr_incl.resize(size_t(P.n_enr));
     for (int32_t k = 0; k < P.n_enr; ++k) P.enr_incl[k] = keys[k % n_init].inc;
+    P.enr_edge.resize(size_t(P.n_enr) * 2);
+    P.enr_alpha.resize(size_t(P.n_enr));
+    for (int32_t k = 0; k < P.n_enr; ++k) {
+        P.enr_edge[size_t(k) * 2] = keys[k % n_init].lo;
+        P.enr_edge[size_t(k) * 2 + 1] = keys[k % n_init].hi;
+    }
     P.coords.resize(size_t(P.n_nodes + P.n_enr) * P.dim);
     for (int32_t k = 0; k < n_init; ++k) {
         const double* x0 = &P.coords[size_t(keys[k].lo) * P.dim];
         const double* x1 = &P.coords[size_t(keys[k].hi) * P.dim];
         EdgeFn fn{x0, x1, &P.incl[keys[k].inc], P.dim};
         double a = intersect_edge(fn);
+        P.enr_alpha[k] = a;
+        if (coh) P.enr_alpha[n_init + k] = a;
         double* xe = &P.coords[size_t(P.n_nodes + k) * P.dim];
         for (int d = 0; d < P.dim; ++d) xe[d] = x0[d] + a * (x1[d] - x0[d]);
         if (coh) {
@@ -1232,6 +1242,8 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
         B->bc_val.resize(size_t(T[4]));
         B->node_detect.resize(size_t(T[1]));
         B->enr_incl.resize(size_t(T[2]));
+        B->enr_edge.resize(size_t(T[2]) * 2);
+        B->enr_alpha.resize(size_t(T[2]));
         for (int i = 0; i < n; ++i) {
             const pgh_problem& P = *inst[i];
             const int64_t* o = &B->inst_off[size_t(i) * C];
@@ -1249,7 +1261,12 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
             }
             for (int32_t a = 0; a < P.n_nodes; ++a)
                 B->node_detect[size_t(r0) + a] = P.node_detect[a] < 0 ? -1 : P.node_detect[a];
-            for (int32_t k = 0; k < P.n_enr; ++k) B->enr_incl[size_t(o[2]) + k] = int32_t(o[7] + P.enr_incl[k]);
+            for (int32_t k = 0; k < P.n_enr; ++k) {
+                B->enr_incl[size_t(o[2]) + k] = int32_t(o[7] + P.enr_incl[k]);
+                B->enr_edge[(size_t(o[2]) + k) * 2] = r0 + P.enr_edge[size_t(k) * 2];
+                B->enr_edge[(size_t(o[2]) + k) * 2 + 1] = r0 + P.enr_edge[size_t(k) * 2 + 1];
+                B->enr_alpha[size_t(o[2]) + k] = P.enr_alpha[k];
+            }
             for (int32_t k = 0; k < P.n_pres; ++k) B->bc_val[size_t(o[4]) + k] = P.bc_val[k];
             for (int32_t e = 0; e < P.n_elem; ++e) {
                 for (int k = 0; k < 3; ++k) B->conn[size_t(e0 + e) * 3 + k] = r0 + P.conn[size_t(e) * 3 + k];
@@ -1347,6 +1364,14 @@ int pgh_inclusions(const pgh_problem* h, const double** centers, const double** 
 const int32_t* pgh_node_detect(const pgh_problem* h) { return h ? h->node_detect.data() : nullptr; }
 
 const double* pgh_bc_values(const pgh_problem* h) { return h ? h->bc_val.data() : nullptr; }
+
+int pgh_enrich_edges(const pgh_problem* h, const int32_t** edges, const double** alpha) {
+    if (!h || !edges || !alpha) { g_err = "null argument"; return PG_E_ARG; }
+    if (h->enr_alpha.size() != size_t(h->n_enr)) { g_err = "enrichment edges unknown for this problem"; return PG_E_ARG; }
+    *edges = h->enr_edge.data();
+    *alpha = h->enr_alpha.data();
+    return PG_OK;
+}
 
 const int32_t* pgh_enrich_inclusion(const pgh_problem* h) {
     return h && !h->enr_incl.empty() ? h->enr_incl.data() : nullptr;

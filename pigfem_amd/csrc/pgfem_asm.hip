@@ -111,6 +111,15 @@ __global__ void k_fill_double(int64_t n, double* __restrict__ p, double v) {
     if (i < n) p[i] = v;
 }
 
+// Row i of `in` (width w) to row pos[i] of `out`.
+__global__ void k_scatter_rows(int64_t n, int w, const int64_t* __restrict__ pos, const double* __restrict__ in,
+                               double* __restrict__ out) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n * w) return;
+    const int64_t i = k / w, j = k % w;
+    out[pos[i] * w + j] = in[k];
+}
+
 // --------------------------------------------------------------------------
 // U in global-dof order -> node-major (node*dim + comp).  The reference gathers
 // U per element through NodalData (formElementalVector, src/Utilities.cpp:599-617).
@@ -1751,6 +1760,124 @@ __global__ void k_therm_pair(DevProb p, const double* __restrict__ stage, const 
 }
 
 // --------------------------------------------------------------------------
+// Fields of the VTK writers from device data (Writer_VTK::gather_nodal_data and
+// gather_element_data, src/Writer_VTK.cpp:642-1000).
+// --------------------------------------------------------------------------
+// Nodal displacement for plotting: a regular node's own U; an enrichment node
+// U(n1) (1 - alpha) + U(n2) alpha + U_enr * 1 (Problem::interpolate_nodal_enrichment,
+// src/Problem.cpp:538-570, scale factor 1): the field's value at the node.
+__global__ void k_plot_nodes(DevProb p, const double* __restrict__ Un, const int32_t* __restrict__ edge,
+                             const double* __restrict__ alpha, double* __restrict__ out) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int D = p.dim;
+    if (k >= int64_t(p.n_nodes) + p.n_enr) return;
+    if (k < p.n_nodes) {
+        for (int i = 0; i < D; ++i) out[k * D + i] = Un[k * D + i];
+        return;
+    }
+    const int64_t j = k - p.n_nodes;
+    const int n1 = edge[2 * j], n2 = edge[2 * j + 1];
+    const double a = alpha[j];
+    for (int i = 0; i < D; ++i)
+        out[k * D + i] = Un[int64_t(n1) * D + i] * (1.0 - a) + Un[int64_t(n2) * D + i] * a + Un[k * D + i] * 1.0;
+}
+
+// SmallStrainFast (src/Utilities.cpp:300-362): each strain component summed
+// over the nodes in turn; grad(l, a) and u(l, i) give node l's gradient and U.
+template <int D, typename G, typename UF>
+__device__ inline void small_strain_fast(int nl, G grad, UF u, double* e) {
+    for (int k = 0; k < (D == 2 ? 3 : 6); ++k) e[k] = 0.0;
+    if constexpr (D == 2) {
+        for (int n = 0; n < nl; ++n) e[0] += grad(n, 0) * u(n, 0);
+        for (int n = 0; n < nl; ++n) e[1] += grad(n, 1) * u(n, 1);
+        for (int n = 0; n < nl; ++n) e[2] += grad(n, 1) * u(n, 0) + grad(n, 0) * u(n, 1);
+    } else {
+        for (int n = 0; n < nl; ++n) e[0] += grad(n, 0) * u(n, 0);
+        for (int n = 0; n < nl; ++n) e[1] += grad(n, 1) * u(n, 1);
+        for (int n = 0; n < nl; ++n) e[2] += grad(n, 2) * u(n, 2);
+        for (int n = 0; n < nl; ++n) e[3] += grad(n, 2) * u(n, 1) + grad(n, 1) * u(n, 2);
+        for (int n = 0; n < nl; ++n) e[4] += grad(n, 2) * u(n, 0) + grad(n, 0) * u(n, 2);
+        for (int n = 0; n < nl; ++n) e[5] += grad(n, 1) * u(n, 0) + grad(n, 0) * u(n, 1);
+    }
+}
+
+// One thread per VTK cell (an uncut element, or one integration child of a cut
+// element: Writer_VTK::store_elements, src/Writer_VTK.cpp:221-312): the
+// integrals over the cell's qps of 1, the strain and the stress, each qp
+// adding (J w) times its value (Integrator::integrateElem with the strain and
+// stress functors and the volume functor, include/Integrator.h:151-228,
+// include/AssemblerSmallStrainStructural.h:23-79).  out: per cell
+// [volume, strain (nstr), stress (nstr)].  cells[c] = (element, child or -1).
+template <int D, bool POLY>
+__global__ __launch_bounds__(128) void k_plot_cells(DevProb p, const double* __restrict__ Un,
+                                                    const int2* __restrict__ cells, int n,
+                                                    double* __restrict__ out) {
+    constexpr int NEN = D + 1, NS = (D == 2 ? 3 : 6), NQ = (D == 2 ? 4 : 5), S = 1 + 2 * NS;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int e = cells[c].x, kc = cells[c].y;
+    double px[NEN][3], pdN[NEN][D], pg[NEN][3];
+    for (int m = 0; m < NEN; ++m) {
+        const int node = p.conn[int64_t(e) * NEN + m];
+        for (int i = 0; i < 3; ++i) px[m][i] = i < D ? p.coords[int64_t(node) * D + i] : 0.0;
+        for (int a = 0; a < D; ++a) pdN[m][a] = simplex_dNdr(m, a);
+    }
+    const double J = transform<D>(NEN, pdN, px, pg);
+    double vol = 0.0, ie[NS], is[NS];
+    for (int k = 0; k < NS; ++k) ie[k] = is[k] = 0.0;
+    if (kc < 0) {
+        double eps[NS], sig[NS];
+        small_strain_fast<D>(
+            NEN, [&](int m, int a) { return pg[m][a]; },
+            [&](int m, int i) { return Un[int64_t(p.conn[int64_t(e) * NEN + m]) * D + i]; }, eps);
+        lei_stress<D>(p.mats[p.elem_mat[e]].D, eps, sig);
+        for (int q = 0; q < NQ; ++q) {
+            const double wgt = J * (D == 2 ? TRI_QW[q] : TET_QW[q]);
+            vol += wgt * 1.0;
+            for (int k = 0; k < NS; ++k) ie[k] += wgt * eps[k];
+            for (int k = 0; k < NS; ++k) is[k] += wgt * sig[k];
+        }
+    } else {
+        const int ci = p.cut_of_elem[e];
+        const pg_cut_rec& rec = p.cut[ci];
+        const int nloc = NEN + rec.n_enr;
+        constexpr int CN = (D == 2 && POLY) ? 6 : 4;
+        const int ct = rec.child_type[kc], ccn = PG_CHILD_NN(ct);
+        double cx[CN][3], cg[CN][3], w;
+        int cl[CN];
+        for (int m = 0; m < CN; ++m) {
+            cl[m] = m < ccn ? rec.child_node[PG_CHILD_STRIDE(D) * kc + m] : -1;
+            const int l = cl[m] < 0 ? 0 : cl[m];
+            const int node = l < NEN ? p.conn[int64_t(e) * NEN + l] : rec.enr[l - NEN];
+            for (int i = 0; i < 3; ++i) cx[m][i] = (m < ccn && i < D) ? p.coords[int64_t(node) * D + i] : 0.0;
+        }
+        const double* Dm = p.mats[rec.child_mat[kc]].D;
+        auto nodeof = [&](int l) { return l < NEN ? p.conn[int64_t(e) * NEN + l] : rec.enr[l - NEN]; };
+        for (int q = 0; q < PG_CHILD_NQP(ct); ++q) {
+            const double Jc = child_qp<D, POLY>(ct, q, ccn, cx, cg, w);
+            auto grad = [&](int l, int a) {
+                if (l < NEN) return pg[l][a];
+                double g = 0.0;
+                for (int m = 0; m < CN; ++m)
+                    if (cl[m] == l) g = cg[m][a];
+                return g;
+            };
+            double eps[NS], sig[NS];
+            small_strain_fast<D>(nloc, grad, [&](int l, int i) { return Un[int64_t(nodeof(l)) * D + i]; }, eps);
+            lei_stress<D>(Dm, eps, sig);
+            const double wgt = Jc * w;
+            vol += wgt * 1.0;
+            for (int k = 0; k < NS; ++k) ie[k] += wgt * eps[k];
+            for (int k = 0; k < NS; ++k) is[k] += wgt * sig[k];
+        }
+    }
+    double* o = out + int64_t(c) * S;
+    o[0] = vol;
+    for (int k = 0; k < NS; ++k) o[1 + k] = ie[k];
+    for (int k = 0; k < NS; ++k) o[1 + NS + k] = is[k];
+}
+
+// --------------------------------------------------------------------------
 // Node-group kernels of the class-binned schedule.  A block owns NG nodes of a
 // group list (rows of the node-pair pattern).  Phase A takes one lane per
 // (node a, incident element e) incidence: for an uncut element it computes the
@@ -2968,6 +3095,13 @@ struct pg_asm {
     double thermal_exp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool thermal_set[8] = {false, false, false, false, false, false, false, false};
     double temp_change = 0.0;
+    // VTK cells (uncut elements and integration children), built at the first
+    // pg_asm_plot_fields: (element, child or -1), polygon children listed apart
+    int2* plot_cells = nullptr;
+    int32_t n_plot[2] = {0, 0};  // cells without / with polygon children
+    int64_t n_cells = 0;
+    int64_t* plot_pos = nullptr;  // each listed cell's position in the writer's cell order
+    double* plot_tmp = nullptr;   // the kernels' rows, in list order
     int mat_type[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // PG_MAT_* of each material
     unsigned coh_law_mask = 0;                   // bit PG_MAT_* set for each law of a cohesive child
 };
@@ -3591,6 +3725,8 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
     if (s->cohesive)
         for (int c = 0; c < s->n_cut; ++c)
             for (int k = 0; k < s->cut[c].n_coh; ++k) h->coh_law_mask |= 1u << s->mats[s->cut[c].coh_mat[k]].type;
+    h->n_cells = int64_t(s->n_elem) - s->n_cut;
+    for (int c = 0; c < s->n_cut; ++c) h->n_cells += s->cut[c].n_child;
     // incidence records in n2e order (host only; the kernels read them regrouped
     // per node group below), nen + 1 words, the last a tag.  Uncut
     // element: its nodes, tag -1 - material.  Cut element: the staged row of
@@ -4220,6 +4356,80 @@ int pg_asm_sensitivity_dPdd(pg_asm* h, const pg_sens_param* prm, unsigned flags,
     const unsigned gp = unsigned((d.n_pairs + h->pair_block - 1) / h->pair_block);
     if (d.dim == 2) k_pair_staged<2><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, nullptr, ddPdd, PG_FUNC);
     else k_pair_staged<3><<<gp, h->pair_block, 0, st>>>(d, h->stage, h->stage_k, h->stage_p, nullptr, ddPdd, PG_FUNC);
+    HIP_OK(hipGetLastError());
+    return record_done(h, st);
+}
+
+int pg_asm_plot_cells(const pg_asm* h, int64_t* n_cells) {
+    if (!h || !n_cells) { g_err = "null argument"; return PG_E_ARG; }
+    *n_cells = h->n_cells;
+    return PG_OK;
+}
+
+int pg_asm_plot_fields(pg_asm* h, const double* dU, const int32_t* denr_edge, const double* denr_alpha,
+                       double* dnodes, double* dcells, void* stream) {
+    if (!h) { g_err = "null handle"; return PG_E_ARG; }
+    const DevProb& d = h->d;
+    if (!dU || !dnodes || !dcells || (d.n_enr > 0 && (!denr_edge || !denr_alpha))) {
+        g_err = "missing device buffer";
+        return PG_E_ARG;
+    }
+    if (!h->plot_cells) {
+        // cell order of Writer_VTK::store_elements: elements in order, each cut
+        // element's integration children in child order
+        std::vector<int32_t> coe(size_t(d.n_elem));
+        std::vector<pg_cut_rec> cut(size_t(d.n_cut));
+        HIP_OK(hipMemcpy(coe.data(), d.cut_of_elem, coe.size() * 4, hipMemcpyDeviceToHost));
+        if (d.n_cut) HIP_OK(hipMemcpy(cut.data(), d.cut, cut.size() * sizeof(pg_cut_rec), hipMemcpyDeviceToHost));
+        std::vector<int2> plain, poly;
+        std::vector<int64_t> order, qorder;  // each listed cell's position in store_elements order
+        int64_t pos = 0;
+        for (int32_t e = 0; e < d.n_elem; ++e) {
+            if (coe[e] < 0) {
+                plain.push_back(make_int2(e, -1));
+                order.push_back(pos++);
+                continue;
+            }
+            const pg_cut_rec& r = cut[coe[e]];
+            for (int k = 0; k < r.n_child; ++k) {
+                const bool pc = r.child_type[k] == PG_CHILD_POLY5 || r.child_type[k] == PG_CHILD_POLY6;
+                (pc ? poly : plain).push_back(make_int2(e, k));
+                (pc ? qorder : order).push_back(pos++);
+            }
+        }
+        order.insert(order.end(), qorder.begin(), qorder.end());
+        const size_t nc = order.size();
+        if (int rc = alloc(h, &h->plot_cells, nc)) return rc;
+        if (!plain.empty())
+            HIP_OK(hipMemcpy(h->plot_cells, plain.data(), plain.size() * sizeof(int2), hipMemcpyHostToDevice));
+        if (!poly.empty())
+            HIP_OK(hipMemcpy(h->plot_cells + plain.size(), poly.data(), poly.size() * sizeof(int2), hipMemcpyHostToDevice));
+        h->n_plot[0] = int32_t(plain.size());
+        h->n_plot[1] = int32_t(poly.size());
+        h->n_cells = int64_t(nc);
+        if (int rc = alloc(h, &h->plot_pos, nc)) return rc;
+        HIP_OK(hipMemcpy(h->plot_pos, order.data(), nc * sizeof(int64_t), hipMemcpyHostToDevice));
+        constexpr int S2 = 1 + 2 * 3, S3 = 1 + 2 * 6;
+        if (int rc = alloc(h, &h->plot_tmp, nc * size_t(d.dim == 2 ? S2 : S3))) return rc;
+    }
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const int64_t nall = int64_t(d.n_nodes) + d.n_enr;
+    const int64_t nu = nall * d.dim;
+    if (int rc = order_after_last(h, st)) return rc;
+    k_permute_u<<<dim3(unsigned((nu + 255) / 256)), dim3(256), 0, st>>>(nu, d.node_dof, dU, h->Un);
+    k_plot_nodes<<<dim3(unsigned((nall + 255) / 256)), dim3(256), 0, st>>>(d, h->Un, denr_edge, denr_alpha, dnodes);
+    const int S = d.dim == 2 ? 1 + 2 * 3 : 1 + 2 * 6;
+    const int n0 = h->n_plot[0], n1 = h->n_plot[1];
+    if (d.dim == 2) {
+        if (n0) k_plot_cells<2, false><<<unsigned((n0 + 127) / 128), 128, 0, st>>>(d, h->Un, h->plot_cells, n0, h->plot_tmp);
+        if (n1)
+            k_plot_cells<2, true><<<unsigned((n1 + 127) / 128), 128, 0, st>>>(d, h->Un, h->plot_cells + n0, n1,
+                                                                            h->plot_tmp + int64_t(n0) * S);
+    } else if (n0) {
+        k_plot_cells<3, false><<<unsigned((n0 + 127) / 128), 128, 0, st>>>(d, h->Un, h->plot_cells, n0, h->plot_tmp);
+    }
+    k_scatter_rows<<<dim3(unsigned((h->n_cells * S + 255) / 256)), dim3(256), 0, st>>>(h->n_cells, S, h->plot_pos,
+                                                                                       h->plot_tmp, dcells);
     HIP_OK(hipGetLastError());
     return record_done(h, st);
 }

@@ -192,6 +192,13 @@ class RVEProblem:
         self.node_detect = _arr(L.pgh_node_detect(h), self.n_nodes, np.int32)
         ei = L.pgh_enrich_inclusion(h)
         self.enr_incl = _arr(ei, self.n_enr, np.int32) if ei else None
+        ep, ap = C.POINTER(C.c_int32)(), C.POINTER(C.c_double)()
+        if L.pgh_enrich_edges(h, C.byref(ep), C.byref(ap)) == 0 and self.n_enr:
+            self.enr_edge = _arr(ep, 2 * self.n_enr, np.int32).reshape(self.n_enr, 2)
+            self.enr_alpha = _arr(ap, self.n_enr, np.float64)
+        else:
+            self.enr_edge = np.zeros((0, 2), np.int32) if self.n_enr == 0 else None
+            self.enr_alpha = np.zeros(0) if self.n_enr == 0 else None
         self.bc_values = _arr(L.pgh_bc_values(h), self.n_pres, np.float64)
         cptr, rptr, ni = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.c_int32()
         check(L.pgh_inclusions(h, C.byref(cptr), C.byref(rptr), C.byref(ni)), "pgh_inclusions")
