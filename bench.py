@@ -177,7 +177,9 @@ def measure(prob, cfg, args, full):
     pre_ms, pair_ms = float(kt[:, 1].mean()), float(kt[:, 2].mean())
     clean_ms, dirty_ms = float(kt[:, 3].mean()), float(kt[:, 4].mean())
     B_alg = asm.algorithmic_bytes()
-    cut_bytes = 128 * prob.n_cut + 16 * prob.n_cqp
+    import ctypes
+    from pigfem_amd import _ffi
+    cut_bytes = ctypes.sizeof(_ffi.PgCutRec) * prob.n_cut + 16 * prob.n_cqp
     pair_bytes = B_alg - cut_bytes
     achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
     whole = B_alg / (results["mixed"]["ms"] * 1e-3) / 1e9

@@ -94,7 +94,8 @@ typedef struct pg_material {
  * src/Polygon.cpp:51-228) in 2D, tet4 in 3D.  Child k's nodes start at
  * child_node[PG_CHILD_STRIDE(dim) * k]: at most 4 children of at most 6 nodes
  * in 2D, at most 8 tet4 children in 3D.  A cohesive child is a CohEdge2
- * (4 local nodes: 2 lower then 2 upper) or CohTri3 (6).
+ * (4 local nodes: 2 lower then 2 upper), CohTri3 (6) or CohQuad4 (8: the
+ * interface of a tet4 cut two nodes against two, src/CohQuad4.cpp).
  * ------------------------------------------------------------------------- */
 #define PG_CHILD_TRI3   0
 #define PG_CHILD_QUAD4  1
@@ -122,11 +123,12 @@ typedef struct pg_cut_rec {
     uint8_t child_type[PG_MAX_CHILD];
     uint8_t child_mat[PG_MAX_CHILD];
     uint8_t child_node[32];         /* child k's nodes from PG_CHILD_STRIDE(dim) * k    */
-    uint8_t coh_nodes[PG_MAX_COH][6];
+    uint8_t coh_nodes[PG_MAX_COH][8];
     uint8_t coh_mat[PG_MAX_COH];
-    uint8_t coh_nn;                 /* nodes per cohesive child: 4 (CohEdge2) or 6 (CohTri3) */
-    uint8_t pad[2];
-} pg_cut_rec;                       /* 128 bytes */
+    uint8_t coh_nn;                 /* nodes per cohesive child: 4 (CohEdge2), 6 (CohTri3)
+                                       or 8 (CohQuad4) */
+    uint8_t pad[4];
+} pg_cut_rec;                       /* 136 bytes */
 
 /* ----------------------------------------------------------------------------
  * Static problem description (set-up output of the host harness; the

@@ -290,11 +290,11 @@ static int64_t cache_size(const oracle* o, int e) {
     const pg_cut_rec* r = &d->cut[c];
     int nd = (d->nen + r->n_enr) * o->nndof;
     int ncq = r->n_coh * (r->coh_nn == 4 ? 2 : 4);
-    return (int64_t)r->n_vqp * (2 + nstr * nd) + (int64_t)ncq * (2 + 3 + dim * dim);
+    return (int64_t)r->n_vqp * (2 + nstr * nd) + (int64_t)ncq * (2 + 4 + dim * dim);
 }
 
 /* Mesh::store_shape_functions + storeBmats for element e: per volumetric qp
- * [W, J, B...], per cohesive qp [W, J, N(3), R(dim*dim)]. */
+ * [W, J, B...], per cohesive qp [W, J, N(4), R(dim*dim)]. */
 static int store_shapes(oracle* o, int e, double* out) {
     const pg_problem_desc* d = o->d;
     int dim = d->dim, nstr = o->nstr, nen = d->nen;
@@ -397,13 +397,25 @@ static int store_shapes(oracle* o, int e, double* out) {
     int cnn = r->coh_nn / 2, cnq = r->coh_nn == 4 ? 2 : 4;
     for (int k = 0; k < r->n_coh; ++k)
         for (int q = 0; q < cnq; ++q) {
-            double cN[3] = {0, 0, 0}, dNr[3][2], R[3][3], dA;
+            double cN[4] = {0, 0, 0, 0}, dNr[4][2], R[3][3], dA;
             double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
             if (r->coh_nn == 4) {
                 double s = q == 0 ? -G2 : G2;
                 w = 1.0;
                 cN[0] = 0.5 * (1.0 - s); cN[1] = 0.5 * (1 + s);
                 dNr[0][0] = -0.5; dNr[1][0] = 0.5;
+            } else if (r->coh_nn == 8) {
+                /* CohQuad4::q_point / compute_shape / compute_shape_grad
+                 * (src/CohQuad4.cpp:58-140): edge qps (q % 2, q / 2), w = w0 w1 */
+                double e0 = (q % 2) == 0 ? -G2 : G2, e1 = (q / 2) == 0 ? -G2 : G2;
+                w = 1.0 * 1.0;
+                double rr = e0, ss = e1;
+                cN[0] = 0.25 * (1.0 - rr) * (1.0 - ss); cN[1] = 0.25 * (1.0 + rr) * (1.0 - ss);
+                cN[2] = 0.25 * (1.0 + rr) * (1.0 + ss); cN[3] = 0.25 * (1.0 - rr) * (1.0 + ss);
+                dNr[0][0] = -0.25 * (1.0 - ss); dNr[0][1] = -0.25 * (1.0 - rr);
+                dNr[1][0] = 0.25 * (1.0 - ss);  dNr[1][1] = -0.25 * (1.0 + rr);
+                dNr[2][0] = 0.25 * (1.0 + ss);  dNr[2][1] = 0.25 * (1.0 + rr);
+                dNr[3][0] = -0.25 * (1.0 + ss); dNr[3][1] = 0.25 * (1.0 - rr);
             } else {
                 w = TRI_W[q];
                 cN[0] = 1.0 - TRI_X[q] - TRI_Y[q]; cN[1] = TRI_X[q]; cN[2] = TRI_Y[q];
@@ -431,10 +443,10 @@ static int store_shapes(oracle* o, int e, double* out) {
             for (int i = 0; i < dim; ++i)
                 for (int a = 0; a < dim - 1; ++a) R[a][i] = Jc[a][i] / mag[a];
             out[0] = w; out[1] = dA;
-            for (int n = 0; n < 3; ++n) out[2 + n] = n < cnn ? cN[n] * 1.0 : 0.0;
+            for (int n = 0; n < 4; ++n) out[2 + n] = n < cnn ? cN[n] * 1.0 : 0.0;
             for (int i = 0; i < dim; ++i)
-                for (int j = 0; j < dim; ++j) out[5 + i * dim + j] = R[i][j];
-            out += 2 + 3 + dim * dim;
+                for (int j = 0; j < dim; ++j) out[6 + i * dim + j] = R[i][j];
+            out += 2 + 4 + dim * dim;
         }
     return 0;
 }
@@ -796,7 +808,7 @@ static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int f
             int cnn = r->coh_nn / 2, cnq = r->coh_nn == 4 ? 2 : 4, ncd = r->coh_nn * dim;
             int cq = 0;
             for (int k = 0; k < r->n_coh; ++k) {
-                double Kc[18 * 18], Pc[18], Kq[18 * 18], Pq[18], Uc[18];
+                double Kc[24 * 24], Pc[24], Kq[24 * 24], Pq[24], Uc[24];
                 memset(Kc, 0, sizeof(double) * ncd * ncd);
                 memset(Pc, 0, sizeof(double) * ncd);
                 for (int n = 0; n < r->coh_nn; ++n)
@@ -806,7 +818,7 @@ static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int f
                     const double* N = cc + 2;
                     double R[3][3];
                     for (int i = 0; i < dim; ++i)
-                        for (int j = 0; j < dim; ++j) R[i][j] = cc[5 + i * dim + j];
+                        for (int j = 0; j < dim; ++j) R[i][j] = cc[6 + i * dim + j];
                     /* cohesiveDeltaFast */
                     double dl[3] = {0, 0, 0}, del[3];
                     for (int i = 0; i < dim; ++i)
@@ -859,7 +871,7 @@ static int assemble_elem(oracle* o, int e, const double* U, double* isv_t, int f
                         for (int i = 0; i < ncd * ncd; ++i) Kc[i] += wgt * Kq[i];
                     if (func)
                         for (int i = 0; i < ncd; ++i) Pc[i] += wgt * Pq[i];
-                    cc += 2 + 3 + dim * dim;
+                    cc += 2 + 4 + dim * dim;
                 }
                 /* fold into the element matrix */
                 for (int n1 = 0; n1 < r->coh_nn; ++n1) {
@@ -1119,17 +1131,17 @@ int oracle_sens_dPdd(oracle* o, const pg_sens_param* prm, unsigned flags, const 
                         strcpy(o->err, "Sensitivity is not implemented for the bilinear cohesive model");
                         return -2;
                     }
-                    double dPc[18], Uc[18];
+                    double dPc[24], Uc[24];
                     memset(dPc, 0, sizeof(double) * ncd);
                     for (int n = 0; n < r->coh_nn; ++n)
                         for (int j = 0; j < dim; ++j) Uc[n * dim + j] = Ue[r->coh_nodes[k][n] * dim + j];
                     for (int q = 0; q < cnq; ++q) {
-                        const double* c0 = ccq + (size_t)(k * cnq + q) * (2 + 3 + dim * dim);
+                        const double* c0 = ccq + (size_t)(k * cnq + q) * (2 + 4 + dim * dim);
                         const double W = c0[0], J = c0[1];
                         const double* N = c0 + 2;
                         double R[3][3];
                         for (int i = 0; i < dim; ++i)
-                            for (int j = 0; j < dim; ++j) R[i][j] = c0[5 + i * dim + j];
+                            for (int j = 0; j < dim; ++j) R[i][j] = c0[6 + i * dim + j];
                         double dl[3] = {0, 0, 0}, del[3], dt[3], tR[3];
                         for (int i = 0; i < dim; ++i)
                             for (int n = 0; n < cnn; ++n) dl[i] += N[n] * (Uc[(n + cnn) * dim + i] - Uc[n * dim + i]);

@@ -57,12 +57,12 @@ class PgCutRec(C.Structure):
         ("n_enr", C.c_uint8), ("n_child", C.c_uint8), ("n_coh", C.c_uint8), ("n_vqp", C.c_uint8),
         ("child_type", C.c_uint8 * PG_MAX_CHILD), ("child_mat", C.c_uint8 * PG_MAX_CHILD),
         ("child_node", C.c_uint8 * 32),
-        ("coh_nodes", (C.c_uint8 * 6) * PG_MAX_COH), ("coh_mat", C.c_uint8 * PG_MAX_COH),
-        ("coh_nn", C.c_uint8), ("pad", C.c_uint8 * 2),
+        ("coh_nodes", (C.c_uint8 * 8) * PG_MAX_COH), ("coh_mat", C.c_uint8 * PG_MAX_COH),
+        ("coh_nn", C.c_uint8), ("pad", C.c_uint8 * 4),
     ]
 
 
-assert C.sizeof(PgCutRec) == 128
+assert C.sizeof(PgCutRec) == 136
 
 
 def child_nodes(rec, dim: int, k: int) -> list:

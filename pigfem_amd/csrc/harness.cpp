@@ -503,10 +503,6 @@ bool detect_tet4(const int nd[4], const int32_t gid[4], bool cohesive, Detect& D
         } else {
             fail(PG_E_MESH, "tet4 element has 2 nodes separated in an unknown manner");
         }
-        if (cohesive)
-            fail(PG_E_UNSUPPORTED, "3D cohesive quadrilateral interfaces (CohQuad4, a tet4 with two nodes on each "
-                                   "side) are not supported; node-centred inclusions (PGH_PLACE_AT_NODES) cut "
-                                   "tets one node against three only");
     }
     for (int k = 0; k < D.ncut; ++k) D.slot_incl[k] = vec[1];
     // decomposePrisms, src/tet4.cpp:673-735

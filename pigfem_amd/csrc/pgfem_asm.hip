@@ -270,13 +270,24 @@ __device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3],
 // CohEdge2 2-pt Gauss src/CohEdge2.cpp:70-78,186-209; CohTri3 tri3 4-pt rule).
 template <int D>
 __device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], double* N, double R[D][D], double& w) {
-    double dNr[3][2];
+    double dNr[4][2];
     const int cnn = coh_nn / 2;
     if (coh_nn == 4) {
         const double xi = (q == 0) ? -PG_GAUSS2 : PG_GAUSS2;
         N[0] = 0.5 * (1.0 - xi); N[1] = 0.5 * (1 + xi);
         dNr[0][0] = -0.5; dNr[1][0] = 0.5;
         w = 1.0;
+    } else if (coh_nn == 8) {
+        // CohQuad4 (src/CohQuad4.cpp:58-140): the tensor product of the edge's
+        // 2-pt Gauss rule, qp q at (xi_{q % 2}, xi_{q / 2}), w = 1 * 1; quad4 shapes
+        const double r = (q & 1) ? PG_GAUSS2 : -PG_GAUSS2, s = (q & 2) ? PG_GAUSS2 : -PG_GAUSS2;
+        N[0] = 0.25 * (1.0 - r) * (1.0 - s); N[1] = 0.25 * (1.0 + r) * (1.0 - s);
+        N[2] = 0.25 * (1.0 + r) * (1.0 + s); N[3] = 0.25 * (1.0 - r) * (1.0 + s);
+        dNr[0][0] = -0.25 * (1.0 - s); dNr[0][1] = -0.25 * (1.0 - r);
+        dNr[1][0] = 0.25 * (1.0 - s);  dNr[1][1] = -0.25 * (1.0 + r);
+        dNr[2][0] = 0.25 * (1.0 + s);  dNr[2][1] = 0.25 * (1.0 + r);
+        dNr[3][0] = -0.25 * (1.0 + s); dNr[3][1] = 0.25 * (1.0 - r);
+        w = 1.0 * 1.0;
     } else {
         const double qx = pick(TRI_QX, q), qy = pick(TRI_QY, q);
         N[0] = 1.0 - qx - qy; N[1] = qx; N[2] = qy;
@@ -485,7 +496,7 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     __shared__ double wq[MAXQ];            // J w of each volumetric qp
     __shared__ double sq[MAXQ][NS];        // stress at each volumetric qp
     __shared__ int mq[MAXQ];               // material of each volumetric qp (its child's)
-    __shared__ double cN[MAXC][3], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
+    __shared__ double cN[MAXC][4], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
     __shared__ pg_cut_rec rec;
     const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
     stage_dmats<NS>(p, sD);
@@ -559,8 +570,8 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     const int ncq = rec.n_coh * cnq;
     for (int cq = lane; cq < ncq; cq += 64) {
         const int k = cq / cnq, q = cq % cnq;
-        double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
-        for (int n = 0; n < 3; ++n)
+        double lo[4][3], ulo[12], uup[12], N[4], R[D][D], w, RDR[D][D], tR[D];
+        for (int n = 0; n < 4; ++n)
             for (int i = 0; i < D; ++i) {
                 const bool in = n < cnn;
                 lo[n][i] = in ? lx[rec.coh_nodes[k][n]][i] : 0.0;
@@ -576,7 +587,7 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
             for (int j = 0; j < D; ++j) cRDR[cq][i][j] = RDR[i][j];
             ctR[cq][i] = tR[i];
         }
-        for (int n = 0; n < 3; ++n) cN[cq][n] = N[n];
+        for (int n = 0; n < 4; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
         cw[cq] = dA * w;
     }
     __syncthreads();
@@ -732,7 +743,7 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
     __shared__ double wq[MAXQ];            // J w of each volumetric qp
     __shared__ double sq[MAXQ][NS];        // dstress at each volumetric qp (0 where not visited)
     __shared__ int bq[MAXQ];               // the qp whose B each qp uses
-    __shared__ double cN[MAXC][3], cw[MAXC], ctR[MAXC][D];
+    __shared__ double cN[MAXC][4], cw[MAXC], ctR[MAXC][D];
     __shared__ pg_cut_rec rec;
     const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
     constexpr int RW = int(sizeof(pg_cut_rec) / 4);
@@ -820,8 +831,8 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
         const pg_material& m = p.mats[rec.coh_mat[k]];
         const bool right = rec.coh_mat[k] == s.mat;
         const bool visit = right || m.type == PG_MAT_OP || m.type == PG_MAT_BL;
-        double lo[3][3], N[3], R[D][D], w, tR[D];
-        for (int n = 0; n < 3; ++n)
+        double lo[4][3], N[4], R[D][D], w, tR[D];
+        for (int n = 0; n < 4; ++n)
             for (int i = 0; i < D; ++i) lo[n][i] = n < cnn ? lx[rec.coh_nodes[k][n]][i] : 0.0;
         const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
         for (int i = 0; i < D; ++i) tR[i] = 0.0;
@@ -848,7 +859,7 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
             }
         }
         for (int i = 0; i < D; ++i) ctR[cq][i] = tR[i];
-        for (int n = 0; n < 3; ++n) cN[cq][n] = N[n];
+        for (int n = 0; n < 4; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
         cw[cq] = dA * w;
     }
     __syncthreads();
@@ -911,7 +922,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     // nodes of the largest child
     constexpr int CN = (D == 2 && MAXQ > 8) ? 6 : 4;
     __shared__ double qs[EPW][MAXQ][NS];
-    __shared__ double cN[EPW][CUT_MAXC][3];
+    __shared__ double cN[EPW][CUT_MAXC][4];
     __shared__ double cw[EPW][CUT_MAXC];
     __shared__ double cRDR[EPW][CUT_MAXC][D][D];
     __shared__ double ctR[EPW][CUT_MAXC][D];
@@ -1056,8 +1067,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         {
             // ---- cohesive qp ----
             const int k = cq / cnq, q = cq % cnq;
-            double lo[3][3], ulo[9], uup[9], N[3], R[D][D], w, RDR[D][D], tR[D];
-            for (int n = 0; n < 3; ++n)
+            double lo[4][3], ulo[12], uup[12], N[4], R[D][D], w, RDR[D][D], tR[D];
+            for (int n = 0; n < 4; ++n)
                 for (int i = 0; i < D; ++i) {
                     const bool in = n < cnn;
                     lo[n][i] = in ? xs[g][rec.coh_nodes[k][n]][i] : 0.0;
@@ -1073,7 +1084,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 for (int j = 0; j < D; ++j) cRDR[g][cq][i][j] = RDR[i][j];
                 ctR[g][cq][i] = tR[i];
             }
-            for (int n = 0; n < 3; ++n) cN[g][cq][n] = N[n];
+            for (int n = 0; n < 4; ++n) cN[g][cq][n] = n < cnn ? N[n] : 0.0;
             cw[g][cq] = dA * w;
         }
     }
@@ -3387,7 +3398,10 @@ int check_desc(const pg_problem_desc* s) {
             nq += PG_CHILD_NQP(t);
         }
         if (nq != r.n_vqp) { g_err = "cut record qp count mismatch"; return PG_E_ARG; }
-        if (r.n_coh && r.coh_nn != (s->dim == 2 ? 4 : 6)) { g_err = "bad cohesive child"; return PG_E_UNSUPPORTED; }
+        if (r.n_coh && (s->dim == 2 ? r.coh_nn != 4 : (r.coh_nn != 6 && r.coh_nn != 8))) {
+            g_err = "bad cohesive child";
+            return PG_E_UNSUPPORTED;
+        }
         if (r.isv_off < 0 || r.isv_off + r.n_coh * (r.coh_nn == 4 ? 2 : 4) > s->n_cqp) {
             g_err = "cut record ISV range out of bounds";
             return PG_E_ARG;

@@ -104,6 +104,11 @@ def small_config(kind: str = "c0", n: int = 24, n_incl: int = 3, seed: int = 11)
         return RVEConfig(dim=2, n=n, n_incl=120 if n_incl == 3 else n_incl, r_min=0.02, r_max=0.035, gap_h=0.05,
                          seed=1 if seed == 11 else seed, cohesive=coh, place=_ffi.PGH_PLACE_BEST_EFFORT,
                          name=f"small-{kind}-{n}")
+    if kind in ("tet-op-q", "tet-bl-q"):
+        # spheres anywhere: the interfaces cut tets one node against three
+        # (CohTri3) and two against two (CohQuad4)
+        return RVEConfig(dim=3, n=n, n_incl=n_incl, r_min=0.12, r_max=0.2, seed=seed,
+                         cohesive=_ffi.PGH_COH_OP if kind == "tet-op-q" else _ffi.PGH_COH_BL, name=f"small-{kind}-{n}")
     if kind in ("tet-op", "tet-bl"):
         # spheres centred at mesh nodes, r < h / 2: CohTri3 interfaces only
         h = 1.0 / n

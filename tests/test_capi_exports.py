@@ -36,5 +36,5 @@ def test_ffi_signatures_cover_header():
 
 
 def test_struct_layouts():
-    assert ctypes.sizeof(_ffi.PgCutRec) == 128
+    assert ctypes.sizeof(_ffi.PgCutRec) == 136
     assert ctypes.sizeof(_ffi.PgMaterial) == 8 + 36 * 8 + 7 * 8 + 8

@@ -16,9 +16,10 @@ from tests.helpers import full_matrix, rel, u_scale
 pytestmark = pytest.mark.gpu
 
 # multi*: 120 circles packed 0.05 h apart, so that tri3 elements are cut by two
-# or three inclusions (polygon children)
+# or three inclusions (polygon children); tet-*-q: spheres anywhere, so that tet4
+# interfaces are CohTri3 and CohQuad4
 CASES = [("c0", 24, 3), ("bl", 24, 3), ("op", 24, 3), ("tet", 8, 1), ("tet-op", 6, 3), ("multi", 24, 3),
-         ("multi-op", 24, 3), ("multi-bl", 24, 3)]
+         ("multi-op", 24, 3), ("multi-bl", 24, 3), ("tet-op-q", 8, 1), ("tet-bl-q", 8, 1)]
 
 
 def _run(asm, p, U, ic, func=True, jac=True):
@@ -156,7 +157,8 @@ def test_device_api_matches_host_api():
     assert np.array_equal(dit.download(p.n_cqp), Ih)
 
 
-@pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1), ("tet-op", 6, 3), ("multi-op", 24, 3)])
+@pytest.mark.parametrize("kind,n,ni", [("op", 24, 3), ("tet", 8, 1), ("tet-op", 6, 3), ("multi-op", 24, 3),
+                                      ("tet-op-q", 8, 1)])
 def test_every_schedule_variant(kind, n, ni):
     """All node-group sizes, group-kernel forms (and the global-memory fallback
     that groups overflowing the LDS tables take) and cut-kernel variants give

@@ -165,10 +165,17 @@ def test_incidence_is_ascending_and_complete():
 
 
 def test_rejects_unsupported():
-    # a large sphere cuts tets two nodes against two: a quadrilateral cohesive
-    # interface (CohQuad4), which the harness does not build
-    with pytest.raises(Exception, match="CohQuad4"):
-        RVEProblem(RVEConfig(dim=3, n=8, n_incl=1, r_min=0.3, r_max=0.3, cohesive=_ffi.PGH_COH_OP))
+    # spheres packed closer than a tet: elements between two inclusions (the
+    # reference's own cases are incomplete, src/tet4.cpp:566-610) are refused
+    with pytest.raises(Exception, match="tet4 element cut by"):
+        RVEProblem(RVEConfig(dim=3, n=4, n_incl=40, r_min=0.1, r_max=0.15, gap_h=0.01,
+                             place=_ffi.PGH_PLACE_BEST_EFFORT))
+    # a large sphere cuts tets one node against three (CohTri3) and two against
+    # two (CohQuad4: 4 lower + 4 upper nodes, 4 qps)
+    p = RVEProblem(RVEConfig(dim=3, n=8, n_incl=1, r_min=0.3, r_max=0.3, cohesive=_ffi.PGH_COH_OP))
+    nn = [p.cut_record(i).coh_nn for i in range(p.n_cut)]
+    assert 6 in nn and 8 in nn
+    assert p.n_cqp == 4 * sum(p.cut_record(i).n_coh for i in range(p.n_cut))
     # node-centred spheres cut one node against three only: CohTri3 interfaces
     p = RVEProblem(small_config("tet-op", n=6, n_incl=2))
     assert p.n_cut > 0 and p.n_cqp == 4 * sum(p.cut_record(i).n_coh for i in range(p.n_cut))

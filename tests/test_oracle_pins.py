@@ -232,7 +232,7 @@ def test_zero_opening_tangent():
     assert np.allclose(D, 50.0 * np.e / 1e-3 * np.eye(2), rtol=1e-12)
 
 
-@pytest.mark.parametrize("kind", ["op", "bl", "tet-op", "tet-bl"])
+@pytest.mark.parametrize("kind", ["op", "bl", "tet-op", "tet-bl", "tet-op-q", "tet-bl-q"])
 def test_assembled_tangent_central_difference(kind):
     """K of a cohesive RVE is the derivative of P (signs of the cohesive fold, the
     rotation, the opening and the law's tangent all enter)."""
@@ -246,6 +246,10 @@ def test_assembled_tangent_central_difference(kind):
     rng = np.random.default_rng(0)
     enr = p.node_dof[p.n_nodes:].ravel()
     cols = list(rng.choice(enr, 6, replace=False)) + list(rng.choice(p.n_dof, 4, replace=False))
+    if kind.endswith("-q"):  # the enrichment dofs of a CohQuad4 interface
+        c = next(c for c in range(p.n_cut) if p.cut[c].coh_nn == 8)
+        r = p.cut[c]
+        cols += [int(p.node_dof[r.enr[r.coh_nodes[0][n] - 4]][j]) for n in (0, 5) for j in range(3)]
     h = 1e-6 * u_scale(cfg)
     for j in cols:
         e = np.zeros(p.n_dof)
@@ -314,16 +318,66 @@ def test_cohtri3_geometry():
             for q in range(4):
                 W, J = cache[off], cache[off + 1]
                 N = cache[off + 2: off + 5]
-                R = cache[off + 5: off + 14].reshape(3, 3)
+                R = cache[off + 6: off + 15].reshape(3, 3)
                 s += W * J
                 assert abs(N.sum() - 1.0) < 1e-14 and (N >= 0).all()
                 assert np.allclose(R[2], nrm / np.linalg.norm(nrm), atol=1e-12)
                 assert np.allclose(R[0], t0 / np.linalg.norm(t0), atol=1e-12)
                 assert np.allclose(R[1], t1 / np.linalg.norm(t1), atol=1e-12)
-                off += 14
+                off += 15
             assert abs(s - area) <= 1e-12 * area
             checked += 1
     assert checked > 0
+
+
+def test_cohquad4_geometry():
+    """CohQuad4 (a tet4 cut two nodes against two, src/CohQuad4.cpp): its four
+    qps are the tensor product of the edge's 2-point Gauss rule with weight 1;
+    the lower-face quad4 shapes are a partition of unity; the last row of R is
+    the unit normal x_r cross x_s and the first two rows x_r, x_s normalised
+    (compute_rotation_general, src/CohesiveElem.cpp:207-238); the qps' w dA sum
+    to the area of the bilinear face up to the 2 x 2 rule's error on a warped
+    face (fine midpoint rule, 1e-2; a wrong weight or point set is off by O(1))."""
+    p = RVEProblem(small_config("tet-op-q", n=8, n_incl=1))
+    o = Oracle(p)
+    g = 1.0 / np.sqrt(3.0)
+    checked = 0
+    for ci in range(p.n_cut):
+        rec = p.cut[ci]
+        if rec.coh_nn != 8:
+            continue
+        e = int(p.cut_elem[ci])
+        local = list(p.conn[e]) + [rec.enr[k] for k in range(rec.n_enr)]
+        nd = 3 * len(local)
+        cache = elem_cache(o, e)
+        off = rec.n_vqp * (2 + 6 * nd)
+        X = np.array([p.coords[local[rec.coh_nodes[0][n]]] for n in range(4)])
+
+        def frame(r, s_):
+            dN = 0.25 * np.array([[-(1 - s_), -(1 - r)], [1 - s_, -(1 + r)], [1 + s_, 1 + r], [-(1 + s_), 1 - r]])
+            J = dN.T @ X
+            return J[0], J[1], np.cross(J[0], J[1])
+        s = 0.0
+        for q in range(4):
+            W, J = cache[off], cache[off + 1]
+            N = cache[off + 2: off + 6]
+            R = cache[off + 6: off + 15].reshape(3, 3)
+            r, s_ = (g if q % 2 else -g), (g if q // 2 else -g)
+            xr, xs, nr = frame(r, s_)
+            assert W == 1.0 and abs(N.sum() - 1.0) < 1e-15 and (N > 0).all()
+            assert abs(N[0] - 0.25 * (1 - r) * (1 - s_)) < 1e-15
+            assert abs(J - np.linalg.norm(nr)) < 1e-14 * np.linalg.norm(nr)
+            assert np.allclose(R[2], nr / np.linalg.norm(nr), atol=1e-12)
+            assert np.allclose(R[0], xr / np.linalg.norm(xr), atol=1e-12)
+            assert np.allclose(R[1], xs / np.linalg.norm(xs), atol=1e-12)
+            s += W * J
+            off += 15
+        m = 200
+        t = -1 + (np.arange(m) + 0.5) * (2.0 / m)
+        area = sum(np.linalg.norm(frame(a, b)[2]) for a in t for b in t) * (2.0 / m) ** 2
+        assert abs(s - area) <= 1e-2 * area
+        checked += 1
+    assert checked >= 4
 
 
 def _voigt(G):
@@ -635,7 +689,7 @@ def test_cohesive_integration_with_a_linear_law(kind):
             s += cache[off] * cache[off + 1]
             if dim == 2:
                 assert cache[off] == 1.0 and abs(cache[off + 1] - 0.5 * size) <= 1e-14 * size
-            off += 2 + 3 + dim * dim
+            off += 2 + 4 + dim * dim
         assert abs(s - size) <= 1e-13 * size
         # a small jump, opening along the normal (linear branch, no contact)
         v = 1e-5 * (R[-1] + 0.3 * R[0])
