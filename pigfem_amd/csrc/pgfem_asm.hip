@@ -265,19 +265,24 @@ __device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3],
     return transform<D>(cnn, cdN, cx, cg);
 }
 
+// Lower nodes of a cohesive child: CohEdge2 in 2D, CohTri3 or CohQuad4 in 3D.
+template <int D> struct CohLim { static constexpr int N = D == 2 ? 2 : 4; };
+
 // Cohesive qp geometry: N, rotation R and dA on the lower surface
 // (CohesiveElem::ShapeFunctions / compute_rotation_general, src/CohesiveElem.cpp:145-238;
-// CohEdge2 2-pt Gauss src/CohEdge2.cpp:70-78,186-209; CohTri3 tri3 4-pt rule).
+// CohEdge2 2-pt Gauss src/CohEdge2.cpp:70-78,186-209; CohTri3 tri3 4-pt rule;
+// CohQuad4 2 x 2 Gauss).
 template <int D>
 __device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], double* N, double R[D][D], double& w) {
-    double dNr[4][2];
+    double dNr[CohLim<D>::N][2];
     const int cnn = coh_nn / 2;
-    if (coh_nn == 4) {
+    if constexpr (D == 2) {
         const double xi = (q == 0) ? -PG_GAUSS2 : PG_GAUSS2;
         N[0] = 0.5 * (1.0 - xi); N[1] = 0.5 * (1 + xi);
         dNr[0][0] = -0.5; dNr[1][0] = 0.5;
         w = 1.0;
     } else if (coh_nn == 8) {
+      if constexpr (D == 3) {
         // CohQuad4 (src/CohQuad4.cpp:58-140): the tensor product of the edge's
         // 2-pt Gauss rule, qp q at (xi_{q % 2}, xi_{q / 2}), w = 1 * 1; quad4 shapes
         const double r = (q & 1) ? PG_GAUSS2 : -PG_GAUSS2, s = (q & 2) ? PG_GAUSS2 : -PG_GAUSS2;
@@ -288,11 +293,14 @@ __device__ inline double cohesive_qp(int coh_nn, int q, const double (*lx)[3], d
         dNr[2][0] = 0.25 * (1.0 + s);  dNr[2][1] = 0.25 * (1.0 + r);
         dNr[3][0] = -0.25 * (1.0 + s); dNr[3][1] = 0.25 * (1.0 - r);
         w = 1.0 * 1.0;
+      }
     } else {
+      if constexpr (D == 3) {
         const double qx = pick(TRI_QX, q), qy = pick(TRI_QY, q);
         N[0] = 1.0 - qx - qy; N[1] = qx; N[2] = qy;
         dNr[0][0] = -1.0; dNr[0][1] = -1.0; dNr[1][0] = 1.0; dNr[1][1] = 0.0; dNr[2][0] = 0.0; dNr[2][1] = 1.0;
         w = pick(TRI_QW, q);
+      }
     }
     double Jc[2][3] = {{0, 0, 0}, {0, 0, 0}};
     for (int n = 0; n < cnn; ++n)
@@ -496,7 +504,8 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     __shared__ double wq[MAXQ];            // J w of each volumetric qp
     __shared__ double sq[MAXQ][NS];        // stress at each volumetric qp
     __shared__ int mq[MAXQ];               // material of each volumetric qp (its child's)
-    __shared__ double cN[MAXC][4], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
+    constexpr int CNN = CohLim<D>::N;
+    __shared__ double cN[MAXC][CNN], cw[MAXC], cRDR[MAXC][D][D], ctR[MAXC][D];
     __shared__ pg_cut_rec rec;
     const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
     stage_dmats<NS>(p, sD);
@@ -570,8 +579,8 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
     const int ncq = rec.n_coh * cnq;
     for (int cq = lane; cq < ncq; cq += 64) {
         const int k = cq / cnq, q = cq % cnq;
-        double lo[4][3], ulo[12], uup[12], N[4], R[D][D], w, RDR[D][D], tR[D];
-        for (int n = 0; n < 4; ++n)
+        double lo[CNN][3], ulo[CNN * D], uup[CNN * D], N[CNN], R[D][D], w, RDR[D][D], tR[D];
+        for (int n = 0; n < CNN; ++n)
             for (int i = 0; i < D; ++i) {
                 const bool in = n < cnn;
                 lo[n][i] = in ? lx[rec.coh_nodes[k][n]][i] : 0.0;
@@ -587,7 +596,7 @@ __global__ __launch_bounds__(64) void k_cut_plain(DevProb p, const double* __res
             for (int j = 0; j < D; ++j) cRDR[cq][i][j] = RDR[i][j];
             ctR[cq][i] = tR[i];
         }
-        for (int n = 0; n < 4; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
+        for (int n = 0; n < CNN; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
         cw[cq] = dA * w;
     }
     __syncthreads();
@@ -743,7 +752,8 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
     __shared__ double wq[MAXQ];            // J w of each volumetric qp
     __shared__ double sq[MAXQ][NS];        // dstress at each volumetric qp (0 where not visited)
     __shared__ int bq[MAXQ];               // the qp whose B each qp uses
-    __shared__ double cN[MAXC][4], cw[MAXC], ctR[MAXC][D];
+    constexpr int CNN = CohLim<D>::N;
+    __shared__ double cN[MAXC][CNN], cw[MAXC], ctR[MAXC][D];
     __shared__ pg_cut_rec rec;
     const int c = list ? list[blockIdx.x] : int(blockIdx.x), lane = threadIdx.x;
     constexpr int RW = int(sizeof(pg_cut_rec) / 4);
@@ -831,8 +841,8 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
         const pg_material& m = p.mats[rec.coh_mat[k]];
         const bool right = rec.coh_mat[k] == s.mat;
         const bool visit = right || m.type == PG_MAT_OP || m.type == PG_MAT_BL;
-        double lo[4][3], N[4], R[D][D], w, tR[D];
-        for (int n = 0; n < 4; ++n)
+        double lo[CNN][3], N[CNN], R[D][D], w, tR[D];
+        for (int n = 0; n < CNN; ++n)
             for (int i = 0; i < D; ++i) lo[n][i] = n < cnn ? lx[rec.coh_nodes[k][n]][i] : 0.0;
         const double dA = cohesive_qp<D>(rec.coh_nn, q, lo, N, R, w);
         for (int i = 0; i < D; ++i) tR[i] = 0.0;
@@ -859,7 +869,7 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
             }
         }
         for (int i = 0; i < D; ++i) ctR[cq][i] = tR[i];
-        for (int n = 0; n < 4; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
+        for (int n = 0; n < CNN; ++n) cN[cq][n] = n < cnn ? N[n] : 0.0;
         cw[cq] = dA * w;
     }
     __syncthreads();
@@ -906,7 +916,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                                             const double* __restrict__ isv_c, double* __restrict__ isv_t,
                                             TK* __restrict__ Kst, double* __restrict__ Pst, int* __restrict__ err) {
     constexpr int NS = (D == 2 ? 3 : 6), NEN = D + 1, GS = 64 / EPW;
-    constexpr int CUT_MAXC = MAXC;
+    // MAXC == 0: a mesh without cohesive children (no cohesive qp phase, fold or
+    // LDS arrays)
+    constexpr bool COH = MAXC > 0;
+    constexpr int CUT_MAXC = COH ? MAXC : 1;
     __shared__ double xs[EPW][MAXN][3];
     __shared__ double us[EPW][MAXN * D];
     // gradients: a parent node's is the same at every qp (linear simplex
@@ -922,7 +935,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     // nodes of the largest child
     constexpr int CN = (D == 2 && MAXQ > 8) ? 6 : 4;
     __shared__ double qs[EPW][MAXQ][NS];
-    __shared__ double cN[EPW][CUT_MAXC][4];
+    __shared__ double cN[EPW][CUT_MAXC][CohLim<D>::N];
     __shared__ double cw[EPW][CUT_MAXC];
     __shared__ double cRDR[EPW][CUT_MAXC][D][D];
     __shared__ double ctR[EPW][CUT_MAXC][D];
@@ -972,8 +985,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         if (mm < MAX_LDS_MATS) sLC[mm][mj] = v;
         if (mm < MAX_LDS_MATS && mj == 1) sLM[mm][0] = TK(v);  // D[0][1] = lambda
         if (mm < MAX_LDS_MATS && mj == 2) sLM[mm][1] = TK(v);  // D[last][last] = mu
-        if (threadIdx.x < 48) (&sCM[cm].sigma_c)[cf] = cv0;
-        if (threadIdx.x < MAX_LDS_MATS) sCM[ct8].type = cty0;
+        if (COH && threadIdx.x < 48) (&sCM[cm].sigma_c)[cf] = cv0;
+        if (COH && threadIdx.x < MAX_LDS_MATS) sCM[ct8].type = cty0;
     }
 #pragma unroll
     for (int k = 0; k < RWL; ++k) {
@@ -996,11 +1009,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     const int nvq = live ? rec.n_vqp : 0;
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
-    const int ncq = live ? rec.n_coh * cnq : 0;
+    const int ncq = (COH && live) ? rec.n_coh * cnq : 0;
     // the committed history of this lane's first cohesive qp goes out before
     // the volumetric work (any valid index when the element has none)
     double hist0 = 0.0;
-    if (isv_c && p.n_cqp > 0) hist0 = isv_c[ncq > 0 ? rec.isv_off + (gl < ncq ? gl : ncq - 1) : 0];
+    if (COH && isv_c && p.n_cqp > 0) hist0 = isv_c[ncq > 0 ? rec.isv_off + (gl < ncq ? gl : ncq - 1) : 0];
     for (int t = gl; t < nvq; t += GS) {
         {
             // ---- volumetric qp ----
@@ -1063,12 +1076,14 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             for (int s = 0; s < NS; ++s) qs[g][t][s] = sig[s];
         }
     }
+    if constexpr (COH)
     for (int cq = gl; cq < ncq; cq += GS) {
         {
             // ---- cohesive qp ----
             const int k = cq / cnq, q = cq % cnq;
-            double lo[4][3], ulo[12], uup[12], N[4], R[D][D], w, RDR[D][D], tR[D];
-            for (int n = 0; n < 4; ++n)
+            constexpr int CNN = CohLim<D>::N;
+            double lo[CNN][3], ulo[CNN * D], uup[CNN * D], N[CNN], R[D][D], w, RDR[D][D], tR[D];
+            for (int n = 0; n < CNN; ++n)
                 for (int i = 0; i < D; ++i) {
                     const bool in = n < cnn;
                     lo[n][i] = in ? xs[g][rec.coh_nodes[k][n]][i] : 0.0;
@@ -1084,7 +1099,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
                 for (int j = 0; j < D; ++j) cRDR[g][cq][i][j] = RDR[i][j];
                 ctR[g][cq][i] = tR[i];
             }
-            for (int n = 0; n < 4; ++n) cN[g][cq][n] = n < cnn ? N[n] : 0.0;
+            for (int n = 0; n < CNN; ++n) cN[g][cq][n] = n < cnn ? N[n] : 0.0;
             cw[g][cq] = dA * w;
         }
     }
@@ -1169,6 +1184,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
         TK coh[D][D];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) coh[i][j] = TK(0);
+        if constexpr (COH)
         for (int k = 0; k < rec.n_coh; ++k) {
             int m1 = -1, m2 = -1;
             for (int m = 0; m < rec.coh_nn; ++m) {
@@ -1208,6 +1224,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             a += qw[g][q] * pt;
         }
         double pc = 0.0;
+        if constexpr (COH)
         for (int k = 0; k < rec.n_coh; ++k)
             for (int m = 0; m < rec.coh_nn; ++m)
                 if (rec.coh_nodes[k][m] == l) {
@@ -3633,8 +3650,19 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
         const int n0 = h->n_class[0], n1 = h->n_class[1];
         const int l0 = 0, l1 = n0;  // class positions of the two classes
         const int cv = h->cut_variant;
-#define CUT_SMALL2(TK, KST, EPW, OCC) \
-    k_cut<2, TK, EPW, 7, 8, 2, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
+        // a mesh without cohesive qps (n_cqp == 0: check_desc then admits no
+        // cohesive child) launches the instances with the cohesive code compiled out
+        const bool coh = d.n_cqp > 0;
+#define CUT_SMALL2(TK, KST, EPW, OCC)                                                                             \
+    (coh ? k_cut<2, TK, EPW, 7, 8, 2, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, \
+                                                                               h->Pc, h->err)                    \
+         : k_cut<2, TK, EPW, 7, 8, 0, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, \
+                                                                               h->Pc, h->err))
+#define CUT_GEN(D_, TK, KST, EPW, MAXN, MAXQ)                                                                       \
+    (coh ? k_cut<D_, TK, EPW, MAXN, MAXQ, PG_MAX_COH * 4, 1><<<(n1 + EPW - 1) / EPW, 64, 0, st>>>(                 \
+               d, l1, n1, h->Un, dIc, dIt, KST, h->Pc, h->err)                                                     \
+         : k_cut<D_, TK, EPW, MAXN, MAXQ, 0, 1><<<(n1 + EPW - 1) / EPW, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, KST, \
+                                                                                   h->Pc, h->err))
         if (d.dim == 2) {
             if (n0) {
                 if (fp32 && !cut64) {
@@ -3650,14 +3678,17 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
                 }
             }
             if (n1) {
-                if (fp32 && !cut64) k_cut<2, float, 2, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 1) / 2, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<2, double, 2, MAXN2, MAXQ2, PG_MAX_COH * 4, 1><<<(n1 + 1) / 2, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32 && !cut64) CUT_GEN(2, float, h->Kc32, 2, MAXN2, MAXQ2);
+                else CUT_GEN(2, double, h->Kc64, 2, MAXN2, MAXQ2);
             }
         } else {
             // small class: elements per wave and the register cap (waves per
             // EU) by the cut variant
-#define CUT_SMALL3(TK, KST, EPW, OCC) \
-    k_cut<3, TK, EPW, 8, 30, 4, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST, h->Pc, h->err)
+#define CUT_SMALL3(TK, KST, EPW, OCC)                                                                              \
+    (coh ? k_cut<3, TK, EPW, 8, 30, 4, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,  \
+                                                                                h->Pc, h->err)                     \
+         : k_cut<3, TK, EPW, 8, 30, 0, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,  \
+                                                                                h->Pc, h->err))
             if (n0) {
                 if (fp32 && !cut64) {
                     if (cv == 1) CUT_SMALL3(float, h->Kc32, 2, 3);
@@ -3673,11 +3704,12 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             }
 #undef CUT_SMALL3
             if (n1) {
-                if (fp32 && !cut64) k_cut<3, float, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc32, h->Pc, h->err);
-                else k_cut<3, double, 1, MAXN3, MAXQ3, PG_MAX_COH * 4, 1><<<n1, 64, 0, st>>>(d, l1, n1, h->Un, dIc, dIt, h->Kc64, h->Pc, h->err);
+                if (fp32 && !cut64) CUT_GEN(3, float, h->Kc32, 1, MAXN3, MAXQ3);
+                else CUT_GEN(3, double, h->Kc64, 1, MAXN3, MAXQ3);
             }
         }
 #undef CUT_SMALL2
+#undef CUT_GEN
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
         if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
