@@ -904,11 +904,16 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
 // Cut-element kernel of the class-binned schedule (src/Mesh.cpp:1521-1641 +
 // src/Assembler.cpp:223-365).  EPW intersected elements share one wavefront, a
 // group of 64/EPW lanes each:
-//   phase 1: one lane per child qp computes the child Jacobian in FP64 (sliver
-//            children), the gradients of all local nodes, wJ and the FP64 stress;
-//            one lane per cohesive qp runs the FP64 law and the history update;
-//   phase 2: every lane accumulates its K entries b_i . (wJ D) b_j over the qps in
-//            TK, adds the cohesive fold and stores; P (FP64) likewise.
+//   phase 1: one lane per volumetric unit computes the child Jacobian in FP64
+//            (sliver children), the gradients of all local nodes, wJ and the
+//            FP64 stress; one lane per cohesive qp runs the FP64 law and the
+//            history update;
+//   phase 2: every lane accumulates its K entries b_i . (wJ D) b_j over the units
+//            in TK, adds the cohesive fold and stores; P (FP64) likewise.
+// A unit is a child qp in 2D.  In 3D every child is a linear tet4, whose
+// gradients, Jacobian and so strain and stress are the same at each of its five
+// qps: the unit is the child, weighted by J times the sum of the rule's weights
+// (as the node-group kernels weight an uncut tet4), and MAXQ counts children.
 // --------------------------------------------------------------------------
 template <int D, typename TK, int EPW, int MAXN, int MAXQ, int MAXC, int OCC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(OCC))) k_cut(DevProb p, int cbase, int n_list,
@@ -1006,7 +1011,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     const int nloc = live ? NEN + rec.n_enr : 0;
     const int nd = nloc * D;
 
-    const int nvq = live ? rec.n_vqp : 0;
+    const int nvq = live ? (D == 3 ? rec.n_child : rec.n_vqp) : 0;  // volumetric units
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
     const int ncq = (COH && live) ? rec.n_coh * cnq : 0;
@@ -1016,12 +1021,17 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     if (COH && isv_c && p.n_cqp > 0) hist0 = isv_c[ncq > 0 ? rec.isv_off + (gl < ncq ? gl : ncq - 1) : 0];
     for (int t = gl; t < nvq; t += GS) {
         {
-            // ---- volumetric qp ----
+            // ---- volumetric unit: child k's qp q (2D) or child k (3D) ----
             int k = 0, q = t;
-            for (; k < rec.n_child; ++k) {
-                const int nq = PG_CHILD_NQP(rec.child_type[k]);
-                if (q < nq) break;
-                q -= nq;
+            if constexpr (D == 3) {
+                k = t;
+                q = 0;
+            } else {
+                for (; k < rec.n_child; ++k) {
+                    const int nq = PG_CHILD_NQP(rec.child_type[k]);
+                    if (q < nq) break;
+                    q -= nq;
+                }
             }
             const int ct = rec.child_type[k];
             const int ccn = PG_CHILD_NN(ct);
@@ -1033,6 +1043,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             }
             const double Jc = child_qp<D, CN == 6>(ct, q, ccn, cx, cg, w);
             if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
+            if constexpr (D == 3) {
+                w = 0.0;
+                for (int r = 0; r < 5; ++r) w += TET_QW[r];
+            }
             double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
             for (int n = 0; n < NEN; ++n) {
                 for (int i = 0; i < D; ++i) px[n][i] = xs[g][n][i];
@@ -3685,27 +3699,27 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
             // small class: elements per wave and the register cap (waves per
             // EU) by the cut variant
 #define CUT_SMALL3(TK, KST, EPW, OCC)                                                                              \
-    (coh ? k_cut<3, TK, EPW, 8, 30, 4, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,  \
-                                                                                h->Pc, h->err)                     \
-         : k_cut<3, TK, EPW, 8, 30, 0, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,  \
-                                                                                h->Pc, h->err))
+    (coh ? k_cut<3, TK, EPW, 8, 6, 4, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,   \
+                                                                               h->Pc, h->err)                      \
+         : k_cut<3, TK, EPW, 8, 6, 0, OCC><<<(n0 + EPW - 1) / EPW, 64, 0, st>>>(d, l0, n0, h->Un, dIc, dIt, KST,   \
+                                                                               h->Pc, h->err))
             if (n0) {
                 if (fp32 && !cut64) {
-                    if (cv == 1) CUT_SMALL3(float, h->Kc32, 2, 3);
-                    else if (cv == 2) CUT_SMALL3(float, h->Kc32, 1, 1);
-                    else if (cv == 3) CUT_SMALL3(float, h->Kc32, 2, 2);
-                    else CUT_SMALL3(float, h->Kc32, 2, 1);
+                    if (cv == 1) CUT_SMALL3(float, h->Kc32, 4, 3);
+                    else if (cv == 2) CUT_SMALL3(float, h->Kc32, 2, 1);
+                    else if (cv == 3) CUT_SMALL3(float, h->Kc32, 8, 1);
+                    else CUT_SMALL3(float, h->Kc32, 4, 1);
                 } else {
-                    if (cv == 1) CUT_SMALL3(double, h->Kc64, 2, 3);
-                    else if (cv == 2) CUT_SMALL3(double, h->Kc64, 1, 1);
-                    else if (cv == 3) CUT_SMALL3(double, h->Kc64, 2, 2);
-                    else CUT_SMALL3(double, h->Kc64, 2, 1);
+                    if (cv == 1) CUT_SMALL3(double, h->Kc64, 4, 3);
+                    else if (cv == 2) CUT_SMALL3(double, h->Kc64, 2, 1);
+                    else if (cv == 3) CUT_SMALL3(double, h->Kc64, 8, 1);
+                    else CUT_SMALL3(double, h->Kc64, 4, 1);
                 }
             }
 #undef CUT_SMALL3
             if (n1) {
-                if (fp32 && !cut64) CUT_GEN(3, float, h->Kc32, 1, MAXN3, MAXQ3);
-                else CUT_GEN(3, double, h->Kc64, 1, MAXN3, MAXQ3);
+                if (fp32 && !cut64) CUT_GEN(3, float, h->Kc32, 1, MAXN3, PG_MAX_CHILD);
+                else CUT_GEN(3, double, h->Kc64, 1, MAXN3, PG_MAX_CHILD);
             }
         }
 #undef CUT_SMALL2
