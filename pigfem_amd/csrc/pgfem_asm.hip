@@ -2704,7 +2704,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const double* __restrict__ Un, double* __restrict__ K, double* __restrict__ P, unsigned flags, int ngroups,
     int per) {
     constexpr int D = 3, NEN = 4, NQ = 5;
-    constexpr int MAXI = G3_NI * NG, MAXP = G3_NP * NG;
+    constexpr int MAXI = G3_NI * NG;
     static_assert(NG <= 32, "the first wavefront holds the group's nodes");
     __shared__ int s_node[NG], s_kb[NG], s_dof[NG], s_eb[NG], s_S[NG];
     __shared__ TC s_g[MAXI][NEN][D];
@@ -2800,29 +2800,23 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         __syncthreads();
 
         // ---- phase B: node pairs ----
-        for (int kq = tid; kq < MAXP; kq += BS) {
-            const int lo = kq / G3_NP, slot = kq - lo * G3_NP;
-            if (s_node[lo] < 0) continue;
-            double acc[D][D];
-            for (int i = 0; i < D; ++i)
-                for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
-            double pa[D] = {0.0, 0.0, 0.0};
-            const bool isdiag = slot == sT.diag;
-            if (isdiag) {
-                // every incidence of the node, in ascending element order
-                const int t0 = lo * G3_NI;
-#pragma unroll 4
-                for (int t = t0; t < t0 + G3_NI; ++t) {
-                    if (func)
-                        for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-                    if (jac)
-                        for (int i = 0; i < D; ++i)
-                            for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[t][i * D + j];
-                }
-            } else if (jac) {
+        // lanes [0, NOFF): one off-diagonal pair each (node lo, its k-th
+        // off-diagonal slot), the closed-form blocks of its contributions;
+        // lanes [DB, DB + NDE): one entry each of a node's diagonal block
+        // (c < 9) or internal force (c >= 9), summed over its incidences.  DB
+        // starts a wavefront, so no wavefront runs both kinds of work.
+        constexpr int NOFF = (G3_NP - 1) * NG, DB = (NOFF + 63) / 64 * 64, NDE = (D * D + D) * NG;
+        for (int kq = tid; kq < DB + NDE; kq += BS) {
+            if (kq < NOFF) {
+                const int lo = kq / (G3_NP - 1), k = kq - lo * (G3_NP - 1);
+                if (s_node[lo] < 0 || !jac) continue;
+                const int slot = k < sT.diag ? k : k + 1;
+                double acc[D][D];
+                for (int i = 0; i < D; ++i)
+                    for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
 #pragma unroll
-                for (int k = 0; k < G3_CAP; ++k) {
-                    const unsigned code = sT.pc[slot][k];
+                for (int r = 0; r < G3_CAP; ++r) {
+                    const unsigned code = sT.pc[slot][r];
                     if (code == 0xffu) continue;
                     const int t = lo * G3_NI + int(code >> 2), lb = int(code & 3);
                     TC blk[D][D];
@@ -2830,14 +2824,25 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     for (int i = 0; i < D; ++i)
                         for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
                 }
-            }
-            if (jac) {
                 const int64_t k0v = int64_t(s_kb[lo]) + 3 * slot;
                 for (int i = 0; i < D; ++i)
                     for (int j = 0; j < D; ++j) K[k0v + int64_t(i) * 3 * G3_NP + j] = acc[i][j];
+            } else if (kq >= DB) {
+                const int e = kq - DB, lo = e / (D * D + D), c = e - lo * (D * D + D);
+                if (s_node[lo] < 0 || (c < D * D ? !jac : !func)) continue;
+                // every incidence of the node, in ascending element order
+                const int t0 = lo * G3_NI;
+                double a = 0.0;
+                if (c < D * D) {
+#pragma unroll 8
+                    for (int t = t0; t < t0 + G3_NI; ++t) a += s_dblk[t][c];
+                    K[int64_t(s_kb[lo]) + 3 * sT.diag + int64_t(c / D) * 3 * G3_NP + c % D] = a;
+                } else {
+#pragma unroll 8
+                    for (int t = t0; t < t0 + G3_NI; ++t) a += s_P[t][c - D * D];
+                    P[s_dof[lo] + c - D * D] = a;
+                }
             }
-            if (isdiag && func)
-                for (int i = 0; i < D; ++i) P[s_dof[lo] + i] = pa[i];
         }
         __syncthreads();  // the next group rewrites the tables
     }
