@@ -378,8 +378,11 @@ int pg_asm_sizes(const pg_asm* h, int64_t* n_dof, int64_t* nnz_total,
  *   kernel_ms[1] U permutation + element (plain) or cut-element (binned) kernels,
  *   kernel_ms[2] the node-pair kernels ([3] + [4]),
  *   kernel_ms[3] binned: the node groups without cut incidences, which run on
- *                the handle's second stream during the cut kernels (0 if none),
- *   kernel_ms[4] the remaining node groups (plain: the staged node-pair kernel). */
+ *                the handle's second stream alongside the cut kernels and the
+ *                remaining groups (0 if none),
+ *   kernel_ms[4] the remaining node groups, which start when the cut kernels
+ *                end (plain: the staged node-pair kernel).
+ * With the second stream, [3] and [4] overlap in time and [2] is their sum. */
 int pg_asm_last_timing(const pg_asm* h, double kernel_ms[5]);
 int pg_asm_set_timing(pg_asm* h, int enable);
 

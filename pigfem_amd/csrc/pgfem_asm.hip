@@ -2879,6 +2879,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ double s_P[MAXI][D];
     __shared__ double s_dblk[MAXI][D * D];    // diagonal block (la, la) of each incidence
     __shared__ TC sLM[MAX_LDS_MATS][2];       // lambda, mu of each LEI material
+    __shared__ int s_dq[NG];                  // local index of each node's diagonal pair (-1: none)
     __shared__ int s_bad;
 
     const bool func = (flags & PG_FUNC) != 0, jac = (flags & PG_JAC) != 0;
@@ -2911,6 +2912,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         s_qbeg[tid] = r.z;
         s_iptr[tid + 1] = ci;
         s_qptr[tid + 1] = cq;
+        s_dq[tid] = -1;
     }
     if (tid == 0) {
         s_iptr[0] = 0;
@@ -3011,29 +3013,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const bool use_lists = staged && !s_bad;
 
     // ---- phase B: node pairs ----
+    // Staged: the off-diagonal pairs first (a diagonal pair's lane only notes
+    // its index), then one lane per entry of a node's diagonal block or internal
+    // force, summed over the node's incidences -- so that no wavefront runs both
+    // the contribution blocks and those sums.
     for (int kq = threadIdx.x; kq < nq; kq += blockDim.x) {
         const PairMeta<D> pm = (D == 2 && kq == int(threadIdx.x)) ? pm0 : pair_meta<D, NG>(p, s_qptr, s_qbeg, s_node, ng, kq, jac);
         const int lo = pm.lo, a = pm.a, q = pm.q, b = pm.b;
+        if (use_lists && a == b) {
+            s_dq[lo] = kq;
+            continue;
+        }
         const bool diag = (a == b) && func;
         double acc[D][D];
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
         double pa[D];
         for (int i = 0; i < D; ++i) pa[i] = 0.0;
-        if (use_lists && a == b) {
-            // diagonal pair: every incidence of a, in ascending element order
-            // (an incidence without a row holds zeros: no test, so that the
-            // LDS reads of later incidences go out early)
-            const int t0 = s_iptr[lo], t1 = s_iptr[lo + 1];
-#pragma unroll 4
-            for (int t = t0; t < t1; ++t) {
-                if (func)
-                    for (int i = 0; i < D; ++i) pa[i] += s_P[t][i];
-                if (jac)
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += s_dblk[t][i * D + j];
-            }
-        } else if (use_lists) {
+        if (use_lists) {
             // the slot's contributions, stored by rank = ascending element order
             const int s = kq;
             if (jac) {
@@ -3082,6 +3079,29 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (diag)
             for (int i = 0; i < D; ++i) P[pm.dof[i]] = pa[i];
     }
+    if (!use_lists) return;
+    __syncthreads();
+    // diagonal entries: every incidence of the node in ascending element order
+    // (an incidence without a row holds zeros: no test, so that the LDS reads of
+    // later incidences go out early)
+    constexpr int NDE = D * D + D;
+    for (int e = threadIdx.x; e < ng * NDE; e += blockDim.x) {
+        const int lo = e / NDE, c = e - lo * NDE;
+        const int kq = s_dq[lo];
+        if (kq < 0 || (c < D * D ? !jac : !func)) continue;
+        const int t0 = s_iptr[lo], t1 = s_iptr[lo + 1];
+        const int64_t q = int64_t(s_qbeg[lo]) + (kq - s_qptr[lo]);
+        double a = 0.0;
+        if (c < D * D) {
+#pragma unroll 8
+            for (int t = t0; t < t1; ++t) a += s_dblk[t][c];
+            K[p.npat_map[q * D * D + c]] = a;
+        } else {
+#pragma unroll 8
+            for (int t = t0; t < t1; ++t) a += s_P[t][c - D * D];
+            P[p.node_dof[int64_t(s_node[lo]) * D + c - D * D]] = a;
+        }
+    }
 }
 
 }  // namespace
@@ -3105,7 +3125,7 @@ struct pg_asm {
     // host-API buffers
     double *dU = nullptr, *dK = nullptr, *dP = nullptr, *dIc = nullptr, *dIt = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t side = nullptr;     // second stream: node groups without cut elements overlap the cut kernel
     hipEvent_t sync[2] = {nullptr, nullptr};
     // completion of the handle's last call that used its scratch (Un, staged
@@ -3730,16 +3750,20 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
 #undef CUT_SMALL2
 #undef CUT_GEN
         if (h->timing) HIP_OK(hipEventRecord(h->ev[1], st));
-        if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
         if (h->timing) HIP_OK(hipEventRecord(h->ev[5], st));
         // the groups of enrichment nodes (the tail of the list) may take twice
         // the threads per pair (group variant bit 4): they have few incidences
-        // and many pairs
+        // and many pairs.  With overlap they start as soon as the cut kernels
+        // end, while the side stream's groups may still run (the two sets of
+        // groups write disjoint rows of K and entries of P); st joins the side
+        // stream after them.
         if (!overlap && grid) grid_groups(orr, st, tv0);
         const int first = overlap ? h->nclean[orr] : skip(orr), nenr = h->nenrg[orr];
         const int nreg = h->ngroups[orr] - first - nenr;
         groups(orr, st, first, nreg, tv0);
         groups(orr, st, first + nreg, nenr, tv0 || (h->group_variant & 16) != 0);
+        if (h->timing) HIP_OK(hipEventRecord(h->ev[6], st));
+        if (overlap) HIP_OK(hipStreamWaitEvent(st, h->sync[1], 0));
     }
     HIP_OK(hipGetLastError());
     if (h->timing) HIP_OK(hipEventRecord(h->ev[2], st));
@@ -4295,7 +4319,7 @@ int pg_asm_assemble_device(pg_asm* h, const double* dU, const double* dIc, doubl
         float tot = 0, cut = 0, dirty = 0, clean = 0;
         HIP_OK(hipEventElapsedTime(&tot, h->ev[0], h->ev[2]));
         HIP_OK(hipEventElapsedTime(&cut, h->ev[0], h->ev[1]));
-        HIP_OK(hipEventElapsedTime(&dirty, h->ev[5], h->ev[2]));
+        HIP_OK(hipEventElapsedTime(&dirty, h->ev[5], (flags & (PG_FP32 | PG_BATCHED)) ? h->ev[6] : h->ev[2]));
         if (ov) HIP_OK(hipEventElapsedTime(&clean, h->ev[3], h->ev[4]));
         h->last_ms[0] = tot;
         h->last_ms[1] = cut;
