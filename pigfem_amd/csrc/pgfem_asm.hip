@@ -265,6 +265,12 @@ __device__ inline double child_qp(int ct, int q, int cnn, const double (*cx)[3],
     return transform<D>(cnn, cdN, cx, cg);
 }
 
+// Volumetric units of a child in the binned cut kernel: one for a linear simplex
+// (constant gradients), one per qp otherwise.
+__host__ __device__ inline int cut_units(int ct) {
+    return (ct == PG_CHILD_TRI3 || ct == PG_CHILD_TET4) ? 1 : PG_CHILD_NQP(ct);
+}
+
 // Lower nodes of a cohesive child: CohEdge2 in 2D, CohTri3 or CohQuad4 in 3D.
 template <int D> struct CohLim { static constexpr int N = D == 2 ? 2 : 4; };
 
@@ -910,10 +916,10 @@ __global__ __launch_bounds__(64) void k_sens_cut(DevProb p, const double* __rest
 //            history update;
 //   phase 2: every lane accumulates its K entries b_i . (wJ D) b_j over the units
 //            in TK, adds the cohesive fold and stores; P (FP64) likewise.
-// A unit is a child qp in 2D.  In 3D every child is a linear tet4, whose
-// gradients, Jacobian and so strain and stress are the same at each of its five
-// qps: the unit is the child, weighted by J times the sum of the rule's weights
-// (as the node-group kernels weight an uncut tet4), and MAXQ counts children.
+// A linear simplex child (tri3, tet4) has the same gradients, Jacobian and so
+// strain and stress at each of its qps: it is one unit, weighted by J times the
+// sum of its rule's weights (as the node-group kernels weight an uncut element).
+// A quad4 or polygon child contributes one unit per qp.  MAXQ counts units.
 // --------------------------------------------------------------------------
 template <int D, typename TK, int EPW, int MAXN, int MAXQ, int MAXC, int OCC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(OCC))) k_cut(DevProb p, int cbase, int n_list,
@@ -1011,7 +1017,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     const int nloc = live ? NEN + rec.n_enr : 0;
     const int nd = nloc * D;
 
-    const int nvq = live ? (D == 3 ? rec.n_child : rec.n_vqp) : 0;  // volumetric units
+    int nvq = 0;  // volumetric units
+    if (live)
+        for (int k = 0; k < rec.n_child; ++k) nvq += cut_units(rec.child_type[k]);
     const int cnn = rec.coh_nn / 2;
     const int cnq = (rec.coh_nn == 4) ? 2 : 4;
     const int ncq = (COH && live) ? rec.n_coh * cnq : 0;
@@ -1021,16 +1029,16 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
     if (COH && isv_c && p.n_cqp > 0) hist0 = isv_c[ncq > 0 ? rec.isv_off + (gl < ncq ? gl : ncq - 1) : 0];
     for (int t = gl; t < nvq; t += GS) {
         {
-            // ---- volumetric unit: child k's qp q (2D) or child k (3D) ----
+            // ---- volumetric unit: child k (simplex) or child k's qp q ----
             int k = 0, q = t;
             if constexpr (D == 3) {
                 k = t;
                 q = 0;
             } else {
                 for (; k < rec.n_child; ++k) {
-                    const int nq = PG_CHILD_NQP(rec.child_type[k]);
-                    if (q < nq) break;
-                    q -= nq;
+                    const int nu = cut_units(rec.child_type[k]);
+                    if (q < nu) break;
+                    q -= nu;
                 }
             }
             const int ct = rec.child_type[k];
@@ -1043,9 +1051,12 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves
             }
             const double Jc = child_qp<D, CN == 6>(ct, q, ccn, cx, cg, w);
             if (Jc < 0.0) atomicOr(err, 1);  // store_shape_functions aborts (src/Mesh.cpp:1561)
-            if constexpr (D == 3) {
+            if (ct == PG_CHILD_TRI3 || ct == PG_CHILD_TET4) {
                 w = 0.0;
-                for (int r = 0; r < 5; ++r) w += TET_QW[r];
+                if constexpr (D == 2)
+                    for (int r = 0; r < 4; ++r) w += TRI_QW[r];
+                else
+                    for (int r = 0; r < 5; ++r) w += TET_QW[r];
             }
             double px[NEN][3], pdN[NEN][D], pgr[NEN][3];
             for (int n = 0; n < NEN; ++n) {
