@@ -43,21 +43,22 @@ def workload(cfg, batch, uscale):
 
 
 def build_problem(name, device_pattern=True):
-    """The workload's set-up; the pattern stage runs on the GPU unless
-    device_pattern is False.  Returns the problem, its config, the batch's
-    configs (or None) and the set-up wall seconds (total and per stage)."""
+    """The workload's set-up; detection, enrichment and the pattern stage run
+    on the GPU unless device_pattern is False.  Returns the problem, its
+    config, the batch's configs (or None) and the set-up wall seconds (total
+    and per stage)."""
     import dataclasses
     from pigfem_amd import RVEProblem, batch_config, config
     t0 = time.time()
     if name.upper() == "C5":
         cfgs = batch_config(256)
-        prob = RVEProblem.batch(cfgs, device_pattern=device_pattern)
+        prob = RVEProblem.batch(cfgs, device_pattern=device_pattern, device_setup=device_pattern)
         cfg = dataclasses.replace(cfgs[0], name="C5")
     else:
         cfgs = None
         cfg = config(name)
-        prob = RVEProblem(cfg, device_pattern=device_pattern)
-    setup = {"total": time.time() - t0, "pattern_on": "gpu" if device_pattern else "host"}
+        prob = RVEProblem(cfg, device_pattern=device_pattern, device_setup=device_pattern)
+    setup = {"total": time.time() - t0, "setup_on": "gpu" if device_pattern else "host"}
     setup.update({k: round(v, 4) for k, v in prob.stage_seconds().items()})
     log(f"[bench] {cfg.name} built in {setup['total']:.1f}s ({setup}): {prob.stats()}")
     return prob, cfg, cfgs, setup
@@ -321,7 +322,7 @@ def main():
     ap.add_argument("--per-instance", action="store_true", help="C5 main config: also time the instances one by one")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--host-pattern", action="store_true",
-                    help="build the pattern stage of the set-up on the host instead of the GPU")
+                    help="run the set-up (detection, enrichment, pattern) on the host instead of the GPU")
     ap.add_argument("--tiles", default=None,
                     help="group_nodes,cut_variant,group_variant for every schedule instead of autotuning "
                          "(short profiler runs)")

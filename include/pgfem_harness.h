@@ -46,7 +46,7 @@ typedef struct pgh_rve_params {
     double  sigma_c, delta_c, delta_c1, delta_c2, beta, alpha;
     double  strain;         /* applied u_x on RIGHT = strain * width            */
     int32_t max_tries;      /* rejection-sampling budget (0 = default)          */
-    int32_t flags;          /* PGH_PLACE_* | PGH_PATTERN_DEVICE | PGH_SCALAR_DOFS */
+    int32_t flags;          /* PGH_PLACE_* | PGH_PATTERN_DEVICE | PGH_SETUP_DEVICE | PGH_SCALAR_DOFS */
     double  kappa_matrix, kappa_fiber;  /* PGH_SCALAR_DOFS: conductivities      */
 } pgh_rve_params;
 
@@ -62,6 +62,15 @@ typedef struct pgh_rve_params {
  * of on the host; the result is identical.  For a batch, the flag is read from
  * params[0]. */
 #define PGH_PATTERN_DEVICE 4
+/* Run node detection (Mesh::detect_node, src/Mesh.cpp:2883-2930), element
+ * analysis and child topology (Mesh::analyze_elements, src/Mesh.cpp:2962-3071)
+ * and enrichment (Mesh::add_enrichments, src/Mesh.cpp:3104-3251: node
+ * positions by the edge intersection, cut records, ISV offsets) on the GPU;
+ * the result is identical to the host build, array for array, and so are the
+ * error codes and messages.  Mesh generation and inclusion placement (the
+ * synthetic input) stay on the host.  For a batch, each instance reads its own
+ * flag. */
+#define PGH_SETUP_DEVICE 8
 /* A scalar field (heat conduction, ProblemLinearThermal): one dof per node,
  * LinearThermalMaterial materials (kappa_matrix, kappa_fiber), T prescribed on
  * the LEFT (0) and RIGHT (strain) sides, perfect bond only (no cohesive law).

@@ -25,6 +25,7 @@ PGH_COH_NONE, PGH_COH_BL, PGH_COH_OP, PGH_COH_BL_NU, PGH_COH_OP_NU = range(5)
 PGH_PLACE_BEST_EFFORT = 1
 PGH_PLACE_AT_NODES = 2
 PGH_PATTERN_DEVICE = 4
+PGH_SETUP_DEVICE = 8
 PGH_SCALAR_DOFS = 16
 PGH_BATCH_COLS = 8
 PGH_N_STAGES = 7

@@ -42,7 +42,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     objs = []
     for src, kind in (("harness.cpp", "cxx"), ("ksp.cpp", "rocalution"), ("pgfem_asm.hip", "hip"),
-                      ("pattern_dev.hip", "hip")):
+                      ("pattern_dev.hip", "hip"), ("setup_dev.hip", "hip")):
         s = os.path.join(CSRC, src)
         o = os.path.join(OUT, src.rsplit(".", 1)[0] + ".o")
         if kind == "cxx":

@@ -15,7 +15,9 @@
 #include <thread>
 #include <vector>
 
+#include "cut_tables.h"
 #include "pattern_dev.h"
+#include "setup_dev.h"
 
 namespace {
 
@@ -49,13 +51,8 @@ struct CutInfo {
     int ncut;                       // cut edges (enrichment slots before mirrors)
     uint8_t cut_edge[PG_MAX_ENR];   // local edge index per slot
     int32_t slot_incl[PG_MAX_ENR];  // inclusion per slot
-    int nchild;
-    uint8_t child_nn[PG_MAX_CHILD];
-    uint8_t child_nodes[PG_MAX_CHILD][6];
-    int32_t child_mat[PG_MAX_CHILD];
-    int ncoh;
-    uint8_t coh_nn;
-    uint8_t coh_nodes[PG_MAX_COH][8];
+    int ncq;                        // cohesive qps
+    pg_cut_rec rec;                 // children and cohesive children (pgh::cut_record)
 };
 
 }  // namespace
@@ -266,30 +263,9 @@ void place_inclusions(pgh_problem& P) {
 
 // --------------------------------------------------------------------------
 // Node detection, Mesh::detect_node (src/Mesh.cpp:2883-2930): inclusions are
-// checked from the last to the first; the first one containing the node wins.
+// checked from the last to the first; the first one containing the node wins
+// (pgh::detect_one per inclusion).
 // --------------------------------------------------------------------------
-int detect_one(const pgh_problem& P, const Incl& I, const double* x) {
-    if (P.dim == 2) {
-        // Circle_Inclusion::get_domain_lims / detect_node, src/InclusionCircle.cpp:141-163
-        if (!(I.c[0] - I.r <= x[0] && I.c[1] - I.r <= x[1] && x[0] <= I.c[0] + I.r && x[1] <= I.c[1] + I.r))
-            return 0;
-        double dx = x[0] - I.c[0], dy = x[1] - I.c[1];
-        double val = (std::sqrt(dx * dx + dy * dy) - I.r) / I.r;
-        if (val > 1e-12) return 0;
-        if (val < -1e-12) return 2;
-        return 1;
-    }
-    // Sphere_Inclusion, src/InclusionSphere.cpp:163-183 (z upper bound uses center[1])
-    if (!(I.c[0] - I.r <= x[0] && I.c[1] - I.r <= x[1] && I.c[2] - I.r <= x[2] &&
-          x[0] <= I.c[0] + I.r && x[1] <= I.c[1] + I.r && x[2] <= I.c[1] + I.r))
-        return 0;
-    double gx = (x[0] - I.c[0]) / I.r, gy = (x[1] - I.c[1]) / I.r, gz = (x[2] - I.c[2]) / I.r;
-    double val = gx * gx + gy * gy + gz * gz - 1.0;
-    if (val > 1e-12) return 0;
-    if (val < -1e-12) return 2;
-    return 1;
-}
-
 void detect_nodes(pgh_problem& P) {
     const double reach = 2.0 * P.prm.r_max + P.prm.gap;
     Grid G(std::max(reach, 1e-3), P.dim);
@@ -310,7 +286,7 @@ void detect_nodes(pgh_problem& P) {
                     for (int32_t o : G.bins[G.bin(i, j, k)]) cand.push_back(o);
         std::sort(cand.begin(), cand.end(), std::greater<int32_t>());
         for (int32_t j : cand) {
-            int res = detect_one(P, P.incl[j], x);
+            int res = pgh::detect_one(P.dim, P.incl[j].c, P.incl[j].r, x);
             if (res == 0) continue;
             if (res == 1) fail(PG_E_MESH, "node " + std::to_string(nd) + " lies on an inclusion surface");
             P.node_detect[nd] = j;
@@ -319,280 +295,9 @@ void detect_nodes(pgh_problem& P) {
     }
 }
 
-// --------------------------------------------------------------------------
-// Element detection tables.
-//   Tri3::detection, polygon-decomposition branch (src/tri3.cpp:283-452): one
-//   inclusion against the matrix (two children), two inclusions with or
-//   without matrix nodes, three inclusions (three triangles around a
-//   hexagon of matrix); children of 5 or 6 nodes are polygons.
-//   Tet4::detection, prism-decomposition branch: src/tet4.cpp:493-558, then
-//   Tet4::decomposePrisms / partitionPrism: src/tet4.cpp:673-870.
-// Output uses the cohesive (C-1) numbering; transformDetectionCohToReg folds
-// mirrors back for perfect bond (src/elem.cpp:547-559).
-// mat codes: -1 = matrix (element material), k >= 0 inclusion k.
-// --------------------------------------------------------------------------
-struct Detect {
-    int ncut = 0;
-    uint8_t cut_edge[PG_MAX_ENR];
-    int32_t slot_incl[PG_MAX_ENR];  // inclusion of each cut-edge slot
-    std::vector<std::vector<int>> ints;
-    std::vector<int> int_mat;  // detection value whose material the child takes
-    std::vector<std::vector<int>> cohs;
-};
-
-void set_slots(Detect& D, std::initializer_list<int> edges, std::initializer_list<int> incl) {
-    D.ncut = int(edges.size());
-    int s = 0;
-    for (int e : edges) D.cut_edge[s++] = uint8_t(e);
-    s = 0;
-    for (int i : incl) D.slot_incl[s++] = i;
-}
-
-void set_tri3(Detect& D, int ce0, int ce1, int sep, int other, const int a[3], bool sep_lower) {
-    set_slots(D, {ce0, ce1}, {std::max(sep, other), std::max(sep, other)});
-    D.int_mat = {sep, other};
-    if (sep_lower) {
-        D.ints = {{a[0], 4, 3}, {a[1], a[2], 5, 6}};
-        D.cohs = {{3, 4, 5, 6}};
-    } else {
-        D.ints = {{a[0], 6, 5}, {a[1], a[2], 3, 4}};
-        D.cohs = {{4, 3, 6, 5}};
-    }
-}
-
-bool detect_tri3(const int nd[3], Detect& D) {
-    const int n0 = nd[0], n1 = nd[1], n2 = nd[2];
-    if (n0 == n1 && n1 == n2) return false;
-    const int vmin = std::min(n0, std::min(n1, n2));
-    const int uniq = 1 + (n0 != n1 ? 1 : 0) + ((n2 != n0 && n2 != n1) ? 1 : 0);
-    if (uniq == 2 && vmin < 0) {  // one inclusion against the matrix, src/tri3.cpp:336-367
-        if (n1 == n2) {  // node 0 separate
-            const int a[3] = {0, 1, 2};
-            set_tri3(D, 2, 0, n0, n1, a, n0 < n1);
-        } else if (n0 == n2) {  // node 1 separate
-            const int a[3] = {1, 2, 0};
-            set_tri3(D, 0, 1, n1, n0, a, n1 < n0);
-        } else {  // node 2 separate
-            const int a[3] = {2, 0, 1};
-            set_tri3(D, 1, 2, n2, n0, a, n2 < n0);
-        }
-    } else if (uniq == 2) {  // two inclusions, no matrix node, src/tri3.cpp:370-398
-        if (n1 == n2) {
-            set_slots(D, {2, 0, 2, 0}, {n0, n0, n1, n1});
-            D.int_mat = {n0, -1, n1};
-            D.ints = {{0, 8, 7}, {4, 6, 5, 3}, {1, 2, 9, 10}};
-        } else if (n0 == n2) {
-            set_slots(D, {0, 1, 0, 1}, {n1, n1, n2, n2});
-            D.int_mat = {n1, -1, n2};
-            D.ints = {{1, 8, 7}, {4, 6, 5, 3}, {2, 0, 9, 10}};
-        } else {
-            set_slots(D, {1, 2, 1, 2}, {n2, n2, n0, n0});
-            D.int_mat = {n2, -1, n0};
-            D.ints = {{2, 8, 7}, {4, 6, 5, 3}, {0, 1, 9, 10}};
-        }
-        D.cohs = {{4, 3, 8, 7}, {5, 6, 9, 10}};
-    } else if (vmin < 0) {  // two inclusions and a matrix node: a pentagon, src/tri3.cpp:407-433
-        if (n0 < 0) {
-            set_slots(D, {1, 0, 2, 1}, {n1, n1, n2, n2});
-            D.int_mat = {n1, -1, n2};
-            D.ints = {{1, 7, 8}, {0, 4, 3, 6, 5}, {2, 9, 10}};
-        } else if (n1 < 0) {
-            set_slots(D, {2, 1, 0, 2}, {n2, n2, n0, n0});
-            D.int_mat = {n2, -1, n0};
-            D.ints = {{2, 7, 8}, {1, 4, 3, 6, 5}, {0, 9, 10}};
-        } else {
-            set_slots(D, {0, 2, 1, 0}, {n0, n0, n1, n1});
-            D.int_mat = {n0, -1, n1};
-            D.ints = {{0, 7, 8}, {2, 4, 3, 6, 5}, {1, 9, 10}};
-        }
-        D.cohs = {{3, 4, 7, 8}, {5, 6, 9, 10}};
-    } else {  // three inclusions: three triangles and a hexagon, src/tri3.cpp:437-444
-        set_slots(D, {0, 2, 2, 1, 1, 0}, {n0, n0, n2, n2, n1, n1});
-        D.int_mat = {n0, n2, n1, -1};
-        D.ints = {{0, 9, 10}, {2, 11, 12}, {1, 13, 14}, {8, 7, 6, 5, 4, 3}};
-        D.cohs = {{3, 4, 9, 10}, {5, 6, 11, 12}, {7, 8, 13, 14}};
-    }
-    return true;
-}
-
-// Tet4::partitionPrism, src/tet4.cpp:771-870 (including its comparison of
-// g[3] with g[5] in the second branch).
-std::vector<std::vector<int>> partition_prism(const std::vector<int>& l, const std::vector<int64_t>& g,
-                                              bool bottom_known) {
-    std::vector<std::vector<int>> r;
-    if (bottom_known) {
-        if (g[0] < g[1] && g[0] < g[2]) {
-            if (g[1] < g[2]) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[4]}, {l[0], l[4], l[5], l[3]}};
-            else r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[5]}, {l[0], l[4], l[5], l[3]}};
-        } else if (g[1] < g[0] && g[1] < g[2]) {
-            if (g[0] < g[2]) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[3]}, {l[1], l[5], l[3], l[4]}};
-            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[5]}, {l[1], l[5], l[3], l[4]}};
-        } else if (g[2] < g[0] && g[2] < g[1]) {
-            if (g[0] < g[1]) r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[3]}, {l[2], l[3], l[4], l[5]}};
-            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[4]}, {l[2], l[3], l[4], l[5]}};
-        } else {
-            fail(PG_E_MESH, "impossible case for prism decomposition");
-        }
-    } else {
-        int min_enrich = 0;
-        if (g[1] < g[2] && g[1] < g[4] && g[1] < g[5]) min_enrich = 1;
-        else if (g[2] < g[1] && g[2] < g[4] && g[3] < g[5]) min_enrich = 2;
-        else if (g[4] < g[1] && g[4] < g[2] && g[4] < g[5]) min_enrich = 4;
-        else if (g[5] < g[1] && g[5] < g[2] && g[5] < g[4]) min_enrich = 5;
-        bool a15 = (min_enrich == 1 || min_enrich == 5);
-        if (g[0] < g[3]) {
-            if (a15) r = {{l[0], l[1], l[2], l[5]}, {l[0], l[1], l[5], l[4]}, {l[0], l[4], l[5], l[3]}};
-            else r = {{l[0], l[1], l[2], l[4]}, {l[0], l[4], l[2], l[5]}, {l[0], l[4], l[5], l[3]}};
-        } else if (g[3] < g[0]) {
-            if (a15) r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[5]}, {l[1], l[5], l[3], l[4]}};
-            else r = {{l[0], l[1], l[2], l[3]}, {l[1], l[2], l[3], l[4]}, {l[2], l[3], l[4], l[5]}};
-        } else {
-            fail(PG_E_MESH, "impossible case for prism decomposition");
-        }
-    }
-    return r;
-}
-
-bool detect_tet4(const int nd[4], const int32_t gid[4], bool cohesive, Detect& D) {
-    int n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-    bool n0e1 = n0 == n1, n0e2 = n0 == n2, n0e3 = n0 == n3, n1e2 = n1 == n2, n1e3 = n1 == n3, n2e3 = n2 == n3;
-    std::vector<int> vec = {n0, n1, n2, n3};
-    std::sort(vec.begin(), vec.end());
-    vec.erase(std::unique(vec.begin(), vec.end()), vec.end());
-    if (vec.size() == 1) return false;
-    if (vec.size() > 2) fail(PG_E_UNSUPPORTED, "tet4 element cut by more than one interface");
-    if (vec[0] >= 0) fail(PG_E_UNSUPPORTED, "tet4 element cut by two inclusions is not supported");
-    int count0 = int(std::count(nd, nd + 4, vec[0]));
-    std::vector<std::vector<int>> ints, cohs;
-    std::vector<int> mats;
-    const std::vector<int> c_lo = {4, 6, 5, 7, 9, 8}, c_hi = {4, 5, 6, 7, 8, 9};
-    if (count0 == 1 || count0 == 3) {
-        if (n1e2 && n1e3) {
-            D.ncut = 3; D.cut_edge[0] = 0; D.cut_edge[1] = 3; D.cut_edge[2] = 2; mats = {n0, n1};
-            if (n0 < n1) { ints = {{4, 5, 6, 0}, {1, 3, 2, 7, 8, 9}}; cohs = {c_lo}; }
-            else { ints = {{7, 8, 9, 0}, {1, 3, 2, 4, 5, 6}}; cohs = {c_hi}; }
-        } else if (n0e2 && n0e3) {
-            D.ncut = 3; D.cut_edge[0] = 0; D.cut_edge[1] = 1; D.cut_edge[2] = 4; mats = {n1, n0};
-            if (n1 < n0) { ints = {{4, 5, 6, 1}, {0, 2, 3, 7, 8, 9}}; cohs = {c_lo}; }
-            else { ints = {{7, 8, 9, 1}, {0, 2, 3, 4, 5, 6}}; cohs = {c_hi}; }
-        } else if (n0e1 && n0e3) {
-            D.ncut = 3; D.cut_edge[0] = 1; D.cut_edge[1] = 2; D.cut_edge[2] = 5; mats = {n2, n1};
-            if (n2 < n0) { ints = {{4, 5, 6, 2}, {1, 0, 3, 7, 8, 9}}; cohs = {c_lo}; }
-            else { ints = {{7, 8, 9, 2}, {1, 0, 3, 4, 5, 6}}; cohs = {c_hi}; }
-        } else if (n0e1 && n1e2) {
-            D.ncut = 3; D.cut_edge[0] = 3; D.cut_edge[1] = 4; D.cut_edge[2] = 5; mats = {n3, n1};
-            if (n3 < n1) { ints = {{4, 5, 6, 3}, {0, 1, 2, 7, 8, 9}}; cohs = {c_lo}; }
-            else { ints = {{7, 8, 9, 3}, {0, 1, 2, 4, 5, 6}}; cohs = {c_hi}; }
-        } else {
-            fail(PG_E_MESH, "tet4 element has 1 node separated in an unknown manner");
-        }
-    } else {
-        const std::vector<int> q_lo = {4, 5, 6, 7, 8, 9, 10, 11}, q_hi = {7, 6, 5, 4, 11, 10, 9, 8};
-        if (n0e1 && n2e3 && !n0e2) {
-            D.ncut = 4; D.cut_edge[0] = 1; D.cut_edge[1] = 2; D.cut_edge[2] = 3; D.cut_edge[3] = 4; mats = {n0, n2};
-            if (n0 < n2) { ints = {{0, 5, 6, 1, 4, 7}, {2, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
-            else { ints = {{0, 9, 10, 1, 8, 11}, {2, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
-        } else if (n1e2 && n0e3 && !n0e1) {
-            D.ncut = 4; D.cut_edge[0] = 2; D.cut_edge[1] = 0; D.cut_edge[2] = 4; D.cut_edge[3] = 5; mats = {n1, n0};
-            if (n1 < n3) { ints = {{1, 5, 6, 2, 4, 7}, {0, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
-            else { ints = {{1, 9, 10, 2, 8, 11}, {0, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
-        } else if (n0e2 && n1e3 && !n0e1) {
-            D.ncut = 4; D.cut_edge[0] = 0; D.cut_edge[1] = 1; D.cut_edge[2] = 5; D.cut_edge[3] = 3; mats = {n2, n1};
-            if (n0 < n1) { ints = {{2, 5, 6, 0, 4, 7}, {1, 9, 8, 3, 10, 11}}; cohs = {q_lo}; }
-            else { ints = {{2, 9, 10, 0, 8, 11}, {1, 5, 4, 3, 6, 7}}; cohs = {q_hi}; }
-        } else {
-            fail(PG_E_MESH, "tet4 element has 2 nodes separated in an unknown manner");
-        }
-    }
-    for (int k = 0; k < D.ncut; ++k) D.slot_incl[k] = vec[1];
-    // decomposePrisms, src/tet4.cpp:673-735
-    for (size_t o = 0; o < ints.size(); ++o) {
-        if (ints[o].size() == 6) {
-            std::vector<int64_t> ids(6);
-            int known = 0;
-            for (int k = 0; k < 6; ++k) {
-                if (ints[o][k] < 4) { ids[k] = gid[ints[o][k]]; ++known; }
-                else ids[k] = ints[o][k];
-            }
-            auto dec = partition_prism(ints[o], ids, known != 2);
-            for (auto& t : dec) { D.ints.push_back(t); D.int_mat.push_back(mats[o]); }
-        } else {
-            D.ints.push_back(ints[o]);
-            D.int_mat.push_back(mats[o]);
-        }
-    }
-    D.cohs = cohs;
-    return true;
-}
-
-// --------------------------------------------------------------------------
-// Edge intersection, Inclusion::find_intersection (src/Inclusion.cpp:32-120):
-// GSL Steffensen iteration from alpha = 0.5 (GSL 2.1 roots/steffenson.c, the
-// published algorithm: Newton steps, Aitken delta^2 acceleration from the 3rd
-// iterate on), stopping when |alpha - alpha_prev| < 1e-7, 25 iterations max.
-// The reference then falls back to GSL Brent on [0,1]; here the fallback is
-// bisection to the same 1e-7 bracket (a path the generated RVEs do not take).
-// --------------------------------------------------------------------------
-struct EdgeFn {
-    const double* x0; const double* x1; const Incl* I; int dim;
-    void fdf(double a, double& f, double& df) const {
-        double g[3], dd[3];
-        for (int d = 0; d < dim; ++d) {
-            g[d] = x0[d] + a * (x1[d] - x0[d]) - I->c[d];
-            dd[d] = x1[d] - x0[d];
-        }
-        if (dim == 2) {
-            // Circle_Inclusion::int_f / int_df, src/InclusionCircle.cpp:248-287
-            f = (g[0] / I->r) * (g[0] / I->r) + (g[1] / I->r) * (g[1] / I->r) - 1;
-            df = (1 / (I->r * I->r)) * 2.0 * (g[0] * dd[0] + g[1] * dd[1]);
-        } else {
-            f = (g[0] / I->r) * (g[0] / I->r) + (g[1] / I->r) * (g[1] / I->r) + (g[2] / I->r) * (g[2] / I->r) - 1.0;
-            df = (1 / (I->r * I->r)) * 2.0 * (g[0] * dd[0] + g[1] * dd[1] + g[2] * dd[2]);
-        }
-    }
-    double f(double a) const { double F, D; fdf(a, F, D); return F; }
-};
-
-double intersect_edge(const EdgeFn& fn) {
-    if (fn.f(0.0) * fn.f(1.0) >= 0) fail(PG_E_MESH, "no valid intersection on a cut edge");
-    double x = 0.5, x_1 = 0.0, fx, dfx;
-    fn.fdf(x, fx, dfx);
-    int count = 1, iter = 0;
-    double root = 0.5, alpha = 0.5, alpha0;
-    bool done = false;
-    do {
-        ++iter;
-        if (dfx == 0.0) break;
-        double x_new = x - fx / dfx, f_new, df_new;
-        fn.fdf(x_new, f_new, df_new);
-        double x1_old = x_1;
-        x_1 = x;
-        double xo = x;
-        x = x_new;
-        fx = f_new;
-        dfx = df_new;
-        if (count < 3) { root = x_new; ++count; }
-        else {
-            double u = xo - x1_old, v = x_new - 2 * xo + x1_old;
-            root = (v == 0) ? x_new : x1_old - u * u / v;
-        }
-        alpha0 = alpha;
-        alpha = root;
-        done = std::fabs(alpha - alpha0) < 1e-7;
-    } while (!done && iter < 25);
-    if (iter == 25 || alpha < 0.0 || alpha > 1.0 || !done) {
-        double lo = 0.0, hi = 1.0, flo = fn.f(lo);
-        for (int it = 0; it < 100 && hi - lo >= 1e-7; ++it) {
-            double mid = 0.5 * (lo + hi), fm = fn.f(mid);
-            if ((fm < 0) == (flo < 0)) { lo = mid; flo = fm; } else hi = mid;
-        }
-        alpha = 0.5 * (lo + hi);
-    }
-    return alpha;
-}
-
-const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};                          // src/tri3.cpp:22-27
-const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};  // src/tet4.cpp:22-30
+// Element detection tables (pgh::detect_tri3, pgh::detect_tet4) and the edge
+// intersection (pgh::intersect_edge): cut_tables.h, shared with the device
+// set-up.
 
 void build_materials(pgh_problem& P) {
     const pgh_rve_params& p = P.prm;
@@ -659,13 +364,18 @@ void analyze_and_enrich(pgh_problem& P) {
     P.elem_mat.assign(P.n_elem, 0);
     P.cut_of_elem.assign(P.n_elem, -1);
     std::vector<CutInfo> infos;
-    std::vector<int> nd(nen);
+    int nd[4];
     // Mesh::analyze_elements, src/Mesh.cpp:2962-3071
     for (int32_t e = 0; e < P.n_elem; ++e) {
         const int32_t* c = &P.conn[size_t(e) * nen];
         for (int k = 0; k < nen; ++k) nd[k] = P.node_detect[c[k]];
-        Detect D;
-        bool cut = (nen == 3) ? detect_tri3(nd.data(), D) : detect_tet4(nd.data(), c, coh, D);
+        pgh::CutDetect D;
+        bool cut = false;
+        if (nen == 3) {
+            cut = pgh::detect_tri3(nd, D);
+        } else if (int rc = pgh::detect_tet4(nd, c, D, &cut)) {
+            fail(pgh::cut_error_code(rc), pgh::cut_error_text(rc));
+        }
         if (!cut) {
             if (nd[0] != -1) P.elem_mat[e] = 1;
             continue;
@@ -674,23 +384,8 @@ void analyze_and_enrich(pgh_problem& P) {
         ci.elem = e;
         ci.ncut = D.ncut;
         for (int s = 0; s < D.ncut; ++s) { ci.cut_edge[s] = D.cut_edge[s]; ci.slot_incl[s] = D.slot_incl[s]; }
-        ci.nchild = int(D.ints.size());
-        if (ci.nchild > (P.dim == 2 ? 4 : PG_MAX_CHILD)) fail(PG_E_UNSUPPORTED, "too many integration children");
-        for (int k = 0; k < ci.nchild; ++k) {
-            ci.child_nn[k] = uint8_t(D.ints[k].size());
-            for (size_t j = 0; j < D.ints[k].size(); ++j) {
-                int v = D.ints[k][j];
-                if (!coh && v >= nen + D.ncut) v -= D.ncut;  // transformDetectionCohToReg
-                ci.child_nodes[k][j] = uint8_t(v);
-            }
-            ci.child_mat[k] = D.int_mat[k] < 0 ? 0 : 1;
-        }
-        if (coh) {
-            ci.ncoh = int(D.cohs.size());
-            ci.coh_nn = uint8_t(D.cohs[0].size());
-            for (int k = 0; k < ci.ncoh; ++k)
-                for (size_t j = 0; j < D.cohs[k].size(); ++j) ci.coh_nodes[k][j] = uint8_t(D.cohs[k][j]);
-        }
+        ci.ncq = pgh::cut_record(D, P.dim, nen, coh, ci.rec);
+        if (ci.ncq < 0) fail(pgh::cut_error_code(-ci.ncq), pgh::cut_error_text(-ci.ncq));
         infos.push_back(ci);
     }
 
@@ -701,8 +396,8 @@ void analyze_and_enrich(pgh_problem& P) {
     auto edge_of = [&](const CutInfo& ci, int s, int32_t& lo, int32_t& hi) {
         const int32_t* c = &P.conn[size_t(ci.elem) * nen];
         int le = ci.cut_edge[s];
-        int a = nen == 3 ? TRI_EDGE[le][0] : TET_EDGE[le][0];
-        int b = nen == 3 ? TRI_EDGE[le][1] : TET_EDGE[le][1];
+        int a = nen == 3 ? pgh::tri_edge(le, 0) : pgh::tet_edge(le, 0);
+        int b = nen == 3 ? pgh::tri_edge(le, 1) : pgh::tet_edge(le, 1);
         lo = std::min(c[a], c[b]);
         hi = std::max(c[a], c[b]);
     };
@@ -734,8 +429,10 @@ void analyze_and_enrich(pgh_problem& P) {
     for (int32_t k = 0; k < n_init; ++k) {
         const double* x0 = &P.coords[size_t(keys[k].lo) * P.dim];
         const double* x1 = &P.coords[size_t(keys[k].hi) * P.dim];
-        EdgeFn fn{x0, x1, &P.incl[keys[k].inc], P.dim};
-        double a = intersect_edge(fn);
+        const Incl& I = P.incl[keys[k].inc];
+        int rc = 0;
+        double a = pgh::intersect_edge(P.dim, x0, x1, I.c, I.r, &rc);
+        if (rc) fail(pgh::cut_error_code(rc), pgh::cut_error_text(rc));
         P.enr_alpha[k] = a;
         if (coh) P.enr_alpha[n_init + k] = a;
         double* xe = &P.coords[size_t(P.n_nodes + k) * P.dim];
@@ -747,14 +444,14 @@ void analyze_and_enrich(pgh_problem& P) {
     }
 
     // Records: attach enrichment nodes per cut-edge slot (add_enrich_node_to_elem,
-    // src/Mesh.cpp:3661-3720), mirrors at slot + ncut; qp and ISV bookkeeping.
+    // src/Mesh.cpp:3661-3720), mirrors at slot + ncut; ISV offsets.
     P.cut.resize(infos.size());
     P.cut_elem.resize(infos.size());
     int32_t isv = 0;
     for (size_t i = 0; i < infos.size(); ++i) {
         const CutInfo& ci = infos[i];
         pg_cut_rec& R = P.cut[i];
-        std::memset(&R, 0, sizeof(R));
+        R = ci.rec;
         for (int s = 0; s < ci.ncut; ++s) {
             Key k; edge_of(ci, s, k.lo, k.hi); k.inc = ci.slot_incl[s];
             auto it = std::lower_bound(keys.begin(), keys.end(), k, less);
@@ -762,36 +459,48 @@ void analyze_and_enrich(pgh_problem& P) {
             R.enr[s] = P.n_nodes + idx;
             if (coh) R.enr[s + ci.ncut] = P.n_nodes + n_init + idx;
         }
-        R.n_enr = uint8_t(coh ? 2 * ci.ncut : ci.ncut);
-        R.n_child = uint8_t(ci.nchild);
-        int nvqp = 0;
-        for (int k = 0; k < ci.nchild; ++k) {
-            int nn = ci.child_nn[k];
-            if (P.dim == 2) {
-                const int types[7] = {-1, -1, -1, PG_CHILD_TRI3, PG_CHILD_QUAD4, PG_CHILD_POLY5, PG_CHILD_POLY6};
-                R.child_type[k] = uint8_t(types[nn]);
-            } else {
-                R.child_type[k] = uint8_t(PG_CHILD_TET4);
-            }
-            R.child_mat[k] = uint8_t(ci.child_mat[k]);
-            for (int j = 0; j < nn; ++j) R.child_node[PG_CHILD_STRIDE(P.dim) * k + j] = ci.child_nodes[k][j];
-            nvqp += PG_CHILD_NQP(R.child_type[k]);
-        }
-        R.n_vqp = uint8_t(nvqp);
-        R.n_coh = uint8_t(coh ? ci.ncoh : 0);
-        R.coh_nn = ci.coh_nn;
         R.isv_off = isv;
-        for (int k = 0; k < R.n_coh; ++k) {
-            for (int j = 0; j < ci.coh_nn; ++j) R.coh_nodes[k][j] = ci.coh_nodes[k][j];
-            R.coh_mat[k] = 2;
-            isv += (ci.coh_nn == 4) ? 2 : 4;  // CohEdge2 2-pt Gauss, CohTri3 4-pt rule
-        }
+        isv += ci.ncq;
         P.cut_elem[i] = ci.elem;
         P.cut_of_elem[ci.elem] = int32_t(i);
         // base material of a cut element: the matrix (analyze_elements leaves it)
         P.elem_mat[ci.elem] = 0;
     }
     P.n_cqp = isv;
+}
+
+// Detection, element analysis and enrichment on the GPU (setup_dev.hip): the
+// same outputs as detect_nodes + analyze_and_enrich.
+void setup_on_device(pgh_problem& P) {
+    pgh::SetupIn in;
+    in.dim = P.dim;
+    in.nen = P.nen;
+    in.cohesive = P.prm.cohesive != PGH_COH_NONE;
+    in.n_nodes = P.n_nodes;
+    in.n_elem = P.n_elem;
+    in.n_incl = int32_t(P.incl.size());
+    in.coords = P.coords.data();
+    in.conn = P.conn.data();
+    in.incl_c = P.incl_c.data();
+    in.incl_r = P.incl_r.data();
+    in.cell = std::max(2.0 * P.prm.r_max + P.prm.gap, 1e-3);  // detect_nodes' inclusion grid
+    pgh::SetupOut o;
+    std::string err;
+    if (int rc = pgh::setup_device(in, o, err)) fail(rc, err);
+    P.node_detect = std::move(o.node_detect);
+    P.elem_mat = std::move(o.elem_mat);
+    P.cut_of_elem = std::move(o.cut_of_elem);
+    P.cut_elem = std::move(o.cut_elem);
+    P.cut = std::move(o.cut);
+    P.n_enr = o.n_enr;
+    P.enr_incl = std::move(o.enr_incl);
+    P.enr_edge = std::move(o.enr_edge);
+    P.enr_alpha = std::move(o.enr_alpha);
+    P.coords.resize(size_t(P.n_nodes + P.n_enr) * P.dim);
+    std::copy(o.enr_coords.begin(), o.enr_coords.end(), P.coords.begin() + size_t(P.n_nodes) * P.dim);
+    P.n_cqp = o.n_cqp;
+    P.stage_s[2] += o.seconds[0];
+    P.stage_s[3] += o.seconds[1];
 }
 
 // --------------------------------------------------------------------------
@@ -1009,7 +718,8 @@ void timed(pgh_problem& P, int k, F&& f) {
     P.stage_s[k] += now_s() - t0;
 }
 
-constexpr int32_t kKnownFlags = PGH_PLACE_BEST_EFFORT | PGH_PLACE_AT_NODES | PGH_PATTERN_DEVICE | PGH_SCALAR_DOFS;
+constexpr int32_t kKnownFlags =
+    PGH_PLACE_BEST_EFFORT | PGH_PLACE_AT_NODES | PGH_PATTERN_DEVICE | PGH_SETUP_DEVICE | PGH_SCALAR_DOFS;
 
 }  // namespace
 
@@ -1037,11 +747,16 @@ int pgh_build_rve(const pgh_rve_params* p, pgh_problem** out) {
         pgh_problem& Q = *P;
         timed(Q, 0, [&] { generate_mesh(Q); });
         timed(Q, 1, [&] { place_inclusions(Q); });
-        timed(Q, 2, [&] { detect_nodes(Q); });
-        timed(Q, 3, [&] {
+        if (p->flags & PGH_SETUP_DEVICE) {
             build_materials(Q);
-            analyze_and_enrich(Q);
-        });
+            setup_on_device(Q);  // adds its own stage times
+        } else {
+            timed(Q, 2, [&] { detect_nodes(Q); });
+            timed(Q, 3, [&] {
+                build_materials(Q);
+                analyze_and_enrich(Q);
+            });
+        }
         timed(Q, 4, [&] { number_dofs(Q); });
         timed(Q, 5, [&] { build_pattern_any(Q, (p->flags & PGH_PATTERN_DEVICE) != 0); });
     } catch (const HarnessError& e) {
@@ -1156,9 +871,13 @@ int pgh_build_batch(const pgh_rve_params* params, int32_t n, pgh_problem** out) 
                 P->cohesive = p.cohesive != PGH_COH_NONE;
                 generate_mesh(*P);
                 place_inclusions(*P);
-                detect_nodes(*P);
                 build_materials(*P);
-                analyze_and_enrich(*P);
+                if (p.flags & PGH_SETUP_DEVICE) {
+                    setup_on_device(*P);
+                } else {
+                    detect_nodes(*P);
+                    analyze_and_enrich(*P);
+                }
                 number_dofs(*P);
             } catch (const HarnessError& e) {
                 code[i] = e.code;

@@ -15,34 +15,15 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include "dev_pool.h"
 #include "pattern_dev.h"
 
 namespace {
 
-#define DEV_OK(x)                                                            \
-    do {                                                                     \
-        hipError_t e_ = (x);                                                 \
-        if (e_ != hipSuccess) {                                              \
-            err = std::string(#x) + ": " + hipGetErrorString(e_);            \
-            return PG_E_HIP;                                                 \
-        }                                                                    \
-    } while (0)
-
-// Device buffers freed together at the end of the build.
-struct Pool {
-    std::vector<void*> p;
-    ~Pool() {
-        for (void* q : p) (void)hipFree(q);
-    }
-    template <typename T>
-    hipError_t get(T** out, size_t n) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
-        if (e == hipSuccess) p.push_back(q);
-        *out = static_cast<T*>(q);
-        return e;
-    }
-};
+using pgh::Pool;
+using pgh::blocks;
+using pgh::down;
+using pgh::prefix;
 
 struct Mesh {
     int nen;
@@ -134,32 +115,10 @@ __global__ void k_entries_out(int64_t n, const uint64_t* keys, const int32_t* va
     atomicAdd(&c[row], 1);
 }
 
-inline unsigned blocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
-
 int bits_for(int64_t v) {
     int b = 1;
     while ((int64_t(1) << b) <= v) ++b;
     return b;
-}
-
-// exclusive prefix of cnt[0 .. n) into ptr[0 .. n] (ptr[0] = 0), on the device
-template <typename TI, typename TO>
-int prefix(Pool& pool, const TI* cnt, TO* ptr, int64_t n, std::string& err) {
-    DEV_OK(hipMemset(ptr, 0, sizeof(TO)));
-    if (n == 0) return PG_OK;
-    size_t tb = 0;
-    DEV_OK(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt, ptr + 1, int(n)));
-    char* tmp = nullptr;
-    DEV_OK(pool.get(&tmp, tb));
-    DEV_OK(hipcub::DeviceScan::InclusiveSum(tmp, tb, cnt, ptr + 1, int(n)));
-    return PG_OK;
-}
-
-template <typename T>
-int down(std::vector<T>& v, const T* src, int64_t n, std::string& err) {
-    v.resize(size_t(n));
-    if (n) DEV_OK(hipMemcpy(v.data(), src, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
-    return PG_OK;
 }
 
 }  // namespace

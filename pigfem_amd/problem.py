@@ -147,13 +147,15 @@ class RVEProblem:
     """A built RVE (or a batch of RVEs merged into one block-diagonal problem):
     the static problem description plus numpy views of it."""
 
-    def __init__(self, cfg: RVEConfig, _cfgs=None, device_pattern: bool = False):
+    def __init__(self, cfg: RVEConfig, _cfgs=None, device_pattern: bool = False, device_setup: bool = False):
         """device_pattern: build the pattern stage (incidence, node-pair and
-        AIJ patterns, scatter indices) on the GPU; the result is identical."""
+        AIJ patterns, scatter indices) on the GPU; device_setup: run node
+        detection, element analysis and enrichment on the GPU.  Either result
+        is identical to the host build."""
         self.cfg = cfg
         L = lib()
         h = C.c_void_p()
-        dev = _ffi.PGH_PATTERN_DEVICE if device_pattern else 0
+        dev = (_ffi.PGH_PATTERN_DEVICE if device_pattern else 0) | (_ffi.PGH_SETUP_DEVICE if device_setup else 0)
         if _cfgs is None:
             self._params = cfg.params()
             self._params.flags |= dev
@@ -217,9 +219,9 @@ class RVEProblem:
                          .reshape(ni.value + 1, _ffi.PGH_BATCH_COLS) if ni.value else None)
 
     @classmethod
-    def batch(cls, cfgs, device_pattern: bool = False):
+    def batch(cls, cfgs, device_pattern: bool = False, device_setup: bool = False):
         """Merge independent RVEs into one problem (pgh_build_batch)."""
-        return cls(cfgs[0], _cfgs=list(cfgs), device_pattern=device_pattern)
+        return cls(cfgs[0], _cfgs=list(cfgs), device_pattern=device_pattern, device_setup=device_setup)
 
     STAGES = ("mesh", "place", "detect", "enrich", "dofs", "pattern", "instances")
 
