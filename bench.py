@@ -168,9 +168,10 @@ def measure(prob, cfg, args, full):
     # per-kernel durations with HIP events on the launch streams (mixed
     # schedule).  For the roofline the node-pair kernels run alone: the tuned
     # tiles with every node group after the cut kernels on one stream (group
-    # variant bit 2), so no other kernel shares the GPU while they run.  The
-    # timed steps above keep the tuned schedule, whose side-stream groups run
-    # alongside the cut kernels; its stream times are reported beside.
+    # variant bit 2) and of the tuned group size (bits 5-6 clear), so no other
+    # kernel shares the GPU while they run.  The timed steps above keep the
+    # tuned schedule, whose side-stream groups run alongside the cut kernels;
+    # its stream times are reported beside.
     asm.mode = "mixed"
 
     def kernel_times(tiles):
@@ -182,7 +183,7 @@ def measure(prob, cfg, args, full):
             kt.append(asm.last_timing())
         asm.set_timing(False)
         return np.array(kt).mean(axis=0)
-    roof_tiles = (tuned["mixed"][0], tuned["mixed"][1], tuned["mixed"][2] | 4)
+    roof_tiles = (tuned["mixed"][0], tuned["mixed"][1], (tuned["mixed"][2] | 4) & ~(3 << 5))
     ko = kernel_times(tuned["mixed"])
     kt = kernel_times(roof_tiles)
     pre_ms, pair_ms = float(kt[1]), float(kt[2])
