@@ -2508,6 +2508,11 @@ struct GridTpl {
 };
 static_assert(sizeof(GridTpl) % 4 == 0, "the template is copied to LDS as words");
 constexpr int GRID_NI[2] = {8, 4}, GRID_NP[2] = {9, 5}, GRID_DIAG[2] = {4, 2};
+// the same by arithmetic for the kernels: a parity-indexed read of the arrays
+// is a global memory load, whose wait also waits for every store before it
+__device__ inline int grid_ni(int pr) { return 8 >> pr; }
+__device__ inline int grid_np(int pr) { return 9 - 4 * pr; }
+__device__ inline int grid_diag(int pr) { return 4 >> pr; }
 
 // Node-group kernel for the nodes of the structured grid whose incidences are
 // all uncut (tri3).  Each node comes with one record -- node, index of its first
@@ -2561,7 +2566,7 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
     for (int g = g0; g < g1; ++g) {
         const bool live = tid < NG && r.x >= 0;
         const int par = (inf >> 24) & 1;
-        int ci = live ? GRID_NI[par] : 0, cq = live ? GRID_NP[par] : 0;
+        int ci = live ? grid_ni(par) : 0, cq = live ? grid_np(par) : 0;
 #pragma unroll
         for (int off = 1; off < NG; off <<= 1) {
             const int ti = __shfl_up(ci, off), tq = __shfl_up(cq, off);
@@ -2647,14 +2652,14 @@ __global__ __launch_bounds__(BS) void k_pair_struct(DevProb p, const int32_t* __
                 if (s_qptr[mid] <= kq) lo = mid; else hi = mid;
             }
             const int slot = kq - s_qptr[lo];
-            const int pr = (s_inf[lo] >> 24) & 1, np = GRID_NP[pr];
-            const bool isdiag = slot == GRID_DIAG[pr];
+            const int pr = (s_inf[lo] >> 24) & 1, np = grid_np(pr);
+            const bool isdiag = slot == grid_diag(pr);
             double acc[D][D];
             for (int i = 0; i < D; ++i)
                 for (int j = 0; j < D; ++j) acc[i][j] = 0.0;
             double pa[D] = {0.0, 0.0};
             if (isdiag) {
-                const int nd = GRID_NI[pr];
+                const int nd = grid_ni(pr);
                 for (int rr = 0; rr < nd; ++rr) {
                     if (func)
                         for (int i = 0; i < D; ++i) pa[i] += s_dP[lo][rr][i];
