@@ -99,3 +99,24 @@ def test_newton_history_matches_the_committed_fixture(case, mode):
     assert [x[3] for x in a if x[2]] == [x[3] for x in b if x[2]]
     if case == "c2":
         assert out["t_reached"] == pytest.approx(ref["t_reached"], rel=1e-12)
+
+
+def test_newton_c3_first_five_load_steps_match_the_committed_fixture():
+    """The 2000 x 2000 Ortiz-Pandolfi RVE (8 M tri3, 3000 fibres, 9.9 M dofs):
+    the first five load steps of the 2 % history with the mixed-precision GPU
+    assembly take the oracle's iteration counts, two Newton iterations per
+    step (tests/golden/newton_c3s5_oracle.json, recorded with the oracle's
+    FP64 assembly; both runs factorise K_ff on the host with the same direct
+    solver)."""
+    import json
+    from tests.golden.newton_history import fixture, problem, run
+    from pigfem_amd.newton import gpu_assemble_fn
+    ref = json.load(open(fixture("c3s5")))
+    prob = problem("c3s5")
+    print(f"newton c3s5 mixed: {prob.stats()}", flush=True)
+    out = run(prob, gpu_assemble_fn(AssemblerGPU(prob, "mixed")), "mixed", max_steps=5)
+    print(f"newton c3s5 mixed: {out['status']} at t = {out['t_reached']} ({out['seconds']:.0f}s): {out['counts']}",
+          flush=True)
+    assert out["status"] == ref["status"] == "converged"
+    assert [tuple(c) for c in out["counts"]] == [tuple(c) for c in ref["counts"]]
+    assert out["t_reached"] == pytest.approx(ref["t_reached"], rel=1e-12)
