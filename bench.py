@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--per-instance", action="store_true", help="C5 main config: also time the instances one by one")
+    ap.add_argument("--no-per-instance", action="store_true",
+                    help="C5 as a further workload: skip timing its instances one by one")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--host-pattern", action="store_true",
                     help="run the set-up (detection, enrichment, pattern) on the host instead of the GPU")
@@ -426,6 +428,9 @@ def main():
             "grid_nodes": m2["grid_nodes"], "setup_s": s2["total"],
             "setup_stages_s": s2, "tiles": m2["tuned"],
         }
+        if cf2 is not None and not args.no_per_instance:
+            # the batch config quotes per-instance rates too: each RVE on its own
+            out[name]["per_instance"] = per_instance_rates(cf2, m2["tuned"]["mixed"], m2["uscale"], r2["mixed"]["ms"])
         del p2, m2
         gc.collect()
     print(json.dumps(out), flush=True)
