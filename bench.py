@@ -291,7 +291,13 @@ def cpu_baselines(prob, cfg, U, args):
         t = time.perf_counter()
         o.assemble(U, np.zeros(prob.n_cqp))
         best = min(best, time.perf_counter() - t)
-    cpu = {"value": m / best, "unit": "elements/s", "cores": 1, "kind": "port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    cpu = {"value": m / best, "unit": "elements/s", "cores": 1, "kind": "port", "cpu_model": model,
            "sample": f"first {m} elements (element-loop order) of the {cfg.name} RVE, K+R, FP64, "
                      f"best of 2 single-thread runs of the oracle restatement"}
     log(f"[bench] cpu baseline: {cpu['value']:.4g} elements/s over {m} elements")
@@ -305,7 +311,7 @@ def cpu_baselines(prob, cfg, U, args):
         o.assemble(U, np.zeros(prob.n_cqp), threads=nt)
         best_t = min(best_t, time.perf_counter() - t)
     o.close()
-    cpu_threads = {"value": m / best_t, "unit": "elements/s", "cores": nt, "kind": "port",
+    cpu_threads = {"value": m / best_t, "unit": "elements/s", "cores": nt, "kind": "port", "cpu_model": model,
                    "sample": f"the same {m} elements, colour by colour on {nt} OpenMP threads, best of 3"}
     log(f"[bench] cpu baseline, {nt} threads: {cpu_threads['value']:.4g} elements/s")
     return cpu, cpu_threads
