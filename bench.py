@@ -81,7 +81,13 @@ def node_pair_kernels(dim, tiles, grid_nodes):
     names = []
     if grid_nodes > 0 and not v & (128 | 8):
         gs = g if overlap else g_rest
-        names.append(f"k_pair_struct{'' if dim == 2 else '3'}<float, {gs}, {tv}, {bs(gs, tv)}>")
+        if dim == 2:
+            names.append(f"k_pair_struct<float, {gs}, {tv}, {bs(gs, tv)}>")
+        else:
+            # the tet4 grid's table of element tangents (the generator's cube: one spacing,
+            # two materials), unless group variant bit 10 turns it off
+            tab = "false" if v & 1024 else "true"
+            names.append(f"k_pair_struct3<float, {gs}, {tv}, {tab}, {bs(gs, tv)}>")
     for ng in ((g, g_rest) if overlap else (g_rest,)):
         name = (f"k_pair_lists<{dim}, float, float, {ng}, {tv}, {bs(ng, tv)}>" if lists
                 else f"k_pair_group<{dim}, float, {ng}, {tv}, {bs(ng, tv)}>")

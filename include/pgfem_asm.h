@@ -410,8 +410,17 @@ int pg_asm_set_timing(pg_asm* h, int enable);
  *                of the one that computes their records from the grid; bits
  *                8-9 = k: each block of that kernel takes 2^k consecutive
  *                groups, the next group's records loading during the current
- *                group's work.
- * Every choice gives bit-identical results. */
+ *                group's work; bit 10 set = with FP32 tangents, the
+ *                structured tet4 nodes evaluate each incidence's blocks
+ *                instead of reading them from the grid's table of element
+ *                tangents (by material and incidence; built at create when the
+ *                cube grid has one spacing and its elements at most two
+ *                materials).
+ * Every choice gives bit-identical results, except that the FP32 tangent
+ * blocks that structured tet4 nodes read from the table (bits 3, 7 and 10
+ * clear) are rounded once from exact values, where the other kernels round
+ * products of FP32 gradients: those K values agree to FP32 rounding, and P and
+ * the ISVs stay bit-identical. */
 int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_variant);
 
 /* Algorithmic bytes of one K+R assembly of this problem (the roofline

@@ -2708,25 +2708,47 @@ struct GridTpl3 {
 };
 static_assert(sizeof(GridTpl3) % 4 == 0, "the template is copied to LDS as words");
 
+// Element tangents of the structured tet4 grid from a table.  On a cube grid
+// of spacing h every tet has gradients g = G / h with G integer and J = Jt h^3
+// (Jt = +-1), so the isotropic block (owner, owner-first node lb) of
+// incidence r, J w (lambda g_a,i g_b,j + mu g_a,j g_b,i + delta_ij mu g_a.g_b)
+// (w = the sum of the rule's weights), is h Jt w (lambda G_a,i G_b,j + ...):
+// it depends on the material, the incidence and h, not on U.  GridTab3 holds
+// these blocks in FP32, computed in FP64 from the integer G and rounded once,
+// for materials 0 .. G3_MT - 1.  pg_asm_create uses it only when every
+// structured node's 26 neighbours lie at the grid offsets times one spacing h
+// to 1e-10 h and every element of those nodes has a material in the table, so
+// a table block equals the element's exact FP32 tangent to within FP32
+// rounding.
+constexpr int G3_MT = 2;
+struct GridTab3 {
+    float tk[G3_MT][G3_NI][3][9];  // block (owner, owner-first node 1 + k), row-major 3 x 3
+};
+static_assert(sizeof(GridTab3) % 16 == 0, "the table is copied to LDS as 16-byte words");
+
 // Node-group kernel for the structured tet4 nodes whose incidences are all
 // uncut.  Each node comes with one record -- node, index of its first K value
 // (row 0, first column; rows 1 and 2 follow after 3 np values each), first
 // dof, element id of its cube's first tet -- and the row stride S.  The
 // arithmetic, the summation order and the values are those of k_pair_lists,
 // bit for bit; the loads are the records, then the element data.
-template <typename TC, int NG, int TV, int BS = GroupThreads<3, NG, TV>::value>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_struct3(
-    DevProb p, const int32_t* __restrict__ tpl, const int4* __restrict__ srec, const int32_t* __restrict__ sinf,
-    const double* __restrict__ Un, double* __restrict__ K, double* __restrict__ P, unsigned flags, int ngroups,
-    int per) {
+template <typename TC, int NG, int TV, bool TAB, int BS = GroupThreads<3, NG, TV>::value>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(TAB && !TV ? 5 : 4))) void k_pair_struct3(
+    DevProb p, const int32_t* __restrict__ tpl, const int4* __restrict__ tab, const int4* __restrict__ srec,
+    const int32_t* __restrict__ sinf, const double* __restrict__ Un, double* __restrict__ K, double* __restrict__ P,
+    unsigned flags, int ngroups, int per) {
+    static_assert(!TAB || sizeof(TC) == 4, "the table holds FP32 tangents");
     constexpr int D = 3, NEN = 4, NQ = 5;
     constexpr int MAXI = G3_NI * NG;
     static_assert(NG <= 32, "the first wavefront holds the group's nodes");
     __shared__ int s_node[NG], s_kb[NG], s_dof[NG], s_eb[NG], s_S[NG];
-    __shared__ TC s_g[MAXI][NEN][D];
-    __shared__ TC s_wlm[MAXI][2];           // w lambda, w mu of each incidence
+    constexpr int MX = TAB ? 1 : MAXI;      // the table form keeps no per-incidence blocks
+    __shared__ TC s_g[MX][NEN][D];
+    __shared__ TC s_wlm[MX][2];             // w lambda, w mu of each incidence
     __shared__ double s_P[MAXI][D];
-    __shared__ double s_dblk[MAXI][D * D];   // diagonal block (la, la) of each incidence
+    __shared__ double s_dblk[MX][D * D];     // diagonal block (la, la) of each incidence
+    __shared__ uint8_t s_mi[TAB ? MAXI : 1]; // material of each incidence (table form)
+    __shared__ float sTk[TAB ? G3_MT * G3_NI * 3 * 9 : 4];  // GridTab3
     __shared__ TC sLM[MAX_LDS_MATS][2];      // lambda, mu of each LEI material
     __shared__ double sLC[MAX_LDS_MATS][LeiCoef<D>::N];  // LEI stress coefficients
     __shared__ GridTpl3 sT;
@@ -2750,6 +2772,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         tw[k] = tpl[w < TW ? w : TW - 1];
     }
     stage_lm<TC, BS>(p, 6, sLM);
+    if constexpr (TAB)
+        for (int k = tid; k < int(sizeof(GridTab3) / 16); k += BS) reinterpret_cast<int4*>(sTk)[k] = tab[k];
     for (int k = tid; k < MAX_LDS_MATS * LeiCoef<D>::N; k += BS) {
         const int m = k / LeiCoef<D>::N, j = k % LeiCoef<D>::N;
         const double v = lei_coef<D>(p, m < p.n_mat ? m : p.n_mat - 1, j);
@@ -2797,15 +2821,20 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int mi = p.elem_mat[e] & (MAX_LDS_MATS - 1);
             // every load of the incidence is out before the arithmetic
             __builtin_amdgcn_sched_barrier(0);
+            if constexpr (TAB) {
+                s_mi[i] = uint8_t(mi);
+                if (!func) continue;
+            }
             double lc[LeiCoef<D>::N];
             for (int k = 0; k < LeiCoef<D>::N; ++k) lc[k] = sLC[mi][k];
             double gr[NEN][D];
             const double J = simplex_grads<D, double>(x, gr);
             const double w = J * wsum;
+            plain_p<D>(gr, u, 0, w, lc, s_P[i]);
+            if constexpr (TAB) continue;
             TC gt[NEN][D];
             for (int k = 0; k < NEN; ++k)
                 for (int c = 0; c < D; ++c) { gt[k][c] = TC(gr[k][c]); s_g[i][k][c] = gt[k][c]; }
-            plain_p<D>(gr, u, 0, w, lc, s_P[i]);
             if (jac) {
                 const TC wl = TC(w) * sLM[mi][0], wm = TC(w) * sLM[mi][1];
                 s_wlm[i][0] = wl;
@@ -2835,10 +2864,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const unsigned code = sT.pc[slot][r];
                     if (code == 0xffu) continue;
                     const int t = lo * G3_NI + int(code >> 2), lb = int(code & 3);
-                    TC blk[D][D];
-                    iso_block<D, TC>(s_g[t][0], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
-                    for (int i = 0; i < D; ++i)
-                        for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                    if constexpr (TAB) {
+                        const float* tb = &sTk[((int(s_mi[t]) * G3_NI + int(code >> 2)) * 3 + lb - 1) * 9];
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(tb[i * D + j]);
+                    } else {
+                        TC blk[D][D];
+                        iso_block<D, TC>(s_g[t][0], s_g[t][lb], s_wlm[t][0], s_wlm[t][1], blk);
+                        for (int i = 0; i < D; ++i)
+                            for (int j = 0; j < D; ++j) acc[i][j] += double(blk[i][j]);
+                    }
                 }
                 const int64_t k0v = int64_t(s_kb[lo]) + 3 * slot;
                 for (int i = 0; i < D; ++i)
@@ -2850,8 +2885,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int t0 = lo * G3_NI;
                 double a = 0.0;
                 if (c < D * D) {
+                    if constexpr (TAB) {
+                        // minus the FP64 sum of each incidence's three table
+                        // blocks, in the order diag_acc forms it
 #pragma unroll 8
-                    for (int t = t0; t < t0 + G3_NI; ++t) a += s_dblk[t][c];
+                        for (int t = t0; t < t0 + G3_NI; ++t) {
+                            const float* tb = &sTk[(int(s_mi[t]) * G3_NI + (t - t0)) * 27 + c];
+                            double v = -double(tb[0]);
+                            v = v - double(tb[9]);
+                            v = v - double(tb[18]);
+                            a += v;
+                        }
+                    } else {
+#pragma unroll 8
+                        for (int t = t0; t < t0 + G3_NI; ++t) a += s_dblk[t][c];
+                    }
                     K[int64_t(s_kb[lo]) + 3 * sT.diag + int64_t(c / D) * 3 * G3_NP + c % D] = a;
                 } else {
 #pragma unroll 8
@@ -3161,6 +3209,7 @@ struct pg_asm {
     int32_t* stpl = nullptr;        // GridTpl words
     int4* srec = nullptr;           // per structured node: node, first K value, first dof, cell element
     int32_t* sinf = nullptr;        // per structured node: row stride | parity << 24
+    int4* stab3 = nullptr;          // GridTab3 (tet4 grid, FP32 tangents), or null: no table
     int32_t n_poly = 0;             // cut elements with polygon children (2D)
     int32_t* poly_list = nullptr;   // their cut indices (the FP64 drop-in's polygon launch)
     bool timing = false;
@@ -3170,7 +3219,9 @@ struct pg_asm {
     int cut_variant = 0;   // small-class cut kernel variant (elements per wave, occupancy)
     int group_variant = 0; // bit 0: two threads per node pair; bit 1: list form (2D); bit 2: no overlap;
                            // bit 3: global-memory path; bit 4: two threads per pair for enrichment-node groups;
-                           // bits 5-6: 1 + option index of the group size of the groups after the cut kernels
+                           // bits 5-6: 1 + option index of the group size of the groups after the cut kernels;
+                           // bit 7: structured nodes through the generic kernel; bits 8-9: log2 groups per block
+                           // of the structured kernel; bit 10: FP32 structured tet4 nodes without GridTab3
     int64_t alg_bytes = 0;
     int64_t dev_bytes = 0;  // device memory held by the handle
     // thermal pre-load: each material's expansion ratio (0 where not set) and
@@ -3371,11 +3422,57 @@ bool build_grid_tpl3(GridTpl3& T) {
     return true;
 }
 
+// GridTab3 from the template, the grid spacing h and the materials' lambda =
+// D[0][1] and mu = D[5][5] (pg_asm_create accepts isotropic LEI materials
+// only): the gradients of each tet on the unit grid are integers, so the
+// blocks are computed in FP64 from exact inputs and rounded to FP32 once.
+void build_grid_tab3(const GridTpl3& T, const pg_problem_desc* s, double h, GridTab3& G) {
+    memset(&G, 0, sizeof G);
+    const double QW[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+    double wsum = 0.0;
+    for (int k = 0; k < 5; ++k) wsum += QW[k];
+    for (int r = 0; r < G3_NI; ++r) {
+        int d[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) d[k][c] = T.nd[r][k + 1][c] - T.nd[r][0][c];
+        int cf[3][3];
+        cf[0][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
+        cf[0][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
+        cf[0][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
+        cf[1][0] = d[0][2] * d[2][1] - d[0][1] * d[2][2];
+        cf[1][1] = d[0][0] * d[2][2] - d[0][2] * d[2][0];
+        cf[1][2] = d[0][1] * d[2][0] - d[0][0] * d[2][1];
+        cf[2][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+        cf[2][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
+        cf[2][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+        const int Jt = d[0][0] * cf[0][0] + d[0][1] * cf[0][1] + d[0][2] * cf[0][2];  // +-1 on the unit grid
+        double G4[4][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) G4[k + 1][c] = double(cf[k][c] * Jt);
+        for (int c = 0; c < 3; ++c) G4[0][c] = -(G4[1][c] + G4[2][c] + G4[3][c]);
+        for (int m = 0; m < s->n_mat && m < G3_MT; ++m) {
+            const double lam = s->mats[m].D[1], mu = s->mats[m].D[35], w = double(Jt) * wsum * h;
+            for (int lb = 1; lb < 4; ++lb) {
+                const double* ga = G4[0];
+                const double* gb = G4[lb];
+                const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        G.tk[m][r][lb - 1][i * 3 + j] =
+                            float(w * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dot : 0.0)));
+            }
+        }
+    }
+}
+
 // Whether regular node a of a tet4 mesh with row stride S is a node of the
 // structured grid whose incidences, element nodes, pattern columns, scatter
 // indices and dofs are exactly what GridTpl3 gives; then its k_pair_struct3
-// record and word.
-bool grid_node3(const pg_problem_desc* s, const GridTpl3& T, int64_t S, int64_t a, int32_t rec[4], int32_t& inf) {
+// record and word.  tab_ok is cleared unless the node's 26 neighbours lie at
+// the grid offsets times h to 1e-10 h and its elements' materials are in
+// GridTab3.
+bool grid_node3(const pg_problem_desc* s, const GridTpl3& T, int64_t S, int64_t a, int32_t rec[4], int32_t& inf,
+                double h, bool& tab_ok) {
     const int64_t t0 = s->n2e_ptr[a], ni = s->n2e_ptr[a + 1] - t0;
     const int64_t p0 = s->npat_ptr[a], np = s->npat_ptr[a + 1] - p0;
     if (ni != G3_NI || np != G3_NP) return false;
@@ -3389,6 +3486,18 @@ bool grid_node3(const pg_problem_desc* s, const GridTpl3& T, int64_t S, int64_t 
         if (s->n2e[t0 + r] != e || s->cut_of_elem[e] >= 0) return false;
         for (int k = 0; k < 4; ++k)
             if (s->conn[e * 4 + owner_first(4, T.la[r], k)] != a + off(T.nd[r][k])) return false;
+        if (s->elem_mat[e] < 0 || s->elem_mat[e] >= G3_MT) tab_ok = false;
+    }
+    if (tab_ok) {
+        const double* xa = &s->coords[a * 3];
+        for (int k = -1; k <= 1 && tab_ok; ++k)
+            for (int j = -1; j <= 1 && tab_ok; ++j)
+                for (int i = -1; i <= 1 && tab_ok; ++i) {
+                    const double* xb = &s->coords[(a + i + j * S + k * S * S) * 3];
+                    if (!(std::fabs(xb[0] - (xa[0] + i * h)) <= 1e-10 * h && std::fabs(xb[1] - (xa[1] + j * h)) <= 1e-10 * h &&
+                          std::fabs(xb[2] - (xa[2] + k * h)) <= 1e-10 * h))
+                        tab_ok = false;
+                }
     }
     const int64_t kb = s->npat_map[p0 * 9];
     const int32_t dof = s->node_dof[a * 3];
@@ -3629,7 +3738,12 @@ int launch(pg_asm* h, const double* dU, const double* dIc, double* dIt, unsigned
 #define STRUCT3(TC) \
     do { if (ng == 8) STRUCT2(TC, 8); else if (ng == 16) STRUCT2(TC, 16); else STRUCT2(TC, 32); } while (0)
 #define STRUCT1T(TC, NGV, TVV) \
-    k_pair_struct3<TC, NGV, TVV><<<grid_n, GroupThreads<3, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->srec, h->sinf, h->Un, dK, dP, flags, ngr, per)
+    do { \
+        if (std::is_same<TC, float>::value && h->stab3 && !(h->group_variant & 1024)) \
+            k_pair_struct3<float, NGV, TVV, true><<<grid_n, GroupThreads<3, NGV, TVV>::value, 0, s>>>(d, h->stpl, h->stab3, h->srec, h->sinf, h->Un, dK, dP, flags, ngr, per); \
+        else \
+            k_pair_struct3<TC, NGV, TVV, false><<<grid_n, GroupThreads<3, NGV, TVV>::value, 0, s>>>(d, h->stpl, nullptr, h->srec, h->sinf, h->Un, dK, dP, flags, ngr, per); \
+    } while (0)
 #define STRUCT2T(TC, NGV) \
     do { if (tv) STRUCT1T(TC, NGV, 1); else STRUCT1T(TC, NGV, 0); } while (0)
 #define STRUCT3T(TC) \
@@ -3992,14 +4106,28 @@ int pg_asm_create(const pg_problem_desc* s, pg_asm** out) {
         if (s->dim == 3 && build_grid_tpl3(T3)) {
             // the generator's cube has S^3 nodes, S = n + 1
             int64_t S = int64_t(std::llround(std::cbrt(double(s->n_nodes))));
+            // the table of element tangents needs one spacing h (the first cube's)
+            const double h3 = s->n_nodes > 1 ? s->coords[3] - s->coords[0] : 0.0;
+            bool tab3_ok = h3 > 0.0;
             if (S >= 3 && S * S * S == s->n_nodes && S < (int64_t(1) << 24))
                 for (int64_t a = 0; a < s->n_nodes; ++a) {
                     int32_t rec[4], inf = 0;
-                    if (dirty[a] || !grid_node3(s, T3, S, a, rec, inf)) continue;
+                    if (dirty[a] || !grid_node3(s, T3, S, a, rec, inf, h3, tab3_ok)) continue;
                     grid[a] = 1;
                     srec_v.insert(srec_v.end(), rec, rec + 4);
                     sinf_v.push_back(inf);
                 }
+            if (tab3_ok && !sinf_v.empty()) {
+                GridTab3 G3;
+                build_grid_tab3(T3, s, h3, G3);
+                int4* g_;
+                if ((rc = alloc(h, &g_, sizeof(GridTab3) / 16))) return fail(rc);
+                if (hipMemcpy(g_, &G3, sizeof(GridTab3), hipMemcpyHostToDevice) != hipSuccess) {
+                    g_err = "hipMemcpy of the grid tangent table failed";
+                    return fail(PG_E_HIP);
+                }
+                h->stab3 = g_;
+            }
         }
         h->n_struct = int64_t(sinf_v.size());
         // padding records (node -1) up to a whole number of the largest group
@@ -4302,7 +4430,7 @@ int pg_asm_set_tiles(pg_asm* h, int group_nodes, int cut_variant, int group_vari
                                                         : (group_nodes == 2 || group_nodes == 4 || group_nodes == 8));
     if (!ok_g) { g_err = "node-group size must be 8, 16 or 32 (2D), 2, 4 or 8 (3D)"; return PG_E_ARG; }
     if (cut_variant < 0 || cut_variant > 3) { g_err = "cut variant must be 0..3"; return PG_E_ARG; }
-    if (group_variant < 0 || group_variant > 1023) { g_err = "group variant must be 0..1023"; return PG_E_ARG; }
+    if (group_variant < 0 || group_variant > 2047) { g_err = "group variant must be 0..2047"; return PG_E_ARG; }
     h->group_nodes = group_nodes;
     h->cut_variant = cut_variant;
     h->group_variant = group_variant;
